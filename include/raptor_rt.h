@@ -1,0 +1,171 @@
+/* raptor_rt.h — C-ABI of libraptor_rt.so, the MI355X (gfx950) ray-tracing
+ * hot path of RAPTor: BVH build, closest-hit and any-hit traversal,
+ * Moller-Trumbore intersection and Lambert shading of primary, shadow and
+ * diffuse-bounce rays.
+ *
+ * The ABI is plain C: opaque handles, plain pointers and sizes. Every call
+ * returns RT_OK (0) or a negative RT_ERR_* code; rt_last_error() gives the
+ * text. No C++ exception crosses it. Input pointers are borrowed for the
+ * duration of a call; device memory is owned by the handles; outputs go to
+ * caller memory. One rt_ctx is used by one host thread at a time and owns one
+ * HIP stream; calls are synchronous.
+ *
+ * Semantics follow the reference's scalar (IEEE) path: a triangle is hit when
+ * rt/triangle.h:190-235 accepts it, hit ids are emplace-order indices
+ * (rt/primitive_store.h:17-110), u weights vertex c and v weights vertex b.
+ */
+#ifndef RAPTOR_RT_H
+#define RAPTOR_RT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RT_OK 0
+#define RT_ERR_INVALID (-1)
+#define RT_ERR_HIP (-2)
+#define RT_ERR_OOM (-3)
+#define RT_ERR_UNSUPPORTED (-4)
+
+/* Pointer-location flag for trace calls: inputs/outputs are device pointers. */
+#define RT_DEVICE_PTRS 1u
+
+typedef struct rt_ctx rt_ctx;
+typedef struct rt_scene rt_scene;
+typedef struct rt_accel rt_accel;
+
+/* A ray: origin, tmin, direction, tmax (32 bytes). A triangle is accepted when
+ * tmin < t; closest hit also requires t < tmax; any hit requires t < tmax. The
+ * reference fixes tmin at EPSILON = 1e-3 (common/common.cc:14), closest-hit
+ * tmax at MAX_DIST (ssd/bvh.cc:649-709) and any-hit tmax at the shadow-ray
+ * length (ssd/bvh.cc:711-759). */
+typedef struct {
+    float o[3], tmin, d[3], tmax;
+} rt_ray;
+
+/* A hit (16 bytes): distance, barycentrics, emplace-order triangle id or -1. */
+typedef struct {
+    float t, u, v;
+    int32_t prim;
+} rt_hit;
+
+/* Pinhole camera as rt/camera.h:52-62 holds it (x_a_res = y_a_res = 1). */
+typedef struct {
+    float pos[3];
+    float x_axis[3], y_axis[3], z_axis[3];
+    float w, h, t;
+    int32_t x_res, y_res;
+} rt_camera;
+
+/* Point light, rt/light.h:17-18: rgb in [0, 255], distance falloff d. */
+typedef struct {
+    float pos[3];
+    float rgb[3];
+    float d;
+} rt_light;
+
+/* BVH build options. width: 4 or 8 children per node; leaf_size: most
+ * triangles per leaf (1..8); sah_bins: bins of the SAH refinement of the top
+ * levels (0 = off). */
+typedef struct {
+    int32_t width;
+    int32_t leaf_size;
+    int32_t sah_bins;
+    int32_t reserved;
+} rt_build_params;
+
+/* Frame options: bounce = 1 adds one cosine-weighted diffuse bounce with
+ * direct light at the bounce hit; spp samples per pixel; seed keys the
+ * counter-based sampler. */
+typedef struct {
+    int32_t bounce;
+    int32_t spp;
+    uint64_t seed;
+    int32_t flags;
+    int32_t reserved;
+} rt_render_params;
+
+/* Frame statistics. rays: primary, shadow, bounce, bounce-shadow.
+ * ms: device time of the frame's kernels (HIP events on the context stream):
+ * [0] whole frame, [1] primary closest hit, [2] shadow any hit,
+ * [3] bounce closest hit, [4] bounce-shadow any hit, [5] ray generation and
+ * shading. visits: [0] node visits, [1] triangle tests (0 unless counting is
+ * on). */
+typedef struct {
+    uint64_t rays[4];
+    float ms[6];
+    uint64_t visits[2];
+} rt_stats;
+
+/* BVH statistics. */
+typedef struct {
+    int64_t nodes, leaves, triangles;
+    int32_t width, depth;
+    int64_t bytes;
+    float build_ms;
+    float sah_cost;
+} rt_accel_info;
+
+int32_t rt_ctx_create(int32_t hip_device, rt_ctx **out);
+void rt_ctx_destroy(rt_ctx *ctx);
+const char *rt_last_error(const rt_ctx *ctx);
+
+/* Scene in emplace order: abc is n x 9 floats (a, b, c), albedo n x 3 in
+ * [0, 1] or NULL (treated as 1), flags one byte per triangle (bit0 light,
+ * bit1 transparent; both skipped by any-hit queries) or NULL. Replaces
+ * primitive_store + triangle construction (rt/primitive_store.h:17-110,
+ * rt/triangle.h:124-176). */
+int32_t rt_scene_create(rt_ctx *ctx, const float *abc, const float *albedo, const uint8_t *flags, int64_t n,
+                        rt_scene **out);
+void rt_scene_destroy(rt_scene *scene);
+/* Per-triangle normals as the reference computes them (n x 3 floats, host). */
+int32_t rt_scene_normals(rt_scene *scene, float *n3);
+
+/* Build the acceleration structure. Replaces bvh::bvh(primitive_store&)
+ * (ssd/bvh.h:27-32, ssd/bvh_builder.cc:18-178); the caller's triangle order is
+ * not permuted. params may be NULL (width 4, leaf size 4). */
+int32_t rt_accel_build(rt_scene *scene, const rt_build_params *params, rt_accel **out);
+void rt_accel_destroy(rt_accel *accel);
+int32_t rt_accel_info_get(const rt_accel *accel, rt_accel_info *info);
+
+/* Closest hit for n rays. Replaces bvh::find_nearest_object(ray*)
+ * (ssd/bvh.cc:649-709) and its packet form (ssd/bvh.cc:406-447). */
+int32_t rt_trace_closest(rt_accel *accel, const rt_ray *rays, int64_t n, rt_hit *out, uint32_t flags);
+/* Any hit (occlusion) for n rays; occluded[i] = 1 when blocked. Replaces
+ * bvh::found_nearer_object(ray*, t) (ssd/bvh.cc:711-759) and its packet form
+ * (ssd/bvh.cc:449-500). */
+int32_t rt_trace_occluded(rt_accel *accel, const rt_ray *rays, int64_t n, uint8_t *occluded, uint32_t flags);
+
+/* One frame: primary rays, one shadow ray per hit per light, Lambert shading
+ * (phong_shader(255 * albedo, 1.0)), optional diffuse bounce. Replaces
+ * ray_tracer(ssd*, light_list&, primitive_store&, camera&)
+ * (rt/raytracer.h:105, rt/raytracer.cc:563-598). rgb: x_res*y_res*3 floats on
+ * the 0-255 scale of camera::image (rt/camera.h:281-285), row y at
+ * rgb[3*(y*x_res + x)]. primary, rgb, bounce may each be NULL. With
+ * RT_DEVICE_PTRS in params->flags the output pointers are device pointers. */
+int32_t rt_render(rt_accel *accel, const rt_camera *cam, const rt_light *lights, int32_t nlights,
+                  const rt_render_params *params, float *rgb, rt_hit *primary, rt_hit *bounce, rt_stats *stats);
+
+/* Count node visits and triangle tests in later calls (0/1). */
+int32_t rt_set_counting(rt_ctx *ctx, int32_t on);
+
+/* Scene generators (host only). Each returns the triangle count; with
+ * abc == NULL only the count is computed.
+ * Random triangles: centroids uniform in [0,1)^3, each vertex the centroid plus
+ * a uniform offset in [-spread, spread)^3; albedo uniform in [lo, hi)^3.
+ * Terrain: RAPTor/terrain_generator/height_map.h:16-43 over
+ * common/perlin_noise_2d.cc:3-47 on a grid x grid vertex lattice spanning
+ * [0,1] in x and z, two triangles per cell. */
+int64_t rt_gen_random_triangles(uint64_t seed, int64_t n, float spread, float albedo_lo, float albedo_hi,
+                                float *abc, float *albedo);
+int64_t rt_gen_terrain(int32_t grid, float amplitude, float zoom, float persistence, int32_t octaves,
+                       int32_t noise_seed, uint64_t albedo_seed, float albedo_lo, float albedo_hi, float *abc,
+                       float *albedo);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RAPTOR_RT_H */

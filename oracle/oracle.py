@@ -1,0 +1,269 @@
+"""Oracle — TEST INFRASTRUCTURE ONLY.
+
+ctypes binding of liboracle_rt.so, the CPU restatement of RAPTor's ray tracer
+hot path (see vec.hpp). Only tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg import this module.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle_rt.so")
+
+HIT_DTYPE = np.dtype([("t", "<f4"), ("u", "<f4"), ("v", "<f4"), ("prim", "<i4")])
+
+
+class oracle_camera(C.Structure):
+    _fields_ = [("pos", C.c_float * 3), ("x_axis", C.c_float * 3), ("y_axis", C.c_float * 3),
+                ("z_axis", C.c_float * 3), ("w", C.c_float), ("h", C.c_float), ("t", C.c_float),
+                ("x_res", C.c_int32), ("y_res", C.c_int32)]
+
+
+class oracle_light(C.Structure):
+    _fields_ = [("pos", C.c_float * 3), ("rgb", C.c_float * 3), ("d", C.c_float)]
+
+
+class oracle_render_params(C.Structure):
+    _fields_ = [("bounce", C.c_int32), ("spp", C.c_int32), ("seed", C.c_uint64), ("pixel_stride", C.c_int32),
+                ("nthreads", C.c_int32), ("packet", C.c_int32)]
+
+
+class oracle_render_out(C.Structure):
+    _fields_ = [("rgb", C.c_void_p), ("prim", C.c_void_p), ("tuv", C.c_void_p), ("vis", C.c_void_p),
+                ("bounce_prim", C.c_void_p), ("bounce_tuv", C.c_void_p), ("bounce_vis", C.c_void_p),
+                ("counters", C.c_void_p)]
+
+
+def build():
+    import subprocess
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        build()
+    lib = C.CDLL(LIB_PATH)
+    vp, i32, i64, u32, u64, f = C.c_void_p, C.c_int32, C.c_int64, C.c_uint32, C.c_uint64, C.c_float
+    sig = {
+        "oracle_scene_create": (vp, [vp, i64, vp]),
+        "oracle_scene_destroy": (None, [vp]),
+        "oracle_scene_build": (i32, [vp, f, f]),
+        "oracle_scene_node_count": (i32, [vp]),
+        "oracle_bvh_check": (i32, [vp, vp]),
+        "oracle_scene_normals": (None, [vp, vp]),
+        "oracle_trace_closest": (None, [vp, vp, i64, vp, vp, i32, i32, vp]),
+        "oracle_trace_occluded": (None, [vp, vp, i64, vp, i32, i32, vp]),
+        "oracle_render": (i32, [vp, vp, C.POINTER(oracle_camera), C.POINTER(oracle_light), i32,
+                                C.POINTER(oracle_render_params), C.POINTER(oracle_render_out)]),
+        "oracle_min_element_n": (i32, [vp, vp, i32]),
+        "oracle_min_element_range": (i32, [vp, vp, i32, i32]),
+        "oracle_morton": (i32, [f, f, f, f, f, f]),
+        "oracle_morton_sort": (None, [vp, i64, vp, vp]),
+        "oracle_slab": (f, [vp, vp, vp, vp]),
+        "oracle_combined_sa": (f, [vp, vp, vp, vp]),
+        "oracle_intersect": (i32, [vp, vp, vp, vp]),
+        "oracle_offset_start_point": (None, [vp, vp, i32, vp]),
+        "oracle_bounce_dir": (None, [u64, u32, u32, u32, vp, vp]),
+        "oracle_sincos_turns": (None, [f, vp, vp]),
+        "oracle_pcg32_stream": (u64, [u64, u64, i32, vp]),
+        "oracle_splitmix64_stream": (u64, [u64, i32, vp]),
+        "oracle_primary_ray": (None, [C.POINTER(oracle_camera), i32, i32, vp, vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+LIB = _load()
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def f32a(*v):
+    return np.asarray(v, np.float32)
+
+
+def camera_c(cam):
+    c = oracle_camera()
+    for k in range(3):
+        c.pos[k], c.x_axis[k], c.y_axis[k], c.z_axis[k] = cam.pos[k], cam.x_axis[k], cam.y_axis[k], cam.z_axis[k]
+    c.w, c.h, c.t, c.x_res, c.y_res = cam.w, cam.h, cam.t, cam.x_res, cam.y_res
+    return c
+
+
+def lights_c(lights):
+    arr = (oracle_light * max(1, len(lights)))()
+    for i, l in enumerate(lights):
+        for k in range(3):
+            arr[i].pos[k] = l.pos[k]
+            arr[i].rgb[k] = l.rgb[k]
+        arr[i].d = l.d
+    return arr
+
+
+class OracleScene:
+    """A triangle store with the reference's AAC BVH (built on construction)."""
+
+    def __init__(self, abc, flags=None, build=True, alpha=0.4, delta=20.0):
+        self.abc = np.ascontiguousarray(abc, np.float32).reshape(-1, 9)
+        self.n = self.abc.shape[0]
+        self.flags = None if flags is None else np.ascontiguousarray(flags, np.uint8)
+        self.h = LIB.oracle_scene_create(_p(self.abc), self.n, _p(self.flags))
+        self.root = LIB.oracle_scene_build(self.h, alpha, delta) if build else -1
+
+    def bvh_check(self):
+        st = np.zeros(4, np.int32)
+        err = LIB.oracle_bvh_check(self.h, _p(st))
+        return err, dict(leaves=int(st[0]), interior=int(st[1]), depth=int(st[2]), max_leaf=int(st[3]))
+
+    def normals(self):
+        out = np.empty((self.n, 3), np.float32)
+        LIB.oracle_scene_normals(self.h, _p(out))
+        return out
+
+    def trace_closest(self, rays, use_bvh=True, nthreads=8):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        n = rays.shape[0]
+        prim = np.empty(n, np.int32)
+        tuv = np.empty((n, 3), np.float32)
+        cnt = np.zeros(2, np.uint64)
+        LIB.oracle_trace_closest(self.h, _p(rays), n, _p(prim), _p(tuv), 1 if use_bvh else 0, nthreads, _p(cnt))
+        out = np.empty(n, HIT_DTYPE)
+        out["t"], out["u"], out["v"], out["prim"] = tuv[:, 0], tuv[:, 1], tuv[:, 2], prim
+        return out, cnt
+
+    def trace_occluded(self, rays, use_bvh=True, nthreads=8):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        n = rays.shape[0]
+        occ = np.empty(n, np.uint8)
+        cnt = np.zeros(2, np.uint64)
+        LIB.oracle_trace_occluded(self.h, _p(rays), n, _p(occ), 1 if use_bvh else 0, nthreads, _p(cnt))
+        return occ, cnt
+
+    def render(self, albedo, camera, lights, bounce=False, spp=1, seed=0, pixel_stride=1, nthreads=8, packet=False):
+        npix = camera.x_res * camera.y_res
+        nl = len(lights)
+        rgb = np.full((npix, 3), np.nan, np.float32)
+        prim = np.full(npix, -2, np.int32)
+        tuv = np.full((npix, 3), np.nan, np.float32)
+        vis = np.full((npix, max(1, nl)), 255, np.uint8)
+        bprim = np.full(npix, -2, np.int32)
+        btuv = np.full((npix, 3), np.nan, np.float32)
+        bvis = np.full((npix, max(1, nl)), 255, np.uint8)
+        cnt = np.zeros(6, np.uint64)
+        out = oracle_render_out(_p(rgb), _p(prim), _p(tuv), _p(vis), _p(bprim), _p(btuv), _p(bvis), _p(cnt))
+        p = oracle_render_params(1 if bounce else 0, spp, seed, pixel_stride, nthreads, 1 if packet else 0)
+        alb = np.ascontiguousarray(albedo, np.float32)
+        rc = LIB.oracle_render(self.h, _p(alb), C.byref(camera_c(camera)), lights_c(lights), nl, C.byref(p),
+                               C.byref(out))
+        if rc != 0:
+            raise RuntimeError(f"oracle_render failed ({rc})")
+        return dict(rgb=rgb, prim=prim, tuv=tuv, vis=vis, bounce_prim=bprim, bounce_tuv=btuv, bounce_vis=bvis,
+                    counters=cnt)
+
+    def close(self):
+        if self.h:
+            LIB.oracle_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def intersect(abc9, ogn, d):
+    tuv = np.empty(3, np.float32)
+    hit = LIB.oracle_intersect(_p(np.asarray(abc9, np.float32)), _p(np.asarray(ogn, np.float32)),
+                               _p(np.asarray(d, np.float32)), _p(tuv))
+    return bool(hit), tuv
+
+
+def slab(low, high, ogn, i_rd):
+    return LIB.oracle_slab(_p(f32a(*low)), _p(f32a(*high)), _p(f32a(*ogn)), _p(f32a(*i_rd)))
+
+
+def combined_sa(lo1, hi1, lo2, hi2):
+    return LIB.oracle_combined_sa(_p(f32a(*lo1)), _p(f32a(*hi1)), _p(f32a(*lo2)), _p(f32a(*hi2)))
+
+
+def min_element_n(d):
+    d = np.asarray(d, np.float32)
+    m = np.zeros(1, np.float32)
+    i = LIB.oracle_min_element_n(_p(d), _p(m), d.size)
+    return i, float(m[0])
+
+
+def min_element_range(d, begin, end):
+    d = np.asarray(d, np.float32)
+    m = np.zeros(1, np.float32)
+    i = LIB.oracle_min_element_range(_p(d), _p(m), begin, end)
+    return i, float(m[0])
+
+
+def morton(x, y, z, xm, ym, zm):
+    return LIB.oracle_morton(x, y, z, xm, ym, zm)
+
+
+def morton_sort(codes):
+    codes = np.ascontiguousarray(codes, np.int32)
+    s = np.empty_like(codes)
+    o = np.empty_like(codes)
+    LIB.oracle_morton_sort(_p(codes), codes.size, _p(s), _p(o))
+    return s, o
+
+
+def offset_start_point(dst, geo, d=1):
+    out = np.empty(3, np.float32)
+    LIB.oracle_offset_start_point(_p(f32a(*dst)), _p(f32a(*geo)), d, _p(out))
+    return out
+
+
+def bounce_dir(seed, pixel, sample, bounce, n):
+    out = np.empty(3, np.float32)
+    LIB.oracle_bounce_dir(seed, pixel, sample, bounce, _p(np.asarray(n, np.float32)), _p(out))
+    return out
+
+
+def sincos_turns(r):
+    s = np.zeros(1, np.float32)
+    c = np.zeros(1, np.float32)
+    LIB.oracle_sincos_turns(r, _p(s), _p(c))
+    return float(s[0]), float(c[0])
+
+
+def pcg32_stream(initstate, initseq, count):
+    out = np.empty(count, np.uint32)
+    LIB.oracle_pcg32_stream(initstate, initseq, count, _p(out))
+    return out
+
+
+def splitmix64_stream(seed, count):
+    out = np.empty(count, np.uint64)
+    LIB.oracle_splitmix64_stream(seed, count, _p(out))
+    return out
+
+
+def primary_rays(camera):
+    """All primary rays of a camera as (n, 8) float32 (o, tmin=EPSILON, d, tmax=MAX_DIST)."""
+    n = camera.x_res * camera.y_res
+    rays = np.empty((n, 8), np.float32)
+    c = camera_c(camera)
+    o = np.empty(3, np.float32)
+    d = np.empty(3, np.float32)
+    for y in range(camera.y_res):
+        for x in range(camera.x_res):
+            LIB.oracle_primary_ray(C.byref(c), x, y, _p(o), _p(d))
+            i = y * camera.x_res + x
+            rays[i, 0:3] = o
+            rays[i, 3] = np.float32(1e-3)
+            rays[i, 4:7] = d
+            rays[i, 7] = np.finfo(np.float32).max
+    return rays

@@ -1,0 +1,105 @@
+/* Oracle — TEST INFRASTRUCTURE ONLY.
+ *
+ * C entry points of the CPU restatement of RAPTor's ray tracer hot path, for
+ * ctypes. Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+ * load liboracle_rt.so; the product library never links it.
+ */
+#ifndef RT_ORACLE_H
+#define RT_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct oracle_scene oracle_scene;
+
+/* Pinhole camera as rt/camera.h:52-62 stores it: position, basis, image-plane
+ * half widths w and h at distance t, resolution. */
+typedef struct {
+    float pos[3];
+    float x_axis[3], y_axis[3], z_axis[3];
+    float w, h, t;
+    int32_t x_res, y_res;
+} oracle_camera;
+
+/* Point light, rt/light.h:17-18: rgb in [0,255], falloff d. */
+typedef struct {
+    float pos[3];
+    float rgb[3];
+    float d;
+} oracle_light;
+
+/* Render options. bounce: add one cosine-weighted diffuse bounce with direct
+ * light at the bounce hit. spp > 1 jitters the sample inside the pixel. */
+typedef struct {
+    int32_t bounce;
+    int32_t spp;
+    uint64_t seed;
+    int32_t pixel_stride; /* trace pixels p with p % stride == 0 */
+    int32_t nthreads;
+    int32_t packet;       /* 1: SSE packet path for primary and shadow rays */
+} oracle_render_params;
+
+/* Outputs; any pointer may be NULL. Arrays are indexed by pixel p = y*W + x.
+ * tuv are (t, u, v) triples, vis is one byte per light (1 = lit). */
+typedef struct {
+    float *rgb;
+    int32_t *prim;
+    float *tuv;
+    uint8_t *vis;
+    int32_t *bounce_prim;
+    float *bounce_tuv;
+    uint8_t *bounce_vis;
+    /* [0] primary, [1] shadow, [2] bounce, [3] bounce-shadow rays,
+     * [4] child box tests, [5] triangle tests */
+    uint64_t *counters;
+} oracle_render_out;
+
+oracle_scene *oracle_scene_create(const float *abc, int64_t n, const uint8_t *flags);
+void oracle_scene_destroy(oracle_scene *s);
+/* AAC build (alpha, delta as ssd/bvh_builder.h:23). Returns the root. */
+int32_t oracle_scene_build(oracle_scene *s, float alpha, float delta);
+int32_t oracle_scene_node_count(const oracle_scene *s);
+/* Checks BVH invariants; returns 0 when they hold. stats: [leaves, interior,
+ * max depth, max leaf size]. */
+int32_t oracle_bvh_check(const oracle_scene *s, int32_t *stats);
+void oracle_scene_normals(const oracle_scene *s, float *n3);
+
+/* rays: n x 8 floats (o.xyz, tmin, d.xyz, tmax); tmin is EPSILON as the
+ * reference fixes it. Closest hit ignores tmax (the reference starts from
+ * MAX_DIST); any hit uses tmax as the shadow-ray length. use_bvh = 0 runs the
+ * brute-force checker instead of the AAC BVH. */
+void oracle_trace_closest(const oracle_scene *s, const float *rays, int64_t n, int32_t *prim, float *tuv,
+                          int32_t use_bvh, int32_t nthreads, uint64_t *counters);
+void oracle_trace_occluded(const oracle_scene *s, const float *rays, int64_t n, uint8_t *occluded,
+                           int32_t use_bvh, int32_t nthreads, uint64_t *counters);
+
+int32_t oracle_render(const oracle_scene *s, const float *albedo, const oracle_camera *cam,
+                      const oracle_light *lights, int32_t nlights, const oracle_render_params *p,
+                      oracle_render_out *out);
+
+/* Known-answer hooks. */
+int32_t oracle_min_element_n(const float *d, float *m, int32_t n);
+int32_t oracle_min_element_range(const float *d, float *m, int32_t begin, int32_t end);
+int32_t oracle_morton(float x, float y, float z, float xm, float ym, float zm);
+void oracle_morton_sort(const int32_t *codes, int64_t n, int32_t *sorted, int32_t *order);
+float oracle_slab(const float *low, const float *high, const float *ogn, const float *i_rd);
+float oracle_combined_sa(const float *lo1, const float *hi1, const float *lo2, const float *hi2);
+/* Scalar Moller-Trumbore on one triangle; returns 1 and writes t,u,v on a hit. */
+int32_t oracle_intersect(const float *abc, const float *ogn, const float *dir, float *tuv);
+void oracle_offset_start_point(const float *dst, const float *geo_norm, int32_t d, float *out);
+/* Counter-based bounce sampling, for checking the HIP version. */
+void oracle_bounce_dir(uint64_t seed, uint32_t pixel, uint32_t sample, uint32_t bounce, const float *n, float *dir);
+void oracle_sincos_turns(float r, float *s, float *c);
+uint64_t oracle_pcg32_stream(uint64_t initstate, uint64_t initseq, int32_t count, uint32_t *out);
+uint64_t oracle_splitmix64_stream(uint64_t seed, int32_t count, uint64_t *out);
+/* Primary ray of rt/camera.h:127-158 (EXACT_NORMALISE). */
+void oracle_primary_ray(const oracle_camera *cam, int32_t x, int32_t y, float *ogn, float *dir);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RT_ORACLE_H */

@@ -1,0 +1,281 @@
+// C-ABI entry points of libraptor_rt (declared in include/raptor_rt.h).
+#include <algorithm>
+#include <cstring>
+#include <new>
+
+#include "rt_device.hpp"
+#include "rt_internal.hpp"
+
+namespace rt {
+
+int32_t fail(rt_ctx *ctx, int32_t code, const std::string &msg) {
+    if (ctx) ctx->last_error = msg;
+    return code;
+}
+
+namespace {
+
+// rt/triangle.h:171-175: n = normalise(cross(b - a, c - a)) (EXACT_NORMALISE),
+// plus the shading inputs of each triangle.
+__global__ void __launch_bounds__(256) k_scene_prep(const float *__restrict__ abc, const float *__restrict__ albedo,
+                                                    const uint8_t *__restrict__ flags, int64_t n,
+                                                    float4 *__restrict__ tri_n, float4 *__restrict__ tri_c,
+                                                    float4 *__restrict__ kd) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float *t = abc + 9 * i;
+    const rtd::V3 a = rtd::mk(t[0], t[1], t[2]);
+    const rtd::V3 b = rtd::mk(t[3], t[4], t[5]);
+    const rtd::V3 c = rtd::mk(t[6], t[7], t[8]);
+    const rtd::V3 dir_b = rtd::sub(b, a);
+    const rtd::V3 dir_c = rtd::sub(c, a);
+    const rtd::V3 nn = rtd::normalise_exact(rtd::cross(dir_b, dir_c));
+    const int32_t f = flags ? static_cast<int32_t>(flags[i]) : 0;
+    tri_n[i] = make_float4(nn.x, nn.y, nn.z, __int_as_float(f));
+    tri_c[i] = make_float4(c.x, c.y, c.z, 0.0f);
+    // phong_shader(rgb = 255 * albedo, kd = 1.0): kd = rgb * 1.0f
+    // (rt/materials/phong_shader.h:135-137).
+    float4 k = make_float4(255.0f, 255.0f, 255.0f, 0.0f);
+    if (albedo) {
+        k.x = (albedo[3 * i + 0] * 255.0f) * 1.0f;
+        k.y = (albedo[3 * i + 1] * 255.0f) * 1.0f;
+        k.z = (albedo[3 * i + 2] * 255.0f) * 1.0f;
+    }
+    kd[i] = k;
+}
+
+template <class T>
+hipError_t upload(T **dst, const T *src, int64_t count, hipStream_t s) {
+    hipError_t e = hipMalloc(dst, std::max<int64_t>(1, count) * sizeof(T));
+    if (e != hipSuccess) return e;
+    if (count > 0 && src) e = hipMemcpyAsync(*dst, src, count * sizeof(T), hipMemcpyHostToDevice, s);
+    return e;
+}
+
+void free_scene(rt_scene *sc) {
+    if (!sc) return;
+    if (sc->d_abc) (void)hipFree(sc->d_abc);
+    if (sc->d_n) (void)hipFree(sc->d_n);
+    if (sc->d_c) (void)hipFree(sc->d_c);
+    if (sc->d_kd) (void)hipFree(sc->d_kd);
+    if (sc->d_flags) (void)hipFree(sc->d_flags);
+    delete sc;
+}
+
+int32_t set_device(rt_ctx *ctx) {
+    RT_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    return RT_OK;
+}
+
+}  // namespace
+}  // namespace rt
+
+using rt::fail;
+
+extern "C" {
+
+int32_t rt_ctx_create(int32_t hip_device, rt_ctx **out) {
+    if (!out) return RT_ERR_INVALID;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return RT_ERR_HIP;
+    if (hip_device < 0 || hip_device >= count) return RT_ERR_INVALID;
+    rt_ctx *ctx = new (std::nothrow) rt_ctx();
+    if (!ctx) return RT_ERR_OOM;
+    ctx->device = hip_device;
+    if (hipSetDevice(hip_device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return RT_ERR_HIP;
+    }
+    for (auto &e : ctx->ev) {
+        if (hipEventCreate(&e) != hipSuccess) {
+            rt_ctx_destroy(ctx);
+            return RT_ERR_HIP;
+        }
+    }
+    if (hipMalloc(&ctx->d_counters, 4 * sizeof(unsigned long long)) != hipSuccess) {
+        rt_ctx_destroy(ctx);
+        return RT_ERR_HIP;
+    }
+    (void)hipMemset(ctx->d_counters, 0, 4 * sizeof(unsigned long long));
+    *out = ctx;
+    return RT_OK;
+}
+
+void rt_ctx_destroy(rt_ctx *ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto &e : ctx->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (ctx->d_counters) (void)hipFree(ctx->d_counters);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char *rt_last_error(const rt_ctx *ctx) { return ctx ? ctx->last_error.c_str() : "null context"; }
+
+int32_t rt_scene_create(rt_ctx *ctx, const float *abc, const float *albedo, const uint8_t *flags, int64_t n,
+                        rt_scene **out) {
+    if (!ctx || !out) return RT_ERR_INVALID;
+    *out = nullptr;
+    if (n < 0 || (n > 0 && !abc)) return fail(ctx, RT_ERR_INVALID, "rt_scene_create: bad triangle array");
+    if (int32_t rc = rt::set_device(ctx)) return rc;
+    rt_scene *sc = new (std::nothrow) rt_scene();
+    if (!sc) return fail(ctx, RT_ERR_OOM, "rt_scene_create: host allocation");
+    sc->ctx = ctx;
+    sc->n = n;
+    hipStream_t s = ctx->stream;
+    float *d_alb = nullptr;
+    hipError_t e = rt::upload(&sc->d_abc, abc, 9 * n, s);
+    if (e == hipSuccess && flags) e = rt::upload(&sc->d_flags, flags, n, s);
+    if (e == hipSuccess && albedo) e = rt::upload(&d_alb, albedo, 3 * n, s);
+    if (e == hipSuccess) e = hipMalloc(&sc->d_n, std::max<int64_t>(1, n) * sizeof(float4));
+    if (e == hipSuccess) e = hipMalloc(&sc->d_c, std::max<int64_t>(1, n) * sizeof(float4));
+    if (e == hipSuccess) e = hipMalloc(&sc->d_kd, std::max<int64_t>(1, n) * sizeof(float4));
+    if (e == hipSuccess && n > 0) {
+        hipLaunchKernelGGL(rt::k_scene_prep, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, s, sc->d_abc,
+                           d_alb, sc->d_flags, n, sc->d_n, sc->d_c, sc->d_kd);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (d_alb) (void)hipFree(d_alb);
+    if (e != hipSuccess) {
+        rt::free_scene(sc);
+        return fail(ctx, e == hipErrorOutOfMemory ? RT_ERR_OOM : RT_ERR_HIP,
+                    std::string("rt_scene_create: ") + hipGetErrorString(e));
+    }
+    *out = sc;
+    return RT_OK;
+}
+
+void rt_scene_destroy(rt_scene *scene) {
+    if (!scene) return;
+    (void)hipSetDevice(scene->ctx->device);
+    (void)hipStreamSynchronize(scene->ctx->stream);
+    rt::free_scene(scene);
+}
+
+int32_t rt_scene_normals(rt_scene *scene, float *n3) {
+    if (!scene || !n3) return RT_ERR_INVALID;
+    rt_ctx *ctx = scene->ctx;
+    if (int32_t rc = rt::set_device(ctx)) return rc;
+    std::vector<float4> tmp(std::max<int64_t>(1, scene->n));
+    RT_HIP_CHECK(ctx, hipMemcpyAsync(tmp.data(), scene->d_n, scene->n * sizeof(float4), hipMemcpyDeviceToHost,
+                                     ctx->stream));
+    RT_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    for (int64_t i = 0; i < scene->n; ++i) {
+        n3[3 * i + 0] = tmp[i].x;
+        n3[3 * i + 1] = tmp[i].y;
+        n3[3 * i + 2] = tmp[i].z;
+    }
+    return RT_OK;
+}
+
+int32_t rt_accel_build(rt_scene *scene, const rt_build_params *params, rt_accel **out) {
+    if (!scene || !out) return RT_ERR_INVALID;
+    *out = nullptr;
+    rt_ctx *ctx = scene->ctx;
+    if (int32_t rc = rt::set_device(ctx)) return rc;
+    rt_build_params p{4, 4, 0, 0};
+    if (params) p = *params;
+    return rt::build_accel(scene, p, out);
+}
+
+void rt_accel_destroy(rt_accel *accel) {
+    if (!accel) return;
+    (void)hipSetDevice(accel->scene->ctx->device);
+    (void)hipStreamSynchronize(accel->scene->ctx->stream);
+    rt::free_frame_workspace(accel);
+    if (accel->d_nodes) (void)hipFree(accel->d_nodes);
+    if (accel->d_tris) (void)hipFree(accel->d_tris);
+    delete accel;
+}
+
+int32_t rt_accel_info_get(const rt_accel *a, rt_accel_info *info) {
+    if (!a || !info) return RT_ERR_INVALID;
+    info->nodes = a->nodes;
+    info->leaves = a->leaves;
+    info->triangles = a->scene->n;
+    info->width = a->width;
+    info->depth = a->depth;
+    const int64_t node_bytes = (a->width == 4) ? 128 : 256;
+    info->bytes = a->nodes * node_bytes + a->scene->n * static_cast<int64_t>(sizeof(rt::LeafTri));
+    info->build_ms = a->build_ms;
+    info->sah_cost = a->sah_cost;
+    return RT_OK;
+}
+
+static int32_t trace_common(rt_accel *accel, const rt_ray *rays, int64_t n, void *out, size_t out_elem,
+                            uint32_t flags, bool closest) {
+    if (!accel || (n > 0 && (!rays || !out)) || n < 0) return RT_ERR_INVALID;
+    rt_ctx *ctx = accel->scene->ctx;
+    if (int32_t rc = rt::set_device(ctx)) return rc;
+    if (n == 0) return RT_OK;
+    hipStream_t s = ctx->stream;
+    const bool dev = (flags & RT_DEVICE_PTRS) != 0;
+    const rt_ray *d_rays = rays;
+    void *d_out = out;
+    rt_ray *tmp_rays = nullptr;
+    void *tmp_out = nullptr;
+    if (!dev) {
+        RT_HIP_CHECK(ctx, hipMalloc(&tmp_rays, n * sizeof(rt_ray)));
+        hipError_t e = hipMalloc(&tmp_out, n * out_elem);
+        if (e != hipSuccess) {
+            (void)hipFree(tmp_rays);
+            return fail(ctx, RT_ERR_OOM, "trace staging buffers");
+        }
+        (void)hipMemcpyAsync(tmp_rays, rays, n * sizeof(rt_ray), hipMemcpyHostToDevice, s);
+        d_rays = tmp_rays;
+        d_out = tmp_out;
+    }
+    int32_t rc = closest ? rt::launch_closest(accel, d_rays, n, reinterpret_cast<rt_hit *>(d_out), s)
+                         : rt::launch_occluded(accel, d_rays, n, reinterpret_cast<uint8_t *>(d_out), s);
+    if (rc == RT_OK && !dev) {
+        hipError_t e = hipMemcpyAsync(out, d_out, n * out_elem, hipMemcpyDeviceToHost, s);
+        if (e != hipSuccess) rc = fail(ctx, RT_ERR_HIP, hipGetErrorString(e));
+    }
+    hipError_t e = hipStreamSynchronize(s);
+    if (rc == RT_OK && e != hipSuccess) rc = fail(ctx, RT_ERR_HIP, std::string("trace: ") + hipGetErrorString(e));
+    if (tmp_rays) (void)hipFree(tmp_rays);
+    if (tmp_out) (void)hipFree(tmp_out);
+    return rc;
+}
+
+int32_t rt_trace_closest(rt_accel *accel, const rt_ray *rays, int64_t n, rt_hit *out, uint32_t flags) {
+    return trace_common(accel, rays, n, out, sizeof(rt_hit), flags, true);
+}
+
+int32_t rt_trace_occluded(rt_accel *accel, const rt_ray *rays, int64_t n, uint8_t *occluded, uint32_t flags) {
+    return trace_common(accel, rays, n, occluded, 1, flags, false);
+}
+
+int32_t rt_render(rt_accel *accel, const rt_camera *cam, const rt_light *lights, int32_t nlights,
+                  const rt_render_params *params, float *rgb, rt_hit *primary, rt_hit *bounce, rt_stats *stats) {
+    if (!accel || !cam || (nlights > 0 && !lights)) return RT_ERR_INVALID;
+    rt_ctx *ctx = accel->scene->ctx;
+    if (int32_t rc = rt::set_device(ctx)) return rc;
+    rt_render_params p{0, 1, 0, 0, 0};
+    if (params) p = *params;
+    if (ctx->counting) RT_HIP_CHECK(ctx, hipMemsetAsync(ctx->d_counters, 0, 4 * sizeof(unsigned long long), ctx->stream));
+    int32_t rc = rt::render_frame(accel, *cam, lights, nlights, p, rgb, primary, bounce, stats);
+    if (rc == RT_OK && stats) {
+        stats->visits[0] = stats->visits[1] = 0;
+        if (ctx->counting) {
+            unsigned long long v[4];
+            RT_HIP_CHECK(ctx, hipMemcpy(v, ctx->d_counters, sizeof(v), hipMemcpyDeviceToHost));
+            stats->visits[0] = v[0];
+            stats->visits[1] = v[1];
+            if (v[2]) return fail(ctx, RT_ERR_INVALID, "traversal stack overflow");
+        }
+    }
+    return rc;
+}
+
+int32_t rt_set_counting(rt_ctx *ctx, int32_t on) {
+    if (!ctx) return RT_ERR_INVALID;
+    ctx->counting = on != 0;
+    return RT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,514 @@
+// One frame of the tracer as a wavefront pipeline of kernels on the context
+// stream. Replaces the per-pixel loop of ray_tracer() (rt/raytracer.cc:563-598)
+// and ray_trace_engine::ray_trace (rt/raytracer.cc:20-87, 168-461):
+//
+//   k_raygen        primary rays, rt/camera.h:127-145 (EXACT_NORMALISE)
+//   closest hit     primary rays (trace.hip)
+//   k_shade_hit<0>  for each primary hit: move to the hit and re-project onto
+//                   the plane (rt/ray.cc:15-20, rt/triangle.h:464-498), one
+//                   shadow ray per light (rt/ray.cc:91-104, rt/ray.h:111-157,
+//                   rt/light.h:96-100) and the diffuse bounce ray; hits are
+//                   compacted into queues with one atomic per wavefront
+//   any hit         shadow queue
+//   closest hit     bounce queue
+//   k_shade_hit<1>  shadow rays of the bounce hits (same arithmetic)
+//   any hit         bounce-shadow queue
+//   k_compose       Lambert shading, rt/materials/phong_shader.cc:21-64 with
+//                   rt/light.h:44-59, bounce = (kd1 / 255) * direct(x2)
+//
+// Queue lengths stay on the device; trace kernels read them, so the frame has
+// no host round trip until its statistics are read back.
+#include <algorithm>
+
+#include "rt_device.hpp"
+#include "rt_internal.hpp"
+
+namespace rt {
+
+namespace {
+
+struct CamArgs {
+    float px, py, pz;
+    float xx, xy, xz, yx, yy, yz, zx, zy, zz;
+    float x_m, y_m, x_inc, y_inc, t;
+    int32_t x_res, y_res;
+};
+
+__global__ void __launch_bounds__(256) k_raygen(CamArgs c, int spp, uint64_t seed, int64_t nsamples,
+                                                rt_ray *__restrict__ rays) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= nsamples) return;
+    const int64_t p = i / spp;
+    const int s = static_cast<int>(i - p * spp);
+    const int x = static_cast<int>(p % c.x_res);
+    const int y = static_cast<int>(p / c.x_res);
+    float xf = static_cast<float>(x), yf = static_cast<float>(y);
+    if (spp > 1) {
+        const uint64_t k = rtd::sample_key(seed, static_cast<uint32_t>(p), static_cast<uint32_t>(s), 255u);
+        xf = xf + rtd::unit24(static_cast<uint32_t>(k >> 40));
+        yf = yf + rtd::unit24(static_cast<uint32_t>((k >> 16) & 0xFFFFFFu));
+    }
+    const float x_t = (xf * c.x_inc) + c.x_m;
+    const float y_t = (yf * c.y_inc) + c.y_m;
+    // ((X * x_t) + (Y * y_t)) + (Z * t), component by component.
+    const rtd::V3 scr = rtd::mk(((c.xx * x_t) + (c.yx * y_t)) + (c.zx * c.t), ((c.xy * x_t) + (c.yy * y_t)) + (c.zy * c.t),
+                                ((c.xz * x_t) + (c.yz * y_t)) + (c.zz * c.t));
+    const rtd::V3 d = rtd::normalise_exact(scr);
+    float4 *r = reinterpret_cast<float4 *>(rays) + 2 * i;
+    r[0] = make_float4(c.px, c.py, c.pz, rtd::kEpsilon);
+    r[1] = make_float4(d.x, d.y, d.z, rtd::kMaxDist);
+}
+
+struct LightArgs {
+    float4 pos_d[8];   // position, falloff d
+    float4 rgb[8];     // rgb / 255
+    int n;
+};
+
+struct ShadeArgs {
+    const rt_ray *rays;      // rays that were traced
+    const rt_hit *hits;      // their hits
+    const uint32_t *count;   // number of rays (nullptr: n_static)
+    int64_t n_static;
+    const float4 *tri_n, *tri_c;
+    LightArgs lights;
+    int bounce;
+    int spp;
+    uint64_t seed;
+    const int32_t *sample_of;   // for bounce hits: the sample index of each bounce ray
+    int32_t *qidx;              // per ray: compacted hit index or -1
+    float4 *nshade;             // per compacted hit: shading normal
+    int32_t *hit_sample;        // per compacted hit: sample index
+    rt_ray *shadow_q;           // per compacted hit x light
+    rt_ray *bounce_q;           // per compacted hit (stage 0 with bounce)
+    uint32_t *q_count;          // compacted hit counter
+};
+
+// rt/ray.cc:15-20 + rt/triangle.h:464-498.
+__device__ __forceinline__ void surface_at(rtd::V3 o, rtd::V3 d, float t, rtd::V3 n, rtd::V3 c, rtd::V3 *dst,
+                                           rtd::V3 *geo, rtd::V3 *shade) {
+    rtd::V3 p = rtd::add(o, rtd::mul(d, t));
+    const float denom = rtd::dot(n, d);
+    const float num = rtd::dot(n, rtd::sub(c, p));
+    const float dl = num / denom;
+    p = rtd::add(p, rtd::mul(d, dl));
+    *dst = p;
+    if (denom > 0.0f) {
+        *geo = rtd::neg(n);
+        *shade = rtd::neg(n);
+    } else {
+        *geo = n;
+        *shade = n;
+    }
+}
+
+template <int STAGE>
+__global__ void __launch_bounds__(256) k_shade_hit(ShadeArgs a) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t n = a.count ? static_cast<int64_t>(a.count[0]) : a.n_static;
+    const bool in = i < n;
+    float4 h = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(-1));
+    if (in) h = reinterpret_cast<const float4 *>(a.hits)[i];
+    const int32_t prim = __float_as_int(h.w);
+    const bool hit = in && prim >= 0;
+    // Wave-level compaction: one atomic per wavefront.
+    const uint64_t mask = __ballot(hit);
+    const int lane = threadIdx.x & 63;
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const int leader = mask ? (__ffsll(static_cast<long long>(mask)) - 1) : 0;
+    uint32_t base = 0;
+    if (mask && lane == leader) base = atomicAdd(a.q_count, static_cast<uint32_t>(__popcll(mask)));
+    base = __shfl(base, leader, 64);
+    if (!in) return;
+    if (!hit) {
+        a.qidx[i] = -1;
+        return;
+    }
+    const uint32_t q = base + static_cast<uint32_t>(__popcll(mask & lt));
+    a.qidx[i] = static_cast<int32_t>(q);
+    const float4 r0 = reinterpret_cast<const float4 *>(a.rays)[2 * i];
+    const float4 r1 = reinterpret_cast<const float4 *>(a.rays)[2 * i + 1];
+    const float4 tn = a.tri_n[prim];
+    const float4 tc = a.tri_c[prim];
+    rtd::V3 dst, geo, shn;
+    surface_at(rtd::f4v(r0), rtd::f4v(r1), h.x, rtd::f4v(tn), rtd::f4v(tc), &dst, &geo, &shn);
+    a.nshade[q] = make_float4(shn.x, shn.y, shn.z, 0.0f);
+    const int32_t sample = (STAGE == 0) ? static_cast<int32_t>(i) : a.sample_of[i];
+    a.hit_sample[q] = sample;
+    const rtd::V3 org = rtd::offset_start_point(dst, geo, 1);
+    for (int l = 0; l < a.lights.n; ++l) {
+        const float4 lp = a.lights.pos_d[l];
+        // ray::set_up(o, dst): dir = dst - o, length = |dir|, dir /= length.
+        const rtd::V3 dv = rtd::sub(rtd::f4v(lp), org);
+        const float len = rtd::magnitude(dv);
+        const rtd::V3 dn = rtd::divs(dv, len);
+        float4 *sr = reinterpret_cast<float4 *>(a.shadow_q) + 2 * (static_cast<int64_t>(q) * a.lights.n + l);
+        sr[0] = make_float4(org.x, org.y, org.z, rtd::kEpsilon);
+        sr[1] = make_float4(dn.x, dn.y, dn.z, len);
+    }
+    if (STAGE == 0 && a.bounce) {
+        const int64_t p = i / a.spp;
+        const int s = static_cast<int>(i - p * a.spp);
+        const rtd::V3 bd = rtd::cosine_direction(
+            rtd::sample_key(a.seed, static_cast<uint32_t>(p), static_cast<uint32_t>(s), 1u), shn);
+        float4 *br = reinterpret_cast<float4 *>(a.bounce_q) + 2 * static_cast<int64_t>(q);
+        br[0] = make_float4(org.x, org.y, org.z, rtd::kEpsilon);
+        br[1] = make_float4(bd.x, bd.y, bd.z, rtd::kMaxDist);
+    }
+}
+
+struct ComposeArgs {
+    int64_t npix;
+    int spp;
+    int bounce;
+    const rt_hit *hits1;
+    const int32_t *qidx1;        // per sample
+    const float4 *nshade1;       // per primary hit
+    const rt_ray *shadow1;       // per primary hit x light
+    const uint8_t *occ1;
+    const int32_t *qidx2;        // per primary hit (bounce ray): bounce hit index or -1
+    const rt_hit *hits2;         // per primary hit (bounce ray)
+    const float4 *nshade2;       // per bounce hit
+    const rt_ray *shadow2;
+    const uint8_t *occ2;
+    const float4 *kd;
+    LightArgs lights;
+    float *rgb;
+};
+
+struct Col {
+    float r, g, b;
+};
+
+// rt/materials/phong_shader.cc:21-64 for kd = 255 * albedo, ks = ka = 0.
+__device__ __forceinline__ Col direct(const rt_ray *shadow, const uint8_t *occ, int64_t q, rtd::V3 nsh, float4 kd,
+                                      const LightArgs &L) {
+    Col c{0.0f, 0.0f, 0.0f};
+    for (int l = 0; l < L.n; ++l) {
+        const int64_t k = q * L.n + l;
+        const float4 s1 = reinterpret_cast<const float4 *>(shadow)[2 * k + 1];
+        const float shade = rtd::dot(rtd::f4v(s1), nsh);
+        if (shade < 0.0f) continue;
+        const float scr = shade * kd.x, scg = shade * kd.y, scb = shade * kd.z;
+        const float magn = (occ[k] ? 0.0f : 1.0f) / 1.0f;
+        const float dist = s1.w * L.pos_d[l].w;
+        const float d_scale = (dist * dist) + 1.0f;
+        const float4 rgb = L.rgb[l];
+        c.r = c.r + ((rgb.x / d_scale) * (scr * magn));
+        c.g = c.g + ((rgb.y / d_scale) * (scg * magn));
+        c.b = c.b + ((rgb.z / d_scale) * (scb * magn));
+    }
+    c.r = c.r + 0.0f;
+    c.g = c.g + 0.0f;
+    c.b = c.b + 0.0f;
+    return c;
+}
+
+__global__ void __launch_bounds__(256) k_compose(ComposeArgs a) {
+    const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p >= a.npix) return;
+    Col acc{0.0f, 0.0f, 0.0f};
+    for (int s = 0; s < a.spp; ++s) {
+        const int64_t i = p * a.spp + s;
+        Col c{0.0f, 0.0f, 0.0f};
+        const int32_t q = a.qidx1[i];
+        if (q >= 0) {
+            const int32_t prim = __float_as_int(reinterpret_cast<const float4 *>(a.hits1)[i].w);
+            const float4 kd1 = a.kd[prim];
+            c = direct(a.shadow1, a.occ1, q, rtd::f4v(a.nshade1[q]), kd1, a.lights);
+            if (a.bounce) {
+                const int32_t q2 = a.qidx2[q];
+                if (q2 >= 0) {
+                    const int32_t prim2 = __float_as_int(reinterpret_cast<const float4 *>(a.hits2)[q].w);
+                    const Col c2 = direct(a.shadow2, a.occ2, q2, rtd::f4v(a.nshade2[q2]), a.kd[prim2], a.lights);
+                    c.r = c.r + ((kd1.x / 255.0f) * c2.r);
+                    c.g = c.g + ((kd1.y / 255.0f) * c2.g);
+                    c.b = c.b + ((kd1.z / 255.0f) * c2.b);
+                }
+            }
+        }
+        acc.r = acc.r + c.r;
+        acc.g = acc.g + c.g;
+        acc.b = acc.b + c.b;
+    }
+    const float fs = static_cast<float>(a.spp);
+    a.rgb[3 * p + 0] = acc.r / fs;
+    a.rgb[3 * p + 1] = acc.g / fs;
+    a.rgb[3 * p + 2] = acc.b / fs;
+}
+
+// Per-sample bounce hits for the caller: the bounce queue is in compaction
+// order, so scatter back through the per-sample queue index.
+__global__ void __launch_bounds__(256) k_bounce_out(int64_t nsamples, const int32_t *qidx1, const rt_hit *hits2,
+                                                    rt_hit *out) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= nsamples) return;
+    const int32_t q = qidx1[i];
+    float4 v = make_float4(rtd::kMaxDist, 0.0f, 0.0f, __int_as_float(-1));
+    if (q >= 0) v = reinterpret_cast<const float4 *>(hits2)[q];
+    reinterpret_cast<float4 *>(out)[i] = v;
+}
+
+__global__ void k_zero_u32(uint32_t *p, int n) {
+    if (threadIdx.x < n) p[threadIdx.x] = 0u;
+}
+
+// Frame workspace, kept on the accel between frames.
+struct Workspace {
+    int64_t nsamples = 0;
+    int nlights = 0;
+    void *mem = nullptr;
+    size_t bytes = 0;
+    rt_ray *rays1, *shadow1, *bounce, *shadow2;
+    rt_hit *hits1, *hits2;
+    int32_t *qidx1, *qidx2, *hit_sample1, *hit_sample2;
+    float4 *nshade1, *nshade2;
+    uint8_t *occ1, *occ2;
+    float *rgb;
+    rt_hit *bounce_out;
+    uint32_t *counts;  // [0] primary hits, [1] bounce hits
+};
+
+template <class T>
+T *carve(char *&p, int64_t count) {
+    T *r = reinterpret_cast<T *>(p);
+    p += ((count * static_cast<int64_t>(sizeof(T)) + 255) / 256) * 256;
+    return r;
+}
+
+size_t layout(Workspace &w, char *base, int64_t ns, int nl) {
+    char *p = base;
+    w.rays1 = carve<rt_ray>(p, ns);
+    w.hits1 = carve<rt_hit>(p, ns);
+    w.qidx1 = carve<int32_t>(p, ns);
+    w.nshade1 = carve<float4>(p, ns);
+    w.hit_sample1 = carve<int32_t>(p, ns);
+    w.shadow1 = carve<rt_ray>(p, ns * nl);
+    w.occ1 = carve<uint8_t>(p, ns * nl);
+    w.bounce = carve<rt_ray>(p, ns);
+    w.hits2 = carve<rt_hit>(p, ns);
+    w.qidx2 = carve<int32_t>(p, ns);
+    w.nshade2 = carve<float4>(p, ns);
+    w.hit_sample2 = carve<int32_t>(p, ns);
+    w.shadow2 = carve<rt_ray>(p, ns * nl);
+    w.occ2 = carve<uint8_t>(p, ns * nl);
+    w.rgb = carve<float>(p, 3 * ns);
+    w.bounce_out = carve<rt_hit>(p, ns);
+    w.counts = carve<uint32_t>(p, 4);
+    return static_cast<size_t>(p - base);
+}
+
+}  // namespace
+
+static Workspace *get_workspace(rt_accel *a, int64_t ns, int nl, int32_t *err) {
+    Workspace *w = reinterpret_cast<Workspace *>(a->frame_ws);
+    if (!w) {
+        w = new Workspace();
+        a->frame_ws = w;
+    }
+    if (w->nsamples >= ns && w->nlights >= nl && w->mem) {
+        layout(*w, reinterpret_cast<char *>(w->mem), w->nsamples, w->nlights);
+        return w;
+    }
+    if (w->mem) (void)hipFree(w->mem);
+    w->mem = nullptr;
+    Workspace probe;
+    const size_t bytes = layout(probe, reinterpret_cast<char *>(0), ns, nl) + 256;
+    hipError_t e = hipMalloc(&w->mem, bytes);
+    if (e != hipSuccess) {
+        *err = fail(a->scene->ctx, e == hipErrorOutOfMemory ? RT_ERR_OOM : RT_ERR_HIP,
+                    std::string("frame workspace: ") + hipGetErrorString(e));
+        w->mem = nullptr;
+        w->nsamples = 0;
+        return nullptr;
+    }
+    w->bytes = bytes;
+    w->nsamples = ns;
+    w->nlights = nl;
+    layout(*w, reinterpret_cast<char *>(w->mem), ns, nl);
+    return w;
+}
+
+int32_t render_frame(rt_accel *a, const rt_camera &cam, const rt_light *lights, int nlights,
+                     const rt_render_params &p, float *rgb, rt_hit *primary, rt_hit *bounce_hits,
+                     rt_stats *stats) {
+    rt_ctx *ctx = a->scene->ctx;
+    hipStream_t s = ctx->stream;
+    if (nlights < 0 || nlights > 8) return fail(ctx, RT_ERR_INVALID, "nlights must be 0..8");
+    if (cam.x_res <= 0 || cam.y_res <= 0) return fail(ctx, RT_ERR_INVALID, "camera resolution must be positive");
+    const int spp = std::max(1, p.spp);
+    const int64_t npix = static_cast<int64_t>(cam.x_res) * cam.y_res;
+    const int64_t ns = npix * spp;
+    if (ns >= (int64_t(1) << 31)) return fail(ctx, RT_ERR_INVALID, "too many samples per frame");
+    int32_t err = RT_OK;
+    Workspace *w = get_workspace(a, ns, std::max(1, nlights), &err);
+    if (!w) return err;
+
+    CamArgs c;
+    c.px = cam.pos[0]; c.py = cam.pos[1]; c.pz = cam.pos[2];
+    c.xx = cam.x_axis[0]; c.xy = cam.x_axis[1]; c.xz = cam.x_axis[2];
+    c.yx = cam.y_axis[0]; c.yy = cam.y_axis[1]; c.yz = cam.y_axis[2];
+    c.zx = cam.z_axis[0]; c.zy = cam.z_axis[1]; c.zz = cam.z_axis[2];
+    // rt/camera.h:54-62
+    c.x_m = -cam.w;
+    c.y_m = -cam.h;
+    c.x_inc = (cam.w * 2.0f) / static_cast<float>(cam.x_res);
+    c.y_inc = (cam.h * 2.0f) / static_cast<float>(cam.y_res);
+    c.t = cam.t;
+    c.x_res = cam.x_res;
+    c.y_res = cam.y_res;
+    LightArgs L;
+    L.n = nlights;
+    for (int l = 0; l < nlights; ++l) {
+        L.pos_d[l] = make_float4(lights[l].pos[0], lights[l].pos[1], lights[l].pos[2], lights[l].d);
+        L.rgb[l] = make_float4(lights[l].rgb[0] / 255.0f, lights[l].rgb[1] / 255.0f, lights[l].rgb[2] / 255.0f, 0.0f);
+    }
+    const bool timed = stats != nullptr;
+    hipEvent_t *ev = ctx->ev;
+    auto rec = [&](int k) -> int32_t {
+        if (timed) RT_HIP_CHECK(ctx, hipEventRecord(ev[k], s));
+        return RT_OK;
+    };
+    int32_t rc;
+    const int nl1 = std::max(1, nlights);
+    auto grid = [](int64_t n) { return dim3(static_cast<unsigned>((n + 255) / 256)); };
+
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, w->counts, 4);
+    if ((rc = rec(0))) return rc;
+    hipLaunchKernelGGL(k_raygen, grid(ns), dim3(256), 0, s, c, spp, p.seed, ns, w->rays1);
+    if ((rc = rec(1))) return rc;
+    if ((rc = launch_closest(a, w->rays1, ns, w->hits1, s))) return rc;
+    if ((rc = rec(2))) return rc;
+    ShadeArgs sa{};
+    sa.rays = w->rays1;
+    sa.hits = w->hits1;
+    sa.count = nullptr;
+    sa.n_static = ns;
+    sa.tri_n = a->scene->d_n;
+    sa.tri_c = a->scene->d_c;
+    sa.lights = L;
+    sa.bounce = p.bounce;
+    sa.spp = spp;
+    sa.seed = p.seed;
+    sa.sample_of = nullptr;
+    sa.qidx = w->qidx1;
+    sa.nshade = w->nshade1;
+    sa.hit_sample = w->hit_sample1;
+    sa.shadow_q = w->shadow1;
+    sa.bounce_q = w->bounce;
+    sa.q_count = w->counts + 0;
+    hipLaunchKernelGGL(k_shade_hit<0>, grid(ns), dim3(256), 0, s, sa);
+    RT_HIP_CHECK(ctx, hipGetLastError());
+    if ((rc = rec(3))) return rc;
+    if (nlights > 0 && (rc = launch_occluded_q(a, w->shadow1, w->counts + 0, nlights, ns * nlights, w->occ1, s)))
+        return rc;
+    if ((rc = rec(4))) return rc;
+    if (p.bounce) {
+        if ((rc = launch_closest_q(a, w->bounce, w->counts + 0, 1, ns, w->hits2, s))) return rc;
+        if ((rc = rec(5))) return rc;
+        ShadeArgs sb{};
+        sb.rays = w->bounce;
+        sb.hits = w->hits2;
+        sb.count = w->counts + 0;
+        sb.n_static = ns;
+        sb.tri_n = a->scene->d_n;
+        sb.tri_c = a->scene->d_c;
+        sb.lights = L;
+        sb.bounce = 0;
+        sb.spp = spp;
+        sb.seed = p.seed;
+        sb.sample_of = w->hit_sample1;
+        sb.qidx = w->qidx2;
+        sb.nshade = w->nshade2;
+        sb.hit_sample = w->hit_sample2;
+        sb.shadow_q = w->shadow2;
+        sb.bounce_q = nullptr;
+        sb.q_count = w->counts + 1;
+        hipLaunchKernelGGL(k_shade_hit<1>, grid(ns), dim3(256), 0, s, sb);
+        RT_HIP_CHECK(ctx, hipGetLastError());
+        if ((rc = rec(6))) return rc;
+        if (nlights > 0 && (rc = launch_occluded_q(a, w->shadow2, w->counts + 1, nlights, ns * nlights, w->occ2, s)))
+            return rc;
+        if ((rc = rec(7))) return rc;
+    } else {
+        if ((rc = rec(5))) return rc;
+        if ((rc = rec(6))) return rc;
+        if ((rc = rec(7))) return rc;
+    }
+    ComposeArgs ca{};
+    ca.npix = npix;
+    ca.spp = spp;
+    ca.bounce = p.bounce;
+    ca.hits1 = w->hits1;
+    ca.qidx1 = w->qidx1;
+    ca.nshade1 = w->nshade1;
+    ca.shadow1 = w->shadow1;
+    ca.occ1 = w->occ1;
+    ca.qidx2 = w->qidx2;
+    ca.hits2 = w->hits2;
+    ca.nshade2 = w->nshade2;
+    ca.shadow2 = w->shadow2;
+    ca.occ2 = w->occ2;
+    ca.kd = a->scene->d_kd;
+    ca.lights = L;
+    const bool dev_out = (p.flags & RT_DEVICE_PTRS) != 0;
+    ca.rgb = (dev_out && rgb) ? rgb : w->rgb;
+    hipLaunchKernelGGL(k_compose, grid(npix), dim3(256), 0, s, ca);
+    RT_HIP_CHECK(ctx, hipGetLastError());
+    if ((rc = rec(8))) return rc;
+    (void)nl1;
+
+    // Outputs.
+    if (rgb && !dev_out)
+        RT_HIP_CHECK(ctx, hipMemcpyAsync(rgb, w->rgb, 3 * npix * sizeof(float), hipMemcpyDeviceToHost, s));
+    if (primary)
+        RT_HIP_CHECK(ctx, hipMemcpyAsync(primary, w->hits1, ns * sizeof(rt_hit),
+                                         dev_out ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+    if (bounce_hits) {
+        if (!p.bounce) return fail(ctx, RT_ERR_INVALID, "bounce hits requested without bounce");
+        hipLaunchKernelGGL(k_bounce_out, grid(ns), dim3(256), 0, s, ns, w->qidx1, w->hits2, w->bounce_out);
+        RT_HIP_CHECK(ctx, hipMemcpyAsync(bounce_hits, w->bounce_out, ns * sizeof(rt_hit),
+                                         dev_out ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+    }
+    uint32_t counts[2] = {0, 0};
+    if (stats) RT_HIP_CHECK(ctx, hipMemcpyAsync(counts, w->counts, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    RT_HIP_CHECK(ctx, hipStreamSynchronize(s));
+    if (stats) {
+        stats->rays[0] = static_cast<uint64_t>(ns);
+        stats->rays[1] = static_cast<uint64_t>(counts[0]) * nlights;
+        stats->rays[2] = p.bounce ? counts[0] : 0;
+        stats->rays[3] = p.bounce ? static_cast<uint64_t>(counts[1]) * nlights : 0;
+        float ms;
+        RT_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ev[0], ev[8]));
+        stats->ms[0] = ms;
+        RT_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ev[1], ev[2]));
+        stats->ms[1] = ms;
+        RT_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ev[3], ev[4]));
+        stats->ms[2] = ms;
+        RT_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ev[4], ev[5]));
+        stats->ms[3] = ms;
+        RT_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ev[6], ev[7]));
+        stats->ms[4] = ms;
+        float shade = 0.0f;
+        RT_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ev[0], ev[1]));
+        shade += ms;
+        RT_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ev[2], ev[3]));
+        shade += ms;
+        RT_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ev[5], ev[6]));
+        shade += ms;
+        RT_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ev[7], ev[8]));
+        shade += ms;
+        stats->ms[5] = shade;
+    }
+    return RT_OK;
+}
+
+void free_frame_workspace(rt_accel *a) {
+    Workspace *w = reinterpret_cast<Workspace *>(a->frame_ws);
+    if (!w) return;
+    if (w->mem) (void)hipFree(w->mem);
+    delete w;
+    a->frame_ws = nullptr;
+}
+
+}  // namespace rt

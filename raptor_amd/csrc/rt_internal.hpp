@@ -1,0 +1,119 @@
+// Internal types of libraptor_rt: the context, scene and acceleration
+// structure behind the opaque C handles, and the wide-node layout.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/raptor_rt.h"
+
+namespace rt {
+
+// Child encoding inside a wide node: >= 0 is an interior node index;
+// kLeafBit | (first << 4) | (count - 1) is a leaf of count triangles starting
+// at position first of the BVH-ordered triangle array; kEmpty is an unused slot.
+constexpr int32_t kEmpty = -1;
+constexpr uint32_t kLeafBit = 0x80000000u;
+constexpr int kMaxLeafSize = 16;
+
+__host__ __device__ inline bool is_leaf(int32_t c) { return c != kEmpty && (static_cast<uint32_t>(c) & kLeafBit); }
+__host__ __device__ inline int32_t leaf_code(int32_t first, int32_t count) {
+    return static_cast<int32_t>(kLeafBit | (static_cast<uint32_t>(first) << 4) | static_cast<uint32_t>(count - 1));
+}
+__host__ __device__ inline int32_t leaf_first(int32_t c) { return static_cast<int32_t>((static_cast<uint32_t>(c) & 0x7fffffffu) >> 4); }
+__host__ __device__ inline int32_t leaf_count(int32_t c) { return static_cast<int32_t>(static_cast<uint32_t>(c) & 0xfu) + 1; }
+
+// Wide node of W children, structure of arrays so one lane fetches the whole
+// node with 16-byte loads: lo_x[W], hi_x[W], lo_y[W], hi_y[W], lo_z[W], hi_z[W],
+// child[W]. 112 bytes for W = 4 (padded to 128), 224 for W = 8 (padded to 256).
+template <int W>
+struct alignas(W == 4 ? 128 : 256) WideNode {
+    float lo_x[W], hi_x[W], lo_y[W], hi_y[W], lo_z[W], hi_z[W];
+    int32_t child[W];
+    int32_t pad[W == 4 ? 4 : 8];
+};
+static_assert(sizeof(WideNode<4>) == 128, "BVH4 node is one 128-byte line");
+static_assert(sizeof(WideNode<8>) == 256, "BVH8 node is two 128-byte lines");
+
+// BVH-ordered triangle for the leaf test: a (xyz) + emplace id (w bits),
+// e1 = c - a, e2 = b - a. 48 bytes.
+struct alignas(16) LeafTri {
+    float4 a_id;
+    float4 e1;
+    float4 e2;
+};
+
+}  // namespace rt
+
+struct rt_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string last_error;
+    bool counting = false;
+    hipEvent_t ev[16] = {};
+    // Device counters for visits (node, tri) when counting is on.
+    unsigned long long *d_counters = nullptr;
+};
+
+struct rt_scene {
+    rt_ctx *ctx = nullptr;
+    int64_t n = 0;
+    float *d_abc = nullptr;    // n x 9, emplace order
+    float4 *d_n = nullptr;     // shading normal (xyz), flags in w bits
+    float4 *d_c = nullptr;     // vertex c (xyz) for the plane re-projection
+    float4 *d_kd = nullptr;    // kd = (255 * albedo) * 1.0
+    uint8_t *d_flags = nullptr;
+};
+
+struct rt_accel {
+    rt_scene *scene = nullptr;
+    int width = 4;
+    int leaf_size = 4;
+    int32_t root = 0;          // child code of the root (interior index or leaf)
+    int64_t nodes = 0, leaves = 0;
+    int depth = 0;
+    void *d_nodes = nullptr;   // WideNode<W>[nodes]
+    rt::LeafTri *d_tris = nullptr;  // BVH order
+    float build_ms = 0.0f;
+    float sah_cost = 0.0f;
+    float box_pad = 0.0f;
+    void *frame_ws = nullptr;  // frame workspace (render.hip)
+};
+
+namespace rt {
+
+// Error helpers: record text on the context and return a code.
+int32_t fail(rt_ctx *ctx, int32_t code, const std::string &msg);
+
+#define RT_HIP_CHECK(ctx, expr)                                                                          \
+    do {                                                                                                 \
+        hipError_t _e = (expr);                                                                          \
+        if (_e != hipSuccess) {                                                                          \
+            return ::rt::fail((ctx), (_e == hipErrorOutOfMemory) ? RT_ERR_OOM : RT_ERR_HIP,              \
+                              std::string(#expr) + ": " + hipGetErrorString(_e));                      \
+        }                                                                                                \
+    } while (0)
+
+// Build entry (bvh_build.hip).
+int32_t build_accel(rt_scene *scene, const rt_build_params &p, rt_accel **out);
+
+// Trace launchers (trace.hip). Rays, hits and flags are device pointers.
+int32_t launch_closest(rt_accel *a, const rt_ray *d_rays, int64_t n, rt_hit *d_hits, hipStream_t s);
+int32_t launch_occluded(rt_accel *a, const rt_ray *d_rays, int64_t n, uint8_t *d_occ, hipStream_t s);
+// Variants that read the ray count from device memory (queues of unknown size).
+// n = d_count[0] * mul, at most max_n.
+int32_t launch_closest_q(rt_accel *a, const rt_ray *d_rays, const uint32_t *d_count, uint32_t mul, int64_t max_n,
+                         rt_hit *d_hits, hipStream_t s);
+int32_t launch_occluded_q(rt_accel *a, const rt_ray *d_rays, const uint32_t *d_count, uint32_t mul, int64_t max_n,
+                          uint8_t *d_occ, hipStream_t s);
+
+// Frame (render.hip).
+void free_frame_workspace(rt_accel *a);
+int32_t render_frame(rt_accel *a, const rt_camera &cam, const rt_light *lights, int nlights,
+                     const rt_render_params &p, float *rgb, rt_hit *primary, rt_hit *bounce, rt_stats *stats);
+
+}  // namespace rt

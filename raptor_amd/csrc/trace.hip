@@ -1,0 +1,355 @@
+// Wide-BVH traversal kernels: closest hit and any hit.
+//
+// Replaces the scalar and 4-wide packet traversals of ssd/bvh.cc:341-759.
+// One ray per lane of a 64-wide wavefront. A lane visits a node by loading the
+// W child boxes (structure of arrays, 16-byte loads), testing them, sorting
+// the hits near-to-far with a sorting network and keeping the nearest as the
+// next node; the others go on a per-lane stack whose first kLdsDepth entries
+// live in LDS (one column per lane, so a wave's accesses hit 64 different
+// banks) and whose remainder spills to private memory. Stack entries carry
+// their entry distance so that closest-hit pops skip subtrees beyond the
+// current hit. Leaves are tested with the reference's Moller-Trumbore
+// (rt_device.hpp); an exact tie in t keeps the lowest emplace id.
+//
+// The box test is conservative: boxes are padded at build time, the inverse
+// direction never divides by zero, t_far is stretched by 2^-20 and the cull
+// against the current hit by 2^-18, so no box holding an accepted triangle is
+// skipped (the reference's own slack is the absolute +100*EPSILON of
+// ssd/bvh.cc:620,627,705).
+#include "rt_device.hpp"
+#include "rt_internal.hpp"
+
+namespace rt {
+
+namespace {
+
+constexpr int kLdsDepth = 16;
+constexpr int kSpillDepth = 112;
+constexpr float kFarStretch = 1.00000095f;   // 1 + 2^-20
+constexpr float kHitStretch = 1.0000038f;    // 1 + 2^-18
+
+struct TraceArgs {
+    const void *nodes;
+    const LeafTri *tris;
+    int32_t root;
+    const rt_ray *rays;
+    const uint32_t *count;   // if non-null, n = count[0] * count_mul
+    uint32_t count_mul;
+    int64_t n;
+    rt_hit *hits;
+    uint8_t *occ;
+    unsigned long long *counters;  // [0] node visits, [1] triangle tests
+    uint32_t *overflow;            // set to 1 if a stack overflowed
+};
+
+__device__ __forceinline__ float safe_inv(float d) {
+    return 1.0f / ((fabsf(d) > 1e-20f) ? d : copysignf(1e-20f, d));
+}
+
+template <int W>
+struct NodeView {
+    float lx[W], hx[W], ly[W], hy[W], lz[W], hz[W];
+    int32_t c[W];
+};
+
+template <int W>
+__device__ __forceinline__ void load_node(const WideNode<W> *__restrict__ nodes, int32_t idx, NodeView<W> &v) {
+    const float4 *p = reinterpret_cast<const float4 *>(nodes + idx);
+    constexpr int Q = W / 4;  // float4 per array
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const float4 a = p[0 * Q + q], b = p[1 * Q + q], c = p[2 * Q + q];
+        const float4 d = p[3 * Q + q], e = p[4 * Q + q], f = p[5 * Q + q];
+        const int4 g = reinterpret_cast<const int4 *>(p)[6 * Q + q];
+        v.lx[4 * q + 0] = a.x; v.lx[4 * q + 1] = a.y; v.lx[4 * q + 2] = a.z; v.lx[4 * q + 3] = a.w;
+        v.hx[4 * q + 0] = b.x; v.hx[4 * q + 1] = b.y; v.hx[4 * q + 2] = b.z; v.hx[4 * q + 3] = b.w;
+        v.ly[4 * q + 0] = c.x; v.ly[4 * q + 1] = c.y; v.ly[4 * q + 2] = c.z; v.ly[4 * q + 3] = c.w;
+        v.hy[4 * q + 0] = d.x; v.hy[4 * q + 1] = d.y; v.hy[4 * q + 2] = d.z; v.hy[4 * q + 3] = d.w;
+        v.lz[4 * q + 0] = e.x; v.lz[4 * q + 1] = e.y; v.lz[4 * q + 2] = e.z; v.lz[4 * q + 3] = e.w;
+        v.hz[4 * q + 0] = f.x; v.hz[4 * q + 1] = f.y; v.hz[4 * q + 2] = f.z; v.hz[4 * q + 3] = f.w;
+        v.c[4 * q + 0] = g.x; v.c[4 * q + 1] = g.y; v.c[4 * q + 2] = g.z; v.c[4 * q + 3] = g.w;
+    }
+}
+
+template <int W>
+__device__ __forceinline__ void box_test(const NodeView<W> &v, rtd::V3 o, rtd::V3 id, float tlimit, float *tn) {
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+        const float x0 = (v.lx[k] - o.x) * id.x, x1 = (v.hx[k] - o.x) * id.x;
+        const float y0 = (v.ly[k] - o.y) * id.y, y1 = (v.hy[k] - o.y) * id.y;
+        const float z0 = (v.lz[k] - o.z) * id.z, z1 = (v.hz[k] - o.z) * id.z;
+        const float tnear = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.0f));
+        const float tfar = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1));
+        const bool hit = (v.c[k] != kEmpty) && (tnear <= tfar * kFarStretch) && (tnear <= tlimit);
+        tn[k] = hit ? tnear : INFINITY;
+    }
+}
+
+__device__ __forceinline__ void cswap(float &ta, int32_t &ca, float &tb, int32_t &cb) {
+    const bool sw = tb < ta;
+    const float t = sw ? tb : ta;
+    const float u = sw ? ta : tb;
+    const int32_t c = sw ? cb : ca;
+    const int32_t d = sw ? ca : cb;
+    ta = t;
+    tb = u;
+    ca = c;
+    cb = d;
+}
+
+template <int W>
+__device__ __forceinline__ void sort_children(float *t, int32_t *c);
+
+template <>
+__device__ __forceinline__ void sort_children<4>(float *t, int32_t *c) {
+    cswap(t[0], c[0], t[1], c[1]);
+    cswap(t[2], c[2], t[3], c[3]);
+    cswap(t[0], c[0], t[2], c[2]);
+    cswap(t[1], c[1], t[3], c[3]);
+    cswap(t[1], c[1], t[2], c[2]);
+}
+
+template <>
+__device__ __forceinline__ void sort_children<8>(float *t, int32_t *c) {
+    // Batcher's odd-even merge sort network for 8 keys (19 comparators).
+    cswap(t[0], c[0], t[1], c[1]); cswap(t[2], c[2], t[3], c[3]);
+    cswap(t[4], c[4], t[5], c[5]); cswap(t[6], c[6], t[7], c[7]);
+    cswap(t[0], c[0], t[2], c[2]); cswap(t[1], c[1], t[3], c[3]);
+    cswap(t[4], c[4], t[6], c[6]); cswap(t[5], c[5], t[7], c[7]);
+    cswap(t[1], c[1], t[2], c[2]); cswap(t[5], c[5], t[6], c[6]);
+    cswap(t[0], c[0], t[4], c[4]); cswap(t[1], c[1], t[5], c[5]);
+    cswap(t[2], c[2], t[6], c[6]); cswap(t[3], c[3], t[7], c[7]);
+    cswap(t[2], c[2], t[4], c[4]); cswap(t[3], c[3], t[5], c[5]);
+    cswap(t[1], c[1], t[2], c[2]); cswap(t[3], c[3], t[4], c[4]);
+    cswap(t[5], c[5], t[6], c[6]);
+}
+
+// Per-lane stack: LDS column for the first kLdsDepth entries, private memory
+// beyond.
+struct LaneStack {
+    int2 (*lds)[64];
+    int lane;
+    int sp;
+    int2 spill[kSpillDepth];
+    bool overflow;
+    __device__ __forceinline__ void push(int32_t code, float t) {
+        const int2 e = make_int2(code, __float_as_int(t));
+        if (sp < kLdsDepth) {
+            lds[sp][lane] = e;
+        } else if (sp < kLdsDepth + kSpillDepth) {
+            spill[sp - kLdsDepth] = e;
+        } else {
+            overflow = true;
+            return;
+        }
+        ++sp;
+    }
+    __device__ __forceinline__ int2 pop() {
+        --sp;
+        return (sp < kLdsDepth) ? lds[sp][lane] : spill[sp - kLdsDepth];
+    }
+};
+
+template <int W, bool CLOSEST, bool COUNT>
+__global__ void __launch_bounds__(64) k_trace(TraceArgs a) {
+    __shared__ int2 lds_stack[kLdsDepth][64];
+    const int lane = threadIdx.x;
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * 64 + lane;
+    const int64_t n = a.count ? static_cast<int64_t>(a.count[0]) * a.count_mul : a.n;
+    if (i >= n) return;
+    const WideNode<W> *__restrict__ nodes = reinterpret_cast<const WideNode<W> *>(a.nodes);
+    const float4 r0 = reinterpret_cast<const float4 *>(a.rays)[2 * i];
+    const float4 r1 = reinterpret_cast<const float4 *>(a.rays)[2 * i + 1];
+    const rtd::V3 o = rtd::mk(r0.x, r0.y, r0.z);
+    const rtd::V3 d = rtd::mk(r1.x, r1.y, r1.z);
+    const float tmin = r0.w;
+    const float tmax = r1.w;
+    const rtd::V3 id = rtd::mk(safe_inv(d.x), safe_inv(d.y), safe_inv(d.z));
+
+    LaneStack st;
+    st.lds = lds_stack;
+    st.lane = lane;
+    st.sp = 0;
+    st.overflow = false;
+
+    float best_t = tmax, best_u = 0.0f, best_v = 0.0f;
+    int32_t best_id = -1;
+    bool occluded = false;
+    uint32_t visits = 0, tests = 0;
+    int32_t cur = a.root;
+    while (true) {
+        if (cur >= 0) {
+            NodeView<W> v;
+            load_node<W>(nodes, cur, v);
+            if (COUNT) ++visits;
+            float tn[W];
+            box_test<W>(v, o, id, best_t * kHitStretch, tn);
+            int32_t cc[W];
+#pragma unroll
+            for (int k = 0; k < W; ++k) cc[k] = v.c[k];
+            if (CLOSEST) sort_children<W>(tn, cc);
+            cur = kEmpty;
+            float cur_t = INFINITY;
+#pragma unroll
+            for (int k = W - 1; k >= 0; --k) {
+                if (tn[k] < INFINITY) {
+                    if (cur != kEmpty) st.push(cur, cur_t);
+                    cur = cc[k];
+                    cur_t = tn[k];
+                }
+            }
+        } else if (cur != kEmpty) {
+            const int32_t first = leaf_first(cur);
+            const int32_t count = leaf_count(cur);
+            for (int k = 0; k < count; ++k) {
+                const LeafTri lt = a.tris[first + k];
+                const int32_t pid = __float_as_int(lt.a_id.w);
+                if (COUNT) ++tests;
+                if (!CLOSEST && (__float_as_int(lt.e1.w) & 3)) continue;  // lights, transparent
+                float t, u, vv;
+                if (rtd::intersect_mt(o, d, rtd::f4v(lt.a_id), rtd::f4v(lt.e1), rtd::f4v(lt.e2), tmin, &t, &u, &vv)) {
+                    if (CLOSEST) {
+                        if (t < best_t || (t == best_t && best_id >= 0 && pid < best_id)) {
+                            best_t = t;
+                            best_u = u;
+                            best_v = vv;
+                            best_id = pid;
+                        }
+                    } else if (t < tmax) {
+                        occluded = true;
+                        break;
+                    }
+                }
+            }
+            cur = kEmpty;
+            if (!CLOSEST && occluded) break;
+        }
+        if (cur == kEmpty) {
+            bool found = false;
+            while (st.sp > 0) {
+                const int2 e = st.pop();
+                if (!CLOSEST || __int_as_float(e.y) <= best_t * kHitStretch) {
+                    cur = e.x;
+                    found = true;
+                    break;
+                }
+            }
+            if (!found) break;
+        }
+    }
+    if (CLOSEST) {
+        rt_hit h;
+        h.t = (best_id >= 0) ? best_t : rtd::kMaxDist;
+        h.u = (best_id >= 0) ? best_u : 0.0f;
+        h.v = (best_id >= 0) ? best_v : 0.0f;
+        h.prim = best_id;
+        reinterpret_cast<float4 *>(a.hits)[i] = make_float4(h.t, h.u, h.v, __int_as_float(h.prim));
+    } else {
+        a.occ[i] = occluded ? 1 : 0;
+    }
+    if (COUNT) {
+        unsigned long long vs = visits, ts = tests;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            vs += __shfl_xor(vs, off, 64);
+            ts += __shfl_xor(ts, off, 64);
+        }
+        if (lane == 0) {
+            atomicAdd(&a.counters[0], vs);
+            atomicAdd(&a.counters[1], ts);
+        }
+    }
+    if (st.overflow && a.overflow) atomicOr(a.overflow, 1u);
+}
+
+template <bool CLOSEST>
+int32_t launch(rt_accel *acc, TraceArgs &ta, int64_t max_n, hipStream_t s) {
+    rt_ctx *ctx = acc->scene->ctx;
+    if (max_n <= 0) return RT_OK;
+    ta.nodes = acc->d_nodes;
+    ta.tris = acc->d_tris;
+    ta.root = acc->root;
+    ta.counters = ctx->counting ? ctx->d_counters : nullptr;
+    ta.overflow = ctx->counting ? reinterpret_cast<uint32_t *>(ctx->d_counters + 2) : nullptr;
+    const int64_t blocks = (max_n + 63) / 64;
+    if (acc->root == kEmpty) {
+        // Empty scene: every ray misses.
+        return RT_OK;
+    }
+    const bool count = ta.counters != nullptr;
+    if (acc->width == 4) {
+        if (count)
+            hipLaunchKernelGGL((k_trace<4, CLOSEST, true>), dim3(blocks), dim3(64), 0, s, ta);
+        else
+            hipLaunchKernelGGL((k_trace<4, CLOSEST, false>), dim3(blocks), dim3(64), 0, s, ta);
+    } else {
+        if (count)
+            hipLaunchKernelGGL((k_trace<8, CLOSEST, true>), dim3(blocks), dim3(64), 0, s, ta);
+        else
+            hipLaunchKernelGGL((k_trace<8, CLOSEST, false>), dim3(blocks), dim3(64), 0, s, ta);
+    }
+    RT_HIP_CHECK(ctx, hipGetLastError());
+    return RT_OK;
+}
+
+__global__ void k_fill_miss(rt_hit *hits, uint8_t *occ, const uint32_t *count, uint32_t mul, int64_t n_static) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t n = count ? static_cast<int64_t>(count[0]) * mul : n_static;
+    if (i >= n) return;
+    if (hits) reinterpret_cast<float4 *>(hits)[i] = make_float4(rtd::kMaxDist, 0.0f, 0.0f, __int_as_float(-1));
+    if (occ) occ[i] = 0;
+}
+
+int32_t empty_scene(rt_accel *acc, rt_hit *hits, uint8_t *occ, const uint32_t *count, uint32_t mul, int64_t max_n,
+                    hipStream_t s) {
+    if (max_n <= 0) return RT_OK;
+    hipLaunchKernelGGL(k_fill_miss, dim3((max_n + 255) / 256), dim3(256), 0, s, hits, occ, count, mul, max_n);
+    RT_HIP_CHECK(acc->scene->ctx, hipGetLastError());
+    return RT_OK;
+}
+
+}  // namespace
+
+int32_t launch_closest(rt_accel *acc, const rt_ray *d_rays, int64_t n, rt_hit *d_hits, hipStream_t s) {
+    if (acc->root == kEmpty) return empty_scene(acc, d_hits, nullptr, nullptr, 1, n, s);
+    TraceArgs ta{};
+    ta.rays = d_rays;
+    ta.n = n;
+    ta.hits = d_hits;
+    return launch<true>(acc, ta, n, s);
+}
+
+int32_t launch_occluded(rt_accel *acc, const rt_ray *d_rays, int64_t n, uint8_t *d_occ, hipStream_t s) {
+    if (acc->root == kEmpty) return empty_scene(acc, nullptr, d_occ, nullptr, 1, n, s);
+    TraceArgs ta{};
+    ta.rays = d_rays;
+    ta.n = n;
+    ta.occ = d_occ;
+    return launch<false>(acc, ta, n, s);
+}
+
+int32_t launch_closest_q(rt_accel *acc, const rt_ray *d_rays, const uint32_t *d_count, uint32_t mul, int64_t max_n,
+                         rt_hit *d_hits, hipStream_t s) {
+    if (acc->root == kEmpty) return empty_scene(acc, d_hits, nullptr, d_count, mul, max_n, s);
+    TraceArgs ta{};
+    ta.rays = d_rays;
+    ta.count = d_count;
+    ta.count_mul = mul;
+    ta.n = max_n;
+    ta.hits = d_hits;
+    return launch<true>(acc, ta, max_n, s);
+}
+
+int32_t launch_occluded_q(rt_accel *acc, const rt_ray *d_rays, const uint32_t *d_count, uint32_t mul, int64_t max_n,
+                          uint8_t *d_occ, hipStream_t s) {
+    if (acc->root == kEmpty) return empty_scene(acc, nullptr, d_occ, d_count, mul, max_n, s);
+    TraceArgs ta{};
+    ta.rays = d_rays;
+    ta.count = d_count;
+    ta.count_mul = mul;
+    ta.n = max_n;
+    ta.occ = d_occ;
+    return launch<false>(acc, ta, max_n, s);
+}
+
+}  // namespace rt

@@ -1,0 +1,270 @@
+"""ctypes binding of libraptor_rt.so (include/raptor_rt.h) for tests and benchmarks.
+
+The library is the product: HIP kernels for gfx950 behind a C-ABI. This module
+only marshals numpy arrays and torch device pointers across that ABI. It has no
+fallback: if the shared library is missing, importing this module raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libraptor_rt.so")
+
+RT_OK = 0
+RT_DEVICE_PTRS = 1
+
+EPSILON = np.float32(1.0e-3)  # common/common.cc:14
+MAX_DIST = np.float32(np.finfo(np.float32).max)  # common/common.cc:17
+
+HIT_DTYPE = np.dtype([("t", "<f4"), ("u", "<f4"), ("v", "<f4"), ("prim", "<i4")])
+
+# Symbols include/raptor_rt.h declares.
+EXPORTS = [
+    "rt_ctx_create", "rt_ctx_destroy", "rt_last_error", "rt_scene_create", "rt_scene_destroy",
+    "rt_scene_normals", "rt_accel_build", "rt_accel_destroy", "rt_accel_info_get", "rt_trace_closest",
+    "rt_trace_occluded", "rt_render", "rt_set_counting", "rt_gen_random_triangles", "rt_gen_terrain",
+]
+
+
+class rt_camera(C.Structure):
+    _fields_ = [("pos", C.c_float * 3), ("x_axis", C.c_float * 3), ("y_axis", C.c_float * 3),
+                ("z_axis", C.c_float * 3), ("w", C.c_float), ("h", C.c_float), ("t", C.c_float),
+                ("x_res", C.c_int32), ("y_res", C.c_int32)]
+
+
+class rt_light(C.Structure):
+    _fields_ = [("pos", C.c_float * 3), ("rgb", C.c_float * 3), ("d", C.c_float)]
+
+
+class rt_build_params(C.Structure):
+    _fields_ = [("width", C.c_int32), ("leaf_size", C.c_int32), ("sah_bins", C.c_int32), ("reserved", C.c_int32)]
+
+
+class rt_render_params(C.Structure):
+    _fields_ = [("bounce", C.c_int32), ("spp", C.c_int32), ("seed", C.c_uint64), ("flags", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
+class rt_stats(C.Structure):
+    _fields_ = [("rays", C.c_uint64 * 4), ("ms", C.c_float * 6), ("visits", C.c_uint64 * 2)]
+
+
+class rt_accel_info(C.Structure):
+    _fields_ = [("nodes", C.c_int64), ("leaves", C.c_int64), ("triangles", C.c_int64), ("width", C.c_int32),
+                ("depth", C.c_int32), ("bytes", C.c_int64), ("build_ms", C.c_float), ("sah_cost", C.c_float)]
+
+
+def load_library(path=LIB_PATH):
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with `python raptor_amd/build.py` (no fallback exists)")
+    lib = C.CDLL(path)
+    vp, i32, i64, u32, u64, fp = C.c_void_p, C.c_int32, C.c_int64, C.c_uint32, C.c_uint64, C.c_void_p
+    sig = {
+        "rt_ctx_create": (i32, [i32, C.POINTER(vp)]),
+        "rt_ctx_destroy": (None, [vp]),
+        "rt_last_error": (C.c_char_p, [vp]),
+        "rt_scene_create": (i32, [vp, fp, fp, fp, i64, C.POINTER(vp)]),
+        "rt_scene_destroy": (None, [vp]),
+        "rt_scene_normals": (i32, [vp, fp]),
+        "rt_accel_build": (i32, [vp, C.POINTER(rt_build_params), C.POINTER(vp)]),
+        "rt_accel_destroy": (None, [vp]),
+        "rt_accel_info_get": (i32, [vp, C.POINTER(rt_accel_info)]),
+        "rt_trace_closest": (i32, [vp, fp, i64, fp, u32]),
+        "rt_trace_occluded": (i32, [vp, fp, i64, fp, u32]),
+        "rt_render": (i32, [vp, C.POINTER(rt_camera), C.POINTER(rt_light), i32, C.POINTER(rt_render_params),
+                            fp, fp, fp, C.POINTER(rt_stats)]),
+        "rt_set_counting": (i32, [vp, i32]),
+        "rt_gen_random_triangles": (i64, [u64, i64, C.c_float, C.c_float, C.c_float, fp, fp]),
+        "rt_gen_terrain": (i64, [i32, C.c_float, C.c_float, C.c_float, i32, i32, u64, C.c_float, C.c_float, fp, fp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+_LIB = load_library()
+
+
+def lib():
+    return _LIB
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        assert a.flags["C_CONTIGUOUS"]
+        return a.ctypes.data
+    return int(a.data_ptr())  # torch tensor
+
+
+class RTError(RuntimeError):
+    pass
+
+
+# ---------------------------------------------------------------- generators
+
+def gen_random_triangles(seed, n, spread=0.01, albedo_lo=0.0, albedo_hi=1.0):
+    """Triangles with centroids uniform in [0,1)^3, vertices centroid + U[-spread, spread)^3."""
+    abc = np.empty((n, 9), np.float32)
+    alb = np.empty((n, 3), np.float32)
+    r = _LIB.rt_gen_random_triangles(seed, n, spread, albedo_lo, albedo_hi, _ptr(abc), _ptr(alb))
+    if r != n:
+        raise RTError("rt_gen_random_triangles failed")
+    return abc, alb
+
+
+def gen_terrain(grid, amplitude=0.15, zoom=0.02, persistence=0.5, octaves=8, noise_seed=57, albedo_seed=2,
+                albedo_lo=0.2, albedo_hi=0.8):
+    """fBm value-noise heightfield of grid x grid vertices spanning [0,1] in x and z."""
+    n = _LIB.rt_gen_terrain(grid, amplitude, zoom, persistence, octaves, noise_seed, albedo_seed, albedo_lo,
+                            albedo_hi, None, None)
+    if n < 0:
+        raise RTError("rt_gen_terrain: bad grid")
+    abc = np.empty((n, 9), np.float32)
+    alb = np.empty((n, 3), np.float32)
+    _LIB.rt_gen_terrain(grid, amplitude, zoom, persistence, octaves, noise_seed, albedo_seed, albedo_lo, albedo_hi,
+                        _ptr(abc), _ptr(alb))
+    return abc, alb
+
+
+# ---------------------------------------------------------------- handles
+
+class Context:
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        rc = _LIB.rt_ctx_create(device, C.byref(h))
+        if rc != RT_OK:
+            raise RTError(f"rt_ctx_create failed ({rc})")
+        self.h = h
+
+    def check(self, rc, what):
+        if rc != RT_OK:
+            raise RTError(f"{what}: {rc} {_LIB.rt_last_error(self.h).decode()}")
+
+    def set_counting(self, on):
+        self.check(_LIB.rt_set_counting(self.h, 1 if on else 0), "rt_set_counting")
+
+    def close(self):
+        if self.h:
+            _LIB.rt_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Scene:
+    def __init__(self, ctx, abc, albedo=None, flags=None):
+        self.ctx = ctx
+        abc = np.ascontiguousarray(abc, np.float32).reshape(-1, 9)
+        self.n = abc.shape[0]
+        alb = None if albedo is None else np.ascontiguousarray(albedo, np.float32).reshape(-1, 3)
+        fl = None if flags is None else np.ascontiguousarray(flags, np.uint8)
+        h = C.c_void_p()
+        ctx.check(_LIB.rt_scene_create(ctx.h, _ptr(abc), _ptr(alb), _ptr(fl), self.n, C.byref(h)), "rt_scene_create")
+        self.h = h
+
+    def normals(self):
+        out = np.empty((self.n, 3), np.float32)
+        self.ctx.check(_LIB.rt_scene_normals(self.h, _ptr(out)), "rt_scene_normals")
+        return out
+
+    def close(self):
+        if self.h:
+            _LIB.rt_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Accel:
+    def __init__(self, scene, width=4, leaf_size=4, sah_bins=0):
+        self.scene = scene
+        self.ctx = scene.ctx
+        p = rt_build_params(width, leaf_size, sah_bins, 0)
+        h = C.c_void_p()
+        self.ctx.check(_LIB.rt_accel_build(scene.h, C.byref(p), C.byref(h)), "rt_accel_build")
+        self.h = h
+
+    def info(self):
+        i = rt_accel_info()
+        self.ctx.check(_LIB.rt_accel_info_get(self.h, C.byref(i)), "rt_accel_info_get")
+        return {k: getattr(i, k) for k, _ in rt_accel_info._fields_}
+
+    def trace_closest(self, rays):
+        """rays: (n, 8) float32 (o, tmin, d, tmax) on the host, or a device tensor."""
+        if isinstance(rays, np.ndarray):
+            rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+            out = np.empty(rays.shape[0], HIT_DTYPE)
+            self.ctx.check(_LIB.rt_trace_closest(self.h, _ptr(rays), rays.shape[0], _ptr(out), 0), "rt_trace_closest")
+            return out
+        import torch
+        n = rays.shape[0]
+        out = torch.empty((n, 4), dtype=torch.float32, device=rays.device)
+        self.ctx.check(_LIB.rt_trace_closest(self.h, _ptr(rays), n, _ptr(out), RT_DEVICE_PTRS), "rt_trace_closest")
+        return out
+
+    def trace_occluded(self, rays):
+        if isinstance(rays, np.ndarray):
+            rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+            out = np.empty(rays.shape[0], np.uint8)
+            self.ctx.check(_LIB.rt_trace_occluded(self.h, _ptr(rays), rays.shape[0], _ptr(out), 0),
+                           "rt_trace_occluded")
+            return out
+        import torch
+        n = rays.shape[0]
+        out = torch.empty(n, dtype=torch.uint8, device=rays.device)
+        self.ctx.check(_LIB.rt_trace_occluded(self.h, _ptr(rays), n, _ptr(out), RT_DEVICE_PTRS), "rt_trace_occluded")
+        return out
+
+    def render(self, camera, lights, bounce=False, spp=1, seed=0, want_hits=True, stats=True):
+        """One frame; returns dict(rgb (H,W,3), primary hits, bounce hits, stats)."""
+        cam = camera.to_c()
+        nl = len(lights)
+        larr = (rt_light * max(1, nl))(*[l.to_c() for l in lights])
+        p = rt_render_params(1 if bounce else 0, spp, seed, 0, 0)
+        npix = camera.x_res * camera.y_res
+        rgb = np.empty((camera.y_res, camera.x_res, 3), np.float32)
+        prim = np.empty(npix * spp, HIT_DTYPE) if want_hits else None
+        bnc = np.empty(npix * spp, HIT_DTYPE) if (want_hits and bounce) else None
+        st = rt_stats()
+        self.ctx.check(_LIB.rt_render(self.h, C.byref(cam), larr, nl, C.byref(p), _ptr(rgb), _ptr(prim), _ptr(bnc),
+                                      C.byref(st) if stats else None), "rt_render")
+        out = {"rgb": rgb, "primary": prim, "bounce": bnc}
+        if stats:
+            out["stats"] = {"rays": list(st.rays), "ms": list(st.ms), "visits": list(st.visits)}
+        return out
+
+    def render_device(self, camera, lights, rgb_dev, bounce=False, spp=1, seed=0):
+        """Frame into a device buffer (no host copies); returns the stats dict."""
+        cam = camera.to_c()
+        nl = len(lights)
+        larr = (rt_light * max(1, nl))(*[l.to_c() for l in lights])
+        p = rt_render_params(1 if bounce else 0, spp, seed, RT_DEVICE_PTRS, 0)
+        st = rt_stats()
+        self.ctx.check(_LIB.rt_render(self.h, C.byref(cam), larr, nl, C.byref(p), _ptr(rgb_dev), None, None,
+                                      C.byref(st)), "rt_render")
+        return {"rays": list(st.rays), "ms": list(st.ms), "visits": list(st.visits)}
+
+    def close(self):
+        if self.h:
+            _LIB.rt_accel_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,113 @@
+"""Synthetic workloads: scene, camera and light of each benchmark configuration.
+
+* random65k  — 65,536 random triangles (seed 1; centroids uniform in [0,1)^3,
+  vertices offset within +-0.01, albedo uniform in [0,1)^3), pinhole camera at
+  (0.5, 0.5, -1.5) looking +z with half widths 0.35 at distance 1, 256x256,
+  point light (0.5, 2, 0.5), one shadow ray per hit.
+* terrain1m  — 708 x 708 heightfield (999,698 triangles; fBm value noise,
+  8 octaves, lacunarity 2, gain 0.5, amplitude 0.15, zoom 0.02, noise seed 57,
+  albedo uniform in [0.2, 0.8)^3 with seed 2), 1920x1080 primary rays + one
+  shadow ray per hit + one cosine-weighted bounce with its shadow ray. Camera
+  at (0.5, 0.55, -0.35) looking at (0.5, 0, 0.6), half width 0.5.
+
+A light is rt/light.h:17-18's point light: rgb in [0,255], falloff d, radius 0.
+"""
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import rt
+
+
+@dataclass
+class Camera:
+    pos: tuple
+    x_axis: tuple
+    y_axis: tuple
+    z_axis: tuple
+    w: float
+    h: float
+    t: float
+    x_res: int
+    y_res: int
+
+    def to_c(self):
+        c = rt.rt_camera()
+        for k in range(3):
+            c.pos[k] = self.pos[k]
+            c.x_axis[k] = self.x_axis[k]
+            c.y_axis[k] = self.y_axis[k]
+            c.z_axis[k] = self.z_axis[k]
+        c.w, c.h, c.t = self.w, self.h, self.t
+        c.x_res, c.y_res = self.x_res, self.y_res
+        return c
+
+    def as_dict(self):
+        return dict(pos=list(self.pos), x_axis=list(self.x_axis), y_axis=list(self.y_axis), z_axis=list(self.z_axis),
+                    w=self.w, h=self.h, t=self.t, x_res=self.x_res, y_res=self.y_res)
+
+
+@dataclass
+class Light:
+    pos: tuple
+    rgb: tuple = (255.0, 255.0, 255.0)
+    d: float = 0.0
+
+    def to_c(self):
+        l = rt.rt_light()
+        for k in range(3):
+            l.pos[k] = self.pos[k]
+            l.rgb[k] = self.rgb[k]
+        l.d = self.d
+        return l
+
+
+def f32(v):
+    return tuple(float(np.float32(x)) for x in v)
+
+
+def look_at(pos, target, up=(0.0, 1.0, 0.0)):
+    """Camera basis (x, y, z) for a camera at pos looking at target, in float64, rounded to float32."""
+    p = np.asarray(pos, np.float64)
+    z = np.asarray(target, np.float64) - p
+    z /= np.linalg.norm(z)
+    x = np.cross(np.asarray(up, np.float64), z)
+    x /= np.linalg.norm(x)
+    y = np.cross(z, x)
+    return f32(x + 0.0), f32(y + 0.0), f32(z + 0.0)
+
+
+@dataclass
+class Workload:
+    name: str
+    abc: np.ndarray
+    albedo: np.ndarray
+    camera: Camera
+    lights: list
+    bounce: bool
+    spp: int = 1
+    seed: int = 0
+    meta: dict = field(default_factory=dict)
+
+
+def random65k(n=65536, res=256):
+    abc, alb = rt.gen_random_triangles(1, n, 0.01, 0.0, 1.0)
+    cam = Camera(f32((0.5, 0.5, -1.5)), (1.0, 0.0, 0.0), (0.0, 1.0, 0.0), (0.0, 0.0, 1.0), float(np.float32(0.35)),
+                 float(np.float32(0.35)), 1.0, res, res)
+    return Workload("random65k", abc, alb, cam, [Light(f32((0.5, 2.0, 0.5)))], bounce=False,
+                    meta=dict(triangles=n, resolution=[res, res]))
+
+
+def terrain(grid=708, x_res=1920, y_res=1080, bounce=True, seed=7):
+    abc, alb = rt.gen_terrain(grid, 0.15, 0.02, 0.5, 8, 57, 2, 0.2, 0.8)
+    pos = (0.5, 0.55, -0.35)
+    x, y, z = look_at(pos, (0.5, 0.0, 0.6))
+    w = 0.5
+    h = w * y_res / x_res
+    cam = Camera(f32(pos), x, y, z, float(np.float32(w)), float(np.float32(h)), 1.0, x_res, y_res)
+    return Workload(f"terrain{grid}", abc, alb, cam, [Light(f32((0.5, 2.0, 0.5)))], bounce=bounce, seed=seed,
+                    meta=dict(triangles=int(abc.shape[0]), grid=grid, resolution=[x_res, y_res]))
+
+
+def terrain1m(x_res=1920, y_res=1080):
+    return terrain(708, x_res, y_res)
