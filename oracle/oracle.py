@@ -179,19 +179,24 @@ class OracleScene:
             pass
 
 
+# Arrays are bound to names before their pointers are taken so that they stay
+# alive for the duration of each call.
+
 def intersect(abc9, ogn, d):
     tuv = np.empty(3, np.float32)
-    hit = LIB.oracle_intersect(_p(np.asarray(abc9, np.float32)), _p(np.asarray(ogn, np.float32)),
-                               _p(np.asarray(d, np.float32)), _p(tuv))
+    a, o, dd = np.asarray(abc9, np.float32), np.asarray(ogn, np.float32), np.asarray(d, np.float32)
+    hit = LIB.oracle_intersect(_p(a), _p(o), _p(dd), _p(tuv))
     return bool(hit), tuv
 
 
 def slab(low, high, ogn, i_rd):
-    return LIB.oracle_slab(_p(f32a(*low)), _p(f32a(*high)), _p(f32a(*ogn)), _p(f32a(*i_rd)))
+    a, b, c, d = f32a(*low), f32a(*high), f32a(*ogn), f32a(*i_rd)
+    return LIB.oracle_slab(_p(a), _p(b), _p(c), _p(d))
 
 
 def combined_sa(lo1, hi1, lo2, hi2):
-    return LIB.oracle_combined_sa(_p(f32a(*lo1)), _p(f32a(*hi1)), _p(f32a(*lo2)), _p(f32a(*hi2)))
+    a, b, c, d = f32a(*lo1), f32a(*hi1), f32a(*lo2), f32a(*hi2)
+    return LIB.oracle_combined_sa(_p(a), _p(b), _p(c), _p(d))
 
 
 def min_element_n(d):
@@ -222,13 +227,15 @@ def morton_sort(codes):
 
 def offset_start_point(dst, geo, d=1):
     out = np.empty(3, np.float32)
-    LIB.oracle_offset_start_point(_p(f32a(*dst)), _p(f32a(*geo)), d, _p(out))
+    a, b = f32a(*dst), f32a(*geo)
+    LIB.oracle_offset_start_point(_p(a), _p(b), d, _p(out))
     return out
 
 
 def bounce_dir(seed, pixel, sample, bounce, n):
     out = np.empty(3, np.float32)
-    LIB.oracle_bounce_dir(seed, pixel, sample, bounce, _p(np.asarray(n, np.float32)), _p(out))
+    nn = np.asarray(n, np.float32)
+    LIB.oracle_bounce_dir(seed, pixel, sample, bounce, _p(nn), _p(out))
     return out
 
 
