@@ -1,0 +1,182 @@
+#!/usr/bin/env python3
+"""Throughput of the ray-tracing hot path on one MI355X.
+
+Workload (one "step" = one frame): the 999,698-triangle fBm terrain (708 x 708
+grid), 1920x1080 primary rays, one shadow ray per hit, one cosine-weighted
+diffuse bounce per hit and one shadow ray per bounce hit, FP32. The scene and
+its BVH are resident in HBM before the timed region; the BVH build time is
+reported separately. value = rays traced (counted on the device) / wall time.
+
+The CPU baseline is the oracle's restatement of the reference's scalar tracer
+(the reference's own algorithm: AAC BVH, scalar traversal, same shading) on the
+host cores of the same machine, timed on a bounded sample of the same frame.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mrays/s at 1M tris (primary+shadow+diffuse); batched & adjoint vs plain on GPU"
+PEAK_HBM_GBS = 8000.0
+STAGES = ["primary_closest", "shadow_anyhit", "bounce_closest", "bounce_shadow_anyhit"]
+
+
+def algorithmic_bytes(width, rays, node_visits, tri_tests, closest):
+    """Bytes a launch must move: each ray in (32 B), each result out (16 B hit
+    or 1 B flag), each node visit reads width child boxes (24 B) and child
+    codes (4 B), each triangle test reads a, c-a, b-a (36 B) and the id (4 B)."""
+    return rays * (32 + (16 if closest else 1)) + node_visits * width * 28 + tri_tests * 40
+
+
+def cpu_baseline(w, target_s):
+    import oracle.oracle as O
+
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    cores = min(cores, int(os.environ.get("OMP_NUM_THREADS", cores)))
+    t0 = time.perf_counter()
+    scene = O.OracleScene(w.abc)
+    build_s = time.perf_counter() - t0
+    npix = w.camera.x_res * w.camera.y_res
+
+    def run(stride, threads):
+        t = time.perf_counter()
+        r = scene.render(w.albedo, w.camera, w.lights, bounce=w.bounce, seed=w.seed, pixel_stride=stride,
+                         nthreads=threads)
+        dt = time.perf_counter() - t
+        return int(r["counters"][:4].sum()), dt
+
+    # Probe on a sparse sample, then size the sample for about target_s seconds.
+    rays, dt = run(997, cores)
+    rate = rays / max(dt, 1e-6)
+    frame_rays = rays * 997
+    stride = max(1, int(frame_rays / max(rate * target_s, 1.0)))
+    rays, dt = run(stride, cores)
+    rays1, dt1 = run(max(stride, 1) * cores, 1)
+    return {
+        "value": rays / dt / 1e6,
+        "unit": "Mrays/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"every {stride}th pixel of the 1920x1080 frame ({rays} rays, {dt:.1f} s) on {cores} threads; "
+                  f"scalar IEEE path, AAC BVH built in {build_s:.2f} s",
+        "single_thread_value": rays1 / dt1 / 1e6,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--width", type=int, default=4)
+    ap.add_argument("--leaf-size", type=int, default=4)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (each GPU would trace its own share "
+              "of the rays against a replicated BVH). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from raptor_amd import rt, workloads
+
+    print("bench: generating terrain and building the BVH", file=sys.stderr, flush=True)
+    w = workloads.terrain1m()
+    ctx = rt.Context(0)
+    scene = rt.Scene(ctx, w.abc, w.albedo)
+    accel = rt.Accel(scene, width=args.width, leaf_size=args.leaf_size)
+    info = accel.info()
+    rgb = torch.empty((w.camera.y_res, w.camera.x_res, 3), dtype=torch.float32, device="cuda:0")
+
+    for _ in range(args.warmup):
+        accel.render_device(w.camera, w.lights, rgb, bounce=w.bounce, seed=w.seed)
+    print("bench: timing", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    per_step = []
+    for _ in range(args.steps):
+        per_step.append(accel.render_device(w.camera, w.lights, rgb, bounce=w.bounce, seed=w.seed))
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+
+    total_rays = sum(sum(s["rays"]) for s in per_step)
+    value = total_rays / elapsed / 1e6
+    rays = per_step[-1]["rays"]
+
+    # Dominant kernel and its roofline: one extra frame with visit counting.
+    ctx.set_counting(True)
+    counted = accel.render_device(w.camera, w.lights, rgb, bounce=w.bounce, seed=w.seed)
+    ctx.set_counting(False)
+    stage_ms = [sum(s["ms"][1 + k] for s in per_step) / len(per_step) for k in range(4)]
+    dom = max(range(4), key=lambda k: stage_ms[k])
+    closest = dom in (0, 2)
+    bytes_per_launch = algorithmic_bytes(args.width, counted["rays"][dom], counted["visits"][2 * dom],
+                                         counted["visits"][2 * dom + 1], closest)
+    achieved = bytes_per_launch / (stage_ms[dom] * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_terrain1m.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            t = json.load(f)
+        if t.get("kernel_stage") == STAGES[dom] and t.get("width") == args.width:
+            traffic = t.get("bytes_per_launch")
+
+    line = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "terrain1m: 999,698-triangle fBm terrain (708x708 grid), 1920x1080 primary + 1 shadow ray "
+                        "per hit + 1 cosine-weighted bounce + 1 bounce shadow ray, FP32",
+            "triangles": int(scene.n),
+            "resolution": [w.camera.x_res, w.camera.y_res],
+            "bvh_width": args.width,
+            "leaf_size": args.leaf_size,
+        },
+        "rays_per_step": {"primary": rays[0], "shadow": rays[1], "bounce": rays[2], "bounce_shadow": rays[3]},
+        "stage_ms": {STAGES[k]: round(stage_ms[k], 4) for k in range(4)},
+        "bvh": {"nodes": info["nodes"], "leaves": info["leaves"], "depth": info["depth"],
+                "bytes": info["bytes"], "build_ms": round(info["build_ms"], 2)},
+        "roofline": {
+            "bound": "hbm",
+            "kernel": f"k_trace<{args.width}> {STAGES[dom]}",
+            "achieved": round(achieved, 2),
+            "peak": PEAK_HBM_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / PEAK_HBM_GBS, 4),
+            "traffic": traffic,
+            "algorithmic_bytes_per_launch": int(bytes_per_launch),
+            "launch_ms": round(stage_ms[dom], 4),
+            "node_visits_per_ray": round(counted["visits"][2 * dom] / max(1, counted["rays"][dom]), 3),
+            "tri_tests_per_ray": round(counted["visits"][2 * dom + 1] / max(1, counted["rays"][dom]), 3),
+        },
+    }
+    if not args.no_cpu_baseline:
+        print("bench: CPU baseline", file=sys.stderr, flush=True)
+        cb = cpu_baseline(w, args.cpu_seconds)
+        line["cpu_baseline"] = cb
+        line["gpu_over_cpu"] = round(value / cb["value"], 2)
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

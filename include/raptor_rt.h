@@ -91,12 +91,13 @@ typedef struct {
  * ms: device time of the frame's kernels (HIP events on the context stream):
  * [0] whole frame, [1] primary closest hit, [2] shadow any hit,
  * [3] bounce closest hit, [4] bounce-shadow any hit, [5] ray generation and
- * shading. visits: [0] node visits, [1] triangle tests (0 unless counting is
- * on). */
+ * shading. visits: node visits and triangle tests of each of the four trace
+ * launches, [2*k] and [2*k+1] for k = primary, shadow, bounce, bounce-shadow
+ * (all zero unless counting is on). */
 typedef struct {
     uint64_t rays[4];
     float ms[6];
-    uint64_t visits[2];
+    uint64_t visits[8];
 } rt_stats;
 
 /* BVH statistics. */

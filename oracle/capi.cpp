@@ -179,6 +179,7 @@ int32_t oracle_render(const oracle_scene *s, const float *albedo, const oracle_c
                       const oracle_light *lights, int32_t nlights, const oracle_render_params *p,
                       oracle_render_out *out) {
     if (p->packet) return -1;  // packet path: not available in this build of the oracle
+    if (nlights < 0 || nlights > 8) return -2;
     return render_frame(s->s, albedo, *cam, lights, nlights, *p, *out);
 }
 

@@ -104,8 +104,9 @@ Ray shadow_ray(const SurfaceHit &s, const V3 &light_c) {
 
 // rt/materials/phong_shader.cc:21-64 for phong_shader(255*albedo, kd = 1):
 // ks = ka = 0, so only the diffuse term and the zero ambient remain.
-Colour shade_direct(const Colour &kd, const SurfaceHit &s, const std::vector<Ray> &illum,
-                    const std::vector<PointLight> &lights) {
+constexpr int kMaxLights = 8;
+
+Colour shade_direct(const Colour &kd, const SurfaceHit &s, const Ray *illum, const std::vector<PointLight> &lights) {
     Colour c;
     for (size_t l = 0; l < lights.size(); ++l) {
         const Ray &ir = illum[l];
@@ -152,7 +153,7 @@ struct LocalCounters {
 // fraction of unoccluded shadow rays (rt/raytracer.cc:44-64).
 Colour direct_light(const FrameCtx &f, const SurfaceHit &sh, const Colour &kd, uint8_t *vis_out, int ray_kind,
                     LocalCounters *lc) {
-    std::vector<Ray> illum(f.lights.size());
+    Ray illum[kMaxLights];
     for (size_t l = 0; l < f.lights.size(); ++l) {
         illum[l] = shadow_ray(sh, f.lights[l].c);
         const bool occluded = found_nearer_object(*f.s, illum[l], illum[l].length, &lc->trav);

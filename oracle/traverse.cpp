@@ -2,7 +2,7 @@
 //
 // Scalar BVH traversal, ssd/bvh.cc:586-759, with the leaf loops of
 // ssd/bvh_node.h:152-190, plus a brute-force checker.
-#include <vector>
+#include <cstdlib>
 
 #include "oracle.hpp"
 
@@ -19,10 +19,18 @@ struct StackElement {
 // 100 * EPSILON as the reference forms it (a float product).
 const float kSlack = 100.0f * kEpsilon;
 
+// common/common.h:63-65 MAX_BVH_STACK_HEIGHT is 100; the reference asserts on
+// overflow (ssd/bvh.cc:636). A fixed array in the caller's frame keeps each
+// query allocation-free and re-entrant.
+constexpr int kStackHeight = 128;
+struct Stack {
+    StackElement e[kStackHeight];
+};
+
 // ssd/bvh.cc:586-647 — near-first descent to the next leaf. Returns false when
 // both children are culled; the far child is pushed when both are needed.
-bool find_leaf_node(const Scene &s, const Ray &r, int *entry_idx, std::vector<StackElement> &stack, int *sp,
-                    const V3 &i_rd, float t_max, Counters *c) {
+bool find_leaf_node(const Scene &s, const Ray &r, int *entry_idx, Stack &stack, int *sp, const V3 &i_rd,
+                    float t_max, Counters *c) {
     int cur = *entry_idx;
     while (true) {
         const BvhNode &node = s.nodes[cur];
@@ -44,9 +52,9 @@ bool find_leaf_node(const Scene &s, const Ray &r, int *entry_idx, std::vector<St
             cur = near_idx;
         } else {
             ++*sp;
-            if (*sp >= static_cast<int>(stack.size())) stack.resize(stack.size() * 2);
-            stack[*sp].idx = far_idx;
-            stack[*sp].t_min = far_dist;
+            if (*sp >= kStackHeight) std::abort();  // the reference asserts here too
+            stack.e[*sp].idx = far_idx;
+            stack.e[*sp].t_min = far_dist;
             cur = near_idx;
         }
     }
@@ -95,13 +103,12 @@ int find_nearest_object(const Scene &s, const Ray &r, Hit *h, Counters *c) {
     if (slab_distance(s.nodes[s.root], r.ogn, ray_dir_inv) == kMaxDist) {
         return -1;
     }
-    std::vector<StackElement> stack(128);
+    Stack stack;
     int sp = 0;
-    stack[0].idx = s.root;
-    stack[0].t_min = kMaxDist;
+    stack.e[0].idx = s.root;
+    stack.e[0].t_min = kMaxDist;
     int entry_idx = s.root;
     float entry_t_min = kMaxDist;
-    (void)entry_t_min;
     Hit nearest;
     int hit_object = -1;
     while (true) {
@@ -115,8 +122,8 @@ int find_nearest_object(const Scene &s, const Ray &r, Hit *h, Counters *c) {
                 *h = nearest;
                 return hit_object;
             }
-            entry_idx = stack[sp].idx;
-            entry_t_min = stack[sp].t_min;
+            entry_idx = stack.e[sp].idx;
+            entry_t_min = stack.e[sp].t_min;
             --sp;
         } while (entry_t_min > (nearest.d + kSlack));
     }
@@ -125,17 +132,17 @@ int find_nearest_object(const Scene &s, const Ray &r, Hit *h, Counters *c) {
 // ssd/bvh.cc:711-759
 bool found_nearer_object(const Scene &s, const Ray &r, float t, Counters *c) {
     if (s.root < 0) return false;
-    std::vector<StackElement> stack(128);
+    Stack stack;
     int sp = 0;
-    stack[0].idx = s.root;
-    stack[0].t_min = kMaxDist;
+    stack.e[0].idx = s.root;
+    stack.e[0].t_min = kMaxDist;
     int entry_idx = s.root;
     const V3 ray_dir_inv = 1.0f / r.dir;
     while (true) {
         const bool leaf = find_leaf_node(s, r, &entry_idx, stack, &sp, ray_dir_inv, t, c);
         if (leaf && test_leaf_node_nearer(s, s.nodes[entry_idx], r, t, c)) return true;
         if (sp == 0) return false;
-        entry_idx = stack[sp].idx;
+        entry_idx = stack.e[sp].idx;
         --sp;
     }
 }

@@ -93,11 +93,11 @@ int32_t rt_ctx_create(int32_t hip_device, rt_ctx **out) {
             return RT_ERR_HIP;
         }
     }
-    if (hipMalloc(&ctx->d_counters, 4 * sizeof(unsigned long long)) != hipSuccess) {
+    if (hipMalloc(&ctx->d_counters, 16 * sizeof(unsigned long long)) != hipSuccess) {
         rt_ctx_destroy(ctx);
         return RT_ERR_HIP;
     }
-    (void)hipMemset(ctx->d_counters, 0, 4 * sizeof(unsigned long long));
+    (void)hipMemset(ctx->d_counters, 0, 16 * sizeof(unsigned long long));
     *out = ctx;
     return RT_OK;
 }
@@ -257,16 +257,17 @@ int32_t rt_render(rt_accel *accel, const rt_camera *cam, const rt_light *lights,
     if (int32_t rc = rt::set_device(ctx)) return rc;
     rt_render_params p{0, 1, 0, 0, 0};
     if (params) p = *params;
-    if (ctx->counting) RT_HIP_CHECK(ctx, hipMemsetAsync(ctx->d_counters, 0, 4 * sizeof(unsigned long long), ctx->stream));
+    if (ctx->counting)
+        RT_HIP_CHECK(ctx, hipMemsetAsync(ctx->d_counters, 0, 16 * sizeof(unsigned long long), ctx->stream));
     int32_t rc = rt::render_frame(accel, *cam, lights, nlights, p, rgb, primary, bounce, stats);
+    ctx->count_slot = 0;
     if (rc == RT_OK && stats) {
-        stats->visits[0] = stats->visits[1] = 0;
+        for (auto &v : stats->visits) v = 0;
         if (ctx->counting) {
-            unsigned long long v[4];
+            unsigned long long v[16];
             RT_HIP_CHECK(ctx, hipMemcpy(v, ctx->d_counters, sizeof(v), hipMemcpyDeviceToHost));
-            stats->visits[0] = v[0];
-            stats->visits[1] = v[1];
-            if (v[2]) return fail(ctx, RT_ERR_INVALID, "traversal stack overflow");
+            for (int k = 0; k < 8; ++k) stats->visits[k] = v[k];
+            if (v[15]) return fail(ctx, RT_ERR_INVALID, "traversal stack overflow");
         }
     }
     return rc;

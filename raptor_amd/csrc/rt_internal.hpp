@@ -55,8 +55,10 @@ struct rt_ctx {
     std::string last_error;
     bool counting = false;
     hipEvent_t ev[16] = {};
-    // Device counters for visits (node, tri) when counting is on.
+    // Device counters when counting is on: (node visits, triangle tests) for
+    // slots 0..3, the stack-overflow flag at index 15.
     unsigned long long *d_counters = nullptr;
+    int count_slot = 0;
 };
 
 struct rt_scene {

@@ -269,8 +269,8 @@ int32_t launch(rt_accel *acc, TraceArgs &ta, int64_t max_n, hipStream_t s) {
     ta.nodes = acc->d_nodes;
     ta.tris = acc->d_tris;
     ta.root = acc->root;
-    ta.counters = ctx->counting ? ctx->d_counters : nullptr;
-    ta.overflow = ctx->counting ? reinterpret_cast<uint32_t *>(ctx->d_counters + 2) : nullptr;
+    ta.counters = ctx->counting ? ctx->d_counters + 2 * ctx->count_slot : nullptr;
+    ta.overflow = ctx->counting ? reinterpret_cast<uint32_t *>(ctx->d_counters + 15) : nullptr;
     const int64_t blocks = (max_n + 63) / 64;
     if (acc->root == kEmpty) {
         // Empty scene: every ray misses.

@@ -48,7 +48,7 @@ class rt_render_params(C.Structure):
 
 
 class rt_stats(C.Structure):
-    _fields_ = [("rays", C.c_uint64 * 4), ("ms", C.c_float * 6), ("visits", C.c_uint64 * 2)]
+    _fields_ = [("rays", C.c_uint64 * 4), ("ms", C.c_float * 6), ("visits", C.c_uint64 * 8)]
 
 
 class rt_accel_info(C.Structure):
