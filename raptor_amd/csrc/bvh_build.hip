@@ -8,7 +8,7 @@
 //   3. radix sort     stable (code, index) sort (radix_sort.hpp)
 //   4. k_karras       binary radix tree over the sorted codes (Karras 2012),
 //                     duplicates split by sorted position
-//   5. k_refit        bottom-up boxes with one atomic arrival flag per node
+//   5. refit          bottom-up boxes, one launch per tree level
 //   6. k_collapse     top-down, one level per launch: each wide node takes its
 //                     binary node's children and keeps opening the child of
 //                     largest surface area until it has W children; subtrees
@@ -106,11 +106,15 @@ __global__ void __launch_bounds__(256) k_morton(const float4 *__restrict__ aabb_
     if (i >= n) return;
     const float4 lo = aabb_lo[i], hi = aabb_hi[i];
     const float c[3] = {(lo.x + hi.x) * 0.5f, (lo.y + hi.y) * 0.5f, (lo.z + hi.z) * 0.5f};
+    // One cell size for all axes (the largest centroid extent): a flat scene
+    // such as a heightfield then spends few code bits on its thin axis.
+    float ext = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) ext = fmaxf(ext, ord2f(bounds[3 + k]) - ord2f(bounds[k]));
     uint32_t q[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        const float bmin = ord2f(bounds[k]), bmax = ord2f(bounds[3 + k]);
-        const float ext = bmax - bmin;
+        const float bmin = ord2f(bounds[k]);
         const float f = (ext > 0.0f) ? (c[k] - bmin) / ext : 0.0f;
         q[k] = static_cast<uint32_t>(fminf(fmaxf(f * 1024.0f, 0.0f), 1023.0f));
     }
@@ -179,6 +183,8 @@ __global__ void __launch_bounds__(256) k_gather_leaf_boxes(const uint32_t *__res
     leaf_hi[k] = aabb_hi[i];
 }
 
+constexpr int kMaxLevels = 256;
+
 __device__ __forceinline__ void child_box(int32_t c, const float4 *node_lo, const float4 *node_hi,
                                           const float4 *leaf_lo, const float4 *leaf_hi, float4 *lo, float4 *hi) {
     if (c >= 0) {
@@ -190,27 +196,45 @@ __device__ __forceinline__ void child_box(int32_t c, const float4 *node_lo, cons
     }
 }
 
-// Bottom-up refit: the second thread to reach a node computes its box.
-__global__ void __launch_bounds__(256) k_refit(int64_t n, const int2 *__restrict__ child,
-                                               const int32_t *__restrict__ parent,
-                                               const int32_t *__restrict__ leaf_parent, const float4 *leaf_lo,
-                                               const float4 *leaf_hi, float4 *node_lo, float4 *node_hi,
-                                               uint32_t *__restrict__ flags) {
-    const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    int32_t p = leaf_parent[k];
-    while (p >= 0) {
-        __threadfence();
-        if (atomicAdd(&flags[p], 1u) == 0u) return;
-        __threadfence();
-        const int2 ch = child[p];
-        float4 l0, h0, l1, h1;
-        child_box(ch.x, node_lo, node_hi, leaf_lo, leaf_hi, &l0, &h0);
-        child_box(ch.y, node_lo, node_hi, leaf_lo, leaf_hi, &l1, &h1);
-        node_lo[p] = make_float4(fminf(l0.x, l1.x), fminf(l0.y, l1.y), fminf(l0.z, l1.z), 0.0f);
-        node_hi[p] = make_float4(fmaxf(h0.x, h1.x), fmaxf(h0.y, h1.y), fmaxf(h0.z, h1.z), 0.0f);
+// Refit without cross-workgroup fences: MI355X's L2 is per XCD, so a device-
+// scope fence per tree level (the usual atomic bottom-up refit) writes back L2
+// every step. Instead each internal node finds its depth by walking parent
+// links, nodes are bucketed by depth, and one launch per level (deepest first)
+// unions the two child boxes; kernel boundaries order the levels.
+__global__ void __launch_bounds__(256) k_node_depth(int64_t n_internal, const int32_t *__restrict__ parent,
+                                                    int32_t *__restrict__ depth, uint32_t *__restrict__ level_count) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n_internal) return;
+    int32_t d = 0;
+    int32_t p = static_cast<int32_t>(i);
+    while (p != 0) {
         p = parent[p];
+        ++d;
     }
+    depth[i] = d;
+    atomicAdd(&level_count[min(d, kMaxLevels - 1)], 1u);
+}
+
+__global__ void __launch_bounds__(256) k_bucket_by_depth(int64_t n_internal, const int32_t *__restrict__ depth,
+                                                         uint32_t *__restrict__ cursor, int32_t *__restrict__ order) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n_internal) return;
+    const uint32_t pos = atomicAdd(&cursor[depth[i]], 1u);
+    order[pos] = static_cast<int32_t>(i);
+}
+
+__global__ void __launch_bounds__(256) k_refit_level(const int32_t *__restrict__ nodes_at_level, int count,
+                                                     const int2 *__restrict__ child, const float4 *leaf_lo,
+                                                     const float4 *leaf_hi, float4 *node_lo, float4 *node_hi) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    const int32_t p = nodes_at_level[k];
+    const int2 ch = child[p];
+    float4 l0, h0, l1, h1;
+    child_box(ch.x, node_lo, node_hi, leaf_lo, leaf_hi, &l0, &h0);
+    child_box(ch.y, node_lo, node_hi, leaf_lo, leaf_hi, &l1, &h1);
+    node_lo[p] = make_float4(fminf(l0.x, l1.x), fminf(l0.y, l1.y), fminf(l0.z, l1.z), 0.0f);
+    node_hi[p] = make_float4(fmaxf(h0.x, h1.x), fmaxf(h0.y, h1.y), fmaxf(h0.z, h1.z), 0.0f);
 }
 
 __device__ __forceinline__ float half_area(float4 lo, float4 hi) {
@@ -431,7 +455,7 @@ int32_t build_accel(rt_scene *scene, const rt_build_params &p, rt_accel **out) {
         return RT_OK;
     }
 
-    DevBuf b_child, b_range, b_parent, b_leaf_parent, b_leaf_lo, b_leaf_hi, b_node_lo, b_node_hi, b_flags;
+    DevBuf b_child, b_range, b_parent, b_leaf_parent, b_leaf_lo, b_leaf_hi, b_node_lo, b_node_hi;
     BUILD_CHECK(dalloc<int2>(b_child, n - 1));
     BUILD_CHECK(dalloc<int2>(b_range, n - 1));
     BUILD_CHECK(dalloc<int32_t>(b_parent, n - 1));
@@ -440,19 +464,51 @@ int32_t build_accel(rt_scene *scene, const rt_build_params &p, rt_accel **out) {
     BUILD_CHECK(dalloc<float4>(b_leaf_hi, n));
     BUILD_CHECK(dalloc<float4>(b_node_lo, n - 1));
     BUILD_CHECK(dalloc<float4>(b_node_hi, n - 1));
-    BUILD_CHECK(dalloc<uint32_t>(b_flags, n - 1));
-    BUILD_CHECK(hipMemsetAsync(b_flags.p, 0, (n - 1) * sizeof(uint32_t), s));
     const int iblocks = static_cast<int>((n - 1 + 255) / 256);
     hipLaunchKernelGGL(k_karras, dim3(iblocks), dim3(256), 0, s, (const uint32_t *)b_codes.p, n, (int2 *)b_child.p,
                        (int2 *)b_range.p, (int32_t *)b_parent.p, (int32_t *)b_leaf_parent.p);
     hipLaunchKernelGGL(k_gather_leaf_boxes, dim3(blocks), dim3(256), 0, s, (const uint32_t *)b_idx.p, n,
                        (const float4 *)b_aabb_lo.p, (const float4 *)b_aabb_hi.p, (float4 *)b_leaf_lo.p,
                        (float4 *)b_leaf_hi.p);
-    hipLaunchKernelGGL(k_refit, dim3(blocks), dim3(256), 0, s, n, (const int2 *)b_child.p,
-                       (const int32_t *)b_parent.p, (const int32_t *)b_leaf_parent.p, (const float4 *)b_leaf_lo.p,
-                       (const float4 *)b_leaf_hi.p, (float4 *)b_node_lo.p, (float4 *)b_node_hi.p,
-                       (uint32_t *)b_flags.p);
-    BUILD_CHECK(hipGetLastError());
+    // Level-synchronous refit (see k_node_depth).
+    {
+        DevBuf b_depth, b_levels, b_order;
+        BUILD_CHECK(dalloc<int32_t>(b_depth, n - 1));
+        BUILD_CHECK(dalloc<uint32_t>(b_levels, 2 * kMaxLevels));
+        BUILD_CHECK(dalloc<int32_t>(b_order, n - 1));
+        uint32_t *level_count = (uint32_t *)b_levels.p, *cursor = level_count + kMaxLevels;
+        BUILD_CHECK(hipMemsetAsync(level_count, 0, 2 * kMaxLevels * sizeof(uint32_t), s));
+        hipLaunchKernelGGL(k_node_depth, dim3(iblocks), dim3(256), 0, s, n - 1, (const int32_t *)b_parent.p,
+                           (int32_t *)b_depth.p, level_count);
+        uint32_t counts[kMaxLevels];
+        BUILD_CHECK(hipMemcpyAsync(counts, level_count, sizeof(counts), hipMemcpyDeviceToHost, s));
+        BUILD_CHECK(hipStreamSynchronize(s));
+        if (counts[kMaxLevels - 1]) {
+            cleanup();
+            return fail(ctx, RT_ERR_INVALID, "binary tree deeper than 255 levels");
+        }
+        uint32_t offs[kMaxLevels];
+        uint32_t run = 0;
+        int max_level = 0;
+        for (int l = 0; l < kMaxLevels; ++l) {
+            offs[l] = run;
+            run += counts[l];
+            if (counts[l]) max_level = l;
+        }
+        BUILD_CHECK(hipMemcpyAsync(cursor, offs, sizeof(offs), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_bucket_by_depth, dim3(iblocks), dim3(256), 0, s, n - 1, (const int32_t *)b_depth.p,
+                           cursor, (int32_t *)b_order.p);
+        for (int l = max_level; l >= 0; --l) {
+            if (!counts[l]) continue;
+            hipLaunchKernelGGL(k_refit_level, dim3((counts[l] + 255) / 256), dim3(256), 0, s,
+                               (const int32_t *)b_order.p + offs[l], static_cast<int>(counts[l]),
+                               (const int2 *)b_child.p, (const float4 *)b_leaf_lo.p, (const float4 *)b_leaf_hi.p,
+                               (float4 *)b_node_lo.p, (float4 *)b_node_hi.p);
+        }
+        BUILD_CHECK(hipGetLastError());
+        BUILD_CHECK(hipStreamSynchronize(s));
+        a->binary_depth = max_level + 1;
+    }
 
     // Wide nodes: at most n - 1 (each wide node consumes >= 1 binary node).
     const size_t node_bytes = (W == 4) ? sizeof(WideNode<4>) : sizeof(WideNode<8>);

@@ -378,7 +378,7 @@ int32_t render_frame(rt_accel *a, const rt_camera &cam, const rt_light *lights, 
     hipLaunchKernelGGL(k_raygen, grid(ns), dim3(256), 0, s, c, spp, p.seed, ns, w->rays1);
     if ((rc = rec(1))) return rc;
     ctx->count_slot = 0;
-    if ((rc = launch_closest(a, w->rays1, ns, w->hits1, s))) return rc;
+    if ((rc = launch_closest(a, w->rays1, ns, w->hits1, s, 0))) return rc;
     if ((rc = rec(2))) return rc;
     ShadeArgs sa{};
     sa.rays = w->rays1;
@@ -402,12 +402,12 @@ int32_t render_frame(rt_accel *a, const rt_camera &cam, const rt_light *lights, 
     RT_HIP_CHECK(ctx, hipGetLastError());
     if ((rc = rec(3))) return rc;
     ctx->count_slot = 1;
-    if (nlights > 0 && (rc = launch_occluded_q(a, w->shadow1, w->counts + 0, nlights, ns * nlights, w->occ1, s)))
+    if (nlights > 0 && (rc = launch_occluded_q(a, w->shadow1, w->counts + 0, nlights, ns * nlights, w->occ1, s, 1)))
         return rc;
     if ((rc = rec(4))) return rc;
     if (p.bounce) {
         ctx->count_slot = 2;
-        if ((rc = launch_closest_q(a, w->bounce, w->counts + 0, 1, ns, w->hits2, s))) return rc;
+        if ((rc = launch_closest_q(a, w->bounce, w->counts + 0, 1, ns, w->hits2, s, 2))) return rc;
         if ((rc = rec(5))) return rc;
         ShadeArgs sb{};
         sb.rays = w->bounce;
@@ -431,7 +431,7 @@ int32_t render_frame(rt_accel *a, const rt_camera &cam, const rt_light *lights, 
         RT_HIP_CHECK(ctx, hipGetLastError());
         if ((rc = rec(6))) return rc;
         ctx->count_slot = 3;
-        if (nlights > 0 && (rc = launch_occluded_q(a, w->shadow2, w->counts + 1, nlights, ns * nlights, w->occ2, s)))
+        if (nlights > 0 && (rc = launch_occluded_q(a, w->shadow2, w->counts + 1, nlights, ns * nlights, w->occ2, s, 3)))
             return rc;
         if ((rc = rec(7))) return rc;
     } else {

@@ -78,6 +78,7 @@ struct rt_accel {
     int32_t root = 0;          // child code of the root (interior index or leaf)
     int64_t nodes = 0, leaves = 0;
     int depth = 0;
+    int binary_depth = 0;
     void *d_nodes = nullptr;   // WideNode<W>[nodes]
     rt::LeafTri *d_tris = nullptr;  // BVH order
     float build_ms = 0.0f;
@@ -104,14 +105,16 @@ int32_t fail(rt_ctx *ctx, int32_t code, const std::string &msg);
 int32_t build_accel(rt_scene *scene, const rt_build_params &p, rt_accel **out);
 
 // Trace launchers (trace.hip). Rays, hits and flags are device pointers.
-int32_t launch_closest(rt_accel *a, const rt_ray *d_rays, int64_t n, rt_hit *d_hits, hipStream_t s);
-int32_t launch_occluded(rt_accel *a, const rt_ray *d_rays, int64_t n, uint8_t *d_occ, hipStream_t s);
-// Variants that read the ray count from device memory (queues of unknown size).
-// n = d_count[0] * mul, at most max_n.
+// stage tags the kernel symbol: 0 primary, 1 shadow, 2 bounce, 3 bounce
+// shadow, 4 / 5 direct API calls (closest / any hit).
+int32_t launch_closest(rt_accel *a, const rt_ray *d_rays, int64_t n, rt_hit *d_hits, hipStream_t s, int stage = 4);
+int32_t launch_occluded(rt_accel *a, const rt_ray *d_rays, int64_t n, uint8_t *d_occ, hipStream_t s, int stage = 5);
+// Variants that read the ray count from device memory (queues of unknown
+// size): n = d_count[0] * mul, at most max_n.
 int32_t launch_closest_q(rt_accel *a, const rt_ray *d_rays, const uint32_t *d_count, uint32_t mul, int64_t max_n,
-                         rt_hit *d_hits, hipStream_t s);
+                         rt_hit *d_hits, hipStream_t s, int stage);
 int32_t launch_occluded_q(rt_accel *a, const rt_ray *d_rays, const uint32_t *d_count, uint32_t mul, int64_t max_n,
-                          uint8_t *d_occ, hipStream_t s);
+                          uint8_t *d_occ, hipStream_t s, int stage);
 
 // Frame (render.hip).
 void free_frame_workspace(rt_accel *a);

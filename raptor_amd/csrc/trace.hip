@@ -150,8 +150,12 @@ struct LaneStack {
     }
 };
 
-template <int W, bool CLOSEST, bool COUNT>
+// STAGE names the launch (0 primary, 2 bounce, 4 API: closest hit; 1 shadow,
+// 3 bounce shadow, 5 API: any hit) so each has its own kernel symbol in
+// profiles; even stages are closest-hit queries.
+template <int W, int STAGE, bool COUNT>
 __global__ void __launch_bounds__(64) k_trace(TraceArgs a) {
+    constexpr bool CLOSEST = (STAGE % 2) == 0;
     __shared__ int2 lds_stack[kLdsDepth][64];
     const int lane = threadIdx.x;
     const int64_t i = static_cast<int64_t>(blockIdx.x) * 64 + lane;
@@ -262,8 +266,23 @@ __global__ void __launch_bounds__(64) k_trace(TraceArgs a) {
     if (st.overflow && a.overflow) atomicOr(a.overflow, 1u);
 }
 
-template <bool CLOSEST>
-int32_t launch(rt_accel *acc, TraceArgs &ta, int64_t max_n, hipStream_t s) {
+template <int W, int STAGE>
+void launch_w(const TraceArgs &ta, int64_t blocks, hipStream_t s) {
+    if (ta.counters)
+        hipLaunchKernelGGL((k_trace<W, STAGE, true>), dim3(blocks), dim3(64), 0, s, ta);
+    else
+        hipLaunchKernelGGL((k_trace<W, STAGE, false>), dim3(blocks), dim3(64), 0, s, ta);
+}
+
+template <int STAGE>
+void launch_stage(int width, const TraceArgs &ta, int64_t blocks, hipStream_t s) {
+    if (width == 4)
+        launch_w<4, STAGE>(ta, blocks, s);
+    else
+        launch_w<8, STAGE>(ta, blocks, s);
+}
+
+int32_t launch(rt_accel *acc, TraceArgs &ta, int64_t max_n, int stage, hipStream_t s) {
     rt_ctx *ctx = acc->scene->ctx;
     if (max_n <= 0) return RT_OK;
     ta.nodes = acc->d_nodes;
@@ -272,21 +291,13 @@ int32_t launch(rt_accel *acc, TraceArgs &ta, int64_t max_n, hipStream_t s) {
     ta.counters = ctx->counting ? ctx->d_counters + 2 * ctx->count_slot : nullptr;
     ta.overflow = ctx->counting ? reinterpret_cast<uint32_t *>(ctx->d_counters + 15) : nullptr;
     const int64_t blocks = (max_n + 63) / 64;
-    if (acc->root == kEmpty) {
-        // Empty scene: every ray misses.
-        return RT_OK;
-    }
-    const bool count = ta.counters != nullptr;
-    if (acc->width == 4) {
-        if (count)
-            hipLaunchKernelGGL((k_trace<4, CLOSEST, true>), dim3(blocks), dim3(64), 0, s, ta);
-        else
-            hipLaunchKernelGGL((k_trace<4, CLOSEST, false>), dim3(blocks), dim3(64), 0, s, ta);
-    } else {
-        if (count)
-            hipLaunchKernelGGL((k_trace<8, CLOSEST, true>), dim3(blocks), dim3(64), 0, s, ta);
-        else
-            hipLaunchKernelGGL((k_trace<8, CLOSEST, false>), dim3(blocks), dim3(64), 0, s, ta);
+    switch (stage) {
+        case 0: launch_stage<0>(acc->width, ta, blocks, s); break;
+        case 1: launch_stage<1>(acc->width, ta, blocks, s); break;
+        case 2: launch_stage<2>(acc->width, ta, blocks, s); break;
+        case 3: launch_stage<3>(acc->width, ta, blocks, s); break;
+        case 4: launch_stage<4>(acc->width, ta, blocks, s); break;
+        default: launch_stage<5>(acc->width, ta, blocks, s); break;
     }
     RT_HIP_CHECK(ctx, hipGetLastError());
     return RT_OK;
@@ -310,26 +321,26 @@ int32_t empty_scene(rt_accel *acc, rt_hit *hits, uint8_t *occ, const uint32_t *c
 
 }  // namespace
 
-int32_t launch_closest(rt_accel *acc, const rt_ray *d_rays, int64_t n, rt_hit *d_hits, hipStream_t s) {
+int32_t launch_closest(rt_accel *acc, const rt_ray *d_rays, int64_t n, rt_hit *d_hits, hipStream_t s, int stage) {
     if (acc->root == kEmpty) return empty_scene(acc, d_hits, nullptr, nullptr, 1, n, s);
     TraceArgs ta{};
     ta.rays = d_rays;
     ta.n = n;
     ta.hits = d_hits;
-    return launch<true>(acc, ta, n, s);
+    return launch(acc, ta, n, stage, s);
 }
 
-int32_t launch_occluded(rt_accel *acc, const rt_ray *d_rays, int64_t n, uint8_t *d_occ, hipStream_t s) {
+int32_t launch_occluded(rt_accel *acc, const rt_ray *d_rays, int64_t n, uint8_t *d_occ, hipStream_t s, int stage) {
     if (acc->root == kEmpty) return empty_scene(acc, nullptr, d_occ, nullptr, 1, n, s);
     TraceArgs ta{};
     ta.rays = d_rays;
     ta.n = n;
     ta.occ = d_occ;
-    return launch<false>(acc, ta, n, s);
+    return launch(acc, ta, n, stage, s);
 }
 
 int32_t launch_closest_q(rt_accel *acc, const rt_ray *d_rays, const uint32_t *d_count, uint32_t mul, int64_t max_n,
-                         rt_hit *d_hits, hipStream_t s) {
+                         rt_hit *d_hits, hipStream_t s, int stage) {
     if (acc->root == kEmpty) return empty_scene(acc, d_hits, nullptr, d_count, mul, max_n, s);
     TraceArgs ta{};
     ta.rays = d_rays;
@@ -337,11 +348,11 @@ int32_t launch_closest_q(rt_accel *acc, const rt_ray *d_rays, const uint32_t *d_
     ta.count_mul = mul;
     ta.n = max_n;
     ta.hits = d_hits;
-    return launch<true>(acc, ta, max_n, s);
+    return launch(acc, ta, max_n, stage, s);
 }
 
 int32_t launch_occluded_q(rt_accel *acc, const rt_ray *d_rays, const uint32_t *d_count, uint32_t mul, int64_t max_n,
-                          uint8_t *d_occ, hipStream_t s) {
+                          uint8_t *d_occ, hipStream_t s, int stage) {
     if (acc->root == kEmpty) return empty_scene(acc, nullptr, d_occ, d_count, mul, max_n, s);
     TraceArgs ta{};
     ta.rays = d_rays;
@@ -349,7 +360,7 @@ int32_t launch_occluded_q(rt_accel *acc, const rt_ray *d_rays, const uint32_t *d
     ta.count_mul = mul;
     ta.n = max_n;
     ta.occ = d_occ;
-    return launch<false>(acc, ta, max_n, s);
+    return launch(acc, ta, max_n, stage, s);
 }
 
 }  // namespace rt
