@@ -20,6 +20,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <vector>
 
 #include "radix_sort.hpp"
 #include "rt_device.hpp"
@@ -198,29 +199,26 @@ __device__ __forceinline__ void child_box(int32_t c, const float4 *node_lo, cons
 
 // Refit without cross-workgroup fences: MI355X's L2 is per XCD, so a device-
 // scope fence per tree level (the usual atomic bottom-up refit) writes back L2
-// every step. Instead each internal node finds its depth by walking parent
-// links, nodes are bucketed by depth, and one launch per level (deepest first)
-// unions the two child boxes; kernel boundaries order the levels.
-__global__ void __launch_bounds__(256) k_node_depth(int64_t n_internal, const int32_t *__restrict__ parent,
-                                                    int32_t *__restrict__ depth, uint32_t *__restrict__ level_count) {
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n_internal) return;
-    int32_t d = 0;
-    int32_t p = static_cast<int32_t>(i);
-    while (p != 0) {
-        p = parent[p];
-        ++d;
-    }
-    depth[i] = d;
-    atomicAdd(&level_count[min(d, kMaxLevels - 1)], 1u);
-}
-
-__global__ void __launch_bounds__(256) k_bucket_by_depth(int64_t n_internal, const int32_t *__restrict__ depth,
-                                                         uint32_t *__restrict__ cursor, int32_t *__restrict__ order) {
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n_internal) return;
-    const uint32_t pos = atomicAdd(&cursor[depth[i]], 1u);
-    order[pos] = static_cast<int32_t>(i);
+// every step. Instead a breadth-first sweep from the root lists the internal
+// nodes level by level (one launch per level, wave-aggregated appends), and
+// one launch per level, deepest first, unions the two child boxes; kernel
+// boundaries order the levels.
+__global__ void __launch_bounds__(256) k_bfs_level(const int32_t *__restrict__ level, int count,
+                                                   const int2 *__restrict__ child, int32_t *__restrict__ next,
+                                                   uint32_t *__restrict__ next_count) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    int2 ch = make_int2(-1, -1);
+    if (k < count) ch = child[level[k]];
+    const bool a = ch.x >= 0, b = ch.y >= 0;
+    const uint64_t ma = __ballot(a), mb = __ballot(b);
+    const int lane = threadIdx.x & 63;
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const uint32_t total = static_cast<uint32_t>(__popcll(ma) + __popcll(mb));
+    uint32_t base = 0;
+    if (lane == 0 && total) base = atomicAdd(next_count, total);
+    base = __shfl(base, 0, 64);
+    if (a) next[base + __popcll(ma & lt)] = ch.x;
+    if (b) next[base + __popcll(ma) + __popcll(mb & lt)] = ch.y;
 }
 
 __global__ void __launch_bounds__(256) k_refit_level(const int32_t *__restrict__ nodes_at_level, int count,
@@ -470,44 +468,40 @@ int32_t build_accel(rt_scene *scene, const rt_build_params &p, rt_accel **out) {
     hipLaunchKernelGGL(k_gather_leaf_boxes, dim3(blocks), dim3(256), 0, s, (const uint32_t *)b_idx.p, n,
                        (const float4 *)b_aabb_lo.p, (const float4 *)b_aabb_hi.p, (float4 *)b_leaf_lo.p,
                        (float4 *)b_leaf_hi.p);
-    // Level-synchronous refit (see k_node_depth).
+    // Level-synchronous refit (see k_bfs_level).
     {
-        DevBuf b_depth, b_levels, b_order;
-        BUILD_CHECK(dalloc<int32_t>(b_depth, n - 1));
-        BUILD_CHECK(dalloc<uint32_t>(b_levels, 2 * kMaxLevels));
+        DevBuf b_order, b_cnt;
         BUILD_CHECK(dalloc<int32_t>(b_order, n - 1));
-        uint32_t *level_count = (uint32_t *)b_levels.p, *cursor = level_count + kMaxLevels;
-        BUILD_CHECK(hipMemsetAsync(level_count, 0, 2 * kMaxLevels * sizeof(uint32_t), s));
-        hipLaunchKernelGGL(k_node_depth, dim3(iblocks), dim3(256), 0, s, n - 1, (const int32_t *)b_parent.p,
-                           (int32_t *)b_depth.p, level_count);
-        uint32_t counts[kMaxLevels];
-        BUILD_CHECK(hipMemcpyAsync(counts, level_count, sizeof(counts), hipMemcpyDeviceToHost, s));
-        BUILD_CHECK(hipStreamSynchronize(s));
-        if (counts[kMaxLevels - 1]) {
-            cleanup();
-            return fail(ctx, RT_ERR_INVALID, "binary tree deeper than 255 levels");
+        BUILD_CHECK(dalloc<uint32_t>(b_cnt, 1));
+        int32_t *order = (int32_t *)b_order.p;
+        uint32_t *cnt = (uint32_t *)b_cnt.p;
+        const int32_t zero = 0;
+        BUILD_CHECK(hipMemcpyAsync(order, &zero, sizeof(int32_t), hipMemcpyHostToDevice, s));
+        std::vector<std::pair<uint32_t, uint32_t>> levels;  // (offset, count)
+        uint32_t off = 0, count = 1;
+        while (count > 0) {
+            levels.push_back({off, count});
+            if (static_cast<int>(levels.size()) > kMaxLevels) {
+                cleanup();
+                return fail(ctx, RT_ERR_INVALID, "binary tree deeper than 256 levels");
+            }
+            BUILD_CHECK(hipMemsetAsync(cnt, 0, sizeof(uint32_t), s));
+            hipLaunchKernelGGL(k_bfs_level, dim3((count + 255) / 256), dim3(256), 0, s, order + off,
+                               static_cast<int>(count), (const int2 *)b_child.p, order + off + count, cnt);
+            uint32_t next = 0;
+            BUILD_CHECK(hipMemcpyAsync(&next, cnt, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+            BUILD_CHECK(hipStreamSynchronize(s));
+            off += count;
+            count = next;
         }
-        uint32_t offs[kMaxLevels];
-        uint32_t run = 0;
-        int max_level = 0;
-        for (int l = 0; l < kMaxLevels; ++l) {
-            offs[l] = run;
-            run += counts[l];
-            if (counts[l]) max_level = l;
-        }
-        BUILD_CHECK(hipMemcpyAsync(cursor, offs, sizeof(offs), hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL(k_bucket_by_depth, dim3(iblocks), dim3(256), 0, s, n - 1, (const int32_t *)b_depth.p,
-                           cursor, (int32_t *)b_order.p);
-        for (int l = max_level; l >= 0; --l) {
-            if (!counts[l]) continue;
-            hipLaunchKernelGGL(k_refit_level, dim3((counts[l] + 255) / 256), dim3(256), 0, s,
-                               (const int32_t *)b_order.p + offs[l], static_cast<int>(counts[l]),
-                               (const int2 *)b_child.p, (const float4 *)b_leaf_lo.p, (const float4 *)b_leaf_hi.p,
-                               (float4 *)b_node_lo.p, (float4 *)b_node_hi.p);
+        for (int l = static_cast<int>(levels.size()) - 1; l >= 0; --l) {
+            hipLaunchKernelGGL(k_refit_level, dim3((levels[l].second + 255) / 256), dim3(256), 0, s,
+                               order + levels[l].first, static_cast<int>(levels[l].second), (const int2 *)b_child.p,
+                               (const float4 *)b_leaf_lo.p, (const float4 *)b_leaf_hi.p, (float4 *)b_node_lo.p,
+                               (float4 *)b_node_hi.p);
         }
         BUILD_CHECK(hipGetLastError());
-        BUILD_CHECK(hipStreamSynchronize(s));
-        a->binary_depth = max_level + 1;
+        a->binary_depth = static_cast<int>(levels.size());
     }
 
     // Wide nodes: at most n - 1 (each wide node consumes >= 1 binary node).

@@ -34,14 +34,39 @@ struct CamArgs {
     int32_t x_res, y_res;
 };
 
-__global__ void __launch_bounds__(256) k_raygen(CamArgs c, int spp, uint64_t seed, int64_t nsamples,
+// Samples are laid out in 8x8-pixel tiles (the reference's packet shape,
+// rt/raytracer.cc:464-483), so the 64 lanes of a wavefront trace one tile:
+// sample i -> slot i / spp -> tile slot / 64, pixel (slot % 64) within it.
+struct TileMap {
+    int32_t x_res, y_res, tiles_x;
+    __device__ __forceinline__ bool pixel_of_slot(int64_t slot, int *x, int *y) const {
+        const int64_t tile = slot >> 6;
+        const int local = static_cast<int>(slot & 63);
+        *x = static_cast<int>(tile % tiles_x) * 8 + (local & 7);
+        *y = static_cast<int>(tile / tiles_x) * 8 + (local >> 3);
+        return *x < x_res && *y < y_res;
+    }
+    __device__ __forceinline__ int64_t slot_of_pixel(int x, int y) const {
+        const int64_t tile = static_cast<int64_t>(y >> 3) * tiles_x + (x >> 3);
+        return (tile << 6) | ((y & 7) << 3) | (x & 7);
+    }
+};
+
+__global__ void __launch_bounds__(256) k_raygen(CamArgs c, TileMap tm, int spp, uint64_t seed, int64_t nsamples,
                                                 rt_ray *__restrict__ rays) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= nsamples) return;
-    const int64_t p = i / spp;
-    const int s = static_cast<int>(i - p * spp);
-    const int x = static_cast<int>(p % c.x_res);
-    const int y = static_cast<int>(p / c.x_res);
+    const int64_t slot = i / spp;
+    const int s = static_cast<int>(i - slot * spp);
+    int x, y;
+    float4 *r = reinterpret_cast<float4 *>(rays) + 2 * i;
+    if (!tm.pixel_of_slot(slot, &x, &y)) {
+        // Padding slot outside the image: tmax < tmin, so nothing is accepted.
+        r[0] = make_float4(c.px, c.py, c.pz, rtd::kEpsilon);
+        r[1] = make_float4(0.0f, 0.0f, 1.0f, -1.0f);
+        return;
+    }
+    const int64_t p = static_cast<int64_t>(y) * c.x_res + x;
     float xf = static_cast<float>(x), yf = static_cast<float>(y);
     if (spp > 1) {
         const uint64_t k = rtd::sample_key(seed, static_cast<uint32_t>(p), static_cast<uint32_t>(s), 255u);
@@ -54,7 +79,6 @@ __global__ void __launch_bounds__(256) k_raygen(CamArgs c, int spp, uint64_t see
     const rtd::V3 scr = rtd::mk(((c.xx * x_t) + (c.yx * y_t)) + (c.zx * c.t), ((c.xy * x_t) + (c.yy * y_t)) + (c.zy * c.t),
                                 ((c.xz * x_t) + (c.yz * y_t)) + (c.zz * c.t));
     const rtd::V3 d = rtd::normalise_exact(scr);
-    float4 *r = reinterpret_cast<float4 *>(rays) + 2 * i;
     r[0] = make_float4(c.px, c.py, c.pz, rtd::kEpsilon);
     r[1] = make_float4(d.x, d.y, d.z, rtd::kMaxDist);
 }
@@ -66,6 +90,7 @@ struct LightArgs {
 };
 
 struct ShadeArgs {
+    TileMap tm;
     const rt_ray *rays;      // rays that were traced
     const rt_hit *hits;      // their hits
     const uint32_t *count;   // number of rays (nullptr: n_static)
@@ -147,8 +172,11 @@ __global__ void __launch_bounds__(256) k_shade_hit(ShadeArgs a) {
         sr[1] = make_float4(dn.x, dn.y, dn.z, len);
     }
     if (STAGE == 0 && a.bounce) {
-        const int64_t p = i / a.spp;
-        const int s = static_cast<int>(i - p * a.spp);
+        const int64_t slot = i / a.spp;
+        const int s = static_cast<int>(i - slot * a.spp);
+        int x, y;
+        a.tm.pixel_of_slot(slot, &x, &y);
+        const int64_t p = static_cast<int64_t>(y) * a.tm.x_res + x;
         const rtd::V3 bd = rtd::cosine_direction(
             rtd::sample_key(a.seed, static_cast<uint32_t>(p), static_cast<uint32_t>(s), 1u), shn);
         float4 *br = reinterpret_cast<float4 *>(a.bounce_q) + 2 * static_cast<int64_t>(q);
@@ -158,6 +186,7 @@ __global__ void __launch_bounds__(256) k_shade_hit(ShadeArgs a) {
 }
 
 struct ComposeArgs {
+    TileMap tm;
     int64_t npix;
     int spp;
     int bounce;
@@ -207,9 +236,10 @@ __device__ __forceinline__ Col direct(const rt_ray *shadow, const uint8_t *occ, 
 __global__ void __launch_bounds__(256) k_compose(ComposeArgs a) {
     const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (p >= a.npix) return;
+    const int64_t slot = a.tm.slot_of_pixel(static_cast<int>(p % a.tm.x_res), static_cast<int>(p / a.tm.x_res));
     Col acc{0.0f, 0.0f, 0.0f};
     for (int s = 0; s < a.spp; ++s) {
-        const int64_t i = p * a.spp + s;
+        const int64_t i = slot * a.spp + s;
         Col c{0.0f, 0.0f, 0.0f};
         const int32_t q = a.qidx1[i];
         if (q >= 0) {
@@ -237,16 +267,20 @@ __global__ void __launch_bounds__(256) k_compose(ComposeArgs a) {
     a.rgb[3 * p + 2] = acc.b / fs;
 }
 
-// Per-sample bounce hits for the caller: the bounce queue is in compaction
-// order, so scatter back through the per-sample queue index.
-__global__ void __launch_bounds__(256) k_bounce_out(int64_t nsamples, const int32_t *qidx1, const rt_hit *hits2,
-                                                    rt_hit *out) {
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= nsamples) return;
-    const int32_t q = qidx1[i];
+// Hits back in pixel order (p * spp + s) for the caller. For bounce hits the
+// queue is in compaction order, so go through the per-sample queue index.
+__global__ void __launch_bounds__(256) k_hits_out(TileMap tm, int spp, int64_t npix, const int32_t *qidx1,
+                                                  const rt_hit *hits, rt_hit *out) {
+    const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= npix * spp) return;
+    const int64_t p = j / spp;
+    const int s = static_cast<int>(j - p * spp);
+    const int64_t i = tm.slot_of_pixel(static_cast<int>(p % tm.x_res), static_cast<int>(p / tm.x_res)) * spp + s;
+    int64_t k = i;
+    if (qidx1) k = qidx1[i];
     float4 v = make_float4(rtd::kMaxDist, 0.0f, 0.0f, __int_as_float(-1));
-    if (q >= 0) v = reinterpret_cast<const float4 *>(hits2)[q];
-    reinterpret_cast<float4 *>(out)[i] = v;
+    if (k >= 0) v = reinterpret_cast<const float4 *>(hits)[k];
+    reinterpret_cast<float4 *>(out)[j] = v;
 }
 
 __global__ void k_zero_u32(uint32_t *p, int n) {
@@ -338,7 +372,12 @@ int32_t render_frame(rt_accel *a, const rt_camera &cam, const rt_light *lights, 
     if (cam.x_res <= 0 || cam.y_res <= 0) return fail(ctx, RT_ERR_INVALID, "camera resolution must be positive");
     const int spp = std::max(1, p.spp);
     const int64_t npix = static_cast<int64_t>(cam.x_res) * cam.y_res;
-    const int64_t ns = npix * spp;
+    TileMap tm;
+    tm.x_res = cam.x_res;
+    tm.y_res = cam.y_res;
+    tm.tiles_x = (cam.x_res + 7) / 8;
+    const int64_t tiles_y = (cam.y_res + 7) / 8;
+    const int64_t ns = static_cast<int64_t>(tm.tiles_x) * tiles_y * 64 * spp;  // samples incl. tile padding
     if (ns >= (int64_t(1) << 31)) return fail(ctx, RT_ERR_INVALID, "too many samples per frame");
     int32_t err = RT_OK;
     Workspace *w = get_workspace(a, ns, std::max(1, nlights), &err);
@@ -375,12 +414,13 @@ int32_t render_frame(rt_accel *a, const rt_camera &cam, const rt_light *lights, 
 
     hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, w->counts, 4);
     if ((rc = rec(0))) return rc;
-    hipLaunchKernelGGL(k_raygen, grid(ns), dim3(256), 0, s, c, spp, p.seed, ns, w->rays1);
+    hipLaunchKernelGGL(k_raygen, grid(ns), dim3(256), 0, s, c, tm, spp, p.seed, ns, w->rays1);
     if ((rc = rec(1))) return rc;
     ctx->count_slot = 0;
     if ((rc = launch_closest(a, w->rays1, ns, w->hits1, s, 0))) return rc;
     if ((rc = rec(2))) return rc;
     ShadeArgs sa{};
+    sa.tm = tm;
     sa.rays = w->rays1;
     sa.hits = w->hits1;
     sa.count = nullptr;
@@ -410,6 +450,7 @@ int32_t render_frame(rt_accel *a, const rt_camera &cam, const rt_light *lights, 
         if ((rc = launch_closest_q(a, w->bounce, w->counts + 0, 1, ns, w->hits2, s, 2))) return rc;
         if ((rc = rec(5))) return rc;
         ShadeArgs sb{};
+        sb.tm = tm;
         sb.rays = w->bounce;
         sb.hits = w->hits2;
         sb.count = w->counts + 0;
@@ -440,6 +481,7 @@ int32_t render_frame(rt_accel *a, const rt_camera &cam, const rt_light *lights, 
         if ((rc = rec(7))) return rc;
     }
     ComposeArgs ca{};
+    ca.tm = tm;
     ca.npix = npix;
     ca.spp = spp;
     ca.bounce = p.bounce;
@@ -465,20 +507,25 @@ int32_t render_frame(rt_accel *a, const rt_camera &cam, const rt_light *lights, 
     // Outputs.
     if (rgb && !dev_out)
         RT_HIP_CHECK(ctx, hipMemcpyAsync(rgb, w->rgb, 3 * npix * sizeof(float), hipMemcpyDeviceToHost, s));
-    if (primary)
-        RT_HIP_CHECK(ctx, hipMemcpyAsync(primary, w->hits1, ns * sizeof(rt_hit),
+    if (primary) {
+        hipLaunchKernelGGL(k_hits_out, grid(npix * spp), dim3(256), 0, s, tm, spp, npix, nullptr, w->hits1,
+                           w->bounce_out);
+        RT_HIP_CHECK(ctx, hipMemcpyAsync(primary, w->bounce_out, npix * spp * sizeof(rt_hit),
                                          dev_out ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+    }
     if (bounce_hits) {
         if (!p.bounce) return fail(ctx, RT_ERR_INVALID, "bounce hits requested without bounce");
-        hipLaunchKernelGGL(k_bounce_out, grid(ns), dim3(256), 0, s, ns, w->qidx1, w->hits2, w->bounce_out);
-        RT_HIP_CHECK(ctx, hipMemcpyAsync(bounce_hits, w->bounce_out, ns * sizeof(rt_hit),
+        if (primary) RT_HIP_CHECK(ctx, hipStreamSynchronize(s));  // bounce_out is reused
+        hipLaunchKernelGGL(k_hits_out, grid(npix * spp), dim3(256), 0, s, tm, spp, npix, w->qidx1, w->hits2,
+                           w->bounce_out);
+        RT_HIP_CHECK(ctx, hipMemcpyAsync(bounce_hits, w->bounce_out, npix * spp * sizeof(rt_hit),
                                          dev_out ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
     }
     uint32_t counts[2] = {0, 0};
     if (stats) RT_HIP_CHECK(ctx, hipMemcpyAsync(counts, w->counts, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     RT_HIP_CHECK(ctx, hipStreamSynchronize(s));
     if (stats) {
-        stats->rays[0] = static_cast<uint64_t>(ns);
+        stats->rays[0] = static_cast<uint64_t>(npix * spp);
         stats->rays[1] = static_cast<uint64_t>(counts[0]) * nlights;
         stats->rays[2] = p.bounce ? counts[0] : 0;
         stats->rays[3] = p.bounce ? static_cast<uint64_t>(counts[1]) * nlights : 0;
