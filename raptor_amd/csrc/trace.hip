@@ -124,35 +124,13 @@ __device__ __forceinline__ void sort_children<8>(float *t, int32_t *c) {
     cswap(t[5], c[5], t[6], c[6]);
 }
 
-// Per-lane stack: LDS column for the first kLdsDepth entries, private memory
-// beyond.
-struct LaneStack {
-    int2 (*lds)[64];
-    int lane;
-    int sp;
-    int2 spill[kSpillDepth];
-    bool overflow;
-    __device__ __forceinline__ void push(int32_t code, float t) {
-        const int2 e = make_int2(code, __float_as_int(t));
-        if (sp < kLdsDepth) {
-            lds[sp][lane] = e;
-        } else if (sp < kLdsDepth + kSpillDepth) {
-            spill[sp - kLdsDepth] = e;
-        } else {
-            overflow = true;
-            return;
-        }
-        ++sp;
-    }
-    __device__ __forceinline__ int2 pop() {
-        --sp;
-        return (sp < kLdsDepth) ? lds[sp][lane] : spill[sp - kLdsDepth];
-    }
-};
-
 // STAGE names the launch (0 primary, 2 bounce, 4 API: closest hit; 1 shadow,
 // 3 bounce shadow, 5 API: any hit) so each has its own kernel symbol in
 // profiles; even stages are closest-hit queries.
+//
+// The per-lane stack is plain locals: the stack pointer in a VGPR, the first
+// kLdsDepth entries in an LDS column owned by the lane, deeper entries in a
+// private array (scratch) that only deep traversals touch.
 template <int W, int STAGE, bool COUNT>
 __global__ void __launch_bounds__(64) k_trace(TraceArgs a) {
     constexpr bool CLOSEST = (STAGE % 2) == 0;
@@ -170,11 +148,12 @@ __global__ void __launch_bounds__(64) k_trace(TraceArgs a) {
     const float tmax = r1.w;
     const rtd::V3 id = rtd::mk(safe_inv(d.x), safe_inv(d.y), safe_inv(d.z));
 
-    LaneStack st;
-    st.lds = lds_stack;
-    st.lane = lane;
-    st.sp = 0;
-    st.overflow = false;
+    // Two separate arrays so that a spill read never has the type of an LDS
+    // entry read (the compiler would otherwise merge both into a flat load).
+    int32_t spill_code[kSpillDepth];
+    float spill_t[kSpillDepth];
+    int sp = 0;
+    bool overflow = false;
 
     float best_t = tmax, best_u = 0.0f, best_v = 0.0f;
     int32_t best_id = -1;
@@ -197,7 +176,19 @@ __global__ void __launch_bounds__(64) k_trace(TraceArgs a) {
 #pragma unroll
             for (int k = W - 1; k >= 0; --k) {
                 if (tn[k] < INFINITY) {
-                    if (cur != kEmpty) st.push(cur, cur_t);
+                    if (cur != kEmpty) {
+                        const int2 e = make_int2(cur, __float_as_int(cur_t));
+                        if (sp < kLdsDepth) {
+                            lds_stack[sp][lane] = e;
+                            ++sp;
+                        } else if (sp < kLdsDepth + kSpillDepth) {
+                            spill_code[sp - kLdsDepth] = cur;
+                            spill_t[sp - kLdsDepth] = cur_t;
+                            ++sp;
+                        } else {
+                            overflow = true;
+                        }
+                    }
                     cur = cc[k];
                     cur_t = tn[k];
                 }
@@ -230,8 +221,17 @@ __global__ void __launch_bounds__(64) k_trace(TraceArgs a) {
         }
         if (cur == kEmpty) {
             bool found = false;
-            while (st.sp > 0) {
-                const int2 e = st.pop();
+            while (sp > 0) {
+                --sp;
+                // An unconditional LDS read (clamped index) plus a rare scratch
+                // read keeps the compiler from merging both into a flat load.
+                int2 e;
+                if (sp < kLdsDepth) {
+                    e = lds_stack[sp][lane];
+                } else {
+                    e.x = spill_code[sp - kLdsDepth];
+                    e.y = __float_as_int(spill_t[sp - kLdsDepth]);
+                }
                 if (!CLOSEST || __int_as_float(e.y) <= best_t * kHitStretch) {
                     cur = e.x;
                     found = true;
@@ -242,12 +242,10 @@ __global__ void __launch_bounds__(64) k_trace(TraceArgs a) {
         }
     }
     if (CLOSEST) {
-        rt_hit h;
-        h.t = (best_id >= 0) ? best_t : rtd::kMaxDist;
-        h.u = (best_id >= 0) ? best_u : 0.0f;
-        h.v = (best_id >= 0) ? best_v : 0.0f;
-        h.prim = best_id;
-        reinterpret_cast<float4 *>(a.hits)[i] = make_float4(h.t, h.u, h.v, __int_as_float(h.prim));
+        const float ht = (best_id >= 0) ? best_t : rtd::kMaxDist;
+        const float hu = (best_id >= 0) ? best_u : 0.0f;
+        const float hv = (best_id >= 0) ? best_v : 0.0f;
+        reinterpret_cast<float4 *>(a.hits)[i] = make_float4(ht, hu, hv, __int_as_float(best_id));
     } else {
         a.occ[i] = occluded ? 1 : 0;
     }
@@ -263,7 +261,7 @@ __global__ void __launch_bounds__(64) k_trace(TraceArgs a) {
             atomicAdd(&a.counters[1], ts);
         }
     }
-    if (st.overflow && a.overflow) atomicOr(a.overflow, 1u);
+    if (overflow && a.overflow) atomicOr(a.overflow, 1u);
 }
 
 template <int W, int STAGE>
