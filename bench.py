@@ -81,7 +81,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--width", type=int, default=4)
-    ap.add_argument("--leaf-size", type=int, default=4)
+    ap.add_argument("--leaf-size", type=int, default=2)
+    ap.add_argument("--lds-stack", type=int, default=16)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -98,7 +99,7 @@ def main():
     w = workloads.terrain1m()
     ctx = rt.Context(0)
     scene = rt.Scene(ctx, w.abc, w.albedo)
-    accel = rt.Accel(scene, width=args.width, leaf_size=args.leaf_size)
+    accel = rt.Accel(scene, width=args.width, leaf_size=args.leaf_size, lds_stack=args.lds_stack)
     info = accel.info()
     rgb = torch.empty((w.camera.y_res, w.camera.x_res, 3), dtype=torch.float32, device="cuda:0")
 
@@ -155,6 +156,7 @@ def main():
             "resolution": [w.camera.x_res, w.camera.y_res],
             "bvh_width": args.width,
             "leaf_size": args.leaf_size,
+            "lds_stack": args.lds_stack,
         },
         "rays_per_step": {"primary": rays[0], "shadow": rays[1], "bounce": rays[2], "bounce_shadow": rays[3]},
         "stage_ms": {STAGES[k]: round(stage_ms[k], 4) for k in range(4)},
@@ -162,7 +164,7 @@ def main():
                 "bytes": info["bytes"], "build_ms": round(info["build_ms"], 2)},
         "roofline": {
             "bound": "hbm",
-            "kernel": f"k_trace<{args.width}, {dom}, false> ({STAGES[dom]})",
+            "kernel": f"k_trace<{args.width}, {dom}, false, {args.lds_stack}> ({STAGES[dom]})",
             "achieved": round(achieved, 2),
             "peak": PEAK_HBM_GBS,
             "unit": "GB/s",

@@ -66,14 +66,15 @@ typedef struct {
     float d;
 } rt_light;
 
-/* BVH build options. width: 4 or 8 children per node; leaf_size: most
- * triangles per leaf (1..8); sah_bins: bins of the SAH refinement of the top
- * levels (0 = off). */
+/* BVH build and traversal options. width: 4 or 8 children per node;
+ * leaf_size: most triangles per leaf (1..16); sah_bins: bins of the SAH
+ * refinement (0 = off); lds_stack: traversal-stack entries per lane kept in
+ * LDS before spilling to private memory (4, 8 or 16; 0 = 16). */
 typedef struct {
     int32_t width;
     int32_t leaf_size;
     int32_t sah_bins;
-    int32_t reserved;
+    int32_t lds_stack;
 } rt_build_params;
 
 /* Frame options: bounce = 1 adds one cosine-weighted diffuse bounce with

@@ -381,6 +381,7 @@ int32_t build_accel(rt_scene *scene, const rt_build_params &p, rt_accel **out) {
     a->scene = scene;
     a->width = W;
     a->leaf_size = p.leaf_size;
+    a->lds_stack = (p.lds_stack <= 0) ? 16 : (p.lds_stack <= 4 ? 4 : (p.lds_stack <= 8 ? 8 : 16));
     auto cleanup = [&]() {
         if (a->d_nodes) (void)hipFree(a->d_nodes);
         if (a->d_tris) (void)hipFree(a->d_tris);

@@ -23,8 +23,7 @@ namespace rt {
 
 namespace {
 
-constexpr int kLdsDepth = 16;
-constexpr int kSpillDepth = 112;
+constexpr int kStackDepth = 128;  // LDS part + private spill part
 constexpr float kFarStretch = 1.00000095f;   // 1 + 2^-20
 constexpr float kHitStretch = 1.0000038f;    // 1 + 2^-18
 
@@ -128,12 +127,20 @@ __device__ __forceinline__ void sort_children<8>(float *t, int32_t *c) {
 // 3 bounce shadow, 5 API: any hit) so each has its own kernel symbol in
 // profiles; even stages are closest-hit queries.
 //
+// Loop structure (Aila & Laine 2009, "while-while" with speculative
+// traversal): lanes first walk interior nodes until every lane of the
+// wavefront holds a leaf or has nothing left; a lane that finds a leaf early
+// parks it and keeps traversing while others still search. Then all lanes
+// test their leaf triangles together. This keeps the box-test and triangle-test
+// code from being issued, predicated off, on every iteration.
+//
 // The per-lane stack is plain locals: the stack pointer in a VGPR, the first
-// kLdsDepth entries in an LDS column owned by the lane, deeper entries in a
-// private array (scratch) that only deep traversals touch.
-template <int W, int STAGE, bool COUNT>
+// kLdsDepth entries in an LDS column owned by the lane, deeper entries in two
+// private arrays (scratch) that only deep traversals touch.
+template <int W, int STAGE, bool COUNT, int kLdsDepth>
 __global__ void __launch_bounds__(64) k_trace(TraceArgs a) {
     constexpr bool CLOSEST = (STAGE % 2) == 0;
+    constexpr int kSpillDepth = kStackDepth - kLdsDepth;
     __shared__ int2 lds_stack[kLdsDepth][64];
     const int lane = threadIdx.x;
     const int64_t i = static_cast<int64_t>(blockIdx.x) * 64 + lane;
@@ -157,45 +164,78 @@ __global__ void __launch_bounds__(64) k_trace(TraceArgs a) {
 
     float best_t = tmax, best_u = 0.0f, best_v = 0.0f;
     int32_t best_id = -1;
-    bool occluded = false;
     uint32_t visits = 0, tests = 0;
-    int32_t cur = a.root;
-    while (true) {
-        if (cur >= 0) {
-            NodeView<W> v;
-            load_node<W>(nodes, cur, v);
-            if (COUNT) ++visits;
-            float tn[W];
-            box_test<W>(v, o, id, best_t * kHitStretch, tn);
-            int32_t cc[W];
-#pragma unroll
-            for (int k = 0; k < W; ++k) cc[k] = v.c[k];
-            if (CLOSEST) sort_children<W>(tn, cc);
-            cur = kEmpty;
-            float cur_t = INFINITY;
-#pragma unroll
-            for (int k = W - 1; k >= 0; --k) {
-                if (tn[k] < INFINITY) {
-                    if (cur != kEmpty) {
-                        const int2 e = make_int2(cur, __float_as_int(cur_t));
-                        if (sp < kLdsDepth) {
-                            lds_stack[sp][lane] = e;
-                            ++sp;
-                        } else if (sp < kLdsDepth + kSpillDepth) {
-                            spill_code[sp - kLdsDepth] = cur;
-                            spill_t[sp - kLdsDepth] = cur_t;
-                            ++sp;
-                        } else {
-                            overflow = true;
-                        }
-                    }
-                    cur = cc[k];
-                    cur_t = tn[k];
-                }
+
+    // Pop the nearest stacked subtree that can still hold a closer hit.
+    auto pop = [&]() -> int32_t {
+        while (sp > 0) {
+            --sp;
+            int2 e;
+            if (sp < kLdsDepth) {
+                e = lds_stack[sp][lane];
+            } else {
+                e.x = spill_code[sp - kLdsDepth];
+                e.y = __float_as_int(spill_t[sp - kLdsDepth]);
             }
-        } else if (cur != kEmpty) {
-            const int32_t first = leaf_first(cur);
-            const int32_t count = leaf_count(cur);
+            if (!CLOSEST || __int_as_float(e.y) <= best_t * kHitStretch) return e.x;
+        }
+        return kEmpty;
+    };
+    auto push = [&](int32_t code, float t) {
+        if (sp < kLdsDepth) {
+            lds_stack[sp][lane] = make_int2(code, __float_as_int(t));
+            ++sp;
+        } else if (sp < kLdsDepth + kSpillDepth) {
+            spill_code[sp - kLdsDepth] = code;
+            spill_t[sp - kLdsDepth] = t;
+            ++sp;
+        } else {
+            overflow = true;
+        }
+    };
+
+    int32_t cur = a.root;    // interior node, leaf code or kEmpty (done)
+    while (true) {
+        // ---- interior phase
+        int32_t leaf = kEmpty;
+        while (true) {
+            if (cur >= 0) {
+                NodeView<W> v;
+                load_node<W>(nodes, cur, v);
+                if (COUNT) ++visits;
+                float tn[W];
+                box_test<W>(v, o, id, best_t * kHitStretch, tn);
+                int32_t cc[W];
+#pragma unroll
+                for (int k = 0; k < W; ++k) cc[k] = v.c[k];
+                if (CLOSEST) sort_children<W>(tn, cc);
+                int32_t next = kEmpty;
+                float next_t = INFINITY;
+#pragma unroll
+                for (int k = W - 1; k >= 0; --k) {
+                    if (tn[k] < INFINITY) {
+                        if (next != kEmpty) push(next, next_t);
+                        next = cc[k];
+                        next_t = tn[k];
+                    }
+                }
+                cur = (next != kEmpty) ? next : pop();
+            }
+            // Park the first leaf found and keep looking (speculation).
+            if (cur != kEmpty && cur < 0 && leaf == kEmpty) {
+                leaf = cur;
+                cur = pop();
+            }
+            // Leave once no lane is still looking for its first leaf.
+            const bool searching = (cur >= 0) && (leaf == kEmpty);
+            if (!__any(searching)) break;
+            // Lanes holding a leaf whose next node is also a leaf wait here.
+        }
+        // ---- leaf phase
+        bool occluded = false;
+        while (leaf != kEmpty) {
+            const int32_t first = leaf_first(leaf);
+            const int32_t count = leaf_count(leaf);
             for (int k = 0; k < count; ++k) {
                 const LeafTri lt = a.tris[first + k];
                 const int32_t pid = __float_as_int(lt.a_id.w);
@@ -216,30 +256,19 @@ __global__ void __launch_bounds__(64) k_trace(TraceArgs a) {
                     }
                 }
             }
-            cur = kEmpty;
+            leaf = kEmpty;
             if (!CLOSEST && occluded) break;
-        }
-        if (cur == kEmpty) {
-            bool found = false;
-            while (sp > 0) {
-                --sp;
-                // An unconditional LDS read (clamped index) plus a rare scratch
-                // read keeps the compiler from merging both into a flat load.
-                int2 e;
-                if (sp < kLdsDepth) {
-                    e = lds_stack[sp][lane];
-                } else {
-                    e.x = spill_code[sp - kLdsDepth];
-                    e.y = __float_as_int(spill_t[sp - kLdsDepth]);
-                }
-                if (!CLOSEST || __int_as_float(e.y) <= best_t * kHitStretch) {
-                    cur = e.x;
-                    found = true;
-                    break;
-                }
+            // A leaf that was reached after the parked one is tested now.
+            if (cur != kEmpty && cur < 0) {
+                leaf = cur;
+                cur = pop();
             }
-            if (!found) break;
         }
+        if (!CLOSEST && occluded) {
+            best_id = 0;  // marks "occluded" for the store below
+            break;
+        }
+        if (cur == kEmpty) break;
     }
     if (CLOSEST) {
         const float ht = (best_id >= 0) ? best_t : rtd::kMaxDist;
@@ -247,7 +276,7 @@ __global__ void __launch_bounds__(64) k_trace(TraceArgs a) {
         const float hv = (best_id >= 0) ? best_v : 0.0f;
         reinterpret_cast<float4 *>(a.hits)[i] = make_float4(ht, hu, hv, __int_as_float(best_id));
     } else {
-        a.occ[i] = occluded ? 1 : 0;
+        a.occ[i] = (best_id >= 0) ? 1 : 0;
     }
     if (COUNT) {
         unsigned long long vs = visits, ts = tests;
@@ -264,20 +293,30 @@ __global__ void __launch_bounds__(64) k_trace(TraceArgs a) {
     if (overflow && a.overflow) atomicOr(a.overflow, 1u);
 }
 
-template <int W, int STAGE>
-void launch_w(const TraceArgs &ta, int64_t blocks, hipStream_t s) {
+template <int W, int STAGE, int D>
+void launch_wd(const TraceArgs &ta, int64_t blocks, hipStream_t s) {
     if (ta.counters)
-        hipLaunchKernelGGL((k_trace<W, STAGE, true>), dim3(blocks), dim3(64), 0, s, ta);
+        hipLaunchKernelGGL((k_trace<W, STAGE, true, D>), dim3(blocks), dim3(64), 0, s, ta);
     else
-        hipLaunchKernelGGL((k_trace<W, STAGE, false>), dim3(blocks), dim3(64), 0, s, ta);
+        hipLaunchKernelGGL((k_trace<W, STAGE, false, D>), dim3(blocks), dim3(64), 0, s, ta);
+}
+
+template <int W, int STAGE>
+void launch_w(int depth, const TraceArgs &ta, int64_t blocks, hipStream_t s) {
+    if (depth <= 4)
+        launch_wd<W, STAGE, 4>(ta, blocks, s);
+    else if (depth <= 8)
+        launch_wd<W, STAGE, 8>(ta, blocks, s);
+    else
+        launch_wd<W, STAGE, 16>(ta, blocks, s);
 }
 
 template <int STAGE>
-void launch_stage(int width, const TraceArgs &ta, int64_t blocks, hipStream_t s) {
+void launch_stage(int width, int depth, const TraceArgs &ta, int64_t blocks, hipStream_t s) {
     if (width == 4)
-        launch_w<4, STAGE>(ta, blocks, s);
+        launch_w<4, STAGE>(depth, ta, blocks, s);
     else
-        launch_w<8, STAGE>(ta, blocks, s);
+        launch_w<8, STAGE>(depth, ta, blocks, s);
 }
 
 int32_t launch(rt_accel *acc, TraceArgs &ta, int64_t max_n, int stage, hipStream_t s) {
@@ -290,12 +329,12 @@ int32_t launch(rt_accel *acc, TraceArgs &ta, int64_t max_n, int stage, hipStream
     ta.overflow = ctx->counting ? reinterpret_cast<uint32_t *>(ctx->d_counters + 15) : nullptr;
     const int64_t blocks = (max_n + 63) / 64;
     switch (stage) {
-        case 0: launch_stage<0>(acc->width, ta, blocks, s); break;
-        case 1: launch_stage<1>(acc->width, ta, blocks, s); break;
-        case 2: launch_stage<2>(acc->width, ta, blocks, s); break;
-        case 3: launch_stage<3>(acc->width, ta, blocks, s); break;
-        case 4: launch_stage<4>(acc->width, ta, blocks, s); break;
-        default: launch_stage<5>(acc->width, ta, blocks, s); break;
+        case 0: launch_stage<0>(acc->width, acc->lds_stack, ta, blocks, s); break;
+        case 1: launch_stage<1>(acc->width, acc->lds_stack, ta, blocks, s); break;
+        case 2: launch_stage<2>(acc->width, acc->lds_stack, ta, blocks, s); break;
+        case 3: launch_stage<3>(acc->width, acc->lds_stack, ta, blocks, s); break;
+        case 4: launch_stage<4>(acc->width, acc->lds_stack, ta, blocks, s); break;
+        default: launch_stage<5>(acc->width, acc->lds_stack, ta, blocks, s); break;
     }
     RT_HIP_CHECK(ctx, hipGetLastError());
     return RT_OK;

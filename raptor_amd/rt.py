@@ -39,7 +39,7 @@ class rt_light(C.Structure):
 
 
 class rt_build_params(C.Structure):
-    _fields_ = [("width", C.c_int32), ("leaf_size", C.c_int32), ("sah_bins", C.c_int32), ("reserved", C.c_int32)]
+    _fields_ = [("width", C.c_int32), ("leaf_size", C.c_int32), ("sah_bins", C.c_int32), ("lds_stack", C.c_int32)]
 
 
 class rt_render_params(C.Structure):
@@ -190,10 +190,10 @@ class Scene:
 
 
 class Accel:
-    def __init__(self, scene, width=4, leaf_size=4, sah_bins=0):
+    def __init__(self, scene, width=4, leaf_size=2, sah_bins=0, lds_stack=16):
         self.scene = scene
         self.ctx = scene.ctx
-        p = rt_build_params(width, leaf_size, sah_bins, 0)
+        p = rt_build_params(width, leaf_size, sah_bins, lds_stack)
         h = C.c_void_p()
         self.ctx.check(_LIB.rt_accel_build(scene.h, C.byref(p), C.byref(h)), "rt_accel_build")
         self.h = h
