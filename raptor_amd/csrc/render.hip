@@ -106,8 +106,64 @@ struct ShadeArgs {
     int32_t *hit_sample;        // per compacted hit: sample index
     rt_ray *shadow_q;           // per compacted hit x light
     rt_ray *bounce_q;           // per compacted hit (stage 0 with bounce)
-    uint32_t *q_count;          // compacted hit counter
+    const uint32_t *block_base; // exclusive prefix of the per-block hit counts
 };
+
+// Deterministic stream compaction of hits (no contended atomics: a single
+// counter hit once per wavefront serialises across the eight XCDs):
+//   k_count_hits    hits per 256-thread block
+//   k_scan_blocks   exclusive prefix over the block counts, total in [nblocks]
+//   k_shade_hit     each hit's slot = block base + rank inside the block
+// The compacted order is the sample order, so queues are reproducible.
+__global__ void __launch_bounds__(256) k_count_hits(const rt_hit *__restrict__ hits, const uint32_t *count,
+                                                    int64_t n_static, uint32_t *__restrict__ block_counts) {
+    __shared__ uint32_t wsum[4];
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t n = count ? static_cast<int64_t>(count[0]) : n_static;
+    bool hit = false;
+    if (i < n) hit = __float_as_int(reinterpret_cast<const float4 *>(hits)[i].w) >= 0;
+    const uint64_t m = __ballot(hit);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = static_cast<uint32_t>(__popcll(m));
+    __syncthreads();
+    if (threadIdx.x == 0) block_counts[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+}
+
+// One block of 1024 threads scans up to 1024 * kScanPer block counts.
+constexpr int kScanPer = 32;
+__global__ void __launch_bounds__(1024) k_scan_blocks(uint32_t *__restrict__ v, int n) {
+    __shared__ uint32_t wtot[16];
+    const int t = threadIdx.x;
+    uint32_t loc[kScanPer];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k) {
+        const int idx = t * kScanPer + k;
+        loc[k] = (idx < n) ? v[idx] : 0u;
+        sum += loc[k];
+    }
+    const int lane = t & 63, wave = t >> 6;
+    uint32_t x = sum;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wtot[wave] = x;
+    __syncthreads();
+    uint32_t wbase = 0, total = 0;
+    for (int w = 0; w < 16; ++w) {
+        if (w < wave) wbase += wtot[w];
+        total += wtot[w];
+    }
+    uint32_t run = wbase + x - sum;
+#pragma unroll
+    for (int k = 0; k < kScanPer; ++k) {
+        const int idx = t * kScanPer + k;
+        if (idx < n) v[idx] = run;
+        run += loc[k];
+    }
+    if (t == 0) v[n] = total;
+}
 
 // rt/ray.cc:15-20 + rt/triangle.h:464-498.
 __device__ __forceinline__ void surface_at(rtd::V3 o, rtd::V3 d, float t, rtd::V3 n, rtd::V3 c, rtd::V3 *dst,
@@ -136,14 +192,16 @@ __global__ void __launch_bounds__(256) k_shade_hit(ShadeArgs a) {
     if (in) h = reinterpret_cast<const float4 *>(a.hits)[i];
     const int32_t prim = __float_as_int(h.w);
     const bool hit = in && prim >= 0;
-    // Wave-level compaction: one atomic per wavefront.
+    // Rank inside the block: ballot within the wave, wave totals through LDS.
+    __shared__ uint32_t wcount[4];
     const uint64_t mask = __ballot(hit);
     const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    const int leader = mask ? (__ffsll(static_cast<long long>(mask)) - 1) : 0;
-    uint32_t base = 0;
-    if (mask && lane == leader) base = atomicAdd(a.q_count, static_cast<uint32_t>(__popcll(mask)));
-    base = __shfl(base, leader, 64);
+    if (lane == 0) wcount[wave] = static_cast<uint32_t>(__popcll(mask));
+    __syncthreads();
+    uint32_t base = a.block_base[blockIdx.x];
+    for (int w = 0; w < wave; ++w) base += wcount[w];
     if (!in) return;
     if (!hit) {
         a.qidx[i] = -1;
@@ -300,7 +358,8 @@ struct Workspace {
     uint8_t *occ1, *occ2;
     float *rgb;
     rt_hit *bounce_out;
-    uint32_t *counts;  // [0] primary hits, [1] bounce hits
+    uint32_t *counts;  // unused
+    uint32_t *blocks1, *blocks2;  // per-block hit counts, then prefixes; total at [nblocks]
 };
 
 template <class T>
@@ -329,6 +388,9 @@ size_t layout(Workspace &w, char *base, int64_t ns, int nl) {
     w.rgb = carve<float>(p, 3 * ns);
     w.bounce_out = carve<rt_hit>(p, ns);
     w.counts = carve<uint32_t>(p, 4);
+    const int64_t nb = (ns + 255) / 256 + 1;
+    w.blocks1 = carve<uint32_t>(p, nb);
+    w.blocks2 = carve<uint32_t>(p, nb);
     return static_cast<size_t>(p - base);
 }
 
@@ -379,6 +441,8 @@ int32_t render_frame(rt_accel *a, const rt_camera &cam, const rt_light *lights, 
     const int64_t tiles_y = (cam.y_res + 7) / 8;
     const int64_t ns = static_cast<int64_t>(tm.tiles_x) * tiles_y * 64 * spp;  // samples incl. tile padding
     if (ns >= (int64_t(1) << 31)) return fail(ctx, RT_ERR_INVALID, "too many samples per frame");
+    const int nblocks = static_cast<int>((ns + 255) / 256);
+    if (nblocks > 1024 * kScanPer) return fail(ctx, RT_ERR_INVALID, "frame too large for the hit compaction");
     int32_t err = RT_OK;
     Workspace *w = get_workspace(a, ns, std::max(1, nlights), &err);
     if (!w) return err;
@@ -412,13 +476,14 @@ int32_t render_frame(rt_accel *a, const rt_camera &cam, const rt_light *lights, 
     const int nl1 = std::max(1, nlights);
     auto grid = [](int64_t n) { return dim3(static_cast<unsigned>((n + 255) / 256)); };
 
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, w->counts, 4);
     if ((rc = rec(0))) return rc;
     hipLaunchKernelGGL(k_raygen, grid(ns), dim3(256), 0, s, c, tm, spp, p.seed, ns, w->rays1);
     if ((rc = rec(1))) return rc;
     ctx->count_slot = 0;
     if ((rc = launch_closest(a, w->rays1, ns, w->hits1, s, 0))) return rc;
     if ((rc = rec(2))) return rc;
+    hipLaunchKernelGGL(k_count_hits, dim3(nblocks), dim3(256), 0, s, w->hits1, nullptr, ns, w->blocks1);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, s, w->blocks1, nblocks);
     ShadeArgs sa{};
     sa.tm = tm;
     sa.rays = w->rays1;
@@ -437,23 +502,26 @@ int32_t render_frame(rt_accel *a, const rt_camera &cam, const rt_light *lights, 
     sa.hit_sample = w->hit_sample1;
     sa.shadow_q = w->shadow1;
     sa.bounce_q = w->bounce;
-    sa.q_count = w->counts + 0;
+    sa.block_base = w->blocks1;
     hipLaunchKernelGGL(k_shade_hit<0>, grid(ns), dim3(256), 0, s, sa);
     RT_HIP_CHECK(ctx, hipGetLastError());
     if ((rc = rec(3))) return rc;
     ctx->count_slot = 1;
-    if (nlights > 0 && (rc = launch_occluded_q(a, w->shadow1, w->counts + 0, nlights, ns * nlights, w->occ1, s, 1)))
+    if (nlights > 0 && (rc = launch_occluded_q(a, w->shadow1, w->blocks1 + nblocks, nlights, ns * nlights, w->occ1, s, 1)))
         return rc;
     if ((rc = rec(4))) return rc;
     if (p.bounce) {
         ctx->count_slot = 2;
-        if ((rc = launch_closest_q(a, w->bounce, w->counts + 0, 1, ns, w->hits2, s, 2))) return rc;
+        if ((rc = launch_closest_q(a, w->bounce, w->blocks1 + nblocks, 1, ns, w->hits2, s, 2))) return rc;
         if ((rc = rec(5))) return rc;
+        hipLaunchKernelGGL(k_count_hits, dim3(nblocks), dim3(256), 0, s, w->hits2, w->blocks1 + nblocks, ns,
+                           w->blocks2);
+        hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, s, w->blocks2, nblocks);
         ShadeArgs sb{};
         sb.tm = tm;
         sb.rays = w->bounce;
         sb.hits = w->hits2;
-        sb.count = w->counts + 0;
+        sb.count = w->blocks1 + nblocks;
         sb.n_static = ns;
         sb.tri_n = a->scene->d_n;
         sb.tri_c = a->scene->d_c;
@@ -467,12 +535,12 @@ int32_t render_frame(rt_accel *a, const rt_camera &cam, const rt_light *lights, 
         sb.hit_sample = w->hit_sample2;
         sb.shadow_q = w->shadow2;
         sb.bounce_q = nullptr;
-        sb.q_count = w->counts + 1;
+        sb.block_base = w->blocks2;
         hipLaunchKernelGGL(k_shade_hit<1>, grid(ns), dim3(256), 0, s, sb);
         RT_HIP_CHECK(ctx, hipGetLastError());
         if ((rc = rec(6))) return rc;
         ctx->count_slot = 3;
-        if (nlights > 0 && (rc = launch_occluded_q(a, w->shadow2, w->counts + 1, nlights, ns * nlights, w->occ2, s, 3)))
+        if (nlights > 0 && (rc = launch_occluded_q(a, w->shadow2, w->blocks2 + nblocks, nlights, ns * nlights, w->occ2, s, 3)))
             return rc;
         if ((rc = rec(7))) return rc;
     } else {
@@ -522,7 +590,11 @@ int32_t render_frame(rt_accel *a, const rt_camera &cam, const rt_light *lights, 
                                          dev_out ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
     }
     uint32_t counts[2] = {0, 0};
-    if (stats) RT_HIP_CHECK(ctx, hipMemcpyAsync(counts, w->counts, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    if (stats) {
+        RT_HIP_CHECK(ctx, hipMemcpyAsync(counts, w->blocks1 + nblocks, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        if (p.bounce)
+            RT_HIP_CHECK(ctx, hipMemcpyAsync(counts + 1, w->blocks2 + nblocks, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    }
     RT_HIP_CHECK(ctx, hipStreamSynchronize(s));
     if (stats) {
         stats->rays[0] = static_cast<uint64_t>(npix * spp);
