@@ -1,0 +1,111 @@
+"""Profiling helpers for the traversal kernels (run under rocprofv3).
+
+  python3 raptor_amd/profiling.py copy [--mib 1024]
+      one device-to-device copy of a known size: the calibration for the
+      FETCH_SIZE / WRITE_SIZE counters (KiB per dispatch).
+  python3 raptor_amd/profiling.py summarize --stats DIR --fetch DIR --write DIR
+                                          --copy-fetch DIR --copy-write DIR
+                                          --bench JSON --out PREFIX
+      reads rocprofv3 CSV output and writes PREFIX.md (kernel summary) and
+      PREFIX.json (HBM bytes per launch of the dominant kernel), which
+      bench.py reports as roofline.traffic.
+"""
+import argparse
+import csv
+import glob
+import json
+import os
+import sys
+
+
+def cmd_copy(args):
+    import torch
+
+    n = args.mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": n * 4}))
+
+
+def _rows(d, name):
+    files = glob.glob(os.path.join(d, "**", name), recursive=True)
+    if not files:
+        raise SystemExit(f"no {name} under {d}")
+    with open(files[0]) as f:
+        return list(csv.DictReader(f))
+
+
+def _counter(d, counter, match):
+    vals = [float(r["Counter_Value"]) for r in _rows(d, "*counter_collection.csv")
+            if r["Counter_Name"] == counter and match(r["Kernel_Name"])]
+    return sum(vals) / len(vals) if vals else None
+
+
+def cmd_summarize(args):
+    with open(args.bench) as f:
+        bench = json.loads(f.read().strip().splitlines()[-1])
+    kernel = bench["roofline"]["kernel"].split(" (")[0]          # e.g. k_trace<4, 2, false, 8>
+    stage = bench["roofline"]["kernel"].split("(")[-1].rstrip(")")
+    match = lambda name: kernel.replace(" ", "") in name.replace(" ", "")
+    is_copy = lambda name: "copy" in name.lower() or "elementwise" in name.lower()
+    fetch = _counter(args.fetch, "FETCH_SIZE", match)
+    write = _counter(args.write, "WRITE_SIZE", match)
+    copy_bytes = args.copy_bytes
+    cf = _counter(args.copy_fetch, "FETCH_SIZE", is_copy)
+    cw = _counter(args.copy_write, "WRITE_SIZE", is_copy)
+    # Counters are KiB per dispatch; the copy moves copy_bytes each way.
+    fetch_scale = copy_bytes / (cf * 1024.0)
+    write_scale = copy_bytes / (cw * 1024.0)
+    traffic = fetch * 1024.0 * fetch_scale + write * 1024.0 * write_scale
+    stats = _rows(args.stats, "*kernel_stats.csv")
+    lines = [f"# Kernel summary: {bench['config']['workload']}", "",
+             "Command (from the repository root, TMPDIR=/tmp):", "",
+             "    " + args.command, "",
+             "| kernel | calls | avg us | total % |", "|---|---|---|---|"]
+    for r in stats[:16]:
+        name = r["Name"].replace("(anonymous namespace)::", "").split("(")[0].replace("void ", "")
+        lines.append(f"| `{name}` | {r['Calls']} | {float(r['AverageNs']) / 1e3:.2f} | {float(r['Percentage']):.2f} |")
+    dom = [r for r in stats if match(r["Name"])]
+    dom_us = float(dom[0]["AverageNs"]) / 1e3 if dom else None
+    lines += ["", f"Dominant kernel `{kernel}` ({stage}): rocprof average {dom_us:.2f} us, bench HIP-event "
+              f"average {bench['roofline']['launch_ms'] * 1e3:.2f} us.", "",
+              "HBM traffic per launch (rocprofv3 --pmc, separate runs):", "",
+              f"* FETCH_SIZE {fetch:.1f} KiB x calibration {fetch_scale:.4f} "
+              f"(copy of {copy_bytes} B read {cf:.1f} KiB)",
+              f"* WRITE_SIZE {write:.1f} KiB x calibration {write_scale:.4f} "
+              f"(copy of {copy_bytes} B wrote {cw:.1f} KiB)",
+              f"* traffic = {traffic / 1e6:.2f} MB per launch; algorithmic bytes "
+              f"{bench['roofline']['algorithmic_bytes_per_launch'] / 1e6:.2f} MB per launch"]
+    with open(args.out + ".md", "w") as f:
+        f.write("\n".join(lines) + "\n")
+    out = {"kernel": kernel, "kernel_stage": stage, "width": bench["config"]["bvh_width"],
+           "lds_stack": bench["config"].get("lds_stack"), "leaf_size": bench["config"]["leaf_size"],
+           "bytes_per_launch": round(traffic), "fetch_kib": fetch, "write_kib": write,
+           "fetch_scale": fetch_scale, "write_scale": write_scale, "rocprof_avg_us": dom_us}
+    with open(args.out + ".json", "w") as f:
+        json.dump(out, f, indent=1)
+    print(json.dumps(out))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    c = sub.add_parser("copy")
+    c.add_argument("--mib", type=int, default=1024)
+    s = sub.add_parser("summarize")
+    for k in ("stats", "fetch", "write", "copy_fetch", "copy_write", "bench", "out"):
+        s.add_argument("--" + k.replace("_", "-"), dest=k, required=True)
+    s.add_argument("--copy-bytes", type=int, default=1024 * (1 << 20))
+    s.add_argument("--command", default="")
+    args = ap.parse_args()
+    if args.cmd == "copy":
+        cmd_copy(args)
+    else:
+        cmd_summarize(args)
+
+
+if __name__ == "__main__":
+    sys.exit(main())
