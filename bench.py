@@ -82,7 +82,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--width", type=int, default=4)
     ap.add_argument("--leaf-size", type=int, default=2)
-    ap.add_argument("--lds-stack", type=int, default=16)
+    ap.add_argument("--lds-stack", type=int, default=8)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()

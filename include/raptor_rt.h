@@ -69,7 +69,8 @@ typedef struct {
 /* BVH build and traversal options. width: 4 or 8 children per node;
  * leaf_size: most triangles per leaf (1..16); sah_bins: bins of the SAH
  * refinement (0 = off); lds_stack: traversal-stack entries per lane kept in
- * LDS before spilling to private memory (4, 8 or 16; 0 = 16). */
+ * LDS before spilling to private memory (4, 8 or 16; 0 = 16). params == NULL
+ * means width 4, leaf size 2, no refinement, 8 LDS entries. */
 typedef struct {
     int32_t width;
     int32_t leaf_size;
@@ -127,7 +128,7 @@ int32_t rt_scene_normals(rt_scene *scene, float *n3);
 
 /* Build the acceleration structure. Replaces bvh::bvh(primitive_store&)
  * (ssd/bvh.h:27-32, ssd/bvh_builder.cc:18-178); the caller's triangle order is
- * not permuted. params may be NULL (width 4, leaf size 4). */
+ * not permuted. params may be NULL (see rt_build_params). */
 int32_t rt_accel_build(rt_scene *scene, const rt_build_params *params, rt_accel **out);
 void rt_accel_destroy(rt_accel *accel);
 int32_t rt_accel_info_get(const rt_accel *accel, rt_accel_info *info);

@@ -177,7 +177,7 @@ int32_t rt_accel_build(rt_scene *scene, const rt_build_params *params, rt_accel 
     *out = nullptr;
     rt_ctx *ctx = scene->ctx;
     if (int32_t rc = rt::set_device(ctx)) return rc;
-    rt_build_params p{4, 2, 0, 16};
+    rt_build_params p{4, 2, 0, 8};
     if (params) p = *params;
     return rt::build_accel(scene, p, out);
 }
