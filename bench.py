@@ -133,7 +133,9 @@ def main():
     if os.path.exists(tpath):
         with open(tpath) as f:
             t = json.load(f)
-        if t.get("kernel_stage") == STAGES[dom] and t.get("width") == args.width:
+        same = (t.get("kernel_stage") == STAGES[dom] and t.get("width") == args.width
+                and t.get("leaf_size") == args.leaf_size and t.get("lds_stack") == args.lds_stack)
+        if same:
             traffic = t.get("bytes_per_launch")
 
     line = {

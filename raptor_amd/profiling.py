@@ -38,10 +38,13 @@ def _rows(d, name):
         return list(csv.DictReader(f))
 
 
-def _counter(d, counter, match):
+def _counter(d, counter, match, largest=False):
     vals = [float(r["Counter_Value"]) for r in _rows(d, "*counter_collection.csv")
             if r["Counter_Name"] == counter and match(r["Kernel_Name"])]
-    return sum(vals) / len(vals) if vals else None
+    if not vals:
+        return None
+    # The calibration copy is the largest copy dispatch of its run.
+    return max(vals) if largest else sum(vals) / len(vals)
 
 
 def cmd_summarize(args):
@@ -54,8 +57,8 @@ def cmd_summarize(args):
     fetch = _counter(args.fetch, "FETCH_SIZE", match)
     write = _counter(args.write, "WRITE_SIZE", match)
     copy_bytes = args.copy_bytes
-    cf = _counter(args.copy_fetch, "FETCH_SIZE", is_copy)
-    cw = _counter(args.copy_write, "WRITE_SIZE", is_copy)
+    cf = _counter(args.copy_fetch, "FETCH_SIZE", is_copy, largest=True)
+    cw = _counter(args.copy_write, "WRITE_SIZE", is_copy, largest=True)
     # Counters are KiB per dispatch; the copy moves copy_bytes each way.
     fetch_scale = copy_bytes / (cf * 1024.0)
     write_scale = copy_bytes / (cw * 1024.0)
