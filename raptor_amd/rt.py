@@ -13,6 +13,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libraptor_rt.so")
 
 RT_OK = 0
+RT_ERR_INVALID = -1
+RT_ERR_HIP = -2
+RT_ERR_OOM = -3
+RT_ERR_UNSUPPORTED = -4
 RT_DEVICE_PTRS = 1
 
 EPSILON = np.float32(1.0e-3)  # common/common.cc:14

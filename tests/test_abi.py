@@ -1,0 +1,89 @@
+"""The C ABI without a GPU: the library loads, exports what include/raptor_rt.h
+declares, the C++ shim compiles, and the host-side generators behave."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "raptor_rt.h")).read()
+    return sorted(set(re.findall(r"\b(rt_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    from raptor_amd import rt
+
+    names = declared_functions()
+    assert len(names) >= 15
+    missing = [n for n in names if not hasattr(rt.lib(), n)]
+    assert not missing, missing
+    assert sorted(rt.EXPORTS) == names
+
+
+def test_cpp_shim_compiles():
+    src = os.path.join(ROOT, "tests", "cpp", "shim_demo.cpp")
+    res = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-I", os.path.join(ROOT, "include"), src],
+                         capture_output=True, text=True)
+    assert res.returncode == 0, res.stderr
+
+
+def test_header_is_plain_c():
+    res = subprocess.run(["gcc", "-std=c99", "-fsyntax-only", "-Wall", "-x", "c",
+                          os.path.join(ROOT, "include", "raptor_rt.h")], capture_output=True, text=True)
+    assert res.returncode == 0, res.stderr
+
+
+def test_random_triangles_deterministic_and_bounded():
+    from raptor_amd import rt
+
+    a1, b1 = rt.gen_random_triangles(1, 1000, 0.01, 0.0, 1.0)
+    a2, b2 = rt.gen_random_triangles(1, 1000, 0.01, 0.0, 1.0)
+    a3, _ = rt.gen_random_triangles(2, 1000, 0.01, 0.0, 1.0)
+    assert np.array_equal(a1, a2) and np.array_equal(b1, b2) and not np.array_equal(a1, a3)
+    v = a1.reshape(-1, 3, 3)
+    c = v.mean(1)
+    assert np.all(np.abs(v - c[:, None, :]) <= 0.02 + 1e-6)
+    assert a1.min() >= -0.01 - 1e-6 and a1.max() <= 1.01 + 1e-6
+    assert b1.min() >= 0.0 and b1.max() < 1.0
+
+
+def test_terrain_layout():
+    # RAPTor/terrain_generator/height_map.h:16-43: vertex (i, j) at (i*xs, h, j*ys);
+    # two triangles per cell; |h| <= a * sum(p^k) because |noise| <= 1.
+    from raptor_amd import rt
+
+    g = 33
+    abc, alb = rt.gen_terrain(g, 0.15, 0.02, 0.5, 8, 57, 2, 0.2, 0.8)
+    assert abc.shape == (2 * (g - 1) ** 2, 9)
+    xs = np.float32(1.0) / np.float32(g - 1)
+    v = abc.reshape(-1, 3, 3)
+    ii = np.rint(v[..., 0] / xs).astype(int)
+    jj = np.rint(v[..., 2] / xs).astype(int)
+    assert np.array_equal(v[..., 0], (ii.astype(np.float32) * xs).astype(np.float32))
+    # First cell: (r+j, r+j+1, r'+j+1) and (r'+j, r+j, r'+j+1) with r = 0, r' = g.
+    assert (ii[0] == [0, 0, 1]).all() and (jj[0] == [0, 1, 1]).all()
+    assert (ii[1] == [1, 0, 1]).all() and (jj[1] == [0, 0, 1]).all()
+    bound = 0.15 * sum(0.5 ** k for k in range(8))
+    assert np.abs(v[..., 1]).max() <= bound + 1e-6
+    # Shared vertices carry the same height in every triangle that uses them.
+    heights = {}
+    for t in range(v.shape[0]):
+        for k in range(3):
+            key = (ii[t, k], jj[t, k])
+            heights.setdefault(key, v[t, k, 1])
+            assert heights[key] == v[t, k, 1]
+    assert len(heights) == g * g
+    assert alb.min() >= 0.2 and alb.max() < 0.8
+
+
+def test_bad_arguments_are_rejected_without_gpu():
+    from raptor_amd import rt
+
+    assert rt.lib().rt_gen_terrain(1, 0.1, 0.1, 0.5, 2, 1, 1, 0.0, 1.0, None, None) == -1
+    assert rt.lib().rt_gen_random_triangles(1, -5, 0.01, 0.0, 1.0, None, None) == -1
+    assert rt.lib().rt_ctx_create(0, None) == rt.RT_ERR_INVALID

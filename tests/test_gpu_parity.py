@@ -106,3 +106,14 @@ def test_occluded_random_rays(gpu_ctx, rt, O):
     r, _ = O.OracleScene(w.abc).trace_occluded(rays)
     assert np.array_equal(g, r), f"{(g != r).sum()} occlusion flags differ"
     assert 0.05 < r.mean() < 0.95
+
+
+def test_cpp_shim_demo_runs():
+    import os
+    import subprocess
+
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "shim_demo")
+    assert os.path.exists(exe), "build with __graft_entry__.build() first"
+    res = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert res.returncode == 0, res.stdout + res.stderr
+    assert "shim ok" in res.stdout
