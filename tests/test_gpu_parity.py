@@ -117,3 +117,37 @@ def test_cpp_shim_demo_runs():
     res = subprocess.run([exe], capture_output=True, text=True, timeout=60)
     assert res.returncode == 0, res.stdout + res.stderr
     assert "shim ok" in res.stdout
+
+
+def test_render_terrain1m_full_frame(gpu_ctx, rt, O):
+    """The benchmark workload itself: 999,698-triangle terrain, 1920x1080,
+    shadow rays and the diffuse bounce; every pixel against the oracle."""
+    import os
+
+    from raptor_amd import workloads
+
+    w = workloads.terrain1m()
+    scene = rt.Scene(gpu_ctx, w.abc, w.albedo)
+    accel = rt.Accel(scene)
+    out = accel.render(w.camera, w.lights, bounce=True, seed=w.seed)
+    threads = min(16, os.cpu_count() or 1)
+    ref = O.OracleScene(w.abc).render(w.albedo, w.camera, w.lights, bounce=True, seed=w.seed, nthreads=threads)
+    bad, ties = compare_hits(out["primary"], ref["prim"], ref["tuv"])
+    assert not bad.any(), f"{bad.sum()} primary ids differ"
+    bb, bties = compare_hits(out["bounce"], ref["bounce_prim"], ref["bounce_tuv"])
+    bb &= ~ties
+    assert not bb.any(), f"{bb.sum()} bounce ids differ"
+    ok = ~(ties | bties)
+    assert ok.mean() > 0.999
+    hit = (ref["prim"] >= 0) & ok
+    assert tuv_exact(out["primary"], ref["tuv"], hit)
+    e = rel_err(out["rgb"].reshape(-1, 3)[ok], ref["rgb"][ok])
+    assert e.max() <= RAD_TOL, e.max()
+    st = out["stats"]["rays"]
+    assert st[0] == 1920 * 1080
+    assert st[1] == int((ref["prim"] >= 0).sum())
+    assert st[2] == st[1]
+    assert st[3] == int((ref["bounce_prim"] >= 0).sum())
+    # Image-level properties: lit pixels exist, shadows exist, nothing negative.
+    assert (out["rgb"] >= 0).all()
+    assert (ref["vis"][ref["prim"] >= 0, 0] == 0).mean() > 0.01
