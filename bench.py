@@ -83,6 +83,7 @@ def main():
     ap.add_argument("--width", type=int, default=4)
     ap.add_argument("--leaf-size", type=int, default=2)
     ap.add_argument("--lds-stack", type=int, default=8)
+    ap.add_argument("--regroup", type=int, default=0, help="regroup bounce rays before tracing (0/1)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -104,13 +105,13 @@ def main():
     rgb = torch.empty((w.camera.y_res, w.camera.x_res, 3), dtype=torch.float32, device="cuda:0")
 
     for _ in range(args.warmup):
-        accel.render_device(w.camera, w.lights, rgb, bounce=w.bounce, seed=w.seed)
+        accel.render_device(w.camera, w.lights, rgb, bounce=w.bounce, seed=w.seed, regroup=bool(args.regroup))
     print("bench: timing", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     per_step = []
     for _ in range(args.steps):
-        per_step.append(accel.render_device(w.camera, w.lights, rgb, bounce=w.bounce, seed=w.seed))
+        per_step.append(accel.render_device(w.camera, w.lights, rgb, bounce=w.bounce, seed=w.seed, regroup=bool(args.regroup)))
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
 
@@ -120,7 +121,7 @@ def main():
 
     # Dominant kernel and its roofline: one extra frame with visit counting.
     ctx.set_counting(True)
-    counted = accel.render_device(w.camera, w.lights, rgb, bounce=w.bounce, seed=w.seed)
+    counted = accel.render_device(w.camera, w.lights, rgb, bounce=w.bounce, seed=w.seed, regroup=bool(args.regroup))
     ctx.set_counting(False)
     stage_ms = [sum(s["ms"][1 + k] for s in per_step) / len(per_step) for k in range(4)]
     dom = max(range(4), key=lambda k: stage_ms[k])
@@ -159,6 +160,7 @@ def main():
             "bvh_width": args.width,
             "leaf_size": args.leaf_size,
             "lds_stack": args.lds_stack,
+            "regroup_bounce": bool(args.regroup),
         },
         "rays_per_step": {"primary": rays[0], "shadow": rays[1], "bounce": rays[2], "bounce_shadow": rays[3]},
         "stage_ms": {STAGES[k]: round(stage_ms[k], 4) for k in range(4)},

@@ -31,6 +31,9 @@ extern "C" {
 
 /* Pointer-location flag for trace calls: inputs/outputs are device pointers. */
 #define RT_DEVICE_PTRS 1u
+/* rt_render_params.flags: regroup bounce rays by direction octant and origin
+ * before tracing them. */
+#define RT_REGROUP_BOUNCE 2u
 
 typedef struct rt_ctx rt_ctx;
 typedef struct rt_scene rt_scene;

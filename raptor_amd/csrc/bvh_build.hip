@@ -435,6 +435,12 @@ int32_t build_accel(rt_scene *scene, const rt_build_params &p, rt_accel **out) {
     float maxabs = 0.0f;
     for (int k = 6; k < 12; ++k) maxabs = std::max(maxabs, std::fabs(ord2f(hb[k])));
     a->box_pad = std::ldexp(std::max(maxabs, 1e-30f), -15);
+    a->near_limit = 256.0f * maxabs;
+    for (int k = 0; k < 3; ++k) {
+        a->scene_lo[k] = ord2f(hb[6 + k]);
+        const float ext = ord2f(hb[9 + k]) - a->scene_lo[k];
+        a->scene_inv[k] = ext > 0.0f ? 1.0f / ext : 0.0f;
+    }
 
     // Triangles in BVH (sorted) order.
     BUILD_CHECK(hipMalloc(&a->d_tris, n * sizeof(LeafTri)));

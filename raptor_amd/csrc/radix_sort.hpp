@@ -12,6 +12,7 @@
 #include <stdint.h>
 
 namespace rt {
+namespace {  // each including translation unit gets its own copy of the kernels
 
 constexpr int kSortThreads = 256;
 constexpr int kSortItems = 8;
@@ -208,4 +209,5 @@ inline hipError_t radix_sort_pairs(uint32_t *keys, uint32_t *vals, uint32_t *key
     return hipGetLastError();
 }
 
+}  // namespace
 }  // namespace rt

@@ -20,6 +20,7 @@
 // no host round trip until its statistics are read back.
 #include <algorithm>
 
+#include "radix_sort.hpp"
 #include "rt_device.hpp"
 #include "rt_internal.hpp"
 
@@ -243,6 +244,36 @@ __global__ void __launch_bounds__(256) k_shade_hit(ShadeArgs a) {
     }
 }
 
+// Regrouping of the incoherent bounce rays before they are traced: key =
+// direction octant (3 bits) above a 27-bit Morton code of the origin in the
+// scene box, so a wavefront gets rays that leave nearby points in similar
+// directions. Entries past the bounce count sort last (key 0xffffffff).
+__global__ void __launch_bounds__(256) k_bounce_keys(const rt_ray *__restrict__ rays, const uint32_t *count,
+                                                     int64_t n_max, float4 scene_lo, float4 scene_inv,
+                                                     uint32_t *__restrict__ keys, uint32_t *__restrict__ vals) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n_max) return;
+    uint32_t key = 0xffffffffu;
+    if (i < static_cast<int64_t>(count[0])) {
+        const float4 r0 = reinterpret_cast<const float4 *>(rays)[2 * i];
+        const float4 r1 = reinterpret_cast<const float4 *>(rays)[2 * i + 1];
+        const uint32_t oct = (r1.x < 0.0f ? 1u : 0u) | (r1.y < 0.0f ? 2u : 0u) | (r1.z < 0.0f ? 4u : 0u);
+        auto q = [](float v) {
+            return static_cast<uint32_t>(fminf(fmaxf(v * 512.0f, 0.0f), 511.0f));
+        };
+        const uint32_t qx = q((r0.x - scene_lo.x) * scene_inv.x);
+        const uint32_t qy = q((r0.y - scene_lo.y) * scene_inv.y);
+        const uint32_t qz = q((r0.z - scene_lo.z) * scene_inv.z);
+        uint32_t m = 0;
+#pragma unroll
+        for (int b = 0; b < 9; ++b)
+            m |= (((qx >> b) & 1u) << (3 * b)) | (((qy >> b) & 1u) << (3 * b + 1)) | (((qz >> b) & 1u) << (3 * b + 2));
+        key = (oct << 27) | m;
+    }
+    keys[i] = key;
+    vals[i] = static_cast<uint32_t>(i);
+}
+
 struct ComposeArgs {
     TileMap tm;
     int64_t npix;
@@ -360,6 +391,7 @@ struct Workspace {
     rt_hit *bounce_out;
     uint32_t *counts;  // unused
     uint32_t *blocks1, *blocks2;  // per-block hit counts, then prefixes; total at [nblocks]
+    uint32_t *skeys, *svals, *skeys_t, *svals_t, *shist, *ssums;  // bounce regrouping sort
 };
 
 template <class T>
@@ -391,6 +423,12 @@ size_t layout(Workspace &w, char *base, int64_t ns, int nl) {
     const int64_t nb = (ns + 255) / 256 + 1;
     w.blocks1 = carve<uint32_t>(p, nb);
     w.blocks2 = carve<uint32_t>(p, nb);
+    w.skeys = carve<uint32_t>(p, ns);
+    w.svals = carve<uint32_t>(p, ns);
+    w.skeys_t = carve<uint32_t>(p, ns);
+    w.svals_t = carve<uint32_t>(p, ns);
+    w.shist = carve<uint32_t>(p, 256 * ((ns + kSortTile - 1) / kSortTile));
+    w.ssums = carve<uint32_t>(p, kScanTile);
     return static_cast<size_t>(p - base);
 }
 
@@ -511,8 +549,18 @@ int32_t render_frame(rt_accel *a, const rt_camera &cam, const rt_light *lights, 
         return rc;
     if ((rc = rec(4))) return rc;
     if (p.bounce) {
+        const uint32_t *order = nullptr;
+        if (p.flags & RT_REGROUP_BOUNCE) {
+            const float4 lo = make_float4(a->scene_lo[0], a->scene_lo[1], a->scene_lo[2], 0.0f);
+            const float4 inv = make_float4(a->scene_inv[0], a->scene_inv[1], a->scene_inv[2], 0.0f);
+            hipLaunchKernelGGL(k_bounce_keys, grid(ns), dim3(256), 0, s, w->bounce, w->blocks1 + nblocks, ns, lo, inv,
+                               w->skeys, w->svals);
+            RT_HIP_CHECK(ctx, radix_sort_pairs(w->skeys, w->svals, w->skeys_t, w->svals_t, ns, 30, w->shist,
+                                               w->ssums, s));
+            order = w->svals;
+        }
         ctx->count_slot = 2;
-        if ((rc = launch_closest_q(a, w->bounce, w->blocks1 + nblocks, 1, ns, w->hits2, s, 2))) return rc;
+        if ((rc = launch_closest_q(a, w->bounce, w->blocks1 + nblocks, 1, ns, w->hits2, s, 2, order))) return rc;
         if ((rc = rec(5))) return rc;
         hipLaunchKernelGGL(k_count_hits, dim3(nblocks), dim3(256), 0, s, w->hits2, w->blocks1 + nblocks, ns,
                            w->blocks2);

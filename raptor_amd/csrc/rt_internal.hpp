@@ -85,6 +85,8 @@ struct rt_accel {
     float build_ms = 0.0f;
     float sah_cost = 0.0f;
     float box_pad = 0.0f;
+    float near_limit = 0.0f;
+    float scene_lo[3] = {0, 0, 0}, scene_inv[3] = {0, 0, 0};  // scene box origin and 1/extent   // |origin| bound for the fused box test (256x the largest scene coordinate)
     void *frame_ws = nullptr;  // frame workspace (render.hip)
 };
 
@@ -113,7 +115,7 @@ int32_t launch_occluded(rt_accel *a, const rt_ray *d_rays, int64_t n, uint8_t *d
 // Variants that read the ray count from device memory (queues of unknown
 // size): n = d_count[0] * mul, at most max_n.
 int32_t launch_closest_q(rt_accel *a, const rt_ray *d_rays, const uint32_t *d_count, uint32_t mul, int64_t max_n,
-                         rt_hit *d_hits, hipStream_t s, int stage);
+                         rt_hit *d_hits, hipStream_t s, int stage, const uint32_t *order = nullptr);
 int32_t launch_occluded_q(rt_accel *a, const rt_ray *d_rays, const uint32_t *d_count, uint32_t mul, int64_t max_n,
                           uint8_t *d_occ, hipStream_t s, int stage);
 

@@ -28,6 +28,8 @@ constexpr float kFarStretch = 1.00000095f;   // 1 + 2^-20
 constexpr float kHitStretch = 1.0000038f;    // 1 + 2^-18
 
 struct TraceArgs {
+    const uint32_t *order;   // if non-null, lane i traces ray order[i] (regrouped rays)
+    float near_limit;        // origins with all |o_k| <= near_limit use the fused box test
     const void *nodes;
     const LeafTri *tris;
     int32_t root;
@@ -70,13 +72,32 @@ __device__ __forceinline__ void load_node(const WideNode<W> *__restrict__ nodes,
     }
 }
 
-template <int W>
-__device__ __forceinline__ void box_test(const NodeView<W> &v, rtd::V3 o, rtd::V3 id, float tlimit, float *tn) {
+// Slab test of W children. With FUSED the plane distances are
+// fma(plane, 1/d, -o/d): one operation instead of two, with an extra error of
+// about |o| * 2^-24 in position, which the build's box padding (2^-15 of the
+// largest scene coordinate) covers while |o| stays within 256x the scene;
+// rays from farther away use the subtract-then-multiply form.
+template <int W, bool FUSED>
+__device__ __forceinline__ void box_test(const NodeView<W> &v, rtd::V3 o, rtd::V3 id, rtd::V3 oid, float tlimit,
+                                         float *tn) {
 #pragma unroll
     for (int k = 0; k < W; ++k) {
-        const float x0 = (v.lx[k] - o.x) * id.x, x1 = (v.hx[k] - o.x) * id.x;
-        const float y0 = (v.ly[k] - o.y) * id.y, y1 = (v.hy[k] - o.y) * id.y;
-        const float z0 = (v.lz[k] - o.z) * id.z, z1 = (v.hz[k] - o.z) * id.z;
+        float x0, x1, y0, y1, z0, z1;
+        if (FUSED) {
+            x0 = fmaf(v.lx[k], id.x, oid.x);
+            x1 = fmaf(v.hx[k], id.x, oid.x);
+            y0 = fmaf(v.ly[k], id.y, oid.y);
+            y1 = fmaf(v.hy[k], id.y, oid.y);
+            z0 = fmaf(v.lz[k], id.z, oid.z);
+            z1 = fmaf(v.hz[k], id.z, oid.z);
+        } else {
+            x0 = (v.lx[k] - o.x) * id.x;
+            x1 = (v.hx[k] - o.x) * id.x;
+            y0 = (v.ly[k] - o.y) * id.y;
+            y1 = (v.hy[k] - o.y) * id.y;
+            z0 = (v.lz[k] - o.z) * id.z;
+            z1 = (v.hz[k] - o.z) * id.z;
+        }
         const float tnear = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.0f));
         const float tfar = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1));
         const bool hit = (v.c[k] != kEmpty) && (tnear <= tfar * kFarStretch) && (tnear <= tlimit);
@@ -146,14 +167,17 @@ __global__ void __launch_bounds__(64) k_trace(TraceArgs a) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * 64 + lane;
     const int64_t n = a.count ? static_cast<int64_t>(a.count[0]) * a.count_mul : a.n;
     if (i >= n) return;
+    const int64_t ri = a.order ? static_cast<int64_t>(a.order[i]) : i;
     const WideNode<W> *__restrict__ nodes = reinterpret_cast<const WideNode<W> *>(a.nodes);
-    const float4 r0 = reinterpret_cast<const float4 *>(a.rays)[2 * i];
-    const float4 r1 = reinterpret_cast<const float4 *>(a.rays)[2 * i + 1];
+    const float4 r0 = reinterpret_cast<const float4 *>(a.rays)[2 * ri];
+    const float4 r1 = reinterpret_cast<const float4 *>(a.rays)[2 * ri + 1];
     const rtd::V3 o = rtd::mk(r0.x, r0.y, r0.z);
     const rtd::V3 d = rtd::mk(r1.x, r1.y, r1.z);
     const float tmin = r0.w;
     const float tmax = r1.w;
     const rtd::V3 id = rtd::mk(safe_inv(d.x), safe_inv(d.y), safe_inv(d.z));
+    const rtd::V3 oid = rtd::mk(-(o.x * id.x), -(o.y * id.y), -(o.z * id.z));
+    const bool near = fmaxf(fmaxf(fabsf(o.x), fabsf(o.y)), fabsf(o.z)) <= a.near_limit;
 
     // Two separate arrays so that a spill read never has the type of an LDS
     // entry read (the compiler would otherwise merge both into a flat load).
@@ -204,7 +228,10 @@ __global__ void __launch_bounds__(64) k_trace(TraceArgs a) {
                 load_node<W>(nodes, cur, v);
                 if (COUNT) ++visits;
                 float tn[W];
-                box_test<W>(v, o, id, best_t * kHitStretch, tn);
+                if (near)
+                    box_test<W, true>(v, o, id, oid, best_t * kHitStretch, tn);
+                else
+                    box_test<W, false>(v, o, id, oid, best_t * kHitStretch, tn);
                 int32_t cc[W];
 #pragma unroll
                 for (int k = 0; k < W; ++k) cc[k] = v.c[k];
@@ -274,9 +301,9 @@ __global__ void __launch_bounds__(64) k_trace(TraceArgs a) {
         const float ht = (best_id >= 0) ? best_t : rtd::kMaxDist;
         const float hu = (best_id >= 0) ? best_u : 0.0f;
         const float hv = (best_id >= 0) ? best_v : 0.0f;
-        reinterpret_cast<float4 *>(a.hits)[i] = make_float4(ht, hu, hv, __int_as_float(best_id));
+        reinterpret_cast<float4 *>(a.hits)[ri] = make_float4(ht, hu, hv, __int_as_float(best_id));
     } else {
-        a.occ[i] = (best_id >= 0) ? 1 : 0;
+        a.occ[ri] = (best_id >= 0) ? 1 : 0;
     }
     if (COUNT) {
         unsigned long long vs = visits, ts = tests;
@@ -325,6 +352,7 @@ int32_t launch(rt_accel *acc, TraceArgs &ta, int64_t max_n, int stage, hipStream
     ta.nodes = acc->d_nodes;
     ta.tris = acc->d_tris;
     ta.root = acc->root;
+    ta.near_limit = acc->near_limit;
     ta.counters = ctx->counting ? ctx->d_counters + 2 * ctx->count_slot : nullptr;
     ta.overflow = ctx->counting ? reinterpret_cast<uint32_t *>(ctx->d_counters + 15) : nullptr;
     const int64_t blocks = (max_n + 63) / 64;
@@ -377,9 +405,10 @@ int32_t launch_occluded(rt_accel *acc, const rt_ray *d_rays, int64_t n, uint8_t 
 }
 
 int32_t launch_closest_q(rt_accel *acc, const rt_ray *d_rays, const uint32_t *d_count, uint32_t mul, int64_t max_n,
-                         rt_hit *d_hits, hipStream_t s, int stage) {
+                         rt_hit *d_hits, hipStream_t s, int stage, const uint32_t *order) {
     if (acc->root == kEmpty) return empty_scene(acc, d_hits, nullptr, d_count, mul, max_n, s);
     TraceArgs ta{};
+    ta.order = order;
     ta.rays = d_rays;
     ta.count = d_count;
     ta.count_mul = mul;

@@ -18,6 +18,7 @@ RT_ERR_HIP = -2
 RT_ERR_OOM = -3
 RT_ERR_UNSUPPORTED = -4
 RT_DEVICE_PTRS = 1
+RT_REGROUP_BOUNCE = 2
 
 EPSILON = np.float32(1.0e-3)  # common/common.cc:14
 MAX_DIST = np.float32(np.finfo(np.float32).max)  # common/common.cc:17
@@ -233,12 +234,12 @@ class Accel:
         self.ctx.check(_LIB.rt_trace_occluded(self.h, _ptr(rays), n, _ptr(out), RT_DEVICE_PTRS), "rt_trace_occluded")
         return out
 
-    def render(self, camera, lights, bounce=False, spp=1, seed=0, want_hits=True, stats=True):
+    def render(self, camera, lights, bounce=False, spp=1, seed=0, want_hits=True, stats=True, regroup=False):
         """One frame; returns dict(rgb (H,W,3), primary hits, bounce hits, stats)."""
         cam = camera.to_c()
         nl = len(lights)
         larr = (rt_light * max(1, nl))(*[l.to_c() for l in lights])
-        p = rt_render_params(1 if bounce else 0, spp, seed, 0, 0)
+        p = rt_render_params(1 if bounce else 0, spp, seed, RT_REGROUP_BOUNCE if regroup else 0, 0)
         npix = camera.x_res * camera.y_res
         rgb = np.empty((camera.y_res, camera.x_res, 3), np.float32)
         prim = np.empty(npix * spp, HIT_DTYPE) if want_hits else None
@@ -251,12 +252,12 @@ class Accel:
             out["stats"] = {"rays": list(st.rays), "ms": list(st.ms), "visits": list(st.visits)}
         return out
 
-    def render_device(self, camera, lights, rgb_dev, bounce=False, spp=1, seed=0):
+    def render_device(self, camera, lights, rgb_dev, bounce=False, spp=1, seed=0, regroup=False):
         """Frame into a device buffer (no host copies); returns the stats dict."""
         cam = camera.to_c()
         nl = len(lights)
         larr = (rt_light * max(1, nl))(*[l.to_c() for l in lights])
-        p = rt_render_params(1 if bounce else 0, spp, seed, RT_DEVICE_PTRS, 0)
+        p = rt_render_params(1 if bounce else 0, spp, seed, RT_DEVICE_PTRS | (RT_REGROUP_BOUNCE if regroup else 0), 0)
         st = rt_stats()
         self.ctx.check(_LIB.rt_render(self.h, C.byref(cam), larr, nl, C.byref(p), _ptr(rgb_dev), None, None,
                                       C.byref(st)), "rt_render")

@@ -28,10 +28,10 @@ def O():
     return _o
 
 
-def _render_both(rt, O, ctx, w, width=4, leaf=2, stride=1, lds=16):
+def _render_both(rt, O, ctx, w, width=4, leaf=2, stride=1, lds=16, regroup=False):
     scene = rt.Scene(ctx, w.abc, w.albedo)
     accel = rt.Accel(scene, width=width, leaf_size=leaf, lds_stack=lds)
-    out = accel.render(w.camera, w.lights, bounce=w.bounce, spp=w.spp, seed=w.seed)
+    out = accel.render(w.camera, w.lights, bounce=w.bounce, spp=w.spp, seed=w.seed, regroup=regroup)
     ref = O.OracleScene(w.abc).render(w.albedo, w.camera, w.lights, bounce=w.bounce, spp=w.spp, seed=w.seed,
                                       pixel_stride=stride, nthreads=8)
     return out, ref
@@ -69,12 +69,12 @@ def test_render_random65k(gpu_ctx, rt, O):
     assert out["stats"]["rays"][1] == int((ref["prim"] >= 0).sum())
 
 
-@pytest.mark.parametrize("width,lds", [(4, 16), (8, 4)])
-def test_render_terrain_bounce(gpu_ctx, rt, O, width, lds):
+@pytest.mark.parametrize("width,lds,regroup", [(4, 16, False), (8, 4, False), (4, 8, True)])
+def test_render_terrain_bounce(gpu_ctx, rt, O, width, lds, regroup):
     from raptor_amd import workloads
 
     w = workloads.terrain(grid=200, x_res=320, y_res=180, bounce=True)
-    out, ref = _render_both(rt, O, gpu_ctx, w, width=width, lds=lds)
+    out, ref = _render_both(rt, O, gpu_ctx, w, width=width, lds=lds, regroup=regroup)
     bad, ties = compare_hits(out["primary"], ref["prim"], ref["tuv"])
     assert not bad.any(), f"{bad.sum()} primary ids differ"
     bb, bties = compare_hits(out["bounce"], ref["bounce_prim"], ref["bounce_tuv"])
