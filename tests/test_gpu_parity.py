@@ -152,6 +152,6 @@ def test_render_terrain1m_full_frame(gpu_ctx, rt, O):
     # match the oracle away from tie pixels (a tie changes the bounce origin).
     assert st[3] == int((out["bounce"]["prim"] >= 0).sum())
     assert np.array_equal(out["bounce"]["prim"][ok] >= 0, ref["bounce_prim"][ok] >= 0)
-    # Image-level properties: lit pixels exist, shadows exist, nothing negative.
+    # Image-level properties: nothing negative, shading varies across the frame.
     assert (out["rgb"] >= 0).all()
-    assert (ref["vis"][ref["prim"] >= 0, 0] == 0).mean() > 0.01
+    assert out["rgb"].reshape(-1, 3)[ref["prim"] >= 0].std() > 1.0
