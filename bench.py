@@ -75,6 +75,77 @@ def cpu_baseline(w, target_s):
     }
 
 
+def batched_section(ctx, n_scenes, reps):
+    """Batched workload: n_scenes independent scenes (triangle counts
+    log-uniform in [1e3, 2e6], terrain and random triangles 50/50), each with
+    side x side incoherent surface rays (side log-uniform in [64, 1024]),
+    closest-hit queries. The autotuner picks node width (BVH4 vs BVH8 by
+    building both), LDS stack depth, resident wavefronts per CU and chunks per
+    grab; the tuned persistent launch (hipGraph replay) is then timed against
+    one plain launch per scene on the same BVHs, median of reps frames each."""
+    import statistics
+
+    import torch
+
+    from raptor_amd import rt, workloads
+
+    t0 = time.perf_counter()
+    specs = workloads.batched(n_scenes=n_scenes)
+    gen_s = time.perf_counter() - t0
+    scenes, rays, hits_a, hits_b = [], [], [], []
+    for sp in specs:
+        sc = rt.Scene(ctx, sp.abc, sp.albedo)
+        scenes.append(sc)
+        rays.append(rt.surface_rays(sc, sp.seed, sp.rays))
+        hits_a.append(torch.empty((sp.rays, 4), dtype=torch.float32, device="cuda:0"))
+        hits_b.append(torch.empty((sp.rays, 4), dtype=torch.float32, device="cuda:0"))
+    total_rays = sum(sp.rays for sp in specs)
+    total_tris = sum(int(sp.abc.shape[0]) for sp in specs)
+    del specs
+
+    def timed(batch, plain):
+        for _ in range(2):
+            batch.run(plain=plain)
+        return statistics.median(batch.run(plain=plain) for _ in range(reps))
+
+    per_width, best = {}, None
+    for width in (4, 8):
+        accels = [rt.Accel(sc, width=width, leaf_size=2, lds_stack=8) for sc in scenes]
+        build_ms = sum(a.info()["build_ms"] for a in accels)
+        params, tuned_ms = rt.Batch.autotune(ctx, accels, rays, hits_a, reps=3)
+        plain = rt.Batch(ctx, accels, rays, hits_b, use_graph=False)
+        plain_ms = timed(plain, True)
+        plain.close()
+        per_width[f"bvh{width}"] = {"build_ms_total": round(build_ms, 2), "autotuned": params,
+                                    "autotune_ms": round(tuned_ms, 4), "plain_ms": round(plain_ms, 4)}
+        if best is None or tuned_ms < best[2]:
+            best = (width, params, tuned_ms, accels)
+        else:
+            del accels
+    width, params, _, accels = best
+    pers = rt.Batch(ctx, accels, rays, hits_a, use_graph=True, **params)
+    plain = rt.Batch(ctx, accels, rays, hits_b, use_graph=False)
+    pers_ms = timed(pers, False)
+    plain_ms = timed(plain, True)
+    same = all(torch.equal(a.view(torch.int32), b.view(torch.int32)) for a, b in zip(hits_a, hits_b))
+    pers.close()
+    plain.close()
+    return {
+        "workload": f"{n_scenes} scenes, {total_tris} triangles, {total_rays} incoherent surface rays "
+                    "(closest hit), one frame = every scene's batch",
+        "generate_s": round(gen_s, 2),
+        "width": width,
+        "params": params,
+        "persistent_graph_ms": round(pers_ms, 4),
+        "plain_per_scene_ms": round(plain_ms, 4),
+        "persistent_mrays_s": round(total_rays / pers_ms / 1e3, 2),
+        "plain_mrays_s": round(total_rays / plain_ms / 1e3, 2),
+        "speedup": round(plain_ms / pers_ms, 3),
+        "hits_identical": same,
+        "per_width": per_width,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -86,6 +157,8 @@ def main():
     ap.add_argument("--regroup", type=int, default=0, help="regroup bounce rays before tracing (0/1)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batched-scenes", type=int, default=64, help="0 skips the batched workload")
+    ap.add_argument("--batched-reps", type=int, default=10)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each GPU would trace its own share "
@@ -180,6 +253,10 @@ def main():
             "tri_tests_per_ray": round(counted["visits"][2 * dom + 1] / max(1, counted["rays"][dom]), 3),
         },
     }
+    if args.batched_scenes > 0:
+        del rgb, accel, scene
+        print("bench: batched workload", file=sys.stderr, flush=True)
+        line["batched"] = batched_section(ctx, args.batched_scenes, args.batched_reps)
     if not args.no_cpu_baseline:
         print("bench: CPU baseline", file=sys.stderr, flush=True)
         cb = cpu_baseline(w, args.cpu_seconds)

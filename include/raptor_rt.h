@@ -114,6 +114,9 @@ typedef struct {
     float sah_cost;
 } rt_accel_info;
 
+/* Destroy calls release the caller's handle and may come in any order: a
+ * context, scene or accel is freed once no scene, accel or batch built on it
+ * remains. */
 int32_t rt_ctx_create(int32_t hip_device, rt_ctx **out);
 void rt_ctx_destroy(rt_ctx *ctx);
 const char *rt_last_error(const rt_ctx *ctx);
@@ -153,6 +156,47 @@ int32_t rt_trace_occluded(rt_accel *accel, const rt_ray *rays, int64_t n, uint8_
  * RT_DEVICE_PTRS in params->flags the output pointers are device pointers. */
 int32_t rt_render(rt_accel *accel, const rt_camera *cam, const rt_light *lights, int32_t nlights,
                   const rt_render_params *params, float *rgb, rt_hit *primary, rt_hit *bounce, rt_stats *stats);
+
+/* ---- Batched tracing (many scenes in one launch) ----
+ * An item is one scene's BVH with a batch of rays and room for the hits, all
+ * device pointers. Items of one batch share a node width. The reference has no
+ * batching; each item is the work of one bvh::find_nearest_object loop. */
+typedef struct {
+    rt_accel *accel;
+    const rt_ray *rays;
+    int64_t n;
+    rt_hit *hits;
+} rt_batch_item;
+
+/* Persistent-kernel parameters: lds_stack (4, 8, 16), resident wavefronts per
+ * CU, 64-ray chunks taken per atomic, and whether to replay a captured
+ * hipGraph (1) or launch directly (0). Zeros select 8, 16, 1. */
+typedef struct {
+    int32_t lds_stack;
+    int32_t waves_per_cu;
+    int32_t grab;
+    int32_t use_graph;
+} rt_batch_params;
+
+typedef struct rt_batch rt_batch;
+
+int32_t rt_batch_create(rt_ctx *ctx, const rt_batch_item *items, int32_t nitems, const rt_batch_params *params,
+                        rt_batch **out);
+/* mode 0: the persistent work-queue launch (hipGraph replay when captured);
+ * mode 1: one closest-hit launch per item (baseline). ms: device time. */
+int32_t rt_batch_run(rt_batch *batch, int32_t mode, float *ms);
+void rt_batch_destroy(rt_batch *batch);
+/* Times the persistent launch over lds_stack x waves_per_cu x grab (reps runs
+ * each after a warm-up) and returns the fastest parameters (use_graph = 1). */
+int32_t rt_batch_autotune(rt_ctx *ctx, const rt_batch_item *items, int32_t nitems, int32_t reps,
+                          rt_batch_params *best, float *best_ms);
+
+/* Incoherent rays leaving the scene's surfaces, on the device: a uniformly
+ * chosen triangle, a uniform point on it, the origin offset as shadow-ray
+ * origins are (rt/ray.h:111-157) along the triangle normal, and a
+ * cosine-weighted direction about that normal (counter-based sampler keyed by
+ * seed and ray index). rays: n device rt_ray. */
+int32_t rt_gen_surface_rays(rt_scene *scene, uint64_t seed, int64_t n, rt_ray *rays);
 
 /* Count node visits and triangle tests in later calls (0/1). */
 int32_t rt_set_counting(rt_ctx *ctx, int32_t on);

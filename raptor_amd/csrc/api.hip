@@ -44,6 +44,32 @@ __global__ void __launch_bounds__(256) k_scene_prep(const float *__restrict__ ab
     kd[i] = k;
 }
 
+// Surface rays: triangle k = floor(r0 * n); point by the square-root
+// barycentric map (Osada et al. 2002); origin offset along the face normal as
+// rt/ray.h:111-157 offsets shadow-ray origins; cosine-weighted direction.
+__global__ void __launch_bounds__(256) k_surface_rays(const float *__restrict__ abc, const float4 *__restrict__ tri_n,
+                                                      int64_t ntri, uint64_t seed, int64_t n, float4 *__restrict__ rays) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t k0 = rtd::sample_key(seed, static_cast<uint32_t>(i), static_cast<uint32_t>(i >> 32), 254u);
+    const float r0 = rtd::unit24(static_cast<uint32_t>(k0 >> 40));
+    const float r1 = rtd::unit24(static_cast<uint32_t>((k0 >> 16) & 0xFFFFFFu));
+    const float r2 = rtd::unit24(static_cast<uint32_t>(k0 & 0xFFFFu) << 8);
+    int64_t tri = static_cast<int64_t>(static_cast<double>(r0) * static_cast<double>(ntri));
+    if (tri >= ntri) tri = ntri - 1;
+    const float *t = abc + 9 * tri;
+    const float s = sqrtf(r1);
+    const float wa = 1.0f - s, wb = r2 * s, wc = 1.0f - (wa + wb);
+    const rtd::V3 p = rtd::mk((wa * t[0] + wb * t[3]) + wc * t[6], (wa * t[1] + wb * t[4]) + wc * t[7],
+                              (wa * t[2] + wb * t[5]) + wc * t[8]);
+    const float4 nn = tri_n[tri];
+    const rtd::V3 nv = rtd::mk(nn.x, nn.y, nn.z);
+    const rtd::V3 o = rtd::offset_start_point(p, nv, 1);
+    const rtd::V3 d = rtd::cosine_direction(rtd::sample_key(seed, static_cast<uint32_t>(i), static_cast<uint32_t>(i >> 32), 1u), nv);
+    rays[2 * i] = make_float4(o.x, o.y, o.z, rtd::kEpsilon);
+    rays[2 * i + 1] = make_float4(d.x, d.y, d.z, rtd::kMaxDist);
+}
+
 template <class T>
 hipError_t upload(T **dst, const T *src, int64_t count, hipStream_t s) {
     hipError_t e = hipMalloc(dst, std::max<int64_t>(1, count) * sizeof(T));
@@ -68,6 +94,39 @@ int32_t set_device(rt_ctx *ctx) {
 }
 
 }  // namespace
+
+void release_ctx(rt_ctx *ctx) {
+    if (!ctx || ctx->refs.fetch_sub(1) != 1) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto &e : ctx->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (ctx->d_counters) (void)hipFree(ctx->d_counters);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+void release_scene(rt_scene *scene) {
+    if (!scene || scene->refs.fetch_sub(1) != 1) return;
+    rt_ctx *ctx = scene->ctx;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    free_scene(scene);
+    release_ctx(ctx);
+}
+
+void release_accel(rt_accel *accel) {
+    if (!accel || accel->refs.fetch_sub(1) != 1) return;
+    rt_scene *scene = accel->scene;
+    (void)hipSetDevice(scene->ctx->device);
+    (void)hipStreamSynchronize(scene->ctx->stream);
+    free_frame_workspace(accel);
+    if (accel->d_nodes) (void)hipFree(accel->d_nodes);
+    if (accel->d_tris) (void)hipFree(accel->d_tris);
+    delete accel;
+    release_scene(scene);
+}
+
 }  // namespace rt
 
 using rt::fail;
@@ -102,16 +161,7 @@ int32_t rt_ctx_create(int32_t hip_device, rt_ctx **out) {
     return RT_OK;
 }
 
-void rt_ctx_destroy(rt_ctx *ctx) {
-    if (!ctx) return;
-    (void)hipSetDevice(ctx->device);
-    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    for (auto &e : ctx->ev)
-        if (e) (void)hipEventDestroy(e);
-    if (ctx->d_counters) (void)hipFree(ctx->d_counters);
-    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
-    delete ctx;
-}
+void rt_ctx_destroy(rt_ctx *ctx) { rt::release_ctx(ctx); }
 
 const char *rt_last_error(const rt_ctx *ctx) { return ctx ? ctx->last_error.c_str() : "null context"; }
 
@@ -145,16 +195,12 @@ int32_t rt_scene_create(rt_ctx *ctx, const float *abc, const float *albedo, cons
         return fail(ctx, e == hipErrorOutOfMemory ? RT_ERR_OOM : RT_ERR_HIP,
                     std::string("rt_scene_create: ") + hipGetErrorString(e));
     }
+    ctx->refs.fetch_add(1);
     *out = sc;
     return RT_OK;
 }
 
-void rt_scene_destroy(rt_scene *scene) {
-    if (!scene) return;
-    (void)hipSetDevice(scene->ctx->device);
-    (void)hipStreamSynchronize(scene->ctx->stream);
-    rt::free_scene(scene);
-}
+void rt_scene_destroy(rt_scene *scene) { rt::release_scene(scene); }
 
 int32_t rt_scene_normals(rt_scene *scene, float *n3) {
     if (!scene || !n3) return RT_ERR_INVALID;
@@ -179,18 +225,12 @@ int32_t rt_accel_build(rt_scene *scene, const rt_build_params *params, rt_accel 
     if (int32_t rc = rt::set_device(ctx)) return rc;
     rt_build_params p{4, 2, 0, 8};
     if (params) p = *params;
-    return rt::build_accel(scene, p, out);
+    const int32_t rc = rt::build_accel(scene, p, out);
+    if (rc == RT_OK) scene->refs.fetch_add(1);
+    return rc;
 }
 
-void rt_accel_destroy(rt_accel *accel) {
-    if (!accel) return;
-    (void)hipSetDevice(accel->scene->ctx->device);
-    (void)hipStreamSynchronize(accel->scene->ctx->stream);
-    rt::free_frame_workspace(accel);
-    if (accel->d_nodes) (void)hipFree(accel->d_nodes);
-    if (accel->d_tris) (void)hipFree(accel->d_tris);
-    delete accel;
-}
+void rt_accel_destroy(rt_accel *accel) { rt::release_accel(accel); }
 
 int32_t rt_accel_info_get(const rt_accel *a, rt_accel_info *info) {
     if (!a || !info) return RT_ERR_INVALID;
@@ -271,6 +311,19 @@ int32_t rt_render(rt_accel *accel, const rt_camera *cam, const rt_light *lights,
         }
     }
     return rc;
+}
+
+int32_t rt_gen_surface_rays(rt_scene *scene, uint64_t seed, int64_t n, rt_ray *rays) {
+    if (!scene || n < 0 || (n > 0 && !rays)) return RT_ERR_INVALID;
+    rt_ctx *ctx = scene->ctx;
+    if (scene->n <= 0) return fail(ctx, RT_ERR_INVALID, "rt_gen_surface_rays: empty scene");
+    if (int32_t rc = rt::set_device(ctx)) return rc;
+    if (n == 0) return RT_OK;
+    hipLaunchKernelGGL(rt::k_surface_rays, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, ctx->stream,
+                       scene->d_abc, scene->d_n, scene->n, seed, n, reinterpret_cast<float4 *>(rays));
+    RT_HIP_CHECK(ctx, hipGetLastError());
+    RT_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return RT_OK;
 }
 
 int32_t rt_set_counting(rt_ctx *ctx, int32_t on) {

@@ -6,6 +6,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <atomic>
 #include <string>
 #include <vector>
 
@@ -59,6 +60,7 @@ struct rt_ctx {
     // slots 0..3, the stack-overflow flag at index 15.
     unsigned long long *d_counters = nullptr;
     int count_slot = 0;
+    std::atomic<int> refs{1};  // the caller's handle + one per scene
 };
 
 struct rt_scene {
@@ -69,6 +71,7 @@ struct rt_scene {
     float4 *d_c = nullptr;     // vertex c (xyz) for the plane re-projection
     float4 *d_kd = nullptr;    // kd = (255 * albedo) * 1.0
     uint8_t *d_flags = nullptr;
+    std::atomic<int> refs{1};  // the caller's handle + one per accel
 };
 
 struct rt_accel {
@@ -88,9 +91,16 @@ struct rt_accel {
     float near_limit = 0.0f;
     float scene_lo[3] = {0, 0, 0}, scene_inv[3] = {0, 0, 0};  // scene box origin and 1/extent   // |origin| bound for the fused box test (256x the largest scene coordinate)
     void *frame_ws = nullptr;  // frame workspace (render.hip)
+    std::atomic<int> refs{1};  // the caller's handle + one per batch item
 };
 
 namespace rt {
+
+// Handles are reference counted so destroy calls may come in any order: an
+// object is freed once its own handle and every object built on it are gone.
+void release_ctx(rt_ctx *ctx);
+void release_scene(rt_scene *scene);
+void release_accel(rt_accel *accel);
 
 // Error helpers: record text on the context and return a code.
 int32_t fail(rt_ctx *ctx, int32_t code, const std::string &msg);

@@ -1,31 +1,10 @@
-// Wide-BVH traversal kernels: closest hit and any hit.
-//
-// Replaces the scalar and 4-wide packet traversals of ssd/bvh.cc:341-759.
-// One ray per lane of a 64-wide wavefront. A lane visits a node by loading the
-// W child boxes (structure of arrays, 16-byte loads), testing them, sorting
-// the hits near-to-far with a sorting network and keeping the nearest as the
-// next node; the others go on a per-lane stack whose first kLdsDepth entries
-// live in LDS (one column per lane, so a wave's accesses hit 64 different
-// banks) and whose remainder spills to private memory. Stack entries carry
-// their entry distance so that closest-hit pops skip subtrees beyond the
-// current hit. Leaves are tested with the reference's Moller-Trumbore
-// (rt_device.hpp); an exact tie in t keeps the lowest emplace id.
-//
-// The box test is conservative: boxes are padded at build time, the inverse
-// direction never divides by zero, t_far is stretched by 2^-20 and the cull
-// against the current hit by 2^-18, so no box holding an accepted triangle is
-// skipped (the reference's own slack is the absolute +100*EPSILON of
-// ssd/bvh.cc:620,627,705).
-#include "rt_device.hpp"
-#include "rt_internal.hpp"
+// Per-launch traversal kernels (closest hit and any hit) and their launchers.
+// The traversal itself is in traverse.hpp.
+#include "traverse.hpp"
 
 namespace rt {
 
 namespace {
-
-constexpr int kStackDepth = 128;  // LDS part + private spill part
-constexpr float kFarStretch = 1.00000095f;   // 1 + 2^-20
-constexpr float kHitStretch = 1.0000038f;    // 1 + 2^-18
 
 struct TraceArgs {
     const uint32_t *order;   // if non-null, lane i traces ray order[i] (regrouped rays)
@@ -43,264 +22,30 @@ struct TraceArgs {
     uint32_t *overflow;            // set to 1 if a stack overflowed
 };
 
-__device__ __forceinline__ float safe_inv(float d) {
-    return 1.0f / ((fabsf(d) > 1e-20f) ? d : copysignf(1e-20f, d));
-}
-
-template <int W>
-struct NodeView {
-    float lx[W], hx[W], ly[W], hy[W], lz[W], hz[W];
-    int32_t c[W];
-};
-
-template <int W>
-__device__ __forceinline__ void load_node(const WideNode<W> *__restrict__ nodes, int32_t idx, NodeView<W> &v) {
-    const float4 *p = reinterpret_cast<const float4 *>(nodes + idx);
-    constexpr int Q = W / 4;  // float4 per array
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-        const float4 a = p[0 * Q + q], b = p[1 * Q + q], c = p[2 * Q + q];
-        const float4 d = p[3 * Q + q], e = p[4 * Q + q], f = p[5 * Q + q];
-        const int4 g = reinterpret_cast<const int4 *>(p)[6 * Q + q];
-        v.lx[4 * q + 0] = a.x; v.lx[4 * q + 1] = a.y; v.lx[4 * q + 2] = a.z; v.lx[4 * q + 3] = a.w;
-        v.hx[4 * q + 0] = b.x; v.hx[4 * q + 1] = b.y; v.hx[4 * q + 2] = b.z; v.hx[4 * q + 3] = b.w;
-        v.ly[4 * q + 0] = c.x; v.ly[4 * q + 1] = c.y; v.ly[4 * q + 2] = c.z; v.ly[4 * q + 3] = c.w;
-        v.hy[4 * q + 0] = d.x; v.hy[4 * q + 1] = d.y; v.hy[4 * q + 2] = d.z; v.hy[4 * q + 3] = d.w;
-        v.lz[4 * q + 0] = e.x; v.lz[4 * q + 1] = e.y; v.lz[4 * q + 2] = e.z; v.lz[4 * q + 3] = e.w;
-        v.hz[4 * q + 0] = f.x; v.hz[4 * q + 1] = f.y; v.hz[4 * q + 2] = f.z; v.hz[4 * q + 3] = f.w;
-        v.c[4 * q + 0] = g.x; v.c[4 * q + 1] = g.y; v.c[4 * q + 2] = g.z; v.c[4 * q + 3] = g.w;
-    }
-}
-
-// Slab test of W children. With FUSED the plane distances are
-// fma(plane, 1/d, -o/d): one operation instead of two, with an extra error of
-// about |o| * 2^-24 in position, which the build's box padding (2^-15 of the
-// largest scene coordinate) covers while |o| stays within 256x the scene;
-// rays from farther away use the subtract-then-multiply form.
-template <int W, bool FUSED>
-__device__ __forceinline__ void box_test(const NodeView<W> &v, rtd::V3 o, rtd::V3 id, rtd::V3 oid, float tlimit,
-                                         float *tn) {
-#pragma unroll
-    for (int k = 0; k < W; ++k) {
-        float x0, x1, y0, y1, z0, z1;
-        if (FUSED) {
-            x0 = fmaf(v.lx[k], id.x, oid.x);
-            x1 = fmaf(v.hx[k], id.x, oid.x);
-            y0 = fmaf(v.ly[k], id.y, oid.y);
-            y1 = fmaf(v.hy[k], id.y, oid.y);
-            z0 = fmaf(v.lz[k], id.z, oid.z);
-            z1 = fmaf(v.hz[k], id.z, oid.z);
-        } else {
-            x0 = (v.lx[k] - o.x) * id.x;
-            x1 = (v.hx[k] - o.x) * id.x;
-            y0 = (v.ly[k] - o.y) * id.y;
-            y1 = (v.hy[k] - o.y) * id.y;
-            z0 = (v.lz[k] - o.z) * id.z;
-            z1 = (v.hz[k] - o.z) * id.z;
-        }
-        const float tnear = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.0f));
-        const float tfar = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1));
-        const bool hit = (v.c[k] != kEmpty) && (tnear <= tfar * kFarStretch) && (tnear <= tlimit);
-        tn[k] = hit ? tnear : INFINITY;
-    }
-}
-
-__device__ __forceinline__ void cswap(float &ta, int32_t &ca, float &tb, int32_t &cb) {
-    const bool sw = tb < ta;
-    const float t = sw ? tb : ta;
-    const float u = sw ? ta : tb;
-    const int32_t c = sw ? cb : ca;
-    const int32_t d = sw ? ca : cb;
-    ta = t;
-    tb = u;
-    ca = c;
-    cb = d;
-}
-
-template <int W>
-__device__ __forceinline__ void sort_children(float *t, int32_t *c);
-
-template <>
-__device__ __forceinline__ void sort_children<4>(float *t, int32_t *c) {
-    cswap(t[0], c[0], t[1], c[1]);
-    cswap(t[2], c[2], t[3], c[3]);
-    cswap(t[0], c[0], t[2], c[2]);
-    cswap(t[1], c[1], t[3], c[3]);
-    cswap(t[1], c[1], t[2], c[2]);
-}
-
-template <>
-__device__ __forceinline__ void sort_children<8>(float *t, int32_t *c) {
-    // Batcher's odd-even merge sort network for 8 keys (19 comparators).
-    cswap(t[0], c[0], t[1], c[1]); cswap(t[2], c[2], t[3], c[3]);
-    cswap(t[4], c[4], t[5], c[5]); cswap(t[6], c[6], t[7], c[7]);
-    cswap(t[0], c[0], t[2], c[2]); cswap(t[1], c[1], t[3], c[3]);
-    cswap(t[4], c[4], t[6], c[6]); cswap(t[5], c[5], t[7], c[7]);
-    cswap(t[1], c[1], t[2], c[2]); cswap(t[5], c[5], t[6], c[6]);
-    cswap(t[0], c[0], t[4], c[4]); cswap(t[1], c[1], t[5], c[5]);
-    cswap(t[2], c[2], t[6], c[6]); cswap(t[3], c[3], t[7], c[7]);
-    cswap(t[2], c[2], t[4], c[4]); cswap(t[3], c[3], t[5], c[5]);
-    cswap(t[1], c[1], t[2], c[2]); cswap(t[3], c[3], t[4], c[4]);
-    cswap(t[5], c[5], t[6], c[6]);
-}
-
 // STAGE names the launch (0 primary, 2 bounce, 4 API: closest hit; 1 shadow,
 // 3 bounce shadow, 5 API: any hit) so each has its own kernel symbol in
 // profiles; even stages are closest-hit queries.
-//
-// Loop structure (Aila & Laine 2009, "while-while" with speculative
-// traversal): lanes first walk interior nodes until every lane of the
-// wavefront holds a leaf or has nothing left; a lane that finds a leaf early
-// parks it and keeps traversing while others still search. Then all lanes
-// test their leaf triangles together. This keeps the box-test and triangle-test
-// code from being issued, predicated off, on every iteration.
-//
-// The per-lane stack is plain locals: the stack pointer in a VGPR, the first
-// kLdsDepth entries in an LDS column owned by the lane, deeper entries in two
-// private arrays (scratch) that only deep traversals touch.
 template <int W, int STAGE, bool COUNT, int kLdsDepth>
 __global__ void __launch_bounds__(64) k_trace(TraceArgs a) {
     constexpr bool CLOSEST = (STAGE % 2) == 0;
-    constexpr int kSpillDepth = kStackDepth - kLdsDepth;
-    __shared__ int2 lds_stack[kLdsDepth][64];
     const int lane = threadIdx.x;
     const int64_t i = static_cast<int64_t>(blockIdx.x) * 64 + lane;
     const int64_t n = a.count ? static_cast<int64_t>(a.count[0]) * a.count_mul : a.n;
     if (i >= n) return;
     const int64_t ri = a.order ? static_cast<int64_t>(a.order[i]) : i;
-    const WideNode<W> *__restrict__ nodes = reinterpret_cast<const WideNode<W> *>(a.nodes);
     const float4 r0 = reinterpret_cast<const float4 *>(a.rays)[2 * ri];
     const float4 r1 = reinterpret_cast<const float4 *>(a.rays)[2 * ri + 1];
-    const rtd::V3 o = rtd::mk(r0.x, r0.y, r0.z);
-    const rtd::V3 d = rtd::mk(r1.x, r1.y, r1.z);
-    const float tmin = r0.w;
-    const float tmax = r1.w;
-    const rtd::V3 id = rtd::mk(safe_inv(d.x), safe_inv(d.y), safe_inv(d.z));
-    const rtd::V3 oid = rtd::mk(-(o.x * id.x), -(o.y * id.y), -(o.z * id.z));
-    const bool near = fmaxf(fmaxf(fabsf(o.x), fabsf(o.y)), fabsf(o.z)) <= a.near_limit;
-
-    // Two separate arrays so that a spill read never has the type of an LDS
-    // entry read (the compiler would otherwise merge both into a flat load).
-    int32_t spill_code[kSpillDepth];
-    float spill_t[kSpillDepth];
-    int sp = 0;
-    bool overflow = false;
-
-    float best_t = tmax, best_u = 0.0f, best_v = 0.0f;
-    int32_t best_id = -1;
+    float4 hit;
     uint32_t visits = 0, tests = 0;
-
-    // Pop the nearest stacked subtree that can still hold a closer hit.
-    auto pop = [&]() -> int32_t {
-        while (sp > 0) {
-            --sp;
-            int2 e;
-            if (sp < kLdsDepth) {
-                e = lds_stack[sp][lane];
-            } else {
-                e.x = spill_code[sp - kLdsDepth];
-                e.y = __float_as_int(spill_t[sp - kLdsDepth]);
-            }
-            if (!CLOSEST || __int_as_float(e.y) <= best_t * kHitStretch) return e.x;
-        }
-        return kEmpty;
-    };
-    auto push = [&](int32_t code, float t) {
-        if (sp < kLdsDepth) {
-            lds_stack[sp][lane] = make_int2(code, __float_as_int(t));
-            ++sp;
-        } else if (sp < kLdsDepth + kSpillDepth) {
-            spill_code[sp - kLdsDepth] = code;
-            spill_t[sp - kLdsDepth] = t;
-            ++sp;
-        } else {
-            overflow = true;
-        }
-    };
-
-    int32_t cur = a.root;    // interior node, leaf code or kEmpty (done)
-    while (true) {
-        // ---- interior phase
-        int32_t leaf = kEmpty;
-        while (true) {
-            if (cur >= 0) {
-                NodeView<W> v;
-                load_node<W>(nodes, cur, v);
-                if (COUNT) ++visits;
-                float tn[W];
-                if (near)
-                    box_test<W, true>(v, o, id, oid, best_t * kHitStretch, tn);
-                else
-                    box_test<W, false>(v, o, id, oid, best_t * kHitStretch, tn);
-                int32_t cc[W];
-#pragma unroll
-                for (int k = 0; k < W; ++k) cc[k] = v.c[k];
-                if (CLOSEST) sort_children<W>(tn, cc);
-                int32_t next = kEmpty;
-                float next_t = INFINITY;
-#pragma unroll
-                for (int k = W - 1; k >= 0; --k) {
-                    if (tn[k] < INFINITY) {
-                        if (next != kEmpty) push(next, next_t);
-                        next = cc[k];
-                        next_t = tn[k];
-                    }
-                }
-                cur = (next != kEmpty) ? next : pop();
-            }
-            // Park the first leaf found and keep looking (speculation).
-            if (cur != kEmpty && cur < 0 && leaf == kEmpty) {
-                leaf = cur;
-                cur = pop();
-            }
-            // Leave once no lane is still looking for its first leaf.
-            const bool searching = (cur >= 0) && (leaf == kEmpty);
-            if (!__any(searching)) break;
-            // Lanes holding a leaf whose next node is also a leaf wait here.
-        }
-        // ---- leaf phase
-        bool occluded = false;
-        while (leaf != kEmpty) {
-            const int32_t first = leaf_first(leaf);
-            const int32_t count = leaf_count(leaf);
-            for (int k = 0; k < count; ++k) {
-                const LeafTri lt = a.tris[first + k];
-                const int32_t pid = __float_as_int(lt.a_id.w);
-                if (COUNT) ++tests;
-                if (!CLOSEST && (__float_as_int(lt.e1.w) & 3)) continue;  // lights, transparent
-                float t, u, vv;
-                if (rtd::intersect_mt(o, d, rtd::f4v(lt.a_id), rtd::f4v(lt.e1), rtd::f4v(lt.e2), tmin, &t, &u, &vv)) {
-                    if (CLOSEST) {
-                        if (t < best_t || (t == best_t && best_id >= 0 && pid < best_id)) {
-                            best_t = t;
-                            best_u = u;
-                            best_v = vv;
-                            best_id = pid;
-                        }
-                    } else if (t < tmax) {
-                        occluded = true;
-                        break;
-                    }
-                }
-            }
-            leaf = kEmpty;
-            if (!CLOSEST && occluded) break;
-            // A leaf that was reached after the parked one is tested now.
-            if (cur != kEmpty && cur < 0) {
-                leaf = cur;
-                cur = pop();
-            }
-        }
-        if (!CLOSEST && occluded) {
-            best_id = 0;  // marks "occluded" for the store below
-            break;
-        }
-        if (cur == kEmpty) break;
-    }
+    bool overflow = false;
+    trace_ray<W, CLOSEST, COUNT, kLdsDepth>(reinterpret_cast<const WideNode<W> *>(a.nodes), a.tris, a.root,
+                                            a.near_limit, rtd::f4v(r0), rtd::f4v(r1), r0.w, r1.w, lane,
+                                            hit, visits, tests, overflow);
+    const int32_t best_id = __float_as_int(hit.w);
     if (CLOSEST) {
-        const float ht = (best_id >= 0) ? best_t : rtd::kMaxDist;
-        const float hu = (best_id >= 0) ? best_u : 0.0f;
-        const float hv = (best_id >= 0) ? best_v : 0.0f;
+        const float ht = (best_id >= 0) ? hit.x : rtd::kMaxDist;
+        const float hu = (best_id >= 0) ? hit.y : 0.0f;
+        const float hv = (best_id >= 0) ? hit.z : 0.0f;
         reinterpret_cast<float4 *>(a.hits)[ri] = make_float4(ht, hu, hv, __int_as_float(best_id));
     } else {
         a.occ[ri] = (best_id >= 0) ? 1 : 0;

@@ -1,0 +1,273 @@
+// Wide-BVH traversal of one ray per lane, shared by the per-launch kernels
+// (trace.hip) and the persistent batched kernel (batch.hip).
+//
+// Replaces the scalar and 4-wide packet traversals of ssd/bvh.cc:341-759.
+// A lane visits a node by loading the W child boxes (structure of arrays,
+// 16-byte loads), testing them, sorting the hits near-to-far with a sorting
+// network and keeping the nearest as the next node; the others go on a
+// per-lane stack whose first kLdsDepth entries live in LDS (one column per
+// lane, so a wave's accesses hit 64 different banks) and whose remainder
+// spills to private memory. Stack entries carry their entry distance so that
+// closest-hit pops skip subtrees beyond the current hit. Leaves are tested
+// with the reference's Moller-Trumbore (rt_device.hpp); an exact tie in t
+// keeps the lowest emplace id.
+//
+// Loop structure (Aila & Laine 2009, "while-while" with speculative
+// traversal): lanes first walk interior nodes until every lane of the
+// wavefront holds a leaf or has nothing left; a lane that finds a leaf early
+// parks it and keeps traversing while others still search. Then all lanes
+// test their leaf triangles together.
+//
+// The box test is conservative: boxes are padded at build time, the inverse
+// direction never divides by zero, t_far is stretched by 2^-20 and the cull
+// against the current hit by 2^-18, so no box holding an accepted triangle is
+// skipped (the reference's own slack is the absolute +100*EPSILON of
+// ssd/bvh.cc:620,627,705).
+#pragma once
+
+#include "rt_device.hpp"
+#include "rt_internal.hpp"
+
+namespace rt {
+
+constexpr int kStackDepth = 128;  // LDS part + private spill part
+constexpr float kFarStretch = 1.00000095f;   // 1 + 2^-20
+constexpr float kHitStretch = 1.0000038f;    // 1 + 2^-18
+
+__device__ __forceinline__ float safe_inv(float d) {
+    return 1.0f / ((fabsf(d) > 1e-20f) ? d : copysignf(1e-20f, d));
+}
+
+template <int W>
+struct NodeView {
+    float lx[W], hx[W], ly[W], hy[W], lz[W], hz[W];
+    int32_t c[W];
+};
+
+template <int W>
+__device__ __forceinline__ void load_node(const WideNode<W> *__restrict__ nodes, int32_t idx, NodeView<W> &v) {
+    const float4 *p = reinterpret_cast<const float4 *>(nodes + idx);
+    constexpr int Q = W / 4;  // float4 per array
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const float4 a = p[0 * Q + q], b = p[1 * Q + q], c = p[2 * Q + q];
+        const float4 d = p[3 * Q + q], e = p[4 * Q + q], f = p[5 * Q + q];
+        const int4 g = reinterpret_cast<const int4 *>(p)[6 * Q + q];
+        v.lx[4 * q + 0] = a.x; v.lx[4 * q + 1] = a.y; v.lx[4 * q + 2] = a.z; v.lx[4 * q + 3] = a.w;
+        v.hx[4 * q + 0] = b.x; v.hx[4 * q + 1] = b.y; v.hx[4 * q + 2] = b.z; v.hx[4 * q + 3] = b.w;
+        v.ly[4 * q + 0] = c.x; v.ly[4 * q + 1] = c.y; v.ly[4 * q + 2] = c.z; v.ly[4 * q + 3] = c.w;
+        v.hy[4 * q + 0] = d.x; v.hy[4 * q + 1] = d.y; v.hy[4 * q + 2] = d.z; v.hy[4 * q + 3] = d.w;
+        v.lz[4 * q + 0] = e.x; v.lz[4 * q + 1] = e.y; v.lz[4 * q + 2] = e.z; v.lz[4 * q + 3] = e.w;
+        v.hz[4 * q + 0] = f.x; v.hz[4 * q + 1] = f.y; v.hz[4 * q + 2] = f.z; v.hz[4 * q + 3] = f.w;
+        v.c[4 * q + 0] = g.x; v.c[4 * q + 1] = g.y; v.c[4 * q + 2] = g.z; v.c[4 * q + 3] = g.w;
+    }
+}
+
+// Slab test of W children. With FUSED the plane distances are
+// fma(plane, 1/d, -o/d): one operation instead of two, with an extra error of
+// about |o| * 2^-24 in position, which the build's box padding (2^-15 of the
+// largest scene coordinate) covers while |o| stays within 256x the scene;
+// rays from farther away use the subtract-then-multiply form.
+template <int W, bool FUSED>
+__device__ __forceinline__ void box_test(const NodeView<W> &v, rtd::V3 o, rtd::V3 id, rtd::V3 oid, float tlimit,
+                                         float *tn) {
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+        float x0, x1, y0, y1, z0, z1;
+        if (FUSED) {
+            x0 = fmaf(v.lx[k], id.x, oid.x);
+            x1 = fmaf(v.hx[k], id.x, oid.x);
+            y0 = fmaf(v.ly[k], id.y, oid.y);
+            y1 = fmaf(v.hy[k], id.y, oid.y);
+            z0 = fmaf(v.lz[k], id.z, oid.z);
+            z1 = fmaf(v.hz[k], id.z, oid.z);
+        } else {
+            x0 = (v.lx[k] - o.x) * id.x;
+            x1 = (v.hx[k] - o.x) * id.x;
+            y0 = (v.ly[k] - o.y) * id.y;
+            y1 = (v.hy[k] - o.y) * id.y;
+            z0 = (v.lz[k] - o.z) * id.z;
+            z1 = (v.hz[k] - o.z) * id.z;
+        }
+        const float tnear = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.0f));
+        const float tfar = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1));
+        const bool hit = (v.c[k] != kEmpty) && (tnear <= tfar * kFarStretch) && (tnear <= tlimit);
+        tn[k] = hit ? tnear : INFINITY;
+    }
+}
+
+__device__ __forceinline__ void cswap(float &ta, int32_t &ca, float &tb, int32_t &cb) {
+    const bool sw = tb < ta;
+    const float t = sw ? tb : ta;
+    const float u = sw ? ta : tb;
+    const int32_t c = sw ? cb : ca;
+    const int32_t d = sw ? ca : cb;
+    ta = t;
+    tb = u;
+    ca = c;
+    cb = d;
+}
+
+template <int W>
+__device__ __forceinline__ void sort_children(float *t, int32_t *c);
+
+template <>
+__device__ __forceinline__ void sort_children<4>(float *t, int32_t *c) {
+    cswap(t[0], c[0], t[1], c[1]);
+    cswap(t[2], c[2], t[3], c[3]);
+    cswap(t[0], c[0], t[2], c[2]);
+    cswap(t[1], c[1], t[3], c[3]);
+    cswap(t[1], c[1], t[2], c[2]);
+}
+
+template <>
+__device__ __forceinline__ void sort_children<8>(float *t, int32_t *c) {
+    // Batcher's odd-even merge sort network for 8 keys (19 comparators).
+    cswap(t[0], c[0], t[1], c[1]); cswap(t[2], c[2], t[3], c[3]);
+    cswap(t[4], c[4], t[5], c[5]); cswap(t[6], c[6], t[7], c[7]);
+    cswap(t[0], c[0], t[2], c[2]); cswap(t[1], c[1], t[3], c[3]);
+    cswap(t[4], c[4], t[6], c[6]); cswap(t[5], c[5], t[7], c[7]);
+    cswap(t[1], c[1], t[2], c[2]); cswap(t[5], c[5], t[6], c[6]);
+    cswap(t[0], c[0], t[4], c[4]); cswap(t[1], c[1], t[5], c[5]);
+    cswap(t[2], c[2], t[6], c[6]); cswap(t[3], c[3], t[7], c[7]);
+    cswap(t[2], c[2], t[4], c[4]); cswap(t[3], c[3], t[5], c[5]);
+    cswap(t[1], c[1], t[2], c[2]); cswap(t[3], c[3], t[4], c[4]);
+    cswap(t[5], c[5], t[6], c[6]);
+}
+
+// Traversal of one ray (one lane of a 64-thread block). On return hit holds
+// (t, u, v, id) for a closest-hit query (id -1 on a miss) or id >= 0 when an
+// any-hit query found an occluder. The LDS stack is declared here, so its
+// accesses stay LDS instructions after inlining (a pointer parameter loses the
+// address space and lets the compiler fold LDS and scratch reads into flat
+// loads).
+template <int W, bool CLOSEST, bool COUNT, int kLdsDepth>
+__device__ __forceinline__ void trace_ray(const WideNode<W> *__restrict__ nodes, const LeafTri *__restrict__ tris,
+                                          int32_t root, float near_limit, rtd::V3 o, rtd::V3 d, float tmin, float tmax,
+                                          int lane, float4 &hit, uint32_t &visits, uint32_t &tests, bool &overflow) {
+    constexpr int kSpillDepth = kStackDepth - kLdsDepth;
+    __shared__ int2 lds_stack[kLdsDepth][64];
+    const rtd::V3 id = rtd::mk(safe_inv(d.x), safe_inv(d.y), safe_inv(d.z));
+    const rtd::V3 oid = rtd::mk(-(o.x * id.x), -(o.y * id.y), -(o.z * id.z));
+    const bool near = fmaxf(fmaxf(fabsf(o.x), fabsf(o.y)), fabsf(o.z)) <= near_limit;
+
+    // Two separate arrays so that a spill read never has the type of an LDS
+    // entry read (the compiler would otherwise merge both into a flat load).
+    int32_t spill_code[kSpillDepth];
+    float spill_t[kSpillDepth];
+    int sp = 0;
+
+    float best_t = tmax, best_u = 0.0f, best_v = 0.0f;
+    int32_t best_id = -1;
+
+    // Pop the nearest stacked subtree that can still hold a closer hit.
+    auto pop = [&]() -> int32_t {
+        while (sp > 0) {
+            --sp;
+            int2 e;
+            if (sp < kLdsDepth) {
+                e = lds_stack[sp][lane];
+            } else {
+                e.x = spill_code[sp - kLdsDepth];
+                e.y = __float_as_int(spill_t[sp - kLdsDepth]);
+            }
+            if (!CLOSEST || __int_as_float(e.y) <= best_t * kHitStretch) return e.x;
+        }
+        return kEmpty;
+    };
+    auto push = [&](int32_t code, float t) {
+        if (sp < kLdsDepth) {
+            lds_stack[sp][lane] = make_int2(code, __float_as_int(t));
+            ++sp;
+        } else if (sp < kLdsDepth + kSpillDepth) {
+            spill_code[sp - kLdsDepth] = code;
+            spill_t[sp - kLdsDepth] = t;
+            ++sp;
+        } else {
+            overflow = true;
+        }
+    };
+
+    int32_t cur = root;      // interior node, leaf code or kEmpty (done)
+    while (true) {
+        // ---- interior phase
+        int32_t leaf = kEmpty;
+        while (true) {
+            if (cur >= 0) {
+                NodeView<W> v;
+                load_node<W>(nodes, cur, v);
+                if (COUNT) ++visits;
+                float tn[W];
+                if (near)
+                    box_test<W, true>(v, o, id, oid, best_t * kHitStretch, tn);
+                else
+                    box_test<W, false>(v, o, id, oid, best_t * kHitStretch, tn);
+                int32_t cc[W];
+#pragma unroll
+                for (int k = 0; k < W; ++k) cc[k] = v.c[k];
+                if (CLOSEST) sort_children<W>(tn, cc);
+                int32_t next = kEmpty;
+                float next_t = INFINITY;
+#pragma unroll
+                for (int k = W - 1; k >= 0; --k) {
+                    if (tn[k] < INFINITY) {
+                        if (next != kEmpty) push(next, next_t);
+                        next = cc[k];
+                        next_t = tn[k];
+                    }
+                }
+                cur = (next != kEmpty) ? next : pop();
+            }
+            // Park the first leaf found and keep looking (speculation).
+            if (cur != kEmpty && cur < 0 && leaf == kEmpty) {
+                leaf = cur;
+                cur = pop();
+            }
+            // Leave once no lane is still looking for its first leaf.
+            const bool searching = (cur >= 0) && (leaf == kEmpty);
+            if (!__any(searching)) break;
+            // Lanes holding a leaf whose next node is also a leaf wait here.
+        }
+        // ---- leaf phase
+        bool occluded = false;
+        while (leaf != kEmpty) {
+            const int32_t first = leaf_first(leaf);
+            const int32_t count = leaf_count(leaf);
+            for (int k = 0; k < count; ++k) {
+                const LeafTri lt = tris[first + k];
+                const int32_t pid = __float_as_int(lt.a_id.w);
+                if (COUNT) ++tests;
+                if (!CLOSEST && (__float_as_int(lt.e1.w) & 3)) continue;  // lights, transparent
+                float t, u, vv;
+                if (rtd::intersect_mt(o, d, rtd::f4v(lt.a_id), rtd::f4v(lt.e1), rtd::f4v(lt.e2), tmin, &t, &u, &vv)) {
+                    if (CLOSEST) {
+                        if (t < best_t || (t == best_t && best_id >= 0 && pid < best_id)) {
+                            best_t = t;
+                            best_u = u;
+                            best_v = vv;
+                            best_id = pid;
+                        }
+                    } else if (t < tmax) {
+                        occluded = true;
+                        break;
+                    }
+                }
+            }
+            leaf = kEmpty;
+            if (!CLOSEST && occluded) break;
+            // A leaf that was reached after the parked one is tested now.
+            if (cur != kEmpty && cur < 0) {
+                leaf = cur;
+                cur = pop();
+            }
+        }
+        if (!CLOSEST && occluded) {
+            best_id = 0;  // marks "occluded" for the store below
+            break;
+        }
+        if (cur == kEmpty) break;
+    }
+    hit = make_float4(best_t, best_u, best_v, __int_as_float(best_id));
+}
+
+}  // namespace rt

@@ -30,6 +30,7 @@ EXPORTS = [
     "rt_ctx_create", "rt_ctx_destroy", "rt_last_error", "rt_scene_create", "rt_scene_destroy",
     "rt_scene_normals", "rt_accel_build", "rt_accel_destroy", "rt_accel_info_get", "rt_trace_closest",
     "rt_trace_occluded", "rt_render", "rt_set_counting", "rt_gen_random_triangles", "rt_gen_terrain",
+    "rt_batch_create", "rt_batch_run", "rt_batch_destroy", "rt_batch_autotune", "rt_gen_surface_rays",
 ]
 
 
@@ -56,6 +57,14 @@ class rt_stats(C.Structure):
     _fields_ = [("rays", C.c_uint64 * 4), ("ms", C.c_float * 6), ("visits", C.c_uint64 * 8)]
 
 
+class rt_batch_item(C.Structure):
+    _fields_ = [("accel", C.c_void_p), ("rays", C.c_void_p), ("n", C.c_int64), ("hits", C.c_void_p)]
+
+
+class rt_batch_params(C.Structure):
+    _fields_ = [("lds_stack", C.c_int32), ("waves_per_cu", C.c_int32), ("grab", C.c_int32), ("use_graph", C.c_int32)]
+
+
 class rt_accel_info(C.Structure):
     _fields_ = [("nodes", C.c_int64), ("leaves", C.c_int64), ("triangles", C.c_int64), ("width", C.c_int32),
                 ("depth", C.c_int32), ("bytes", C.c_int64), ("build_ms", C.c_float), ("sah_cost", C.c_float)]
@@ -64,6 +73,14 @@ class rt_accel_info(C.Structure):
 def load_library(path=LIB_PATH):
     if not os.path.exists(path):
         raise RuntimeError(f"{path} is missing: build it with `python raptor_amd/build.py` (no fallback exists)")
+    # torch ships its own libamdhip64 under the same soname. Loading torch
+    # first lets this library bind to that runtime, so a process that uses
+    # both holds one HIP runtime; loaded the other way round, two runtimes
+    # compete for the device and torch finds no GPU.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(path)
     vp, i32, i64, u32, u64, fp = C.c_void_p, C.c_int32, C.c_int64, C.c_uint32, C.c_uint64, C.c_void_p
     sig = {
@@ -81,6 +98,12 @@ def load_library(path=LIB_PATH):
         "rt_render": (i32, [vp, C.POINTER(rt_camera), C.POINTER(rt_light), i32, C.POINTER(rt_render_params),
                             fp, fp, fp, C.POINTER(rt_stats)]),
         "rt_set_counting": (i32, [vp, i32]),
+        "rt_batch_create": (i32, [vp, C.POINTER(rt_batch_item), i32, C.POINTER(rt_batch_params), C.POINTER(vp)]),
+        "rt_batch_run": (i32, [vp, i32, C.POINTER(C.c_float)]),
+        "rt_batch_destroy": (None, [vp]),
+        "rt_batch_autotune": (i32, [vp, C.POINTER(rt_batch_item), i32, i32, C.POINTER(rt_batch_params),
+                                    C.POINTER(C.c_float)]),
+        "rt_gen_surface_rays": (i32, [vp, u64, i64, fp]),
         "rt_gen_random_triangles": (i64, [u64, i64, C.c_float, C.c_float, C.c_float, fp, fp]),
         "rt_gen_terrain": (i64, [i32, C.c_float, C.c_float, C.c_float, i32, i32, u64, C.c_float, C.c_float, fp, fp]),
     }
@@ -185,6 +208,56 @@ class Scene:
     def close(self):
         if self.h:
             _LIB.rt_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def surface_rays(scene, seed, n):
+    """n incoherent rays leaving the scene's surfaces, as a (n, 8) float32 device tensor."""
+    import torch
+
+    rays = torch.empty((n, 8), dtype=torch.float32, device="cuda:0")
+    scene.ctx.check(_LIB.rt_gen_surface_rays(scene.h, seed, n, _ptr(rays)), "rt_gen_surface_rays")
+    return rays
+
+
+class Batch:
+    """Many (BVH, rays, hits) items traced by one persistent launch (device tensors)."""
+
+    def __init__(self, ctx, accels, rays, hits, lds_stack=8, waves_per_cu=16, grab=1, use_graph=True):
+        self.ctx = ctx
+        self._keep = (accels, rays, hits)
+        self.items = (rt_batch_item * len(accels))(
+            *[rt_batch_item(a.h.value, _ptr(r), int(r.shape[0]), _ptr(h)) for a, r, h in zip(accels, rays, hits)])
+        self.params = rt_batch_params(lds_stack, waves_per_cu, grab, 1 if use_graph else 0)
+        h = C.c_void_p()
+        ctx.check(_LIB.rt_batch_create(ctx.h, self.items, len(accels), C.byref(self.params), C.byref(h)),
+                  "rt_batch_create")
+        self.h = h
+
+    def run(self, plain=False):
+        ms = C.c_float()
+        self.ctx.check(_LIB.rt_batch_run(self.h, 1 if plain else 0, C.byref(ms)), "rt_batch_run")
+        return ms.value
+
+    @staticmethod
+    def autotune(ctx, accels, rays, hits, reps=3):
+        items = (rt_batch_item * len(accels))(
+            *[rt_batch_item(a.h.value, _ptr(r), int(r.shape[0]), _ptr(h)) for a, r, h in zip(accels, rays, hits)])
+        best = rt_batch_params()
+        ms = C.c_float()
+        ctx.check(_LIB.rt_batch_autotune(ctx.h, items, len(accels), reps, C.byref(best), C.byref(ms)),
+                  "rt_batch_autotune")
+        return {"lds_stack": best.lds_stack, "waves_per_cu": best.waves_per_cu, "grab": best.grab}, ms.value
+
+    def close(self):
+        if self.h:
+            _LIB.rt_batch_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -111,3 +111,32 @@ def terrain(grid=708, x_res=1920, y_res=1080, bounce=True, seed=7):
 
 def terrain1m(x_res=1920, y_res=1080):
     return terrain(708, x_res, y_res)
+
+
+@dataclass
+class BatchScene:
+    kind: str
+    abc: np.ndarray
+    albedo: np.ndarray
+    rays: int
+    seed: int
+
+
+def batched(n_scenes=64, tri_lo=1e3, tri_hi=2e6, side_lo=64, side_hi=1024, seed=3):
+    """Many independent scenes: triangle counts log-uniform in [tri_lo, tri_hi],
+    terrain (grid ceil(sqrt(N/2)) + 1) or random triangles 50/50, ray batches of
+    side x side incoherent surface rays with side log-uniform in [side_lo, side_hi]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for s in range(n_scenes):
+        n = int(np.exp(rng.uniform(np.log(tri_lo), np.log(tri_hi))))
+        side = int(np.exp(rng.uniform(np.log(side_lo), np.log(side_hi))))
+        if rng.uniform() < 0.5:
+            g = int(np.ceil(np.sqrt(n / 2.0))) + 1
+            abc, alb = rt.gen_terrain(g, 0.15, 0.02, 0.5, 8, 57 + s, 100 + s, 0.2, 0.8)
+            kind = "terrain"
+        else:
+            abc, alb = rt.gen_random_triangles(1000 + s, n, 0.01, 0.0, 1.0)
+            kind = "random"
+        out.append(BatchScene(kind, abc, alb, side * side, 5000 + s))
+    return out
