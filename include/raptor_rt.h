@@ -198,6 +198,35 @@ int32_t rt_batch_autotune(rt_ctx *ctx, const rt_batch_item *items, int32_t nitem
  * seed and ray index). rays: n device rt_ray. */
 int32_t rt_gen_surface_rays(rt_scene *scene, uint64_t seed, int64_t n, rt_ray *rays);
 
+/* ---- Differentiable frame (no counterpart in the reference, which renders
+ * forward only: rt/raytracer.cc:20-87, 563-598). ----
+ *
+ * Replaces the scene's per-triangle albedo (n x 3 in [0, 1]); kd becomes
+ * 255 * albedo as at creation. Device pointer with RT_DEVICE_PTRS. */
+int32_t rt_scene_set_albedo(rt_scene *scene, const float *albedo, uint32_t flags);
+
+/* A frame rendered for differentiation. rt_render_tape renders as rt_render
+ * does (rgb: device, y_res x x_res x 3) and keeps what the adjoint needs:
+ * every path record (hits, normals, shadow rays, visibility) when they fit in
+ * max_record_bytes (<= 0: no limit); otherwise only the frame's inputs, and
+ * the adjoint re-traces the frame chunk by chunk with each chunk's records
+ * within max_record_bytes (path replay). The scene's albedo must not change
+ * between the tape and its adjoint. */
+typedef struct rt_tape rt_tape;
+int32_t rt_render_tape(rt_accel *accel, const rt_camera *cam, const rt_light *lights, int32_t nlights,
+                       const rt_render_params *params, float *rgb, int64_t max_record_bytes, rt_tape **tape,
+                       rt_stats *stats);
+/* replay: 1 if the adjoint re-traces; record_bytes: the whole frame's path
+ * records; chunk_tiles: 8x8 tiles per chunk. Any output may be NULL. */
+int32_t rt_tape_info(const rt_tape *tape, int32_t *replay, int64_t *record_bytes, int64_t *chunk_tiles);
+/* Vector-Jacobian product of the taped frame. d_rgb (device, like rgb) is
+ * dL/d(pixel radiance). Adds dL/d(albedo) into d_albedo (device, n x 3) and
+ * dL/d(light) into d_lights (device, nlights x 6: position xyz, then rgb on
+ * the 0-255 scale). Hit points and visibility are held at the forward's.
+ * ms: device time of the adjoint (re-tracing included). */
+int32_t rt_render_adjoint(rt_tape *tape, const float *d_rgb, float *d_albedo, float *d_lights, float *ms);
+void rt_tape_destroy(rt_tape *tape);
+
 /* Count node visits and triangle tests in later calls (0/1). */
 int32_t rt_set_counting(rt_ctx *ctx, int32_t on);
 

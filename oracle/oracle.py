@@ -33,7 +33,9 @@ class oracle_render_params(C.Structure):
 class oracle_render_out(C.Structure):
     _fields_ = [("rgb", C.c_void_p), ("prim", C.c_void_p), ("tuv", C.c_void_p), ("vis", C.c_void_p),
                 ("bounce_prim", C.c_void_p), ("bounce_tuv", C.c_void_p), ("bounce_vis", C.c_void_p),
-                ("counters", C.c_void_p)]
+                ("counters", C.c_void_p), ("rec_prim1", C.c_void_p), ("rec_prim2", C.c_void_p),
+                ("rec_o1", C.c_void_p), ("rec_n1", C.c_void_p), ("rec_o2", C.c_void_p), ("rec_n2", C.c_void_p),
+                ("rec_vis1", C.c_void_p), ("rec_vis2", C.c_void_p)]
 
 
 def build():
@@ -146,7 +148,11 @@ class OracleScene:
         LIB.oracle_trace_occluded(self.h, _p(rays), n, _p(occ), 1 if use_bvh else 0, nthreads, _p(cnt))
         return occ, cnt
 
-    def render(self, albedo, camera, lights, bounce=False, spp=1, seed=0, pixel_stride=1, nthreads=8, packet=False):
+    def render(self, albedo, camera, lights, bounce=False, spp=1, seed=0, pixel_stride=1, nthreads=8, packet=False,
+               records=False):
+        """One frame. records=True adds per-sample path records (index
+        p * spp + s): prim1/prim2 (-1: none), o1/o2 (shadow-ray origins),
+        n1/n2 (shading normals), vis1/vis2 (per light, 1 = lit)."""
         npix = camera.x_res * camera.y_res
         nl = len(lights)
         rgb = np.full((npix, 3), np.nan, np.float32)
@@ -158,14 +164,28 @@ class OracleScene:
         bvis = np.full((npix, max(1, nl)), 255, np.uint8)
         cnt = np.zeros(6, np.uint64)
         out = oracle_render_out(_p(rgb), _p(prim), _p(tuv), _p(vis), _p(bprim), _p(btuv), _p(bvis), _p(cnt))
+        rec = None
+        if records:
+            ns = npix * max(1, spp)
+            rec = dict(prim1=np.full(ns, -2, np.int32), prim2=np.full(ns, -2, np.int32),
+                       o1=np.zeros((ns, 3), np.float32), n1=np.zeros((ns, 3), np.float32),
+                       o2=np.zeros((ns, 3), np.float32), n2=np.zeros((ns, 3), np.float32),
+                       vis1=np.zeros((ns, max(1, nl)), np.uint8), vis2=np.zeros((ns, max(1, nl)), np.uint8))
+            out.rec_prim1, out.rec_prim2 = _p(rec["prim1"]), _p(rec["prim2"])
+            out.rec_o1, out.rec_n1 = _p(rec["o1"]), _p(rec["n1"])
+            out.rec_o2, out.rec_n2 = _p(rec["o2"]), _p(rec["n2"])
+            out.rec_vis1, out.rec_vis2 = _p(rec["vis1"]), _p(rec["vis2"])
         p = oracle_render_params(1 if bounce else 0, spp, seed, pixel_stride, nthreads, 1 if packet else 0)
         alb = np.ascontiguousarray(albedo, np.float32)
         rc = LIB.oracle_render(self.h, _p(alb), C.byref(camera_c(camera)), lights_c(lights), nl, C.byref(p),
                                C.byref(out))
         if rc != 0:
             raise RuntimeError(f"oracle_render failed ({rc})")
-        return dict(rgb=rgb, prim=prim, tuv=tuv, vis=vis, bounce_prim=bprim, bounce_tuv=btuv, bounce_vis=bvis,
-                    counters=cnt)
+        res = dict(rgb=rgb, prim=prim, tuv=tuv, vis=vis, bounce_prim=bprim, bounce_tuv=btuv, bounce_vis=bvis,
+                   counters=cnt)
+        if rec is not None:
+            res["records"] = rec
+        return res
 
     def close(self):
         if self.h:

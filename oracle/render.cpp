@@ -152,7 +152,7 @@ struct LocalCounters {
 // Direct light at a surface hit: one shadow ray per light, visibility as the
 // fraction of unoccluded shadow rays (rt/raytracer.cc:44-64).
 Colour direct_light(const FrameCtx &f, const SurfaceHit &sh, const Colour &kd, uint8_t *vis_out, int ray_kind,
-                    LocalCounters *lc) {
+                    LocalCounters *lc, uint8_t *vis_rec) {
     Ray illum[kMaxLights];
     for (size_t l = 0; l < f.lights.size(); ++l) {
         illum[l] = shadow_ray(sh, f.lights[l].c);
@@ -161,8 +161,29 @@ Colour direct_light(const FrameCtx &f, const SurfaceHit &sh, const Colour &kd, u
         const float made_it = occluded ? 0.0f : 1.0f;
         illum[l].magn = made_it / 1.0f;
         if (vis_out) vis_out[l] = occluded ? 0 : 1;
+        if (vis_rec) vis_rec[l] = occluded ? 0 : 1;
     }
     return shade_direct(kd, sh, illum, f.lights);
+}
+
+// Path record of one surface of one sample.
+void record_surface(int32_t *prim_out, float *o_out, float *n_out, uint8_t *vis_out, int64_t k, int nl, int prim,
+                    const SurfaceHit *sh) {
+    if (prim_out) prim_out[k] = prim;
+    const V3 o = sh ? offset_start_point(sh->dst, sh->geo_norm, 1) : V3(0.0f, 0.0f, 0.0f);
+    const V3 n = sh ? sh->shade_norm : V3(0.0f, 0.0f, 0.0f);
+    if (o_out) {
+        o_out[3 * k + 0] = o.x;
+        o_out[3 * k + 1] = o.y;
+        o_out[3 * k + 2] = o.z;
+    }
+    if (n_out) {
+        n_out[3 * k + 0] = n.x;
+        n_out[3 * k + 1] = n.y;
+        n_out[3 * k + 2] = n.z;
+    }
+    if (vis_out && !sh)
+        for (int l = 0; l < nl; ++l) vis_out[nl * k + l] = 0;
 }
 
 void render_pixel(const FrameCtx &f, int px, int py, LocalCounters *lc) {
@@ -170,8 +191,10 @@ void render_pixel(const FrameCtx &f, int px, int py, LocalCounters *lc) {
     const int64_t p = static_cast<int64_t>(py) * W + px;
     const int nl = static_cast<int>(f.lights.size());
     const int spp = std::max(1, f.p.spp);
+    const oracle_render_out &o = f.out;
     Colour acc;
     for (int s = 0; s < spp; ++s) {
+        const int64_t k = p * spp + s;
         float xf = static_cast<float>(px), yf = static_cast<float>(py);
         if (spp > 1) {
             float jx, jy;
@@ -192,11 +215,15 @@ void render_pixel(const FrameCtx &f, int px, int py, LocalCounters *lc) {
         }
         if (first && f.out.bounce_prim) f.out.bounce_prim[p] = -1;
         Colour c;  // background (0, 0, 0)
+        record_surface(o.rec_prim1, o.rec_o1, o.rec_n1, o.rec_vis1, k, nl, -1, nullptr);
+        record_surface(o.rec_prim2, o.rec_o2, o.rec_n2, o.rec_vis2, k, nl, -1, nullptr);
         if (h.d < kMaxDist) {
             const Triangle &tri = f.s->tris[prim];
             const SurfaceHit sh = surface_at(tri, &r, h.d);
             const Colour kd1 = kd_of(f, prim);
-            c = direct_light(f, sh, kd1, (first && f.out.vis) ? f.out.vis + nl * p : nullptr, 1, lc);
+            record_surface(o.rec_prim1, o.rec_o1, o.rec_n1, nullptr, k, nl, prim, &sh);
+            c = direct_light(f, sh, kd1, (first && f.out.vis) ? f.out.vis + nl * p : nullptr, 1, lc,
+                             o.rec_vis1 ? o.rec_vis1 + nl * k : nullptr);
             if (f.p.bounce) {
                 Ray br;
                 br.ogn = offset_start_point(sh.dst, sh.geo_norm, 1);
@@ -214,9 +241,10 @@ void render_pixel(const FrameCtx &f, int px, int py, LocalCounters *lc) {
                 if (bh.d < kMaxDist) {
                     const Triangle &btri = f.s->tris[bprim];
                     const SurfaceHit bsh = surface_at(btri, &br, bh.d);
+                    record_surface(o.rec_prim2, o.rec_o2, o.rec_n2, nullptr, k, nl, bprim, &bsh);
                     const Colour c2 = direct_light(f, bsh, kd_of(f, bprim),
                                                    (first && f.out.bounce_vis) ? f.out.bounce_vis + nl * p : nullptr,
-                                                   3, lc);
+                                                   3, lc, o.rec_vis2 ? o.rec_vis2 + nl * k : nullptr);
                     c.r = c.r + ((kd1.r / 255.0f) * c2.r);
                     c.g = c.g + ((kd1.g / 255.0f) * c2.g);
                     c.b = c.b + ((kd1.b / 255.0f) * c2.b);

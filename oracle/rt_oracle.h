@@ -55,6 +55,13 @@ typedef struct {
     /* [0] primary, [1] shadow, [2] bounce, [3] bounce-shadow rays,
      * [4] child box tests, [5] triangle tests */
     uint64_t *counters;
+    /* Per-sample path records, index p * spp + s; any may be NULL. prim -1:
+     * no surface. o: the shadow-ray origin (offset hit point), n: the shading
+     * normal, vis: one byte per light (1 = lit). Level 1 is the primary hit,
+     * level 2 the bounce hit. */
+    int32_t *rec_prim1, *rec_prim2;
+    float *rec_o1, *rec_n1, *rec_o2, *rec_n2;
+    uint8_t *rec_vis1, *rec_vis2;
 } oracle_render_out;
 
 oracle_scene *oracle_scene_create(const float *abc, int64_t n, const uint8_t *flags);

@@ -20,45 +20,20 @@
 // no host round trip until its statistics are read back.
 #include <algorithm>
 
+#include "frame.hpp"
 #include "radix_sort.hpp"
 #include "rt_device.hpp"
-#include "rt_internal.hpp"
 
 namespace rt {
 
 namespace {
 
-struct CamArgs {
-    float px, py, pz;
-    float xx, xy, xz, yx, yy, yz, zx, zy, zz;
-    float x_m, y_m, x_inc, y_inc, t;
-    int32_t x_res, y_res;
-};
-
-// Samples are laid out in 8x8-pixel tiles (the reference's packet shape,
-// rt/raytracer.cc:464-483), so the 64 lanes of a wavefront trace one tile:
-// sample i -> slot i / spp -> tile slot / 64, pixel (slot % 64) within it.
-struct TileMap {
-    int32_t x_res, y_res, tiles_x;
-    __device__ __forceinline__ bool pixel_of_slot(int64_t slot, int *x, int *y) const {
-        const int64_t tile = slot >> 6;
-        const int local = static_cast<int>(slot & 63);
-        *x = static_cast<int>(tile % tiles_x) * 8 + (local & 7);
-        *y = static_cast<int>(tile / tiles_x) * 8 + (local >> 3);
-        return *x < x_res && *y < y_res;
-    }
-    __device__ __forceinline__ int64_t slot_of_pixel(int x, int y) const {
-        const int64_t tile = static_cast<int64_t>(y >> 3) * tiles_x + (x >> 3);
-        return (tile << 6) | ((y & 7) << 3) | (x & 7);
-    }
-};
-
-__global__ void __launch_bounds__(256) k_raygen(CamArgs c, TileMap tm, int spp, uint64_t seed, int64_t nsamples,
-                                                rt_ray *__restrict__ rays) {
+__global__ void __launch_bounds__(256) k_raygen(CamArgs c, TileMap tm, int spp, uint64_t seed, int64_t slot0,
+                                                int64_t nsamples, rt_ray *__restrict__ rays) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= nsamples) return;
-    const int64_t slot = i / spp;
-    const int s = static_cast<int>(i - slot * spp);
+    const int64_t slot = slot0 + i / spp;
+    const int s = static_cast<int>(i % spp);
     int x, y;
     float4 *r = reinterpret_cast<float4 *>(rays) + 2 * i;
     if (!tm.pixel_of_slot(slot, &x, &y)) {
@@ -84,12 +59,6 @@ __global__ void __launch_bounds__(256) k_raygen(CamArgs c, TileMap tm, int spp, 
     r[1] = make_float4(d.x, d.y, d.z, rtd::kMaxDist);
 }
 
-struct LightArgs {
-    float4 pos_d[8];   // position, falloff d
-    float4 rgb[8];     // rgb / 255
-    int n;
-};
-
 struct ShadeArgs {
     TileMap tm;
     const rt_ray *rays;      // rays that were traced
@@ -101,6 +70,7 @@ struct ShadeArgs {
     int bounce;
     int spp;
     uint64_t seed;
+    int64_t slot0;              // first slot of the chunk
     const int32_t *sample_of;   // for bounce hits: the sample index of each bounce ray
     int32_t *qidx;              // per ray: compacted hit index or -1
     float4 *nshade;             // per compacted hit: shading normal
@@ -231,8 +201,8 @@ __global__ void __launch_bounds__(256) k_shade_hit(ShadeArgs a) {
         sr[1] = make_float4(dn.x, dn.y, dn.z, len);
     }
     if (STAGE == 0 && a.bounce) {
-        const int64_t slot = i / a.spp;
-        const int s = static_cast<int>(i - slot * a.spp);
+        const int64_t slot = a.slot0 + i / a.spp;
+        const int s = static_cast<int>(i % a.spp);
         int x, y;
         a.tm.pixel_of_slot(slot, &x, &y);
         const int64_t p = static_cast<int64_t>(y) * a.tm.x_res + x;
@@ -276,7 +246,7 @@ __global__ void __launch_bounds__(256) k_bounce_keys(const rt_ray *__restrict__ 
 
 struct ComposeArgs {
     TileMap tm;
-    int64_t npix;
+    int64_t slot0, nslots;
     int spp;
     int bounce;
     const rt_hit *hits1;
@@ -323,9 +293,11 @@ __device__ __forceinline__ Col direct(const rt_ray *shadow, const uint8_t *occ, 
 }
 
 __global__ void __launch_bounds__(256) k_compose(ComposeArgs a) {
-    const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (p >= a.npix) return;
-    const int64_t slot = a.tm.slot_of_pixel(static_cast<int>(p % a.tm.x_res), static_cast<int>(p / a.tm.x_res));
+    const int64_t slot = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;  // in the chunk
+    if (slot >= a.nslots) return;
+    int x, y;
+    if (!a.tm.pixel_of_slot(a.slot0 + slot, &x, &y)) return;
+    const int64_t p = static_cast<int64_t>(y) * a.tm.x_res + x;
     Col acc{0.0f, 0.0f, 0.0f};
     for (int s = 0; s < a.spp; ++s) {
         const int64_t i = slot * a.spp + s;
@@ -372,28 +344,6 @@ __global__ void __launch_bounds__(256) k_hits_out(TileMap tm, int spp, int64_t n
     reinterpret_cast<float4 *>(out)[j] = v;
 }
 
-__global__ void k_zero_u32(uint32_t *p, int n) {
-    if (threadIdx.x < n) p[threadIdx.x] = 0u;
-}
-
-// Frame workspace, kept on the accel between frames.
-struct Workspace {
-    int64_t nsamples = 0;
-    int nlights = 0;
-    void *mem = nullptr;
-    size_t bytes = 0;
-    rt_ray *rays1, *shadow1, *bounce, *shadow2;
-    rt_hit *hits1, *hits2;
-    int32_t *qidx1, *qidx2, *hit_sample1, *hit_sample2;
-    float4 *nshade1, *nshade2;
-    uint8_t *occ1, *occ2;
-    float *rgb;
-    rt_hit *bounce_out;
-    uint32_t *counts;  // unused
-    uint32_t *blocks1, *blocks2;  // per-block hit counts, then prefixes; total at [nblocks]
-    uint32_t *skeys, *svals, *skeys_t, *svals_t, *shist, *ssums;  // bounce regrouping sort
-};
-
 template <class T>
 T *carve(char *&p, int64_t count) {
     T *r = reinterpret_cast<T *>(p);
@@ -401,7 +351,9 @@ T *carve(char *&p, int64_t count) {
     return r;
 }
 
-size_t layout(Workspace &w, char *base, int64_t ns, int nl) {
+}  // namespace
+
+size_t workspace_layout(Workspace &w, char *base, int64_t ns, int nl) {
     char *p = base;
     w.rays1 = carve<rt_ray>(p, ns);
     w.hits1 = carve<rt_hit>(p, ns);
@@ -419,7 +371,6 @@ size_t layout(Workspace &w, char *base, int64_t ns, int nl) {
     w.occ2 = carve<uint8_t>(p, ns * nl);
     w.rgb = carve<float>(p, 3 * ns);
     w.bounce_out = carve<rt_hit>(p, ns);
-    w.counts = carve<uint32_t>(p, 4);
     const int64_t nb = (ns + 255) / 256 + 1;
     w.blocks1 = carve<uint32_t>(p, nb);
     w.blocks2 = carve<uint32_t>(p, nb);
@@ -429,63 +380,56 @@ size_t layout(Workspace &w, char *base, int64_t ns, int nl) {
     w.svals_t = carve<uint32_t>(p, ns);
     w.shist = carve<uint32_t>(p, 256 * ((ns + kSortTile - 1) / kSortTile));
     w.ssums = carve<uint32_t>(p, kScanTile);
-    return static_cast<size_t>(p - base);
+    return static_cast<size_t>(p - base) + 256;
 }
 
-}  // namespace
-
-static Workspace *get_workspace(rt_accel *a, int64_t ns, int nl, int32_t *err) {
-    Workspace *w = reinterpret_cast<Workspace *>(a->frame_ws);
-    if (!w) {
-        w = new Workspace();
-        a->frame_ws = w;
-    }
+int32_t workspace_reserve(rt_ctx *ctx, Workspace *w, int64_t ns, int nl) {
     if (w->nsamples >= ns && w->nlights >= nl && w->mem) {
-        layout(*w, reinterpret_cast<char *>(w->mem), w->nsamples, w->nlights);
-        return w;
+        workspace_layout(*w, reinterpret_cast<char *>(w->mem), w->nsamples, w->nlights);
+        return RT_OK;
     }
     if (w->mem) (void)hipFree(w->mem);
     w->mem = nullptr;
+    w->nsamples = 0;
     Workspace probe;
-    const size_t bytes = layout(probe, reinterpret_cast<char *>(0), ns, nl) + 256;
+    const size_t bytes = workspace_layout(probe, reinterpret_cast<char *>(0), ns, nl);
     hipError_t e = hipMalloc(&w->mem, bytes);
     if (e != hipSuccess) {
-        *err = fail(a->scene->ctx, e == hipErrorOutOfMemory ? RT_ERR_OOM : RT_ERR_HIP,
-                    std::string("frame workspace: ") + hipGetErrorString(e));
         w->mem = nullptr;
-        w->nsamples = 0;
-        return nullptr;
+        return fail(ctx, e == hipErrorOutOfMemory ? RT_ERR_OOM : RT_ERR_HIP,
+                    std::string("frame workspace: ") + hipGetErrorString(e));
     }
     w->bytes = bytes;
     w->nsamples = ns;
     w->nlights = nl;
-    layout(*w, reinterpret_cast<char *>(w->mem), ns, nl);
-    return w;
+    workspace_layout(*w, reinterpret_cast<char *>(w->mem), ns, nl);
+    return RT_OK;
 }
 
-int32_t render_frame(rt_accel *a, const rt_camera &cam, const rt_light *lights, int nlights,
-                     const rt_render_params &p, float *rgb, rt_hit *primary, rt_hit *bounce_hits,
-                     rt_stats *stats) {
-    rt_ctx *ctx = a->scene->ctx;
-    hipStream_t s = ctx->stream;
-    if (nlights < 0 || nlights > 8) return fail(ctx, RT_ERR_INVALID, "nlights must be 0..8");
-    if (cam.x_res <= 0 || cam.y_res <= 0) return fail(ctx, RT_ERR_INVALID, "camera resolution must be positive");
-    const int spp = std::max(1, p.spp);
-    const int64_t npix = static_cast<int64_t>(cam.x_res) * cam.y_res;
-    TileMap tm;
-    tm.x_res = cam.x_res;
-    tm.y_res = cam.y_res;
-    tm.tiles_x = (cam.x_res + 7) / 8;
-    const int64_t tiles_y = (cam.y_res + 7) / 8;
-    const int64_t ns = static_cast<int64_t>(tm.tiles_x) * tiles_y * 64 * spp;  // samples incl. tile padding
-    if (ns >= (int64_t(1) << 31)) return fail(ctx, RT_ERR_INVALID, "too many samples per frame");
-    const int nblocks = static_cast<int>((ns + 255) / 256);
-    if (nblocks > 1024 * kScanPer) return fail(ctx, RT_ERR_INVALID, "frame too large for the hit compaction");
-    int32_t err = RT_OK;
-    Workspace *w = get_workspace(a, ns, std::max(1, nlights), &err);
-    if (!w) return err;
+void workspace_release(Workspace *w) {
+    if (!w) return;
+    if (w->mem) (void)hipFree(w->mem);
+    delete w;
+}
 
-    CamArgs c;
+int32_t plan_frame(rt_accel *a, const rt_camera &cam, const rt_light *lights, int nlights, const rt_render_params &p,
+                   FramePlan *plan) {
+    rt_ctx *ctx = a->scene->ctx;
+    if (nlights < 0 || nlights > kMaxLights) return fail(ctx, RT_ERR_INVALID, "nlights must be 0..8");
+    if (nlights > 0 && !lights) return fail(ctx, RT_ERR_INVALID, "null light array");
+    if (cam.x_res <= 0 || cam.y_res <= 0) return fail(ctx, RT_ERR_INVALID, "camera resolution must be positive");
+    FramePlan &f = *plan;
+    f.a = a;
+    f.p = p;
+    f.spp = std::max(1, p.spp);
+    f.nlights = nlights;
+    f.npix = static_cast<int64_t>(cam.x_res) * cam.y_res;
+    f.tm.x_res = cam.x_res;
+    f.tm.y_res = cam.y_res;
+    f.tm.tiles_x = (cam.x_res + 7) / 8;
+    f.tiles = static_cast<int64_t>(f.tm.tiles_x) * ((cam.y_res + 7) / 8);
+    f.chunk_tiles = f.tiles;
+    CamArgs &c = f.cam;
     c.px = cam.pos[0]; c.py = cam.pos[1]; c.pz = cam.pos[2];
     c.xx = cam.x_axis[0]; c.xy = cam.x_axis[1]; c.xz = cam.x_axis[2];
     c.yx = cam.y_axis[0]; c.yy = cam.y_axis[1]; c.yz = cam.y_axis[2];
@@ -498,24 +442,57 @@ int32_t render_frame(rt_accel *a, const rt_camera &cam, const rt_light *lights, 
     c.t = cam.t;
     c.x_res = cam.x_res;
     c.y_res = cam.y_res;
-    LightArgs L;
-    L.n = nlights;
+    f.lights.n = nlights;
     for (int l = 0; l < nlights; ++l) {
-        L.pos_d[l] = make_float4(lights[l].pos[0], lights[l].pos[1], lights[l].pos[2], lights[l].d);
-        L.rgb[l] = make_float4(lights[l].rgb[0] / 255.0f, lights[l].rgb[1] / 255.0f, lights[l].rgb[2] / 255.0f, 0.0f);
+        f.lights.pos_d[l] = make_float4(lights[l].pos[0], lights[l].pos[1], lights[l].pos[2], lights[l].d);
+        f.lights.rgb[l] =
+            make_float4(lights[l].rgb[0] / 255.0f, lights[l].rgb[1] / 255.0f, lights[l].rgb[2] / 255.0f, 0.0f);
     }
-    const bool timed = stats != nullptr;
+    return RT_OK;
+}
+
+int64_t chunk_bytes(const FramePlan &plan, int64_t ntiles) {
+    Workspace probe;
+    return static_cast<int64_t>(
+        workspace_layout(probe, reinterpret_cast<char *>(0), ntiles * 64 * plan.spp, std::max(1, plan.nlights)));
+}
+
+int64_t tiles_within(const FramePlan &plan, int64_t budget) {
+    int64_t lo = 0, hi = plan.tiles;  // largest k in [0, tiles] with chunk_bytes(k) <= budget
+    if (chunk_bytes(plan, 1) > budget) return 0;
+    lo = 1;
+    while (lo < hi) {
+        const int64_t mid = lo + (hi - lo + 1) / 2;
+        if (chunk_bytes(plan, mid) <= budget)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+static dim3 grid256(int64_t n) { return dim3(static_cast<unsigned>((n + 255) / 256)); }
+
+int32_t trace_chunk(const FramePlan &f, Workspace *w, int64_t tile0, int64_t ntiles, bool timed) {
+    rt_accel *a = f.a;
+    rt_ctx *ctx = a->scene->ctx;
+    hipStream_t s = ctx->stream;
+    const int spp = f.spp;
+    const int nlights = f.nlights;
+    const int64_t ns = ntiles * 64 * spp;  // samples of the chunk, tile padding included
+    if (ns >= (int64_t(1) << 31)) return fail(ctx, RT_ERR_INVALID, "too many samples per chunk");
+    const int nblocks = static_cast<int>((ns + 255) / 256);
+    if (nblocks > 1024 * kScanPer) return fail(ctx, RT_ERR_INVALID, "chunk too large for the hit compaction");
     hipEvent_t *ev = ctx->ev;
     auto rec = [&](int k) -> int32_t {
         if (timed) RT_HIP_CHECK(ctx, hipEventRecord(ev[k], s));
         return RT_OK;
     };
     int32_t rc;
-    const int nl1 = std::max(1, nlights);
-    auto grid = [](int64_t n) { return dim3(static_cast<unsigned>((n + 255) / 256)); };
+    const int64_t slot0 = tile0 * 64;
 
     if ((rc = rec(0))) return rc;
-    hipLaunchKernelGGL(k_raygen, grid(ns), dim3(256), 0, s, c, tm, spp, p.seed, ns, w->rays1);
+    hipLaunchKernelGGL(k_raygen, grid256(ns), dim3(256), 0, s, f.cam, f.tm, spp, f.p.seed, slot0, ns, w->rays1);
     if ((rc = rec(1))) return rc;
     ctx->count_slot = 0;
     if ((rc = launch_closest(a, w->rays1, ns, w->hits1, s, 0))) return rc;
@@ -523,17 +500,18 @@ int32_t render_frame(rt_accel *a, const rt_camera &cam, const rt_light *lights, 
     hipLaunchKernelGGL(k_count_hits, dim3(nblocks), dim3(256), 0, s, w->hits1, nullptr, ns, w->blocks1);
     hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, s, w->blocks1, nblocks);
     ShadeArgs sa{};
-    sa.tm = tm;
+    sa.tm = f.tm;
     sa.rays = w->rays1;
     sa.hits = w->hits1;
     sa.count = nullptr;
     sa.n_static = ns;
     sa.tri_n = a->scene->d_n;
     sa.tri_c = a->scene->d_c;
-    sa.lights = L;
-    sa.bounce = p.bounce;
+    sa.lights = f.lights;
+    sa.bounce = f.p.bounce;
     sa.spp = spp;
-    sa.seed = p.seed;
+    sa.seed = f.p.seed;
+    sa.slot0 = slot0;
     sa.sample_of = nullptr;
     sa.qidx = w->qidx1;
     sa.nshade = w->nshade1;
@@ -541,20 +519,21 @@ int32_t render_frame(rt_accel *a, const rt_camera &cam, const rt_light *lights, 
     sa.shadow_q = w->shadow1;
     sa.bounce_q = w->bounce;
     sa.block_base = w->blocks1;
-    hipLaunchKernelGGL(k_shade_hit<0>, grid(ns), dim3(256), 0, s, sa);
+    hipLaunchKernelGGL(k_shade_hit<0>, grid256(ns), dim3(256), 0, s, sa);
     RT_HIP_CHECK(ctx, hipGetLastError());
     if ((rc = rec(3))) return rc;
     ctx->count_slot = 1;
-    if (nlights > 0 && (rc = launch_occluded_q(a, w->shadow1, w->blocks1 + nblocks, nlights, ns * nlights, w->occ1, s, 1)))
+    if (nlights > 0 &&
+        (rc = launch_occluded_q(a, w->shadow1, w->blocks1 + nblocks, nlights, ns * nlights, w->occ1, s, 1)))
         return rc;
     if ((rc = rec(4))) return rc;
-    if (p.bounce) {
+    if (f.p.bounce) {
         const uint32_t *order = nullptr;
-        if (p.flags & RT_REGROUP_BOUNCE) {
+        if (f.p.flags & RT_REGROUP_BOUNCE) {
             const float4 lo = make_float4(a->scene_lo[0], a->scene_lo[1], a->scene_lo[2], 0.0f);
             const float4 inv = make_float4(a->scene_inv[0], a->scene_inv[1], a->scene_inv[2], 0.0f);
-            hipLaunchKernelGGL(k_bounce_keys, grid(ns), dim3(256), 0, s, w->bounce, w->blocks1 + nblocks, ns, lo, inv,
-                               w->skeys, w->svals);
+            hipLaunchKernelGGL(k_bounce_keys, grid256(ns), dim3(256), 0, s, w->bounce, w->blocks1 + nblocks, ns, lo,
+                               inv, w->skeys, w->svals);
             RT_HIP_CHECK(ctx, radix_sort_pairs(w->skeys, w->svals, w->skeys_t, w->svals_t, ns, 30, w->shist,
                                                w->ssums, s));
             order = w->svals;
@@ -566,17 +545,18 @@ int32_t render_frame(rt_accel *a, const rt_camera &cam, const rt_light *lights, 
                            w->blocks2);
         hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, s, w->blocks2, nblocks);
         ShadeArgs sb{};
-        sb.tm = tm;
+        sb.tm = f.tm;
         sb.rays = w->bounce;
         sb.hits = w->hits2;
         sb.count = w->blocks1 + nblocks;
         sb.n_static = ns;
         sb.tri_n = a->scene->d_n;
         sb.tri_c = a->scene->d_c;
-        sb.lights = L;
+        sb.lights = f.lights;
         sb.bounce = 0;
         sb.spp = spp;
-        sb.seed = p.seed;
+        sb.seed = f.p.seed;
+        sb.slot0 = slot0;
         sb.sample_of = w->hit_sample1;
         sb.qidx = w->qidx2;
         sb.nshade = w->nshade2;
@@ -584,11 +564,12 @@ int32_t render_frame(rt_accel *a, const rt_camera &cam, const rt_light *lights, 
         sb.shadow_q = w->shadow2;
         sb.bounce_q = nullptr;
         sb.block_base = w->blocks2;
-        hipLaunchKernelGGL(k_shade_hit<1>, grid(ns), dim3(256), 0, s, sb);
+        hipLaunchKernelGGL(k_shade_hit<1>, grid256(ns), dim3(256), 0, s, sb);
         RT_HIP_CHECK(ctx, hipGetLastError());
         if ((rc = rec(6))) return rc;
         ctx->count_slot = 3;
-        if (nlights > 0 && (rc = launch_occluded_q(a, w->shadow2, w->blocks2 + nblocks, nlights, ns * nlights, w->occ2, s, 3)))
+        if (nlights > 0 &&
+            (rc = launch_occluded_q(a, w->shadow2, w->blocks2 + nblocks, nlights, ns * nlights, w->occ2, s, 3)))
             return rc;
         if ((rc = rec(7))) return rc;
     } else {
@@ -596,11 +577,32 @@ int32_t render_frame(rt_accel *a, const rt_camera &cam, const rt_light *lights, 
         if ((rc = rec(6))) return rc;
         if ((rc = rec(7))) return rc;
     }
+    return RT_OK;
+}
+
+int32_t collect_chunk(const FramePlan &f, Workspace *w, int64_t ntiles, ChunkStats *stats) {
+    rt_ctx *ctx = f.a->scene->ctx;
+    hipStream_t s = ctx->stream;
+    const int nblocks = static_cast<int>((ntiles * 64 * f.spp + 255) / 256);
+    uint32_t counts[2] = {0, 0};
+    RT_HIP_CHECK(ctx, hipMemcpyAsync(counts, w->blocks1 + nblocks, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    if (f.p.bounce)
+        RT_HIP_CHECK(ctx, hipMemcpyAsync(counts + 1, w->blocks2 + nblocks, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    RT_HIP_CHECK(ctx, hipStreamSynchronize(s));
+    stats->hits1 = counts[0];
+    stats->hits2 = counts[1];
+    for (int k = 0; k < 7; ++k) RT_HIP_CHECK(ctx, hipEventElapsedTime(&stats->ms[k], ctx->ev[k], ctx->ev[k + 1]));
+    return RT_OK;
+}
+
+int32_t compose_chunk(const FramePlan &f, Workspace *w, int64_t tile0, int64_t ntiles, float *rgb) {
+    rt_ctx *ctx = f.a->scene->ctx;
     ComposeArgs ca{};
-    ca.tm = tm;
-    ca.npix = npix;
-    ca.spp = spp;
-    ca.bounce = p.bounce;
+    ca.tm = f.tm;
+    ca.slot0 = tile0 * 64;
+    ca.nslots = ntiles * 64;
+    ca.spp = f.spp;
+    ca.bounce = f.p.bounce;
     ca.hits1 = w->hits1;
     ca.qidx1 = w->qidx1;
     ca.nshade1 = w->nshade1;
@@ -611,74 +613,79 @@ int32_t render_frame(rt_accel *a, const rt_camera &cam, const rt_light *lights, 
     ca.nshade2 = w->nshade2;
     ca.shadow2 = w->shadow2;
     ca.occ2 = w->occ2;
-    ca.kd = a->scene->d_kd;
-    ca.lights = L;
-    const bool dev_out = (p.flags & RT_DEVICE_PTRS) != 0;
-    ca.rgb = (dev_out && rgb) ? rgb : w->rgb;
-    hipLaunchKernelGGL(k_compose, grid(npix), dim3(256), 0, s, ca);
+    ca.kd = f.a->scene->d_kd;
+    ca.lights = f.lights;
+    ca.rgb = rgb;
+    hipLaunchKernelGGL(k_compose, grid256(ca.nslots), dim3(256), 0, ctx->stream, ca);
     RT_HIP_CHECK(ctx, hipGetLastError());
-    if ((rc = rec(8))) return rc;
-    (void)nl1;
+    return RT_OK;
+}
+
+Workspace *accel_workspace(rt_accel *a) {
+    if (!a->frame_ws) a->frame_ws = new Workspace();
+    return reinterpret_cast<Workspace *>(a->frame_ws);
+}
+
+int32_t render_frame(rt_accel *a, const rt_camera &cam, const rt_light *lights, int nlights,
+                     const rt_render_params &p, float *rgb, rt_hit *primary, rt_hit *bounce_hits,
+                     rt_stats *stats) {
+    rt_ctx *ctx = a->scene->ctx;
+    hipStream_t s = ctx->stream;
+    FramePlan f;
+    int32_t rc = plan_frame(a, cam, lights, nlights, p, &f);
+    if (rc) return rc;
+    if (bounce_hits && !p.bounce) return fail(ctx, RT_ERR_INVALID, "bounce hits requested without bounce");
+    const int64_t ns = f.tiles * 64 * f.spp;
+    Workspace *w = accel_workspace(a);
+    if ((rc = workspace_reserve(ctx, w, ns, std::max(1, nlights)))) return rc;
+    if ((rc = trace_chunk(f, w, 0, f.tiles, stats != nullptr))) return rc;
+    const bool dev_out = (p.flags & RT_DEVICE_PTRS) != 0;
+    float *rgb_dst = (dev_out && rgb) ? rgb : w->rgb;
+    if (stats) RT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[8], s));
+    if ((rc = compose_chunk(f, w, 0, f.tiles, rgb_dst))) return rc;
+    if (stats) RT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[9], s));
 
     // Outputs.
+    const int64_t npix = f.npix;
+    const int spp = f.spp;
     if (rgb && !dev_out)
         RT_HIP_CHECK(ctx, hipMemcpyAsync(rgb, w->rgb, 3 * npix * sizeof(float), hipMemcpyDeviceToHost, s));
     if (primary) {
-        hipLaunchKernelGGL(k_hits_out, grid(npix * spp), dim3(256), 0, s, tm, spp, npix, nullptr, w->hits1,
+        hipLaunchKernelGGL(k_hits_out, grid256(npix * spp), dim3(256), 0, s, f.tm, spp, npix, nullptr, w->hits1,
                            w->bounce_out);
         RT_HIP_CHECK(ctx, hipMemcpyAsync(primary, w->bounce_out, npix * spp * sizeof(rt_hit),
                                          dev_out ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
     }
     if (bounce_hits) {
-        if (!p.bounce) return fail(ctx, RT_ERR_INVALID, "bounce hits requested without bounce");
         if (primary) RT_HIP_CHECK(ctx, hipStreamSynchronize(s));  // bounce_out is reused
-        hipLaunchKernelGGL(k_hits_out, grid(npix * spp), dim3(256), 0, s, tm, spp, npix, w->qidx1, w->hits2,
+        hipLaunchKernelGGL(k_hits_out, grid256(npix * spp), dim3(256), 0, s, f.tm, spp, npix, w->qidx1, w->hits2,
                            w->bounce_out);
         RT_HIP_CHECK(ctx, hipMemcpyAsync(bounce_hits, w->bounce_out, npix * spp * sizeof(rt_hit),
                                          dev_out ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
     }
-    uint32_t counts[2] = {0, 0};
-    if (stats) {
-        RT_HIP_CHECK(ctx, hipMemcpyAsync(counts, w->blocks1 + nblocks, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-        if (p.bounce)
-            RT_HIP_CHECK(ctx, hipMemcpyAsync(counts + 1, w->blocks2 + nblocks, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    }
+    ChunkStats cs;
+    if (stats && (rc = collect_chunk(f, w, f.tiles, &cs))) return rc;
     RT_HIP_CHECK(ctx, hipStreamSynchronize(s));
     if (stats) {
         stats->rays[0] = static_cast<uint64_t>(npix * spp);
-        stats->rays[1] = static_cast<uint64_t>(counts[0]) * nlights;
-        stats->rays[2] = p.bounce ? counts[0] : 0;
-        stats->rays[3] = p.bounce ? static_cast<uint64_t>(counts[1]) * nlights : 0;
-        float ms;
-        RT_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ev[0], ev[8]));
-        stats->ms[0] = ms;
-        RT_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ev[1], ev[2]));
-        stats->ms[1] = ms;
-        RT_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ev[3], ev[4]));
-        stats->ms[2] = ms;
-        RT_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ev[4], ev[5]));
-        stats->ms[3] = ms;
-        RT_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ev[6], ev[7]));
-        stats->ms[4] = ms;
-        float shade = 0.0f;
-        RT_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ev[0], ev[1]));
-        shade += ms;
-        RT_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ev[2], ev[3]));
-        shade += ms;
-        RT_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ev[5], ev[6]));
-        shade += ms;
-        RT_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ev[7], ev[8]));
-        shade += ms;
-        stats->ms[5] = shade;
+        stats->rays[1] = cs.hits1 * nlights;
+        stats->rays[2] = p.bounce ? cs.hits1 : 0;
+        stats->rays[3] = p.bounce ? cs.hits2 * nlights : 0;
+        float compose_ms = 0.0f;
+        RT_HIP_CHECK(ctx, hipEventElapsedTime(&compose_ms, ctx->ev[8], ctx->ev[9]));
+        // Trace stages: primary, shadow, bounce, bounce shadow; the rest is shading.
+        stats->ms[1] = cs.ms[1];
+        stats->ms[2] = cs.ms[3];
+        stats->ms[3] = cs.ms[4];
+        stats->ms[4] = cs.ms[6];
+        stats->ms[5] = cs.ms[0] + cs.ms[2] + cs.ms[5] + compose_ms;
+        stats->ms[0] = stats->ms[1] + stats->ms[2] + stats->ms[3] + stats->ms[4] + stats->ms[5];
     }
     return RT_OK;
 }
 
 void free_frame_workspace(rt_accel *a) {
-    Workspace *w = reinterpret_cast<Workspace *>(a->frame_ws);
-    if (!w) return;
-    if (w->mem) (void)hipFree(w->mem);
-    delete w;
+    workspace_release(reinterpret_cast<Workspace *>(a->frame_ws));
     a->frame_ws = nullptr;
 }
 

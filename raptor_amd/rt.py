@@ -31,6 +31,7 @@ EXPORTS = [
     "rt_scene_normals", "rt_accel_build", "rt_accel_destroy", "rt_accel_info_get", "rt_trace_closest",
     "rt_trace_occluded", "rt_render", "rt_set_counting", "rt_gen_random_triangles", "rt_gen_terrain",
     "rt_batch_create", "rt_batch_run", "rt_batch_destroy", "rt_batch_autotune", "rt_gen_surface_rays",
+    "rt_scene_set_albedo", "rt_render_tape", "rt_tape_info", "rt_render_adjoint", "rt_tape_destroy",
 ]
 
 
@@ -104,6 +105,12 @@ def load_library(path=LIB_PATH):
         "rt_batch_autotune": (i32, [vp, C.POINTER(rt_batch_item), i32, i32, C.POINTER(rt_batch_params),
                                     C.POINTER(C.c_float)]),
         "rt_gen_surface_rays": (i32, [vp, u64, i64, fp]),
+        "rt_scene_set_albedo": (i32, [vp, fp, u32]),
+        "rt_render_tape": (i32, [vp, C.POINTER(rt_camera), C.POINTER(rt_light), i32, C.POINTER(rt_render_params), fp,
+                                 i64, C.POINTER(vp), C.POINTER(rt_stats)]),
+        "rt_tape_info": (i32, [vp, C.POINTER(i32), C.POINTER(i64), C.POINTER(i64)]),
+        "rt_render_adjoint": (i32, [vp, fp, fp, fp, C.POINTER(C.c_float)]),
+        "rt_tape_destroy": (None, [vp]),
         "rt_gen_random_triangles": (i64, [u64, i64, C.c_float, C.c_float, C.c_float, fp, fp]),
         "rt_gen_terrain": (i64, [i32, C.c_float, C.c_float, C.c_float, i32, i32, u64, C.c_float, C.c_float, fp, fp]),
     }
@@ -199,6 +206,15 @@ class Scene:
         h = C.c_void_p()
         ctx.check(_LIB.rt_scene_create(ctx.h, _ptr(abc), _ptr(alb), _ptr(fl), self.n, C.byref(h)), "rt_scene_create")
         self.h = h
+
+    def set_albedo(self, albedo):
+        """albedo: (n, 3) float32, a host array or a device tensor."""
+        if isinstance(albedo, np.ndarray):
+            a = np.ascontiguousarray(albedo, np.float32).reshape(-1, 3)
+            self.ctx.check(_LIB.rt_scene_set_albedo(self.h, _ptr(a), 0), "rt_scene_set_albedo")
+        else:
+            a = albedo.detach().contiguous()
+            self.ctx.check(_LIB.rt_scene_set_albedo(self.h, _ptr(a), RT_DEVICE_PTRS), "rt_scene_set_albedo")
 
     def normals(self):
         out = np.empty((self.n, 3), np.float32)
@@ -336,9 +352,55 @@ class Accel:
                                       C.byref(st)), "rt_render")
         return {"rays": list(st.rays), "ms": list(st.ms), "visits": list(st.visits)}
 
+    def render_tape(self, camera, lights, rgb_dev, bounce=False, spp=1, seed=0, max_record_bytes=0):
+        """Frame into rgb_dev that keeps what the adjoint needs; returns (Tape, stats)."""
+        return Tape(self, camera, lights, rgb_dev, bounce, spp, seed, max_record_bytes)
+
     def close(self):
         if self.h:
             _LIB.rt_accel_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Tape:
+    """A frame rendered for differentiation (see rt_render_tape)."""
+
+    def __init__(self, accel, camera, lights, rgb_dev, bounce, spp, seed, max_record_bytes):
+        self.ctx = accel.ctx
+        self.accel = accel
+        self.nlights = len(lights)
+        cam = camera.to_c()
+        larr = (rt_light * max(1, self.nlights))(*[l.to_c() for l in lights])
+        p = rt_render_params(1 if bounce else 0, spp, seed, RT_DEVICE_PTRS, 0)
+        st = rt_stats()
+        h = C.c_void_p()
+        self.ctx.check(_LIB.rt_render_tape(accel.h, C.byref(cam), larr, self.nlights, C.byref(p), _ptr(rgb_dev),
+                                           int(max_record_bytes), C.byref(h), C.byref(st)), "rt_render_tape")
+        self.h = h
+        self.stats = {"rays": list(st.rays), "ms": list(st.ms)}
+
+    def info(self):
+        r, b, t = C.c_int32(), C.c_int64(), C.c_int64()
+        self.ctx.check(_LIB.rt_tape_info(self.h, C.byref(r), C.byref(b), C.byref(t)), "rt_tape_info")
+        return {"replay": bool(r.value), "record_bytes": b.value, "chunk_tiles": t.value}
+
+    def adjoint(self, d_rgb, d_albedo, d_lights):
+        """Adds dL/d albedo and dL/d lights (device tensors) for d_rgb = dL/d image; returns ms."""
+        ms = C.c_float()
+        g = d_rgb.contiguous()
+        self.ctx.check(_LIB.rt_render_adjoint(self.h, _ptr(g), _ptr(d_albedo), _ptr(d_lights) if self.nlights else None,
+                                              C.byref(ms)), "rt_render_adjoint")
+        return ms.value
+
+    def close(self):
+        if self.h:
+            _LIB.rt_tape_destroy(self.h)
             self.h = None
 
     def __del__(self):

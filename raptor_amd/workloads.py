@@ -98,8 +98,8 @@ def random65k(n=65536, res=256):
                     meta=dict(triangles=n, resolution=[res, res]))
 
 
-def terrain(grid=708, x_res=1920, y_res=1080, bounce=True, seed=7):
-    abc, alb = rt.gen_terrain(grid, 0.15, 0.02, 0.5, 8, 57, 2, 0.2, 0.8)
+def terrain(grid=708, x_res=1920, y_res=1080, bounce=True, seed=7, zoom=0.02, amplitude=0.15):
+    abc, alb = rt.gen_terrain(grid, amplitude, zoom, 0.5, 8, 57, 2, 0.2, 0.8)
     pos = (0.5, 0.55, -0.35)
     x, y, z = look_at(pos, (0.5, 0.0, 0.6))
     w = 0.5
@@ -111,6 +111,26 @@ def terrain(grid=708, x_res=1920, y_res=1080, bounce=True, seed=7):
 
 def terrain1m(x_res=1920, y_res=1080):
     return terrain(708, x_res, y_res)
+
+
+def diff_terrain(grid=1414, x_res=1024, y_res=1024, spp=4, seed=11, bounce=True, zoom=0.02, amplitude=0.15):
+    """Differentiable frame: grid x grid fBm terrain (as terrain(), albedo seed
+    2), the terrain camera, spp jittered samples per pixel, direct light + one
+    diffuse bounce, four point lights on a ring of radius 0.6 at height 2 with
+    falloff d = 1. The target is the same frame with albedo * (1 + 0.2 n),
+    n ~ N(0, 1) (seeded), clamped to [0, 1]: meta["target_albedo"]. Small
+    test grids pass a larger zoom (and amplitude) to keep the relief."""
+    w = terrain(grid, x_res, y_res, bounce=bounce, seed=seed, zoom=zoom, amplitude=amplitude)
+    lights = []
+    colours = [(255.0, 240.0, 220.0), (200.0, 220.0, 255.0), (255.0, 200.0, 200.0), (220.0, 255.0, 210.0)]
+    for k in range(4):
+        a = 2.0 * np.pi * (k + 0.5) / 4.0
+        lights.append(Light(f32((0.5 + 0.6 * np.cos(a), 2.0, 0.5 + 0.6 * np.sin(a))), colours[k], 1.0))
+    rng = np.random.default_rng(seed)
+    target = np.clip(w.albedo * (1.0 + 0.2 * rng.standard_normal(w.albedo.shape)), 0.0, 1.0).astype(np.float32)
+    return Workload(f"diff_terrain{grid}", w.abc, w.albedo, w.camera, lights, bounce=bounce, spp=spp, seed=seed,
+                    meta=dict(triangles=int(w.abc.shape[0]), grid=grid, resolution=[x_res, y_res], spp=spp,
+                              target_albedo=target))
 
 
 @dataclass

@@ -17,3 +17,17 @@ def gpu_ctx():
     from raptor_amd import rt
 
     return rt.Context(0)
+
+
+@pytest.fixture(scope="session")
+def rt():
+    from raptor_amd import rt as _rt
+
+    return _rt
+
+
+@pytest.fixture(scope="session")
+def O():
+    import oracle.oracle as _o
+
+    return _o

@@ -14,20 +14,6 @@ pytestmark = pytest.mark.gpu
 RAD_TOL = 1e-5
 
 
-@pytest.fixture(scope="module")
-def rt():
-    from raptor_amd import rt as _rt
-
-    return _rt
-
-
-@pytest.fixture(scope="module")
-def O():
-    import oracle.oracle as _o
-
-    return _o
-
-
 def _render_both(rt, O, ctx, w, width=4, leaf=2, stride=1, lds=16, regroup=False):
     scene = rt.Scene(ctx, w.abc, w.albedo)
     accel = rt.Accel(scene, width=width, leaf_size=leaf, lds_stack=lds)
