@@ -146,6 +146,73 @@ def batched_section(ctx, n_scenes, reps):
     }
 
 
+def adjoint_section(ctx, reps, grid=1414, res=1024, spp=4):
+    """Differentiable frame: grid x grid fBm terrain, res x res at spp samples,
+    direct light + one diffuse bounce, four point lights (falloff 1). Loss =
+    sum (I - I_target)^2 against the frame rendered with perturbed albedo. The
+    forward (taped) frame and the adjoint are timed with device events, median
+    of reps; once with stored path records and once with path replay forced by
+    a record budget of a quarter of the frame's records."""
+    import statistics
+
+    import torch
+
+    from raptor_amd import rt, workloads
+
+    w = workloads.diff_terrain(grid=grid, x_res=res, y_res=res, spp=spp)
+    scene = rt.Scene(ctx, w.abc, w.albedo)
+    accel = rt.Accel(scene, width=4, leaf_size=2, lds_stack=8)
+    n = int(w.abc.shape[0])
+    target_alb = w.meta["target_albedo"]
+    del w.abc
+    img = torch.empty((res, res, 3), dtype=torch.float32, device="cuda:0")
+    tgt = torch.empty_like(img)
+    scene.set_albedo(target_alb)
+    accel.render_device(w.camera, w.lights, tgt, bounce=True, spp=spp, seed=w.seed)
+    scene.set_albedo(w.albedo)
+    d_alb = torch.zeros((n, 3), dtype=torch.float32, device="cuda:0")
+    d_l = torch.zeros((len(w.lights), 6), dtype=torch.float32, device="cuda:0")
+
+    def run(budget):
+        fwd, bwd, rays, info = [], [], 0, None
+        for k in range(reps + 2):
+            tape = accel.render_tape(w.camera, w.lights, img, bounce=True, spp=spp, seed=w.seed,
+                                     max_record_bytes=budget)
+            g = 2.0 * (img - tgt)
+            d_alb.zero_()
+            d_l.zero_()
+            torch.cuda.synchronize()
+            ms_b = tape.adjoint(g, d_alb, d_l)
+            if k >= 2:
+                fwd.append(tape.stats["ms"][0])
+                bwd.append(ms_b)
+            rays = sum(tape.stats["rays"])
+            info = tape.info()
+            tape.close()
+        return statistics.median(fwd), statistics.median(bwd), rays, info
+
+    f_ms, b_ms, rays, info = run(0)
+    loss = float(((img - tgt) ** 2).sum())
+    grad_norm = float(d_alb.norm())
+    rf_ms, rb_ms, _, rinfo = run(info["record_bytes"] // 4)
+    chunks = -(-((res + 7) // 8) ** 2 // rinfo["chunk_tiles"])
+    return {
+        "workload": f"{n}-triangle fBm terrain ({grid}x{grid}), {res}x{res} at {spp} spp, direct + 1 bounce, "
+                    f"4 point lights (falloff 1); loss = sum (I - I_target)^2, target albedo * (1 + 0.2 N(0,1))",
+        "rays_per_frame": rays,
+        "forward_ms": round(f_ms, 4),
+        "adjoint_ms": round(b_ms, 4),
+        "adjoint_over_forward": round(b_ms / f_ms, 4),
+        "forward_mrays_s": round(rays / f_ms / 1e3, 2),
+        "record_bytes": info["record_bytes"],
+        "replay": {"record_budget": info["record_bytes"] // 4, "chunks": chunks, "forward_ms": round(rf_ms, 4),
+                   "adjoint_ms": round(rb_ms, 4), "adjoint_over_forward": round(rb_ms / rf_ms, 4),
+                   "adjoint_mrays_s": round(rays / rb_ms / 1e3, 2)},
+        "loss": loss,
+        "albedo_grad_norm": grad_norm,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -159,6 +226,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batched-scenes", type=int, default=64, help="0 skips the batched workload")
     ap.add_argument("--batched-reps", type=int, default=10)
+    ap.add_argument("--adjoint-reps", type=int, default=10, help="0 skips the differentiable workload")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each GPU would trace its own share "
@@ -257,6 +325,9 @@ def main():
         del rgb, accel, scene
         print("bench: batched workload", file=sys.stderr, flush=True)
         line["batched"] = batched_section(ctx, args.batched_scenes, args.batched_reps)
+    if args.adjoint_reps > 0:
+        print("bench: differentiable workload", file=sys.stderr, flush=True)
+        line["adjoint"] = adjoint_section(ctx, args.adjoint_reps)
     if not args.no_cpu_baseline:
         print("bench: CPU baseline", file=sys.stderr, flush=True)
         cb = cpu_baseline(w, args.cpu_seconds)

@@ -193,14 +193,20 @@ __global__ void __launch_bounds__(256) k_adjoint(AdjArgs a) {
     }
 }
 
-// acc[t] += sum over blocks of part[b][t], blocks in order.
-__global__ void __launch_bounds__(64) k_light_sum(const double *__restrict__ part, int64_t nblocks, int nslots,
-                                                  double *__restrict__ acc) {
-    const int t = threadIdx.x;
-    if (t >= nslots) return;
+// acc[t] += sum over blocks of part[b][t]: one 256-thread block per slot t,
+// each thread a fixed stride of blocks, then a fixed-shape tree (the result
+// does not depend on scheduling).
+__global__ void __launch_bounds__(256) k_light_sum(const double *__restrict__ part, int64_t nblocks,
+                                                   double *__restrict__ acc) {
+    __shared__ double wsum[4];
+    const int t = blockIdx.x;
     double s = 0.0;
-    for (int64_t b = 0; b < nblocks; ++b) s += part[b * kLightSlots + t];
-    acc[t] += s;
+    for (int64_t b = threadIdx.x; b < nblocks; b += 256) s += part[b * kLightSlots + t];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) acc[t] += (wsum[0] + wsum[1]) + (wsum[2] + wsum[3]);
 }
 
 __global__ void k_fill_f64(double *p, int n) {
@@ -256,8 +262,9 @@ int32_t adjoint_chunk(rt_tape *t, int64_t tile0, int64_t ntiles, const float *gr
     const int64_t blocks = (aa.nslots + 255) / 256;
     if (blocks > t->light_blocks) return fail(ctx, RT_ERR_INVALID, "adjoint: chunk larger than the tape's buffers");
     hipLaunchKernelGGL(k_adjoint, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, ctx->stream, aa);
-    hipLaunchKernelGGL(k_light_sum, dim3(1), dim3(64), 0, ctx->stream, t->d_light, blocks,
-                       f.nlights * kLightGrad, t->d_light + t->light_blocks * kLightSlots);
+    if (f.nlights > 0)
+        hipLaunchKernelGGL(k_light_sum, dim3(f.nlights * kLightGrad), dim3(256), 0, ctx->stream, t->d_light, blocks,
+                           t->d_light + t->light_blocks * kLightSlots);
     RT_HIP_CHECK(ctx, hipGetLastError());
     return RT_OK;
 }
@@ -343,25 +350,36 @@ int32_t rt_render_tape(rt_accel *accel, const rt_camera *cam, const rt_light *li
     hipStream_t s = ctx->stream;
     uint64_t rays[4] = {0, 0, 0, 0};
     float stage[5] = {0, 0, 0, 0, 0};
+    const bool single = t->chunk_tiles >= f.tiles;
+    auto add_stats = [&](int64_t nt, const rt::ChunkStats &cs) {
+        rays[0] += static_cast<uint64_t>(nt) * 64 * f.spp;
+        rays[1] += cs.hits1 * nlights;
+        rays[2] += p.bounce ? cs.hits1 : 0;
+        rays[3] += p.bounce ? cs.hits2 * nlights : 0;
+        stage[1] += cs.ms[1];
+        stage[2] += cs.ms[3];
+        stage[3] += cs.ms[4];
+        stage[4] += cs.ms[6];
+    };
     if (hipEventRecord(t->e0, s) != hipSuccess) rc = fail(ctx, RT_ERR_HIP, "hipEventRecord");
     for (int64_t t0 = 0; rc == RT_OK && t0 < f.tiles; t0 += t->chunk_tiles) {
         const int64_t nt = std::min(t->chunk_tiles, f.tiles - t0);
         rc = rt::trace_chunk(f, t->ws, t0, nt, stats != nullptr);
         if (rc == RT_OK) rc = rt::compose_chunk(f, t->ws, t0, nt, rgb);
-        if (rc == RT_OK && stats) {
+        // Several chunks share the workspace: read each chunk's counts before
+        // the next overwrites them (this synchronises).
+        if (rc == RT_OK && stats && !single) {
             rt::ChunkStats cs;
             rc = rt::collect_chunk(f, t->ws, nt, &cs);
-            rays[0] += static_cast<uint64_t>(nt) * 64 * f.spp;
-            rays[1] += cs.hits1 * nlights;
-            rays[2] += p.bounce ? cs.hits1 : 0;
-            rays[3] += p.bounce ? cs.hits2 * nlights : 0;
-            stage[1] += cs.ms[1];
-            stage[2] += cs.ms[3];
-            stage[3] += cs.ms[4];
-            stage[4] += cs.ms[6];
+            add_stats(nt, cs);
         }
     }
     if (rc == RT_OK && hipEventRecord(t->e1, s) != hipSuccess) rc = fail(ctx, RT_ERR_HIP, "hipEventRecord");
+    if (rc == RT_OK && stats && single) {
+        rt::ChunkStats cs;
+        rc = rt::collect_chunk(f, t->ws, f.tiles, &cs);
+        add_stats(f.tiles, cs);
+    }
     if (rc == RT_OK && hipEventSynchronize(t->e1) != hipSuccess) rc = fail(ctx, RT_ERR_HIP, "forward failed");
     if (rc) {
         rt::free_tape(t);

@@ -9,6 +9,10 @@
       reads rocprofv3 CSV output and writes PREFIX.md (kernel summary) and
       PREFIX.json (HBM bytes per launch of the dominant kernel), which
       bench.py reports as roofline.traffic.
+  python3 raptor_amd/profiling.py kernels --stats DIR --match 'SUBSTR[|SUBSTR]' 
+                                          --title TEXT --command TEXT --out FILE
+      writes FILE (markdown): the rocprofv3 kernel statistics of the kernels
+      whose names contain one of the substrings.
 """
 import argparse
 import csv
@@ -93,6 +97,22 @@ def cmd_summarize(args):
     print(json.dumps(out))
 
 
+def cmd_kernels(args):
+    stats = _rows(args.stats, "*kernel_stats.csv")
+    keys = [k.strip() for k in args.match.split("|") if k.strip()]
+    lines = [f"# Kernel summary: {args.title}", "", "Command (from the repository root, TMPDIR=/tmp):", "",
+             "    " + args.command, "", "| kernel | calls | avg us | min us | max us |", "|---|---|---|---|---|"]
+    for r in stats:
+        name = r["Name"].replace("(anonymous namespace)::", "").split("(")[0].replace("void ", "")
+        if not any(k in name for k in keys):
+            continue
+        lines.append(f"| `{name}` | {r['Calls']} | {float(r['AverageNs']) / 1e3:.2f} | "
+                     f"{float(r['MinNs']) / 1e3:.2f} | {float(r['MaxNs']) / 1e3:.2f} |")
+    with open(args.out, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print("\n".join(lines))
+
+
 def main():
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -103,9 +123,14 @@ def main():
         s.add_argument("--" + k.replace("_", "-"), dest=k, required=True)
     s.add_argument("--copy-bytes", type=int, default=1024 * (1 << 20))
     s.add_argument("--command", default="")
+    k = sub.add_parser("kernels")
+    for a in ("stats", "match", "title", "command", "out"):
+        k.add_argument("--" + a, required=True)
     args = ap.parse_args()
     if args.cmd == "copy":
         cmd_copy(args)
+    elif args.cmd == "kernels":
+        cmd_kernels(args)
     else:
         cmd_summarize(args)
 
