@@ -220,6 +220,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--width", type=int, default=4)
     ap.add_argument("--leaf-size", type=int, default=2)
+    ap.add_argument("--sah-bins", type=int, default=16, help="binned-SAH rebuild of the LBVH (0 = keep the LBVH)")
     ap.add_argument("--lds-stack", type=int, default=8)
     ap.add_argument("--regroup", type=int, default=0, help="regroup bounce rays before tracing (0/1)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
@@ -241,7 +242,8 @@ def main():
     w = workloads.terrain1m()
     ctx = rt.Context(0)
     scene = rt.Scene(ctx, w.abc, w.albedo)
-    accel = rt.Accel(scene, width=args.width, leaf_size=args.leaf_size, lds_stack=args.lds_stack)
+    accel = rt.Accel(scene, width=args.width, leaf_size=args.leaf_size, lds_stack=args.lds_stack,
+                     sah_bins=args.sah_bins)
     info = accel.info()
     rgb = torch.empty((w.camera.y_res, w.camera.x_res, 3), dtype=torch.float32, device="cuda:0")
 
@@ -276,7 +278,8 @@ def main():
         with open(tpath) as f:
             t = json.load(f)
         same = (t.get("kernel_stage") == STAGES[dom] and t.get("width") == args.width
-                and t.get("leaf_size") == args.leaf_size and t.get("lds_stack") == args.lds_stack)
+                and t.get("leaf_size") == args.leaf_size and t.get("lds_stack") == args.lds_stack
+                and t.get("sah_bins", 0) == args.sah_bins)
         if same:
             traffic = t.get("bytes_per_launch")
 
@@ -300,6 +303,7 @@ def main():
             "resolution": [w.camera.x_res, w.camera.y_res],
             "bvh_width": args.width,
             "leaf_size": args.leaf_size,
+            "sah_bins": args.sah_bins,
             "lds_stack": args.lds_stack,
             "regroup_bounce": bool(args.regroup),
         },

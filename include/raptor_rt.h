@@ -70,10 +70,11 @@ typedef struct {
 } rt_light;
 
 /* BVH build and traversal options. width: 4 or 8 children per node;
- * leaf_size: most triangles per leaf (1..16); sah_bins: bins of the SAH
- * refinement (0 = off); lds_stack: traversal-stack entries per lane kept in
- * LDS before spilling to private memory (4, 8 or 16; 0 = 16). params == NULL
- * means width 4, leaf size 2, no refinement, 8 LDS entries. */
+ * leaf_size: most triangles per leaf (1..16); sah_bins: bins per axis of the
+ * binned-SAH rebuild of the Morton LBVH (4..32; 0 = keep the LBVH: faster
+ * build, slower traversal); lds_stack: traversal-stack entries per lane kept
+ * in LDS before spilling to private memory (4, 8 or 16; 0 = 16). params ==
+ * NULL means width 4, leaf size 2, 16 SAH bins, 8 LDS entries. */
 typedef struct {
     int32_t width;
     int32_t leaf_size;

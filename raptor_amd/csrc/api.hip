@@ -223,7 +223,7 @@ int32_t rt_accel_build(rt_scene *scene, const rt_build_params *params, rt_accel 
     *out = nullptr;
     rt_ctx *ctx = scene->ctx;
     if (int32_t rc = rt::set_device(ctx)) return rc;
-    rt_build_params p{4, 2, 0, 8};
+    rt_build_params p{4, 2, 16, 8};
     if (params) p = *params;
     const int32_t rc = rt::build_accel(scene, p, out);
     if (rc == RT_OK) scene->refs.fetch_add(1);

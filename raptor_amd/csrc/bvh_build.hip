@@ -353,6 +353,533 @@ __global__ void __launch_bounds__(256) k_leaf_tris(const float *__restrict__ abc
     out[k] = lt;
 }
 
+
+// ---------------------------------------------------------------------------
+// Binned-SAH refinement of the LBVH's upper levels.
+//
+// The LBVH is cut into clusters: subtrees of at most kSahCluster triangles
+// whose parent holds more. The levels above the cut are rebuilt top-down with
+// a binned SAH over the clusters (one pass per level over all clusters: node
+// centroid bounds, bins of count / triangles / box per axis, best plane by
+// A_L * T_L + A_R * T_R, stable partition by a scan), so that splits follow
+// surface area instead of Morton-code midpoints. Nodes of at most
+// kSahSweepItems clusters are split by an exact sweep instead (every cluster
+// centroid is a candidate plane on every axis, one thread per node). The
+// triangles are then laid out in the new cluster order, so
+// every node of the combined tree covers a contiguous range and the collapse
+// can still make leaves of ranges. SAH nodes are appended after the LBVH's
+// internal nodes in the binary arrays; the tree's root becomes the SAH root.
+//
+// Cut size, measured on the 1M-triangle terrain frame (BVH4, leaf 2): the
+// LBVH alone traces 3190 Mrays/s (4.7 ms build); clusters of 8 triangles
+// 3470 (30 ms); single triangles, i.e. the whole tree rebuilt by SAH over the
+// Morton-ordered triangles, 3750-3780 (31 ms with the sweep for small nodes).
+constexpr int kSahCluster = 1;
+constexpr int kSahSweepItems = 16;
+constexpr int kSahBinsMax = 32;
+constexpr int kSahBinWords = 8;            // count, triangles, lo xyz, hi xyz (lo/hi order-mapped)
+constexpr int32_t kSahRef = 1 << 30;       // child ref of SAH node s: kSahRef | s
+constexpr int32_t kNoCluster = INT32_MIN;
+
+struct SahNode {
+    int32_t begin, end;   // item range
+    int32_t id;           // SAH node index
+    int32_t slot;         // bin slot (binned split) or -1
+};
+
+struct SahSplit {
+    int32_t axis;         // 0..2 a plane, -1 middle of the items
+    int32_t bin;          // binned plane: items with bin <= this go left; -1: plane below
+    int32_t nl;           // items on the left
+    float plane;          // swept plane: items with centroid < plane go left
+};
+
+__global__ void __launch_bounds__(256) k_fill_i32(int32_t *p, int64_t n, int32_t v) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void __launch_bounds__(256) k_fill_u32(uint32_t *p, int64_t n, uint32_t v) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__device__ __forceinline__ int range_count(const int2 *range, int32_t x) {
+    if (x < 0) return 1;
+    const int2 r = range[x];
+    return r.y - r.x + 1;
+}
+
+// Cluster roots: children of nodes holding more than kSahCluster triangles
+// that hold at most kSahCluster themselves; recorded at their first sorted
+// position so that the clusters come out in Morton order.
+__global__ void __launch_bounds__(256) k_find_clusters(const int2 *__restrict__ child, const int2 *__restrict__ range,
+                                                       int64_t n, int32_t *__restrict__ cluster_at) {
+    const int64_t x = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (x >= n - 1) return;
+    if (range_count(range, static_cast<int32_t>(x)) <= kSahCluster) return;
+    const int2 ch = child[x];
+    const int32_t c2[2] = {ch.x, ch.y};
+    for (int k = 0; k < 2; ++k) {
+        const int32_t c = c2[k];
+        if (c < 0)
+            cluster_at[~c] = c;
+        else if (range_count(range, c) <= kSahCluster)
+            cluster_at[range[c].x] = c;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_cluster_flags(const int32_t *__restrict__ cluster_at, int64_t n,
+                                                       uint32_t *__restrict__ flags) {
+    const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p < n) flags[p] = cluster_at[p] != kNoCluster ? 1u : 0u;
+}
+
+// Items: lo (xyz, w = cluster ref), hi (xyz, w = triangle count).
+__global__ void __launch_bounds__(256) k_make_items(const int32_t *__restrict__ cluster_at,
+                                                    const uint32_t *__restrict__ pos, int64_t n,
+                                                    const int2 *__restrict__ range, const float4 *__restrict__ node_lo,
+                                                    const float4 *__restrict__ node_hi, const float4 *__restrict__ leaf_lo,
+                                                    const float4 *__restrict__ leaf_hi, float4 *__restrict__ it_lo,
+                                                    float4 *__restrict__ it_hi, int32_t *__restrict__ it_node) {
+    const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int32_t c = cluster_at[p];
+    if (c == kNoCluster) return;
+    const uint32_t j = pos[p];
+    float4 lo, hi;
+    child_box(c, node_lo, node_hi, leaf_lo, leaf_hi, &lo, &hi);
+    it_lo[j] = make_float4(lo.x, lo.y, lo.z, __int_as_float(c));
+    it_hi[j] = make_float4(hi.x, hi.y, hi.z, __int_as_float(range_count(range, c)));
+    it_node[j] = 0;
+}
+
+__global__ void __launch_bounds__(256) k_sah_classify(SahNode *__restrict__ nodes, int count,
+                                                      uint32_t *__restrict__ slots) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    SahNode nd = nodes[k];
+    nd.slot = (nd.end - nd.begin > kSahSweepItems) ? static_cast<int32_t>(atomicAdd(slots, 1u)) : -1;
+    nodes[k] = nd;
+}
+
+__device__ __forceinline__ float3 item_centroid(float4 lo, float4 hi) {
+    return make_float3((lo.x + hi.x) * 0.5f, (lo.y + hi.y) * 0.5f, (lo.z + hi.z) * 0.5f);
+}
+
+// Centroid bounds of each binned node (6 order-mapped words per slot). A wave
+// whose lanes share one node reduces in registers first.
+__global__ void __launch_bounds__(256) k_sah_cbounds(const float4 *__restrict__ it_lo, const float4 *__restrict__ it_hi,
+                                                     const int32_t *__restrict__ it_node, int64_t m,
+                                                     const SahNode *__restrict__ nodes, uint32_t *__restrict__ cb) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    int32_t slot = -1;
+    float3 c = make_float3(0.0f, 0.0f, 0.0f);
+    if (i < m) {
+        const int32_t nd = it_node[i];
+        if (nd >= 0) {
+            slot = nodes[nd].slot;
+            c = item_centroid(it_lo[i], it_hi[i]);
+        }
+    }
+    const int32_t s0 = __shfl(slot, 0, 64);
+    const bool uniform = __ballot(slot == s0) == ~0ull;
+    if (uniform) {
+        if (s0 < 0) return;
+        float v[6] = {c.x, c.y, c.z, c.x, c.y, c.z};
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                const float y = __shfl_xor(v[k], o, 64);
+                v[k] = (k < 3) ? fminf(v[k], y) : fmaxf(v[k], y);
+            }
+        }
+        if ((threadIdx.x & 63) == 0) {
+            uint32_t *b = cb + 6 * static_cast<int64_t>(s0);
+            for (int k = 0; k < 3; ++k) atomicMin(&b[k], f2ord(v[k]));
+            for (int k = 3; k < 6; ++k) atomicMax(&b[k], f2ord(v[k]));
+        }
+        return;
+    }
+    if (slot < 0) return;
+    uint32_t *b = cb + 6 * static_cast<int64_t>(slot);
+    atomicMin(&b[0], f2ord(c.x));
+    atomicMin(&b[1], f2ord(c.y));
+    atomicMin(&b[2], f2ord(c.z));
+    atomicMax(&b[3], f2ord(c.x));
+    atomicMax(&b[4], f2ord(c.y));
+    atomicMax(&b[5], f2ord(c.z));
+}
+
+__device__ __forceinline__ int sah_bin_of(float c, float lo, float hi, int nbins) {
+    const float ext = hi - lo;
+    if (!(ext > 0.0f)) return 0;
+    const int b = static_cast<int>((c - lo) * (static_cast<float>(nbins) / ext));
+    return min(max(b, 0), nbins - 1);
+}
+
+// Bins per binned node: [slot][axis][bin][count, tris, lo xyz, hi xyz]. A
+// block whose items all belong to one node bins into LDS first and then
+// merges its non-empty bins with one set of global atomics.
+__device__ __forceinline__ void bin_add(uint32_t *w, float4 lo, float4 hi, uint32_t tris) {
+    atomicAdd(&w[0], 1u);
+    atomicAdd(&w[1], tris);
+    atomicMin(&w[2], f2ord(lo.x));
+    atomicMin(&w[3], f2ord(lo.y));
+    atomicMin(&w[4], f2ord(lo.z));
+    atomicMax(&w[5], f2ord(hi.x));
+    atomicMax(&w[6], f2ord(hi.y));
+    atomicMax(&w[7], f2ord(hi.z));
+}
+
+__global__ void __launch_bounds__(256) k_sah_bin(const float4 *__restrict__ it_lo, const float4 *__restrict__ it_hi,
+                                                 const int32_t *__restrict__ it_node, int64_t m,
+                                                 const SahNode *__restrict__ nodes, const uint32_t *__restrict__ cb,
+                                                 int nbins, uint32_t *__restrict__ bins) {
+    __shared__ uint32_t lbins[3 * kSahBinsMax * kSahBinWords];
+    __shared__ int32_t block_slot;
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    int32_t slot = -1;
+    if (i < m) {
+        const int32_t nd = it_node[i];
+        if (nd >= 0) slot = nodes[nd].slot;
+    }
+    // Uniform block: every thread's slot equals thread 0's (and is valid).
+    if (threadIdx.x == 0) block_slot = slot;
+    const int words = 3 * nbins * kSahBinWords;
+    for (int k = threadIdx.x; k < words; k += blockDim.x) {
+        const int w = k % kSahBinWords;
+        lbins[k] = (w >= 2 && w < 5) ? 0xffffffffu : 0u;
+    }
+    __syncthreads();
+    const int32_t bs = block_slot;
+    const int uniform = __syncthreads_and(slot == bs && bs >= 0);
+    if (slot >= 0) {
+        const float4 lo = it_lo[i], hi = it_hi[i];
+        const float3 c = item_centroid(lo, hi);
+        const uint32_t tris = static_cast<uint32_t>(__float_as_int(hi.w));
+        const uint32_t *b = cb + 6 * static_cast<int64_t>(slot);
+        const float cc[3] = {c.x, c.y, c.z};
+        for (int a = 0; a < 3; ++a) {
+            const int k = sah_bin_of(cc[a], ord2f(b[a]), ord2f(b[3 + a]), nbins);
+            if (uniform)
+                bin_add(lbins + (a * nbins + k) * kSahBinWords, lo, hi, tris);
+            else
+                bin_add(bins + ((static_cast<int64_t>(slot) * 3 + a) * nbins + k) * kSahBinWords, lo, hi, tris);
+        }
+    }
+    if (!uniform) return;
+    __syncthreads();
+    for (int k = threadIdx.x; k < 3 * nbins; k += blockDim.x) {
+        const uint32_t *l = lbins + k * kSahBinWords;
+        if (l[0] == 0) continue;
+        uint32_t *g = bins + (static_cast<int64_t>(bs) * 3 * nbins + k) * kSahBinWords;
+        atomicAdd(&g[0], l[0]);
+        atomicAdd(&g[1], l[1]);
+        for (int w = 2; w < 5; ++w) atomicMin(&g[w], l[w]);
+        for (int w = 5; w < 8; ++w) atomicMax(&g[w], l[w]);
+    }
+}
+
+struct Box3 {
+    float lx, ly, lz, hx, hy, hz;
+    __device__ void reset() {
+        lx = ly = lz = INFINITY;
+        hx = hy = hz = -INFINITY;
+    }
+    __device__ void add(const uint32_t *w) {
+        lx = fminf(lx, ord2f(w[2]));
+        ly = fminf(ly, ord2f(w[3]));
+        lz = fminf(lz, ord2f(w[4]));
+        hx = fmaxf(hx, ord2f(w[5]));
+        hy = fmaxf(hy, ord2f(w[6]));
+        hz = fmaxf(hz, ord2f(w[7]));
+    }
+    __device__ float half_area() const {
+        const float dx = hx - lx, dy = hy - ly, dz = hz - lz;
+        return dx * dy + dx * dz + dy * dz;
+    }
+};
+
+// Exact sweep over a small node's items: every item centroid is a candidate
+// plane on every axis (items with centroid < plane go left).
+__device__ void sah_sweep(const float4 *it_lo, const float4 *it_hi, int32_t begin, int32_t end, SahSplit *sp) {
+    float best = INFINITY;
+    const int32_t size = end - begin;
+    for (int a = 0; a < 3; ++a) {
+        for (int32_t j = begin; j < end; ++j) {
+            const float3 cj = item_centroid(it_lo[j], it_hi[j]);
+            const float v = a == 0 ? cj.x : (a == 1 ? cj.y : cj.z);
+            float llx = INFINITY, lly = INFINITY, llz = INFINITY, lhx = -INFINITY, lhy = -INFINITY, lhz = -INFINITY;
+            float rlx = INFINITY, rly = INFINITY, rlz = INFINITY, rhx = -INFINITY, rhy = -INFINITY, rhz = -INFINITY;
+            uint32_t lt = 0, rt_ = 0;
+            int32_t nl = 0;
+            for (int32_t k = begin; k < end; ++k) {
+                const float4 lo = it_lo[k], hi = it_hi[k];
+                const float3 ck = item_centroid(lo, hi);
+                const float ca = a == 0 ? ck.x : (a == 1 ? ck.y : ck.z);
+                const uint32_t t = static_cast<uint32_t>(__float_as_int(hi.w));
+                if (ca < v) {
+                    llx = fminf(llx, lo.x); lly = fminf(lly, lo.y); llz = fminf(llz, lo.z);
+                    lhx = fmaxf(lhx, hi.x); lhy = fmaxf(lhy, hi.y); lhz = fmaxf(lhz, hi.z);
+                    lt += t;
+                    ++nl;
+                } else {
+                    rlx = fminf(rlx, lo.x); rly = fminf(rly, lo.y); rlz = fminf(rlz, lo.z);
+                    rhx = fmaxf(rhx, hi.x); rhy = fmaxf(rhy, hi.y); rhz = fmaxf(rhz, hi.z);
+                    rt_ += t;
+                }
+            }
+            if (nl == 0 || nl == size) continue;
+            const float ldx = lhx - llx, ldy = lhy - lly, ldz = lhz - llz;
+            const float rdx = rhx - rlx, rdy = rhy - rly, rdz = rhz - rlz;
+            const float cost = (ldx * ldy + ldx * ldz + ldy * ldz) * static_cast<float>(lt) +
+                               (rdx * rdy + rdx * rdz + rdy * rdz) * static_cast<float>(rt_);
+            if (cost < best) {
+                best = cost;
+                sp->axis = a;
+                sp->bin = -1;
+                sp->nl = nl;
+                sp->plane = v;
+            }
+        }
+    }
+}
+
+// One thread per node: the best binned plane (large nodes) or swept plane
+// (small nodes), else the middle. Children with more than one item get an
+// SAH node index and a place on the next level; single-item sides are linked
+// by k_sah_scatter.
+__global__ void __launch_bounds__(64) k_sah_split(const SahNode *__restrict__ nodes, int count,
+                                                  const float4 *__restrict__ it_lo, const float4 *__restrict__ it_hi,
+                                                  const uint32_t *__restrict__ bins, int nbins,
+                                                  SahSplit *__restrict__ splits, SahNode *__restrict__ next,
+                                                  uint32_t *__restrict__ next_count, uint32_t *__restrict__ sah_count,
+                                                  int2 *__restrict__ sah_child, int2 *__restrict__ sah_items) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    const SahNode nd = nodes[k];
+    const int32_t size = nd.end - nd.begin;
+    SahSplit sp{-1, 0, size / 2, 0.0f};
+    if (nd.slot < 0) {
+        sah_sweep(it_lo, it_hi, nd.begin, nd.end, &sp);
+    } else {
+        float best = INFINITY;
+        for (int a = 0; a < 3; ++a) {
+            const uint32_t *base = bins + (static_cast<int64_t>(nd.slot) * 3 + a) * nbins * kSahBinWords;
+            // Right-to-left suffix areas and counts, then the left sweep.
+            float r_area[kSahBinsMax];
+            uint32_t r_tris[kSahBinsMax];
+            Box3 bx;
+            bx.reset();
+            uint32_t t = 0;
+            for (int b = nbins - 1; b > 0; --b) {
+                const uint32_t *w = base + b * kSahBinWords;
+                if (w[0]) bx.add(w);
+                t += w[1];
+                r_area[b] = bx.half_area();
+                r_tris[b] = t;
+            }
+            bx.reset();
+            t = 0;
+            uint32_t cnt = 0;
+            for (int b = 0; b < nbins - 1; ++b) {
+                const uint32_t *w = base + b * kSahBinWords;
+                if (w[0]) bx.add(w);
+                t += w[1];
+                cnt += w[0];
+                if (cnt == 0 || cnt == static_cast<uint32_t>(size)) continue;
+                const float cost = bx.half_area() * static_cast<float>(t) +
+                                   r_area[b + 1] * static_cast<float>(r_tris[b + 1]);
+                if (cost < best) {
+                    best = cost;
+                    sp.axis = a;
+                    sp.bin = b;
+                    sp.nl = static_cast<int32_t>(cnt);
+                }
+            }
+        }
+    }
+    splits[k] = sp;
+    sah_items[nd.id] = make_int2(nd.begin, nd.end);
+    int2 ch = make_int2(0, 0);
+    const int32_t sizes[2] = {sp.nl, size - sp.nl};
+    const int32_t begins[2] = {nd.begin, nd.begin + sp.nl};
+    for (int side = 0; side < 2; ++side) {
+        if (sizes[side] <= 1) continue;  // linked by the scatter
+        const int32_t id = static_cast<int32_t>(atomicAdd(sah_count, 1u));
+        const uint32_t q = atomicAdd(next_count, 1u);
+        next[q] = SahNode{begins[side], begins[side] + sizes[side], id, -1};
+        if (side == 0)
+            ch.x = kSahRef | id;
+        else
+            ch.y = kSahRef | id;
+    }
+    // Sides of one item are written by the scatter; keep the other half.
+    if (sizes[0] > 1) sah_child[nd.id].x = ch.x;
+    if (sizes[1] > 1) sah_child[nd.id].y = ch.y;
+}
+
+__device__ __forceinline__ bool sah_goes_left(const SahNode &nd, const SahSplit &sp, const uint32_t *cb, int nbins,
+                                              float4 lo, float4 hi, int64_t i) {
+    if (sp.axis < 0) return (i - nd.begin) < sp.nl;
+    const float3 c = item_centroid(lo, hi);
+    const float cc = sp.axis == 0 ? c.x : (sp.axis == 1 ? c.y : c.z);
+    if (sp.bin < 0) return cc < sp.plane;
+    const uint32_t *b = cb + 6 * static_cast<int64_t>(nd.slot);
+    return sah_bin_of(cc, ord2f(b[sp.axis]), ord2f(b[3 + sp.axis]), nbins) <= sp.bin;
+}
+
+// Node index of each item on this level: it_node (items of finished nodes -1).
+__global__ void __launch_bounds__(256) k_sah_flags(const float4 *__restrict__ it_lo, const float4 *__restrict__ it_hi,
+                                                   const int32_t *__restrict__ it_node, int64_t m,
+                                                   const SahNode *__restrict__ nodes, const SahSplit *__restrict__ splits,
+                                                   const uint32_t *__restrict__ cb, int nbins,
+                                                   uint32_t *__restrict__ flags) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const int32_t nd = it_node[i];
+    bool left = false;
+    if (nd >= 0) left = sah_goes_left(nodes[nd], splits[nd], cb, nbins, it_lo[i], it_hi[i], i);
+    flags[i] = left ? 1u : 0u;
+}
+
+// Stable partition of each node's items (flags holds the exclusive scan of
+// the left flags); sides of one item are linked to their parent here.
+__global__ void __launch_bounds__(256) k_sah_scatter(const float4 *__restrict__ it_lo, const float4 *__restrict__ it_hi,
+                                                     const int32_t *__restrict__ it_node, int64_t m,
+                                                     const SahNode *__restrict__ nodes,
+                                                     const SahSplit *__restrict__ splits, const uint32_t *__restrict__ cb,
+                                                     int nbins, const uint32_t *__restrict__ scanned,
+                                                     float4 *__restrict__ out_lo, float4 *__restrict__ out_hi,
+                                                     int32_t *__restrict__ out_node, int2 *__restrict__ sah_child) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const float4 lo = it_lo[i], hi = it_hi[i];
+    const int32_t ndi = it_node[i];
+    if (ndi < 0) {
+        out_lo[i] = lo;
+        out_hi[i] = hi;
+        out_node[i] = -1;
+        return;
+    }
+    const SahNode nd = nodes[ndi];
+    const SahSplit sp = splits[ndi];
+    const bool left = sah_goes_left(nd, sp, cb, nbins, lo, hi, i);
+    const int64_t rl = static_cast<int64_t>(scanned[i]) - static_cast<int64_t>(scanned[nd.begin]);
+    const int64_t dest = left ? nd.begin + rl : nd.begin + sp.nl + (i - nd.begin - rl);
+    out_lo[dest] = lo;
+    out_hi[dest] = hi;
+    const int32_t side_size = left ? sp.nl : (nd.end - nd.begin - sp.nl);
+    if (side_size == 1) {
+        out_node[dest] = -1;
+        const int32_t ref = __float_as_int(lo.w);
+        if (left)
+            sah_child[nd.id].x = ref;
+        else
+            sah_child[nd.id].y = ref;
+    } else {
+        out_node[dest] = -2;  // resolved by k_sah_assign
+    }
+}
+
+// Items of the next level's nodes learn their node index.
+__global__ void __launch_bounds__(64) k_sah_assign(const SahNode *__restrict__ next, int count,
+                                                    int32_t *__restrict__ it_node) {
+    const int k = blockIdx.x;
+    if (k >= count) return;
+    const SahNode nd = next[k];
+    for (int32_t i = nd.begin + threadIdx.x; i < nd.end; i += blockDim.x) it_node[i] = k;
+}
+
+__global__ void __launch_bounds__(256) k_sah_reset(uint32_t *__restrict__ cb, uint32_t *__restrict__ bins, int nslots,
+                                                   int nbins) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t ncb = 6 * static_cast<int64_t>(nslots);
+    const int64_t nbw = static_cast<int64_t>(nslots) * 3 * nbins * kSahBinWords;
+    if (i < ncb) cb[i] = (i % 6) < 3 ? 0xffffffffu : 0u;
+    if (i < nbw) {
+        const int w = static_cast<int>(i % kSahBinWords);
+        bins[i] = (w >= 2 && w < 5) ? 0xffffffffu : 0u;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_item_tris(const float4 *__restrict__ it_hi, int64_t m,
+                                                   uint32_t *__restrict__ out) {
+    const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j < m) out[j] = static_cast<uint32_t>(__float_as_int(it_hi[j].w));
+}
+
+// New sorted position of every triangle: clusters in their final order.
+__global__ void __launch_bounds__(256) k_cluster_newpos(const float4 *__restrict__ it_lo, const float4 *__restrict__ it_hi,
+                                                        const uint32_t *__restrict__ start, int64_t m,
+                                                        const int2 *__restrict__ range, int32_t *__restrict__ newpos) {
+    const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= m) return;
+    const int32_t c = __float_as_int(it_lo[j].w);
+    const int32_t t = __float_as_int(it_hi[j].w);
+    const int32_t b = c < 0 ? ~c : range[c].x;
+    for (int32_t k = 0; k < t; ++k) newpos[b + k] = static_cast<int32_t>(start[j]) + k;
+}
+
+__global__ void __launch_bounds__(256) k_permute_leaves(const int32_t *__restrict__ newpos, int64_t n,
+                                                        const uint32_t *__restrict__ idx, const float4 *__restrict__ llo,
+                                                        const float4 *__restrict__ lhi, uint32_t *__restrict__ idx_out,
+                                                        float4 *__restrict__ llo_out, float4 *__restrict__ lhi_out) {
+    const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int32_t q = newpos[p];
+    idx_out[q] = idx[p];
+    llo_out[q] = llo[p];
+    lhi_out[q] = lhi[p];
+}
+
+// LBVH nodes inside clusters: leaf children and ranges in the new order
+// (nodes above the cut are no longer referenced).
+__global__ void __launch_bounds__(256) k_fix_lbvh(int2 *__restrict__ child, int2 *__restrict__ range, int64_t n,
+                                                  const int32_t *__restrict__ newpos) {
+    const int64_t x = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (x >= n - 1) return;
+    int2 ch = child[x];
+    if (ch.x < 0) ch.x = ~newpos[~ch.x];
+    if (ch.y < 0) ch.y = ~newpos[~ch.y];
+    child[x] = ch;
+    const int2 r = range[x];
+    range[x] = make_int2(newpos[r.x], newpos[r.y]);
+}
+
+__device__ __forceinline__ int32_t sah_resolve(int32_t ref, int64_t n, const int32_t *newpos) {
+    if (ref < 0) return ~newpos[~ref];
+    if (ref & kSahRef) return static_cast<int32_t>(n - 1) + (ref & ~kSahRef);
+    return ref;
+}
+
+__global__ void __launch_bounds__(256) k_write_sah(const int2 *__restrict__ sah_child, const int2 *__restrict__ sah_items,
+                                                   int count, int64_t n, const int32_t *__restrict__ newpos,
+                                                   const uint32_t *__restrict__ start, const float4 *__restrict__ it_hi,
+                                                   int2 *__restrict__ child, int2 *__restrict__ range) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    const int2 sc = sah_child[k];
+    const int2 it = sah_items[k];
+    const int64_t x = n - 1 + k;
+    child[x] = make_int2(sah_resolve(sc.x, n, newpos), sah_resolve(sc.y, n, newpos));
+    const int32_t last = it.y - 1;
+    range[x] = make_int2(static_cast<int32_t>(start[it.x]),
+                         static_cast<int32_t>(start[last]) + __float_as_int(it_hi[last].w) - 1);
+}
+
+hipError_t exclusive_scan_u32(uint32_t *data, int64_t n, uint32_t *tile_sums, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    const int64_t blocks = (n + kScanTile - 1) / kScanTile;
+    if (blocks > kScanTile) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_scan_tiles, dim3(static_cast<unsigned>(blocks)), dim3(kScanThreads), 0, s, data, n, tile_sums);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanThreads), 0, s, tile_sums, static_cast<int>(blocks));
+    hipLaunchKernelGGL(k_scan_add, dim3(static_cast<unsigned>(blocks)), dim3(kScanThreads), 0, s, data, n, tile_sums);
+    return hipGetLastError();
+}
+
 struct DevBuf {
     void *p = nullptr;
     ~DevBuf() {
@@ -442,14 +969,12 @@ int32_t build_accel(rt_scene *scene, const rt_build_params &p, rt_accel **out) {
         a->scene_inv[k] = ext > 0.0f ? 1.0f / ext : 0.0f;
     }
 
-    // Triangles in BVH (sorted) order.
-    BUILD_CHECK(hipMalloc(&a->d_tris, n * sizeof(LeafTri)));
-    hipLaunchKernelGGL(k_leaf_tris, dim3(blocks), dim3(256), 0, s, scene->d_abc, scene->d_flags,
-                       (const uint32_t *)b_idx.p, n, a->d_tris);
-    BUILD_CHECK(hipGetLastError());
-
     if (n == 1) {
         // A single triangle: the root is a leaf, no wide node is needed.
+        BUILD_CHECK(hipMalloc(&a->d_tris, n * sizeof(LeafTri)));
+        hipLaunchKernelGGL(k_leaf_tris, dim3(blocks), dim3(256), 0, s, scene->d_abc, scene->d_flags,
+                           (const uint32_t *)b_idx.p, n, a->d_tris);
+        BUILD_CHECK(hipGetLastError());
         a->root = leaf_code(0, 1);
         a->nodes = 0;
         a->leaves = 1;
@@ -460,44 +985,44 @@ int32_t build_accel(rt_scene *scene, const rt_build_params &p, rt_accel **out) {
         return RT_OK;
     }
 
+    const bool refine = p.sah_bins > 0 && n > kSahCluster;
+    // Binary nodes: the LBVH's n - 1, then up to n - 1 SAH nodes.
+    const int64_t bcap = (n - 1) + (refine ? n : 0);
     DevBuf b_child, b_range, b_parent, b_leaf_parent, b_leaf_lo, b_leaf_hi, b_node_lo, b_node_hi;
-    BUILD_CHECK(dalloc<int2>(b_child, n - 1));
-    BUILD_CHECK(dalloc<int2>(b_range, n - 1));
+    BUILD_CHECK(dalloc<int2>(b_child, bcap));
+    BUILD_CHECK(dalloc<int2>(b_range, bcap));
     BUILD_CHECK(dalloc<int32_t>(b_parent, n - 1));
     BUILD_CHECK(dalloc<int32_t>(b_leaf_parent, n));
     BUILD_CHECK(dalloc<float4>(b_leaf_lo, n));
     BUILD_CHECK(dalloc<float4>(b_leaf_hi, n));
-    BUILD_CHECK(dalloc<float4>(b_node_lo, n - 1));
-    BUILD_CHECK(dalloc<float4>(b_node_hi, n - 1));
+    BUILD_CHECK(dalloc<float4>(b_node_lo, bcap));
+    BUILD_CHECK(dalloc<float4>(b_node_hi, bcap));
     const int iblocks = static_cast<int>((n - 1 + 255) / 256);
     hipLaunchKernelGGL(k_karras, dim3(iblocks), dim3(256), 0, s, (const uint32_t *)b_codes.p, n, (int2 *)b_child.p,
                        (int2 *)b_range.p, (int32_t *)b_parent.p, (int32_t *)b_leaf_parent.p);
     hipLaunchKernelGGL(k_gather_leaf_boxes, dim3(blocks), dim3(256), 0, s, (const uint32_t *)b_idx.p, n,
                        (const float4 *)b_aabb_lo.p, (const float4 *)b_aabb_hi.p, (float4 *)b_leaf_lo.p,
                        (float4 *)b_leaf_hi.p);
-    // Level-synchronous refit (see k_bfs_level).
-    {
-        DevBuf b_order, b_cnt;
-        BUILD_CHECK(dalloc<int32_t>(b_order, n - 1));
-        BUILD_CHECK(dalloc<uint32_t>(b_cnt, 1));
+    // Level-synchronous refit from a root (see k_bfs_level).
+    DevBuf b_order, b_cnt;
+    BUILD_CHECK(dalloc<int32_t>(b_order, n - 1));
+    BUILD_CHECK(dalloc<uint32_t>(b_cnt, 1));
+    auto refit = [&](int32_t root) -> hipError_t {
         int32_t *order = (int32_t *)b_order.p;
         uint32_t *cnt = (uint32_t *)b_cnt.p;
-        const int32_t zero = 0;
-        BUILD_CHECK(hipMemcpyAsync(order, &zero, sizeof(int32_t), hipMemcpyHostToDevice, s));
+        hipError_t e = hipMemcpyAsync(order, &root, sizeof(int32_t), hipMemcpyHostToDevice, s);
+        if (e != hipSuccess) return e;
         std::vector<std::pair<uint32_t, uint32_t>> levels;  // (offset, count)
         uint32_t off = 0, count = 1;
         while (count > 0) {
             levels.push_back({off, count});
-            if (static_cast<int>(levels.size()) > kMaxLevels) {
-                cleanup();
-                return fail(ctx, RT_ERR_INVALID, "binary tree deeper than 256 levels");
-            }
-            BUILD_CHECK(hipMemsetAsync(cnt, 0, sizeof(uint32_t), s));
+            if (static_cast<int>(levels.size()) > kMaxLevels) return hipErrorInvalidValue;
+            if ((e = hipMemsetAsync(cnt, 0, sizeof(uint32_t), s)) != hipSuccess) return e;
             hipLaunchKernelGGL(k_bfs_level, dim3((count + 255) / 256), dim3(256), 0, s, order + off,
                                static_cast<int>(count), (const int2 *)b_child.p, order + off + count, cnt);
             uint32_t next = 0;
-            BUILD_CHECK(hipMemcpyAsync(&next, cnt, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-            BUILD_CHECK(hipStreamSynchronize(s));
+            if ((e = hipMemcpyAsync(&next, cnt, sizeof(uint32_t), hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
+            if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
             off += count;
             count = next;
         }
@@ -507,9 +1032,152 @@ int32_t build_accel(rt_scene *scene, const rt_build_params &p, rt_accel **out) {
                                (const float4 *)b_leaf_lo.p, (const float4 *)b_leaf_hi.p, (float4 *)b_node_lo.p,
                                (float4 *)b_node_hi.p);
         }
-        BUILD_CHECK(hipGetLastError());
         a->binary_depth = static_cast<int>(levels.size());
+        return hipGetLastError();
+    };
+    {
+        const hipError_t e = refit(0);
+        if (e == hipErrorInvalidValue) {
+            cleanup();
+            return fail(ctx, RT_ERR_INVALID, "binary tree deeper than 256 levels");
+        }
+        BUILD_CHECK(e);
     }
+    int32_t root = 0;
+    uint32_t *tri_order = (uint32_t *)b_idx.p;  // sorted position -> triangle
+    DevBuf b_idx2, b_llo2, b_lhi2;
+    if (refine) {
+        const int nbins = std::min(std::max(p.sah_bins, 4), kSahBinsMax);
+        DevBuf b_cat, b_pos;
+        BUILD_CHECK(dalloc<int32_t>(b_cat, n));
+        BUILD_CHECK(dalloc<uint32_t>(b_pos, n));
+        int32_t *cat = (int32_t *)b_cat.p;
+        uint32_t *pos = (uint32_t *)b_pos.p;
+        hipLaunchKernelGGL(k_fill_i32, dim3(blocks), dim3(256), 0, s, cat, n, kNoCluster);
+        hipLaunchKernelGGL(k_find_clusters, dim3(iblocks), dim3(256), 0, s, (const int2 *)b_child.p,
+                           (const int2 *)b_range.p, n, cat);
+        hipLaunchKernelGGL(k_cluster_flags, dim3(blocks), dim3(256), 0, s, cat, n, pos);
+        BUILD_CHECK(exclusive_scan_u32(pos, n, (uint32_t *)b_sums.p, s));
+        int32_t last_cat = 0;
+        uint32_t last_pos = 0;
+        BUILD_CHECK(hipMemcpyAsync(&last_cat, cat + n - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        BUILD_CHECK(hipMemcpyAsync(&last_pos, pos + n - 1, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        BUILD_CHECK(hipStreamSynchronize(s));
+        const int64_t m = static_cast<int64_t>(last_pos) + (last_cat != kNoCluster ? 1 : 0);
+        if (m >= 2) {
+            const int mblocks = static_cast<int>((m + 255) / 256);
+            DevBuf b_lo[2], b_hi[2], b_nd[2], b_flags, b_nodes[2], b_splits, b_sc, b_si, b_ctr, b_cb, b_bins;
+            for (int k = 0; k < 2; ++k) {
+                BUILD_CHECK(dalloc<float4>(b_lo[k], m));
+                BUILD_CHECK(dalloc<float4>(b_hi[k], m));
+                BUILD_CHECK(dalloc<int32_t>(b_nd[k], m));
+                BUILD_CHECK(dalloc<SahNode>(b_nodes[k], m));
+            }
+            BUILD_CHECK(dalloc<uint32_t>(b_flags, m));
+            BUILD_CHECK(dalloc<SahSplit>(b_splits, m));
+            BUILD_CHECK(dalloc<int2>(b_sc, m - 1));
+            BUILD_CHECK(dalloc<int2>(b_si, m - 1));
+            BUILD_CHECK(dalloc<uint32_t>(b_ctr, 4));  // [0] bin slots, [1] next-level nodes, [2] SAH nodes
+            const int64_t max_slots = m / (kSahSweepItems + 1) + 1;
+            BUILD_CHECK(dalloc<uint32_t>(b_cb, 6 * max_slots));
+            BUILD_CHECK(dalloc<uint32_t>(b_bins, max_slots * 3 * nbins * kSahBinWords));
+            hipLaunchKernelGGL(k_make_items, dim3(blocks), dim3(256), 0, s, cat, pos, n, (const int2 *)b_range.p,
+                               (const float4 *)b_node_lo.p, (const float4 *)b_node_hi.p, (const float4 *)b_leaf_lo.p,
+                               (const float4 *)b_leaf_hi.p, (float4 *)b_lo[0].p, (float4 *)b_hi[0].p,
+                               (int32_t *)b_nd[0].p);
+            const SahNode root_node{0, static_cast<int32_t>(m), 0, -1};
+            BUILD_CHECK(hipMemcpyAsync(b_nodes[0].p, &root_node, sizeof(SahNode), hipMemcpyHostToDevice, s));
+            uint32_t *ctr = (uint32_t *)b_ctr.p;
+            const uint32_t init_ctr[4] = {0, 0, 1, 0};
+            BUILD_CHECK(hipMemcpyAsync(ctr, init_ctr, sizeof(init_ctr), hipMemcpyHostToDevice, s));
+            int cur = 0;
+            int count = 1;
+            int levels = 0;
+            while (count > 0) {
+                if (++levels > 4096) {
+                    cleanup();
+                    return fail(ctx, RT_ERR_INVALID, "SAH refinement did not converge");
+                }
+                SahNode *nodes = (SahNode *)b_nodes[cur].p;
+                SahNode *next = (SahNode *)b_nodes[cur ^ 1].p;
+                const float4 *ilo = (const float4 *)b_lo[cur].p, *ihi = (const float4 *)b_hi[cur].p;
+                const int32_t *ind = (const int32_t *)b_nd[cur].p;
+                BUILD_CHECK(hipMemsetAsync(ctr, 0, 2 * sizeof(uint32_t), s));
+                hipLaunchKernelGGL(k_sah_classify, dim3((count + 255) / 256), dim3(256), 0, s, nodes, count, ctr);
+                uint32_t nslots = 0;
+                BUILD_CHECK(hipMemcpyAsync(&nslots, ctr, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+                BUILD_CHECK(hipStreamSynchronize(s));
+                uint32_t *cb = (uint32_t *)b_cb.p, *bins = (uint32_t *)b_bins.p;
+                if (nslots > 0) {
+                    const int64_t words = std::max<int64_t>(6 * int64_t(nslots), int64_t(nslots) * 3 * nbins * kSahBinWords);
+                    hipLaunchKernelGGL(k_sah_reset, dim3(static_cast<unsigned>((words + 255) / 256)), dim3(256), 0, s,
+                                       cb, bins, static_cast<int>(nslots), nbins);
+                    hipLaunchKernelGGL(k_sah_cbounds, dim3(mblocks), dim3(256), 0, s, ilo, ihi, ind, m, nodes, cb);
+                    hipLaunchKernelGGL(k_sah_bin, dim3(mblocks), dim3(256), 0, s, ilo, ihi, ind, m, nodes, cb, nbins,
+                                       bins);
+                }
+                hipLaunchKernelGGL(k_sah_split, dim3((count + 63) / 64), dim3(64), 0, s, nodes, count, ilo, ihi, bins,
+                                   nbins,
+                                   (SahSplit *)b_splits.p, next, ctr + 1, ctr + 2, (int2 *)b_sc.p, (int2 *)b_si.p);
+                hipLaunchKernelGGL(k_sah_flags, dim3(mblocks), dim3(256), 0, s, ilo, ihi, ind, m, nodes,
+                                   (const SahSplit *)b_splits.p, cb, nbins, (uint32_t *)b_flags.p);
+                BUILD_CHECK(exclusive_scan_u32((uint32_t *)b_flags.p, m, (uint32_t *)b_sums.p, s));
+                hipLaunchKernelGGL(k_sah_scatter, dim3(mblocks), dim3(256), 0, s, ilo, ihi, ind, m, nodes,
+                                   (const SahSplit *)b_splits.p, cb, nbins, (const uint32_t *)b_flags.p,
+                                   (float4 *)b_lo[cur ^ 1].p, (float4 *)b_hi[cur ^ 1].p, (int32_t *)b_nd[cur ^ 1].p,
+                                   (int2 *)b_sc.p);
+                uint32_t nnext = 0;
+                BUILD_CHECK(hipMemcpyAsync(&nnext, ctr + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+                BUILD_CHECK(hipStreamSynchronize(s));
+                if (nnext > 0)
+                    hipLaunchKernelGGL(k_sah_assign, dim3(nnext), dim3(64), 0, s, next, static_cast<int>(nnext),
+                                       (int32_t *)b_nd[cur ^ 1].p);
+                BUILD_CHECK(hipGetLastError());
+                cur ^= 1;
+                count = static_cast<int>(nnext);
+            }
+            // Triangles in the final cluster order.
+            const float4 *flo = (const float4 *)b_lo[cur].p, *fhi = (const float4 *)b_hi[cur].p;
+            (void)flo;
+            DevBuf b_start, b_newpos;
+            BUILD_CHECK(dalloc<uint32_t>(b_start, m));
+            BUILD_CHECK(dalloc<int32_t>(b_newpos, n));
+            uint32_t *start = (uint32_t *)b_start.p;
+            int32_t *newpos = (int32_t *)b_newpos.p;
+            hipLaunchKernelGGL(k_item_tris, dim3(mblocks), dim3(256), 0, s, fhi, m, start);
+            BUILD_CHECK(exclusive_scan_u32(start, m, (uint32_t *)b_sums.p, s));
+            hipLaunchKernelGGL(k_cluster_newpos, dim3(mblocks), dim3(256), 0, s, (const float4 *)b_lo[cur].p, fhi,
+                               start, m, (const int2 *)b_range.p, newpos);
+            BUILD_CHECK(dalloc<uint32_t>(b_idx2, n));
+            BUILD_CHECK(dalloc<float4>(b_llo2, n));
+            BUILD_CHECK(dalloc<float4>(b_lhi2, n));
+            hipLaunchKernelGGL(k_permute_leaves, dim3(blocks), dim3(256), 0, s, newpos, n, (const uint32_t *)b_idx.p,
+                               (const float4 *)b_leaf_lo.p, (const float4 *)b_leaf_hi.p, (uint32_t *)b_idx2.p,
+                               (float4 *)b_llo2.p, (float4 *)b_lhi2.p);
+            BUILD_CHECK(hipMemcpyAsync(b_leaf_lo.p, b_llo2.p, n * sizeof(float4), hipMemcpyDeviceToDevice, s));
+            BUILD_CHECK(hipMemcpyAsync(b_leaf_hi.p, b_lhi2.p, n * sizeof(float4), hipMemcpyDeviceToDevice, s));
+            hipLaunchKernelGGL(k_fix_lbvh, dim3(iblocks), dim3(256), 0, s, (int2 *)b_child.p, (int2 *)b_range.p, n,
+                               newpos);
+            hipLaunchKernelGGL(k_write_sah, dim3(static_cast<unsigned>((m - 1 + 255) / 256)), dim3(256), 0, s,
+                               (const int2 *)b_sc.p, (const int2 *)b_si.p, static_cast<int>(m - 1), n, newpos, start,
+                               fhi, (int2 *)b_child.p, (int2 *)b_range.p);
+            BUILD_CHECK(hipGetLastError());
+            tri_order = (uint32_t *)b_idx2.p;
+            root = static_cast<int32_t>(n - 1);
+            const hipError_t e = refit(root);
+            if (e == hipErrorInvalidValue) {
+                cleanup();
+                return fail(ctx, RT_ERR_INVALID, "binary tree deeper than 256 levels");
+            }
+            BUILD_CHECK(e);
+        }
+    }
+
+    // Triangles in BVH order.
+    BUILD_CHECK(hipMalloc(&a->d_tris, n * sizeof(LeafTri)));
+    hipLaunchKernelGGL(k_leaf_tris, dim3(blocks), dim3(256), 0, s, scene->d_abc, scene->d_flags, tri_order, n,
+                       a->d_tris);
+    BUILD_CHECK(hipGetLastError());
 
     // Wide nodes: at most n - 1 (each wide node consumes >= 1 binary node).
     const size_t node_bytes = (W == 4) ? sizeof(WideNode<4>) : sizeof(WideNode<8>);
@@ -520,7 +1188,7 @@ int32_t build_accel(rt_scene *scene, const rt_build_params &p, rt_accel **out) {
     BUILD_CHECK(dalloc<int2>(b_qb, max_nodes));
     BUILD_CHECK(dalloc<unsigned long long>(b_counters, 4));
     BUILD_CHECK(hipMemsetAsync(b_counters.p, 0, 4 * sizeof(unsigned long long), s));
-    const int2 root_job = make_int2(0, 0);
+    const int2 root_job = make_int2(root, 0);
     BUILD_CHECK(hipMemcpyAsync(b_qa.p, &root_job, sizeof(int2), hipMemcpyHostToDevice, s));
     const uint32_t one = 1;
     uint32_t *node_counter = reinterpret_cast<uint32_t *>(b_counters.p);

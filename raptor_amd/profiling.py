@@ -90,6 +90,7 @@ def cmd_summarize(args):
         f.write("\n".join(lines) + "\n")
     out = {"kernel": kernel, "kernel_stage": stage, "width": bench["config"]["bvh_width"],
            "lds_stack": bench["config"].get("lds_stack"), "leaf_size": bench["config"]["leaf_size"],
+           "sah_bins": bench["config"].get("sah_bins", 0),
            "bytes_per_launch": round(traffic), "fetch_kib": fetch, "write_kib": write,
            "fetch_scale": fetch_scale, "write_scale": write_scale, "rocprof_avg_us": dom_us}
     with open(args.out + ".json", "w") as f:

@@ -284,7 +284,7 @@ class Batch:
 
 
 class Accel:
-    def __init__(self, scene, width=4, leaf_size=2, sah_bins=0, lds_stack=8):
+    def __init__(self, scene, width=4, leaf_size=2, sah_bins=16, lds_stack=8):
         self.scene = scene
         self.ctx = scene.ctx
         p = rt_build_params(width, leaf_size, sah_bins, lds_stack)

@@ -14,23 +14,24 @@ pytestmark = pytest.mark.gpu
 RAD_TOL = 1e-5
 
 
-def _render_both(rt, O, ctx, w, width=4, leaf=2, stride=1, lds=16, regroup=False):
+def _render_both(rt, O, ctx, w, width=4, leaf=2, stride=1, lds=16, regroup=False, sah=0):
     scene = rt.Scene(ctx, w.abc, w.albedo)
-    accel = rt.Accel(scene, width=width, leaf_size=leaf, lds_stack=lds)
+    accel = rt.Accel(scene, width=width, leaf_size=leaf, lds_stack=lds, sah_bins=sah)
     out = accel.render(w.camera, w.lights, bounce=w.bounce, spp=w.spp, seed=w.seed, regroup=regroup)
     ref = O.OracleScene(w.abc).render(w.albedo, w.camera, w.lights, bounce=w.bounce, spp=w.spp, seed=w.seed,
                                       pixel_stride=stride, nthreads=8)
     return out, ref
 
 
-@pytest.mark.parametrize("width,leaf,lds", [(4, 1, 16), (4, 2, 4), (4, 4, 8), (8, 2, 4), (8, 4, 16), (8, 8, 8)])
-def test_primary_trace_random65k(gpu_ctx, rt, O, width, leaf, lds):
+@pytest.mark.parametrize("width,leaf,lds,sah", [(4, 1, 16, 0), (4, 2, 4, 16), (4, 4, 8, 0), (8, 2, 4, 32), (8, 4, 16, 0),
+                                               (8, 8, 8, 8)])
+def test_primary_trace_random65k(gpu_ctx, rt, O, width, leaf, lds, sah):
     from raptor_amd import workloads
 
     w = workloads.random65k()
     rays = O.primary_rays(w.camera)
     scene = rt.Scene(gpu_ctx, w.abc, w.albedo)
-    accel = rt.Accel(scene, width=width, leaf_size=leaf, lds_stack=lds)
+    accel = rt.Accel(scene, width=width, leaf_size=leaf, lds_stack=lds, sah_bins=sah)
     g = accel.trace_closest(rays)
     r, _ = O.OracleScene(w.abc).trace_closest(rays)
     rtuv = np.stack([r["t"], r["u"], r["v"]], 1)
@@ -55,12 +56,12 @@ def test_render_random65k(gpu_ctx, rt, O):
     assert out["stats"]["rays"][1] == int((ref["prim"] >= 0).sum())
 
 
-@pytest.mark.parametrize("width,lds,regroup", [(4, 16, False), (8, 4, False), (4, 8, True)])
-def test_render_terrain_bounce(gpu_ctx, rt, O, width, lds, regroup):
+@pytest.mark.parametrize("width,lds,regroup,sah", [(4, 16, False, 0), (8, 4, False, 16), (4, 8, True, 16)])
+def test_render_terrain_bounce(gpu_ctx, rt, O, width, lds, regroup, sah):
     from raptor_amd import workloads
 
     w = workloads.terrain(grid=200, x_res=320, y_res=180, bounce=True)
-    out, ref = _render_both(rt, O, gpu_ctx, w, width=width, lds=lds, regroup=regroup)
+    out, ref = _render_both(rt, O, gpu_ctx, w, width=width, lds=lds, regroup=regroup, sah=sah)
     bad, ties = compare_hits(out["primary"], ref["prim"], ref["tuv"])
     assert not bad.any(), f"{bad.sum()} primary ids differ"
     bb, bties = compare_hits(out["bounce"], ref["bounce_prim"], ref["bounce_tuv"])
