@@ -5,14 +5,16 @@ per-triangle albedo and the point lights, over fixed path records, and the
 project's CPU adjoint of it. Only tests/ import this module.
 
 The records come from the oracle's float frame (OracleScene.render with
-records=True): per sample the primary and bounce hit (triangle, shadow-ray
-origin, shading normal, visibility per light). Over them the radiance is
-(rt/raytracer.cc:20-87, rt/materials/phong_shader.cc:21-64 with kd = 255 *
-albedo and ks = ka = 0, rt/light.h:44-59)
+records=True): per sample the path vertices (primary hit, then each diffuse
+bounce hit: triangle, shadow-ray origin, shading normal, visibility per
+light). Over them the radiance is (rt/raytracer.cc:20-87,
+rt/materials/phong_shader.cc:21-64 with kd = 255 * albedo and ks = ka = 0,
+rt/light.h:44-59; the bounces are this project's definition)
 
-    c = sum_l V1 S1 kd1 I1 + (kd1 / 255) * sum_l V2 S2 kd2 I2,
-    S = dot(dir_l, n) (lights with S < 0 skipped), dir_l = (pos_l - o) / len_l,
-    I = rgb_l / 255 / ((len_l d_l)^2 + 1),    pixel = mean over samples,
+    c = sum_k T_k D_k,   T_0 = 1,  T_k = T_{k-1} * kd_{k-1} / 255,
+    D_k = kd_k * sum_l V_kl S_kl I_kl,   S = dot(dir_l, n) (lights with S < 0
+    skipped), dir_l = (pos_l - o) / len_l,  I = rgb_l / 255 / ((len_l d_l)^2 + 1),
+    pixel = mean over samples,
 
 which this module evaluates in float64 so that central finite differences of
 it are accurate to ~1e-9; the adjoint below is checked against them.
@@ -20,9 +22,9 @@ it are accurate to ~1e-9; the adjoint below is checked against them.
 import numpy as np
 
 
-def _direct(o, n, vis, kd, pos, rgb, d):
-    """Direct light at surfaces o (S,3) with normals n, visibility vis (S,L),
-    kd (S,3). Returns radiance (S,3) and the intermediate terms."""
+def _direct(o, n, vis, pos, rgb, d):
+    """Light sums at surfaces o (S,3) with normals n, visibility vis (S,L):
+    returns s (S,3) = sum_l V S I (so D = kd * s) and the terms."""
     v = pos[None, :, :] - o[:, None, :]                 # (S, L, 3)
     ln = np.sqrt((v * v).sum(-1))                         # (S, L)
     dirn = v / ln[..., None]
@@ -31,8 +33,8 @@ def _direct(o, n, vis, kd, pos, rgb, d):
     D = (ln * d[None, :]) ** 2 + 1.0                      # (S, L)
     inten = (rgb[None, :, :] / 255.0) / D[..., None]      # (S, L, 3)
     w = np.where(act, shade, 0.0)                         # (S, L)
-    c = (w[..., None] * inten * kd[:, None, :]).sum(1)    # (S, 3)
-    return c, dict(ln=ln, dirn=dirn, shade=shade, act=act, D=D, inten=inten)
+    s = (w[..., None] * inten).sum(1)                     # (S, 3)
+    return s, dict(ln=ln, dirn=dirn, shade=shade, act=act, D=D, inten=inten)
 
 
 def _as_f64(albedo, lights):
@@ -42,33 +44,48 @@ def _as_f64(albedo, lights):
     return np.asarray(albedo, np.float64), pos, rgb, d
 
 
-def shade(rec, albedo, lights, spp, bounce):
+def _levels(rec, bounces):
+    return min(int(bounces) + 1, rec["prim"].shape[0])
+
+
+def _walk(rec, kd, pos, rgb, d, bounces):
+    """Per level: (mask of samples with a vertex, kd, light sums, terms)."""
+    out = []
+    alive = np.ones(rec["prim"].shape[1], bool)
+    for k in range(_levels(rec, bounces)):
+        pk = rec["prim"][k]
+        alive = alive & (pk >= 0)
+        if not alive.any():
+            break
+        o = rec["o"][k][alive].astype(np.float64)
+        n = rec["n"][k][alive].astype(np.float64)
+        s, t = _direct(o, n, rec["vis"][k][alive], pos, rgb, d)
+        out.append(dict(mask=alive.copy(), prim=pk[alive], kd=kd[pk[alive]], s=s, t=t, n=n))
+    return out
+
+
+def shade(rec, albedo, lights, spp, bounces):
     """Pixel radiance (npix, 3) in float64. lights: [(pos3, rgb3, d), ...]."""
     alb, pos, rgb, d = _as_f64(albedo, lights)
     kd = 255.0 * alb
-    p1, p2 = rec["prim1"], rec["prim2"]
-    ns = p1.shape[0]
+    ns = rec["prim"].shape[1]
     c = np.zeros((ns, 3))
-    m1 = p1 >= 0
-    c1, _ = _direct(rec["o1"][m1].astype(np.float64), rec["n1"][m1].astype(np.float64), rec["vis1"][m1],
-                    kd[p1[m1]], pos, rgb, d)
-    c[m1] = c1
-    if bounce:
-        m2 = m1 & (p2 >= 0)
-        c2, _ = _direct(rec["o2"][m2].astype(np.float64), rec["n2"][m2].astype(np.float64), rec["vis2"][m2],
-                        kd[p2[m2]], pos, rgb, d)
-        c[m2] += (kd[p1[m2]] / 255.0) * c2
+    T = np.ones((ns, 3))
+    for lv in _walk(rec, kd, pos, rgb, d, bounces):
+        m = lv["mask"]
+        c[m] += T[m] * lv["kd"] * lv["s"]
+        T[m] = T[m] * lv["kd"] / 255.0
     return c.reshape(-1, spp, 3).sum(1) / spp
 
 
 def _light_grads(t, w, kd, n, d):
-    """Light gradients of sum_s w_s . c_s for one surface level."""
+    """Light gradients of sum_s w_s . D_s for one level (w already carries T)."""
     act = t["act"]
     shade = np.where(act, t["shade"], 0.0)
     D, ln, dirn, inten = t["D"], t["ln"], t["dirn"], t["inten"]
-    # dc/drgb (0-255 scale) = V S kd / (255 D)
+    # dD/drgb (0-255 scale) = V S kd / (255 D)
     d_rgb = ((shade / (255.0 * D))[..., None] * (w * kd)[:, None, :]).sum(0)
-    # dc/dpos = V kd I [(n - S dir) / len - S (2 d^2 len / D) dir]
+    # dD/dpos = V kd I [(n - S dir) / len - S (2 d^2 len / D) dir]
     T = ((w * kd)[:, None, :] * inten).sum(-1) * act      # (S, L)
     fall = shade * (2.0 * d[None, :] ** 2 * ln) / D
     g = (n[:, None, :] - shade[..., None] * dirn) / ln[..., None] - fall[..., None] * dirn
@@ -76,41 +93,39 @@ def _light_grads(t, w, kd, n, d):
     return d_pos, d_rgb
 
 
-def adjoint(rec, albedo, lights, spp, bounce, grad):
+def adjoint(rec, albedo, lights, spp, bounces, grad):
     """Vector-Jacobian product of shade() for grad = dL/d pixel (npix, 3).
-    Returns (d_albedo (n,3), d_lights (L,6): position xyz then rgb)."""
+    Returns (d_albedo (n,3), d_lights (L,6): position xyz then rgb).
+
+    With R_k = D_k + T'_k R_{k+1} (T'_k = kd_k / 255) the radiance is
+    c = R_0, and dc/dkd_k = T_k (s_k + R_{k+1} / 255)."""
     alb, pos, rgb, d = _as_f64(albedo, lights)
     kd = 255.0 * alb
     nl = pos.shape[0]
     g = np.repeat(np.asarray(grad, np.float64).reshape(-1, 3), spp, axis=0) / spp
-    p1, p2 = rec["prim1"], rec["prim2"]
+    ns = g.shape[0]
+    levels = _walk(rec, kd, pos, rgb, d, bounces)
+    # Throughput T_k into each level.
+    T = np.ones((ns, 3))
+    Ts = []
+    for lv in levels:
+        m = lv["mask"]
+        Ts.append(T[m].copy())
+        T[m] = T[m] * lv["kd"] / 255.0
     d_kd = np.zeros_like(kd)
     d_l = np.zeros((nl, 6))
-    m1 = p1 >= 0
-    o1, n1 = rec["o1"][m1].astype(np.float64), rec["n1"][m1].astype(np.float64)
-    kd1 = kd[p1[m1]]
-    w1 = g[m1]
-    c1, t1 = _direct(o1, n1, rec["vis1"][m1], kd1, pos, rgb, d)
-    s1 = (np.where(t1["act"], t1["shade"], 0.0)[..., None] * t1["inten"]).sum(1)   # (S, 3)
-    dk1 = w1 * s1
-    if bounce:
-        sel = p2[m1] >= 0
-        m2 = m1.copy()
-        m2[m1] = sel
-        o2, n2 = rec["o2"][m2].astype(np.float64), rec["n2"][m2].astype(np.float64)
-        kd2 = kd[p2[m2]]
-        c2, t2 = _direct(o2, n2, rec["vis2"][m2], kd2, pos, rgb, d)
-        dk1[sel] += w1[sel] * c2 / 255.0
-        w2 = w1[sel] * kd1[sel] / 255.0
-        s2 = (np.where(t2["act"], t2["shade"], 0.0)[..., None] * t2["inten"]).sum(1)
-        np.add.at(d_kd, p2[m2], w2 * s2)
-        dp, dr = _light_grads(t2, w2, kd2, n2, d)
+    R_next = np.zeros((ns, 3))   # R_{k+1}, indexed by sample
+    for k in range(len(levels) - 1, -1, -1):
+        lv = levels[k]
+        m = lv["mask"]
+        w = g[m] * Ts[k]
+        np.add.at(d_kd, lv["prim"], w * (lv["s"] + R_next[m] / 255.0))
+        dp, dr = _light_grads(lv["t"], w, lv["kd"], lv["n"], d)
         d_l[:, :3] += dp
         d_l[:, 3:] += dr
-    np.add.at(d_kd, p1[m1], dk1)
-    dp, dr = _light_grads(t1, w1, kd1, n1, d)
-    d_l[:, :3] += dp
-    d_l[:, 3:] += dr
+        R = np.zeros((ns, 3))
+        R[m] = lv["kd"] * lv["s"] + (lv["kd"] / 255.0) * R_next[m]
+        R_next = R
     return 255.0 * d_kd, d_l
 
 

@@ -33,9 +33,8 @@ class oracle_render_params(C.Structure):
 class oracle_render_out(C.Structure):
     _fields_ = [("rgb", C.c_void_p), ("prim", C.c_void_p), ("tuv", C.c_void_p), ("vis", C.c_void_p),
                 ("bounce_prim", C.c_void_p), ("bounce_tuv", C.c_void_p), ("bounce_vis", C.c_void_p),
-                ("counters", C.c_void_p), ("rec_prim1", C.c_void_p), ("rec_prim2", C.c_void_p),
-                ("rec_o1", C.c_void_p), ("rec_n1", C.c_void_p), ("rec_o2", C.c_void_p), ("rec_n2", C.c_void_p),
-                ("rec_vis1", C.c_void_p), ("rec_vis2", C.c_void_p)]
+                ("counters", C.c_void_p), ("rec_levels", C.c_int32), ("rec_prim", C.c_void_p),
+                ("rec_o", C.c_void_p), ("rec_n", C.c_void_p), ("rec_vis", C.c_void_p)]
 
 
 def build():
@@ -150,9 +149,11 @@ class OracleScene:
 
     def render(self, albedo, camera, lights, bounce=False, spp=1, seed=0, pixel_stride=1, nthreads=8, packet=False,
                records=False):
-        """One frame. records=True adds per-sample path records (index
-        p * spp + s): prim1/prim2 (-1: none), o1/o2 (shadow-ray origins),
-        n1/n2 (shading normals), vis1/vis2 (per light, 1 = lit)."""
+        """One frame; bounce is the number of diffuse bounces (True = 1).
+        records=True adds per-sample path records over the bounce + 1 path
+        vertices, level-major (vertex k, sample p * spp + s): prim (L, S)
+        (-1: none), o (L, S, 3) shadow-ray origins, n (L, S, 3) shading
+        normals, vis (L, S, nlights) 1 = lit."""
         npix = camera.x_res * camera.y_res
         nl = len(lights)
         rgb = np.full((npix, 3), np.nan, np.float32)
@@ -165,17 +166,16 @@ class OracleScene:
         cnt = np.zeros(6, np.uint64)
         out = oracle_render_out(_p(rgb), _p(prim), _p(tuv), _p(vis), _p(bprim), _p(btuv), _p(bvis), _p(cnt))
         rec = None
+        nb = int(bounce)
         if records:
             ns = npix * max(1, spp)
-            rec = dict(prim1=np.full(ns, -2, np.int32), prim2=np.full(ns, -2, np.int32),
-                       o1=np.zeros((ns, 3), np.float32), n1=np.zeros((ns, 3), np.float32),
-                       o2=np.zeros((ns, 3), np.float32), n2=np.zeros((ns, 3), np.float32),
-                       vis1=np.zeros((ns, max(1, nl)), np.uint8), vis2=np.zeros((ns, max(1, nl)), np.uint8))
-            out.rec_prim1, out.rec_prim2 = _p(rec["prim1"]), _p(rec["prim2"])
-            out.rec_o1, out.rec_n1 = _p(rec["o1"]), _p(rec["n1"])
-            out.rec_o2, out.rec_n2 = _p(rec["o2"]), _p(rec["n2"])
-            out.rec_vis1, out.rec_vis2 = _p(rec["vis1"]), _p(rec["vis2"])
-        p = oracle_render_params(1 if bounce else 0, spp, seed, pixel_stride, nthreads, 1 if packet else 0)
+            lv = nb + 1
+            rec = dict(prim=np.full((lv, ns), -2, np.int32), o=np.zeros((lv, ns, 3), np.float32),
+                       n=np.zeros((lv, ns, 3), np.float32), vis=np.zeros((lv, ns, max(1, nl)), np.uint8))
+            out.rec_levels = lv
+            out.rec_prim, out.rec_o, out.rec_n, out.rec_vis = (_p(rec["prim"]), _p(rec["o"]), _p(rec["n"]),
+                                                               _p(rec["vis"]))
+        p = oracle_render_params(nb, spp, seed, pixel_stride, nthreads, 1 if packet else 0)
         alb = np.ascontiguousarray(albedo, np.float32)
         rc = LIB.oracle_render(self.h, _p(alb), C.byref(camera_c(camera)), lights_c(lights), nl, C.byref(p),
                                C.byref(out))

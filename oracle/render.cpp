@@ -136,6 +136,7 @@ struct FrameCtx {
     std::vector<PointLight> lights;
     oracle_render_params p;
     oracle_render_out out;
+    int64_t ns = 0;  // samples per frame (records are level-major)
 };
 
 Colour kd_of(const FrameCtx &f, int prim) {
@@ -215,40 +216,51 @@ void render_pixel(const FrameCtx &f, int px, int py, LocalCounters *lc) {
         }
         if (first && f.out.bounce_prim) f.out.bounce_prim[p] = -1;
         Colour c;  // background (0, 0, 0)
-        record_surface(o.rec_prim1, o.rec_o1, o.rec_n1, o.rec_vis1, k, nl, -1, nullptr);
-        record_surface(o.rec_prim2, o.rec_o2, o.rec_n2, o.rec_vis2, k, nl, -1, nullptr);
+        const int levels = f.p.bounce + 1;
+        for (int lv = 0; lv < o.rec_levels; ++lv)
+            record_surface(o.rec_prim, o.rec_o, o.rec_n, o.rec_vis, static_cast<int64_t>(lv) * f.ns + k, nl, -1,
+                           nullptr);
         if (h.d < kMaxDist) {
             const Triangle &tri = f.s->tris[prim];
-            const SurfaceHit sh = surface_at(tri, &r, h.d);
+            SurfaceHit sh = surface_at(tri, &r, h.d);
             const Colour kd1 = kd_of(f, prim);
-            record_surface(o.rec_prim1, o.rec_o1, o.rec_n1, nullptr, k, nl, prim, &sh);
+            if (o.rec_levels > 0) record_surface(o.rec_prim, o.rec_o, o.rec_n, nullptr, k, nl, prim, &sh);
             c = direct_light(f, sh, kd1, (first && f.out.vis) ? f.out.vis + nl * p : nullptr, 1, lc,
-                             o.rec_vis1 ? o.rec_vis1 + nl * k : nullptr);
-            if (f.p.bounce) {
+                             o.rec_levels > 0 && o.rec_vis ? o.rec_vis + nl * k : nullptr);
+            // Diffuse bounces: c += T (.) direct(x_k) with the throughput
+            // T = prod over earlier vertices of kd / 255 (the Lambert
+            // cosine / pdf ratio cancels); one bounce gives
+            // c + (kd1 / 255) * direct(x2).
+            Colour T{kd1.r / 255.0f, kd1.g / 255.0f, kd1.b / 255.0f};
+            for (int bn = 1; bn < levels; ++bn) {
                 Ray br;
                 br.ogn = offset_start_point(sh.dst, sh.geo_norm, 1);
-                br.dir = cosine_direction(bounce_key(f.p.seed, static_cast<uint32_t>(p), static_cast<uint32_t>(s), 1u),
+                br.dir = cosine_direction(bounce_key(f.p.seed, static_cast<uint32_t>(p), static_cast<uint32_t>(s),
+                                                     static_cast<uint32_t>(bn)),
                                           sh.shade_norm);
                 Hit bh;
                 const int bprim = find_nearest_object(*f.s, br, &bh, &lc->trav);
                 ++lc->rays[2];
-                if (first && f.out.bounce_prim) f.out.bounce_prim[p] = bprim;
-                if (first && f.out.bounce_tuv) {
+                if (bn == 1 && first && f.out.bounce_prim) f.out.bounce_prim[p] = bprim;
+                if (bn == 1 && first && f.out.bounce_tuv) {
                     f.out.bounce_tuv[3 * p + 0] = bh.d;
                     f.out.bounce_tuv[3 * p + 1] = bh.u;
                     f.out.bounce_tuv[3 * p + 2] = bh.v;
                 }
-                if (bh.d < kMaxDist) {
-                    const Triangle &btri = f.s->tris[bprim];
-                    const SurfaceHit bsh = surface_at(btri, &br, bh.d);
-                    record_surface(o.rec_prim2, o.rec_o2, o.rec_n2, nullptr, k, nl, bprim, &bsh);
-                    const Colour c2 = direct_light(f, bsh, kd_of(f, bprim),
-                                                   (first && f.out.bounce_vis) ? f.out.bounce_vis + nl * p : nullptr,
-                                                   3, lc, o.rec_vis2 ? o.rec_vis2 + nl * k : nullptr);
-                    c.r = c.r + ((kd1.r / 255.0f) * c2.r);
-                    c.g = c.g + ((kd1.g / 255.0f) * c2.g);
-                    c.b = c.b + ((kd1.b / 255.0f) * c2.b);
-                }
+                if (!(bh.d < kMaxDist)) break;
+                const Triangle &btri = f.s->tris[bprim];
+                sh = surface_at(btri, &br, bh.d);
+                const Colour kdb = kd_of(f, bprim);
+                const int64_t rk = static_cast<int64_t>(bn) * f.ns + k;
+                if (bn < o.rec_levels) record_surface(o.rec_prim, o.rec_o, o.rec_n, nullptr, rk, nl, bprim, &sh);
+                const Colour cb = direct_light(f, sh, kdb,
+                                               (bn == 1 && first && f.out.bounce_vis) ? f.out.bounce_vis + nl * p
+                                                                                      : nullptr,
+                                               3, lc, (bn < o.rec_levels && o.rec_vis) ? o.rec_vis + nl * rk : nullptr);
+                c.r = c.r + (T.r * cb.r);
+                c.g = c.g + (T.g * cb.g);
+                c.b = c.b + (T.b * cb.b);
+                T = Colour{T.r * (kdb.r / 255.0f), T.g * (kdb.g / 255.0f), T.b * (kdb.b / 255.0f)};
             }
         }
         acc.r += c.r;
@@ -280,6 +292,7 @@ int render_frame(const Scene &s, const float *albedo, const oracle_camera &cam, 
     }
     f.p = p;
     f.out = out;
+    f.ns = static_cast<int64_t>(cam.x_res) * cam.y_res * std::max(1, p.spp);
     const int64_t npix = static_cast<int64_t>(f.cam.x_res) * f.cam.y_res;
     const int stride = std::max(1, p.pixel_stride);
     const int nthreads = std::max(1, p.nthreads);

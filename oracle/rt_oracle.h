@@ -31,8 +31,10 @@ typedef struct {
     float d;
 } oracle_light;
 
-/* Render options. bounce: add one cosine-weighted diffuse bounce with direct
- * light at the bounce hit. spp > 1 jitters the sample inside the pixel. */
+/* Render options. bounce: number of cosine-weighted diffuse bounces, each
+ * with direct light at its hit (radiance c = sum_k T_k direct(x_k), T_k the
+ * product of kd / 255 over the earlier vertices). spp > 1 jitters the sample
+ * inside the pixel. */
 typedef struct {
     int32_t bounce;
     int32_t spp;
@@ -55,13 +57,15 @@ typedef struct {
     /* [0] primary, [1] shadow, [2] bounce, [3] bounce-shadow rays,
      * [4] child box tests, [5] triangle tests */
     uint64_t *counters;
-    /* Per-sample path records, index p * spp + s; any may be NULL. prim -1:
-     * no surface. o: the shadow-ray origin (offset hit point), n: the shading
-     * normal, vis: one byte per light (1 = lit). Level 1 is the primary hit,
-     * level 2 the bounce hit. */
-    int32_t *rec_prim1, *rec_prim2;
-    float *rec_o1, *rec_n1, *rec_o2, *rec_n2;
-    uint8_t *rec_vis1, *rec_vis2;
+    /* Per-sample path records of rec_levels path vertices (0 = primary hit,
+     * k = k-th bounce hit), level-major: index level * (pixels * spp) +
+     * p * spp + s. prim -1: no surface. o: the shadow-ray origin (offset hit
+     * point), n: the shading normal, vis: one byte per light (1 = lit).
+     * rec_levels = 0 records nothing. */
+    int32_t rec_levels;
+    int32_t *rec_prim;
+    float *rec_o, *rec_n;
+    uint8_t *rec_vis;
 } oracle_render_out;
 
 oracle_scene *oracle_scene_create(const float *abc, int64_t n, const uint8_t *flags);

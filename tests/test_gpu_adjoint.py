@@ -35,8 +35,8 @@ def setup(gpu_ctx, rt, O):
     g = A.loss_grad(out["rgb"], target).astype(np.float32)
     # Samples whose primary or bounce id differs between GPU and oracle (exact-t ties).
     gpu = accel.render(w.camera, w.lights, bounce=True, spp=w.spp, seed=w.seed)
-    diff = (gpu["primary"]["prim"] != rec["prim1"]) | (gpu["bounce"]["prim"] != rec["prim2"])
-    touched = np.unique(np.concatenate([rec["prim1"][diff], rec["prim2"][diff], gpu["primary"]["prim"][diff],
+    diff = (gpu["primary"]["prim"] != rec["prim"][0]) | (gpu["bounce"]["prim"] != rec["prim"][1])
+    touched = np.unique(np.concatenate([rec["prim"][0][diff], rec["prim"][1][diff], gpu["primary"]["prim"][diff],
                                         gpu["bounce"]["prim"][diff]]))
     touched = touched[touched >= 0]
     g_dev = torch.from_numpy(g.reshape(w.camera.y_res, w.camera.x_res, 3)).cuda()
@@ -100,7 +100,7 @@ def test_gradients_match_fp64_finite_differences(setup):
         return A.loss(A.shade(rec, a, ls, w.spp, True), target)
 
     rng = np.random.default_rng(1)
-    seen = np.unique(np.concatenate([rec["prim1"][rec["prim1"] >= 0], rec["prim2"][rec["prim2"] >= 0]]))
+    seen = np.unique(rec["prim"][rec["prim"] >= 0])
     seen = np.setdiff1d(seen, s["touched"])
     n_alb = 256 - 6 * len(lights)
     picks = list(zip(rng.choice(seen, n_alb, replace=False), rng.integers(0, 3, n_alb)))

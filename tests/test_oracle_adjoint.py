@@ -27,14 +27,14 @@ def test_records_cover_the_frame(frame):
     w, out, _ = frame
     rec = out["records"]
     ns = w.camera.x_res * w.camera.y_res * w.spp
-    assert rec["prim1"].shape == (ns,)
-    assert (rec["prim1"] >= -1).all() and (rec["prim2"] >= -1).all()
+    assert rec["prim"].shape == (2, ns)
+    assert (rec["prim"][0] >= -1).all() and (rec["prim"][1] >= -1).all()
     # The first sample's ids are the frame's primary and bounce ids.
-    assert np.array_equal(rec["prim1"][:: w.spp], out["prim"])
-    assert np.array_equal(rec["prim2"][:: w.spp], out["bounce_prim"])
-    assert (rec["prim1"] >= 0).mean() > 0.5
-    assert (rec["prim2"] >= 0).sum() > 50
-    assert rec["vis1"][rec["prim1"] >= 0].mean() > 0.3
+    assert np.array_equal(rec["prim"][0][:: w.spp], out["prim"])
+    assert np.array_equal(rec["prim"][1][:: w.spp], out["bounce_prim"])
+    assert (rec["prim"][0] >= 0).mean() > 0.5
+    assert (rec["prim"][1] >= 0).sum() > 50
+    assert rec["vis"][0][rec["prim"][0] >= 0].mean() > 0.3
 
 
 def test_float64_shading_reproduces_float_frame(frame):
@@ -45,18 +45,18 @@ def test_float64_shading_reproduces_float_frame(frame):
     assert np.abs(ref).max() > 1.0
 
 
-def _fd_check(w, out, lights, adjoint_fn, n_albedo=40, seed=0):
+def _fd_check(w, out, lights, adjoint_fn, n_albedo=40, seed=0, bounces=1):
     rec = out["records"]
-    target = A.shade(rec, w.meta["target_albedo"], lights, w.spp, True)
+    target = A.shade(rec, w.meta["target_albedo"], lights, w.spp, bounces)
     alb = w.albedo.astype(np.float64)
 
     def loss_of(a, ls):
-        return A.loss(A.shade(rec, a, ls, w.spp, True), target)
+        return A.loss(A.shade(rec, a, ls, w.spp, bounces), target)
 
-    g = A.loss_grad(A.shade(rec, alb, lights, w.spp, True), target)
-    d_alb, d_l = adjoint_fn(rec, alb, lights, w.spp, True, g)
+    g = A.loss_grad(A.shade(rec, alb, lights, w.spp, bounces), target)
+    d_alb, d_l = adjoint_fn(rec, alb, lights, w.spp, bounces, g)
     rng = np.random.default_rng(seed)
-    seen = np.unique(np.concatenate([rec["prim1"][rec["prim1"] >= 0], rec["prim2"][rec["prim2"] >= 0]]))
+    seen = np.unique(rec["prim"][rec["prim"] >= 0])
     picks = [(int(t), int(c)) for t, c in zip(rng.choice(seen, n_albedo), rng.integers(0, 3, n_albedo))]
     got, fd = [], []
     h = 1e-4
@@ -104,3 +104,38 @@ def test_finite_differences_catch_a_dropped_term(frame, monkeypatch):
     got, fd = _fd_check(w, out, lights, A.adjoint, n_albedo=4)
     err = np.abs(got - fd)
     assert (err > 1e-3 * np.abs(fd)).any()
+
+
+@pytest.fixture(scope="module")
+def frame2(O):
+    from raptor_amd import workloads
+
+    w = workloads.diff_terrain(grid=24, x_res=32, y_res=24, spp=2, seed=6, zoom=1.0, amplitude=0.3)
+    out = O.OracleScene(w.abc).render(w.albedo, w.camera, w.lights, bounce=2, spp=w.spp, seed=w.seed, nthreads=4,
+                                      records=True)
+    lights = [(l.pos, l.rgb, l.d) for l in w.lights]
+    return w, out, lights
+
+
+def test_two_bounce_frame(frame2, frame):
+    """Two bounces: the first bounce's vertex is the one-bounce frame's, the
+    float64 shading reproduces the float frame, and the second vertex adds
+    light."""
+    w, out, lights = frame2
+    rec = out["records"]
+    assert rec["prim"].shape[0] == 3
+    assert (rec["prim"][2] >= 0).sum() > 20
+    assert ((rec["prim"][2] >= 0) <= (rec["prim"][1] >= 0)).all()
+    img = A.shade(rec, w.albedo, lights, w.spp, 2)
+    ref = out["rgb"].astype(np.float64)
+    assert np.abs(img - ref).max() <= 2e-6 * max(1.0, np.abs(ref).max())
+    one = A.shade(rec, w.albedo, lights, w.spp, 1)
+    assert (img >= one - 1e-9).all() and (img > one + 1e-6).any()
+
+
+def test_two_bounce_adjoint_matches_finite_differences(frame2):
+    w, out, lights = frame2
+    got, fd = _fd_check(w, out, lights, A.adjoint, bounces=2, seed=3)
+    scale = np.abs(fd).max()
+    err = np.abs(got - fd)
+    assert (err <= 1e-5 * np.abs(fd) + 1e-7 * scale).all(), (err / np.maximum(np.abs(fd), 1e-30)).max()
