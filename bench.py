@@ -213,6 +213,97 @@ def adjoint_section(ctx, reps, grid=1414, res=1024, spp=4):
     }
 
 
+def invrender_section(ctx, steps, n_scenes=16, res=512, spp=2, bounces=2):
+    """Batched inverse rendering: n_scenes scenes (0.5M-16M triangles,
+    terrain / random 50/50), res x res at spp samples, `bounces` diffuse
+    bounces, four lights. Targets are rendered from the generated albedo
+    (ground truth); the albedo starts at 0.5 and `steps` Adam steps (lr 0.02)
+    run, each one persistent path launch (forward + in-lane adjoint) plus the
+    update, replayed from a hipGraph. Also timed: the forward frames alone and
+    the same step as one launch per scene."""
+    import torch
+
+    from raptor_amd import rt, workloads
+
+    t0 = time.perf_counter()
+    ws = workloads.invrender_scenes(n_scenes=n_scenes, res=res)
+    gen_s = time.perf_counter() - t0
+    dev = "cuda:0"
+    scenes, accels, items, truth = [], [], [], []
+    build_ms = 0.0
+    for w in ws:
+        sc = rt.Scene(ctx, w.abc, w.albedo)
+        acc = rt.Accel(sc)
+        build_ms += acc.info()["build_ms"]
+        n = int(w.abc.shape[0])
+        gt = torch.from_numpy(w.albedo).to(dev)
+        alb = gt.clone()
+        shape = (res, res, 3)
+        items.append((acc, w.camera, alb, torch.zeros((n, 3), device=dev), torch.zeros(shape, device=dev),
+                      torch.zeros(shape, device=dev)))
+        scenes.append(sc)
+        accels.append(acc)
+        truth.append(gt)
+        w.abc = None
+    tris = sum(int(t.shape[0]) for t in truth)
+    inv = rt.InvRender(ctx, items, ws[0].lights, spp=spp, bounces=bounces, seed=ws[0].seed, lr=0.02)
+    # Targets from the ground truth.
+    inv.run(0)
+    for it in items:
+        it[4].copy_(it[5])
+    for it in items:
+        it[2].fill_(0.5)
+    _, rays_f, ms_f = inv.run(0)
+    _, rays_fb, ms_fb = inv.run(1)
+    _, _, ms_plain = inv.run(1, plain=True)
+    seen = [it[3].abs().sum(1) > 0 for it in items]  # triangles with a gradient at the start
+    for it in items:
+        it[3].zero_()
+    loss0, _, _ = inv.run(2, steps=1)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    loss1, rays_s, ms_s = inv.run(2, steps=steps - 1)
+    wall = time.perf_counter() - t1
+
+    def rmse(mask_list):
+        num, den = 0.0, 0
+        for it, gt, m in zip(items, truth, mask_list):
+            d = (it[2] - gt)[m] if m is not None else (it[2] - gt)
+            num += float((d.double() ** 2).sum())
+            den += d.numel()
+        return (num / max(den, 1)) ** 0.5
+
+    init_rmse = (sum(float(((0.5 - gt).double() ** 2).sum()) for gt in truth) / (3 * tris)) ** 0.5
+    res_out = {
+        "workload": f"{n_scenes} scenes, {tris} triangles (terrain and random 50/50, 0.5M-16M each), {res}x{res} "
+                    f"at {spp} spp, {bounces} diffuse bounces, 4 point lights; loss sum (I - I_target)^2, targets "
+                    f"from the generated albedo, Adam lr 0.02 from albedo 0.5",
+        "generate_s": round(gen_s, 1),
+        "build_ms_total": round(build_ms, 1),
+        "forward_ms": round(ms_f, 3),
+        "forward_mrays_s": round(rays_f / ms_f / 1e3, 1),
+        "forward_adjoint_ms": round(ms_fb, 3),
+        "forward_adjoint_mrays_s": round(rays_fb / ms_fb / 1e3, 1),
+        "adjoint_over_forward": round((ms_fb - ms_f) / ms_f, 4),
+        "plain_per_scene_forward_adjoint_ms": round(ms_plain, 3),
+        "persistent_speedup": round(ms_plain / ms_fb, 3),
+        "steps": steps,
+        "step_ms": round(ms_s / (steps - 1), 3),
+        "steps_wall_s": round(wall, 2),
+        "loss_first_step": sum(loss0),
+        "loss_last_step": sum(loss1),
+        "albedo_rmse_start": round(init_rmse, 5),
+        "albedo_rmse_all": round(rmse([None] * len(items)), 5),
+        "albedo_rmse_seen": round(rmse(seen), 5),
+        "seen_triangles": int(sum(int(m.sum()) for m in seen)),
+        "albedo_rmse_seen_start": round((sum(float(((0.5 - gt)[m].double() ** 2).sum()) for gt, m in zip(truth, seen))
+                                         / max(1, 3 * sum(int(m.sum()) for m in seen))) ** 0.5, 5),
+        "path_record_bytes": 0,
+    }
+    inv.close()
+    return res_out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -228,6 +319,7 @@ def main():
     ap.add_argument("--batched-scenes", type=int, default=64, help="0 skips the batched workload")
     ap.add_argument("--batched-reps", type=int, default=10)
     ap.add_argument("--adjoint-reps", type=int, default=10, help="0 skips the differentiable workload")
+    ap.add_argument("--invrender-steps", type=int, default=100, help="0 skips batched inverse rendering")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each GPU would trace its own share "
@@ -332,6 +424,9 @@ def main():
     if args.adjoint_reps > 0:
         print("bench: differentiable workload", file=sys.stderr, flush=True)
         line["adjoint"] = adjoint_section(ctx, args.adjoint_reps)
+    if args.invrender_steps > 1:
+        print("bench: batched inverse rendering", file=sys.stderr, flush=True)
+        line["invrender"] = invrender_section(ctx, args.invrender_steps)
     if not args.no_cpu_baseline:
         print("bench: CPU baseline", file=sys.stderr, flush=True)
         cb = cpu_baseline(w, args.cpu_seconds)

@@ -228,6 +228,46 @@ int32_t rt_tape_info(const rt_tape *tape, int32_t *replay, int64_t *record_bytes
 int32_t rt_render_adjoint(rt_tape *tape, const float *d_rgb, float *d_albedo, float *d_lights, float *ms);
 void rt_tape_destroy(rt_tape *tape);
 
+/* ---- Batched inverse rendering (no counterpart in the reference). ----
+ *
+ * Many scenes, each rendered at spp samples per pixel with up to `bounces`
+ * diffuse bounces and the common lights, in one persistent launch: a path
+ * megakernel takes (scene, 8x8 tile) work items from a queue; each lane
+ * traces its pixel's samples, and with a target image computes the gradient
+ * of L = sum (I - target)^2 with respect to every triangle's albedo from the
+ * path vertices it kept (nothing stored in memory, nothing re-traced). A step
+ * of Adam (lr, beta1, beta2, eps; albedo clamped to [0, 1]) follows; one step
+ * is captured as a hipGraph when use_graph is set. All buffers are device
+ * memory: albedo (n x 3, the parameters, updated in place), d_albedo (n x 3,
+ * gradient), target and image (y_res x x_res x 3). Items share a node width. */
+typedef struct {
+    rt_accel *accel;
+    rt_camera cam;
+    float *albedo;
+    float *d_albedo;
+    const float *target;
+    float *image;
+} rt_invrender_item;
+typedef struct {
+    int32_t spp;          /* 1..8 */
+    int32_t bounces;      /* 0..3 */
+    uint64_t seed;
+    float lr, beta1, beta2, eps;
+    int32_t use_graph;
+    int32_t waves_per_cu; /* wavefronts per CU of the persistent launch (0: as many as fit) */
+} rt_invrender_params;
+typedef struct rt_invrender rt_invrender;
+int32_t rt_invrender_create(rt_ctx *ctx, const rt_invrender_item *items, int32_t nitems, const rt_light *lights,
+                            int32_t nlights, const rt_invrender_params *params, rt_invrender **out);
+/* mode 0: render the images; 1: images and the albedo gradients (cleared
+ * first, not applied); 2: `steps` Adam steps (image, gradient, update each).
+ * plain != 0 launches once per scene instead of once for all. loss (nitems,
+ * may be NULL): each scene's L at the last frame; rays: rays traced over the
+ * call; ms: device time of the call. */
+int32_t rt_invrender_run(rt_invrender *r, int32_t mode, int32_t steps, int32_t plain, double *loss, uint64_t *rays,
+                         float *ms);
+void rt_invrender_destroy(rt_invrender *r);
+
 /* Count node visits and triangle tests in later calls (0/1). */
 int32_t rt_set_counting(rt_ctx *ctx, int32_t on);
 

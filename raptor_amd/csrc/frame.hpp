@@ -4,6 +4,7 @@
 // the plan that walks a frame chunk by chunk.
 #pragma once
 
+#include "rt_device.hpp"
 #include "rt_internal.hpp"
 
 namespace rt {
@@ -41,6 +42,78 @@ struct TileMap {
         return (tile << 6) | ((y & 7) << 3) | (x & 7);
     }
 };
+
+// Primary ray of pixel (x, y), sample s (rt/camera.h:127-145,
+// EXACT_NORMALISE): screen = ((X * x_t) + (Y * y_t)) + (Z * t) component by
+// component; samples after the first of a multi-sample pixel are jittered
+// inside it by the counter-based sampler.
+__device__ __forceinline__ rtd::V3 primary_dir(const CamArgs &c, int x, int y, int64_t p, int s, int spp,
+                                               uint64_t seed) {
+    float xf = static_cast<float>(x), yf = static_cast<float>(y);
+    if (spp > 1) {
+        const uint64_t k = rtd::sample_key(seed, static_cast<uint32_t>(p), static_cast<uint32_t>(s), 255u);
+        xf = xf + rtd::unit24(static_cast<uint32_t>(k >> 40));
+        yf = yf + rtd::unit24(static_cast<uint32_t>((k >> 16) & 0xFFFFFFu));
+    }
+    const float x_t = (xf * c.x_inc) + c.x_m;
+    const float y_t = (yf * c.y_inc) + c.y_m;
+    const rtd::V3 scr = rtd::mk(((c.xx * x_t) + (c.yx * y_t)) + (c.zx * c.t), ((c.xy * x_t) + (c.yy * y_t)) + (c.zy * c.t),
+                                ((c.xz * x_t) + (c.yz * y_t)) + (c.zz * c.t));
+    return rtd::normalise_exact(scr);
+}
+
+// rt/ray.cc:15-20 + rt/triangle.h:464-498: move to the hit, re-project onto
+// the triangle's plane, take the normal facing the ray.
+__device__ __forceinline__ void surface_at(rtd::V3 o, rtd::V3 d, float t, rtd::V3 n, rtd::V3 c, rtd::V3 *dst,
+                                           rtd::V3 *geo, rtd::V3 *shade) {
+    rtd::V3 p = rtd::add(o, rtd::mul(d, t));
+    const float denom = rtd::dot(n, d);
+    const float num = rtd::dot(n, rtd::sub(c, p));
+    const float dl = num / denom;
+    p = rtd::add(p, rtd::mul(d, dl));
+    *dst = p;
+    if (denom > 0.0f) {
+        *geo = rtd::neg(n);
+        *shade = rtd::neg(n);
+    } else {
+        *geo = n;
+        *shade = n;
+    }
+}
+
+// Shadow ray to a light (ray::set_up(o, dst), rt/ray.h:44-54): direction and
+// length from the offset origin to the light centre.
+__device__ __forceinline__ void light_dir(rtd::V3 org, float4 light_pos, rtd::V3 *dn, float *len) {
+    const rtd::V3 dv = rtd::sub(rtd::f4v(light_pos), org);
+    *len = rtd::magnitude(dv);
+    *dn = rtd::divs(dv, *len);
+}
+
+struct Rgb {
+    float r, g, b;
+};
+
+// One light's Lambert term, rt/materials/phong_shader.cc:21-64 for kd = 255 *
+// albedo and ks = ka = 0, with rt/light.h:44-59: adds to c unless the light is
+// below the surface.
+__device__ __forceinline__ void lambert_term(Rgb &c, rtd::V3 dn, float len, bool occluded, rtd::V3 nsh, float4 kd,
+                                             float falloff, float4 rgb) {
+    const float shade = rtd::dot(dn, nsh);
+    if (shade < 0.0f) return;
+    const float scr = shade * kd.x, scg = shade * kd.y, scb = shade * kd.z;
+    const float magn = (occluded ? 0.0f : 1.0f) / 1.0f;
+    const float dist = len * falloff;
+    const float d_scale = (dist * dist) + 1.0f;
+    c.r = c.r + ((rgb.x / d_scale) * (scr * magn));
+    c.g = c.g + ((rgb.y / d_scale) * (scg * magn));
+    c.b = c.b + ((rgb.z / d_scale) * (scb * magn));
+}
+
+// kd of a triangle from its albedo: phong_shader(rgb = 255 * albedo, kd = 1.0).
+__device__ __forceinline__ float4 kd_of_albedo(const float *albedo, int32_t prim) {
+    const float *a = albedo + 3 * static_cast<int64_t>(prim);
+    return make_float4((a[0] * 255.0f) * 1.0f, (a[1] * 255.0f) * 1.0f, (a[2] * 255.0f) * 1.0f, 0.0f);
+}
 
 // Path records of one chunk: the rays, hits, queue indices, shading normals,
 // shadow rays and their visibility for the primary and the bounce level. The

@@ -43,18 +43,7 @@ __global__ void __launch_bounds__(256) k_raygen(CamArgs c, TileMap tm, int spp, 
         return;
     }
     const int64_t p = static_cast<int64_t>(y) * c.x_res + x;
-    float xf = static_cast<float>(x), yf = static_cast<float>(y);
-    if (spp > 1) {
-        const uint64_t k = rtd::sample_key(seed, static_cast<uint32_t>(p), static_cast<uint32_t>(s), 255u);
-        xf = xf + rtd::unit24(static_cast<uint32_t>(k >> 40));
-        yf = yf + rtd::unit24(static_cast<uint32_t>((k >> 16) & 0xFFFFFFu));
-    }
-    const float x_t = (xf * c.x_inc) + c.x_m;
-    const float y_t = (yf * c.y_inc) + c.y_m;
-    // ((X * x_t) + (Y * y_t)) + (Z * t), component by component.
-    const rtd::V3 scr = rtd::mk(((c.xx * x_t) + (c.yx * y_t)) + (c.zx * c.t), ((c.xy * x_t) + (c.yy * y_t)) + (c.zy * c.t),
-                                ((c.xz * x_t) + (c.yz * y_t)) + (c.zz * c.t));
-    const rtd::V3 d = rtd::normalise_exact(scr);
+    const rtd::V3 d = primary_dir(c, x, y, p, s, spp, seed);
     r[0] = make_float4(c.px, c.py, c.pz, rtd::kEpsilon);
     r[1] = make_float4(d.x, d.y, d.z, rtd::kMaxDist);
 }
@@ -136,24 +125,6 @@ __global__ void __launch_bounds__(1024) k_scan_blocks(uint32_t *__restrict__ v, 
     if (t == 0) v[n] = total;
 }
 
-// rt/ray.cc:15-20 + rt/triangle.h:464-498.
-__device__ __forceinline__ void surface_at(rtd::V3 o, rtd::V3 d, float t, rtd::V3 n, rtd::V3 c, rtd::V3 *dst,
-                                           rtd::V3 *geo, rtd::V3 *shade) {
-    rtd::V3 p = rtd::add(o, rtd::mul(d, t));
-    const float denom = rtd::dot(n, d);
-    const float num = rtd::dot(n, rtd::sub(c, p));
-    const float dl = num / denom;
-    p = rtd::add(p, rtd::mul(d, dl));
-    *dst = p;
-    if (denom > 0.0f) {
-        *geo = rtd::neg(n);
-        *shade = rtd::neg(n);
-    } else {
-        *geo = n;
-        *shade = n;
-    }
-}
-
 template <int STAGE>
 __global__ void __launch_bounds__(256) k_shade_hit(ShadeArgs a) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -191,11 +162,9 @@ __global__ void __launch_bounds__(256) k_shade_hit(ShadeArgs a) {
     a.hit_sample[q] = sample;
     const rtd::V3 org = rtd::offset_start_point(dst, geo, 1);
     for (int l = 0; l < a.lights.n; ++l) {
-        const float4 lp = a.lights.pos_d[l];
-        // ray::set_up(o, dst): dir = dst - o, length = |dir|, dir /= length.
-        const rtd::V3 dv = rtd::sub(rtd::f4v(lp), org);
-        const float len = rtd::magnitude(dv);
-        const rtd::V3 dn = rtd::divs(dv, len);
+        rtd::V3 dn;
+        float len;
+        light_dir(org, a.lights.pos_d[l], &dn, &len);
         float4 *sr = reinterpret_cast<float4 *>(a.shadow_q) + 2 * (static_cast<int64_t>(q) * a.lights.n + l);
         sr[0] = make_float4(org.x, org.y, org.z, rtd::kEpsilon);
         sr[1] = make_float4(dn.x, dn.y, dn.z, len);
@@ -264,29 +233,18 @@ struct ComposeArgs {
     float *rgb;
 };
 
-struct Col {
-    float r, g, b;
-};
+using Col = Rgb;
 
-// rt/materials/phong_shader.cc:21-64 for kd = 255 * albedo, ks = ka = 0.
+// Direct light at one surface from its shadow rays and their visibility.
 __device__ __forceinline__ Col direct(const rt_ray *shadow, const uint8_t *occ, int64_t q, rtd::V3 nsh, float4 kd,
                                       const LightArgs &L) {
     Col c{0.0f, 0.0f, 0.0f};
     for (int l = 0; l < L.n; ++l) {
         const int64_t k = q * L.n + l;
         const float4 s1 = reinterpret_cast<const float4 *>(shadow)[2 * k + 1];
-        const float shade = rtd::dot(rtd::f4v(s1), nsh);
-        if (shade < 0.0f) continue;
-        const float scr = shade * kd.x, scg = shade * kd.y, scb = shade * kd.z;
-        const float magn = (occ[k] ? 0.0f : 1.0f) / 1.0f;
-        const float dist = s1.w * L.pos_d[l].w;
-        const float d_scale = (dist * dist) + 1.0f;
-        const float4 rgb = L.rgb[l];
-        c.r = c.r + ((rgb.x / d_scale) * (scr * magn));
-        c.g = c.g + ((rgb.y / d_scale) * (scg * magn));
-        c.b = c.b + ((rgb.z / d_scale) * (scb * magn));
+        lambert_term(c, rtd::f4v(s1), s1.w, occ[k] != 0, nsh, kd, L.pos_d[l].w, L.rgb[l]);
     }
-    c.r = c.r + 0.0f;
+    c.r = c.r + 0.0f;  // + ka (zero)
     c.g = c.g + 0.0f;
     c.b = c.b + 0.0f;
     return c;

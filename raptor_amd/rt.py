@@ -32,6 +32,7 @@ EXPORTS = [
     "rt_trace_occluded", "rt_render", "rt_set_counting", "rt_gen_random_triangles", "rt_gen_terrain",
     "rt_batch_create", "rt_batch_run", "rt_batch_destroy", "rt_batch_autotune", "rt_gen_surface_rays",
     "rt_scene_set_albedo", "rt_render_tape", "rt_tape_info", "rt_render_adjoint", "rt_tape_destroy",
+    "rt_invrender_create", "rt_invrender_run", "rt_invrender_destroy",
 ]
 
 
@@ -64,6 +65,17 @@ class rt_batch_item(C.Structure):
 
 class rt_batch_params(C.Structure):
     _fields_ = [("lds_stack", C.c_int32), ("waves_per_cu", C.c_int32), ("grab", C.c_int32), ("use_graph", C.c_int32)]
+
+
+class rt_invrender_item(C.Structure):
+    _fields_ = [("accel", C.c_void_p), ("cam", rt_camera), ("albedo", C.c_void_p), ("d_albedo", C.c_void_p),
+                ("target", C.c_void_p), ("image", C.c_void_p)]
+
+
+class rt_invrender_params(C.Structure):
+    _fields_ = [("spp", C.c_int32), ("bounces", C.c_int32), ("seed", C.c_uint64), ("lr", C.c_float),
+                ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float), ("use_graph", C.c_int32),
+                ("waves_per_cu", C.c_int32)]
 
 
 class rt_accel_info(C.Structure):
@@ -111,6 +123,11 @@ def load_library(path=LIB_PATH):
         "rt_tape_info": (i32, [vp, C.POINTER(i32), C.POINTER(i64), C.POINTER(i64)]),
         "rt_render_adjoint": (i32, [vp, fp, fp, fp, C.POINTER(C.c_float)]),
         "rt_tape_destroy": (None, [vp]),
+        "rt_invrender_create": (i32, [vp, C.POINTER(rt_invrender_item), i32, C.POINTER(rt_light), i32,
+                                      C.POINTER(rt_invrender_params), C.POINTER(vp)]),
+        "rt_invrender_run": (i32, [vp, i32, i32, i32, C.POINTER(C.c_double), C.POINTER(C.c_uint64),
+                                   C.POINTER(C.c_float)]),
+        "rt_invrender_destroy": (None, [vp]),
         "rt_gen_random_triangles": (i64, [u64, i64, C.c_float, C.c_float, C.c_float, fp, fp]),
         "rt_gen_terrain": (i64, [i32, C.c_float, C.c_float, C.c_float, i32, i32, u64, C.c_float, C.c_float, fp, fp]),
     }
@@ -401,6 +418,49 @@ class Tape:
     def close(self):
         if self.h:
             _LIB.rt_tape_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class InvRender:
+    """Batched inverse rendering (see rt_invrender_create). scenes: list of
+    (accel, camera, albedo, d_albedo, target, image) with device tensors."""
+
+    def __init__(self, ctx, scenes, lights, spp=2, bounces=2, seed=0, lr=0.02, beta1=0.9, beta2=0.999, eps=1e-8,
+                 use_graph=True, waves_per_cu=0):
+        self.ctx = ctx
+        self._keep = scenes
+        self.n = len(scenes)
+        items = []
+        for accel, camera, albedo, d_albedo, target, image in scenes:
+            items.append(rt_invrender_item(accel.h.value, camera.to_c(), _ptr(albedo), _ptr(d_albedo),
+                                           _ptr(target) if target is not None else None, _ptr(image)))
+        self.items = (rt_invrender_item * self.n)(*items)
+        larr = (rt_light * max(1, len(lights)))(*[l.to_c() for l in lights])
+        self.params = rt_invrender_params(spp, bounces, seed, lr, beta1, beta2, eps, 1 if use_graph else 0,
+                                          waves_per_cu)
+        h = C.c_void_p()
+        ctx.check(_LIB.rt_invrender_create(ctx.h, self.items, self.n, larr, len(lights), C.byref(self.params),
+                                           C.byref(h)), "rt_invrender_create")
+        self.h = h
+
+    def run(self, mode, steps=1, plain=False):
+        """mode 0 render, 1 render + gradients, 2 Adam steps; returns (loss per scene, rays, ms)."""
+        loss = (C.c_double * self.n)()
+        rays = C.c_uint64()
+        ms = C.c_float()
+        self.ctx.check(_LIB.rt_invrender_run(self.h, mode, steps, 1 if plain else 0, loss, C.byref(rays),
+                                             C.byref(ms)), "rt_invrender_run")
+        return list(loss), rays.value, ms.value
+
+    def close(self):
+        if self.h:
+            _LIB.rt_invrender_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -160,3 +160,26 @@ def batched(n_scenes=64, tri_lo=1e3, tri_hi=2e6, side_lo=64, side_hi=1024, seed=
             kind = "random"
         out.append(BatchScene(kind, abc, alb, side * side, 5000 + s))
     return out
+
+
+def invrender_scenes(n_scenes=16, tri_lo=5e5, tri_hi=1.6e7, res=512, seed=21):
+    """Inverse-rendering batch: triangle counts log-uniform in [tri_lo,
+    tri_hi], terrain (grid ceil(sqrt(N/2)) + 1, relief as the 1M terrain) or
+    random triangles 50/50, each seen res x res by the terrain camera under
+    the four ring lights of diff_terrain. The generated albedo is the ground
+    truth that inverse rendering should recover."""
+    rng = np.random.default_rng(seed)
+    base = diff_terrain(grid=16, x_res=res, y_res=res, spp=2, seed=seed)
+    out = []
+    for s in range(n_scenes):
+        n = int(np.exp(rng.uniform(np.log(tri_lo), np.log(tri_hi))))
+        if rng.uniform() < 0.5:
+            g = int(np.ceil(np.sqrt(n / 2.0))) + 1
+            abc, alb = rt.gen_terrain(g, 0.15, 0.02 * 708.0 / g, 0.5, 8, 57 + s, 300 + s, 0.2, 0.8)
+            kind = "terrain"
+        else:
+            abc, alb = rt.gen_random_triangles(2000 + s, n, 0.01, 0.2, 0.8)
+            kind = "random"
+        out.append(Workload(f"{kind}{abc.shape[0]}", abc, alb, base.camera, base.lights, bounce=True, spp=2,
+                            seed=seed, meta=dict(kind=kind, triangles=int(abc.shape[0]))))
+    return out
