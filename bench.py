@@ -219,8 +219,8 @@ def invrender_section(ctx, steps, n_scenes=16, res=512, spp=2, bounces=2):
     bounces, four lights. Targets are rendered from the generated albedo
     (ground truth); the albedo starts at 0.5 and `steps` Adam steps (lr 0.02)
     run, each one persistent path launch (forward + in-lane adjoint) plus the
-    update, replayed from a hipGraph. Also timed: the forward frames alone and
-    the same step as one launch per scene."""
+    update, replayed from a hipGraph. Also timed (median of 5): the forward
+    frames alone, forward + adjoint, and the same as one launch per scene."""
     import torch
 
     from raptor_amd import rt, workloads
@@ -253,9 +253,14 @@ def invrender_section(ctx, steps, n_scenes=16, res=512, spp=2, bounces=2):
         it[4].copy_(it[5])
     for it in items:
         it[2].fill_(0.5)
-    _, rays_f, ms_f = inv.run(0)
-    _, rays_fb, ms_fb = inv.run(1)
-    _, _, ms_plain = inv.run(1, plain=True)
+
+    def median_run(mode, plain=False, reps=5):
+        runs = [inv.run(mode, plain=plain) for _ in range(reps)]
+        return runs[0][1], sorted(r[2] for r in runs)[reps // 2]
+
+    rays_f, ms_f = median_run(0)
+    rays_fb, ms_fb = median_run(1)
+    _, ms_plain = median_run(1, plain=True)
     seen = [it[3].abs().sum(1) > 0 for it in items]  # triangles with a gradient at the start
     for it in items:
         it[3].zero_()

@@ -68,7 +68,7 @@ def test_tape_forward_equals_render(setup):
     ref = torch.empty_like(torch.from_numpy(img)).cuda()
     s["accel"].render_device(w.camera, w.lights, ref, bounce=True, spp=w.spp, seed=w.seed)
     assert np.array_equal(img.view(np.int32), ref.cpu().numpy().view(np.int32))
-    assert s["n_diff"] < 0.002 * s["rec"]["prim1"].size
+    assert s["n_diff"] < 0.002 * s["rec"]["prim"][0].size
 
 
 def test_adjoint_matches_cpu_adjoint(setup):
