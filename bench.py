@@ -7,9 +7,10 @@ diffuse bounce per hit and one shadow ray per bounce hit, FP32. The scene and
 its BVH are resident in HBM before the timed region; the BVH build time is
 reported separately. value = rays traced (counted on the device) / wall time.
 
-The CPU baseline is the oracle's restatement of the reference's scalar tracer
-(the reference's own algorithm: AAC BVH, scalar traversal, same shading) on the
-host cores of the same machine, timed on a bounded sample of the same frame.
+The CPU baseline is the oracle's restatement of the reference's own CPU
+program for this frame, the SSE packet tracer (SIMD_PACKET_TRACING) over the
+reference's AAC BVH, on the host cores of the same machine, timed on a bounded
+sample of whole frames; the scalar tracer is timed beside it.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -34,7 +35,25 @@ def algorithmic_bytes(width, rays, node_visits, tri_tests, closest):
     return rays * (32 + (16 if closest else 1)) + node_visits * width * 28 + tri_tests * 40
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(w, target_s):
+    """The reference's own CPU program for this frame, restated in the oracle:
+    the SSE packet tracer (SIMD_PACKET_TRACING: 8x8 tiles of 16 four-ray
+    packets, packet traversal and rcp-based Moller-Trumbore, shadow rays
+    packed four at a time) over the reference's AAC BVH, (b) on all host
+    cores with a tile work queue for about target_s seconds of whole frames
+    and (a) on one core over a sample of tiles. The scalar IEEE tracer (the
+    parity oracle) is timed beside it for one frame on all cores."""
     import oracle.oracle as O
 
     cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
@@ -42,36 +61,37 @@ def cpu_baseline(w, target_s):
     t0 = time.perf_counter()
     scene = O.OracleScene(w.abc)
     build_s = time.perf_counter() - t0
-    npix = w.camera.x_res * w.camera.y_res
 
-    def run(stride, threads):
+    def run(stride, threads, packet):
         t = time.perf_counter()
         r = scene.render(w.albedo, w.camera, w.lights, bounce=w.bounce, seed=w.seed, pixel_stride=stride,
-                         nthreads=threads)
-        dt = time.perf_counter() - t
-        return int(r["counters"][:4].sum()), dt
+                         nthreads=threads, packet=packet)
+        return int(r["counters"][:4].sum()), time.perf_counter() - t
 
-    # Whole frames on all cores until about target_s seconds have passed.
     total_rays, total_s, frames = 0, 0.0, 0
     while total_s < target_s or frames == 0:
-        rays, dt = run(1, cores)
+        rays, dt = run(1, cores, True)
         total_rays += rays
         total_s += dt
         frames += 1
-    # One core, as the reference ships (its TBB path does not compile): a
-    # sparse sample of the same frame sized for about a quarter of target_s.
     rate1 = total_rays / total_s / cores
-    stride1 = max(1, int(total_rays / frames / max(rate1 * target_s / 4, 1.0)))
-    rays1, dt1 = run(stride1, 1)
+    tiles = ((w.camera.x_res + 7) // 8) * ((w.camera.y_res + 7) // 8)
+    per_tile = total_rays / frames / tiles
+    stride1 = max(1, int(tiles * per_tile / max(rate1 * target_s / 4, 1.0)))
+    rays1, dt1 = run(stride1, 1, True)
+    srays, sdt = run(1, cores, False)
     return {
         "value": total_rays / total_s / 1e6,
         "unit": "Mrays/s",
         "cores": cores,
         "kind": "port",
-        "sample": f"{frames} whole 1920x1080 frames ({total_rays} rays, {total_s:.1f} s) on {cores} threads; "
-                  f"oracle's scalar IEEE tracer over the reference's AAC BVH (built in {build_s:.2f} s)",
+        "sample": f"{frames} whole 1920x1080 frames ({total_rays} rays, {total_s:.1f} s) on {cores} threads "
+                  f"({cpu_model()}); the oracle's restatement of the reference's SSE packet tracer over the "
+                  f"reference's AAC BVH (built in {build_s:.2f} s)",
         "single_thread_value": rays1 / dt1 / 1e6,
-        "single_thread_sample": f"every {stride1}th pixel ({rays1} rays, {dt1:.1f} s) on 1 thread",
+        "single_thread_sample": f"every {stride1}th 8x8 tile ({rays1} rays, {dt1:.1f} s) on 1 thread",
+        "scalar_value": srays / sdt / 1e6,
+        "scalar_sample": f"one frame of the scalar IEEE tracer on {cores} threads ({srays} rays, {sdt:.1f} s)",
     }
 
 

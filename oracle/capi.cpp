@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "oracle.hpp"
+#include "packet.hpp"
 #include "rt_oracle.h"
 #include "sampling.hpp"
 
@@ -178,7 +179,9 @@ void oracle_trace_occluded(const oracle_scene *s, const float *rays, int64_t n, 
 int32_t oracle_render(const oracle_scene *s, const float *albedo, const oracle_camera *cam,
                       const oracle_light *lights, int32_t nlights, const oracle_render_params *p,
                       oracle_render_out *out) {
-    if (p->packet) return -1;  // packet path: not available in this build of the oracle
+    // The packet frame: one sample per pixel, at most one bounce (the
+    // reference's packet camera has no jitter); pixel_stride strides tiles.
+    if (p->packet && (p->spp > 1 || p->bounce > 1)) return -1;
     if (nlights < 0 || nlights > 8) return -2;
     return render_frame(s->s, albedo, *cam, lights, nlights, *p, *out);
 }
@@ -263,6 +266,78 @@ void oracle_primary_ray(const oracle_camera *cam, int32_t x, int32_t y, float *o
     for (int k = 0; k < 3; ++k) {
         ogn[k] = cam->pos[k];
         dir[k] = d[k];
+    }
+}
+
+}  // extern "C"
+
+extern "C" {
+
+// Packet slab test of one box for four rays: ogn4 / i_rd4 are x[4], y[4], z[4].
+void oracle_packet_slab(const float *low, const float *high, const float *ogn4, const float *i_rd4, float *out4) {
+    oracle::BvhNode n;
+    n.low = oracle::V3(low[0], low[1], low[2]);
+    n.high = oracle::V3(high[0], high[1], high[2]);
+    oracle::PacketRay r;
+    r.ox = _mm_loadu_ps(ogn4);
+    r.oy = _mm_loadu_ps(ogn4 + 4);
+    r.oz = _mm_loadu_ps(ogn4 + 8);
+    r.dx = r.dy = r.dz = _mm_setzero_ps();
+    const __m128 i_rd[3] = {_mm_loadu_ps(i_rd4), _mm_loadu_ps(i_rd4 + 4), _mm_loadu_ps(i_rd4 + 8)};
+    _mm_storeu_ps(out4, oracle::packet_slab(n, r, i_rd));
+}
+
+void oracle_packet_inverse(const float *x4, float *out4) { _mm_storeu_ps(out4, oracle::packet_inverse(_mm_loadu_ps(x4))); }
+void oracle_packet_inverse_sqrt(const float *x4, float *out4) {
+    _mm_storeu_ps(out4, oracle::packet_inverse_sqrt(_mm_loadu_ps(x4)));
+}
+
+static oracle::PacketRay load_packet(const float *rays) {
+    // four consecutive rays of 8 floats: o.xyz, tmin, d.xyz, tmax
+    oracle::PacketRay p;
+    p.ox = _mm_setr_ps(rays[0], rays[8], rays[16], rays[24]);
+    p.oy = _mm_setr_ps(rays[1], rays[9], rays[17], rays[25]);
+    p.oz = _mm_setr_ps(rays[2], rays[10], rays[18], rays[26]);
+    p.dx = _mm_setr_ps(rays[4], rays[12], rays[20], rays[28]);
+    p.dy = _mm_setr_ps(rays[5], rays[13], rays[21], rays[29]);
+    p.dz = _mm_setr_ps(rays[6], rays[14], rays[22], rays[30]);
+    return p;
+}
+
+// Packets of four consecutive rays (n a multiple of 4) through the packet
+// traversal (use_bvh) or packet brute force.
+void oracle_packet_trace_closest(const oracle_scene *s, const float *rays, int64_t n, int32_t *prim, float *tuv,
+                                 int32_t use_bvh) {
+    for (int64_t i = 0; i + 4 <= n; i += 4) {
+        const oracle::PacketRay p = load_packet(rays + 8 * i);
+        oracle::PacketHit h;
+        if (use_bvh)
+            oracle::packet_nearest(s->s, p, &h, nullptr);
+        else
+            oracle::packet_brute_nearest(s->s, p, &h);
+        alignas(16) float d[4], u[4], v[4];
+        alignas(16) int id[4];
+        _mm_store_ps(d, h.d);
+        _mm_store_ps(u, h.u);
+        _mm_store_ps(v, h.v);
+        _mm_store_si128(reinterpret_cast<__m128i *>(id), h.id);
+        for (int k = 0; k < 4; ++k) {
+            prim[i + k] = d[k] < oracle::kMaxDist ? id[k] : -1;
+            tuv[3 * (i + k) + 0] = d[k];
+            tuv[3 * (i + k) + 1] = u[k];
+            tuv[3 * (i + k) + 2] = v[k];
+        }
+    }
+}
+
+void oracle_packet_trace_occluded(const oracle_scene *s, const float *rays, int64_t n, uint8_t *occ, int32_t use_bvh) {
+    for (int64_t i = 0; i + 4 <= n; i += 4) {
+        const oracle::PacketRay p = load_packet(rays + 8 * i);
+        const float *r = rays + 8 * i;
+        const __m128 t = _mm_setr_ps(r[7], r[15], r[23], r[31]);
+        const __m128 c = use_bvh ? oracle::packet_nearer(s->s, p, t, nullptr) : oracle::packet_brute_nearer(s->s, p, t);
+        const int m = _mm_movemask_ps(c);
+        for (int k = 0; k < 4; ++k) occ[i + k] = (m >> k) & 1;
     }
 }
 

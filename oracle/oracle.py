@@ -71,6 +71,11 @@ def _load():
         "oracle_pcg32_stream": (u64, [u64, u64, i32, vp]),
         "oracle_splitmix64_stream": (u64, [u64, i32, vp]),
         "oracle_primary_ray": (None, [C.POINTER(oracle_camera), i32, i32, vp, vp]),
+        "oracle_packet_slab": (None, [vp, vp, vp, vp, vp]),
+        "oracle_packet_inverse": (None, [vp, vp]),
+        "oracle_packet_inverse_sqrt": (None, [vp, vp]),
+        "oracle_packet_trace_closest": (None, [vp, vp, i64, vp, vp, i32]),
+        "oracle_packet_trace_occluded": (None, [vp, vp, i64, vp, i32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -138,6 +143,24 @@ class OracleScene:
         out = np.empty(n, HIT_DTYPE)
         out["t"], out["u"], out["v"], out["prim"] = tuv[:, 0], tuv[:, 1], tuv[:, 2], prim
         return out, cnt
+
+    def packet_trace_closest(self, rays, use_bvh=True):
+        """The SSE packet path on packets of four consecutive rays (n % 4 == 0)."""
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        n = rays.shape[0] - rays.shape[0] % 4
+        prim = np.full(rays.shape[0], -2, np.int32)
+        tuv = np.zeros((rays.shape[0], 3), np.float32)
+        LIB.oracle_packet_trace_closest(self.h, _p(rays), n, _p(prim), _p(tuv), 1 if use_bvh else 0)
+        out = np.empty(rays.shape[0], HIT_DTYPE)
+        out["t"], out["u"], out["v"], out["prim"] = tuv[:, 0], tuv[:, 1], tuv[:, 2], prim
+        return out
+
+    def packet_trace_occluded(self, rays, use_bvh=True):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        n = rays.shape[0] - rays.shape[0] % 4
+        occ = np.zeros(rays.shape[0], np.uint8)
+        LIB.oracle_packet_trace_occluded(self.h, _p(rays), n, _p(occ), 1 if use_bvh else 0)
+        return occ
 
     def trace_occluded(self, rays, use_bvh=True, nthreads=8):
         rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
@@ -207,6 +230,23 @@ def intersect(abc9, ogn, d):
     a, o, dd = np.asarray(abc9, np.float32), np.asarray(ogn, np.float32), np.asarray(d, np.float32)
     hit = LIB.oracle_intersect(_p(a), _p(o), _p(dd), _p(tuv))
     return bool(hit), tuv
+
+
+def packet_slab(low, high, ogn4, i_rd4):
+    """ogn4, i_rd4: (4, 3) per-ray origins and inverse directions."""
+    lo, hi = f32a(*low), f32a(*high)
+    o = np.ascontiguousarray(np.asarray(ogn4, np.float32).T)
+    i = np.ascontiguousarray(np.asarray(i_rd4, np.float32).T)
+    out = np.empty(4, np.float32)
+    LIB.oracle_packet_slab(_p(lo), _p(hi), _p(o), _p(i), _p(out))
+    return out
+
+
+def packet_inverse(x4, sqrt=False):
+    x = np.ascontiguousarray(x4, np.float32)
+    out = np.empty(4, np.float32)
+    (LIB.oracle_packet_inverse_sqrt if sqrt else LIB.oracle_packet_inverse)(_p(x), _p(out))
+    return out
 
 
 def slab(low, high, ogn, i_rd):

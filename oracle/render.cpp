@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "oracle.hpp"
+#include "packet.hpp"
 #include "rt_oracle.h"
 #include "sampling.hpp"
 
@@ -275,6 +276,298 @@ void render_pixel(const FrameCtx &f, int px, int py, LocalCounters *lc) {
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// The SSE packet frame (#ifdef SIMD_PACKET_TRACING, MAXIMUM_PACKET_SIZE 16):
+// one 8x8 tile is 16 packets of 2x2 pixels, rt/raytracer.cc:464-483.
+
+constexpr int kPackets = 16;
+constexpr int kTileRays = kPackets * 4;
+
+// rt/packet_ray.h:255-293 (packet -> tile offset) and :204-250 (ray ->
+// pixel), built the same way: interleaved address bits.
+struct PacketLuts {
+    int xoff[kPackets], yoff[kPackets];
+    int ray_to_pixel[kTileRays];
+    PacketLuts() {
+        for (int i = 0; i < kPackets; ++i) {
+            unsigned x = 0, y = 0, addr = static_cast<unsigned>(i) << 1, mask = 0x2;
+            for (int j = 0; j < 2; ++j) {
+                x += addr & mask;
+                addr >>= 1;
+                y += addr & mask;
+                mask <<= 1;
+            }
+            xoff[i] = static_cast<int>(x);
+            yoff[i] = static_cast<int>(y);
+        }
+        for (int i = 0; i < kTileRays; ++i) {
+            unsigned x = 0, y = 0, addr = static_cast<unsigned>(i), mask = 0x1;
+            for (int j = 0; j < 8; ++j) {
+                x += addr & mask;
+                addr >>= 1;
+                y += addr & mask;
+                mask <<= 1;
+            }
+            ray_to_pixel[i] = static_cast<int>(y * 8 + x);
+        }
+    }
+};
+const PacketLuts &luts() {
+    static const PacketLuts l;
+    return l;
+}
+
+float lane(__m128 v, int k) {
+    alignas(16) float f[4];
+    _mm_store_ps(f, v);
+    return f[k];
+}
+
+Ray lane_ray(const PacketRay &p, int k) {
+    Ray r;
+    r.ogn = V3(lane(p.ox, k), lane(p.oy, k), lane(p.oz, k));
+    r.dir = V3(lane(p.dx, k), lane(p.dy, k), lane(p.dz, k));
+    return r;
+}
+
+// packet_ray::pack (rt/packet_ray.h:71-92): four rays, their lengths.
+PacketRay pack4(const Ray *const *rr, __m128 *t) {
+    PacketRay p;
+    p.ox = _mm_setr_ps(rr[0]->ogn.x, rr[1]->ogn.x, rr[2]->ogn.x, rr[3]->ogn.x);
+    p.oy = _mm_setr_ps(rr[0]->ogn.y, rr[1]->ogn.y, rr[2]->ogn.y, rr[3]->ogn.y);
+    p.oz = _mm_setr_ps(rr[0]->ogn.z, rr[1]->ogn.z, rr[2]->ogn.z, rr[3]->ogn.z);
+    p.dx = _mm_setr_ps(rr[0]->dir.x, rr[1]->dir.x, rr[2]->dir.x, rr[3]->dir.x);
+    p.dy = _mm_setr_ps(rr[0]->dir.y, rr[1]->dir.y, rr[2]->dir.y, rr[3]->dir.y);
+    p.dz = _mm_setr_ps(rr[0]->dir.z, rr[1]->dir.z, rr[2]->dir.z, rr[3]->dir.z);
+    if (t) *t = _mm_setr_ps(rr[0]->length, rr[1]->length, rr[2]->length, rr[3]->length);
+    return p;
+}
+
+// Coherency test of rt/raytracer.cc:172-200 and :93-121: a packet whose
+// lanes disagree in the sign of a direction component goes ray by ray
+// (size 0); the packets are "coherent" when there are more than four and
+// all share packet 0's signs.
+bool coherency(const PacketRay *r, int s, int *size) {
+    bool coherent = s > 4;
+    const int px = _mm_movemask_ps(r[0].dx), py = _mm_movemask_ps(r[0].dy), pz = _mm_movemask_ps(r[0].dz);
+    for (int i = 0; i < s; ++i) {
+        const int x = _mm_movemask_ps(r[i].dx), y = _mm_movemask_ps(r[i].dy), z = _mm_movemask_ps(r[i].dz);
+        if ((x != 0 && x != 0xf) || (y != 0 && y != 0xf) || (z != 0 && z != 0xf)) {
+            coherent = false;
+            size[i] = 0;
+        } else if (x != px || y != py || z != pz) {
+            coherent = false;
+            size[i] = 1;
+        } else {
+            size[i] = 1;
+        }
+    }
+    return coherent;
+}
+
+// rt/raytracer.cc:168-231: closest hits of s packets. The coherent case
+// (frustum wrapper, ssd/bvh.cc:107-117 under #if 1) is a per-packet loop.
+void engine_closest(const Scene &sc, const PacketRay *r, int s, int *ids, Hit *hits, Counters *c) {
+    int size[kPackets];
+    const bool coherent = coherency(r, s, size);
+    for (int i = 0; i < s; ++i) {
+        if (!coherent && size[i] == 0) {
+            for (int k = 0; k < 4; ++k) {
+                const Ray ray = lane_ray(r[i], k);
+                Hit h;
+                ids[4 * i + k] = find_nearest_object(sc, ray, &h, c);
+                hits[4 * i + k] = h;
+            }
+        } else {
+            PacketHit h;
+            packet_nearest(sc, r[i], &h, c);
+            alignas(16) float d[4], u[4], v[4];
+            alignas(16) int id[4];
+            _mm_store_ps(d, h.d);
+            _mm_store_ps(u, h.u);
+            _mm_store_ps(v, h.v);
+            _mm_store_si128(reinterpret_cast<__m128i *>(id), h.id);
+            for (int k = 0; k < 4; ++k) {
+                ids[4 * i + k] = d[k] < kMaxDist ? id[k] : -1;
+                hits[4 * i + k].d = d[k];
+                hits[4 * i + k].u = u[k];
+                hits[4 * i + k].v = v[k];
+            }
+        }
+    }
+}
+
+// rt/raytracer.cc:91-165 shoot_shadow_packet: visibility of s packed shadow
+// packets (sr: their rays, for the ray-by-ray case).
+void engine_occluded(const Scene &sc, const PacketRay *r, const __m128 *t, const Ray *const *sr, int s, bool *occ,
+                     Counters *c) {
+    int size[kPackets];
+    const bool coherent = coherency(r, s, size);
+    for (int j = 0; j < s; ++j) {
+        if (!coherent && size[j] == 0) {
+            for (int k = 0; k < 4; ++k) occ[4 * j + k] = found_nearer_object(sc, *sr[4 * j + k], sr[4 * j + k]->length, c);
+        } else {
+            const int m = _mm_movemask_ps(packet_nearer(sc, r[j], t[j], c));
+            for (int k = 0; k < 4; ++k) occ[4 * j + k] = (m >> k) & 1;
+        }
+    }
+}
+
+// Visibility of n shadow rays packed in order, 16 packets at a time; the
+// remainder (n mod 4) goes ray by ray (rt/raytracer.cc:286-323).
+void shadow_batch(const Scene &sc, const Ray *const *rays, int n, bool *occ, Counters *c) {
+    PacketRay pk[kPackets];
+    __m128 t[kPackets];
+    int done = 0;
+    while (n - done >= 4) {
+        const int np = std::min(kPackets, (n - done) / 4);
+        for (int j = 0; j < np; ++j) pk[j] = pack4(rays + done + 4 * j, &t[j]);
+        engine_occluded(sc, pk, t, rays + done, np, occ + done, c);
+        done += 4 * np;
+    }
+    for (; done < n; ++done) occ[done] = found_nearer_object(sc, *rays[done], rays[done]->length, c);
+}
+
+// Shading of a batch of hit rays: shadow rays per light packed over the
+// batch, then Lambert (the same arithmetic as the scalar frame).
+void shade_batch(const FrameCtx &f, const SurfaceHit *sh, const Colour *kd, int n, Colour *out, LocalCounters *lc,
+                 int ray_kind) {
+    const int nl = static_cast<int>(f.lights.size());
+    std::vector<Ray> illum(static_cast<size_t>(n) * std::max(1, nl));
+    std::vector<const Ray *> ptr(n);
+    std::vector<uint8_t> occ_b(n);
+    bool occ[kTileRays];
+    for (int l = 0; l < nl; ++l) {
+        for (int k = 0; k < n; ++k) {
+            illum[static_cast<size_t>(k) * nl + l] = shadow_ray(sh[k], f.lights[l].c);
+            ptr[k] = &illum[static_cast<size_t>(k) * nl + l];
+        }
+        for (int b = 0; b < n; b += kTileRays) {
+            const int m = std::min(kTileRays, n - b);
+            shadow_batch(*f.s, ptr.data() + b, m, occ, &lc->trav);
+            for (int k = 0; k < m; ++k) illum[static_cast<size_t>(b + k) * nl + l].magn = (occ[k] ? 0.0f : 1.0f) / 1.0f;
+        }
+        lc->rays[ray_kind] += static_cast<uint64_t>(n);
+    }
+    for (int k = 0; k < n; ++k) out[k] = shade_direct(kd[k], sh[k], &illum[static_cast<size_t>(k) * nl], f.lights);
+}
+
+void render_tile_packet(const FrameCtx &f, int tx, int ty, LocalCounters *lc) {
+    const PacketLuts &L = luts();
+    const CameraState &cam = f.cam;
+    PacketRay pk[kPackets];
+    // rt/camera.h:185-229: directions normalised with inverse_sqrt.
+    for (int i = 0; i < kPackets; ++i) {
+        const float x = static_cast<float>(tx + L.xoff[i]), y = static_cast<float>(ty + L.yoff[i]);
+        const __m128 vx = _mm_setr_ps(x, x + 1.0f, x, x + 1.0f);
+        const __m128 vy = _mm_setr_ps(y, y, y + 1.0f, y + 1.0f);
+        const __m128 vx_t = _mm_add_ps(_mm_mul_ps(vx, _mm_set1_ps(cam.x_inc)), _mm_set1_ps(cam.x_m));
+        const __m128 vy_t = _mm_add_ps(_mm_mul_ps(vy, _mm_set1_ps(cam.y_inc)), _mm_set1_ps(cam.y_m));
+        __m128 dx = _mm_mul_ps(_mm_set1_ps(cam.X.x), vx_t);
+        __m128 dy = _mm_mul_ps(_mm_set1_ps(cam.X.y), vx_t);
+        __m128 dz = _mm_mul_ps(_mm_set1_ps(cam.X.z), vx_t);
+        dx = _mm_add_ps(dx, _mm_mul_ps(_mm_set1_ps(cam.Y.x), vy_t));
+        dy = _mm_add_ps(dy, _mm_mul_ps(_mm_set1_ps(cam.Y.y), vy_t));
+        dz = _mm_add_ps(dz, _mm_mul_ps(_mm_set1_ps(cam.Y.z), vy_t));
+        dx = _mm_add_ps(dx, _mm_mul_ps(_mm_set1_ps(cam.Z.x), _mm_set1_ps(cam.t)));
+        dy = _mm_add_ps(dy, _mm_mul_ps(_mm_set1_ps(cam.Z.y), _mm_set1_ps(cam.t)));
+        dz = _mm_add_ps(dz, _mm_mul_ps(_mm_set1_ps(cam.Z.z), _mm_set1_ps(cam.t)));
+        const __m128 len = packet_inverse_sqrt(
+            _mm_add_ps(_mm_add_ps(_mm_mul_ps(dx, dx), _mm_mul_ps(dy, dy)), _mm_mul_ps(dz, dz)));
+        pk[i].ox = _mm_set1_ps(cam.pos.x);
+        pk[i].oy = _mm_set1_ps(cam.pos.y);
+        pk[i].oz = _mm_set1_ps(cam.pos.z);
+        pk[i].dx = _mm_mul_ps(dx, len);
+        pk[i].dy = _mm_mul_ps(dy, len);
+        pk[i].dz = _mm_mul_ps(dz, len);
+    }
+    int ids[kTileRays];
+    Hit hits[kTileRays];
+    engine_closest(*f.s, pk, kPackets, ids, hits, &lc->trav);
+    lc->rays[0] += kTileRays;
+    // Hit rays in ray order: surface, kd.
+    int hit_ray[kTileRays];
+    SurfaceHit sh[kTileRays];
+    Colour kd[kTileRays];
+    Ray rays[kTileRays];
+    int nh = 0;
+    for (int k = 0; k < kTileRays; ++k) {
+        rays[k] = lane_ray(pk[k >> 2], k & 3);
+        if (!(hits[k].d < kMaxDist)) continue;
+        hit_ray[nh] = k;
+        sh[nh] = surface_at(f.s->tris[ids[k]], &rays[k], hits[k].d);
+        kd[nh] = kd_of(f, ids[k]);
+        ++nh;
+    }
+    Colour direct[kTileRays];
+    shade_batch(f, sh, kd, nh, direct, lc, 1);
+    Colour col[kTileRays];
+    for (int k = 0; k < kTileRays; ++k) col[k] = Colour{};
+    for (int j = 0; j < nh; ++j) col[hit_ray[j]] = direct[j];
+    if (f.p.bounce) {
+        // Bounce rays of the hit rays, packed in order and traced through
+        // the same engine (as the reference packs secondary rays,
+        // rt/raytracer.cc:353-394); the remainder goes ray by ray.
+        Ray br[kTileRays];
+        const Ray *bp[kTileRays];
+        for (int j = 0; j < nh; ++j) {
+            const int k = hit_ray[j];
+            const int px = tx + (L.ray_to_pixel[k] & 7), py = ty + (L.ray_to_pixel[k] >> 3);
+            const int64_t p = static_cast<int64_t>(py) * cam.x_res + px;
+            br[j].ogn = offset_start_point(sh[j].dst, sh[j].geo_norm, 1);
+            br[j].dir = cosine_direction(bounce_key(f.p.seed, static_cast<uint32_t>(p), 0u, 1u), sh[j].shade_norm);
+            bp[j] = &br[j];
+        }
+        int bid[kTileRays];
+        Hit bh[kTileRays];
+        const int np = nh / 4;
+        if (np > 0) {
+            PacketRay bpk[kPackets];
+            for (int i = 0; i < np; ++i) bpk[i] = pack4(bp + 4 * i, nullptr);
+            engine_closest(*f.s, bpk, np, bid, bh, &lc->trav);
+        }
+        for (int j = 4 * np; j < nh; ++j) bid[j] = find_nearest_object(*f.s, br[j], &bh[j], &lc->trav);
+        lc->rays[2] += static_cast<uint64_t>(nh);
+        SurfaceHit bsh[kTileRays];
+        Colour bkd[kTileRays];
+        int bsrc[kTileRays];
+        int nb = 0;
+        for (int j = 0; j < nh; ++j) {
+            if (!(bh[j].d < kMaxDist)) continue;
+            bsh[nb] = surface_at(f.s->tris[bid[j]], &br[j], bh[j].d);
+            bkd[nb] = kd_of(f, bid[j]);
+            bsrc[nb] = j;
+            ++nb;
+        }
+        Colour c2[kTileRays];
+        shade_batch(f, bsh, bkd, nb, c2, lc, 3);
+        for (int b = 0; b < nb; ++b) {
+            const int j = bsrc[b];
+            Colour &c = col[hit_ray[j]];
+            c.r = c.r + ((kd[j].r / 255.0f) * c2[b].r);
+            c.g = c.g + ((kd[j].g / 255.0f) * c2[b].g);
+            c.b = c.b + ((kd[j].b / 255.0f) * c2[b].b);
+        }
+    }
+    for (int k = 0; k < kTileRays; ++k) {
+        const int px = tx + (L.ray_to_pixel[k] & 7), py = ty + (L.ray_to_pixel[k] >> 3);
+        if (px >= cam.x_res || py >= cam.y_res) continue;
+        const int64_t p = static_cast<int64_t>(py) * cam.x_res + px;
+        if (f.out.prim) f.out.prim[p] = ids[k];
+        if (f.out.tuv) {
+            f.out.tuv[3 * p + 0] = hits[k].d;
+            f.out.tuv[3 * p + 1] = hits[k].u;
+            f.out.tuv[3 * p + 2] = hits[k].v;
+        }
+        if (f.out.rgb) {
+            f.out.rgb[3 * p + 0] = col[k].r / 1.0f;
+            f.out.rgb[3 * p + 1] = col[k].g / 1.0f;
+            f.out.rgb[3 * p + 2] = col[k].b / 1.0f;
+        }
+    }
+}
+
 }  // namespace
 
 int render_frame(const Scene &s, const float *albedo, const oracle_camera &cam, const oracle_light *lights,
@@ -298,7 +591,21 @@ int render_frame(const Scene &s, const float *albedo, const oracle_camera &cam, 
     const int nthreads = std::max(1, p.nthreads);
     std::atomic<int64_t> next{0};
     std::vector<LocalCounters> counters(nthreads);
+    const int tiles_x = (f.cam.x_res + 7) / 8;
+    const int64_t tiles = static_cast<int64_t>(tiles_x) * ((f.cam.y_res + 7) / 8);
+    auto packet_worker = [&](int tid) {
+        while (true) {
+            const int64_t t = next.fetch_add(1) * stride;  // the packet frame strides over tiles
+            if (t >= tiles) break;
+            render_tile_packet(f, static_cast<int>(t % tiles_x) * 8, static_cast<int>(t / tiles_x) * 8,
+                               &counters[tid]);
+        }
+    };
     auto worker = [&](int tid) {
+        if (p.packet) {
+            packet_worker(tid);
+            return;
+        }
         const int64_t chunk = 64;
         while (true) {
             const int64_t b = next.fetch_add(chunk);

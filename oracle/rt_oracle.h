@@ -39,9 +39,9 @@ typedef struct {
     int32_t bounce;
     int32_t spp;
     uint64_t seed;
-    int32_t pixel_stride; /* trace pixels p with p % stride == 0 */
+    int32_t pixel_stride; /* trace pixels p with p % stride == 0 (packet frame: 8x8 tiles) */
     int32_t nthreads;
-    int32_t packet;       /* 1: SSE packet path for primary and shadow rays */
+    int32_t packet;       /* 1: the SSE packet frame (spp 1, bounce <= 1) */
 } oracle_render_params;
 
 /* Outputs; any pointer may be NULL. Arrays are indexed by pixel p = y*W + x.
@@ -106,6 +106,16 @@ void oracle_bounce_dir(uint64_t seed, uint32_t pixel, uint32_t sample, uint32_t 
 void oracle_sincos_turns(float r, float *s, float *c);
 uint64_t oracle_pcg32_stream(uint64_t initstate, uint64_t initseq, int32_t count, uint32_t *out);
 uint64_t oracle_splitmix64_stream(uint64_t seed, int32_t count, uint64_t *out);
+/* SSE packet path (rt/triangle.h:242-305, ssd/bvh_node.h:101-132,
+ * ssd/bvh.cc:341-500): slab test of one box for four rays (ogn4, i_rd4 as
+ * x[4], y[4], z[4]); rcp / rsqrt with one Newton step; packets of four
+ * consecutive rays through the packet traversal or packet brute force. */
+void oracle_packet_slab(const float *low, const float *high, const float *ogn4, const float *i_rd4, float *out4);
+void oracle_packet_inverse(const float *x4, float *out4);
+void oracle_packet_inverse_sqrt(const float *x4, float *out4);
+void oracle_packet_trace_closest(const oracle_scene *s, const float *rays, int64_t n, int32_t *prim, float *tuv,
+                                 int32_t use_bvh);
+void oracle_packet_trace_occluded(const oracle_scene *s, const float *rays, int64_t n, uint8_t *occ, int32_t use_bvh);
 /* Primary ray of rt/camera.h:127-158 (EXACT_NORMALISE). */
 void oracle_primary_ray(const oracle_camera *cam, int32_t x, int32_t y, float *ogn, float *dir);
 
