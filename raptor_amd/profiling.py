@@ -13,6 +13,12 @@
                                           --title TEXT --command TEXT --out FILE
       writes FILE (markdown): the rocprofv3 kernel statistics of the kernels
       whose names contain one of the substrings.
+  python3 raptor_amd/profiling.py counters --dirs DIR[,DIR...] --title TEXT
+                                           --command TEXT --out FILE
+      writes FILE (markdown): per trace stage of the frame, the averages of
+      VALUBusy, VALUUtilization, OccupancyPercent, MeanOccupancyPerActiveCU,
+      the L2 hit rate (TCC_HIT_sum, TCC_MISS_sum) and instruction counts
+      collected by one rocprofv3 --pmc pass per directory.
 """
 import argparse
 import csv
@@ -114,6 +120,35 @@ def cmd_kernels(args):
     print("\n".join(lines))
 
 
+def cmd_counters(args):
+    import collections
+
+    res = collections.defaultdict(lambda: collections.defaultdict(list))
+    for d in args.dirs.split(","):
+        for r in _rows(d, "*counter_collection.csv"):
+            k = r["Kernel_Name"].replace("void ", "").replace("(anonymous namespace)::", "").split("(rt::TraceArgs")[0]
+            res[k][r["Counter_Name"]].append(float(r["Counter_Value"]))
+    names = {0: "primary closest", 1: "shadow any hit", 2: "bounce closest", 3: "bounce-shadow any hit"}
+    lines = [f"# Counters: {args.title}", "", "Command (one rocprofv3 pass per counter group):", "",
+             "    " + args.command, "", "Averages over the frame launches of each kernel (the counting frame excluded).",
+             "", "| kernel | stage | VALUBusy % | VALUUtilization % | Occupancy % | waves / active CU | L2 hit % | "
+             "SQ_INSTS_VALU | SQ_INSTS_SALU |", "|---|---|---|---|---|---|---|---|---|"]
+    for k in sorted(res):
+        if not k.startswith("rt::k_trace<") or "true" in k:
+            continue
+        d = {c: sum(v) / len(v) for c, v in res[k].items()}
+        st = int(k.split(",")[1])
+        hit = 100 * d.get("TCC_HIT_sum", 0.0) / max(1.0, d.get("TCC_HIT_sum", 0.0) + d.get("TCC_MISS_sum", 0.0))
+        lines.append(f"| `{k}` | {names.get(st, st)} | {d.get('VALUBusy', 0):.1f} | {d.get('VALUUtilization', 0):.1f} | "
+                     f"{d.get('OccupancyPercent', 0):.1f} | {d.get('MeanOccupancyPerActiveCU', 0):.2f} | {hit:.1f} | "
+                     f"{d.get('SQ_INSTS_VALU', 0):.3g} | {d.get('SQ_INSTS_SALU', 0):.3g} |")
+    lines += ["", "VALUBusy: share of the kernel's time the VALUs issue. VALUUtilization: share of the 64 lanes",
+              "active in the VALU instructions issued (SIMD efficiency)."]
+    with open(args.out, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print("\n".join(lines))
+
+
 def main():
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -127,8 +162,13 @@ def main():
     k = sub.add_parser("kernels")
     for a in ("stats", "match", "title", "command", "out"):
         k.add_argument("--" + a, required=True)
+    c2 = sub.add_parser("counters")
+    for a in ("dirs", "title", "command", "out"):
+        c2.add_argument("--" + a, required=True)
     args = ap.parse_args()
-    if args.cmd == "copy":
+    if args.cmd == "counters":
+        cmd_counters(args)
+    elif args.cmd == "copy":
         cmd_copy(args)
     elif args.cmd == "kernels":
         cmd_kernels(args)
