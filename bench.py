@@ -26,6 +26,9 @@ sys.path.insert(0, ROOT)
 METRIC = "Mrays/s at 1M tris (primary+shadow+diffuse); batched & adjoint vs plain on GPU"
 PEAK_HBM_GBS = 8000.0
 STAGES = ["primary_closest", "shadow_anyhit", "bounce_closest", "bounce_shadow_anyhit"]
+# Kernel of each trace stage: the bounce closest-hit launch uses the library's
+# block-local form with ray refills, the others one lane per ray.
+KERNELS = ["k_trace", "k_trace", "k_trace_pers", "k_trace"]
 
 
 def algorithmic_bytes(width, rays, node_visits, tri_tests, closest):
@@ -395,6 +398,7 @@ def main():
         with open(tpath) as f:
             t = json.load(f)
         same = (t.get("kernel_stage") == STAGES[dom] and t.get("width") == args.width
+                and t.get("kernel", "").startswith(KERNELS[dom] + "<")
                 and t.get("leaf_size") == args.leaf_size and t.get("lds_stack") == args.lds_stack
                 and t.get("sah_bins", 0) == args.sah_bins)
         if same:
@@ -430,7 +434,7 @@ def main():
                 "bytes": info["bytes"], "build_ms": round(info["build_ms"], 2)},
         "roofline": {
             "bound": "hbm",
-            "kernel": f"k_trace<{args.width}, {dom}, false, {args.lds_stack}> ({STAGES[dom]})",
+            "kernel": f"{KERNELS[dom]}<{args.width}, {dom}, false, {args.lds_stack}> ({STAGES[dom]})",
             "achieved": round(achieved, 2),
             "peak": PEAK_HBM_GBS,
             "unit": "GB/s",

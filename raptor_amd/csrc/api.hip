@@ -102,6 +102,7 @@ void release_ctx(rt_ctx *ctx) {
     for (auto &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     if (ctx->d_counters) (void)hipFree(ctx->d_counters);
+    if (ctx->d_queue) (void)hipFree(ctx->d_queue);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -152,7 +153,8 @@ int32_t rt_ctx_create(int32_t hip_device, rt_ctx **out) {
             return RT_ERR_HIP;
         }
     }
-    if (hipMalloc(&ctx->d_counters, 16 * sizeof(unsigned long long)) != hipSuccess) {
+    if (hipMalloc(&ctx->d_counters, 16 * sizeof(unsigned long long)) != hipSuccess ||
+        hipMalloc(&ctx->d_queue, 64 * sizeof(uint32_t)) != hipSuccess) {
         rt_ctx_destroy(ctx);
         return RT_ERR_HIP;
     }

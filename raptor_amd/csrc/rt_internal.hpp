@@ -59,6 +59,7 @@ struct rt_ctx {
     // Device counters when counting is on: (node visits, triangle tests) for
     // slots 0..3, the stack-overflow flag at index 15.
     unsigned long long *d_counters = nullptr;
+    uint32_t *d_queue = nullptr;  // ray queue head of the persistent trace kernels
     int count_slot = 0;
     std::atomic<int> refs{1};  // the caller's handle + one per scene
 };
