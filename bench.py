@@ -332,6 +332,62 @@ def invrender_section(ctx, steps, n_scenes=16, res=512, spp=2, bounces=2):
     return res_out
 
 
+def dynamic_section(ctx, w, accel, scene, frames, args):
+    """Per-frame update + refit + frame for a moving terrain (a travelling wave
+    of amplitude 0.01, one phase step per frame), against rebuilding the
+    accelerator every frame as the reference's physics render loop does
+    (RAPTor/physics_engine/simulation_environment.cc:53-58). Vertex positions
+    for every frame are made on the device beforehand (the simulation's part)."""
+    import torch
+
+    from raptor_amd import rt, workloads
+
+    dev = torch.from_numpy(w.abc).to("cuda:0")
+    pos = [workloads.terrain_wave(dev, k / frames, amplitude=0.01).contiguous() for k in range(frames)]
+    rgb = torch.empty((w.camera.y_res, w.camera.x_res, 3), dtype=torch.float32, device="cuda:0")
+    torch.cuda.synchronize()
+
+    def loop(rebuild):  # rebuild: None refits, else the sah_bins of a new build
+        upd = fit = frame = gpu = 0.0
+        rays = 0
+        for k in range(frames):
+            t0 = time.perf_counter()
+            scene.update(pos[k])
+            t1 = time.perf_counter()
+            if rebuild is not None:
+                a = rt.Accel(scene, width=args.width, leaf_size=args.leaf_size, lds_stack=args.lds_stack,
+                             sah_bins=rebuild)
+            else:
+                accel.refit()
+                a = accel
+            t2 = time.perf_counter()
+            st = a.render_device(w.camera, w.lights, rgb, bounce=w.bounce, seed=w.seed)
+            torch.cuda.synchronize()
+            t3 = time.perf_counter()
+            if rebuild is not None:
+                a.close()
+            upd += t1 - t0
+            fit += t2 - t1
+            frame += t3 - t2
+            gpu += st["ms"][0] * 1e-3  # event-timed stages (no workspace allocation)
+            rays += sum(st["rays"])
+        n = frames
+        return {"update_ms": round(upd / n * 1e3, 3), "refit_or_build_ms": round(fit / n * 1e3, 3),
+                "frame_ms": round(frame / n * 1e3, 3), "total_ms": round((upd + fit + frame) / n * 1e3, 3),
+                "frame_gpu_ms": round(gpu / n * 1e3, 3), "trace_mrays_s": round(rays / gpu / 1e6, 1)}
+
+    loop(None)  # warm-up
+    refit = loop(None)
+    lbvh = loop(0)
+    sah = loop(args.sah_bins) if args.sah_bins > 0 else None
+    scene.update(dev)  # back to the rest pose for what follows
+    accel.refit()
+    torch.cuda.synchronize()
+    return {"workload": f"terrain1m under a travelling wave (amplitude 0.01, wavelength 0.25), {frames} frames, "
+                        "update + refit + frame vs update + rebuild + frame",
+            "refit": refit, "rebuild_lbvh": lbvh, "rebuild_sah": sah}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -348,6 +404,7 @@ def main():
     ap.add_argument("--batched-reps", type=int, default=10)
     ap.add_argument("--adjoint-reps", type=int, default=10, help="0 skips the differentiable workload")
     ap.add_argument("--invrender-steps", type=int, default=100, help="0 skips batched inverse rendering")
+    ap.add_argument("--dynamic-frames", type=int, default=8, help="0 skips the moving-scene workload")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each GPU would trace its own share "
@@ -446,6 +503,9 @@ def main():
             "tri_tests_per_ray": round(counted["visits"][2 * dom + 1] / max(1, counted["rays"][dom]), 3),
         },
     }
+    if args.dynamic_frames > 0:
+        print("bench: moving scene", file=sys.stderr, flush=True)
+        line["dynamic"] = dynamic_section(ctx, w, accel, scene, args.dynamic_frames, args)
     if args.batched_scenes > 0:
         del rgb, accel, scene
         print("bench: batched workload", file=sys.stderr, flush=True)

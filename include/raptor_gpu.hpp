@@ -7,6 +7,7 @@
 //   accel.find_nearest_object(&r, &h)             accel.trace_closest(rays, n, hits)   (batched)
 //   accel.found_nearer_object(&r, t)              accel.trace_occluded(rays, n, flags) (batched)
 //   ray_tracer(&accel, lights, prims, cam);       raptor_gpu::ray_tracer(accel, lights, cam_params, cam);
+//   new bih(*tris) every frame                    accel.refit(*tris)   (same triangles, moved)
 //
 // (ssd/bvh.h:27-32, ssd/bvh.cc:649-759, rt/raytracer.h:105.) The store is read
 // through the public triangle accessors get_vertex_a/b/c(), get_light() and
@@ -57,13 +58,11 @@ class gpu_bvh {
             const rt_build_params *params = nullptr)
         : ctx_(ctx) {
         const int64_t n = static_cast<int64_t>(prims.size());
-        std::vector<float> abc(9 * n), albedo(3 * n);
+        const std::vector<float> abc = vertices(prims);
+        std::vector<float> albedo(3 * n);
         std::vector<uint8_t> flags(n);
         for (int64_t i = 0; i < n; ++i) {
             const auto *t = prims.primitive(static_cast<int>(i));
-            const auto a = t->get_vertex_a(), b = t->get_vertex_b(), c = t->get_vertex_c();
-            const float v[9] = {a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z};
-            for (int k = 0; k < 9; ++k) abc[9 * i + k] = v[k];
             float rgb[3];
             albedo_of(static_cast<int>(i), rgb);
             for (int k = 0; k < 3; ++k) albedo[3 * i + k] = rgb[k];
@@ -79,6 +78,16 @@ class gpu_bvh {
     }
     gpu_bvh(const gpu_bvh &) = delete;
     gpu_bvh &operator=(const gpu_bvh &) = delete;
+
+    // The physics render loop builds a new accelerator over the moved
+    // triangles every frame (RAPTor/physics_engine/simulation_environment.cc:53-58).
+    // refit takes the same store after the move (same triangles, same order)
+    // and refits the boxes in place; hits equal those of a new gpu_bvh.
+    void refit(const PrimitiveStore &prims) {
+        const std::vector<float> abc = vertices(prims);
+        ctx_.check(rt_scene_update(scene_, abc.data(), 0), "rt_scene_update");
+        ctx_.check(rt_accel_refit(accel_), "rt_accel_refit");
+    }
 
     // Batched closest hit / any hit; pointers are host memory unless flags
     // carries RT_DEVICE_PTRS.
@@ -113,6 +122,18 @@ class gpu_bvh {
     rt_accel *get() const { return accel_; }
 
   private:
+    static std::vector<float> vertices(const PrimitiveStore &prims) {
+        const int64_t n = static_cast<int64_t>(prims.size());
+        std::vector<float> abc(9 * n);
+        for (int64_t i = 0; i < n; ++i) {
+            const auto *t = prims.primitive(static_cast<int>(i));
+            const auto a = t->get_vertex_a(), b = t->get_vertex_b(), c = t->get_vertex_c();
+            const float v[9] = {a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z};
+            for (int k = 0; k < 9; ++k) abc[9 * i + k] = v[k];
+        }
+        return abc;
+    }
+
     template <class Ray>
     static rt_ray make_ray(const Ray &r, float tmax) {
         const auto o = r.get_ogn();

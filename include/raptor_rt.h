@@ -113,6 +113,7 @@ typedef struct {
     int64_t bytes;
     float build_ms;
     float sah_cost;
+    float refit_ms;  /* the last rt_accel_refit, 0 before any */
 } rt_accel_info;
 
 /* Destroy calls release the caller's handle and may come in any order: a
@@ -139,6 +140,26 @@ int32_t rt_scene_normals(rt_scene *scene, float *n3);
 int32_t rt_accel_build(rt_scene *scene, const rt_build_params *params, rt_accel **out);
 void rt_accel_destroy(rt_accel *accel);
 int32_t rt_accel_info_get(const rt_accel *accel, rt_accel_info *info);
+
+/* ---- Dynamic scenes. The physics engine's render loop builds a new
+ * accelerator over the moved triangles for every frame
+ * (RAPTor/physics_engine/simulation_environment.cc:53-58); these two calls
+ * serve that loop without a rebuild. ----
+ *
+ * rt_scene_update replaces the scene's vertex positions: abc is n x 9 floats
+ * with the scene's own n, on the host, or on the device with RT_DEVICE_PTRS.
+ * Normals follow the new vertices; albedo and flags stay. Every accel built
+ * on the scene must be refit (or rebuilt) before it is traced again.
+ *
+ * rt_accel_refit fits the accel to its scene's current vertices and keeps
+ * its tree: leaf triangles are re-read and every box is recomputed from the
+ * deepest level to the root. Hits are those of a fresh build over the same
+ * triangles (up to exact-t ties); traversal slows as the motion departs from
+ * the one the tree was built for, when a rebuild (rt_accel_build, sah_bins 0
+ * for the fastest) pays again. Batches and inverse-rendering sets built on
+ * the accel see the refit boxes without being recreated. */
+int32_t rt_scene_update(rt_scene *scene, const float *abc, uint32_t flags);
+int32_t rt_accel_refit(rt_accel *accel);
 
 /* Closest hit for n rays. Replaces bvh::find_nearest_object(ray*)
  * (ssd/bvh.cc:649-709) and its packet form (ssd/bvh.cc:406-447). */

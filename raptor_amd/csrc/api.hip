@@ -44,6 +44,20 @@ __global__ void __launch_bounds__(256) k_scene_prep(const float *__restrict__ ab
     kd[i] = k;
 }
 
+// New vertex positions: normal and vertex c again, the flags kept (tri_n.w).
+__global__ void __launch_bounds__(256) k_scene_move(const float *__restrict__ abc, int64_t n,
+                                                    float4 *__restrict__ tri_n, float4 *__restrict__ tri_c) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float *t = abc + 9 * i;
+    const rtd::V3 a = rtd::mk(t[0], t[1], t[2]);
+    const rtd::V3 b = rtd::mk(t[3], t[4], t[5]);
+    const rtd::V3 c = rtd::mk(t[6], t[7], t[8]);
+    const rtd::V3 nn = rtd::normalise_exact(rtd::cross(rtd::sub(b, a), rtd::sub(c, a)));
+    tri_n[i] = make_float4(nn.x, nn.y, nn.z, tri_n[i].w);
+    tri_c[i] = make_float4(c.x, c.y, c.z, 0.0f);
+}
+
 // Surface rays: triangle k = floor(r0 * n); point by the square-root
 // barycentric map (Osada et al. 2002); origin offset along the face normal as
 // rt/ray.h:111-157 offsets shadow-ray origins; cosine-weighted direction.
@@ -124,6 +138,7 @@ void release_accel(rt_accel *accel) {
     free_frame_workspace(accel);
     if (accel->d_nodes) (void)hipFree(accel->d_nodes);
     if (accel->d_tris) (void)hipFree(accel->d_tris);
+    if (accel->d_refit) (void)hipFree(accel->d_refit);
     delete accel;
     release_scene(scene);
 }
@@ -234,6 +249,29 @@ int32_t rt_accel_build(rt_scene *scene, const rt_build_params *params, rt_accel 
 
 void rt_accel_destroy(rt_accel *accel) { rt::release_accel(accel); }
 
+int32_t rt_scene_update(rt_scene *scene, const float *abc, uint32_t flags) {
+    if (!scene || (!abc && scene->n > 0)) return RT_ERR_INVALID;
+    rt_ctx *ctx = scene->ctx;
+    if (int32_t rc = rt::set_device(ctx)) return rc;
+    if (scene->n == 0) return RT_OK;
+    hipStream_t s = ctx->stream;
+    const bool dev = (flags & RT_DEVICE_PTRS) != 0;
+    RT_HIP_CHECK(ctx, hipMemcpyAsync(scene->d_abc, abc, 9 * scene->n * sizeof(float),
+                                     dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(rt::k_scene_move, dim3(static_cast<unsigned>((scene->n + 255) / 256)), dim3(256), 0, s,
+                       scene->d_abc, scene->n, scene->d_n, scene->d_c);
+    RT_HIP_CHECK(ctx, hipGetLastError());
+    // A host source must stay untouched until the copy has read it.
+    if (!dev) RT_HIP_CHECK(ctx, hipStreamSynchronize(s));
+    return RT_OK;
+}
+
+int32_t rt_accel_refit(rt_accel *accel) {
+    if (!accel) return RT_ERR_INVALID;
+    if (int32_t rc = rt::set_device(accel->scene->ctx)) return rc;
+    return rt::refit_accel(accel);
+}
+
 int32_t rt_accel_info_get(const rt_accel *a, rt_accel_info *info) {
     if (!a || !info) return RT_ERR_INVALID;
     info->nodes = a->nodes;
@@ -245,6 +283,7 @@ int32_t rt_accel_info_get(const rt_accel *a, rt_accel_info *info) {
     info->bytes = a->nodes * node_bytes + a->scene->n * static_cast<int64_t>(sizeof(rt::LeafTri));
     info->build_ms = a->build_ms;
     info->sah_cost = a->sah_cost;
+    info->refit_ms = a->refit_ms;
     return RT_OK;
 }
 

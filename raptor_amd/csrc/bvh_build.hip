@@ -906,6 +906,7 @@ int32_t build_accel(rt_scene *scene, const rt_build_params &p, rt_accel **out) {
 
     rt_accel *a = new rt_accel();
     a->scene = scene;
+    a->triangles = n;
     a->width = W;
     a->leaf_size = p.leaf_size;
     a->lds_stack = (p.lds_stack <= 0) ? 16 : (p.lds_stack <= 4 ? 4 : (p.lds_stack <= 8 ? 8 : 16));
@@ -1198,6 +1199,7 @@ int32_t build_accel(rt_scene *scene, const rt_build_params &p, rt_accel **out) {
     int qn = 1;
     int depth = 0;
     int2 *qin = (int2 *)b_qa.p, *qout = (int2 *)b_qb.p;
+    a->level_start.assign({0, 1});  // level L = wide nodes [level_start[L], level_start[L + 1])
     while (qn > 0) {
         BUILD_CHECK(hipMemsetAsync(qout_n, 0, sizeof(uint32_t), s));
         CollapseArgs ca;
@@ -1222,10 +1224,11 @@ int32_t build_accel(rt_scene *scene, const rt_build_params &p, rt_accel **out) {
         else
             hipLaunchKernelGGL(k_collapse<8>, dim3(cb), dim3(256), 0, s, ca);
         BUILD_CHECK(hipGetLastError());
-        uint32_t next = 0;
-        BUILD_CHECK(hipMemcpyAsync(&next, qout_n, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        uint32_t cn[2] = {0, 0};  // node counter, next queue length
+        BUILD_CHECK(hipMemcpyAsync(cn, node_counter, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         BUILD_CHECK(hipStreamSynchronize(s));
-        qn = static_cast<int>(next);
+        qn = static_cast<int>(cn[1]);
+        if (qn > 0) a->level_start.push_back(cn[0]);
         std::swap(qin, qout);
         ++depth;
     }
@@ -1239,6 +1242,193 @@ int32_t build_accel(rt_scene *scene, const rt_build_params &p, rt_accel **out) {
     a->build_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
     *out = a;
 #undef BUILD_CHECK
+    return RT_OK;
+}
+
+namespace {
+
+// Refit scratch: [0, 6) scene bounds lo xyz, hi xyz, [6] padding (floats),
+// then kBoundsBlocks partial bounds per coordinate. The bounds are reduced in
+// two passes with no atomics: a few words updated by every wave serialise
+// across the chip's L2s (8192 waves x 6 atomics took 561 us for 1M triangles).
+constexpr int kBoundsBlocks = 1024;
+constexpr int kRefitWords = 8 + 6 * kBoundsBlocks;
+
+__global__ void __launch_bounds__(256) k_scene_bounds(const float *__restrict__ abc, int64_t n, float *__restrict__ w) {
+    __shared__ float red[6][4];
+    float v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+        const float *t = abc + 9 * i;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            v[c] = fminf(v[c], fminf(t[c], fminf(t[3 + c], t[6 + c])));
+            v[3 + c] = fmaxf(v[3 + c], fmaxf(t[c], fmaxf(t[3 + c], t[6 + c])));
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        float x = v[k];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const float y = __shfl_xor(x, o, 64);
+            x = k < 3 ? fminf(x, y) : fmaxf(x, y);
+        }
+        if ((threadIdx.x & 63) == 0) red[k][threadIdx.x >> 6] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int k = threadIdx.x;
+        float x = red[k][0];
+        for (int q = 1; q < 4; ++q) x = k < 3 ? fminf(x, red[k][q]) : fmaxf(x, red[k][q]);
+        w[8 + k * kBoundsBlocks + blockIdx.x] = x;
+    }
+}
+
+// Second pass over the block partials (one block of 256), then the box
+// padding for the new bounds, never below the one in use.
+__global__ void __launch_bounds__(256) k_refit_pad(float *__restrict__ w, int parts, float old_pad) {
+    __shared__ float red[6][4];
+    for (int k = 0; k < 6; ++k) {
+        float x = k < 3 ? INFINITY : -INFINITY;
+        for (int b = threadIdx.x; b < parts; b += 256) {
+            const float y = w[8 + k * kBoundsBlocks + b];
+            x = k < 3 ? fminf(x, y) : fmaxf(x, y);
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const float y = __shfl_xor(x, o, 64);
+            x = k < 3 ? fminf(x, y) : fmaxf(x, y);
+        }
+        if ((threadIdx.x & 63) == 0) red[k][threadIdx.x >> 6] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float m = 0.0f;
+        for (int k = 0; k < 6; ++k) {
+            float x = red[k][0];
+            for (int q = 1; q < 4; ++q) x = k < 3 ? fminf(x, red[k][q]) : fmaxf(x, red[k][q]);
+            w[k] = x;
+            m = fmaxf(m, fabsf(x));
+        }
+        w[6] = fmaxf(old_pad, ldexpf(fmaxf(m, 1e-30f), -15));
+    }
+}
+
+// Leaf triangles from the scene's current vertices (the emplace id rides in
+// a.w, the flags in e1.w).
+__global__ void __launch_bounds__(256) k_refit_tris(const float *__restrict__ abc, int64_t n, LeafTri *__restrict__ tris) {
+    const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    LeafTri lt = tris[k];
+    const int32_t i = __float_as_int(lt.a_id.w);
+    const float *t = abc + 9 * static_cast<int64_t>(i);
+    const rtd::V3 a = rtd::mk(t[0], t[1], t[2]);
+    const rtd::V3 b = rtd::mk(t[3], t[4], t[5]);
+    const rtd::V3 c = rtd::mk(t[6], t[7], t[8]);
+    // rt/triangle.h:193-194: e1 = c - a, e2 = b - a.
+    const rtd::V3 e1 = rtd::sub(c, a);
+    const rtd::V3 e2 = rtd::sub(b, a);
+    lt.a_id = make_float4(a.x, a.y, a.z, lt.a_id.w);
+    lt.e1 = make_float4(e1.x, e1.y, e1.z, lt.e1.w);
+    lt.e2 = make_float4(e2.x, e2.y, e2.z, 0.0f);
+    tris[k] = lt;
+}
+
+// One level of wide nodes: a leaf child's box from its triangles' vertices
+// (padded), an interior child's box the union of that node's (padded) child
+// boxes, refit one level deeper before.
+template <int W>
+__global__ void __launch_bounds__(256) k_refit_wide(void *nodes_v, int64_t lo, int64_t hi, const LeafTri *__restrict__ tris,
+                                                    const float *__restrict__ abc, const float *__restrict__ w) {
+    const int64_t x = lo + static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (x >= hi) return;
+    const float pad = w[6];
+    WideNode<W> *nodes = reinterpret_cast<WideNode<W> *>(nodes_v);
+    WideNode<W> nd = nodes[x];
+    for (int k = 0; k < W; ++k) {
+        const int32_t c = nd.child[k];
+        if (c == kEmpty) continue;
+        float lx = INFINITY, ly = INFINITY, lz = INFINITY, hx = -INFINITY, hy = -INFINITY, hz = -INFINITY;
+        if (is_leaf(c)) {
+            const int32_t first = leaf_first(c), count = leaf_count(c);
+            for (int j = 0; j < count; ++j) {
+                const float *t = abc + 9 * static_cast<int64_t>(__float_as_int(tris[first + j].a_id.w));
+                for (int v = 0; v < 3; ++v) {
+                    lx = fminf(lx, t[3 * v]);
+                    ly = fminf(ly, t[3 * v + 1]);
+                    lz = fminf(lz, t[3 * v + 2]);
+                    hx = fmaxf(hx, t[3 * v]);
+                    hy = fmaxf(hy, t[3 * v + 1]);
+                    hz = fmaxf(hz, t[3 * v + 2]);
+                }
+            }
+            lx -= pad; ly -= pad; lz -= pad;
+            hx += pad; hy += pad; hz += pad;
+        } else {
+            const WideNode<W> &ch = nodes[c];
+            for (int j = 0; j < W; ++j) {
+                if (ch.child[j] == kEmpty) continue;
+                lx = fminf(lx, ch.lo_x[j]);
+                ly = fminf(ly, ch.lo_y[j]);
+                lz = fminf(lz, ch.lo_z[j]);
+                hx = fmaxf(hx, ch.hi_x[j]);
+                hy = fmaxf(hy, ch.hi_y[j]);
+                hz = fmaxf(hz, ch.hi_z[j]);
+            }
+        }
+        nd.lo_x[k] = lx;
+        nd.lo_y[k] = ly;
+        nd.lo_z[k] = lz;
+        nd.hi_x[k] = hx;
+        nd.hi_y[k] = hy;
+        nd.hi_z[k] = hz;
+    }
+    nodes[x] = nd;
+}
+
+}  // namespace
+
+int32_t refit_accel(rt_accel *a) {
+    rt_ctx *ctx = a->scene->ctx;
+    hipStream_t s = ctx->stream;
+    const int64_t n = a->scene->n;
+    if (n != a->triangles) return fail(ctx, RT_ERR_INVALID, "rt_accel_refit: the scene's triangle count changed");
+    if (n == 0) return RT_OK;
+    const auto t0 = std::chrono::steady_clock::now();
+    if (!a->d_refit) RT_HIP_CHECK(ctx, hipMalloc(&a->d_refit, kRefitWords * sizeof(float)));
+    const float *abc = a->scene->d_abc;
+    const int blocks = static_cast<int>((n + 255) / 256);
+    const int parts = std::min(blocks, kBoundsBlocks);
+    hipLaunchKernelGGL(k_scene_bounds, dim3(parts), dim3(256), 0, s, abc, n, a->d_refit);
+    hipLaunchKernelGGL(k_refit_pad, dim3(1), dim3(256), 0, s, a->d_refit, parts, a->box_pad);
+    hipLaunchKernelGGL(k_refit_tris, dim3(blocks), dim3(256), 0, s, abc, n, a->d_tris);
+    for (int l = static_cast<int>(a->level_start.size()) - 2; l >= 0; --l) {
+        const int64_t lo = a->level_start[l], hi = a->level_start[l + 1];
+        if (hi <= lo) continue;
+        const dim3 g(static_cast<unsigned>((hi - lo + 255) / 256));
+        if (a->width == 4)
+            hipLaunchKernelGGL(k_refit_wide<4>, g, dim3(256), 0, s, a->d_nodes, lo, hi, a->d_tris, abc, a->d_refit);
+        else
+            hipLaunchKernelGGL(k_refit_wide<8>, g, dim3(256), 0, s, a->d_nodes, lo, hi, a->d_tris, abc, a->d_refit);
+    }
+    RT_HIP_CHECK(ctx, hipGetLastError());
+    float hw[7];
+    RT_HIP_CHECK(ctx, hipMemcpyAsync(hw, a->d_refit, sizeof(hw), hipMemcpyDeviceToHost, s));
+    RT_HIP_CHECK(ctx, hipStreamSynchronize(s));
+    float maxabs = 0.0f;
+    for (int k = 0; k < 6; ++k) maxabs = std::max(maxabs, std::fabs(hw[k]));
+    // The padding never shrinks and the fused-test limit never grows, so the
+    // copies of both that batches and inverse-rendering sets took earlier
+    // stay conservative: near_limit * 2^-24 <= box_pad / 2 still holds.
+    a->box_pad = hw[6];
+    a->near_limit = std::min(a->near_limit, 256.0f * maxabs);
+    for (int k = 0; k < 3; ++k) {
+        a->scene_lo[k] = hw[k];
+        const float ext = hw[3 + k] - a->scene_lo[k];
+        a->scene_inv[k] = ext > 0.0f ? 1.0f / ext : 0.0f;
+    }
+    a->refit_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return RT_OK;
 }
 

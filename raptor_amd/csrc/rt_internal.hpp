@@ -87,11 +87,15 @@ struct rt_accel {
     void *d_nodes = nullptr;   // WideNode<W>[nodes]
     rt::LeafTri *d_tris = nullptr;  // BVH order
     float build_ms = 0.0f;
+    float refit_ms = 0.0f;
     float sah_cost = 0.0f;
     float box_pad = 0.0f;
     float near_limit = 0.0f;
     float scene_lo[3] = {0, 0, 0}, scene_inv[3] = {0, 0, 0};  // scene box origin and 1/extent   // |origin| bound for the fused box test (256x the largest scene coordinate)
     void *frame_ws = nullptr;  // frame workspace (render.hip)
+    int64_t triangles = 0;
+    std::vector<int64_t> level_start;
+    float *d_refit = nullptr;  // refit scratch: scene bounds, padding, partial bounds  // wide nodes of tree level L: [level_start[L], level_start[L + 1])
     std::atomic<int> refs{1};  // the caller's handle + one per batch item
 };
 
@@ -132,6 +136,7 @@ int32_t launch_occluded_q(rt_accel *a, const rt_ray *d_rays, const uint32_t *d_c
 
 // Frame (render.hip).
 void free_frame_workspace(rt_accel *a);
+int32_t refit_accel(rt_accel *a);
 int32_t render_frame(rt_accel *a, const rt_camera &cam, const rt_light *lights, int nlights,
                      const rt_render_params &p, float *rgb, rt_hit *primary, rt_hit *bounce, rt_stats *stats);
 

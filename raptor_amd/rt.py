@@ -32,7 +32,7 @@ EXPORTS = [
     "rt_trace_occluded", "rt_render", "rt_set_counting", "rt_gen_random_triangles", "rt_gen_terrain",
     "rt_batch_create", "rt_batch_run", "rt_batch_destroy", "rt_batch_autotune", "rt_gen_surface_rays",
     "rt_scene_set_albedo", "rt_render_tape", "rt_tape_info", "rt_render_adjoint", "rt_tape_destroy",
-    "rt_invrender_create", "rt_invrender_run", "rt_invrender_destroy",
+    "rt_invrender_create", "rt_invrender_run", "rt_invrender_destroy", "rt_scene_update", "rt_accel_refit",
 ]
 
 
@@ -80,7 +80,8 @@ class rt_invrender_params(C.Structure):
 
 class rt_accel_info(C.Structure):
     _fields_ = [("nodes", C.c_int64), ("leaves", C.c_int64), ("triangles", C.c_int64), ("width", C.c_int32),
-                ("depth", C.c_int32), ("bytes", C.c_int64), ("build_ms", C.c_float), ("sah_cost", C.c_float)]
+                ("depth", C.c_int32), ("bytes", C.c_int64), ("build_ms", C.c_float), ("sah_cost", C.c_float),
+                ("refit_ms", C.c_float)]
 
 
 def load_library(path=LIB_PATH):
@@ -118,6 +119,8 @@ def load_library(path=LIB_PATH):
                                     C.POINTER(C.c_float)]),
         "rt_gen_surface_rays": (i32, [vp, u64, i64, fp]),
         "rt_scene_set_albedo": (i32, [vp, fp, u32]),
+        "rt_scene_update": (i32, [vp, fp, u32]),
+        "rt_accel_refit": (i32, [vp]),
         "rt_render_tape": (i32, [vp, C.POINTER(rt_camera), C.POINTER(rt_light), i32, C.POINTER(rt_render_params), fp,
                                  i64, C.POINTER(vp), C.POINTER(rt_stats)]),
         "rt_tape_info": (i32, [vp, C.POINTER(i32), C.POINTER(i64), C.POINTER(i64)]),
@@ -233,6 +236,25 @@ class Scene:
             a = albedo.detach().contiguous()
             self.ctx.check(_LIB.rt_scene_set_albedo(self.h, _ptr(a), RT_DEVICE_PTRS), "rt_scene_set_albedo")
 
+    def update(self, abc):
+        """New vertex positions, (n, 9) float32: a host array or a device tensor.
+        Accels built on the scene must be refit (Accel.refit) or rebuilt
+        before they are traced again."""
+        if isinstance(abc, np.ndarray):
+            a = np.ascontiguousarray(abc, np.float32).reshape(-1, 9)
+            if a.shape[0] != self.n:
+                raise ValueError(f"rt_scene_update: {a.shape[0]} triangles, the scene has {self.n}")
+            self.ctx.check(_LIB.rt_scene_update(self.h, _ptr(a), 0), "rt_scene_update")
+        else:
+            import torch
+
+            a = abc.detach().to(torch.float32).contiguous().reshape(-1, 9)
+            if a.shape[0] != self.n:
+                raise ValueError(f"rt_scene_update: {a.shape[0]} triangles, the scene has {self.n}")
+            torch.cuda.current_stream().synchronize()  # the library's stream reads it next
+            self.ctx.check(_LIB.rt_scene_update(self.h, _ptr(a), RT_DEVICE_PTRS), "rt_scene_update")
+            torch.cuda.synchronize()  # the copy has read a before a is released
+
     def normals(self):
         out = np.empty((self.n, 3), np.float32)
         self.ctx.check(_LIB.rt_scene_normals(self.h, _ptr(out)), "rt_scene_normals")
@@ -308,6 +330,10 @@ class Accel:
         h = C.c_void_p()
         self.ctx.check(_LIB.rt_accel_build(scene.h, C.byref(p), C.byref(h)), "rt_accel_build")
         self.h = h
+
+    def refit(self):
+        """Fit the boxes to the scene's current vertices, keeping the tree."""
+        self.ctx.check(_LIB.rt_accel_refit(self.h), "rt_accel_refit")
 
     def info(self):
         i = rt_accel_info()

@@ -109,6 +109,29 @@ def terrain(grid=708, x_res=1920, y_res=1080, bounce=True, seed=7, zoom=0.02, am
                     meta=dict(triangles=int(abc.shape[0]), grid=grid, resolution=[x_res, y_res]))
 
 
+def terrain_wave(abc, phase, amplitude=0.01, wavelength=0.25):
+    """Terrain vertices under a travelling wave: height (y) += amplitude *
+    sin(2 pi (x / wavelength - phase)) * cos(2 pi z / wavelength). Works on
+    numpy arrays and torch tensors of shape (n, 9)."""
+    mod = np if isinstance(abc, np.ndarray) else __import__("torch")
+    v = abc.reshape(-1, 3, 3)
+    x, z = v[:, :, 0], v[:, :, 2]
+    k = 2.0 * np.pi / wavelength
+    dy = amplitude * mod.sin(k * x - 2.0 * np.pi * phase) * mod.cos(k * z)
+    out = v.clone() if mod is not np else v.copy()
+    out[:, :, 1] = v[:, :, 1] + dy.astype(np.float32) if mod is np else v[:, :, 1] + dy
+    return out.reshape(abc.shape)
+
+
+def drift(abc, step, seed=3, speed=0.002):
+    """Rigid per-triangle motion (bodies of a physics step): triangle i moves
+    by step * speed * u_i with u_i uniform in [-1, 1]^3 (seeded). numpy."""
+    rng = np.random.default_rng(seed)
+    vel = rng.uniform(-1.0, 1.0, (abc.shape[0], 1, 3)).astype(np.float32)
+    v = abc.reshape(-1, 3, 3) + np.float32(step * speed) * vel
+    return v.reshape(abc.shape).astype(np.float32)
+
+
 def terrain1m(x_res=1920, y_res=1080):
     return terrain(708, x_res, y_res)
 

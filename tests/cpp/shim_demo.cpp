@@ -91,7 +91,30 @@ int main() {
         if (id != out.prim || h.d != out.t) ++id_bad;
         if (accel.found_nearer_object(&r, 0.01f)) ++id_bad;  // nothing within 0.01 of z = -1
     }
-    std::printf("%s: %d pixel mismatches, %d lit pixels, %d ray mismatches\n",
-                (bad == 0 && id_bad == 0 && lit > 100) ? "shim ok" : "shim FAILED", bad, lit, id_bad);
-    return (bad == 0 && id_bad == 0 && lit > 100) ? 0 : 1;
+    // The physics loop: move every triangle, refit, and compare with a new
+    // accelerator over the moved store.
+    for (auto &t : prims.tris)
+        for (point *v : {&t.a, &t.b, &t.c}) {
+            v->x += 0.05f * std::sin(7.0f * v->y);
+            v->z += 0.03f;
+        }
+    accel.refit(prims);
+    raptor_gpu::gpu_bvh<primitive_store> fresh(ctx, prims, [&](int i, float *rgb) {
+        for (int k = 0; k < 3; ++k) rgb[k] = alb[3 * i + k];
+    });
+    camera c1{64, std::vector<colour>(64 * 64, colour(-1, -1, -1))};
+    camera c2{64, std::vector<colour>(64 * 64, colour(-1, -1, -1))};
+    raptor_gpu::ray_tracer(accel, lights, cam, c1);
+    raptor_gpu::ray_tracer(fresh, lights, cam, c2);
+    int moved_bad = 0, changed = 0;
+    for (int p = 0; p < 64 * 64; ++p) {
+        if (c1.image[p].r != c2.image[p].r || c1.image[p].g != c2.image[p].g || c1.image[p].b != c2.image[p].b)
+            ++moved_bad;
+        if (c1.image[p].r != c.image[p].r) ++changed;
+    }
+    const bool ok = bad == 0 && id_bad == 0 && lit > 100 && moved_bad == 0 && changed > 100;
+    std::printf("%s: %d pixel mismatches, %d lit pixels, %d ray mismatches, %d refit/rebuild mismatches, "
+                "%d pixels changed by the move\n",
+                ok ? "shim ok" : "shim FAILED", bad, lit, id_bad, moved_bad, changed);
+    return ok ? 0 : 1;
 }
