@@ -117,6 +117,7 @@ void release_ctx(rt_ctx *ctx) {
         if (e) (void)hipEventDestroy(e);
     if (ctx->d_counters) (void)hipFree(ctx->d_counters);
     if (ctx->d_queue) (void)hipFree(ctx->d_queue);
+    ctx->arena.release();
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }

@@ -185,6 +185,7 @@ __global__ void __launch_bounds__(256) k_gather_leaf_boxes(const uint32_t *__res
 }
 
 constexpr int kMaxLevels = 256;
+constexpr int kLevelGroup = 8;  // level launches per host read-back of the level sizes
 
 __device__ __forceinline__ void child_box(int32_t c, const float4 *node_lo, const float4 *node_hi,
                                           const float4 *leaf_lo, const float4 *leaf_hi, float4 *lo, float4 *hi) {
@@ -203,22 +204,46 @@ __device__ __forceinline__ void child_box(int32_t c, const float4 *node_lo, cons
 // nodes level by level (one launch per level, wave-aggregated appends), and
 // one launch per level, deepest first, unions the two child boxes; kernel
 // boundaries order the levels.
-__global__ void __launch_bounds__(256) k_bfs_level(const int32_t *__restrict__ level, int count,
-                                                   const int2 *__restrict__ child, int32_t *__restrict__ next,
-                                                   uint32_t *__restrict__ next_count) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+// Top-down passes keep their levels on the device: level l occupies
+// positions [lvl[2l], lvl[2l + 1]) of the pass's order array, and *total
+// counts what has been enqueued. A level kernel reads its range from lvl, and
+// k_level_close (one thread, launched after it) records the next level's
+// range, so the host reads the level sizes back once per group of levels
+// instead of once per level.
+__global__ void __launch_bounds__(256) k_bfs_level(int32_t *__restrict__ order, const uint32_t *__restrict__ lvl, int l,
+                                                   const int2 *__restrict__ child, uint32_t *__restrict__ total) {
+    const uint32_t lo = lvl[2 * l], hi = lvl[2 * l + 1];
+    const int64_t k = static_cast<int64_t>(lo) + static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k - (threadIdx.x & 63) >= hi) return;  // whole wave past the level
     int2 ch = make_int2(-1, -1);
-    if (k < count) ch = child[level[k]];
+    if (k < hi) ch = child[order[k]];
     const bool a = ch.x >= 0, b = ch.y >= 0;
     const uint64_t ma = __ballot(a), mb = __ballot(b);
     const int lane = threadIdx.x & 63;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    const uint32_t total = static_cast<uint32_t>(__popcll(ma) + __popcll(mb));
+    const uint32_t cnt = static_cast<uint32_t>(__popcll(ma) + __popcll(mb));
     uint32_t base = 0;
-    if (lane == 0 && total) base = atomicAdd(next_count, total);
+    if (lane == 0 && cnt) base = atomicAdd(total, cnt);
     base = __shfl(base, 0, 64);
-    if (a) next[base + __popcll(ma & lt)] = ch.x;
-    if (b) next[base + __popcll(ma) + __popcll(mb & lt)] = ch.y;
+    if (a) order[base + __popcll(ma & lt)] = ch.x;
+    if (b) order[base + __popcll(ma) + __popcll(mb & lt)] = ch.y;
+}
+
+__global__ void k_level_init(int32_t *__restrict__ order, int32_t root, uint32_t *__restrict__ lvl,
+                             uint32_t *__restrict__ total) {
+    if (threadIdx.x == 0) {
+        if (order) order[0] = root;
+        lvl[0] = 0;
+        lvl[1] = 1;
+        *total = 1;
+    }
+}
+
+__global__ void k_level_close(uint32_t *__restrict__ lvl, int l, const uint32_t *__restrict__ total) {
+    if (threadIdx.x == 0) {
+        lvl[2 * l + 2] = lvl[2 * l + 1];
+        lvl[2 * l + 3] = *total;
+    }
 }
 
 __global__ void __launch_bounds__(256) k_refit_level(const int32_t *__restrict__ nodes_at_level, int count,
@@ -244,10 +269,9 @@ struct CollapseArgs {
     const int2 *child;
     const int2 *range;
     const float4 *node_lo, *node_hi, *leaf_lo, *leaf_hi;
-    const int2 *q_in;        // (binary internal node, wide node index)
-    int q_in_n;
-    int2 *q_out;
-    uint32_t *q_out_n;
+    int32_t *q;              // binary node of each wide node, by wide node index
+    const uint32_t *lvl;     // wide nodes of tree level d: [lvl[2d], lvl[2d + 1])
+    int level;
     uint32_t *node_counter;
     void *nodes;
     int leaf_size;
@@ -257,9 +281,10 @@ struct CollapseArgs {
 
 template <int W>
 __global__ void __launch_bounds__(256) k_collapse(CollapseArgs a) {
-    const int qi = blockIdx.x * blockDim.x + threadIdx.x;
-    if (qi >= a.q_in_n) return;
-    const int2 job = a.q_in[qi];
+    const uint32_t lo = a.lvl[2 * a.level], hi = a.lvl[2 * a.level + 1];
+    const int64_t idx = static_cast<int64_t>(lo) + static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= hi) return;
+    const int2 job = make_int2(a.q[idx], static_cast<int32_t>(idx));
     int32_t c[W];
     float area[W];
     int cnt = 2;
@@ -321,10 +346,9 @@ __global__ void __launch_bounds__(256) k_collapse(CollapseArgs a) {
             out.child[k] = leaf_code(r.x, r.y - r.x + 1);
             ++leaves;
         } else {
-            const uint32_t idx = atomicAdd(a.node_counter, 1u);
-            out.child[k] = static_cast<int32_t>(idx);
-            const uint32_t slot = atomicAdd(a.q_out_n, 1u);
-            a.q_out[slot] = make_int2(x, static_cast<int32_t>(idx));
+            const uint32_t nidx = atomicAdd(a.node_counter, 1u);
+            out.child[k] = static_cast<int32_t>(nidx);
+            a.q[nidx] = x;
         }
     }
     for (int k = 0; k < (W == 4 ? 4 : 8); ++k) out.pad[k] = 0;
@@ -371,9 +395,10 @@ __global__ void __launch_bounds__(256) k_leaf_tris(const float *__restrict__ abc
 // internal nodes in the binary arrays; the tree's root becomes the SAH root.
 //
 // Cut size, measured on the 1M-triangle terrain frame (BVH4, leaf 2): the
-// LBVH alone traces 3190 Mrays/s (4.7 ms build); clusters of 8 triangles
-// 3470 (30 ms); single triangles, i.e. the whole tree rebuilt by SAH over the
-// Morton-ordered triangles, 3750-3780 (31 ms with the sweep for small nodes).
+// LBVH alone traces 3190 Mrays/s; clusters of 8 triangles 3470; single
+// triangles, i.e. the whole tree rebuilt by SAH over the Morton-ordered
+// triangles, 3750-3780, for a build 6-7x the LBVH's (30-31 ms against 4.7 ms
+// then; 10.3 ms against 1.8 ms since the level loops stay on the device).
 constexpr int kSahCluster = 1;
 constexpr int kSahSweepItems = 16;
 constexpr int kSahBinsMax = 32;
@@ -392,7 +417,16 @@ struct SahSplit {
     int32_t bin;          // binned plane: items with bin <= this go left; -1: plane below
     int32_t nl;           // items on the left
     float plane;          // swept plane: items with centroid < plane go left
+    int32_t next[2];      // next-level node of the left / right side, -1 for a single item
 };
+
+// Items per block of the binning passes: a block walks kSahChunk tiles of 256
+// items and accumulates runs of tiles that belong to one node in LDS, so the
+// top levels (a few nodes of up to n items) make one set of global atomics
+// per block instead of per wave. Updates of a few words by every wave
+// serialise across the eight XCDs' L2s: the per-wave form took 0.43 ms
+// (bins) and 0.29 ms (centroid bounds) per level on average for 1M items.
+constexpr int kSahChunk = 8;
 
 __global__ void __launch_bounds__(256) k_fill_i32(int32_t *p, int64_t n, int32_t v) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -454,62 +488,111 @@ __global__ void __launch_bounds__(256) k_make_items(const int32_t *__restrict__ 
     it_node[j] = 0;
 }
 
-__global__ void __launch_bounds__(256) k_sah_classify(SahNode *__restrict__ nodes, int count,
-                                                      uint32_t *__restrict__ slots) {
+// Counters of a SAH level: ctr[0] bin slots handed out, ctr[1] nodes of the
+// next level, ctr[2] SAH nodes, ctr[3] partition error (every level kernel
+// returns at once when it is set). count: this level's node count.
+__global__ void __launch_bounds__(256) k_sah_classify(SahNode *__restrict__ nodes, const uint32_t *__restrict__ count,
+                                                      uint32_t *__restrict__ ctr, uint32_t *__restrict__ cb) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= count) return;
+    if (ctr[3] || k >= static_cast<int>(*count)) return;
     SahNode nd = nodes[k];
-    nd.slot = (nd.end - nd.begin > kSahSweepItems) ? static_cast<int32_t>(atomicAdd(slots, 1u)) : -1;
+    nd.slot = (nd.end - nd.begin > kSahSweepItems) ? static_cast<int32_t>(atomicAdd(&ctr[0], 1u)) : -1;
     nodes[k] = nd;
+    if (nd.slot >= 0)
+        for (int w = 0; w < 6; ++w) cb[6 * static_cast<int64_t>(nd.slot) + w] = w < 3 ? 0xffffffffu : 0u;
 }
 
 __device__ __forceinline__ float3 item_centroid(float4 lo, float4 hi) {
     return make_float3((lo.x + hi.x) * 0.5f, (lo.y + hi.y) * 0.5f, (lo.z + hi.z) * 0.5f);
 }
 
-// Centroid bounds of each binned node (6 order-mapped words per slot). A wave
-// whose lanes share one node reduces in registers first.
-__global__ void __launch_bounds__(256) k_sah_cbounds(const float4 *__restrict__ it_lo, const float4 *__restrict__ it_hi,
-                                                     const int32_t *__restrict__ it_node, int64_t m,
-                                                     const SahNode *__restrict__ nodes, uint32_t *__restrict__ cb) {
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    int32_t slot = -1;
-    float3 c = make_float3(0.0f, 0.0f, 0.0f);
-    if (i < m) {
-        const int32_t nd = it_node[i];
-        if (nd >= 0) {
-            slot = nodes[nd].slot;
-            c = item_centroid(it_lo[i], it_hi[i]);
-        }
-    }
-    const int32_t s0 = __shfl(slot, 0, 64);
-    const bool uniform = __ballot(slot == s0) == ~0ull;
-    if (uniform) {
-        if (s0 < 0) return;
-        float v[6] = {c.x, c.y, c.z, c.x, c.y, c.z};
+// Runs of equal key in a wave: after the call the first lane of each run
+// holds the run's minima (v[0..2]), maxima (v[3..5]) and sums (c[0..1]), and
+// the result says whether this lane is such a first lane. Items arrive
+// grouped by node and in Morton order within a node, so a wave's 64 items
+// fall into a few runs of one (node, bin) and one atomic update per run
+// replaces one per item (per-item updates of a node's few bins took up to
+// 1.3 ms per level of 1M items).
+// Lanes are matched by run index, not by key: one key can make several runs
+// in a wave (bins alternate along a node's items), and matching keys across
+// runs would count items twice.
+__device__ __forceinline__ bool run_reduce(int32_t key, float (&v)[6], uint32_t (&c)[2]) {
+    const int lane = threadIdx.x & 63;
+    const bool head = lane == 0 || __shfl_up(key, 1, 64) != key;
+    const int run = __popcll(__ballot(head) & (~0ull >> (63 - lane)));  // heads at lanes <= this one
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int ro = __shfl_down(run, o, 64);
+        const bool take = lane + o < 64 && ro == run;
 #pragma unroll
         for (int k = 0; k < 6; ++k) {
+            const float y = __shfl_down(v[k], o, 64);
+            if (take) v[k] = k < 3 ? fminf(v[k], y) : fmaxf(v[k], y);
+        }
 #pragma unroll
-            for (int o = 32; o > 0; o >>= 1) {
-                const float y = __shfl_xor(v[k], o, 64);
-                v[k] = (k < 3) ? fminf(v[k], y) : fmaxf(v[k], y);
+        for (int k = 0; k < 2; ++k) {
+            const uint32_t y = __shfl_down(c[k], o, 64);
+            if (take) c[k] += y;
+        }
+    }
+    return head;
+}
+
+// Centroid bounds of each binned node (6 order-mapped words per slot). Each
+// wave reduces its runs of one node's items in registers; runs of tiles that
+// all belong to one node then accumulate in LDS, others update the node's
+// words directly.
+__global__ void __launch_bounds__(256) k_sah_cbounds(const float4 *__restrict__ it_lo, const float4 *__restrict__ it_hi,
+                                                     const int32_t *__restrict__ it_node, int64_t m,
+                                                     const SahNode *__restrict__ nodes, uint32_t *__restrict__ cb,
+                                                     const uint32_t *__restrict__ ctr) {
+    __shared__ uint32_t lcb[6];
+    if (ctr[3]) return;  // the same in every thread
+    __shared__ int32_t tile_slot;
+    int32_t cur = -1;  // node slot accumulated in lcb (the same in every thread)
+    if (threadIdx.x < 6) lcb[threadIdx.x] = threadIdx.x < 3 ? 0xffffffffu : 0u;
+    auto flush = [&](int32_t sl) {
+        if (threadIdx.x < 6) {
+            uint32_t *b = cb + 6 * static_cast<int64_t>(sl) + threadIdx.x;
+            if (threadIdx.x < 3)
+                atomicMin(b, lcb[threadIdx.x]);
+            else
+                atomicMax(b, lcb[threadIdx.x]);
+            lcb[threadIdx.x] = threadIdx.x < 3 ? 0xffffffffu : 0u;
+        }
+    };
+    for (int t = 0; t < kSahChunk; ++t) {
+        const int64_t base = (static_cast<int64_t>(blockIdx.x) * kSahChunk + t) * blockDim.x;
+        if (base >= m) break;
+        const int64_t i = base + threadIdx.x;
+        int32_t slot = -1;
+        float3 c = make_float3(0.0f, 0.0f, 0.0f);
+        if (i < m) {
+            const int32_t nd = it_node[i];
+            if (nd >= 0) {
+                slot = nodes[nd].slot;
+                c = item_centroid(it_lo[i], it_hi[i]);
             }
         }
-        if ((threadIdx.x & 63) == 0) {
-            uint32_t *b = cb + 6 * static_cast<int64_t>(s0);
+        if (threadIdx.x == 0) tile_slot = slot;
+        __syncthreads();
+        const int32_t ts = tile_slot;
+        const bool block_uniform = __syncthreads_and(slot == ts) && ts >= 0;
+        if (block_uniform && ts != cur) {
+            if (cur >= 0) flush(cur);
+            __syncthreads();
+            cur = ts;
+        }
+        float v[6] = {c.x, c.y, c.z, c.x, c.y, c.z};
+        uint32_t cnt[2] = {0u, 0u};
+        if (run_reduce(slot, v, cnt) && slot >= 0) {
+            uint32_t *b = block_uniform ? lcb : cb + 6 * static_cast<int64_t>(slot);
             for (int k = 0; k < 3; ++k) atomicMin(&b[k], f2ord(v[k]));
             for (int k = 3; k < 6; ++k) atomicMax(&b[k], f2ord(v[k]));
         }
-        return;
+        __syncthreads();
     }
-    if (slot < 0) return;
-    uint32_t *b = cb + 6 * static_cast<int64_t>(slot);
-    atomicMin(&b[0], f2ord(c.x));
-    atomicMin(&b[1], f2ord(c.y));
-    atomicMin(&b[2], f2ord(c.z));
-    atomicMax(&b[3], f2ord(c.x));
-    atomicMax(&b[4], f2ord(c.y));
-    atomicMax(&b[5], f2ord(c.z));
+    if (cur >= 0) flush(cur);
 }
 
 __device__ __forceinline__ int sah_bin_of(float c, float lo, float hi, int nbins) {
@@ -519,67 +602,92 @@ __device__ __forceinline__ int sah_bin_of(float c, float lo, float hi, int nbins
     return min(max(b, 0), nbins - 1);
 }
 
-// Bins per binned node: [slot][axis][bin][count, tris, lo xyz, hi xyz]. A
-// block whose items all belong to one node bins into LDS first and then
-// merges its non-empty bins with one set of global atomics.
-__device__ __forceinline__ void bin_add(uint32_t *w, float4 lo, float4 hi, uint32_t tris) {
-    atomicAdd(&w[0], 1u);
+// Bins per binned node: [slot][axis][bin][count, tris, lo xyz, hi xyz]. Runs
+// of tiles whose items all belong to one node bin into LDS first and merge
+// their non-empty bins with one set of global atomics. A node's bins are
+// cleared by k_sah_split once read, so they start empty on every level.
+__device__ __forceinline__ void bin_add(uint32_t *w, uint32_t count, uint32_t tris, float lx, float ly, float lz,
+                                        float hx, float hy, float hz) {
+    atomicAdd(&w[0], count);
     atomicAdd(&w[1], tris);
-    atomicMin(&w[2], f2ord(lo.x));
-    atomicMin(&w[3], f2ord(lo.y));
-    atomicMin(&w[4], f2ord(lo.z));
-    atomicMax(&w[5], f2ord(hi.x));
-    atomicMax(&w[6], f2ord(hi.y));
-    atomicMax(&w[7], f2ord(hi.z));
+    atomicMin(&w[2], f2ord(lx));
+    atomicMin(&w[3], f2ord(ly));
+    atomicMin(&w[4], f2ord(lz));
+    atomicMax(&w[5], f2ord(hx));
+    atomicMax(&w[6], f2ord(hy));
+    atomicMax(&w[7], f2ord(hz));
 }
 
 __global__ void __launch_bounds__(256) k_sah_bin(const float4 *__restrict__ it_lo, const float4 *__restrict__ it_hi,
                                                  const int32_t *__restrict__ it_node, int64_t m,
                                                  const SahNode *__restrict__ nodes, const uint32_t *__restrict__ cb,
-                                                 int nbins, uint32_t *__restrict__ bins) {
+                                                 int nbins, uint32_t *__restrict__ bins, const uint32_t *__restrict__ ctr) {
     __shared__ uint32_t lbins[3 * kSahBinsMax * kSahBinWords];
-    __shared__ int32_t block_slot;
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    int32_t slot = -1;
-    if (i < m) {
-        const int32_t nd = it_node[i];
-        if (nd >= 0) slot = nodes[nd].slot;
-    }
-    // Uniform block: every thread's slot equals thread 0's (and is valid).
-    if (threadIdx.x == 0) block_slot = slot;
+    if (ctr[3]) return;  // the same in every thread
+    __shared__ int32_t tile_slot;
     const int words = 3 * nbins * kSahBinWords;
     for (int k = threadIdx.x; k < words; k += blockDim.x) {
         const int w = k % kSahBinWords;
         lbins[k] = (w >= 2 && w < 5) ? 0xffffffffu : 0u;
     }
-    __syncthreads();
-    const int32_t bs = block_slot;
-    const int uniform = __syncthreads_and(slot == bs && bs >= 0);
-    if (slot >= 0) {
-        const float4 lo = it_lo[i], hi = it_hi[i];
-        const float3 c = item_centroid(lo, hi);
-        const uint32_t tris = static_cast<uint32_t>(__float_as_int(hi.w));
-        const uint32_t *b = cb + 6 * static_cast<int64_t>(slot);
-        const float cc[3] = {c.x, c.y, c.z};
-        for (int a = 0; a < 3; ++a) {
-            const int k = sah_bin_of(cc[a], ord2f(b[a]), ord2f(b[3 + a]), nbins);
-            if (uniform)
-                bin_add(lbins + (a * nbins + k) * kSahBinWords, lo, hi, tris);
-            else
-                bin_add(bins + ((static_cast<int64_t>(slot) * 3 + a) * nbins + k) * kSahBinWords, lo, hi, tris);
+    int32_t cur = -1;  // node slot binned in lbins (the same in every thread)
+    auto flush = [&](int32_t sl) {  // LDS bins -> global, then cleared
+        for (int k = threadIdx.x; k < 3 * nbins; k += blockDim.x) {
+            uint32_t *l = lbins + k * kSahBinWords;
+            if (l[0] == 0) continue;
+            uint32_t *g = bins + (static_cast<int64_t>(sl) * 3 * nbins + k) * kSahBinWords;
+            atomicAdd(&g[0], l[0]);
+            atomicAdd(&g[1], l[1]);
+            for (int w = 2; w < 5; ++w) atomicMin(&g[w], l[w]);
+            for (int w = 5; w < 8; ++w) atomicMax(&g[w], l[w]);
+            l[0] = l[1] = 0u;
+            l[2] = l[3] = l[4] = 0xffffffffu;
+            l[5] = l[6] = l[7] = 0u;
         }
+    };
+    for (int t = 0; t < kSahChunk; ++t) {
+        const int64_t base = (static_cast<int64_t>(blockIdx.x) * kSahChunk + t) * blockDim.x;
+        if (base >= m) break;
+        const int64_t i = base + threadIdx.x;
+        int32_t slot = -1;
+        if (i < m) {
+            const int32_t nd = it_node[i];
+            if (nd >= 0) slot = nodes[nd].slot;
+        }
+        if (threadIdx.x == 0) tile_slot = slot;
+        __syncthreads();
+        const int32_t ts = tile_slot;
+        const bool uniform = __syncthreads_and(slot == ts) && ts >= 0;
+        if (uniform && ts != cur) {
+            if (cur >= 0) flush(cur);
+            __syncthreads();
+            cur = ts;
+        }
+        float4 lo = make_float4(0.0f, 0.0f, 0.0f, 0.0f), hi = lo;
+        float cc[3] = {0.0f, 0.0f, 0.0f};
+        const uint32_t *b = cb;
+        if (slot >= 0) {
+            lo = it_lo[i];
+            hi = it_hi[i];
+            const float3 c = item_centroid(lo, hi);
+            cc[0] = c.x;
+            cc[1] = c.y;
+            cc[2] = c.z;
+            b = cb + 6 * static_cast<int64_t>(slot);
+        }
+        for (int a = 0; a < 3; ++a) {
+            const int k = slot >= 0 ? sah_bin_of(cc[a], ord2f(b[a]), ord2f(b[3 + a]), nbins) : 0;
+            float v[6] = {lo.x, lo.y, lo.z, hi.x, hi.y, hi.z};
+            uint32_t c[2] = {1u, static_cast<uint32_t>(__float_as_int(hi.w))};
+            const bool head = run_reduce(slot >= 0 ? slot * kSahBinsMax + k : -1, v, c);
+            if (!head || slot < 0) continue;
+            uint32_t *w = uniform ? lbins + (a * nbins + k) * kSahBinWords
+                                  : bins + ((static_cast<int64_t>(slot) * 3 + a) * nbins + k) * kSahBinWords;
+            bin_add(w, c[0], c[1], v[0], v[1], v[2], v[3], v[4], v[5]);
+        }
+        __syncthreads();
     }
-    if (!uniform) return;
-    __syncthreads();
-    for (int k = threadIdx.x; k < 3 * nbins; k += blockDim.x) {
-        const uint32_t *l = lbins + k * kSahBinWords;
-        if (l[0] == 0) continue;
-        uint32_t *g = bins + (static_cast<int64_t>(bs) * 3 * nbins + k) * kSahBinWords;
-        atomicAdd(&g[0], l[0]);
-        atomicAdd(&g[1], l[1]);
-        for (int w = 2; w < 5; ++w) atomicMin(&g[w], l[w]);
-        for (int w = 5; w < 8; ++w) atomicMax(&g[w], l[w]);
-    }
+    if (cur >= 0) flush(cur);
 }
 
 struct Box3 {
@@ -603,20 +711,20 @@ struct Box3 {
 };
 
 // Exact sweep over a small node's items: every item centroid is a candidate
-// plane on every axis (items with centroid < plane go left).
-__device__ void sah_sweep(const float4 *it_lo, const float4 *it_hi, int32_t begin, int32_t end, SahSplit *sp) {
+// plane on every axis (items with centroid < plane go left). The items are
+// staged in LDS first, item k of lane t at [k][t] (conflict-free columns).
+__device__ void sah_sweep(const float4 (*s_lo)[64], const float4 (*s_hi)[64], int lane, int32_t size, SahSplit *sp) {
     float best = INFINITY;
-    const int32_t size = end - begin;
     for (int a = 0; a < 3; ++a) {
-        for (int32_t j = begin; j < end; ++j) {
-            const float3 cj = item_centroid(it_lo[j], it_hi[j]);
+        for (int32_t j = 0; j < size; ++j) {
+            const float3 cj = item_centroid(s_lo[j][lane], s_hi[j][lane]);
             const float v = a == 0 ? cj.x : (a == 1 ? cj.y : cj.z);
             float llx = INFINITY, lly = INFINITY, llz = INFINITY, lhx = -INFINITY, lhy = -INFINITY, lhz = -INFINITY;
             float rlx = INFINITY, rly = INFINITY, rlz = INFINITY, rhx = -INFINITY, rhy = -INFINITY, rhz = -INFINITY;
             uint32_t lt = 0, rt_ = 0;
             int32_t nl = 0;
-            for (int32_t k = begin; k < end; ++k) {
-                const float4 lo = it_lo[k], hi = it_hi[k];
+            for (int32_t k = 0; k < size; ++k) {
+                const float4 lo = s_lo[k][lane], hi = s_hi[k][lane];
                 const float3 ck = item_centroid(lo, hi);
                 const float ca = a == 0 ? ck.x : (a == 1 ? ck.y : ck.z);
                 const uint32_t t = static_cast<uint32_t>(__float_as_int(hi.w));
@@ -647,27 +755,32 @@ __device__ void sah_sweep(const float4 *it_lo, const float4 *it_hi, int32_t begi
     }
 }
 
-// One thread per node: the best binned plane (large nodes) or swept plane
-// (small nodes), else the middle. Children with more than one item get an
-// SAH node index and a place on the next level; single-item sides are linked
-// by k_sah_scatter.
-__global__ void __launch_bounds__(64) k_sah_split(const SahNode *__restrict__ nodes, int count,
+// One thread per node: the best binned plane (large nodes; their bins are
+// cleared for the next level once read) or swept plane (small nodes), else
+// the middle. Children with more than one item get an SAH node index and a
+// place on the next level; single-item sides are linked by k_sah_scatter.
+__global__ void __launch_bounds__(64) k_sah_split(const SahNode *__restrict__ nodes, const uint32_t *__restrict__ count,
                                                   const float4 *__restrict__ it_lo, const float4 *__restrict__ it_hi,
-                                                  const uint32_t *__restrict__ bins, int nbins,
+                                                  uint32_t *__restrict__ bins, int nbins,
                                                   SahSplit *__restrict__ splits, SahNode *__restrict__ next,
-                                                  uint32_t *__restrict__ next_count, uint32_t *__restrict__ sah_count,
-                                                  int2 *__restrict__ sah_child, int2 *__restrict__ sah_items) {
+                                                  uint32_t *__restrict__ ctr, int2 *__restrict__ sah_child,
+                                                  int2 *__restrict__ sah_items) {
+    __shared__ float4 s_lo[kSahSweepItems][64], s_hi[kSahSweepItems][64];
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= count) return;
+    if (ctr[3] || k >= static_cast<int>(*count)) return;
     const SahNode nd = nodes[k];
     const int32_t size = nd.end - nd.begin;
-    SahSplit sp{-1, 0, size / 2, 0.0f};
+    SahSplit sp{-1, 0, size / 2, 0.0f, {-1, -1}};
     if (nd.slot < 0) {
-        sah_sweep(it_lo, it_hi, nd.begin, nd.end, &sp);
+        for (int32_t j = 0; j < size; ++j) {
+            s_lo[j][threadIdx.x] = it_lo[nd.begin + j];
+            s_hi[j][threadIdx.x] = it_hi[nd.begin + j];
+        }
+        sah_sweep(s_lo, s_hi, threadIdx.x, size, &sp);
     } else {
         float best = INFINITY;
         for (int a = 0; a < 3; ++a) {
-            const uint32_t *base = bins + (static_cast<int64_t>(nd.slot) * 3 + a) * nbins * kSahBinWords;
+            uint32_t *base = bins + (static_cast<int64_t>(nd.slot) * 3 + a) * nbins * kSahBinWords;
             // Right-to-left suffix areas and counts, then the left sweep.
             float r_area[kSahBinsMax];
             uint32_t r_tris[kSahBinsMax];
@@ -699,23 +812,30 @@ __global__ void __launch_bounds__(64) k_sah_split(const SahNode *__restrict__ no
                     sp.nl = static_cast<int32_t>(cnt);
                 }
             }
+            for (int b = 0; b < nbins; ++b) {
+                uint32_t *w = base + b * kSahBinWords;
+                w[0] = w[1] = 0u;
+                w[2] = w[3] = w[4] = 0xffffffffu;
+                w[5] = w[6] = w[7] = 0u;
+            }
         }
     }
-    splits[k] = sp;
     sah_items[nd.id] = make_int2(nd.begin, nd.end);
     int2 ch = make_int2(0, 0);
     const int32_t sizes[2] = {sp.nl, size - sp.nl};
     const int32_t begins[2] = {nd.begin, nd.begin + sp.nl};
     for (int side = 0; side < 2; ++side) {
         if (sizes[side] <= 1) continue;  // linked by the scatter
-        const int32_t id = static_cast<int32_t>(atomicAdd(sah_count, 1u));
-        const uint32_t q = atomicAdd(next_count, 1u);
+        const int32_t id = static_cast<int32_t>(atomicAdd(&ctr[2], 1u));
+        const uint32_t q = atomicAdd(&ctr[1], 1u);
         next[q] = SahNode{begins[side], begins[side] + sizes[side], id, -1};
+        sp.next[side] = static_cast<int32_t>(q);
         if (side == 0)
             ch.x = kSahRef | id;
         else
             ch.y = kSahRef | id;
     }
+    splits[k] = sp;
     // Sides of one item are written by the scatter; keep the other half.
     if (sizes[0] > 1) sah_child[nd.id].x = ch.x;
     if (sizes[1] > 1) sah_child[nd.id].y = ch.y;
@@ -736,9 +856,9 @@ __global__ void __launch_bounds__(256) k_sah_flags(const float4 *__restrict__ it
                                                    const int32_t *__restrict__ it_node, int64_t m,
                                                    const SahNode *__restrict__ nodes, const SahSplit *__restrict__ splits,
                                                    const uint32_t *__restrict__ cb, int nbins,
-                                                   uint32_t *__restrict__ flags) {
+                                                   uint32_t *__restrict__ flags, const uint32_t *__restrict__ ctr) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= m) return;
+    if (ctr[3] || i >= m) return;
     const int32_t nd = it_node[i];
     bool left = false;
     if (nd >= 0) left = sah_goes_left(nodes[nd], splits[nd], cb, nbins, it_lo[i], it_hi[i], i);
@@ -753,9 +873,10 @@ __global__ void __launch_bounds__(256) k_sah_scatter(const float4 *__restrict__ 
                                                      const SahSplit *__restrict__ splits, const uint32_t *__restrict__ cb,
                                                      int nbins, const uint32_t *__restrict__ scanned,
                                                      float4 *__restrict__ out_lo, float4 *__restrict__ out_hi,
-                                                     int32_t *__restrict__ out_node, int2 *__restrict__ sah_child) {
+                                                     int32_t *__restrict__ out_node, int2 *__restrict__ sah_child,
+                                                     uint32_t *__restrict__ ctr) {
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= m) return;
+    if (ctr[3] || i >= m) return;
     const float4 lo = it_lo[i], hi = it_hi[i];
     const int32_t ndi = it_node[i];
     if (ndi < 0) {
@@ -769,6 +890,12 @@ __global__ void __launch_bounds__(256) k_sah_scatter(const float4 *__restrict__ 
     const bool left = sah_goes_left(nd, sp, cb, nbins, lo, hi, i);
     const int64_t rl = static_cast<int64_t>(scanned[i]) - static_cast<int64_t>(scanned[nd.begin]);
     const int64_t dest = left ? nd.begin + rl : nd.begin + sp.nl + (i - nd.begin - rl);
+    // The split's left count comes from the bins and the sides from the
+    // flags; a disagreement would move items out of their node's range.
+    if (left ? dest >= nd.begin + sp.nl : (dest < nd.begin + sp.nl || dest >= nd.end)) {
+        atomicOr(&ctr[3], 1u);
+        return;
+    }
     out_lo[dest] = lo;
     out_hi[dest] = hi;
     const int32_t side_size = left ? sp.nl : (nd.end - nd.begin - sp.nl);
@@ -780,28 +907,23 @@ __global__ void __launch_bounds__(256) k_sah_scatter(const float4 *__restrict__ 
         else
             sah_child[nd.id].y = ref;
     } else {
-        out_node[dest] = -2;  // resolved by k_sah_assign
+        out_node[dest] = sp.next[left ? 0 : 1];
     }
 }
 
-// Items of the next level's nodes learn their node index.
-__global__ void __launch_bounds__(64) k_sah_assign(const SahNode *__restrict__ next, int count,
-                                                    int32_t *__restrict__ it_node) {
-    const int k = blockIdx.x;
-    if (k >= count) return;
-    const SahNode nd = next[k];
-    for (int32_t i = nd.begin + threadIdx.x; i < nd.end; i += blockDim.x) it_node[i] = k;
+// Empty bins for every slot, once per build (k_sah_split clears what it reads).
+__global__ void __launch_bounds__(256) k_sah_reset(uint32_t *__restrict__ bins, int64_t words) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= words) return;
+    const int w = static_cast<int>(i % kSahBinWords);
+    bins[i] = (w >= 2 && w < 5) ? 0xffffffffu : 0u;
 }
 
-__global__ void __launch_bounds__(256) k_sah_reset(uint32_t *__restrict__ cb, uint32_t *__restrict__ bins, int nslots,
-                                                   int nbins) {
-    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    const int64_t ncb = 6 * static_cast<int64_t>(nslots);
-    const int64_t nbw = static_cast<int64_t>(nslots) * 3 * nbins * kSahBinWords;
-    if (i < ncb) cb[i] = (i % 6) < 3 ? 0xffffffffu : 0u;
-    if (i < nbw) {
-        const int w = static_cast<int>(i % kSahBinWords);
-        bins[i] = (w >= 2 && w < 5) ? 0xffffffffu : 0u;
+__global__ void k_sah_close(uint32_t *__restrict__ ctr, uint32_t *__restrict__ count, int l) {
+    if (threadIdx.x == 0) {
+        count[l + 1] = ctr[1];
+        ctr[0] = 0;
+        ctr[1] = 0;
     }
 }
 
@@ -880,16 +1002,61 @@ hipError_t exclusive_scan_u32(uint32_t *data, int64_t n, uint32_t *tile_sums, hi
     return hipGetLastError();
 }
 
+// A build's temporary buffer, carved from the context's arena.
 struct DevBuf {
     void *p = nullptr;
-    ~DevBuf() {
-        if (p) (void)hipFree(p);
-    }
 };
 
+}  // namespace
+
+hipError_t BuildArena::begin() {
+    if (chunks.size() > 1 || (!chunks.empty() && chunks[0].second < high)) {
+        release();
+    }
+    if (chunks.empty() && high > 0) {
+        void *p = nullptr;
+        const hipError_t e = hipMalloc(&p, high);
+        if (e != hipSuccess) return e;
+        chunks.push_back({p, high});
+    }
+    cur = 0;
+    used = 0;
+    taken = 0;
+    return hipSuccess;
+}
+
+hipError_t BuildArena::take(void **p, size_t bytes) {
+    bytes = (std::max<size_t>(1, bytes) + 255) & ~size_t(255);
+    taken += bytes;
+    high = std::max(high, taken);
+    while (cur < chunks.size() && used + bytes > chunks[cur].second) {
+        ++cur;
+        used = 0;
+    }
+    if (cur == chunks.size()) {
+        void *q = nullptr;
+        const size_t size = std::max<size_t>(bytes, size_t(64) << 20);
+        const hipError_t e = hipMalloc(&q, size);
+        if (e != hipSuccess) return e;
+        chunks.push_back({q, size});
+        used = 0;
+    }
+    *p = static_cast<char *>(chunks[cur].first) + used;
+    used += bytes;
+    return hipSuccess;
+}
+
+void BuildArena::release() {
+    for (auto &c : chunks) (void)hipFree(c.first);
+    chunks.clear();
+    cur = used = 0;
+}
+
+namespace {
+
 template <class T>
-hipError_t dalloc(DevBuf &b, size_t count) {
-    return hipMalloc(&b.p, std::max<size_t>(1, count) * sizeof(T));
+hipError_t dalloc_in(BuildArena &ar, DevBuf &b, size_t count) {
+    return ar.take(&b.p, count * sizeof(T));
 }
 
 }  // namespace
@@ -904,6 +1071,10 @@ int32_t build_accel(rt_scene *scene, const rt_build_params &p, rt_accel **out) {
     hipStream_t s = ctx->stream;
     const auto t0 = std::chrono::steady_clock::now();
 
+    {
+        const hipError_t e = ctx->arena.begin();
+        if (e != hipSuccess) return fail(ctx, RT_ERR_OOM, std::string("build scratch: ") + hipGetErrorString(e));
+    }
     rt_accel *a = new rt_accel();
     a->scene = scene;
     a->triangles = n;
@@ -911,6 +1082,7 @@ int32_t build_accel(rt_scene *scene, const rt_build_params &p, rt_accel **out) {
     a->leaf_size = p.leaf_size;
     a->lds_stack = (p.lds_stack <= 0) ? 16 : (p.lds_stack <= 4 ? 4 : (p.lds_stack <= 8 ? 8 : 16));
     auto cleanup = [&]() {
+        (void)hipStreamSynchronize(s);  // the arena's buffers may be reused once this returns
         if (a->d_nodes) (void)hipFree(a->d_nodes);
         if (a->d_tris) (void)hipFree(a->d_tris);
         delete a;
@@ -933,15 +1105,15 @@ int32_t build_accel(rt_scene *scene, const rt_build_params &p, rt_accel **out) {
 
     DevBuf b_aabb_lo, b_aabb_hi, b_bounds, b_codes, b_idx, b_codes_t, b_idx_t, b_hist, b_sums;
     const int64_t tiles = (n + kSortTile - 1) / kSortTile;
-    BUILD_CHECK(dalloc<float4>(b_aabb_lo, n));
-    BUILD_CHECK(dalloc<float4>(b_aabb_hi, n));
-    BUILD_CHECK(dalloc<uint32_t>(b_bounds, 12));
-    BUILD_CHECK(dalloc<uint32_t>(b_codes, n));
-    BUILD_CHECK(dalloc<uint32_t>(b_idx, n));
-    BUILD_CHECK(dalloc<uint32_t>(b_codes_t, n));
-    BUILD_CHECK(dalloc<uint32_t>(b_idx_t, n));
-    BUILD_CHECK(dalloc<uint32_t>(b_hist, 256 * tiles));
-    BUILD_CHECK(dalloc<uint32_t>(b_sums, kScanTile));
+    BUILD_CHECK(dalloc_in<float4>(ctx->arena, b_aabb_lo, n));
+    BUILD_CHECK(dalloc_in<float4>(ctx->arena, b_aabb_hi, n));
+    BUILD_CHECK(dalloc_in<uint32_t>(ctx->arena, b_bounds, 12));
+    BUILD_CHECK(dalloc_in<uint32_t>(ctx->arena, b_codes, n));
+    BUILD_CHECK(dalloc_in<uint32_t>(ctx->arena, b_idx, n));
+    BUILD_CHECK(dalloc_in<uint32_t>(ctx->arena, b_codes_t, n));
+    BUILD_CHECK(dalloc_in<uint32_t>(ctx->arena, b_idx_t, n));
+    BUILD_CHECK(dalloc_in<uint32_t>(ctx->arena, b_hist, 256 * tiles));
+    BUILD_CHECK(dalloc_in<uint32_t>(ctx->arena, b_sums, kScanTile));
     uint32_t init_bounds[12];
     for (int k = 0; k < 12; ++k) {
         const bool is_min = (k < 3) || (k >= 6 && k < 9);
@@ -990,50 +1162,58 @@ int32_t build_accel(rt_scene *scene, const rt_build_params &p, rt_accel **out) {
     // Binary nodes: the LBVH's n - 1, then up to n - 1 SAH nodes.
     const int64_t bcap = (n - 1) + (refine ? n : 0);
     DevBuf b_child, b_range, b_parent, b_leaf_parent, b_leaf_lo, b_leaf_hi, b_node_lo, b_node_hi;
-    BUILD_CHECK(dalloc<int2>(b_child, bcap));
-    BUILD_CHECK(dalloc<int2>(b_range, bcap));
-    BUILD_CHECK(dalloc<int32_t>(b_parent, n - 1));
-    BUILD_CHECK(dalloc<int32_t>(b_leaf_parent, n));
-    BUILD_CHECK(dalloc<float4>(b_leaf_lo, n));
-    BUILD_CHECK(dalloc<float4>(b_leaf_hi, n));
-    BUILD_CHECK(dalloc<float4>(b_node_lo, bcap));
-    BUILD_CHECK(dalloc<float4>(b_node_hi, bcap));
+    BUILD_CHECK(dalloc_in<int2>(ctx->arena, b_child, bcap));
+    BUILD_CHECK(dalloc_in<int2>(ctx->arena, b_range, bcap));
+    BUILD_CHECK(dalloc_in<int32_t>(ctx->arena, b_parent, n - 1));
+    BUILD_CHECK(dalloc_in<int32_t>(ctx->arena, b_leaf_parent, n));
+    BUILD_CHECK(dalloc_in<float4>(ctx->arena, b_leaf_lo, n));
+    BUILD_CHECK(dalloc_in<float4>(ctx->arena, b_leaf_hi, n));
+    BUILD_CHECK(dalloc_in<float4>(ctx->arena, b_node_lo, bcap));
+    BUILD_CHECK(dalloc_in<float4>(ctx->arena, b_node_hi, bcap));
     const int iblocks = static_cast<int>((n - 1 + 255) / 256);
     hipLaunchKernelGGL(k_karras, dim3(iblocks), dim3(256), 0, s, (const uint32_t *)b_codes.p, n, (int2 *)b_child.p,
                        (int2 *)b_range.p, (int32_t *)b_parent.p, (int32_t *)b_leaf_parent.p);
     hipLaunchKernelGGL(k_gather_leaf_boxes, dim3(blocks), dim3(256), 0, s, (const uint32_t *)b_idx.p, n,
                        (const float4 *)b_aabb_lo.p, (const float4 *)b_aabb_hi.p, (float4 *)b_leaf_lo.p,
                        (float4 *)b_leaf_hi.p);
-    // Level-synchronous refit from a root (see k_bfs_level).
-    DevBuf b_order, b_cnt;
-    BUILD_CHECK(dalloc<int32_t>(b_order, n - 1));
-    BUILD_CHECK(dalloc<uint32_t>(b_cnt, 1));
+    // Level-synchronous refit from a root (see k_bfs_level): the levels are
+    // found top-down in groups of kLevelGroup launches per host read-back,
+    // then refit bottom-up.
+    DevBuf b_order, b_lvl;
+    BUILD_CHECK(dalloc_in<int32_t>(ctx->arena, b_order, n - 1));
+    BUILD_CHECK(dalloc_in<uint32_t>(ctx->arena, b_lvl, 2 * (kMaxLevels + kLevelGroup + 2) + 1));
+    uint32_t *lvl = (uint32_t *)b_lvl.p;
+    uint32_t *lvl_total = lvl + 2 * (kMaxLevels + kLevelGroup + 2);
     auto refit = [&](int32_t root) -> hipError_t {
         int32_t *order = (int32_t *)b_order.p;
-        uint32_t *cnt = (uint32_t *)b_cnt.p;
-        hipError_t e = hipMemcpyAsync(order, &root, sizeof(int32_t), hipMemcpyHostToDevice, s);
-        if (e != hipSuccess) return e;
-        std::vector<std::pair<uint32_t, uint32_t>> levels;  // (offset, count)
-        uint32_t off = 0, count = 1;
-        while (count > 0) {
-            levels.push_back({off, count});
-            if (static_cast<int>(levels.size()) > kMaxLevels) return hipErrorInvalidValue;
-            if ((e = hipMemsetAsync(cnt, 0, sizeof(uint32_t), s)) != hipSuccess) return e;
-            hipLaunchKernelGGL(k_bfs_level, dim3((count + 255) / 256), dim3(256), 0, s, order + off,
-                               static_cast<int>(count), (const int2 *)b_child.p, order + off + count, cnt);
-            uint32_t next = 0;
-            if ((e = hipMemcpyAsync(&next, cnt, sizeof(uint32_t), hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
-            if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
-            off += count;
-            count = next;
+        hipLaunchKernelGGL(k_level_init, dim3(1), dim3(64), 0, s, order, root, lvl, lvl_total);
+        std::vector<uint32_t> hl;
+        int levels = -1;
+        for (int l = 0; levels < 0;) {
+            if (l > kMaxLevels) return hipErrorInvalidValue;
+            for (int g = 0; g < kLevelGroup; ++g, ++l) {
+                const int64_t bound = std::min<int64_t>(l < 30 ? (int64_t(1) << l) : n, n - 1);
+                hipLaunchKernelGGL(k_bfs_level, dim3(static_cast<unsigned>((bound + 255) / 256)), dim3(256), 0, s,
+                                   order, lvl, l, (const int2 *)b_child.p, lvl_total);
+                hipLaunchKernelGGL(k_level_close, dim3(1), dim3(64), 0, s, lvl, l, lvl_total);
+            }
+            hl.resize(2 * (l + 1));
+            hipError_t e = hipMemcpyAsync(hl.data(), lvl, hl.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, s);
+            if (e == hipSuccess) e = hipStreamSynchronize(s);
+            if (e != hipSuccess) return e;
+            for (int k = 0; k <= l; ++k)
+                if (hl[2 * k] == hl[2 * k + 1]) {
+                    levels = k;
+                    break;
+                }
         }
-        for (int l = static_cast<int>(levels.size()) - 1; l >= 0; --l) {
-            hipLaunchKernelGGL(k_refit_level, dim3((levels[l].second + 255) / 256), dim3(256), 0, s,
-                               order + levels[l].first, static_cast<int>(levels[l].second), (const int2 *)b_child.p,
-                               (const float4 *)b_leaf_lo.p, (const float4 *)b_leaf_hi.p, (float4 *)b_node_lo.p,
-                               (float4 *)b_node_hi.p);
+        for (int l = levels - 1; l >= 0; --l) {
+            const uint32_t off = hl[2 * l], count = hl[2 * l + 1] - hl[2 * l];
+            hipLaunchKernelGGL(k_refit_level, dim3((count + 255) / 256), dim3(256), 0, s, order + off,
+                               static_cast<int>(count), (const int2 *)b_child.p, (const float4 *)b_leaf_lo.p,
+                               (const float4 *)b_leaf_hi.p, (float4 *)b_node_lo.p, (float4 *)b_node_hi.p);
         }
-        a->binary_depth = static_cast<int>(levels.size());
+        a->binary_depth = levels;
         return hipGetLastError();
     };
     {
@@ -1050,8 +1230,8 @@ int32_t build_accel(rt_scene *scene, const rt_build_params &p, rt_accel **out) {
     if (refine) {
         const int nbins = std::min(std::max(p.sah_bins, 4), kSahBinsMax);
         DevBuf b_cat, b_pos;
-        BUILD_CHECK(dalloc<int32_t>(b_cat, n));
-        BUILD_CHECK(dalloc<uint32_t>(b_pos, n));
+        BUILD_CHECK(dalloc_in<int32_t>(ctx->arena, b_cat, n));
+        BUILD_CHECK(dalloc_in<uint32_t>(ctx->arena, b_pos, n));
         int32_t *cat = (int32_t *)b_cat.p;
         uint32_t *pos = (uint32_t *)b_pos.p;
         hipLaunchKernelGGL(k_fill_i32, dim3(blocks), dim3(256), 0, s, cat, n, kNoCluster);
@@ -1069,19 +1249,19 @@ int32_t build_accel(rt_scene *scene, const rt_build_params &p, rt_accel **out) {
             const int mblocks = static_cast<int>((m + 255) / 256);
             DevBuf b_lo[2], b_hi[2], b_nd[2], b_flags, b_nodes[2], b_splits, b_sc, b_si, b_ctr, b_cb, b_bins;
             for (int k = 0; k < 2; ++k) {
-                BUILD_CHECK(dalloc<float4>(b_lo[k], m));
-                BUILD_CHECK(dalloc<float4>(b_hi[k], m));
-                BUILD_CHECK(dalloc<int32_t>(b_nd[k], m));
-                BUILD_CHECK(dalloc<SahNode>(b_nodes[k], m));
+                BUILD_CHECK(dalloc_in<float4>(ctx->arena, b_lo[k], m));
+                BUILD_CHECK(dalloc_in<float4>(ctx->arena, b_hi[k], m));
+                BUILD_CHECK(dalloc_in<int32_t>(ctx->arena, b_nd[k], m));
+                BUILD_CHECK(dalloc_in<SahNode>(ctx->arena, b_nodes[k], m));
             }
-            BUILD_CHECK(dalloc<uint32_t>(b_flags, m));
-            BUILD_CHECK(dalloc<SahSplit>(b_splits, m));
-            BUILD_CHECK(dalloc<int2>(b_sc, m - 1));
-            BUILD_CHECK(dalloc<int2>(b_si, m - 1));
-            BUILD_CHECK(dalloc<uint32_t>(b_ctr, 4));  // [0] bin slots, [1] next-level nodes, [2] SAH nodes
+            BUILD_CHECK(dalloc_in<uint32_t>(ctx->arena, b_flags, m));
+            BUILD_CHECK(dalloc_in<SahSplit>(ctx->arena, b_splits, m));
+            BUILD_CHECK(dalloc_in<int2>(ctx->arena, b_sc, m - 1));
+            BUILD_CHECK(dalloc_in<int2>(ctx->arena, b_si, m - 1));
+            BUILD_CHECK(dalloc_in<uint32_t>(ctx->arena, b_ctr, 4));  // [0] bin slots, [1] next-level nodes, [2] SAH nodes
             const int64_t max_slots = m / (kSahSweepItems + 1) + 1;
-            BUILD_CHECK(dalloc<uint32_t>(b_cb, 6 * max_slots));
-            BUILD_CHECK(dalloc<uint32_t>(b_bins, max_slots * 3 * nbins * kSahBinWords));
+            BUILD_CHECK(dalloc_in<uint32_t>(ctx->arena, b_cb, 6 * max_slots));
+            BUILD_CHECK(dalloc_in<uint32_t>(ctx->arena, b_bins, max_slots * 3 * nbins * kSahBinWords));
             hipLaunchKernelGGL(k_make_items, dim3(blocks), dim3(256), 0, s, cat, pos, n, (const int2 *)b_range.p,
                                (const float4 *)b_node_lo.p, (const float4 *)b_node_hi.p, (const float4 *)b_leaf_lo.p,
                                (const float4 *)b_leaf_hi.p, (float4 *)b_lo[0].p, (float4 *)b_hi[0].p,
@@ -1091,67 +1271,83 @@ int32_t build_accel(rt_scene *scene, const rt_build_params &p, rt_accel **out) {
             uint32_t *ctr = (uint32_t *)b_ctr.p;
             const uint32_t init_ctr[4] = {0, 0, 1, 0};
             BUILD_CHECK(hipMemcpyAsync(ctr, init_ctr, sizeof(init_ctr), hipMemcpyHostToDevice, s));
-            int cur = 0;
-            int count = 1;
-            int levels = 0;
-            while (count > 0) {
-                if (++levels > 4096) {
+            const int64_t bin_words = max_slots * 3 * nbins * kSahBinWords;
+            hipLaunchKernelGGL(k_sah_reset, dim3(static_cast<unsigned>((bin_words + 255) / 256)), dim3(256), 0, s,
+                               (uint32_t *)b_bins.p, bin_words);
+            const int cblocks = static_cast<int>((m + 256 * kSahChunk - 1) / (256 * kSahChunk));
+            // Level L reads nodes from b_nodes[L & 1] and items from b_lo/hi/nd[L & 1]
+            // and writes the next level to the other buffers; its node count is
+            // scount[L] on the device. Levels are launched in groups with one
+            // read-back of the counts per group; levels after the last one
+            // only carry the finished items over to the other buffers.
+            constexpr int kMaxSahLevels = 4096;
+            DevBuf b_scount;
+            BUILD_CHECK(dalloc_in<uint32_t>(ctx->arena, b_scount, kMaxSahLevels + kLevelGroup + 2));
+            uint32_t *scount = (uint32_t *)b_scount.p;
+            const uint32_t one_node = 1;
+            BUILD_CHECK(hipMemcpyAsync(scount, &one_node, sizeof(uint32_t), hipMemcpyHostToDevice, s));
+            uint32_t *cb = (uint32_t *)b_cb.p, *bins = (uint32_t *)b_bins.p;
+            int launched = 0;
+            for (bool done = false; !done;) {
+                if (launched > kMaxSahLevels) {
                     cleanup();
                     return fail(ctx, RT_ERR_INVALID, "SAH refinement did not converge");
                 }
-                SahNode *nodes = (SahNode *)b_nodes[cur].p;
-                SahNode *next = (SahNode *)b_nodes[cur ^ 1].p;
-                const float4 *ilo = (const float4 *)b_lo[cur].p, *ihi = (const float4 *)b_hi[cur].p;
-                const int32_t *ind = (const int32_t *)b_nd[cur].p;
-                BUILD_CHECK(hipMemsetAsync(ctr, 0, 2 * sizeof(uint32_t), s));
-                hipLaunchKernelGGL(k_sah_classify, dim3((count + 255) / 256), dim3(256), 0, s, nodes, count, ctr);
-                uint32_t nslots = 0;
-                BUILD_CHECK(hipMemcpyAsync(&nslots, ctr, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-                BUILD_CHECK(hipStreamSynchronize(s));
-                uint32_t *cb = (uint32_t *)b_cb.p, *bins = (uint32_t *)b_bins.p;
-                if (nslots > 0) {
-                    const int64_t words = std::max<int64_t>(6 * int64_t(nslots), int64_t(nslots) * 3 * nbins * kSahBinWords);
-                    hipLaunchKernelGGL(k_sah_reset, dim3(static_cast<unsigned>((words + 255) / 256)), dim3(256), 0, s,
-                                       cb, bins, static_cast<int>(nslots), nbins);
-                    hipLaunchKernelGGL(k_sah_cbounds, dim3(mblocks), dim3(256), 0, s, ilo, ihi, ind, m, nodes, cb);
-                    hipLaunchKernelGGL(k_sah_bin, dim3(mblocks), dim3(256), 0, s, ilo, ihi, ind, m, nodes, cb, nbins,
-                                       bins);
+                for (int g = 0; g < kLevelGroup; ++g, ++launched) {
+                    const int cur = launched & 1;
+                    SahNode *nodes = (SahNode *)b_nodes[cur].p;
+                    SahNode *next = (SahNode *)b_nodes[cur ^ 1].p;
+                    const float4 *ilo = (const float4 *)b_lo[cur].p, *ihi = (const float4 *)b_hi[cur].p;
+                    const int32_t *ind = (const int32_t *)b_nd[cur].p;
+                    const uint32_t *count = scount + launched;
+                    const int64_t bound = std::min<int64_t>(launched < 30 ? (int64_t(1) << launched) : m, m);
+                    const unsigned nb256 = static_cast<unsigned>((bound + 255) / 256);
+                    const unsigned nb64 = static_cast<unsigned>((bound + 63) / 64);
+                    hipLaunchKernelGGL(k_sah_classify, dim3(nb256), dim3(256), 0, s, nodes, count, ctr, cb);
+                    hipLaunchKernelGGL(k_sah_cbounds, dim3(cblocks), dim3(256), 0, s, ilo, ihi, ind, m, nodes, cb,
+                                       (const uint32_t *)ctr);
+                    hipLaunchKernelGGL(k_sah_bin, dim3(cblocks), dim3(256), 0, s, ilo, ihi, ind, m, nodes, cb, nbins,
+                                       bins, (const uint32_t *)ctr);
+                    hipLaunchKernelGGL(k_sah_split, dim3(nb64), dim3(64), 0, s, nodes, count, ilo, ihi, bins, nbins,
+                                       (SahSplit *)b_splits.p, next, ctr, (int2 *)b_sc.p, (int2 *)b_si.p);
+                    hipLaunchKernelGGL(k_sah_flags, dim3(mblocks), dim3(256), 0, s, ilo, ihi, ind, m, nodes,
+                                       (const SahSplit *)b_splits.p, cb, nbins, (uint32_t *)b_flags.p,
+                                       (const uint32_t *)ctr);
+                    BUILD_CHECK(exclusive_scan_u32((uint32_t *)b_flags.p, m, (uint32_t *)b_sums.p, s));
+                    hipLaunchKernelGGL(k_sah_scatter, dim3(mblocks), dim3(256), 0, s, ilo, ihi, ind, m, nodes,
+                                       (const SahSplit *)b_splits.p, cb, nbins, (const uint32_t *)b_flags.p,
+                                       (float4 *)b_lo[cur ^ 1].p, (float4 *)b_hi[cur ^ 1].p,
+                                       (int32_t *)b_nd[cur ^ 1].p, (int2 *)b_sc.p, ctr);
+                    hipLaunchKernelGGL(k_sah_close, dim3(1), dim3(64), 0, s, ctr, scount, launched);
                 }
-                hipLaunchKernelGGL(k_sah_split, dim3((count + 63) / 64), dim3(64), 0, s, nodes, count, ilo, ihi, bins,
-                                   nbins,
-                                   (SahSplit *)b_splits.p, next, ctr + 1, ctr + 2, (int2 *)b_sc.p, (int2 *)b_si.p);
-                hipLaunchKernelGGL(k_sah_flags, dim3(mblocks), dim3(256), 0, s, ilo, ihi, ind, m, nodes,
-                                   (const SahSplit *)b_splits.p, cb, nbins, (uint32_t *)b_flags.p);
-                BUILD_CHECK(exclusive_scan_u32((uint32_t *)b_flags.p, m, (uint32_t *)b_sums.p, s));
-                hipLaunchKernelGGL(k_sah_scatter, dim3(mblocks), dim3(256), 0, s, ilo, ihi, ind, m, nodes,
-                                   (const SahSplit *)b_splits.p, cb, nbins, (const uint32_t *)b_flags.p,
-                                   (float4 *)b_lo[cur ^ 1].p, (float4 *)b_hi[cur ^ 1].p, (int32_t *)b_nd[cur ^ 1].p,
-                                   (int2 *)b_sc.p);
-                uint32_t nnext = 0;
-                BUILD_CHECK(hipMemcpyAsync(&nnext, ctr + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-                BUILD_CHECK(hipStreamSynchronize(s));
-                if (nnext > 0)
-                    hipLaunchKernelGGL(k_sah_assign, dim3(nnext), dim3(64), 0, s, next, static_cast<int>(nnext),
-                                       (int32_t *)b_nd[cur ^ 1].p);
                 BUILD_CHECK(hipGetLastError());
-                cur ^= 1;
-                count = static_cast<int>(nnext);
+                std::vector<uint32_t> hs(launched + 1);
+                uint32_t hc[4] = {0, 0, 0, 0};
+                BUILD_CHECK(hipMemcpyAsync(hs.data(), scount, hs.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+                BUILD_CHECK(hipMemcpyAsync(hc, ctr, sizeof(hc), hipMemcpyDeviceToHost, s));
+                BUILD_CHECK(hipStreamSynchronize(s));
+                if (hc[3]) {
+                    cleanup();
+                    return fail(ctx, RT_ERR_HIP, "SAH refinement: inconsistent partition (internal error)");
+                }
+                for (uint32_t c : hs) done = done || c == 0;
             }
+            const int cur = launched & 1;  // where the last level left the items
             // Triangles in the final cluster order.
             const float4 *flo = (const float4 *)b_lo[cur].p, *fhi = (const float4 *)b_hi[cur].p;
             (void)flo;
             DevBuf b_start, b_newpos;
-            BUILD_CHECK(dalloc<uint32_t>(b_start, m));
-            BUILD_CHECK(dalloc<int32_t>(b_newpos, n));
+            BUILD_CHECK(dalloc_in<uint32_t>(ctx->arena, b_start, m));
+            BUILD_CHECK(dalloc_in<int32_t>(ctx->arena, b_newpos, n));
             uint32_t *start = (uint32_t *)b_start.p;
             int32_t *newpos = (int32_t *)b_newpos.p;
             hipLaunchKernelGGL(k_item_tris, dim3(mblocks), dim3(256), 0, s, fhi, m, start);
             BUILD_CHECK(exclusive_scan_u32(start, m, (uint32_t *)b_sums.p, s));
             hipLaunchKernelGGL(k_cluster_newpos, dim3(mblocks), dim3(256), 0, s, (const float4 *)b_lo[cur].p, fhi,
                                start, m, (const int2 *)b_range.p, newpos);
-            BUILD_CHECK(dalloc<uint32_t>(b_idx2, n));
-            BUILD_CHECK(dalloc<float4>(b_llo2, n));
-            BUILD_CHECK(dalloc<float4>(b_lhi2, n));
+            BUILD_CHECK(dalloc_in<uint32_t>(ctx->arena, b_idx2, n));
+            BUILD_CHECK(dalloc_in<float4>(ctx->arena, b_llo2, n));
+            BUILD_CHECK(dalloc_in<float4>(ctx->arena, b_lhi2, n));
             hipLaunchKernelGGL(k_permute_leaves, dim3(blocks), dim3(256), 0, s, newpos, n, (const uint32_t *)b_idx.p,
                                (const float4 *)b_leaf_lo.p, (const float4 *)b_leaf_hi.p, (uint32_t *)b_idx2.p,
                                (float4 *)b_llo2.p, (float4 *)b_lhi2.p);
@@ -1184,54 +1380,61 @@ int32_t build_accel(rt_scene *scene, const rt_build_params &p, rt_accel **out) {
     const size_t node_bytes = (W == 4) ? sizeof(WideNode<4>) : sizeof(WideNode<8>);
     const int64_t max_nodes = std::max<int64_t>(1, n - 1);
     BUILD_CHECK(hipMalloc(&a->d_nodes, max_nodes * node_bytes));
-    DevBuf b_qa, b_qb, b_counters;
-    BUILD_CHECK(dalloc<int2>(b_qa, max_nodes));
-    BUILD_CHECK(dalloc<int2>(b_qb, max_nodes));
-    BUILD_CHECK(dalloc<unsigned long long>(b_counters, 4));
+    // Wide nodes are numbered level by level (a level's nodes are handed out
+    // while the level above is collapsed), so q[wide index] = binary node is
+    // the work list and the levels are ranges of wide indices.
+    DevBuf b_q, b_counters;
+    BUILD_CHECK(dalloc_in<int32_t>(ctx->arena, b_q, max_nodes));
+    BUILD_CHECK(dalloc_in<unsigned long long>(ctx->arena, b_counters, 4));
     BUILD_CHECK(hipMemsetAsync(b_counters.p, 0, 4 * sizeof(unsigned long long), s));
-    const int2 root_job = make_int2(root, 0);
-    BUILD_CHECK(hipMemcpyAsync(b_qa.p, &root_job, sizeof(int2), hipMemcpyHostToDevice, s));
-    const uint32_t one = 1;
     uint32_t *node_counter = reinterpret_cast<uint32_t *>(b_counters.p);
-    uint32_t *qout_n = node_counter + 1;
     unsigned long long *leaf_counter = reinterpret_cast<unsigned long long *>(b_counters.p) + 1;
-    BUILD_CHECK(hipMemcpyAsync(node_counter, &one, sizeof(uint32_t), hipMemcpyHostToDevice, s));
-    int qn = 1;
-    int depth = 0;
-    int2 *qin = (int2 *)b_qa.p, *qout = (int2 *)b_qb.p;
-    a->level_start.assign({0, 1});  // level L = wide nodes [level_start[L], level_start[L + 1])
-    while (qn > 0) {
-        BUILD_CHECK(hipMemsetAsync(qout_n, 0, sizeof(uint32_t), s));
-        CollapseArgs ca;
-        ca.child = (const int2 *)b_child.p;
-        ca.range = (const int2 *)b_range.p;
-        ca.node_lo = (const float4 *)b_node_lo.p;
-        ca.node_hi = (const float4 *)b_node_hi.p;
-        ca.leaf_lo = (const float4 *)b_leaf_lo.p;
-        ca.leaf_hi = (const float4 *)b_leaf_hi.p;
-        ca.q_in = qin;
-        ca.q_in_n = qn;
-        ca.q_out = qout;
-        ca.q_out_n = qout_n;
-        ca.node_counter = node_counter;
-        ca.nodes = a->d_nodes;
-        ca.leaf_size = p.leaf_size;
-        ca.pad = a->box_pad;
-        ca.leaf_counter = leaf_counter;
-        const int cb = (qn + 255) / 256;
-        if (W == 4)
-            hipLaunchKernelGGL(k_collapse<4>, dim3(cb), dim3(256), 0, s, ca);
-        else
-            hipLaunchKernelGGL(k_collapse<8>, dim3(cb), dim3(256), 0, s, ca);
+    hipLaunchKernelGGL(k_level_init, dim3(1), dim3(64), 0, s, (int32_t *)b_q.p, root, lvl, node_counter);
+    int depth = -1;
+    std::vector<uint32_t> hl;
+    for (int d = 0; depth < 0;) {
+        if (d > kMaxLevels) {
+            cleanup();
+            return fail(ctx, RT_ERR_INVALID, "wide tree deeper than 256 levels");
+        }
+        for (int g = 0; g < kLevelGroup; ++g, ++d) {
+            CollapseArgs ca;
+            ca.child = (const int2 *)b_child.p;
+            ca.range = (const int2 *)b_range.p;
+            ca.node_lo = (const float4 *)b_node_lo.p;
+            ca.node_hi = (const float4 *)b_node_hi.p;
+            ca.leaf_lo = (const float4 *)b_leaf_lo.p;
+            ca.leaf_hi = (const float4 *)b_leaf_hi.p;
+            ca.q = (int32_t *)b_q.p;
+            ca.lvl = lvl;
+            ca.level = d;
+            ca.node_counter = node_counter;
+            ca.nodes = a->d_nodes;
+            ca.leaf_size = p.leaf_size;
+            ca.pad = a->box_pad;
+            ca.leaf_counter = leaf_counter;
+            int64_t bound = 1;
+            for (int k = 0; k < d && bound < max_nodes; ++k) bound *= W;
+            bound = std::min<int64_t>(bound, max_nodes);
+            const dim3 g_dim(static_cast<unsigned>((bound + 255) / 256));
+            if (W == 4)
+                hipLaunchKernelGGL(k_collapse<4>, g_dim, dim3(256), 0, s, ca);
+            else
+                hipLaunchKernelGGL(k_collapse<8>, g_dim, dim3(256), 0, s, ca);
+            hipLaunchKernelGGL(k_level_close, dim3(1), dim3(64), 0, s, lvl, d, node_counter);
+        }
         BUILD_CHECK(hipGetLastError());
-        uint32_t cn[2] = {0, 0};  // node counter, next queue length
-        BUILD_CHECK(hipMemcpyAsync(cn, node_counter, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        hl.resize(2 * (d + 1));
+        BUILD_CHECK(hipMemcpyAsync(hl.data(), lvl, hl.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
         BUILD_CHECK(hipStreamSynchronize(s));
-        qn = static_cast<int>(cn[1]);
-        if (qn > 0) a->level_start.push_back(cn[0]);
-        std::swap(qin, qout);
-        ++depth;
+        for (int k = 0; k <= d; ++k)
+            if (hl[2 * k] == hl[2 * k + 1]) {
+                depth = k;
+                break;
+            }
     }
+    a->level_start.assign(depth + 1, 0);  // level L = wide nodes [level_start[L], level_start[L + 1])
+    for (int k = 0; k <= depth; ++k) a->level_start[k] = hl[2 * k];
     uint64_t counters[2];
     BUILD_CHECK(hipMemcpyAsync(counters, b_counters.p, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
     BUILD_CHECK(hipStreamSynchronize(s));

@@ -50,6 +50,22 @@ struct alignas(16) LeafTri {
 
 }  // namespace rt
 
+namespace rt {
+
+// Scratch memory of BVH builds, kept on the context between builds: device
+// chunks handed out by a bump pointer. A build that outgrows the chunks takes
+// another; the next build merges them into one chunk of the high-water mark,
+// so repeated builds of one scene allocate nothing.
+struct BuildArena {
+    std::vector<std::pair<void *, size_t>> chunks;
+    size_t cur = 0, used = 0, taken = 0, high = 0;
+    hipError_t begin();
+    hipError_t take(void **p, size_t bytes);
+    void release();
+};
+
+}  // namespace rt
+
 struct rt_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -60,6 +76,7 @@ struct rt_ctx {
     // slots 0..3, the stack-overflow flag at index 15.
     unsigned long long *d_counters = nullptr;
     uint32_t *d_queue = nullptr;  // ray queue head of the persistent trace kernels
+    rt::BuildArena arena;         // build scratch (bvh_build.hip)
     int count_slot = 0;
     std::atomic<int> refs{1};  // the caller's handle + one per scene
 };
