@@ -43,10 +43,13 @@ __host__ __device__ inline float ord2f(uint32_t u) {
 }
 
 // bounds[0..2] min of centroid, [3..5] max of centroid, [6..8] min of AABB,
-// [9..11] max of AABB, all order-mapped.
+// [9..11] max of AABB, all order-mapped. Each block writes its 12 partial
+// bounds to parts[k * gridDim.x + block] and k_bounds_final reduces them: one
+// atomic per block on 12 words took 97 us for 1M triangles (the updates
+// serialise across the XCDs' L2s), the second pass takes a few.
 __global__ void __launch_bounds__(256) k_tri_bounds(const float *__restrict__ abc, int64_t n,
                                                     float4 *__restrict__ aabb_lo, float4 *__restrict__ aabb_hi,
-                                                    uint32_t *__restrict__ bounds) {
+                                                    uint32_t *__restrict__ parts) {
     __shared__ uint32_t red[12][4];
     const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     float v[12];
@@ -84,10 +87,35 @@ __global__ void __launch_bounds__(256) k_tri_bounds(const float *__restrict__ ab
         const bool is_min = (k < 3) || (k >= 6 && k < 9);
         uint32_t r = red[k][0];
         for (int w = 1; w < 4; ++w) r = is_min ? min(r, red[k][w]) : max(r, red[k][w]);
-        if (is_min)
-            atomicMin(&bounds[k], r);
-        else
-            atomicMax(&bounds[k], r);
+        parts[static_cast<int64_t>(k) * gridDim.x + blockIdx.x] = r;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_bounds_final(const uint32_t *__restrict__ parts, int nparts,
+                                                      uint32_t *__restrict__ bounds) {
+    __shared__ uint32_t red[12][4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int k = 0; k < 12; ++k) {
+        const bool is_min = (k < 3) || (k >= 6 && k < 9);
+        uint32_t r = is_min ? 0xffffffffu : 0u;
+        for (int b = threadIdx.x; b < nparts; b += blockDim.x) {
+            const uint32_t y = parts[static_cast<int64_t>(k) * nparts + b];
+            r = is_min ? min(r, y) : max(r, y);
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const uint32_t y = static_cast<uint32_t>(__shfl_xor(static_cast<int>(r), o, 64));
+            r = is_min ? min(r, y) : max(r, y);
+        }
+        if (lane == 0) red[k][wave] = r;
+    }
+    __syncthreads();
+    if (threadIdx.x < 12) {
+        const int k = threadIdx.x;
+        const bool is_min = (k < 3) || (k >= 6 && k < 9);
+        uint32_t r = red[k][0];
+        for (int w = 1; w < 4; ++w) r = is_min ? min(r, red[k][w]) : max(r, red[k][w]);
+        bounds[k] = r;
     }
 }
 
@@ -388,7 +416,8 @@ __global__ void __launch_bounds__(256) k_leaf_tris(const float *__restrict__ abc
 // A_L * T_L + A_R * T_R, stable partition by a scan), so that splits follow
 // surface area instead of Morton-code midpoints. Nodes of at most
 // kSahSweepItems clusters are split by an exact sweep instead (every cluster
-// centroid is a candidate plane on every axis, one thread per node). The
+// centroid is a candidate plane on every axis, one wave per node and one
+// candidate per lane). The
 // triangles are then laid out in the new cluster order, so
 // every node of the combined tree covers a contiguous range and the collapse
 // can still make leaves of ranges. SAH nodes are appended after the LBVH's
@@ -398,9 +427,9 @@ __global__ void __launch_bounds__(256) k_leaf_tris(const float *__restrict__ abc
 // LBVH alone traces 3190 Mrays/s; clusters of 8 triangles 3470; single
 // triangles, i.e. the whole tree rebuilt by SAH over the Morton-ordered
 // triangles, 3750-3780, for a build 6-7x the LBVH's (30-31 ms against 4.7 ms
-// then; 10.3 ms against 1.8 ms since the level loops stay on the device).
+// then; 8.7 ms against 1.8 ms since the level loops stay on the device).
 constexpr int kSahCluster = 1;
-constexpr int kSahSweepItems = 16;
+constexpr int kSahSweepItems = 64;  // one item per lane of the sweeping wave
 constexpr int kSahBinsMax = 32;
 constexpr int kSahBinWords = 8;            // count, triangles, lo xyz, hi xyz (lo/hi order-mapped)
 constexpr int32_t kSahRef = 1 << 30;       // child ref of SAH node s: kSahRef | s
@@ -506,6 +535,22 @@ __device__ __forceinline__ float3 item_centroid(float4 lo, float4 hi) {
     return make_float3((lo.x + hi.x) * 0.5f, (lo.y + hi.y) * 0.5f, (lo.z + hi.z) * 0.5f);
 }
 
+// bin_add on an LDS copy of the bins (the builtins take an LDS pointer, so
+// the updates are ds atomics that the workgroup barrier waits for).
+__device__ __forceinline__ void bin_add_lds(uint32_t *w, uint32_t count, uint32_t tris, float lx, float ly, float lz,
+                                            float hx, float hy, float hz) {
+    typedef __attribute__((address_space(3))) uint32_t lds_u32;
+    lds_u32 *l = (lds_u32 *)(w);
+    __hip_atomic_fetch_add(&l[0], count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_add(&l[1], tris, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_min(&l[2], f2ord(lx), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_min(&l[3], f2ord(ly), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_min(&l[4], f2ord(lz), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_max(&l[5], f2ord(hx), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_max(&l[6], f2ord(hy), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_max(&l[7], f2ord(hz), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 // Runs of equal key in a wave: after the call the first lane of each run
 // holds the run's minima (v[0..2]), maxima (v[3..5]) and sums (c[0..1]), and
 // the result says whether this lane is such a first lane. Items arrive
@@ -518,7 +563,11 @@ __device__ __forceinline__ float3 item_centroid(float4 lo, float4 hi) {
 // runs would count items twice.
 __device__ __forceinline__ bool run_reduce(int32_t key, float (&v)[6], uint32_t (&c)[2]) {
     const int lane = threadIdx.x & 63;
-    const bool head = lane == 0 || __shfl_up(key, 1, 64) != key;
+    // Every lane takes part in the shuffle: written inside lane == 0 || ...,
+    // it ran with lane 0 masked off, and lane 1 compared its key with a stale
+    // register of the inactive lane 0 (runs then merged across nodes).
+    const int32_t prev = __shfl_up(key, 1, 64);
+    const bool head = (lane == 0) | (prev != key);
     const int run = __popcll(__ballot(head) & (~0ull >> (63 - lane)));  // heads at lanes <= this one
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
@@ -586,9 +635,18 @@ __global__ void __launch_bounds__(256) k_sah_cbounds(const float4 *__restrict__ 
         float v[6] = {c.x, c.y, c.z, c.x, c.y, c.z};
         uint32_t cnt[2] = {0u, 0u};
         if (run_reduce(slot, v, cnt) && slot >= 0) {
-            uint32_t *b = block_uniform ? lcb : cb + 6 * static_cast<int64_t>(slot);
-            for (int k = 0; k < 3; ++k) atomicMin(&b[k], f2ord(v[k]));
-            for (int k = 3; k < 6; ++k) atomicMax(&b[k], f2ord(v[k]));
+            if (block_uniform) {  // separate from the global path: see k_sah_bin
+                typedef __attribute__((address_space(3))) uint32_t lds_u32;
+                lds_u32 *b = (lds_u32 *)(lcb);
+                for (int k = 0; k < 3; ++k)
+                    __hip_atomic_fetch_min(&b[k], f2ord(v[k]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                for (int k = 3; k < 6; ++k)
+                    __hip_atomic_fetch_max(&b[k], f2ord(v[k]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            } else {
+                uint32_t *b = cb + 6 * static_cast<int64_t>(slot);
+                for (int k = 0; k < 3; ++k) atomicMin(&b[k], f2ord(v[k]));
+                for (int k = 3; k < 6; ++k) atomicMax(&b[k], f2ord(v[k]));
+            }
         }
         __syncthreads();
     }
@@ -681,52 +739,60 @@ __global__ void __launch_bounds__(256) k_sah_bin(const float4 *__restrict__ it_l
             uint32_t c[2] = {1u, static_cast<uint32_t>(__float_as_int(hi.w))};
             const bool head = run_reduce(slot >= 0 ? slot * kSahBinsMax + k : -1, v, c);
             if (!head || slot < 0) continue;
-            uint32_t *w = uniform ? lbins + (a * nbins + k) * kSahBinWords
-                                  : bins + ((static_cast<int64_t>(slot) * 3 + a) * nbins + k) * kSahBinWords;
-            bin_add(w, c[0], c[1], v[0], v[1], v[2], v[3], v[4], v[5]);
+            // LDS and global updates stay separate calls: through one generic
+            // pointer they become flat atomics, and flat atomics that land in
+            // LDS were not covered by the barrier's wait (items moved between
+            // neighbouring nodes' bins across a flush).
+            if (uniform)
+                bin_add_lds(lbins + (a * nbins + k) * kSahBinWords, c[0], c[1], v[0], v[1], v[2], v[3], v[4], v[5]);
+            else
+                bin_add(bins + ((static_cast<int64_t>(slot) * 3 + a) * nbins + k) * kSahBinWords, c[0], c[1], v[0],
+                        v[1], v[2], v[3], v[4], v[5]);
         }
         __syncthreads();
     }
     if (cur >= 0) flush(cur);
 }
 
-struct Box3 {
-    float lx, ly, lz, hx, hy, hz;
-    __device__ void reset() {
-        lx = ly = lz = INFINITY;
-        hx = hy = hz = -INFINITY;
-    }
-    __device__ void add(const uint32_t *w) {
-        lx = fminf(lx, ord2f(w[2]));
-        ly = fminf(ly, ord2f(w[3]));
-        lz = fminf(lz, ord2f(w[4]));
-        hx = fmaxf(hx, ord2f(w[5]));
-        hy = fmaxf(hy, ord2f(w[6]));
-        hz = fmaxf(hz, ord2f(w[7]));
-    }
-    __device__ float half_area() const {
-        const float dx = hx - lx, dy = hy - ly, dz = hz - lz;
-        return dx * dy + dx * dz + dy * dz;
-    }
-};
-
-// Exact sweep over a small node's items: every item centroid is a candidate
-// plane on every axis (items with centroid < plane go left). The items are
-// staged in LDS first, item k of lane t at [k][t] (conflict-free columns).
-__device__ void sah_sweep(const float4 (*s_lo)[64], const float4 (*s_hi)[64], int lane, int32_t size, SahSplit *sp) {
-    float best = INFINITY;
-    for (int a = 0; a < 3; ++a) {
-        for (int32_t j = 0; j < size; ++j) {
-            const float3 cj = item_centroid(s_lo[j][lane], s_hi[j][lane]);
-            const float v = a == 0 ? cj.x : (a == 1 ? cj.y : cj.z);
+// One wave per node. A node of at most kSahSweepItems items is split by an
+// exact sweep: lane j holds item j, its centroid is the candidate plane of the
+// lane on each axis (items with centroid < plane go left), and every lane
+// walks the node's items from LDS (broadcast reads) to price its candidates.
+// The winner is the least cost, ties to the lowest (axis, item) pair, as a
+// sequential scan over axes then items with a strict < would pick. Larger
+// nodes take the best binned plane (lane 0 scans the bins, then clears them
+// for the next level). With neither, the middle of the items. Sides with more
+// than one item get an SAH node index and a place on the next level;
+// single-item sides are linked by k_sah_scatter.
+__device__ SahSplit decide_split(const SahNode &nd, const float4 *__restrict__ it_lo, const float4 *__restrict__ it_hi,
+                                 uint32_t *__restrict__ bins, int nbins, float4 (*s_lo)[kSahSweepItems],
+                                 float4 (*s_hi)[kSahSweepItems], int wave, int lane) {
+    const int32_t size = nd.end - nd.begin;
+    SahSplit sp{-1, 0, size / 2, 0.0f, {-1, -1}};
+    if (nd.slot < 0) {
+        float4 mylo = make_float4(0.0f, 0.0f, 0.0f, 0.0f), myhi = mylo;
+        if (lane < size) {
+            mylo = it_lo[nd.begin + lane];
+            myhi = it_hi[nd.begin + lane];
+            s_lo[wave][lane] = mylo;
+            s_hi[wave][lane] = myhi;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        const float3 mc = item_centroid(mylo, myhi);
+        float best = INFINITY;
+        int best_key = 0x7fffffff;  // axis * 64 + item
+        int32_t best_nl = 0;
+        for (int a = 0; a < 3; ++a) {
+            const float v = a == 0 ? mc.x : (a == 1 ? mc.y : mc.z);
             float llx = INFINITY, lly = INFINITY, llz = INFINITY, lhx = -INFINITY, lhy = -INFINITY, lhz = -INFINITY;
             float rlx = INFINITY, rly = INFINITY, rlz = INFINITY, rhx = -INFINITY, rhy = -INFINITY, rhz = -INFINITY;
             uint32_t lt = 0, rt_ = 0;
             int32_t nl = 0;
-            for (int32_t k = 0; k < size; ++k) {
-                const float4 lo = s_lo[k][lane], hi = s_hi[k][lane];
-                const float3 ck = item_centroid(lo, hi);
-                const float ca = a == 0 ? ck.x : (a == 1 ? ck.y : ck.z);
+            for (int32_t q = 0; q < size; ++q) {
+                const float4 lo = s_lo[wave][q], hi = s_hi[wave][q];
+                const float3 cq = item_centroid(lo, hi);
+                const float ca = a == 0 ? cq.x : (a == 1 ? cq.y : cq.z);
                 const uint32_t t = static_cast<uint32_t>(__float_as_int(hi.w));
                 if (ca < v) {
                     llx = fminf(llx, lo.x); lly = fminf(lly, lo.y); llz = fminf(llz, lo.z);
@@ -739,106 +805,186 @@ __device__ void sah_sweep(const float4 (*s_lo)[64], const float4 (*s_hi)[64], in
                     rt_ += t;
                 }
             }
-            if (nl == 0 || nl == size) continue;
+            if (lane >= size || nl == 0 || nl == size) continue;
             const float ldx = lhx - llx, ldy = lhy - lly, ldz = lhz - llz;
             const float rdx = rhx - rlx, rdy = rhy - rly, rdz = rhz - rlz;
             const float cost = (ldx * ldy + ldx * ldz + ldy * ldz) * static_cast<float>(lt) +
                                (rdx * rdy + rdx * rdz + rdy * rdz) * static_cast<float>(rt_);
-            if (cost < best) {
+            if (cost < best) {  // per lane the axes come in order, so the first win stays
                 best = cost;
-                sp->axis = a;
-                sp->bin = -1;
-                sp->nl = nl;
-                sp->plane = v;
+                best_key = a * 64 + lane;
+                best_nl = nl;
             }
         }
-    }
-}
-
-// One thread per node: the best binned plane (large nodes; their bins are
-// cleared for the next level once read) or swept plane (small nodes), else
-// the middle. Children with more than one item get an SAH node index and a
-// place on the next level; single-item sides are linked by k_sah_scatter.
-__global__ void __launch_bounds__(64) k_sah_split(const SahNode *__restrict__ nodes, const uint32_t *__restrict__ count,
-                                                  const float4 *__restrict__ it_lo, const float4 *__restrict__ it_hi,
-                                                  uint32_t *__restrict__ bins, int nbins,
-                                                  SahSplit *__restrict__ splits, SahNode *__restrict__ next,
-                                                  uint32_t *__restrict__ ctr, int2 *__restrict__ sah_child,
-                                                  int2 *__restrict__ sah_items) {
-    __shared__ float4 s_lo[kSahSweepItems][64], s_hi[kSahSweepItems][64];
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (ctr[3] || k >= static_cast<int>(*count)) return;
-    const SahNode nd = nodes[k];
-    const int32_t size = nd.end - nd.begin;
-    SahSplit sp{-1, 0, size / 2, 0.0f, {-1, -1}};
-    if (nd.slot < 0) {
-        for (int32_t j = 0; j < size; ++j) {
-            s_lo[j][threadIdx.x] = it_lo[nd.begin + j];
-            s_hi[j][threadIdx.x] = it_hi[nd.begin + j];
+        // Least (cost, key) over the wave.
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const float oc = __shfl_xor(best, o, 64);
+            const int ok = __shfl_xor(best_key, o, 64);
+            const int on = __shfl_xor(best_nl, o, 64);
+            if (oc < best || (oc == best && ok < best_key)) {
+                best = oc;
+                best_key = ok;
+                best_nl = on;
+            }
         }
-        sah_sweep(s_lo, s_hi, threadIdx.x, size, &sp);
+        if (best_key != 0x7fffffff) {
+            const int a = best_key >> 6, j = best_key & 63;
+            const float4 lo = s_lo[wave][j], hi = s_hi[wave][j];
+            const float3 cj = item_centroid(lo, hi);
+            sp.axis = a;
+            sp.bin = -1;
+            sp.nl = best_nl;
+            sp.plane = a == 0 ? cj.x : (a == 1 ? cj.y : cj.z);
+        }
     } else {
+        // Lane b holds bin b; inclusive prefix and suffix scans over the lanes
+        // give every plane's left and right count, triangles and box.
         float best = INFINITY;
+        int best_key = 0x7fffffff;  // axis * 64 + bin
+        int32_t best_nl = 0;
         for (int a = 0; a < 3; ++a) {
-            uint32_t *base = bins + (static_cast<int64_t>(nd.slot) * 3 + a) * nbins * kSahBinWords;
-            // Right-to-left suffix areas and counts, then the left sweep.
-            float r_area[kSahBinsMax];
-            uint32_t r_tris[kSahBinsMax];
-            Box3 bx;
-            bx.reset();
-            uint32_t t = 0;
-            for (int b = nbins - 1; b > 0; --b) {
-                const uint32_t *w = base + b * kSahBinWords;
-                if (w[0]) bx.add(w);
-                t += w[1];
-                r_area[b] = bx.half_area();
-                r_tris[b] = t;
-            }
-            bx.reset();
-            t = 0;
-            uint32_t cnt = 0;
-            for (int b = 0; b < nbins - 1; ++b) {
-                const uint32_t *w = base + b * kSahBinWords;
-                if (w[0]) bx.add(w);
-                t += w[1];
-                cnt += w[0];
-                if (cnt == 0 || cnt == static_cast<uint32_t>(size)) continue;
-                const float cost = bx.half_area() * static_cast<float>(t) +
-                                   r_area[b + 1] * static_cast<float>(r_tris[b + 1]);
-                if (cost < best) {
-                    best = cost;
-                    sp.axis = a;
-                    sp.bin = b;
-                    sp.nl = static_cast<int32_t>(cnt);
-                }
-            }
-            for (int b = 0; b < nbins; ++b) {
-                uint32_t *w = base + b * kSahBinWords;
-                w[0] = w[1] = 0u;
+            uint32_t c = 0, t = 0;
+            float b6[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+            if (lane < nbins) {
+                uint32_t *w = bins + ((static_cast<int64_t>(nd.slot) * 3 + a) * nbins + lane) * kSahBinWords;
+                c = w[0];
+                t = w[1];
+                if (c)
+                    for (int q = 0; q < 6; ++q) b6[q] = ord2f(w[2 + q]);
+                w[0] = w[1] = 0u;  // cleared for the next level
                 w[2] = w[3] = w[4] = 0xffffffffu;
                 w[5] = w[6] = w[7] = 0u;
             }
+            uint32_t lc = c, lt = t, rc = c, rt_ = t;
+            float l6[6], r6[6];
+            for (int q = 0; q < 6; ++q) l6[q] = r6[q] = b6[q];
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t yc = __shfl_up(lc, o, 64), yt = __shfl_up(lt, o, 64);
+                const uint32_t zc = __shfl_down(rc, o, 64), zt = __shfl_down(rt_, o, 64);
+                const bool up = lane >= o, down = lane + o < 64;
+                if (up) {
+                    lc += yc;
+                    lt += yt;
+                }
+                if (down) {
+                    rc += zc;
+                    rt_ += zt;
+                }
+#pragma unroll
+                for (int q = 0; q < 6; ++q) {
+                    const float yl = __shfl_up(l6[q], o, 64), zr = __shfl_down(r6[q], o, 64);
+                    if (up) l6[q] = q < 3 ? fminf(l6[q], yl) : fmaxf(l6[q], yl);
+                    if (down) r6[q] = q < 3 ? fminf(r6[q], zr) : fmaxf(r6[q], zr);
+                }
+            }
+            // The right side of plane b is the suffix from bin b + 1.
+            const uint32_t nrt = __shfl_down(rt_, 1, 64);
+            float n6[6];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) n6[q] = __shfl_down(r6[q], 1, 64);
+            if (lane >= nbins - 1 || lc == 0 || lc == static_cast<uint32_t>(size)) continue;
+            const float ldx = l6[3] - l6[0], ldy = l6[4] - l6[1], ldz = l6[5] - l6[2];
+            const float rdx = n6[3] - n6[0], rdy = n6[4] - n6[1], rdz = n6[5] - n6[2];
+            const float cost = (ldx * ldy + ldx * ldz + ldy * ldz) * static_cast<float>(lt) +
+                               (rdx * rdy + rdx * rdz + rdy * rdz) * static_cast<float>(nrt);
+            if (cost < best) {
+                best = cost;
+                best_key = a * 64 + lane;
+                best_nl = static_cast<int32_t>(lc);
+            }
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const float oc = __shfl_xor(best, o, 64);
+            const int ok = __shfl_xor(best_key, o, 64);
+            const int on = __shfl_xor(best_nl, o, 64);
+            if (oc < best || (oc == best && ok < best_key)) {
+                best = oc;
+                best_key = ok;
+                best_nl = on;
+            }
+        }
+        if (best_key != 0x7fffffff) {
+            sp.axis = best_key >> 6;
+            sp.bin = best_key & 63;
+            sp.nl = best_nl;
         }
     }
-    sah_items[nd.id] = make_int2(nd.begin, nd.end);
-    int2 ch = make_int2(0, 0);
-    const int32_t sizes[2] = {sp.nl, size - sp.nl};
-    const int32_t begins[2] = {nd.begin, nd.begin + sp.nl};
-    for (int side = 0; side < 2; ++side) {
-        if (sizes[side] <= 1) continue;  // linked by the scatter
-        const int32_t id = static_cast<int32_t>(atomicAdd(&ctr[2], 1u));
-        const uint32_t q = atomicAdd(&ctr[1], 1u);
-        next[q] = SahNode{begins[side], begins[side] + sizes[side], id, -1};
-        sp.next[side] = static_cast<int32_t>(q);
-        if (side == 0)
-            ch.x = kSahRef | id;
-        else
-            ch.y = kSahRef | id;
+    return sp;
+}
+
+// The level's node count is only known on the device, so the grid is fixed
+// and every wave walks the nodes with a stride (a grid sized for the upper
+// bound of 2^level nodes launched up to a million mostly idle waves). Each
+// node's children of more than one item are counted into kids[k] (zeros up to
+// the bound, which the scan covers); their SAH node indices and next-level
+// places come from the scan of kids in k_sah_link. A pair of atomics per node
+// on two shared counters serialised across the XCDs and cost 7 ms per level
+// on the deep levels.
+constexpr int kSplitBlocks = 2048;
+
+__global__ void __launch_bounds__(256) k_sah_split(const SahNode *__restrict__ nodes, const uint32_t *__restrict__ count,
+                                                   int64_t bound, const float4 *__restrict__ it_lo,
+                                                   const float4 *__restrict__ it_hi, uint32_t *__restrict__ bins,
+                                                   int nbins, SahSplit *__restrict__ splits,
+                                                   uint32_t *__restrict__ kids, const uint32_t *__restrict__ ctr) {
+    __shared__ float4 s_lo[4][kSahSweepItems], s_hi[4][kSahSweepItems];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (ctr[3]) return;
+    const int n = static_cast<int>(*count);
+    for (int k = blockIdx.x * 4 + wave; k < n; k += gridDim.x * 4) {  // the same in every lane of the wave
+        const SahNode nd = nodes[k];
+        const SahSplit sp = decide_split(nd, it_lo, it_hi, bins, nbins, s_lo, s_hi, wave, lane);
+        if (lane == 0) {
+            const int32_t size = nd.end - nd.begin;
+            splits[k] = sp;
+            kids[k] = static_cast<uint32_t>((sp.nl > 1) + (size - sp.nl > 1));
+        }
     }
-    splits[k] = sp;
-    // Sides of one item are written by the scatter; keep the other half.
-    if (sizes[0] > 1) sah_child[nd.id].x = ch.x;
-    if (sizes[1] > 1) sah_child[nd.id].y = ch.y;
+    for (int64_t k = n + static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; k < bound;
+         k += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        kids[k] = 0u;
+}
+
+// Links each node to its children (kids holds the exclusive scan): side s of
+// node k with more than one item becomes SAH node ctr[2] + kids[k] + s' and
+// next-level node kids[k] + s' (s' counts such sides before it). The last
+// node leaves the next level's size in ctr[1].
+__global__ void __launch_bounds__(256) k_sah_link(const SahNode *__restrict__ nodes, const uint32_t *__restrict__ count,
+                                                  SahSplit *__restrict__ splits, const uint32_t *__restrict__ kids,
+                                                  SahNode *__restrict__ next, uint32_t *__restrict__ ctr,
+                                                  int2 *__restrict__ sah_child, int2 *__restrict__ sah_items) {
+    if (ctr[3]) return;
+    const int n = static_cast<int>(*count);
+    const uint32_t id_base = ctr[2];
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const SahNode nd = nodes[k];
+        SahSplit sp = splits[k];
+        const int32_t size = nd.end - nd.begin;
+        const int32_t sizes[2] = {sp.nl, size - sp.nl};
+        const int32_t begins[2] = {nd.begin, nd.begin + sp.nl};
+        uint32_t q = kids[k];
+        sah_items[nd.id] = make_int2(nd.begin, nd.end);
+        int2 ch = make_int2(0, 0);
+        for (int side = 0; side < 2; ++side) {
+            if (sizes[side] <= 1) continue;  // linked by the scatter
+            const int32_t id = static_cast<int32_t>(id_base + q);
+            next[q] = SahNode{begins[side], begins[side] + sizes[side], id, -1};
+            sp.next[side] = static_cast<int32_t>(q);
+            if (side == 0)
+                ch.x = kSahRef | id;
+            else
+                ch.y = kSahRef | id;
+            ++q;
+        }
+        splits[k] = sp;
+        // Sides of one item are written by the scatter; keep the other half.
+        if (sizes[0] > 1) sah_child[nd.id].x = ch.x;
+        if (sizes[1] > 1) sah_child[nd.id].y = ch.y;
+        if (k == n - 1) ctr[1] = q;
+    }
 }
 
 __device__ __forceinline__ bool sah_goes_left(const SahNode &nd, const SahSplit &sp, const uint32_t *cb, int nbins,
@@ -922,6 +1068,7 @@ __global__ void __launch_bounds__(256) k_sah_reset(uint32_t *__restrict__ bins, 
 __global__ void k_sah_close(uint32_t *__restrict__ ctr, uint32_t *__restrict__ count, int l) {
     if (threadIdx.x == 0) {
         count[l + 1] = ctr[1];
+        ctr[2] += ctr[1];  // SAH node indices handed out on this level
         ctr[0] = 0;
         ctr[1] = 0;
     }
@@ -1114,15 +1261,13 @@ int32_t build_accel(rt_scene *scene, const rt_build_params &p, rt_accel **out) {
     BUILD_CHECK(dalloc_in<uint32_t>(ctx->arena, b_idx_t, n));
     BUILD_CHECK(dalloc_in<uint32_t>(ctx->arena, b_hist, 256 * tiles));
     BUILD_CHECK(dalloc_in<uint32_t>(ctx->arena, b_sums, kScanTile));
-    uint32_t init_bounds[12];
-    for (int k = 0; k < 12; ++k) {
-        const bool is_min = (k < 3) || (k >= 6 && k < 9);
-        init_bounds[k] = is_min ? 0xffffffffu : 0u;
-    }
-    BUILD_CHECK(hipMemcpyAsync(b_bounds.p, init_bounds, sizeof(init_bounds), hipMemcpyHostToDevice, s));
+    DevBuf b_parts;
+    BUILD_CHECK(dalloc_in<uint32_t>(ctx->arena, b_parts, 12 * static_cast<int64_t>((n + 255) / 256)));
     const int blocks = static_cast<int>((n + 255) / 256);
     hipLaunchKernelGGL(k_tri_bounds, dim3(blocks), dim3(256), 0, s, scene->d_abc, n, (float4 *)b_aabb_lo.p,
-                       (float4 *)b_aabb_hi.p, (uint32_t *)b_bounds.p);
+                       (float4 *)b_aabb_hi.p, (uint32_t *)b_parts.p);
+    hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(256), 0, s, (const uint32_t *)b_parts.p, blocks,
+                       (uint32_t *)b_bounds.p);
     hipLaunchKernelGGL(k_morton, dim3(blocks), dim3(256), 0, s, (const float4 *)b_aabb_lo.p,
                        (const float4 *)b_aabb_hi.p, n, (const uint32_t *)b_bounds.p, (uint32_t *)b_codes.p,
                        (uint32_t *)b_idx.p);
@@ -1256,6 +1401,8 @@ int32_t build_accel(rt_scene *scene, const rt_build_params &p, rt_accel **out) {
             }
             BUILD_CHECK(dalloc_in<uint32_t>(ctx->arena, b_flags, m));
             BUILD_CHECK(dalloc_in<SahSplit>(ctx->arena, b_splits, m));
+            DevBuf b_kids;  // children per node of a level, then their exclusive scan
+            BUILD_CHECK(dalloc_in<uint32_t>(ctx->arena, b_kids, m));
             BUILD_CHECK(dalloc_in<int2>(ctx->arena, b_sc, m - 1));
             BUILD_CHECK(dalloc_in<int2>(ctx->arena, b_si, m - 1));
             BUILD_CHECK(dalloc_in<uint32_t>(ctx->arena, b_ctr, 4));  // [0] bin slots, [1] next-level nodes, [2] SAH nodes
@@ -1302,14 +1449,19 @@ int32_t build_accel(rt_scene *scene, const rt_build_params &p, rt_accel **out) {
                     const uint32_t *count = scount + launched;
                     const int64_t bound = std::min<int64_t>(launched < 30 ? (int64_t(1) << launched) : m, m);
                     const unsigned nb256 = static_cast<unsigned>((bound + 255) / 256);
-                    const unsigned nb64 = static_cast<unsigned>((bound + 63) / 64);
+                    const unsigned nb4 = static_cast<unsigned>((bound + 3) / 4);
                     hipLaunchKernelGGL(k_sah_classify, dim3(nb256), dim3(256), 0, s, nodes, count, ctr, cb);
                     hipLaunchKernelGGL(k_sah_cbounds, dim3(cblocks), dim3(256), 0, s, ilo, ihi, ind, m, nodes, cb,
                                        (const uint32_t *)ctr);
                     hipLaunchKernelGGL(k_sah_bin, dim3(cblocks), dim3(256), 0, s, ilo, ihi, ind, m, nodes, cb, nbins,
                                        bins, (const uint32_t *)ctr);
-                    hipLaunchKernelGGL(k_sah_split, dim3(nb64), dim3(64), 0, s, nodes, count, ilo, ihi, bins, nbins,
-                                       (SahSplit *)b_splits.p, next, ctr, (int2 *)b_sc.p, (int2 *)b_si.p);
+                    hipLaunchKernelGGL(k_sah_split, dim3(std::min<unsigned>(nb4, kSplitBlocks)), dim3(256), 0, s,
+                                       nodes, count, bound, ilo, ihi, bins, nbins, (SahSplit *)b_splits.p,
+                                       (uint32_t *)b_kids.p, (const uint32_t *)ctr);
+                    BUILD_CHECK(exclusive_scan_u32((uint32_t *)b_kids.p, bound, (uint32_t *)b_sums.p, s));
+                    hipLaunchKernelGGL(k_sah_link, dim3(std::min<unsigned>(nb256, kSplitBlocks)), dim3(256), 0, s,
+                                       (const SahNode *)nodes, count, (SahSplit *)b_splits.p,
+                                       (const uint32_t *)b_kids.p, next, ctr, (int2 *)b_sc.p, (int2 *)b_si.p);
                     hipLaunchKernelGGL(k_sah_flags, dim3(mblocks), dim3(256), 0, s, ilo, ihi, ind, m, nodes,
                                        (const SahSplit *)b_splits.p, cb, nbins, (uint32_t *)b_flags.p,
                                        (const uint32_t *)ctr);

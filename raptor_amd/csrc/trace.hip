@@ -34,24 +34,29 @@ __global__ void __launch_bounds__(64) k_trace(TraceArgs a) {
     const int lane = threadIdx.x;
     const int64_t i = static_cast<int64_t>(blockIdx.x) * 64 + lane;
     const int64_t n = a.count ? static_cast<int64_t>(a.count[0]) * a.count_mul : a.n;
-    if (i >= n) return;
-    const int64_t ri = a.order ? static_cast<int64_t>(a.order[i]) : i;
-    const float4 r0 = reinterpret_cast<const float4 *>(a.rays)[2 * ri];
-    const float4 r1 = reinterpret_cast<const float4 *>(a.rays)[2 * ri + 1];
-    float4 hit;
+    // Lanes past the end leave at once, except when counting: the count
+    // reduction below shuffles across the whole wave, and a lane that has
+    // returned would lend it a stale register.
+    if (!COUNT && i >= n) return;
     uint32_t visits = 0, tests = 0;
     bool overflow = false;
-    trace_ray<W, CLOSEST, COUNT, kLdsDepth>(reinterpret_cast<const WideNode<W> *>(a.nodes), a.tris, a.root,
-                                            a.near_limit, rtd::f4v(r0), rtd::f4v(r1), r0.w, r1.w, lane,
-                                            hit, visits, tests, overflow);
-    const int32_t best_id = __float_as_int(hit.w);
-    if (CLOSEST) {
-        const float ht = (best_id >= 0) ? hit.x : rtd::kMaxDist;
-        const float hu = (best_id >= 0) ? hit.y : 0.0f;
-        const float hv = (best_id >= 0) ? hit.z : 0.0f;
-        reinterpret_cast<float4 *>(a.hits)[ri] = make_float4(ht, hu, hv, __int_as_float(best_id));
-    } else {
-        a.occ[ri] = (best_id >= 0) ? 1 : 0;
+    if (i < n) {
+        const int64_t ri = a.order ? static_cast<int64_t>(a.order[i]) : i;
+        const float4 r0 = reinterpret_cast<const float4 *>(a.rays)[2 * ri];
+        const float4 r1 = reinterpret_cast<const float4 *>(a.rays)[2 * ri + 1];
+        float4 hit;
+        trace_ray<W, CLOSEST, COUNT, kLdsDepth>(reinterpret_cast<const WideNode<W> *>(a.nodes), a.tris, a.root,
+                                                a.near_limit, rtd::f4v(r0), rtd::f4v(r1), r0.w, r1.w, lane,
+                                                hit, visits, tests, overflow);
+        const int32_t best_id = __float_as_int(hit.w);
+        if (CLOSEST) {
+            const float ht = (best_id >= 0) ? hit.x : rtd::kMaxDist;
+            const float hu = (best_id >= 0) ? hit.y : 0.0f;
+            const float hv = (best_id >= 0) ? hit.z : 0.0f;
+            reinterpret_cast<float4 *>(a.hits)[ri] = make_float4(ht, hu, hv, __int_as_float(best_id));
+        } else {
+            a.occ[ri] = (best_id >= 0) ? 1 : 0;
+        }
     }
     if (COUNT) {
         unsigned long long vs = visits, ts = tests;
