@@ -27,14 +27,32 @@ class oracle_light(C.Structure):
 
 class oracle_render_params(C.Structure):
     _fields_ = [("bounce", C.c_int32), ("spp", C.c_int32), ("seed", C.c_uint64), ("pixel_stride", C.c_int32),
-                ("nthreads", C.c_int32), ("packet", C.c_int32)]
+                ("nthreads", C.c_int32), ("packet", C.c_int32), ("materials", C.c_void_p),
+                ("nmaterials", C.c_int32), ("tri_material", C.c_void_p)]
 
 
 class oracle_render_out(C.Structure):
     _fields_ = [("rgb", C.c_void_p), ("prim", C.c_void_p), ("tuv", C.c_void_p), ("vis", C.c_void_p),
                 ("bounce_prim", C.c_void_p), ("bounce_tuv", C.c_void_p), ("bounce_vis", C.c_void_p),
                 ("counters", C.c_void_p), ("rec_levels", C.c_int32), ("rec_prim", C.c_void_p),
-                ("rec_o", C.c_void_p), ("rec_n", C.c_void_p), ("rec_vis", C.c_void_p)]
+                ("rec_o", C.c_void_p), ("rec_n", C.c_void_p), ("rec_vis", C.c_void_p),
+                ("sec_counters", C.c_void_p)]
+
+
+# Phong material fields, in the order the oracle (and include/raptor_rt.h's
+# rt_material) stores them: rt/materials/phong_shader.h:38-60.
+MATERIAL_FIELDS = ("ka", "kd", "ks", "s", "tran", "ri", "rf")
+
+
+def material_array(materials):
+    """materials: list of dicts with ka, kd, ks (rgb in [0, 255]) and s, tran,
+    ri, rf; returns an (n, 13) float32 array."""
+    rows = []
+    for m in materials:
+        rows.append(list(m.get("ka", (0.0, 0.0, 0.0))) + list(m.get("kd", (0.0, 0.0, 0.0))) +
+                    list(m.get("ks", (0.0, 0.0, 0.0))) +
+                    [m.get("s", 0.0), m.get("tran", 0.0), m.get("ri", 1.0), m.get("rf", 0.0)])
+    return np.asarray(rows, np.float32).reshape(-1, 13)
 
 
 def build():
@@ -58,6 +76,8 @@ def _load():
         "oracle_trace_occluded": (None, [vp, vp, i64, vp, i32, i32, vp]),
         "oracle_render": (i32, [vp, vp, C.POINTER(oracle_camera), C.POINTER(oracle_light), i32,
                                 C.POINTER(oracle_render_params), C.POINTER(oracle_render_out)]),
+        "oracle_reflect_dir": (None, [vp, vp, vp]),
+        "oracle_refract_dir": (i32, [vp, vp, f, i32, vp]),
         "oracle_min_element_n": (i32, [vp, vp, i32]),
         "oracle_min_element_range": (i32, [vp, vp, i32, i32]),
         "oracle_morton": (i32, [f, f, f, f, f, f]),
@@ -171,12 +191,16 @@ class OracleScene:
         return occ, cnt
 
     def render(self, albedo, camera, lights, bounce=False, spp=1, seed=0, pixel_stride=1, nthreads=8, packet=False,
-               records=False):
+               records=False, materials=None, tri_material=None):
         """One frame; bounce is the number of diffuse bounces (True = 1).
         records=True adds per-sample path records over the bounce + 1 path
         vertices, level-major (vertex k, sample p * spp + s): prim (L, S)
         (-1: none), o (L, S, 3) shadow-ray origins, n (L, S, 3) shading
-        normals, vis (L, S, nlights) 1 = lit."""
+        normals, vis (L, S, nlights) 1 = lit. materials (list of dicts, see
+        material_array) with tri_material (one index per triangle, -1: the
+        Lambert albedo default) renders phong materials with mirror and
+        refraction rays; counters then gain sec_counters (secondary rays,
+        their shadow rays)."""
         npix = camera.x_res * camera.y_res
         nl = len(lights)
         rgb = np.full((npix, 3), np.nan, np.float32)
@@ -199,13 +223,20 @@ class OracleScene:
             out.rec_prim, out.rec_o, out.rec_n, out.rec_vis = (_p(rec["prim"]), _p(rec["o"]), _p(rec["n"]),
                                                                _p(rec["vis"]))
         p = oracle_render_params(nb, spp, seed, pixel_stride, nthreads, 1 if packet else 0)
+        sec = np.zeros(2, np.uint64)
+        out.sec_counters = _p(sec)
+        if materials is not None:
+            mats = material_array(materials) if not isinstance(materials, np.ndarray) else \
+                np.ascontiguousarray(materials, np.float32).reshape(-1, 13)
+            tm = np.ascontiguousarray(tri_material, np.int32)
+            p.materials, p.nmaterials, p.tri_material = _p(mats), mats.shape[0], _p(tm)
         alb = np.ascontiguousarray(albedo, np.float32)
         rc = LIB.oracle_render(self.h, _p(alb), C.byref(camera_c(camera)), lights_c(lights), nl, C.byref(p),
                                C.byref(out))
         if rc != 0:
             raise RuntimeError(f"oracle_render failed ({rc})")
         res = dict(rgb=rgb, prim=prim, tuv=tuv, vis=vis, bounce_prim=bprim, bounce_tuv=btuv, bounce_vis=bvis,
-                   counters=cnt)
+                   counters=cnt, sec_counters=sec)
         if rec is not None:
             res["records"] = rec
         return res
@@ -334,3 +365,20 @@ def primary_rays(camera):
             rays[i, 4:7] = d
             rays[i, 7] = np.finfo(np.float32).max
     return rays
+
+
+def reflect_dir(d, n):
+    """rt/ray.cc:121-131 direction of the mirror ray (float32)."""
+    d, n = f32a(*d), f32a(*n)
+    out = np.zeros(3, np.float32)
+    LIB.oracle_reflect_dir(_p(d), _p(n), _p(out))
+    return out
+
+
+def refract_dir(d, n, ri, leaving):
+    """rt/ray.cc:200-227 direction of the refracted ray, or None on total
+    internal reflection (n faces the ray)."""
+    d, n = f32a(*d), f32a(*n)
+    out = np.zeros(3, np.float32)
+    ok = LIB.oracle_refract_dir(_p(d), _p(n), float(ri), 1 if leaving else 0, _p(out))
+    return out if ok else None

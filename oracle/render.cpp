@@ -9,6 +9,7 @@
 // threads with a per-thread traversal stack (the reference stack at
 // ssd/bvh.h:84 is a member and is not re-entrant).
 #include <atomic>
+#include <cmath>
 #include <thread>
 #include <vector>
 
@@ -276,6 +277,184 @@ void render_pixel(const FrameCtx &f, int px, int py, LocalCounters *lc) {
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// Phong materials with mirror and refraction rays: rt/raytracer.cc:20-87 (one
+// hit: normal, shadow rays, shade, secondary rays), phong_shader::shade
+// (rt/materials/phong_shader.cc:21-111) and ray::reflect / ray::refract
+// without DIFFUSE_REFLECTIONS (rt/ray.cc:118-258): one reflected and one
+// refracted ray while the ray's power stays above MIN_REFLECTIVE_POWER.
+
+// common/common.h:30-32. A double literal: the float power is compared in
+// double, so a power of exactly 0.1f (above 0.1) still recurses.
+constexpr double kMinReflectivePower = 0.1;
+
+struct Phong {
+    Colour ka, kd, ks;
+    float s = 0.0f, tran = 0.0f, ri = 1.0f, rf = 0.0f;
+};
+
+Phong material_of(const FrameCtx &f, int prim) {
+    Phong m;
+    const int32_t mi = f.p.tri_material ? f.p.tri_material[prim] : -1;
+    if (f.p.materials && mi >= 0 && mi < f.p.nmaterials) {
+        const float *q = f.p.materials + 13 * static_cast<int64_t>(mi);
+        m.ka = Colour{q[0], q[1], q[2]};
+        m.kd = Colour{q[3], q[4], q[5]};
+        m.ks = Colour{q[6], q[7], q[8]};
+        m.s = q[9];
+        m.tran = q[10];
+        m.ri = q[11];
+        m.rf = q[12];
+    } else {
+        m.kd = kd_of(f, prim);  // phong_shader(255 * albedo, 1): ks = 0 * rgb, ka = 0
+    }
+    return m;
+}
+
+// ray::reflect (rt/ray.cc:121-131): dir - n * 2 (dir . n).
+V3 reflect_dir(const V3 &dir, const V3 &n) {
+    const float rdn = 2.0f * dot(dir, n);
+    return dir - n * rdn;
+}
+
+// ray::refract (rt/ray.cc:200-227): Snell through n (facing the ray), the
+// index inverted for a ray entering the front face (hit_t::out_in); false on
+// total internal reflection.
+bool refract_dir(const V3 &dir, const V3 &n, float ri, bool leaving, V3 *out) {
+    const float rdn = -dot(dir, n);
+    if (!leaving) ri = 1.0f / ri;
+    const float cos_t2 = 1.0f - ri * ri * (1.0f - rdn * rdn);
+    if (cos_t2 < 0.0f) return false;
+    const float offset = (ri * rdn - std::sqrt(cos_t2));
+    *out = (dir * ri) + (n * offset);
+    return true;
+}
+
+struct PhongCounters {
+    uint64_t secondary = 0, secondary_shadow = 0;
+};
+
+Colour trace_phong(const FrameCtx &f, Ray &r, bool primary, LocalCounters *lc, PhongCounters *pc, int32_t *prim_out,
+                   float *tuv_out, uint8_t *vis_out) {
+    Hit h;
+    const int prim = find_nearest_object(*f.s, r, &h, &lc->trav);
+    if (primary) {
+        ++lc->rays[0];
+        if (prim_out) *prim_out = prim;
+        if (tuv_out) {
+            tuv_out[0] = h.d;
+            tuv_out[1] = h.u;
+            tuv_out[2] = h.v;
+        }
+    } else {
+        ++pc->secondary;
+    }
+    Colour c;  // camera background (0, 0, 0)
+    if (!(h.d < kMaxDist)) return c;
+    const Triangle &tri = f.s->tris[prim];
+    const bool leaving = dot(tri.n, r.dir) > 0.0f;  // hit_t::in_out (rt/triangle.h:487-496)
+    const SurfaceHit sh = surface_at(tri, &r, h.d);
+    const Phong m = material_of(f, prim);
+    const V3 &n = sh.shade_norm;
+    for (size_t l = 0; l < f.lights.size(); ++l) {
+        Ray illum = shadow_ray(sh, f.lights[l].c);
+        const bool occluded = found_nearer_object(*f.s, illum, illum.length, &lc->trav);
+        if (primary) {
+            ++lc->rays[1];
+        } else {
+            ++pc->secondary_shadow;
+        }
+        illum.magn = (occluded ? 0.0f : 1.0f) / 1.0f;
+        if (vis_out) vis_out[l] = occluded ? 0 : 1;
+        const float shade = dot(illum.dir, n);
+        if (shade < 0.0f) continue;
+        // Reflection of the light ray about the normal.
+        const float shade_x2 = 2.0f * shade;
+        const V3 ref_dir(illum.dir.x - n.x * shade_x2, illum.dir.y - n.y * shade_x2, illum.dir.z - n.z * shade_x2);
+        const float ray_dot_reflection = dot(r.dir, ref_dir);
+        Colour sc{shade * m.kd.r, shade * m.kd.g, shade * m.kd.b};
+        if (ray_dot_reflection > 0.0f) {
+            const float sp = std::pow(ray_dot_reflection, m.s);
+            sc.r += sp * m.ks.r;
+            sc.g += sp * m.ks.g;
+            sc.b += sp * m.ks.b;
+        }
+        // rt/light.h:44-59 (no spot): rgb / ((len * d)^2 + 1).
+        const float dist = illum.length * f.lights[l].d;
+        const float d_scale = (dist * dist) + 1.0f;
+        const Colour li{f.lights[l].rgb.r / d_scale, f.lights[l].rgb.g / d_scale, f.lights[l].rgb.b / d_scale};
+        c.r += li.r * (sc.r * illum.magn);
+        c.g += li.g * (sc.g * illum.magn);
+        c.b += li.b * (sc.b * illum.magn);
+    }
+    c.r += m.ka.r;
+    c.g += m.ka.g;
+    c.b += m.ka.b;
+    if (m.rf > 0.0f) {
+        // ray::reflect: dir - n * 2 (dir . n) from a point moved off the front.
+        const float power = r.magn * m.rf;
+        if (static_cast<double>(power) > kMinReflectivePower) {
+            Ray rl;
+            rl.ogn = offset_start_point(sh.dst, sh.geo_norm, 1);
+            rl.dir = reflect_dir(r.dir, n);
+            rl.magn = power;
+            const Colour pix = trace_phong(f, rl, false, lc, pc, nullptr, nullptr, nullptr);
+            const float w = m.rf / 1.0f;
+            c.r += pix.r * w;
+            c.g += pix.g * w;
+            c.b += pix.b * w;
+        }
+    }
+    if (m.tran > 0.0f) {
+        const float power = r.magn * m.tran;
+        V3 rdir;
+        if (static_cast<double>(power) > kMinReflectivePower && refract_dir(r.dir, n, m.ri, leaving, &rdir)) {
+            {
+                Ray rr;
+                rr.ogn = offset_start_point(sh.dst, sh.geo_norm, -1);
+                rr.dir = rdir;
+                rr.magn = power;
+                const Colour pix = trace_phong(f, rr, false, lc, pc, nullptr, nullptr, nullptr);
+                const float w = m.tran / 1.0f;
+                c.r += pix.r * w;
+                c.g += pix.g * w;
+                c.b += pix.b * w;
+            }
+        }
+    }
+    return c;
+}
+
+void render_pixel_phong(const FrameCtx &f, int px, int py, LocalCounters *lc, PhongCounters *pc) {
+    const int64_t p = static_cast<int64_t>(py) * f.cam.x_res + px;
+    const int nl = static_cast<int>(f.lights.size());
+    const int spp = std::max(1, f.p.spp);
+    Colour acc;
+    for (int s = 0; s < spp; ++s) {
+        float xf = static_cast<float>(px), yf = static_cast<float>(py);
+        if (spp > 1) {
+            float jx, jy;
+            pixel_jitter(f.p.seed, static_cast<uint32_t>(p), static_cast<uint32_t>(s), &jx, &jy);
+            xf = xf + jx;
+            yf = yf + jy;
+        }
+        Ray r = primary_ray(f.cam, xf, yf);
+        const bool first = (s == 0);
+        const Colour c = trace_phong(f, r, true, lc, pc, first ? f.out.prim + (f.out.prim ? p : 0) : nullptr,
+                                     (first && f.out.tuv) ? f.out.tuv + 3 * p : nullptr,
+                                     (first && f.out.vis) ? f.out.vis + nl * p : nullptr);
+        acc.r += c.r;
+        acc.g += c.g;
+        acc.b += c.b;
+    }
+    if (f.out.rgb) {
+        const float inv = static_cast<float>(spp);
+        f.out.rgb[3 * p + 0] = acc.r / inv;
+        f.out.rgb[3 * p + 1] = acc.g / inv;
+        f.out.rgb[3 * p + 2] = acc.b / inv;
+    }
+}
 
 // ---------------------------------------------------------------------------
 // The SSE packet frame (#ifdef SIMD_PACKET_TRACING, MAXIMUM_PACKET_SIZE 16):
@@ -591,6 +770,7 @@ int render_frame(const Scene &s, const float *albedo, const oracle_camera &cam, 
     const int nthreads = std::max(1, p.nthreads);
     std::atomic<int64_t> next{0};
     std::vector<LocalCounters> counters(nthreads);
+    std::vector<PhongCounters> pcounters(nthreads);
     const int tiles_x = (f.cam.x_res + 7) / 8;
     const int64_t tiles = static_cast<int64_t>(tiles_x) * ((f.cam.y_res + 7) / 8);
     auto packet_worker = [&](int tid) {
@@ -613,8 +793,12 @@ int render_frame(const Scene &s, const float *albedo, const oracle_camera &cam, 
             for (int64_t k = b; k < b + chunk; ++k) {
                 const int64_t pix = k * stride;
                 if (pix >= npix) break;
-                render_pixel(f, static_cast<int>(pix % f.cam.x_res), static_cast<int>(pix / f.cam.x_res),
-                             &counters[tid]);
+                if (p.materials)
+                    render_pixel_phong(f, static_cast<int>(pix % f.cam.x_res), static_cast<int>(pix / f.cam.x_res),
+                                       &counters[tid], &pcounters[tid]);
+                else
+                    render_pixel(f, static_cast<int>(pix % f.cam.x_res), static_cast<int>(pix / f.cam.x_res),
+                                 &counters[tid]);
             }
         }
     };
@@ -622,6 +806,13 @@ int render_frame(const Scene &s, const float *albedo, const oracle_camera &cam, 
     for (int t = 1; t < nthreads; ++t) pool.emplace_back(worker, t);
     worker(0);
     for (auto &th : pool) th.join();
+    if (out.sec_counters) {
+        out.sec_counters[0] = out.sec_counters[1] = 0;
+        for (auto &c : pcounters) {
+            out.sec_counters[0] += c.secondary;
+            out.sec_counters[1] += c.secondary_shadow;
+        }
+    }
     if (out.counters) {
         for (int k = 0; k < 6; ++k) out.counters[k] = 0;
         for (auto &c : counters) {
@@ -634,3 +825,24 @@ int render_frame(const Scene &s, const float *albedo, const oracle_camera &cam, 
 }
 
 }  // namespace oracle
+
+extern "C" {
+
+void oracle_reflect_dir(const float *d, const float *n, float *out) {
+    const oracle::V3 r = oracle::reflect_dir(oracle::V3(d[0], d[1], d[2]), oracle::V3(n[0], n[1], n[2]));
+    out[0] = r.x;
+    out[1] = r.y;
+    out[2] = r.z;
+}
+
+int32_t oracle_refract_dir(const float *d, const float *n, float ri, int32_t leaving, float *out) {
+    oracle::V3 r;
+    if (!oracle::refract_dir(oracle::V3(d[0], d[1], d[2]), oracle::V3(n[0], n[1], n[2]), ri, leaving != 0, &r))
+        return 0;
+    out[0] = r.x;
+    out[1] = r.y;
+    out[2] = r.z;
+    return 1;
+}
+
+}  // extern "C"

@@ -42,6 +42,15 @@ typedef struct {
     int32_t pixel_stride; /* trace pixels p with p % stride == 0 (packet frame: 8x8 tiles) */
     int32_t nthreads;
     int32_t packet;       /* 1: the SSE packet frame (spp 1, bounce <= 1) */
+    /* Phong materials (rt/materials/phong_shader.h:38-60, coloured form):
+     * nmaterials x 13 floats ka rgb, kd rgb, ks rgb, s, tran, ri, rf, and one
+     * index per triangle (-1: phong_shader(255 * albedo, 1), the Lambert
+     * default). NULL materials: every triangle takes the default. With
+     * materials the frame recurses into mirror and refraction rays
+     * (rt/ray.cc:118-258) and needs bounce 0, packet 0 and no records. */
+    const float *materials;
+    int32_t nmaterials;
+    const int32_t *tri_material;
 } oracle_render_params;
 
 /* Outputs; any pointer may be NULL. Arrays are indexed by pixel p = y*W + x.
@@ -66,6 +75,8 @@ typedef struct {
     int32_t *rec_prim;
     float *rec_o, *rec_n;
     uint8_t *rec_vis;
+    /* With materials: [0] reflected + refracted rays, [1] their shadow rays. */
+    uint64_t *sec_counters;
 } oracle_render_out;
 
 oracle_scene *oracle_scene_create(const float *abc, int64_t n, const uint8_t *flags);
@@ -116,6 +127,11 @@ void oracle_packet_inverse_sqrt(const float *x4, float *out4);
 void oracle_packet_trace_closest(const oracle_scene *s, const float *rays, int64_t n, int32_t *prim, float *tuv,
                                  int32_t use_bvh);
 void oracle_packet_trace_occluded(const oracle_scene *s, const float *rays, int64_t n, uint8_t *occ, int32_t use_bvh);
+/* Secondary-ray directions of rt/ray.cc:121-131 (reflect) and :200-227
+ * (refract; n faces the ray, leaving = the ray exits through the back face;
+ * returns 0 on total internal reflection). */
+void oracle_reflect_dir(const float *d, const float *n, float *out);
+int32_t oracle_refract_dir(const float *d, const float *n, float ri, int32_t leaving, float *out);
 /* Primary ray of rt/camera.h:127-158 (EXACT_NORMALISE). */
 void oracle_primary_ray(const oracle_camera *cam, int32_t x, int32_t y, float *ogn, float *dir);
 

@@ -132,6 +132,55 @@ def drift(abc, step, seed=3, speed=0.002):
     return v.reshape(abc.shape).astype(np.float32)
 
 
+def quad(p0, p1, p2, p3):
+    """Two triangles (p0, p1, p2) and (p0, p2, p3) as a (2, 9) float32 array."""
+    p0, p1, p2, p3 = (np.asarray(p, np.float32) for p in (p0, p1, p2, p3))
+    return np.stack([np.concatenate([p0, p1, p2]), np.concatenate([p0, p2, p3])]).astype(np.float32)
+
+
+def material_scene(n_objects=400, res=96, seed=5, mirror=None, glass=None, mirrored=False, light=(0.3, 3.0, -0.2),
+                   size=0.08, floor=True):
+    """Small scene for the phong materials: a diffuse floor at y = -1 (unless
+    floor is False), random triangles of the given size (diffuse, albedo
+    seeded) in x, z in [-0.8, 0.8], y in [0.2, 0.7], one light, a camera at
+    (0, 1.6, -2.2) looking at (0, 0, 0.3). mirror: reflectance of a mirror wall
+    at z = 1 behind the objects (kd = ks = 0); glass: (tran, ri) of a
+    transparent wall at z = -0.9 between the camera and the objects.
+    mirrored: the objects and the light reflected about z = 1 and no wall, the
+    scene that the mirror shows. Returns (Workload, materials, tri_material,
+    flags); the floor and the objects take the Lambert default (-1)."""
+    rng = np.random.default_rng(seed)
+    c = rng.uniform((-0.8, 0.2, -0.8), (0.8, 0.7, 0.8), (n_objects, 1, 3))
+    objs = (c + rng.uniform(-size, size, (n_objects, 3, 3))).astype(np.float32).reshape(-1, 9)
+    if mirrored:
+        v = objs.reshape(-1, 3, 3)
+        v[:, :, 2] = np.float32(2.0) - v[:, :, 2]
+        light = (light[0], light[1], 2.0 - light[2])
+    parts = ([quad((-3, -1, -3), (3, -1, -3), (3, -1, 3), (-3, -1, 3))] if floor else []) + [objs]
+    kinds = [-1] * sum(q.shape[0] for q in parts)
+    flags = [0] * len(kinds)
+    materials = []
+    if mirror is not None and not mirrored:
+        parts.append(quad((-3, -1, 1), (3, -1, 1), (3, 3, 1), (-3, 3, 1)))
+        materials.append(dict(rf=float(mirror)))
+        kinds += [len(materials) - 1] * 2
+        flags += [0, 0]
+    if glass is not None:
+        tran, ri = glass
+        parts.append(quad((-1.5, -1.2, -0.9), (1.5, -1.2, -0.9), (1.5, 1.6, -0.9), (-1.5, 1.6, -0.9)))
+        materials.append(dict(tran=float(tran), ri=float(ri)))
+        kinds += [len(materials) - 1] * 2
+        flags += [2, 2]  # transparent: shadow rays pass (the reference marks materials with tran > 0)
+    abc = np.concatenate(parts).astype(np.float32)
+    albedo = rng.uniform(0.2, 0.9, (abc.shape[0], 3)).astype(np.float32)
+    pos = (0.0, 1.6, -2.2)
+    x, y, z = look_at(pos, (0.0, 0.0, 0.3))
+    cam = Camera(f32(pos), x, y, z, 0.6, 0.6, 1.0, res, res)
+    w = Workload("materials", abc, albedo, cam, [Light(f32(light), (255.0, 240.0, 230.0), 0.1)], bounce=False,
+                 meta=dict(triangles=int(abc.shape[0])))
+    return w, materials, np.asarray(kinds, np.int32), np.asarray(flags, np.uint8)
+
+
 def terrain1m(x_res=1920, y_res=1080):
     return terrain(708, x_res, y_res)
 
