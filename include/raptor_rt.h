@@ -161,6 +161,34 @@ int32_t rt_accel_info_get(const rt_accel *accel, rt_accel_info *info);
 int32_t rt_scene_update(rt_scene *scene, const float *abc, uint32_t flags);
 int32_t rt_accel_refit(rt_accel *accel);
 
+/* ---- Phong materials. ----
+ * One material: the state of a phong_shader
+ * (rt/materials/phong_shader.h:34-53, 67-73): ka, kd, ks colours on the
+ * 0-255 scale, shininess s, transmittance tran, refractive index ri,
+ * reflectance rf. */
+typedef struct {
+    float ka[3], kd[3], ks[3];
+    float shininess, transmittance, refractive_index, reflectance;
+} rt_material;
+
+/* Gives the scene's triangles phong materials for rt_render: tri_material[i]
+ * indexes materials (host arrays), -1 keeps triangle i's Lambert shading
+ * phong_shader(255 * albedo, 1.0). rt_render then shades every hit as
+ * phong_shader::shade does (rt/materials/phong_shader.cc:21-111): diffuse and
+ * specular light per light, ambient, and the mirror (rf > 0) and refraction
+ * (tran > 0) rays traced recursively while the ray power stays above
+ * MIN_REFLECTIVE_POWER (common/common.h:30-32). The ranges the reference
+ * asserts are errors here: colours in [0, 255], tran and rf in [0, 0.95], ri
+ * in [1, 5] when tran > 0. A transparent material makes its triangles
+ * transparent to shadow rays (material(tran > 0),
+ * rt/materials/material.h:50-66), so a triangle's flags bit1 must be set
+ * exactly when its material has tran > 0. nmaterials 0 removes the
+ * materials. Only rt_render reads them; the diffuse bounce and the
+ * differentiable frames keep Lambert shading and reject a scene with
+ * materials. */
+int32_t rt_scene_set_materials(rt_scene *scene, const rt_material *materials, int32_t nmaterials,
+                               const int32_t *tri_material);
+
 /* Closest hit for n rays. Replaces bvh::find_nearest_object(ray*)
  * (ssd/bvh.cc:649-709) and its packet form (ssd/bvh.cc:406-447). */
 int32_t rt_trace_closest(rt_accel *accel, const rt_ray *rays, int64_t n, rt_hit *out, uint32_t flags);

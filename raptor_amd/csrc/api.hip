@@ -99,6 +99,8 @@ void free_scene(rt_scene *sc) {
     if (sc->d_c) (void)hipFree(sc->d_c);
     if (sc->d_kd) (void)hipFree(sc->d_kd);
     if (sc->d_flags) (void)hipFree(sc->d_flags);
+    if (sc->d_mat) (void)hipFree(sc->d_mat);
+    if (sc->d_tri_mat) (void)hipFree(sc->d_tri_mat);
     delete sc;
 }
 
@@ -267,6 +269,82 @@ int32_t rt_scene_update(rt_scene *scene, const float *abc, uint32_t flags) {
     return RT_OK;
 }
 
+int32_t rt_scene_set_materials(rt_scene *scene, const rt_material *materials, int32_t nmaterials,
+                               const int32_t *tri_material) {
+    if (!scene) return RT_ERR_INVALID;
+    rt_ctx *ctx = scene->ctx;
+    if (int32_t rc = rt::set_device(ctx)) return rc;
+    if (nmaterials < 0 || (nmaterials > 0 && (!materials || (scene->n > 0 && !tri_material))))
+        return fail(ctx, RT_ERR_INVALID, "rt_scene_set_materials: bad material arrays");
+    hipStream_t s = ctx->stream;
+    auto drop = [&]() {
+        (void)hipStreamSynchronize(s);
+        if (scene->d_mat) (void)hipFree(scene->d_mat);
+        if (scene->d_tri_mat) (void)hipFree(scene->d_tri_mat);
+        scene->d_mat = nullptr;
+        scene->d_tri_mat = nullptr;
+        scene->nmat = 0;
+    };
+    if (nmaterials == 0) {
+        drop();
+        return RT_OK;
+    }
+    // The ranges phong_shader asserts (rt/materials/phong_shader.h:38-51).
+    std::vector<float4> table(4 * static_cast<size_t>(nmaterials));
+    for (int32_t m = 0; m < nmaterials; ++m) {
+        const rt_material &q = materials[m];
+        bool ok = q.transmittance >= 0.0f && q.transmittance <= 0.95f && q.reflectance >= 0.0f &&
+                  q.reflectance <= 0.95f &&
+                  (q.transmittance == 0.0f || (q.refractive_index >= 1.0f && q.refractive_index <= 5.0f));
+        for (int k = 0; k < 3; ++k)
+            for (const float v : {q.ka[k], q.kd[k], q.ks[k]}) ok = ok && v >= 0.0f && v <= 255.0f;
+        if (!ok)
+            return fail(ctx, RT_ERR_INVALID,
+                        "rt_scene_set_materials: material " + std::to_string(m) +
+                            " out of range (colours 0..255, tran and rf 0..0.95, ri 1..5 when tran > 0)");
+        table[4 * m + 0] = make_float4(q.ka[0], q.ka[1], q.ka[2], q.shininess);
+        table[4 * m + 1] = make_float4(q.kd[0], q.kd[1], q.kd[2], q.transmittance);
+        table[4 * m + 2] = make_float4(q.ks[0], q.ks[1], q.ks[2], q.refractive_index);
+        table[4 * m + 3] = make_float4(q.reflectance, 0.0f, 0.0f, 0.0f);
+    }
+    // A triangle is transparent to shadow rays exactly when its material is
+    // (material(tran > 0), rt/materials/material.h:50-66).
+    std::vector<uint8_t> flags(std::max<int64_t>(1, scene->n), 0);
+    if (scene->d_flags && scene->n > 0) {
+        RT_HIP_CHECK(ctx, hipMemcpyAsync(flags.data(), scene->d_flags, scene->n, hipMemcpyDeviceToHost, s));
+        RT_HIP_CHECK(ctx, hipStreamSynchronize(s));
+    }
+    for (int64_t i = 0; i < scene->n; ++i) {
+        const int32_t m = tri_material[i];
+        if (m < -1 || m >= nmaterials)
+            return fail(ctx, RT_ERR_INVALID,
+                        "rt_scene_set_materials: triangle " + std::to_string(i) + " has no material " +
+                            std::to_string(m));
+        const bool transparent = m >= 0 && materials[m].transmittance > 0.0f;
+        if (transparent != ((flags[i] & 2) != 0))
+            return fail(ctx, RT_ERR_INVALID,
+                        "rt_scene_set_materials: triangle " + std::to_string(i) +
+                            (transparent ? " has a transparent material but no transparent flag"
+                                         : " is flagged transparent but its material is not"));
+    }
+    drop();
+    float4 *d_mat = nullptr;
+    int32_t *d_tri = nullptr;
+    hipError_t e = rt::upload(&d_mat, table.data(), static_cast<int64_t>(table.size()), s);
+    if (e == hipSuccess) e = rt::upload(&d_tri, tri_material, scene->n, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        if (d_mat) (void)hipFree(d_mat);
+        if (d_tri) (void)hipFree(d_tri);
+        return fail(ctx, e == hipErrorOutOfMemory ? RT_ERR_OOM : RT_ERR_HIP,
+                    std::string("rt_scene_set_materials: ") + hipGetErrorString(e));
+    }
+    scene->d_mat = d_mat;
+    scene->d_tri_mat = d_tri;
+    scene->nmat = nmaterials;
+    return RT_OK;
+}
+
 int32_t rt_accel_refit(rt_accel *accel) {
     if (!accel) return RT_ERR_INVALID;
     if (int32_t rc = rt::set_device(accel->scene->ctx)) return rc;
@@ -341,7 +419,13 @@ int32_t rt_render(rt_accel *accel, const rt_camera *cam, const rt_light *lights,
     if (params) p = *params;
     if (ctx->counting)
         RT_HIP_CHECK(ctx, hipMemsetAsync(ctx->d_counters, 0, 16 * sizeof(unsigned long long), ctx->stream));
-    int32_t rc = rt::render_frame(accel, *cam, lights, nlights, p, rgb, primary, bounce, stats);
+    int32_t rc = RT_OK;
+    if (accel->scene->d_mat) {
+        rc = bounce ? fail(ctx, RT_ERR_UNSUPPORTED, "rt_render: the diffuse bounce is not defined with materials")
+                    : rt::render_frame_materials(accel, *cam, lights, nlights, p, rgb, primary, stats);
+    } else {
+        rc = rt::render_frame(accel, *cam, lights, nlights, p, rgb, primary, bounce, stats);
+    }
     ctx->count_slot = 0;
     if (rc == RT_OK && stats) {
         for (auto &v : stats->visits) v = 0;

@@ -1,0 +1,514 @@
+// Frames with phong materials: the reference's recursive shading of one hit
+// (rt/raytracer.cc:20-87) with phong_shader::shade
+// (rt/materials/phong_shader.cc:21-111) and the mirror and refraction rays of
+// ray::reflect / ray::refract without DIFFUSE_REFLECTIONS (rt/ray.cc:118-258),
+// run as a wavefront of ray generations:
+//
+//   generation 0     the primary rays of every sample (8x8 tiles, as rt_render)
+//   closest hit
+//   k_mat_classify   hit flag and number of children of each ray (mirror ray
+//                    while the ray power times rf stays above
+//                    MIN_REFLECTIVE_POWER, refraction ray likewise unless
+//                    totally reflected)
+//   scans            compacted hit slots and child slots, in ray order
+//   k_mat_emit       per hit: surface record, one shadow ray per light, the
+//                    children into the next generation's queue
+//   any hit          shadow rays
+//   k_mat_local      local colour (lights, specular, ambient) of every hit
+//
+// until a generation has no children. Then k_mat_gather walks the
+// generations backwards and folds each hit's children into it in the
+// reference's order, c = (local + rf c_reflected) + tran c_refracted, so the
+// colour of a ray is the one the recursion returns, rounding included.
+// Children sit in the next generation in the order of their parents and
+// every sum has a fixed order: frames are reproducible bit for bit.
+#include <algorithm>
+#include <type_traits>
+#include <vector>
+
+#include "frame.hpp"
+#include "radix_sort.hpp"
+#include "rt_device.hpp"
+
+namespace rt {
+
+namespace {
+
+// common/common.h:30-32 is a double literal, so the float power is compared
+// in double (a power of exactly 0.1f lies above it and still recurses).
+constexpr double kMinReflectivePower = 0.1;
+
+// Four float4 per material: {ka rgb, shininess}, {kd rgb, transmittance},
+// {ks rgb, refractive index}, {reflectance, 0, 0, 0}.
+struct Mat {
+    float4 ka_s, kd_tran, ks_ri;
+    float rf;
+};
+
+// The triangle's material: its table entry, or phong_shader(255 * albedo, 1)
+// (kd from the scene, ks = ka = 0, s = 0, no mirror or refraction).
+__device__ __forceinline__ Mat mat_of(const float4 *mats, const int32_t *tri_mat, const float4 *kd, int32_t prim) {
+    Mat m;
+    const int32_t mi = tri_mat[prim];
+    if (mi < 0) {
+        const float4 k = kd[prim];
+        m.ka_s = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        m.kd_tran = make_float4(k.x, k.y, k.z, 0.0f);
+        m.ks_ri = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+        m.rf = 0.0f;
+    } else {
+        const float4 *q = mats + 4 * static_cast<int64_t>(mi);
+        m.ka_s = q[0];
+        m.kd_tran = q[1];
+        m.ks_ri = q[2];
+        m.rf = q[3].x;
+    }
+    return m;
+}
+
+// ray::refract (rt/ray.cc:200-227): n faces the ray; the index is inverted
+// for a ray entering the front face (hit_t::out_in); false on total internal
+// reflection.
+__device__ __forceinline__ bool refract_dir(rtd::V3 d, rtd::V3 n, float ri, bool leaving, rtd::V3 *out) {
+    const float rdn = -rtd::dot(d, n);
+    if (!leaving) ri = 1.0f / ri;
+    const float cos_t2 = 1.0f - ri * ri * (1.0f - rdn * rdn);
+    if (cos_t2 < 0.0f) return false;
+    const float offset = (ri * rdn - sqrtf(cos_t2));
+    *out = rtd::add(rtd::mul(d, ri), rtd::mul(n, offset));
+    return true;
+}
+
+// ray::reflect (rt/ray.cc:121-131): dir - n * 2 (dir . n).
+__device__ __forceinline__ rtd::V3 reflect_dir(rtd::V3 d, rtd::V3 n) {
+    const float rdn = 2.0f * rtd::dot(d, n);
+    return rtd::sub(d, rtd::mul(n, rdn));
+}
+
+// One generation: rays (2 float4 each), their power, their closest hits.
+struct GenArgs {
+    const float4 *rays;
+    const float *power;
+    const float4 *hits;  // t, u, v, prim
+    int64_t n;
+    const float4 *tri_n, *tri_c, *kd, *mats;
+    const int32_t *tri_mat;
+};
+
+// Padding slots past the image edge get tmax < tmin, so they never hit.
+__global__ void __launch_bounds__(256) k_mat_raygen(CamArgs c, TileMap tm, int spp, uint64_t seed, int64_t ns,
+                                                    float4 *__restrict__ rays, float *__restrict__ power) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= ns) return;
+    const int64_t slot = i / spp;
+    const int s = static_cast<int>(i % spp);
+    power[i] = 1.0f;
+    int x, y;
+    if (!tm.pixel_of_slot(slot, &x, &y)) {
+        rays[2 * i] = make_float4(c.px, c.py, c.pz, rtd::kEpsilon);
+        rays[2 * i + 1] = make_float4(0.0f, 0.0f, 1.0f, -1.0f);
+        return;
+    }
+    const int64_t p = static_cast<int64_t>(y) * c.x_res + x;
+    const rtd::V3 d = primary_dir(c, x, y, p, s, spp, seed);
+    rays[2 * i] = make_float4(c.px, c.py, c.pz, rtd::kEpsilon);
+    rays[2 * i + 1] = make_float4(d.x, d.y, d.z, rtd::kMaxDist);
+}
+
+// The surface at ray i's hit and which children it spawns.
+struct HitSurface {
+    rtd::V3 dst, geo, shn, rdir;
+    Mat m;
+    bool refl, refr;
+};
+
+__device__ __forceinline__ HitSurface surface_of(const GenArgs &a, int64_t i) {
+    HitSurface h;
+    const float4 r0 = a.rays[2 * i], r1 = a.rays[2 * i + 1];
+    const float4 hit = a.hits[i];
+    const int32_t prim = __float_as_int(hit.w);
+    const float4 tn = a.tri_n[prim];
+    const rtd::V3 d = rtd::f4v(r1);
+    const bool leaving = rtd::dot(rtd::f4v(tn), d) > 0.0f;  // hit_t::in_out (rt/triangle.h:487-496)
+    surface_at(rtd::f4v(r0), d, hit.x, rtd::f4v(tn), rtd::f4v(a.tri_c[prim]), &h.dst, &h.geo, &h.shn);
+    h.m = mat_of(a.mats, a.tri_mat, a.kd, prim);
+    const float power = a.power[i];
+    h.refl = h.m.rf > 0.0f && static_cast<double>(power * h.m.rf) > kMinReflectivePower;
+    h.refr = h.m.kd_tran.w > 0.0f && static_cast<double>(power * h.m.kd_tran.w) > kMinReflectivePower &&
+             refract_dir(d, h.shn, h.m.ks_ri.w, leaving, &h.rdir);
+    return h;
+}
+
+// flags[i] = 1 for a hit, kids[i] = its number of children; entry n of both
+// is 0 so that the exclusive scans over n + 1 entries end with the totals.
+__global__ void __launch_bounds__(256) k_mat_classify(GenArgs a, uint32_t *__restrict__ flags,
+                                                      uint32_t *__restrict__ kids) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i > a.n) return;
+    uint32_t f = 0, k = 0;
+    if (i < a.n && __float_as_int(a.hits[i].w) >= 0) {
+        f = 1;
+        const HitSurface h = surface_of(a, i);
+        k = static_cast<uint32_t>(h.refl) + static_cast<uint32_t>(h.refr);
+    }
+    flags[i] = f;
+    kids[i] = k;
+}
+
+struct EmitArgs {
+    GenArgs g;
+    const uint32_t *hit_pos, *kid_pos;  // exclusive scans of the flags and the child counts
+    LightArgs lights;
+    float4 *rec_n;      // per hit: shading normal, triangle (int bits)
+    float4 *rec_d;      // per hit: incoming direction
+    int2 *rec_kids;     // per hit: first child slot, children (bit 0 mirror, bit 1 refraction)
+    float4 *shadow;     // per hit x light: 2 float4
+    float4 *next_rays;  // children, 2 float4 each
+    float *next_power;
+};
+
+__global__ void __launch_bounds__(256) k_mat_emit(EmitArgs e) {
+    const GenArgs &a = e.g;
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= a.n || __float_as_int(a.hits[i].w) < 0) return;
+    const HitSurface h = surface_of(a, i);
+    const int64_t q = e.hit_pos[i];
+    const float4 r1 = a.rays[2 * i + 1];
+    const uint32_t c0 = e.kid_pos[i];
+    e.rec_n[q] = make_float4(h.shn.x, h.shn.y, h.shn.z, a.hits[i].w);
+    e.rec_d[q] = make_float4(r1.x, r1.y, r1.z, 0.0f);
+    e.rec_kids[q] = make_int2(static_cast<int32_t>(c0),
+                              static_cast<int32_t>(h.refl) | (static_cast<int32_t>(h.refr) << 1));
+    // Shadow rays and the mirror ray start off the front of the surface.
+    const rtd::V3 org = rtd::offset_start_point(h.dst, h.geo, 1);
+    for (int l = 0; l < e.lights.n; ++l) {
+        rtd::V3 dn;
+        float len;
+        light_dir(org, e.lights.pos_d[l], &dn, &len);
+        float4 *sr = e.shadow + 2 * (q * e.lights.n + l);
+        sr[0] = make_float4(org.x, org.y, org.z, rtd::kEpsilon);
+        sr[1] = make_float4(dn.x, dn.y, dn.z, len);
+    }
+    const float power = a.power[i];
+    int64_t c = c0;
+    if (h.refl) {
+        const rtd::V3 rd = reflect_dir(rtd::f4v(r1), h.shn);
+        e.next_rays[2 * c] = make_float4(org.x, org.y, org.z, rtd::kEpsilon);
+        e.next_rays[2 * c + 1] = make_float4(rd.x, rd.y, rd.z, rtd::kMaxDist);
+        e.next_power[c] = power * h.m.rf;
+        ++c;
+    }
+    if (h.refr) {
+        // The refraction ray starts off the back of the surface.
+        const rtd::V3 o2 = rtd::offset_start_point(h.dst, h.geo, -1);
+        e.next_rays[2 * c] = make_float4(o2.x, o2.y, o2.z, rtd::kEpsilon);
+        e.next_rays[2 * c + 1] = make_float4(h.rdir.x, h.rdir.y, h.rdir.z, rtd::kMaxDist);
+        e.next_power[c] = power * h.m.kd_tran.w;
+    }
+}
+
+struct LocalArgs {
+    const float4 *rec_n, *rec_d;
+    const float4 *shadow;
+    const uint8_t *occ;
+    int64_t nhits;
+    LightArgs lights;
+    const float4 *kd, *mats;
+    const int32_t *tri_mat;
+    float4 *colour;  // per hit: local colour
+};
+
+// phong_shader::shade without the secondary rays
+// (rt/materials/phong_shader.cc:21-64) with the light intensity of
+// rt/light.h:44-59: diffuse and specular per light, then the ambient term.
+__global__ void __launch_bounds__(256) k_mat_local(LocalArgs a) {
+    const int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (q >= a.nhits) return;
+    const float4 nn = a.rec_n[q];
+    const rtd::V3 n = rtd::f4v(nn), d = rtd::f4v(a.rec_d[q]);
+    const Mat m = mat_of(a.mats, a.tri_mat, a.kd, __float_as_int(nn.w));
+    Rgb c{0.0f, 0.0f, 0.0f};
+    for (int l = 0; l < a.lights.n; ++l) {
+        const int64_t k = q * a.lights.n + l;
+        const float4 sd = a.shadow[2 * k + 1];
+        const rtd::V3 dn = rtd::f4v(sd);
+        const float magn = (a.occ[k] ? 0.0f : 1.0f) / 1.0f;
+        const float shade = rtd::dot(dn, n);
+        if (shade < 0.0f) continue;
+        // The light ray reflected about the normal.
+        const float shade_x2 = 2.0f * shade;
+        const rtd::V3 ref_dir = rtd::mk(dn.x - n.x * shade_x2, dn.y - n.y * shade_x2, dn.z - n.z * shade_x2);
+        const float rdr = rtd::dot(d, ref_dir);
+        float scr = shade * m.kd_tran.x, scg = shade * m.kd_tran.y, scb = shade * m.kd_tran.z;
+        if (rdr > 0.0f) {
+            const float sp = powf(rdr, m.ka_s.w);
+            scr += sp * m.ks_ri.x;
+            scg += sp * m.ks_ri.y;
+            scb += sp * m.ks_ri.z;
+        }
+        const float dist = sd.w * a.lights.pos_d[l].w;
+        const float d_scale = (dist * dist) + 1.0f;
+        const float4 rgb = a.lights.rgb[l];
+        c.r = c.r + ((rgb.x / d_scale) * (scr * magn));
+        c.g = c.g + ((rgb.y / d_scale) * (scg * magn));
+        c.b = c.b + ((rgb.z / d_scale) * (scb * magn));
+    }
+    c.r += m.ka_s.x;
+    c.g += m.ka_s.y;
+    c.b += m.ka_s.z;
+    a.colour[q] = make_float4(c.r, c.g, c.b, 0.0f);
+}
+
+struct GatherArgs {
+    float4 *colour;  // this generation's hits: local colour in, full colour out
+    const int2 *rec_kids;
+    const float4 *rec_n;  // triangle of each hit in w
+    int64_t nhits;
+    const float4 *kd, *mats;
+    const int32_t *tri_mat;
+    const uint32_t *child_hit_pos;  // next generation: exclusive scan of its hit flags
+    const float4 *child_colour;     // next generation: full colour of its hits
+};
+
+// The colour a child ray returns: its hit's colour, or the background (0).
+__device__ __forceinline__ float4 child_colour(const GatherArgs &a, int64_t j) {
+    const uint32_t k = a.child_hit_pos[j];
+    if (a.child_hit_pos[j + 1] == k) return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    return a.child_colour[k];
+}
+
+// rt/materials/phong_shader.cc:66-111: c += reflected * (rf / 1), then
+// c += refracted * (tran / 1).
+__global__ void __launch_bounds__(256) k_mat_gather(GatherArgs a) {
+    const int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (q >= a.nhits) return;
+    const int2 kids = a.rec_kids[q];
+    if (kids.y == 0) return;
+    const Mat m = mat_of(a.mats, a.tri_mat, a.kd, __float_as_int(a.rec_n[q].w));
+    float4 c = a.colour[q];
+    int64_t j = kids.x;
+    if (kids.y & 1) {
+        const float4 p = child_colour(a, j++);
+        const float w = m.rf / 1.0f;
+        c.x += p.x * w;
+        c.y += p.y * w;
+        c.z += p.z * w;
+    }
+    if (kids.y & 2) {
+        const float4 p = child_colour(a, j);
+        const float w = m.kd_tran.w / 1.0f;
+        c.x += p.x * w;
+        c.y += p.y * w;
+        c.z += p.z * w;
+    }
+    a.colour[q] = c;
+}
+
+// Pixel = (sum of its samples' colours) / spp, samples in order.
+__global__ void __launch_bounds__(256) k_mat_resolve(TileMap tm, int spp, int64_t npix,
+                                                     const uint32_t *__restrict__ hit_pos,
+                                                     const float4 *__restrict__ colour, float *__restrict__ rgb) {
+    const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p >= npix) return;
+    const int x = static_cast<int>(p % tm.x_res), y = static_cast<int>(p / tm.x_res);
+    const int64_t i0 = tm.slot_of_pixel(x, y) * spp;
+    float r = 0.0f, g = 0.0f, b = 0.0f;
+    for (int s = 0; s < spp; ++s) {
+        const uint32_t k = hit_pos[i0 + s];
+        if (hit_pos[i0 + s + 1] == k) continue;  // background: + 0
+        const float4 v = colour[k];
+        r += v.x;
+        g += v.y;
+        b += v.z;
+    }
+    const float inv = static_cast<float>(spp);
+    rgb[3 * p + 0] = r / inv;
+    rgb[3 * p + 1] = g / inv;
+    rgb[3 * p + 2] = b / inv;
+}
+
+// Primary hits of every sample in pixel order (pixel * spp + sample).
+__global__ void __launch_bounds__(256) k_mat_hits_out(TileMap tm, int spp, int64_t npix,
+                                                      const float4 *__restrict__ hits, float4 *__restrict__ out) {
+    const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= npix * spp) return;
+    const int64_t p = k / spp;
+    const int s = static_cast<int>(k % spp);
+    const int x = static_cast<int>(p % tm.x_res), y = static_cast<int>(p / tm.x_res);
+    out[k] = hits[tm.slot_of_pixel(x, y) * spp + s];
+}
+
+hipError_t scan_u32(uint32_t *data, int64_t n, uint32_t *tile_sums, hipStream_t s) {
+    const int64_t blocks = (n + kScanTile - 1) / kScanTile;
+    if (n <= 0 || blocks > kScanTile) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_scan_tiles, dim3(static_cast<unsigned>(blocks)), dim3(kScanThreads), 0, s, data, n,
+                       tile_sums);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanThreads), 0, s, tile_sums, static_cast<int>(blocks));
+    hipLaunchKernelGGL(k_scan_add, dim3(static_cast<unsigned>(blocks)), dim3(kScanThreads), 0, s, data, n,
+                       tile_sums);
+    return hipGetLastError();
+}
+
+dim3 grid_of(int64_t n) { return dim3(static_cast<unsigned>(std::max<int64_t>(1, (n + 255) / 256))); }
+
+// What the backward pass needs of one generation.
+struct Gen {
+    int64_t n = 0, nhits = 0;
+    uint32_t *hit_pos = nullptr;
+    float4 *colour = nullptr, *rec_n = nullptr;
+    int2 *rec_kids = nullptr;
+};
+
+// The power of a ray falls by a factor of at least 0.95 (the largest rf and
+// transmittance) per generation, so a chain from a primary ray of power 1
+// stays above 0.1 for at most 45 generations.
+constexpr int kMaxGenerations = 64;
+
+}  // namespace
+
+int32_t render_frame_materials(rt_accel *a, const rt_camera &cam, const rt_light *lights, int nlights,
+                               const rt_render_params &p, float *rgb, rt_hit *primary, rt_stats *stats) {
+    rt_scene *sc = a->scene;
+    rt_ctx *ctx = sc->ctx;
+    hipStream_t s = ctx->stream;
+    if (p.bounce) return fail(ctx, RT_ERR_UNSUPPORTED, "rt_render: the diffuse bounce is not defined with materials");
+    FramePlan f;
+    int32_t rc = plan_frame(a, cam, lights, nlights, p, &f);
+    if (rc) return rc;
+    const int nl = f.lights.n;
+    const int64_t ns = f.tiles * 64 * f.spp;
+    if (ns + 1 > int64_t(kScanTile) * kScanTile) return fail(ctx, RT_ERR_INVALID, "rt_render: frame too large");
+    BuildArena &ar = ctx->arena;
+    RT_HIP_CHECK(ctx, ar.begin());
+    auto take = [&](auto **ptr, int64_t count) -> int32_t {
+        void *v = nullptr;
+        const hipError_t e = ar.take(&v, static_cast<size_t>(std::max<int64_t>(1, count)) * sizeof(**ptr));
+        if (e != hipSuccess) return fail(ctx, RT_ERR_OOM, std::string("rt_render: ") + hipGetErrorString(e));
+        *ptr = reinterpret_cast<std::remove_reference_t<decltype(*ptr)>>(v);
+        return RT_OK;
+    };
+    float4 *rays = nullptr;
+    float *power = nullptr;
+    uint32_t *sums = nullptr;
+    if ((rc = take(&rays, 2 * ns)) || (rc = take(&power, ns)) || (rc = take(&sums, kScanTile))) return rc;
+    const bool timed = stats != nullptr;
+    if (timed) RT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[0], s));
+    hipLaunchKernelGGL(k_mat_raygen, grid_of(ns), dim3(256), 0, s, f.cam, f.tm, f.spp, p.seed, ns, rays, power);
+    uint64_t ray_count[4] = {static_cast<uint64_t>(f.npix * f.spp), 0, 0, 0};
+    float trace_ms[2] = {0.0f, 0.0f};  // closest hit and any hit, summed over generations
+    std::vector<Gen> gens;
+    float4 *primary_hits = nullptr;
+    for (int64_t n = ns; n > 0;) {
+        if (static_cast<int>(gens.size()) == kMaxGenerations)
+            return fail(ctx, RT_ERR_INVALID, "rt_render: secondary rays did not die out");
+        const bool first = gens.empty();
+        Gen gen;
+        gen.n = n;
+        float4 *hits = nullptr;
+        uint32_t *kids = nullptr;
+        if ((rc = take(&hits, n)) || (rc = take(&gen.hit_pos, n + 1)) || (rc = take(&kids, n + 1))) return rc;
+        if (first) primary_hits = hits;
+        if (timed) RT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[1], s));
+        ctx->count_slot = first ? 0 : 2;  // node-visit counters: primary or secondary closest hit
+        if ((rc = launch_closest(a, reinterpret_cast<const rt_ray *>(rays), n, reinterpret_cast<rt_hit *>(hits), s,
+                                 first ? 0 : 2)))
+            return rc;
+        if (timed) RT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[2], s));
+        const GenArgs g{rays, power, hits, n, sc->d_n, sc->d_c, sc->d_kd, sc->d_mat, sc->d_tri_mat};
+        hipLaunchKernelGGL(k_mat_classify, grid_of(n + 1), dim3(256), 0, s, g, gen.hit_pos, kids);
+        RT_HIP_CHECK(ctx, scan_u32(gen.hit_pos, n + 1, sums, s));
+        RT_HIP_CHECK(ctx, scan_u32(kids, n + 1, sums, s));
+        uint32_t tot[2];
+        RT_HIP_CHECK(ctx, hipMemcpyAsync(&tot[0], gen.hit_pos + n, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        RT_HIP_CHECK(ctx, hipMemcpyAsync(&tot[1], kids + n, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        RT_HIP_CHECK(ctx, hipStreamSynchronize(s));
+        if (timed) {
+            float ms = 0.0f;
+            RT_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ctx->ev[1], ctx->ev[2]));
+            trace_ms[0] += ms;
+        }
+        const int64_t nh = tot[0], nk = tot[1];
+        gen.nhits = nh;
+        if (!first) ray_count[2] += static_cast<uint64_t>(n);
+        ray_count[first ? 1 : 3] += static_cast<uint64_t>(nh * nl);
+        float4 *next_rays = nullptr;
+        float *next_power = nullptr;
+        if (nh > 0) {
+            float4 *rec_d = nullptr, *shadow = nullptr;
+            uint8_t *occ = nullptr;
+            if ((rc = take(&gen.rec_n, nh)) || (rc = take(&rec_d, nh)) || (rc = take(&gen.rec_kids, nh)) ||
+                (rc = take(&gen.colour, nh)) || (rc = take(&shadow, 2 * nh * nl)) || (rc = take(&occ, nh * nl)) ||
+                (rc = take(&next_rays, 2 * nk)) || (rc = take(&next_power, nk)))
+                return rc;
+            const EmitArgs e{g, gen.hit_pos, kids, f.lights, gen.rec_n, rec_d, gen.rec_kids, shadow, next_rays,
+                             next_power};
+            hipLaunchKernelGGL(k_mat_emit, grid_of(n), dim3(256), 0, s, e);
+            if (nl > 0) {
+                if (timed) RT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[3], s));
+                ctx->count_slot = first ? 1 : 3;
+                if ((rc = launch_occluded(a, reinterpret_cast<const rt_ray *>(shadow), nh * nl, occ, s,
+                                          first ? 1 : 3)))
+                    return rc;
+                if (timed) RT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[4], s));
+            }
+            const LocalArgs la{gen.rec_n, rec_d, shadow, occ, nh, f.lights, sc->d_kd, sc->d_mat, sc->d_tri_mat,
+                               gen.colour};
+            hipLaunchKernelGGL(k_mat_local, grid_of(nh), dim3(256), 0, s, la);
+            RT_HIP_CHECK(ctx, hipGetLastError());
+            if (timed && nl > 0) {
+                RT_HIP_CHECK(ctx, hipEventSynchronize(ctx->ev[4]));
+                float ms = 0.0f;
+                RT_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ctx->ev[3], ctx->ev[4]));
+                trace_ms[1] += ms;
+            }
+        }
+        gens.push_back(gen);
+        rays = next_rays;
+        power = next_power;
+        n = (nh > 0) ? nk : 0;
+    }
+    // Fold every generation's colours into its parents, deepest first.
+    for (int k = static_cast<int>(gens.size()) - 2; k >= 0; --k) {
+        const Gen &g = gens[k], &c = gens[k + 1];
+        if (g.nhits == 0) continue;
+        const GatherArgs ga{g.colour, g.rec_kids, g.rec_n, g.nhits, sc->d_kd, sc->d_mat, sc->d_tri_mat, c.hit_pos,
+                            c.colour};
+        hipLaunchKernelGGL(k_mat_gather, grid_of(g.nhits), dim3(256), 0, s, ga);
+    }
+    const bool dev_out = (p.flags & RT_DEVICE_PTRS) != 0;
+    float *rgb_dev = nullptr;
+    if (rgb && dev_out) {
+        rgb_dev = rgb;
+    } else if ((rc = take(&rgb_dev, 3 * f.npix))) {
+        return rc;
+    }
+    hipLaunchKernelGGL(k_mat_resolve, grid_of(f.npix), dim3(256), 0, s, f.tm, f.spp, f.npix, gens[0].hit_pos,
+                       gens[0].colour, rgb_dev);
+    RT_HIP_CHECK(ctx, hipGetLastError());
+    if (timed) RT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[5], s));
+    if (rgb && !dev_out)
+        RT_HIP_CHECK(ctx, hipMemcpyAsync(rgb, rgb_dev, 3 * f.npix * sizeof(float), hipMemcpyDeviceToHost, s));
+    if (primary) {
+        float4 *hout = nullptr;
+        if ((rc = take(&hout, f.npix * f.spp))) return rc;
+        hipLaunchKernelGGL(k_mat_hits_out, grid_of(f.npix * f.spp), dim3(256), 0, s, f.tm, f.spp, f.npix,
+                           primary_hits, hout);
+        RT_HIP_CHECK(ctx, hipMemcpyAsync(primary, hout, f.npix * f.spp * sizeof(rt_hit),
+                                         dev_out ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+    }
+    RT_HIP_CHECK(ctx, hipGetLastError());
+    RT_HIP_CHECK(ctx, hipStreamSynchronize(s));
+    if (stats) {
+        for (int k = 0; k < 4; ++k) stats->rays[k] = ray_count[k];
+        float total = 0.0f;
+        RT_HIP_CHECK(ctx, hipEventElapsedTime(&total, ctx->ev[0], ctx->ev[5]));
+        for (auto &v : stats->ms) v = 0.0f;
+        stats->ms[0] = total;
+        stats->ms[1] = trace_ms[0];  // closest hit of every generation
+        stats->ms[2] = trace_ms[1];  // any hit of every generation
+        stats->ms[5] = total - trace_ms[0] - trace_ms[1];
+    }
+    return RT_OK;
+}
+
+}  // namespace rt

@@ -89,6 +89,12 @@ struct rt_scene {
     float4 *d_c = nullptr;     // vertex c (xyz) for the plane re-projection
     float4 *d_kd = nullptr;    // kd = (255 * albedo) * 1.0
     uint8_t *d_flags = nullptr;
+    // Phong materials (rt_scene_set_materials): 4 float4 per material, the
+    // material of each triangle (-1: Lambert from the albedo); both null when
+    // none are set.
+    float4 *d_mat = nullptr;
+    int32_t *d_tri_mat = nullptr;
+    int32_t nmat = 0;
     std::atomic<int> refs{1};  // the caller's handle + one per accel
 };
 
@@ -108,11 +114,11 @@ struct rt_accel {
     float sah_cost = 0.0f;
     float box_pad = 0.0f;
     float near_limit = 0.0f;
-    float scene_lo[3] = {0, 0, 0}, scene_inv[3] = {0, 0, 0};  // scene box origin and 1/extent   // |origin| bound for the fused box test (256x the largest scene coordinate)
+    float scene_lo[3] = {0, 0, 0}, scene_inv[3] = {0, 0, 0};  // scene box origin and 1/extent
     void *frame_ws = nullptr;  // frame workspace (render.hip)
     int64_t triangles = 0;
-    std::vector<int64_t> level_start;
-    float *d_refit = nullptr;  // refit scratch: scene bounds, padding, partial bounds  // wide nodes of tree level L: [level_start[L], level_start[L + 1])
+    std::vector<int64_t> level_start;  // wide nodes of tree level L: [level_start[L], level_start[L + 1])
+    float *d_refit = nullptr;          // refit scratch: scene bounds, padding, partial bounds
     std::atomic<int> refs{1};  // the caller's handle + one per batch item
 };
 
@@ -156,5 +162,8 @@ void free_frame_workspace(rt_accel *a);
 int32_t refit_accel(rt_accel *a);
 int32_t render_frame(rt_accel *a, const rt_camera &cam, const rt_light *lights, int nlights,
                      const rt_render_params &p, float *rgb, rt_hit *primary, rt_hit *bounce, rt_stats *stats);
+// Frame of a scene with phong materials (materials.hip).
+int32_t render_frame_materials(rt_accel *a, const rt_camera &cam, const rt_light *lights, int nlights,
+                               const rt_render_params &p, float *rgb, rt_hit *primary, rt_stats *stats);
 
 }  // namespace rt

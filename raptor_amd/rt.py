@@ -33,6 +33,7 @@ EXPORTS = [
     "rt_batch_create", "rt_batch_run", "rt_batch_destroy", "rt_batch_autotune", "rt_gen_surface_rays",
     "rt_scene_set_albedo", "rt_render_tape", "rt_tape_info", "rt_render_adjoint", "rt_tape_destroy",
     "rt_invrender_create", "rt_invrender_run", "rt_invrender_destroy", "rt_scene_update", "rt_accel_refit",
+    "rt_scene_set_materials",
 ]
 
 
@@ -121,6 +122,7 @@ def load_library(path=LIB_PATH):
         "rt_scene_set_albedo": (i32, [vp, fp, u32]),
         "rt_scene_update": (i32, [vp, fp, u32]),
         "rt_accel_refit": (i32, [vp]),
+        "rt_scene_set_materials": (i32, [vp, fp, i32, fp]),
         "rt_render_tape": (i32, [vp, C.POINTER(rt_camera), C.POINTER(rt_light), i32, C.POINTER(rt_render_params), fp,
                                  i64, C.POINTER(vp), C.POINTER(rt_stats)]),
         "rt_tape_info": (i32, [vp, C.POINTER(i32), C.POINTER(i64), C.POINTER(i64)]),
@@ -189,6 +191,24 @@ def gen_terrain(grid, amplitude=0.15, zoom=0.02, persistence=0.5, octaves=8, noi
 
 # ---------------------------------------------------------------- handles
 
+def material_table(materials):
+    """Rows of rt_material from dicts with the phong_shader fields: ka, kd, ks
+    (rgb, 0..255; default 0), s (shininess, 0), tran (transmittance, 0), ri
+    (refractive index, 1), rf (reflectance, 0). Rows may also be given as
+    13 floats."""
+    rows = []
+    for m in materials:
+        if isinstance(m, dict):
+            row = [*m.get("ka", (0, 0, 0)), *m.get("kd", (0, 0, 0)), *m.get("ks", (0, 0, 0)), m.get("s", 0.0),
+                   m.get("tran", 0.0), m.get("ri", 1.0), m.get("rf", 0.0)]
+        else:
+            row = list(m)
+        if len(row) != 13:
+            raise ValueError("a material has 13 floats: ka rgb, kd rgb, ks rgb, s, tran, ri, rf")
+        rows.append(row)
+    return np.asarray(rows, np.float32).reshape(-1, 13)
+
+
 class Context:
     def __init__(self, device=0):
         h = C.c_void_p()
@@ -254,6 +274,20 @@ class Scene:
             torch.cuda.current_stream().synchronize()  # the library's stream reads it next
             self.ctx.check(_LIB.rt_scene_update(self.h, _ptr(a), RT_DEVICE_PTRS), "rt_scene_update")
             torch.cuda.synchronize()  # the copy has read a before a is released
+
+    def set_materials(self, materials, tri_material):
+        """Phong materials for rt_render: materials (m, 13) float32 rows of
+        rt_material (ka rgb, kd rgb, ks rgb, shininess, transmittance,
+        refractive index, reflectance), tri_material (n,) int32 indices, -1
+        for Lambert shading from the albedo. No materials (m = 0) removes
+        them."""
+        if not isinstance(materials, np.ndarray):
+            materials = material_table(materials)
+        m = np.ascontiguousarray(materials, np.float32).reshape(-1, 13)
+        k = np.ascontiguousarray(tri_material, np.int32).reshape(-1)
+        if m.shape[0] > 0 and k.shape[0] != self.n:
+            raise ValueError(f"rt_scene_set_materials: {k.shape[0]} indices, the scene has {self.n} triangles")
+        self.ctx.check(_LIB.rt_scene_set_materials(self.h, _ptr(m), m.shape[0], _ptr(k)), "rt_scene_set_materials")
 
     def normals(self):
         out = np.empty((self.n, 3), np.float32)
