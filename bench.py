@@ -388,6 +388,46 @@ def dynamic_section(ctx, w, accel, scene, frames, args):
             "refit": refit, "rebuild_lbvh": lbvh, "rebuild_sah": sah}
 
 
+def materials_section(ctx, frames, args):
+    """Frames of the terrain with phong materials (a mirror wall, a glass pane,
+    wet rock): primary rays, one shadow ray per hit, and mirror and refraction
+    rays traced while their power stays above 0.1, each with its shadow ray.
+    Rays per second count every ray traced; the wall time includes the
+    per-generation read-back of the ray counts."""
+    import torch
+
+    from raptor_amd import rt, workloads
+
+    w, mats, kinds, flags = workloads.material_terrain()
+    scene = rt.Scene(ctx, w.abc, w.albedo, flags)
+    scene.set_materials(mats, kinds)
+    accel = rt.Accel(scene, width=args.width, leaf_size=args.leaf_size, lds_stack=args.lds_stack,
+                     sah_bins=args.sah_bins)
+    rgb = torch.empty((w.camera.y_res, w.camera.x_res, 3), dtype=torch.float32, device="cuda:0")
+    accel.render_device(w.camera, w.lights, rgb)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    per = [accel.render_device(w.camera, w.lights, rgb) for _ in range(frames)]
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / frames
+    rays = per[-1]["rays"]
+    gpu_ms = sum(s["ms"][0] for s in per) / frames
+    out = {
+        "workload": "terrain1m with phong materials (4 terrain materials incl. wet rock rf 0.25, mirror wall rf 0.8, "
+                    "glass pane tran 0.6 ri 1.5), 1920x1080, 1 light, recursion to MIN_REFLECTIVE_POWER",
+        "frames": frames,
+        "rays_per_frame": {"primary": rays[0], "shadow": rays[1], "secondary": rays[2], "secondary_shadow": rays[3]},
+        "frame_ms": round(wall * 1e3, 3),
+        "frame_gpu_ms": round(gpu_ms, 3),
+        "closest_hit_ms": round(sum(s["ms"][1] for s in per) / frames, 3),
+        "any_hit_ms": round(sum(s["ms"][2] for s in per) / frames, 3),
+        "mrays_s": round(sum(rays) / wall / 1e6, 1),
+    }
+    accel.close()
+    scene.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -405,6 +445,7 @@ def main():
     ap.add_argument("--adjoint-reps", type=int, default=10, help="0 skips the differentiable workload")
     ap.add_argument("--invrender-steps", type=int, default=100, help="0 skips batched inverse rendering")
     ap.add_argument("--dynamic-frames", type=int, default=8, help="0 skips the moving-scene workload")
+    ap.add_argument("--material-frames", type=int, default=5, help="0 skips the phong-material workload")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each GPU would trace its own share "
@@ -506,6 +547,9 @@ def main():
     if args.dynamic_frames > 0:
         print("bench: moving scene", file=sys.stderr, flush=True)
         line["dynamic"] = dynamic_section(ctx, w, accel, scene, args.dynamic_frames, args)
+    if args.material_frames > 0:
+        print("bench: phong materials", file=sys.stderr, flush=True)
+        line["materials"] = materials_section(ctx, args.material_frames, args)
     if args.batched_scenes > 0:
         del rgb, accel, scene
         print("bench: batched workload", file=sys.stderr, flush=True)

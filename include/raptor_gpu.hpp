@@ -19,6 +19,7 @@
 #include <cstdint>
 #include <stdexcept>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "raptor_rt.h"
@@ -56,8 +57,8 @@ class gpu_bvh {
     template <class AlbedoFn>
     gpu_bvh(const context &ctx, const PrimitiveStore &prims, AlbedoFn albedo_of,
             const rt_build_params *params = nullptr)
-        : ctx_(ctx) {
-        const int64_t n = static_cast<int64_t>(prims.size());
+        : ctx_(ctx), n_(static_cast<int64_t>(prims.size())) {
+        const int64_t n = n_;
         const std::vector<float> abc = vertices(prims);
         std::vector<float> albedo(3 * n);
         std::vector<uint8_t> flags(n);
@@ -87,6 +88,33 @@ class gpu_bvh {
         const std::vector<float> abc = vertices(prims);
         ctx_.check(rt_scene_update(scene_, abc.data(), 0), "rt_scene_update");
         ctx_.check(rt_accel_refit(accel_), "rt_accel_refit");
+    }
+
+    // Phong materials for ray_tracer: material_of(i) returns the rt_material
+    // of triangle i (the state of its phong_shader,
+    // rt/materials/phong_shader.h:34-53, 67-73) or nullptr to keep the
+    // Lambert albedo. Triangles returning the same pointer share a table
+    // entry. A transparent material (transmittance > 0) needs a triangle whose
+    // is_transparent() is true, as material(tran > 0) makes it in the
+    // reference (rt/materials/material.h:50-66).
+    template <class MaterialOf>
+    void set_materials(MaterialOf material_of) {
+        const int64_t n = n_;
+        std::vector<rt_material> table;
+        std::vector<int32_t> index(static_cast<size_t>(n), -1);
+        std::unordered_map<const rt_material *, int32_t> seen;
+        for (int64_t i = 0; i < n; ++i) {
+            const rt_material *m = material_of(static_cast<int>(i));
+            if (!m) continue;
+            auto it = seen.find(m);
+            if (it == seen.end()) {
+                it = seen.emplace(m, static_cast<int32_t>(table.size())).first;
+                table.push_back(*m);
+            }
+            index[static_cast<size_t>(i)] = it->second;
+        }
+        ctx_.check(rt_scene_set_materials(scene_, table.data(), static_cast<int32_t>(table.size()), index.data()),
+                   "rt_scene_set_materials");
     }
 
     // Batched closest hit / any hit; pointers are host memory unless flags
@@ -120,6 +148,7 @@ class gpu_bvh {
     }
 
     rt_accel *get() const { return accel_; }
+    rt_scene *scene() const { return scene_; }
 
   private:
     static std::vector<float> vertices(const PrimitiveStore &prims) {
@@ -151,6 +180,7 @@ class gpu_bvh {
     }
 
     const context &ctx_;
+    int64_t n_ = 0;
     rt_scene *scene_ = nullptr;
     rt_accel *accel_ = nullptr;
 };

@@ -161,29 +161,43 @@ int32_t rt_accel_info_get(const rt_accel *accel, rt_accel_info *info);
 int32_t rt_scene_update(rt_scene *scene, const float *abc, uint32_t flags);
 int32_t rt_accel_refit(rt_accel *accel);
 
-/* ---- Phong materials. ----
- * One material: the state of a phong_shader
+/* ---- Materials. ----
+ * kind RT_MATERIAL_PHONG: the state of a phong_shader
  * (rt/materials/phong_shader.h:34-53, 67-73): ka, kd, ks colours on the
  * 0-255 scale, shininess s, transmittance tran, refractive index ri,
- * reflectance rf. */
+ * reflectance rf; chroma, diffuse, roughness and extinction are unused.
+ * kind RT_MATERIAL_COOK_TORRANCE: the state of a cook_torrance_cxy
+ * (rt/materials/cook_torrance_cxy.h:14-30, 45-54): chroma (x, y), diffuse
+ * reflectance rd (diffuse), surface roughness sr (roughness), imaginary
+ * refractive index ri_i (extinction), and reflectance, transmittance,
+ * refractive_index holding its specular reflectance rs, specular
+ * transmittance ts and real index ri_r; ka, kd, ks and shininess are
+ * unused. */
+#define RT_MATERIAL_PHONG 0
+#define RT_MATERIAL_COOK_TORRANCE 1
 typedef struct {
     float ka[3], kd[3], ks[3];
     float shininess, transmittance, refractive_index, reflectance;
+    int32_t kind;
+    float chroma[2];
+    float diffuse, roughness, extinction;
 } rt_material;
 
-/* Gives the scene's triangles phong materials for rt_render: tri_material[i]
+/* Gives the scene's triangles materials for rt_render: tri_material[i]
  * indexes materials (host arrays), -1 keeps triangle i's Lambert shading
- * phong_shader(255 * albedo, 1.0). rt_render then shades every hit as
- * phong_shader::shade does (rt/materials/phong_shader.cc:21-111): diffuse and
- * specular light per light, ambient, and the mirror (rf > 0) and refraction
- * (tran > 0) rays traced recursively while the ray power stays above
- * MIN_REFLECTIVE_POWER (common/common.h:30-32). The ranges the reference
- * asserts are errors here: colours in [0, 255], tran and rf in [0, 0.95], ri
- * in [1, 5] when tran > 0. A transparent material makes its triangles
- * transparent to shadow rays (material(tran > 0),
+ * phong_shader(255 * albedo, 1.0). rt_render then shades every hit as the
+ * material's shade does (rt/materials/phong_shader.cc:21-111,
+ * rt/materials/cook_torrance_cxy.cc:12-90): light per light, ambient
+ * (phong), and the mirror (reflectance > 0) and refraction
+ * (transmittance > 0) rays traced recursively while the ray power stays
+ * above MIN_REFLECTIVE_POWER (common/common.h:30-32). The ranges the
+ * reference asserts are errors here: phong colours in [0, 255], tran and rf
+ * in [0, 0.95], ri in [1, 5] when tran > 0; Cook-Torrance x, y, rd, rs, ts,
+ * ri_i in [0, 1], sr in [0, 0.5], ri_r in [0, 5]. A transparent material
+ * makes its triangles transparent to shadow rays (material(tran > 0),
  * rt/materials/material.h:50-66), so a triangle's flags bit1 must be set
- * exactly when its material has tran > 0. nmaterials 0 removes the
- * materials. Only rt_render reads them; the diffuse bounce and the
+ * exactly when its material's transmittance is > 0. nmaterials 0 removes
+ * the materials. Only rt_render reads them; the diffuse bounce and the
  * differentiable frames keep Lambert shading and reject a scene with
  * materials. */
 int32_t rt_scene_set_materials(rt_scene *scene, const rt_material *materials, int32_t nmaterials,

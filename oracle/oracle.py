@@ -41,18 +41,23 @@ class oracle_render_out(C.Structure):
 
 # Phong material fields, in the order the oracle (and include/raptor_rt.h's
 # rt_material) stores them: rt/materials/phong_shader.h:38-60.
-MATERIAL_FIELDS = ("ka", "kd", "ks", "s", "tran", "ri", "rf")
-
-
 def material_array(materials):
-    """materials: list of dicts with ka, kd, ks (rgb in [0, 255]) and s, tran,
-    ri, rf; returns an (n, 13) float32 array."""
+    """materials: list of dicts. Phong (kind 0, the default): ka, kd, ks (rgb
+    in [0, 255]), s, tran, ri, rf. Cook-Torrance (kind 1, cook_torrance_cxy):
+    x, y (chroma), rd, rs, ts, sr, ri_r, ri_i. Returns an (n, 19) float32
+    array in the oracle's row layout."""
     rows = []
     for m in materials:
-        rows.append(list(m.get("ka", (0.0, 0.0, 0.0))) + list(m.get("kd", (0.0, 0.0, 0.0))) +
-                    list(m.get("ks", (0.0, 0.0, 0.0))) +
-                    [m.get("s", 0.0), m.get("tran", 0.0), m.get("ri", 1.0), m.get("rf", 0.0)])
-    return np.asarray(rows, np.float32).reshape(-1, 13)
+        if m.get("kind", 0) == 1:
+            rows.append([0.0] * 10 + [m.get("ts", 0.0), m.get("ri_r", 1.0), m.get("rs", 0.0), 1.0,
+                                      m.get("x", 0.33), m.get("y", 0.33), m.get("rd", 0.5), m.get("sr", 0.0),
+                                      m.get("ri_i", 0.0)])
+        else:
+            rows.append(list(m.get("ka", (0.0, 0.0, 0.0))) + list(m.get("kd", (0.0, 0.0, 0.0))) +
+                        list(m.get("ks", (0.0, 0.0, 0.0))) +
+                        [m.get("s", 0.0), m.get("tran", 0.0), m.get("ri", 1.0), m.get("rf", 0.0)] +
+                        [0.0, 0.33, 0.33, 0.5, 0.0, 0.0])
+    return np.asarray(rows, np.float32).reshape(-1, 19)
 
 
 def build():
@@ -227,7 +232,7 @@ class OracleScene:
         out.sec_counters = _p(sec)
         if materials is not None:
             mats = material_array(materials) if not isinstance(materials, np.ndarray) else \
-                np.ascontiguousarray(materials, np.float32).reshape(-1, 13)
+                np.ascontiguousarray(materials, np.float32).reshape(-1, 19)
             tm = np.ascontiguousarray(tri_material, np.int32)
             p.materials, p.nmaterials, p.tri_material = _p(mats), mats.shape[0], _p(tm)
         alb = np.ascontiguousarray(albedo, np.float32)

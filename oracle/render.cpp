@@ -289,16 +289,84 @@ void render_pixel(const FrameCtx &f, int px, int py, LocalCounters *lc) {
 // double, so a power of exactly 0.1f (above 0.1) still recurses.
 constexpr double kMinReflectivePower = 0.1;
 
+constexpr int kMaterialFloats = 19;
+constexpr int kCookTorrance = 1;
+
 struct Phong {
     Colour ka, kd, ks;
     float s = 0.0f, tran = 0.0f, ri = 1.0f, rf = 0.0f;
+    int kind = 0;  // 0 phong_shader, 1 cook_torrance_cxy
+    float cx = 0.33f, cy = 0.33f, rd = 0.5f, sr = 0.0f, rii = 0.0f;
 };
+
+// XYZ -> RGB for the primaries and white point of rt/materials/material.h:17-45,
+// evaluated in float in the order written there.
+struct Cie {
+    float m[3][3];
+};
+
+Cie xyz_to_rgb_matrix() {
+    const float xr = 0.640f, yr = 0.330f, xg = 0.290f, yg = 0.600f, xb = 0.150f, yb = 0.060f;
+    const float xw = 0.3333f, yw = 0.3333f;
+    const float crd = ((1.0f / yw) * (xw * (yg - yb) - yw * (xg - xb) + xg * yb - xb * yg));
+    const float cgd = ((1.0f / yw) * (xw * (yb - yr) - yw * (xb - xr) - xr * yb + xb * yr));
+    const float cbd = ((1.0f / yw) * (xw * (yr - yg) - yw * (xr - xg) + xr * yg - xg * yr));
+    Cie c;
+    c.m[0][0] = (yg - yb - xb * yg + yb * xg) / crd;
+    c.m[0][1] = (xb - xg - xb * yg + xg * yb) / crd;
+    c.m[0][2] = (xg * yb - xb * yg) / crd;
+    c.m[1][0] = (yb - yr - yb * xr + yr * xb) / cgd;
+    c.m[1][1] = (xr - xb - xr * yb + xb * yr) / cgd;
+    c.m[1][2] = (xb * yr - xr * yb) / cgd;
+    c.m[2][0] = (yr - yg - yr * xg + yg * xr) / cbd;
+    c.m[2][1] = (xg - xr - xg * yr + xr * yg) / cbd;
+    c.m[2][2] = (xr * yg - xg * yr) / cbd;
+    return c;
+}
+
+const Cie kCie = xyz_to_rgb_matrix();
+
+// cxy_to_rgb (rt/materials/material.h:326-343): chroma (x, y) at luminance
+// lum, clamped to [0, 1] and scaled to 0-255.
+Colour cxy_to_rgb(float x, float y, float lum) {
+    float cie[3];
+    cie[0] = lum * x / y;
+    cie[1] = lum;
+    cie[2] = lum * (1.0f / y - 1.0f) - cie[0];
+    float out[3];
+    for (int k = 0; k < 3; ++k)
+        out[k] = (kCie.m[k][0] * cie[0]) + (kCie.m[k][1] * cie[1]) + (kCie.m[k][2] * cie[2]);
+    return Colour{std::max(0.0f, std::min(1.0f, out[0])) * 255.0f, std::max(0.0f, std::min(1.0f, out[1])) * 255.0f,
+                  std::max(0.0f, std::min(1.0f, out[2])) * 255.0f};
+}
+
+// cook_torrance_cxy::shade for one light (rt/materials/cook_torrance_cxy.cc:33-76)
+// with schlick_fresnell, geometric_attenuation_factor and
+// gaussian_facet_distribution (rt/materials/material.h:146-215).
+void cook_torrance_term(const Phong &m, const V3 &n, const V3 &dir, float nv, const V3 &ldir, float shade,
+                        float magn, const Colour &li, Colour *c) {
+    const V3 half((ldir.x + dir.x) * 0.5f, (ldir.y + dir.y) * 0.5f, (ldir.z + dir.z) * 0.5f);
+    const float nh = dot(n, half);
+    const float vh = dot(half, dir);
+    const float nl = shade;
+    const float f = (m.sr + (1.0f - m.sr) * std::pow(1.0f - nv, 5.0f));
+    const float g0 = (2.0f * nh * nv) / vh;
+    const float g1 = (2.0f * nh * nl) / vh;
+    const float g = std::min(1.0f, std::max(0.0f, std::min(g0, g1)));
+    const float a_div_m = nh / m.sr;
+    const float ro = 5.0f * std::exp(-(a_div_m * a_div_m));
+    const float s = m.rf * ((g * f * ro) / (nl * nv));
+    const Colour rgb = cxy_to_rgb(m.cx, m.cy, (shade * (m.rd + s) * magn));
+    c->r += rgb.r * li.r;
+    c->g += rgb.g * li.g;
+    c->b += rgb.b * li.b;
+}
 
 Phong material_of(const FrameCtx &f, int prim) {
     Phong m;
     const int32_t mi = f.p.tri_material ? f.p.tri_material[prim] : -1;
     if (f.p.materials && mi >= 0 && mi < f.p.nmaterials) {
-        const float *q = f.p.materials + 13 * static_cast<int64_t>(mi);
+        const float *q = f.p.materials + kMaterialFloats * static_cast<int64_t>(mi);
         m.ka = Colour{q[0], q[1], q[2]};
         m.kd = Colour{q[3], q[4], q[5]};
         m.ks = Colour{q[6], q[7], q[8]};
@@ -306,6 +374,12 @@ Phong material_of(const FrameCtx &f, int prim) {
         m.tran = q[10];
         m.ri = q[11];
         m.rf = q[12];
+        m.kind = static_cast<int>(q[13]);
+        m.cx = q[14];
+        m.cy = q[15];
+        m.rd = q[16];
+        m.sr = q[17];
+        m.rii = q[18];
     } else {
         m.kd = kd_of(f, prim);  // phong_shader(255 * albedo, 1): ks = 0 * rgb, ka = 0
     }
@@ -357,6 +431,7 @@ Colour trace_phong(const FrameCtx &f, Ray &r, bool primary, LocalCounters *lc, P
     const SurfaceHit sh = surface_at(tri, &r, h.d);
     const Phong m = material_of(f, prim);
     const V3 &n = sh.shade_norm;
+    const float nv = dot(n, r.dir);  // cook_torrance_cxy: one dot product for every light
     for (size_t l = 0; l < f.lights.size(); ++l) {
         Ray illum = shadow_ray(sh, f.lights[l].c);
         const bool occluded = found_nearer_object(*f.s, illum, illum.length, &lc->trav);
@@ -369,6 +444,13 @@ Colour trace_phong(const FrameCtx &f, Ray &r, bool primary, LocalCounters *lc, P
         if (vis_out) vis_out[l] = occluded ? 0 : 1;
         const float shade = dot(illum.dir, n);
         if (shade < 0.0f) continue;
+        if (m.kind == kCookTorrance) {
+            const float dist = illum.length * f.lights[l].d;
+            const float d_scale = (dist * dist) + 1.0f;
+            const Colour li{f.lights[l].rgb.r / d_scale, f.lights[l].rgb.g / d_scale, f.lights[l].rgb.b / d_scale};
+            cook_torrance_term(m, n, r.dir, nv, illum.dir, shade, illum.magn, li, &c);
+            continue;
+        }
         // Reflection of the light ray about the normal.
         const float shade_x2 = 2.0f * shade;
         const V3 ref_dir(illum.dir.x - n.x * shade_x2, illum.dir.y - n.y * shade_x2, illum.dir.z - n.z * shade_x2);
@@ -388,9 +470,11 @@ Colour trace_phong(const FrameCtx &f, Ray &r, bool primary, LocalCounters *lc, P
         c.g += li.g * (sc.g * illum.magn);
         c.b += li.b * (sc.b * illum.magn);
     }
-    c.r += m.ka.r;
-    c.g += m.ka.g;
-    c.b += m.ka.b;
+    if (m.kind != kCookTorrance) {  // cook_torrance_cxy has no ambient term
+        c.r += m.ka.r;
+        c.g += m.ka.g;
+        c.b += m.ka.b;
+    }
     if (m.rf > 0.0f) {
         // ray::reflect: dir - n * 2 (dir . n) from a point moved off the front.
         const float power = r.magn * m.rf;

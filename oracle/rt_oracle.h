@@ -42,12 +42,17 @@ typedef struct {
     int32_t pixel_stride; /* trace pixels p with p % stride == 0 (packet frame: 8x8 tiles) */
     int32_t nthreads;
     int32_t packet;       /* 1: the SSE packet frame (spp 1, bounce <= 1) */
-    /* Phong materials (rt/materials/phong_shader.h:38-60, coloured form):
-     * nmaterials x 13 floats ka rgb, kd rgb, ks rgb, s, tran, ri, rf, and one
-     * index per triangle (-1: phong_shader(255 * albedo, 1), the Lambert
-     * default). NULL materials: every triangle takes the default. With
-     * materials the frame recurses into mirror and refraction rays
-     * (rt/ray.cc:118-258) and needs bounce 0, packet 0 and no records. */
+    /* Materials: nmaterials x 19 floats ka rgb, kd rgb, ks rgb, s, tran,
+     * ri, rf, kind, cx, cy, rd, sr, ri_i, and one index per triangle (-1:
+     * phong_shader(255 * albedo, 1), the Lambert default). kind 0 is
+     * phong_shader (rt/materials/phong_shader.h:34-53, coloured form; the
+     * last five floats unused); kind 1 is cook_torrance_cxy
+     * (rt/materials/cook_torrance_cxy.h:14-30) with chroma (cx, cy), diffuse
+     * reflectance rd, roughness sr, imaginary index ri_i, and rf, tran, ri
+     * holding its rs, ts, ri_r (ka, kd, ks, s unused). NULL materials: every
+     * triangle takes the default. With materials the frame recurses into
+     * mirror and refraction rays (rt/ray.cc:118-258) and needs bounce 0,
+     * packet 0 and no records. */
     const float *materials;
     int32_t nmaterials;
     const int32_t *tri_material;

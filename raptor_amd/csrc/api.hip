@@ -269,6 +269,8 @@ int32_t rt_scene_update(rt_scene *scene, const float *abc, uint32_t flags) {
     return RT_OK;
 }
 
+static_assert(sizeof(rt_material) == 76, "rt_material is 19 four-byte fields");
+
 int32_t rt_scene_set_materials(rt_scene *scene, const rt_material *materials, int32_t nmaterials,
                                const int32_t *tri_material) {
     if (!scene) return RT_ERR_INVALID;
@@ -289,23 +291,38 @@ int32_t rt_scene_set_materials(rt_scene *scene, const rt_material *materials, in
         drop();
         return RT_OK;
     }
-    // The ranges phong_shader asserts (rt/materials/phong_shader.h:38-51).
-    std::vector<float4> table(4 * static_cast<size_t>(nmaterials));
+    // The ranges the shaders assert (rt/materials/phong_shader.h:38-51,
+    // rt/materials/cook_torrance_cxy.h:20-29).
+    std::vector<float4> table(rt::kMaterialFloat4s * static_cast<size_t>(nmaterials));
+    auto in = [](float v, float lo, float hi) { return v >= lo && v <= hi; };
     for (int32_t m = 0; m < nmaterials; ++m) {
         const rt_material &q = materials[m];
-        bool ok = q.transmittance >= 0.0f && q.transmittance <= 0.95f && q.reflectance >= 0.0f &&
-                  q.reflectance <= 0.95f &&
-                  (q.transmittance == 0.0f || (q.refractive_index >= 1.0f && q.refractive_index <= 5.0f));
-        for (int k = 0; k < 3; ++k)
-            for (const float v : {q.ka[k], q.kd[k], q.ks[k]}) ok = ok && v >= 0.0f && v <= 255.0f;
+        bool ok = true;
+        std::string range;
+        if (q.kind == RT_MATERIAL_PHONG) {
+            ok = in(q.transmittance, 0.0f, 0.95f) && in(q.reflectance, 0.0f, 0.95f) &&
+                 (q.transmittance == 0.0f || in(q.refractive_index, 1.0f, 5.0f));
+            for (int k = 0; k < 3; ++k)
+                for (const float v : {q.ka[k], q.kd[k], q.ks[k]}) ok = ok && in(v, 0.0f, 255.0f);
+            range = "colours 0..255, tran and rf 0..0.95, ri 1..5 when tran > 0";
+        } else if (q.kind == RT_MATERIAL_COOK_TORRANCE) {
+            ok = in(q.chroma[0], 0.0f, 1.0f) && in(q.chroma[1], 0.0f, 1.0f) && in(q.diffuse, 0.0f, 1.0f) &&
+                 in(q.reflectance, 0.0f, 1.0f) && in(q.transmittance, 0.0f, 1.0f) && in(q.roughness, 0.0f, 0.5f) &&
+                 in(q.refractive_index, 0.0f, 5.0f) && in(q.extinction, 0.0f, 1.0f);
+            range = "x, y, rd, rs, ts, ri_i 0..1, sr 0..0.5, ri_r 0..5";
+        } else {
+            ok = false;
+            range = "unknown kind " + std::to_string(q.kind);
+        }
         if (!ok)
             return fail(ctx, RT_ERR_INVALID,
-                        "rt_scene_set_materials: material " + std::to_string(m) +
-                            " out of range (colours 0..255, tran and rf 0..0.95, ri 1..5 when tran > 0)");
-        table[4 * m + 0] = make_float4(q.ka[0], q.ka[1], q.ka[2], q.shininess);
-        table[4 * m + 1] = make_float4(q.kd[0], q.kd[1], q.kd[2], q.transmittance);
-        table[4 * m + 2] = make_float4(q.ks[0], q.ks[1], q.ks[2], q.refractive_index);
-        table[4 * m + 3] = make_float4(q.reflectance, 0.0f, 0.0f, 0.0f);
+                        "rt_scene_set_materials: material " + std::to_string(m) + " out of range (" + range + ")");
+        float4 *row = &table[rt::kMaterialFloat4s * static_cast<size_t>(m)];
+        row[0] = make_float4(q.ka[0], q.ka[1], q.ka[2], q.shininess);
+        row[1] = make_float4(q.kd[0], q.kd[1], q.kd[2], q.transmittance);
+        row[2] = make_float4(q.ks[0], q.ks[1], q.ks[2], q.refractive_index);
+        row[3] = make_float4(q.reflectance, static_cast<float>(q.kind), q.chroma[0], q.chroma[1]);
+        row[4] = make_float4(q.diffuse, q.roughness, q.extinction, 0.0f);
     }
     // A triangle is transparent to shadow rays exactly when its material is
     // (material(tran > 0), rt/materials/material.h:50-66).

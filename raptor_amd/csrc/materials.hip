@@ -1,6 +1,7 @@
-// Frames with phong materials: the reference's recursive shading of one hit
+// Frames with materials: the reference's recursive shading of one hit
 // (rt/raytracer.cc:20-87) with phong_shader::shade
-// (rt/materials/phong_shader.cc:21-111) and the mirror and refraction rays of
+// (rt/materials/phong_shader.cc:21-111) or cook_torrance_cxy::shade
+// (rt/materials/cook_torrance_cxy.cc:12-90) and the mirror and refraction rays of
 // ray::reflect / ray::refract without DIFFUSE_REFLECTIONS (rt/ray.cc:118-258),
 // run as a wavefront of ray generations:
 //
@@ -14,7 +15,9 @@
 //   k_mat_emit       per hit: surface record, one shadow ray per light, the
 //                    children into the next generation's queue
 //   any hit          shadow rays
-//   k_mat_local      local colour (lights, specular, ambient) of every hit
+//   k_mat_local      local colour of every hit (phong: diffuse, specular,
+//                    ambient; Cook-Torrance: diffuse and microfacet specular
+//                    through the chroma-to-RGB conversion)
 //
 // until a generation has no children. Then k_mat_gather walks the
 // generations backwards and folds each hit's children into it in the
@@ -38,11 +41,17 @@ namespace {
 // in double (a power of exactly 0.1f lies above it and still recurses).
 constexpr double kMinReflectivePower = 0.1;
 
-// Four float4 per material: {ka rgb, shininess}, {kd rgb, transmittance},
-// {ks rgb, refractive index}, {reflectance, 0, 0, 0}.
+constexpr int kCookTorrance = 1;
+
+// A material row (kMaterialFloat4s float4): {ka rgb, shininess},
+// {kd rgb, transmittance}, {ks rgb, refractive index},
+// {reflectance, kind, chroma x, chroma y}, {diffuse, roughness, extinction, 0}.
+// Cook-Torrance keeps rs, ts, ri_r where phong keeps rf, tran, ri.
 struct Mat {
     float4 ka_s, kd_tran, ks_ri;
     float rf;
+    int kind;
+    float cx, cy, rd, sr;
 };
 
 // The triangle's material: its table entry, or phong_shader(255 * albedo, 1)
@@ -56,14 +65,83 @@ __device__ __forceinline__ Mat mat_of(const float4 *mats, const int32_t *tri_mat
         m.kd_tran = make_float4(k.x, k.y, k.z, 0.0f);
         m.ks_ri = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
         m.rf = 0.0f;
+        m.kind = 0;
+        m.cx = m.cy = m.rd = m.sr = 0.0f;
     } else {
-        const float4 *q = mats + 4 * static_cast<int64_t>(mi);
+        const float4 *q = mats + kMaterialFloat4s * static_cast<int64_t>(mi);
         m.ka_s = q[0];
         m.kd_tran = q[1];
         m.ks_ri = q[2];
-        m.rf = q[3].x;
+        const float4 q3 = q[3], q4 = q[4];
+        m.rf = q3.x;
+        m.kind = static_cast<int>(q3.y);
+        m.cx = q3.z;
+        m.cy = q3.w;
+        m.rd = q4.x;
+        m.sr = q4.y;
     }
     return m;
+}
+
+// XYZ -> RGB for the primaries and white point of
+// rt/materials/material.h:17-45, in float and in the order written there.
+struct CieMatrix {
+    float m[3][3];
+};
+
+__device__ __forceinline__ CieMatrix cie_matrix() {
+    const float xr = 0.640f, yr = 0.330f, xg = 0.290f, yg = 0.600f, xb = 0.150f, yb = 0.060f;
+    const float xw = 0.3333f, yw = 0.3333f;
+    const float crd = ((1.0f / yw) * (xw * (yg - yb) - yw * (xg - xb) + xg * yb - xb * yg));
+    const float cgd = ((1.0f / yw) * (xw * (yb - yr) - yw * (xb - xr) - xr * yb + xb * yr));
+    const float cbd = ((1.0f / yw) * (xw * (yr - yg) - yw * (xr - xg) + xr * yg - xg * yr));
+    CieMatrix c;
+    c.m[0][0] = (yg - yb - xb * yg + yb * xg) / crd;
+    c.m[0][1] = (xb - xg - xb * yg + xg * yb) / crd;
+    c.m[0][2] = (xg * yb - xb * yg) / crd;
+    c.m[1][0] = (yb - yr - yb * xr + yr * xb) / cgd;
+    c.m[1][1] = (xr - xb - xr * yb + xb * yr) / cgd;
+    c.m[1][2] = (xb * yr - xr * yb) / cgd;
+    c.m[2][0] = (yr - yg - yr * xg + yg * xr) / cbd;
+    c.m[2][1] = (xg - xr - xg * yr + xr * yg) / cbd;
+    c.m[2][2] = (xr * yg - xg * yr) / cbd;
+    return c;
+}
+
+// cxy_to_rgb (rt/materials/material.h:326-343): chroma (x, y) at luminance
+// lum, clamped to [0, 1] and scaled to 0-255.
+__device__ __forceinline__ Rgb cxy_to_rgb(const CieMatrix &cm, float x, float y, float lum) {
+    const float c0 = lum * x / y;
+    const float c1 = lum;
+    const float c2 = lum * (1.0f / y - 1.0f) - c0;
+    float o[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) o[k] = (cm.m[k][0] * c0) + (cm.m[k][1] * c1) + (cm.m[k][2] * c2);
+    return Rgb{fmaxf(0.0f, fminf(1.0f, o[0])) * 255.0f, fmaxf(0.0f, fminf(1.0f, o[1])) * 255.0f,
+               fmaxf(0.0f, fminf(1.0f, o[2])) * 255.0f};
+}
+
+// cook_torrance_cxy::shade for one light (rt/materials/cook_torrance_cxy.cc:33-76)
+// with schlick_fresnell, geometric_attenuation_factor and
+// gaussian_facet_distribution (rt/materials/material.h:146-215); adds
+// rgb x light intensity to c.
+__device__ __forceinline__ void cook_torrance_term(const Mat &m, const CieMatrix &cm, rtd::V3 n, rtd::V3 d, float nv,
+                                                   rtd::V3 l, float shade, float magn, float4 li, Rgb &c) {
+    const rtd::V3 half = rtd::mk((l.x + d.x) * 0.5f, (l.y + d.y) * 0.5f, (l.z + d.z) * 0.5f);
+    const float nh = rtd::dot(n, half);
+    const float vh = rtd::dot(half, d);
+    const float nl = shade;
+    const float f = (m.sr + (1.0f - m.sr) * powf(1.0f - nv, 5.0f));
+    const float g0 = (2.0f * nh * nv) / vh;
+    const float g1 = (2.0f * nh * nl) / vh;
+    const float g = fminf(1.0f, fmaxf(0.0f, fminf(g0, g1)));
+    const float a_div_m = nh / m.sr;
+    const float ro = 5.0f * expf(-(a_div_m * a_div_m));
+    const float s = m.rf * ((g * f * ro) / (nl * nv));
+    const Rgb rgb = cxy_to_rgb(cm, m.cx, m.cy, (shade * (m.rd + s) * magn));
+    c.r += rgb.r * li.x;
+    c.g += rgb.g * li.y;
+    c.b += rgb.b * li.z;
 }
 
 // ray::refract (rt/ray.cc:200-227): n faces the ray; the index is inverted
@@ -218,15 +296,19 @@ struct LocalArgs {
     float4 *colour;  // per hit: local colour
 };
 
-// phong_shader::shade without the secondary rays
-// (rt/materials/phong_shader.cc:21-64) with the light intensity of
-// rt/light.h:44-59: diffuse and specular per light, then the ambient term.
+// The material's shade without the secondary rays, with the light intensity
+// of rt/light.h:44-59: phong_shader (rt/materials/phong_shader.cc:21-64),
+// diffuse and specular per light, then the ambient term; or
+// cook_torrance_cxy (rt/materials/cook_torrance_cxy.cc:33-76) per light.
 __global__ void __launch_bounds__(256) k_mat_local(LocalArgs a) {
     const int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (q >= a.nhits) return;
     const float4 nn = a.rec_n[q];
     const rtd::V3 n = rtd::f4v(nn), d = rtd::f4v(a.rec_d[q]);
     const Mat m = mat_of(a.mats, a.tri_mat, a.kd, __float_as_int(nn.w));
+    const bool ct = m.kind == kCookTorrance;
+    const CieMatrix cm = cie_matrix();
+    const float nv = rtd::dot(n, d);  // cook_torrance_cxy: one dot product for every light
     Rgb c{0.0f, 0.0f, 0.0f};
     for (int l = 0; l < a.lights.n; ++l) {
         const int64_t k = q * a.lights.n + l;
@@ -235,6 +317,14 @@ __global__ void __launch_bounds__(256) k_mat_local(LocalArgs a) {
         const float magn = (a.occ[k] ? 0.0f : 1.0f) / 1.0f;
         const float shade = rtd::dot(dn, n);
         if (shade < 0.0f) continue;
+        if (ct) {
+            const float dist = sd.w * a.lights.pos_d[l].w;
+            const float d_scale = (dist * dist) + 1.0f;
+            const float4 rgb = a.lights.rgb[l];
+            const float4 li = make_float4(rgb.x / d_scale, rgb.y / d_scale, rgb.z / d_scale, 0.0f);
+            cook_torrance_term(m, cm, n, d, nv, dn, shade, magn, li, c);
+            continue;
+        }
         // The light ray reflected about the normal.
         const float shade_x2 = 2.0f * shade;
         const rtd::V3 ref_dir = rtd::mk(dn.x - n.x * shade_x2, dn.y - n.y * shade_x2, dn.z - n.z * shade_x2);
@@ -253,9 +343,11 @@ __global__ void __launch_bounds__(256) k_mat_local(LocalArgs a) {
         c.g = c.g + ((rgb.y / d_scale) * (scg * magn));
         c.b = c.b + ((rgb.z / d_scale) * (scb * magn));
     }
-    c.r += m.ka_s.x;
-    c.g += m.ka_s.y;
-    c.b += m.ka_s.z;
+    if (!ct) {  // cook_torrance_cxy has no ambient term
+        c.r += m.ka_s.x;
+        c.g += m.ka_s.y;
+        c.b += m.ka_s.z;
+    }
     a.colour[q] = make_float4(c.r, c.g, c.b, 0.0f);
 }
 

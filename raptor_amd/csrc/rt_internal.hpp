@@ -81,6 +81,13 @@ struct rt_ctx {
     std::atomic<int> refs{1};  // the caller's handle + one per scene
 };
 
+namespace rt {
+// Device material row: {ka rgb, shininess}, {kd rgb, transmittance},
+// {ks rgb, refractive index}, {reflectance, kind, chroma x, chroma y},
+// {diffuse, roughness, extinction, 0}.
+constexpr int kMaterialFloat4s = 5;
+}  // namespace rt
+
 struct rt_scene {
     rt_ctx *ctx = nullptr;
     int64_t n = 0;
@@ -89,9 +96,9 @@ struct rt_scene {
     float4 *d_c = nullptr;     // vertex c (xyz) for the plane re-projection
     float4 *d_kd = nullptr;    // kd = (255 * albedo) * 1.0
     uint8_t *d_flags = nullptr;
-    // Phong materials (rt_scene_set_materials): 4 float4 per material, the
-    // material of each triangle (-1: Lambert from the albedo); both null when
-    // none are set.
+    // Materials (rt_scene_set_materials): kMaterialFloat4s float4 per
+    // material, the material of each triangle (-1: Lambert from the albedo);
+    // both null when none are set.
     float4 *d_mat = nullptr;
     int32_t *d_tri_mat = nullptr;
     int32_t nmat = 0;

@@ -191,22 +191,46 @@ def gen_terrain(grid, amplitude=0.15, zoom=0.02, persistence=0.5, octaves=8, noi
 
 # ---------------------------------------------------------------- handles
 
+MATERIAL_PHONG, MATERIAL_COOK_TORRANCE = 0, 1
+# rt_material (include/raptor_rt.h), 76 bytes.
+MATERIAL_DTYPE = np.dtype([("ka", "<f4", 3), ("kd", "<f4", 3), ("ks", "<f4", 3), ("shininess", "<f4"),
+                           ("transmittance", "<f4"), ("refractive_index", "<f4"), ("reflectance", "<f4"),
+                           ("kind", "<i4"), ("chroma", "<f4", 2), ("diffuse", "<f4"), ("roughness", "<f4"),
+                           ("extinction", "<f4")])
+
+
 def material_table(materials):
-    """Rows of rt_material from dicts with the phong_shader fields: ka, kd, ks
+    """An rt_material array from dicts. Phong (the default kind): ka, kd, ks
     (rgb, 0..255; default 0), s (shininess, 0), tran (transmittance, 0), ri
-    (refractive index, 1), rf (reflectance, 0). Rows may also be given as
-    13 floats."""
-    rows = []
-    for m in materials:
-        if isinstance(m, dict):
-            row = [*m.get("ka", (0, 0, 0)), *m.get("kd", (0, 0, 0)), *m.get("ks", (0, 0, 0)), m.get("s", 0.0),
-                   m.get("tran", 0.0), m.get("ri", 1.0), m.get("rf", 0.0)]
+    (refractive index, 1), rf (reflectance, 0). Cook-Torrance (kind 1,
+    cook_torrance_cxy): x, y (chroma, 0.33), rd (0.5), rs, ts, sr (0), ri_r
+    (1), ri_i (0). A row of 13 floats (ka, kd, ks, s, tran, ri, rf) is a
+    phong material."""
+    out = np.zeros(len(materials), MATERIAL_DTYPE)
+    for i, m in enumerate(materials):
+        r = out[i]
+        if not isinstance(m, dict):
+            v = [float(x) for x in m]
+            if len(v) != 13:
+                raise ValueError("a phong row has 13 floats: ka rgb, kd rgb, ks rgb, s, tran, ri, rf")
+            m = dict(ka=v[0:3], kd=v[3:6], ks=v[6:9], s=v[9], tran=v[10], ri=v[11], rf=v[12])
+        if m.get("kind", MATERIAL_PHONG) == MATERIAL_COOK_TORRANCE:
+            r["kind"] = MATERIAL_COOK_TORRANCE
+            r["chroma"] = (m.get("x", 0.33), m.get("y", 0.33))
+            r["diffuse"] = m.get("rd", 0.5)
+            r["reflectance"] = m.get("rs", 0.0)
+            r["transmittance"] = m.get("ts", 0.0)
+            r["roughness"] = m.get("sr", 0.0)
+            r["refractive_index"] = m.get("ri_r", 1.0)
+            r["extinction"] = m.get("ri_i", 0.0)
         else:
-            row = list(m)
-        if len(row) != 13:
-            raise ValueError("a material has 13 floats: ka rgb, kd rgb, ks rgb, s, tran, ri, rf")
-        rows.append(row)
-    return np.asarray(rows, np.float32).reshape(-1, 13)
+            r["kind"] = MATERIAL_PHONG
+            r["ka"], r["kd"], r["ks"] = m.get("ka", (0, 0, 0)), m.get("kd", (0, 0, 0)), m.get("ks", (0, 0, 0))
+            r["shininess"] = m.get("s", 0.0)
+            r["transmittance"] = m.get("tran", 0.0)
+            r["refractive_index"] = m.get("ri", 1.0)
+            r["reflectance"] = m.get("rf", 0.0)
+    return out
 
 
 class Context:
@@ -276,14 +300,13 @@ class Scene:
             torch.cuda.synchronize()  # the copy has read a before a is released
 
     def set_materials(self, materials, tri_material):
-        """Phong materials for rt_render: materials (m, 13) float32 rows of
-        rt_material (ka rgb, kd rgb, ks rgb, shininess, transmittance,
-        refractive index, reflectance), tri_material (n,) int32 indices, -1
-        for Lambert shading from the albedo. No materials (m = 0) removes
-        them."""
-        if not isinstance(materials, np.ndarray):
-            materials = material_table(materials)
-        m = np.ascontiguousarray(materials, np.float32).reshape(-1, 13)
+        """Materials for rt_render: materials is an array of MATERIAL_DTYPE
+        or anything material_table takes (dicts, phong rows of 13 floats);
+        tri_material (n,) int32 indices, -1 for Lambert shading from the
+        albedo. No materials removes them."""
+        if not (isinstance(materials, np.ndarray) and materials.dtype == MATERIAL_DTYPE):
+            materials = material_table(list(materials))
+        m = np.ascontiguousarray(materials)
         k = np.ascontiguousarray(tri_material, np.int32).reshape(-1)
         if m.shape[0] > 0 and k.shape[0] != self.n:
             raise ValueError(f"rt_scene_set_materials: {k.shape[0]} indices, the scene has {self.n} triangles")

@@ -181,6 +181,36 @@ def material_scene(n_objects=400, res=96, seed=5, mirror=None, glass=None, mirro
     return w, materials, np.asarray(kinds, np.int32), np.asarray(flags, np.uint8)
 
 
+def material_terrain(grid=708, x_res=1920, y_res=1080):
+    """The terrain of terrain1m with phong materials: each grid cell takes one
+    of four materials (seeded), one of them wet rock (rf 0.25, shiny); a
+    mirror wall (rf 0.8) stands behind the far edge at z = 1.02 and a glass
+    pane (tran 0.6, ri 1.5) covers the left of the view at z = -0.1. One
+    light. Returns (Workload, materials, tri_material, flags)."""
+    w = terrain(grid, x_res, y_res, bounce=False)
+    n = w.abc.shape[0]
+    mirror = quad((-0.3, -0.2, 1.02), (1.3, -0.2, 1.02), (1.3, 1.2, 1.02), (-0.3, 1.2, 1.02))
+    glass = quad((-0.2, -0.3, -0.1), (0.45, -0.3, -0.1), (0.45, 0.9, -0.1), (-0.2, 0.9, -0.1))
+    abc = np.concatenate([w.abc, mirror, glass]).astype(np.float32)
+    albedo = np.concatenate([w.albedo, np.full((4, 3), 0.5, np.float32)])
+    materials = [
+        dict(kd=(150.0, 130.0, 100.0), ks=(40.0, 40.0, 40.0), s=8.0),
+        dict(kd=(90.0, 140.0, 70.0), ka=(3.0, 5.0, 2.0)),
+        dict(kd=(120.0, 120.0, 125.0), ks=(120.0, 120.0, 120.0), s=32.0, rf=0.25),
+        dict(kd=(170.0, 150.0, 120.0), ks=(20.0, 20.0, 20.0), s=4.0),
+        dict(rf=0.8, ks=(30.0, 30.0, 30.0), s=64.0),
+        dict(tran=0.6, ri=1.5, ks=(60.0, 60.0, 60.0), s=48.0),
+    ]
+    cell = np.arange(n, dtype=np.uint64) // np.uint64(2)
+    kinds = ((cell * np.uint64(2654435761)) >> np.uint64(13)) % np.uint64(4)
+    kinds = np.concatenate([kinds.astype(np.int32), [4, 4, 5, 5]]).astype(np.int32)
+    flags = np.zeros(abc.shape[0], np.uint8)
+    flags[-2:] = 2  # the glass is transparent to shadow rays
+    out = Workload("material_terrain", abc, albedo, w.camera, w.lights, bounce=False,
+                   meta=dict(triangles=int(abc.shape[0]), grid=grid, resolution=[x_res, y_res]))
+    return out, materials, kinds, flags
+
+
 def terrain1m(x_res=1920, y_res=1080):
     return terrain(708, x_res, y_res)
 

@@ -112,9 +112,33 @@ int main() {
             ++moved_bad;
         if (c1.image[p].r != c.image[p].r) ++changed;
     }
-    const bool ok = bad == 0 && id_bad == 0 && lit > 100 && moved_bad == 0 && changed > 100;
+    // Materials through the shim: every other triangle a shiny mirror,
+    // against the same table set directly on a second scene.
+    const rt_material shiny{{2.0f, 2.0f, 2.0f}, {120.0f, 90.0f, 60.0f}, {80.0f, 80.0f, 80.0f}, 16.0f, 0.0f, 1.0f, 0.5f};
+    accel.set_materials([&](int i) { return (i % 2) ? &shiny : nullptr; });
+    camera c3{64, std::vector<colour>(64 * 64, colour(-1, -1, -1))};
+    raptor_gpu::ray_tracer(accel, lights, cam, c3);
+    std::vector<int32_t> idx(n);
+    for (int64_t i = 0; i < n; ++i) idx[i] = (i % 2) ? 0 : -1;
+    if (rt_scene_set_materials(fresh.scene(), &shiny, 1, idx.data()) != RT_OK) {
+        std::printf("rt_scene_set_materials failed\n");
+        return 1;
+    }
+    std::vector<float> ref3(3 * 64 * 64);
+    rt_stats st{};
+    if (rt_render(fresh.get(), &cam, lights.data(), 1, nullptr, ref3.data(), nullptr, nullptr, &st) != RT_OK) {
+        std::printf("rt_render with materials failed\n");
+        return 1;
+    }
+    int mat_bad = 0;
+    for (int p = 0; p < 64 * 64; ++p)
+        if (c3.image[p].r != ref3[3 * p] || c3.image[p].g != ref3[3 * p + 1] || c3.image[p].b != ref3[3 * p + 2])
+            ++mat_bad;
+    const bool ok = bad == 0 && id_bad == 0 && lit > 100 && moved_bad == 0 && changed > 100 && mat_bad == 0 &&
+                    st.rays[2] > 0;
     std::printf("%s: %d pixel mismatches, %d lit pixels, %d ray mismatches, %d refit/rebuild mismatches, "
-                "%d pixels changed by the move\n",
-                ok ? "shim ok" : "shim FAILED", bad, lit, id_bad, moved_bad, changed);
+                "%d pixels changed by the move, %d material mismatches, %llu mirror rays\n",
+                ok ? "shim ok" : "shim FAILED", bad, lit, id_bad, moved_bad, changed, mat_bad,
+                static_cast<unsigned long long>(st.rays[2]));
     return ok ? 0 : 1;
 }

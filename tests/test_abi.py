@@ -87,3 +87,15 @@ def test_bad_arguments_are_rejected_without_gpu():
     assert rt.lib().rt_gen_terrain(1, 0.1, 0.1, 0.5, 2, 1, 1, 0.0, 1.0, None, None) == -1
     assert rt.lib().rt_gen_random_triangles(1, -5, 0.01, 0.0, 1.0, None, None) == -1
     assert rt.lib().rt_ctx_create(0, None) == rt.RT_ERR_INVALID
+
+
+def test_material_row_matches_the_c_struct():
+    """rt_material is 19 four-byte fields (include/raptor_rt.h); the Python
+    rows must have the same layout."""
+    from raptor_amd import rt
+
+    assert rt.MATERIAL_DTYPE.itemsize == 76
+    assert rt.MATERIAL_DTYPE.fields["kind"][1] == 52
+    t = rt.material_table([dict(kd=(1, 2, 3), rf=0.5), dict(kind=1, x=0.2, rs=0.4, ts=0.3, ri_r=1.2)])
+    assert t["reflectance"][0] == np.float32(0.5) and t["kind"][1] == 1
+    assert t["transmittance"][1] == np.float32(0.3) and t["refractive_index"][1] == np.float32(1.2)

@@ -164,3 +164,51 @@ def test_material_errors(gpu_ctx, rt):
     with pytest.raises(rt.RTError, match="bounce"):
         accel.render(w.camera, w.lights, bounce=True)
     assert accel.render(w.camera, w.lights)["stats"]["rays"][2] > 0
+
+
+def test_material_terrain_frame(gpu_ctx, rt, O):
+    """The 1M-triangle terrain with a mirror wall, a glass pane and wet rock
+    (the benchmarked material workload) at 480 x 270."""
+    from raptor_amd import workloads
+
+    w, m, k, f = workloads.material_terrain(x_res=480, y_res=270)
+    out, ref = _both(rt, O, gpu_ctx, w, m, k, f)
+    e = _check(out, ref)
+    rays = out["stats"]["rays"]
+    assert rays[2] > 0.5 * rays[0] and rays[3] > 0.25 * rays[0]
+    assert np.median(e) < 1e-7
+
+
+def test_cook_torrance_frame(gpu_ctx, rt, O):
+    """cook_torrance_cxy objects and mirror wall next to a phong glass pane:
+    chroma conversion, recursion through rs and ts, mixed material kinds."""
+    w, m, k, f = _scene(res=96, mirror=0.8, glass=(0.6, 1.5))
+    # Material 0 is the mirror wall, 1 the glass; objects take material 2.
+    m = [dict(kind=1, x=0.30, y=0.32, rd=0.05, rs=0.8, sr=0.1),
+         m[1],
+         dict(kind=1, x=0.36, y=0.38, rd=0.7, rs=0.3, sr=0.2)]
+    k = k.copy()
+    obj = k == -1
+    obj[:2] = False
+    k[obj] = 2
+    out, ref = _both(rt, O, gpu_ctx, w, m, k, f)
+    _check(out, ref)
+    assert out["stats"]["rays"][2] > 96 * 96 * 0.5
+    # A glass with cook_torrance transmittance, multi-sample.
+    m2 = [m[0], dict(kind=1, ts=0.5, ri_r=1.3, rd=0.1), m[2]]
+    out, ref = _both(rt, O, gpu_ctx, w, m2, k, f, spp=2, seed=5)
+    _check(out, ref, spp=2)
+
+
+def test_cook_torrance_errors(gpu_ctx, rt):
+    w, m, k, f = _scene(res=32)
+    scene = rt.Scene(gpu_ctx, w.abc, w.albedo, f)
+    zeros = np.zeros(w.abc.shape[0], np.int32)
+    with pytest.raises(rt.RTError, match="out of range"):
+        scene.set_materials([dict(kind=1, sr=0.6)], zeros)
+    with pytest.raises(rt.RTError, match="out of range"):
+        scene.set_materials([dict(kind=1, x=1.5)], zeros)
+    bad = rt.material_table([dict()])
+    bad["kind"] = 7
+    with pytest.raises(rt.RTError, match="unknown kind"):
+        scene.set_materials(bad, zeros)

@@ -149,3 +149,91 @@ def test_reflection_keeps_the_tangent_and_flips_the_normal_part():
     # Axis-aligned normals reflect exactly.
     d = np.array([0.3, -0.7, 0.648074], np.float32)
     assert np.array_equal(O.reflect_dir(d, (0, 1, 0)), np.array([d[0], -d[1], d[2]], np.float32))
+
+
+def test_ambient_term_is_added_once_per_hit():
+    """ka alone (kd = ks = 0): every hit pixel is exactly ka, lit or not;
+    with a mirror of reflectance rf facing it, ka + rf ka for pixels whose
+    mirror ray comes back to an ambient surface."""
+    w, _, _, f = _w(res=40)
+    n = w.abc.shape[0]
+    ka = (7.0, 11.0, 13.0)
+    r = O.OracleScene(w.abc, f).render(w.albedo, w.camera, w.lights, nthreads=4, materials=[dict(ka=ka)],
+                                       tri_material=np.zeros(n, np.int32))
+    hit = r["prim"] >= 0
+    assert hit.mean() > 0.5
+    assert np.array_equal(r["rgb"][hit], np.tile(np.float32(ka), (int(hit.sum()), 1)))
+    assert np.all(r["rgb"][~hit] == 0)
+
+
+def _ct(**kw):
+    return dict(kind=1, **kw)
+
+
+def test_cook_torrance_white_diffuse_is_lambert():
+    """cook_torrance_cxy with rs = 0 and the white chroma of the colour-space
+    conversion (0.3333, 0.3333) gives r = g = b = 255 * shade * rd: the phong
+    frame with kd = 255 * rd."""
+    w, _, _, f = _w(res=48)
+    n = w.abc.shape[0]
+    k = np.zeros(n, np.int32)
+    sc = O.OracleScene(w.abc, f)
+    ct = sc.render(w.albedo, w.camera, w.lights, nthreads=4, materials=[_ct(x=0.3333, y=0.3333, rd=0.5)],
+                   tri_material=k)
+    ph = sc.render(w.albedo, w.camera, w.lights, nthreads=4, materials=[dict(kd=(127.5, 127.5, 127.5))],
+                   tri_material=k)
+    lit = ph["rgb"].max(1) > 0
+    assert lit.mean() > 0.3
+    assert np.array_equal(lit, ct["rgb"].max(1) > 0)
+    e = rel_err(ct["rgb"][lit], ph["rgb"][lit], floor=1e-2)
+    assert e.max() < 1e-5, e.max()
+
+
+def test_cook_torrance_primary_chroma_is_a_primary():
+    """The red primary's chroma (0.64, 0.33) converts to pure red, the green
+    primary's (0.29, 0.60) to pure green (the XYZ -> RGB matrix maps each
+    primary to its own channel)."""
+    w, _, _, f = _w(res=40)
+    n = w.abc.shape[0]
+    for (x, y), ch in (((0.64, 0.33), 0), ((0.29, 0.60), 1)):
+        r = O.OracleScene(w.abc, f).render(w.albedo, w.camera, w.lights, nthreads=4,
+                                           materials=[_ct(x=x, y=y, rd=0.3)], tri_material=np.zeros(n, np.int32))
+        rgb = r["rgb"].astype(np.float64)
+        lit = rgb[:, ch] > 1.0
+        assert lit.mean() > 0.3
+        others = [c for c in range(3) if c != ch]
+        assert np.all(rgb[lit][:, others] <= 1e-4 * rgb[lit][:, [ch]])
+
+
+def test_cook_torrance_specular_lobe_vanishes():
+    """The shading normal faces the ray, so nv = n.v < 0 < nl = n.l and the
+    two geometric terms g0 = 2 nh nv / vh and g1 = 2 nh nl / vh have product
+    4 nh^2 nv nl / vh^2 <= 0: min(g0, g1) <= 0 and the clamped g is 0. The
+    specular term s = rs g f ro / (nl nv) is then a zero, and below the
+    reflection threshold (rs < 0.1) neither rs nor the roughness changes the
+    frame, bit for bit."""
+    w, _, _, f = _w(res=48)
+    n = w.abc.shape[0]
+    k = np.zeros(n, np.int32)
+    sc = O.OracleScene(w.abc, f)
+    frames = [sc.render(w.albedo, w.camera, w.lights, nthreads=4,
+                        materials=[_ct(x=0.31, y=0.35, rd=0.3, rs=rs, sr=sr)], tri_material=k)
+              for rs, sr in ((0.0, 0.0), (0.09, 0.2), (0.05, 0.5))]
+    assert (frames[0]["rgb"].max(1) > 0).mean() > 0.3
+    for fr in frames[1:]:
+        assert np.array_equal(fr["rgb"].view(np.uint32), frames[0]["rgb"].view(np.uint32))
+        assert int(fr["sec_counters"][0]) == 0
+
+
+def test_cook_torrance_recursion_uses_rs_and_ts():
+    abc, albedo, cam, lights, _, kinds = _corridor(0.5)
+    mats = [_ct(rs=0.5, rd=0.4, sr=0.1)]
+    r = O.OracleScene(abc).render(albedo, cam, lights, nthreads=2, materials=mats, tri_material=kinds)
+    npix = cam.x_res * cam.y_res
+    assert int(r["sec_counters"][0]) == 3 * npix
+    w1, m1, k1, f1 = _w(res=64, glass=(0.6, 1.0))
+    m1 = [_ct(ts=0.6, ri_r=1.0, rd=0.0)]
+    r1 = O.OracleScene(w1.abc, f1).render(w1.albedo, w1.camera, w1.lights, nthreads=4, materials=m1,
+                                          tri_material=np.where(k1 >= 0, 0, -1).astype(np.int32))
+    glass = r1["prim"] >= w1.abc.shape[0] - 2
+    assert int(r1["sec_counters"][0]) == int(glass.sum())
