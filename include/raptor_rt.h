@@ -203,6 +203,46 @@ typedef struct {
 int32_t rt_scene_set_materials(rt_scene *scene, const rt_material *materials, int32_t nmaterials,
                                const int32_t *tri_material);
 
+/* ---- Camera post-process: the step after the image buffer ----
+ * Images are camera::image as rt_render writes it: x_res * y_res * 3 floats
+ * on the 0-255 scale, pixel (x, y) at 3 * (x + y * x_res), on the host or,
+ * with RT_DEVICE_PTRS, on the device. ms (may be NULL) receives the device
+ * time of the call's kernels.
+ *
+ * rt_image_tone_map replaces camera::tone_map (rt/camera.cc:1124-1461) with
+ * glare off: mode is tone_mapping_mode_t (rt/camera.h:43-45): 0 none,
+ * 1 global_contrast, 4 global_non_linear, 5 global_exposure,
+ * 6 global_avg_luminance, 7 global_max_luminance, 9 global_ferwerda; the
+ * local modes 2, 3 (histogram) and 8 (bilateral) return RT_ERR_UNSUPPORTED.
+ * key, xw, yw, desaturate (ds), stretch (sc) and adapt (ta) with time_step
+ * are its arguments; adaptation_level is the camera's adaptation state (read
+ * when adapting, always written), log_avg_luminance receives the adapted
+ * log-average luminance. The log-average is summed in double (the
+ * reference sums in float, pixel by pixel). */
+typedef struct {
+    int32_t mode;
+    float key, xw, yw;
+    int32_t desaturate, stretch, adapt;
+    float time_step;
+    float adaptation_level;
+    float log_avg_luminance;
+} rt_tone_params;
+int32_t rt_image_tone_map(rt_ctx *ctx, float *rgb, int32_t x_res, int32_t y_res, rt_tone_params *params,
+                          uint32_t flags, float *ms);
+/* camera::gamma_correct (rt/camera.cc:1466-1501) of npix pixels, in place. */
+int32_t rt_image_gamma(rt_ctx *ctx, float *rgb, int64_t npix, float gamma, uint32_t flags, float *ms);
+/* clip_image_to_rgb (bgr = 0) or clip_image_to_bgr (bgr = 1)
+ * (rt/camera.h:289-351): an x_res x y_res image with x_samples x y_samples
+ * samples per output pixel box-filtered to (x_res / x_samples) x
+ * (y_res / y_samples) pixels, clamped to [0, 255] and truncated to bytes. */
+int32_t rt_image_clip(rt_ctx *ctx, const float *rgb, int32_t x_res, int32_t y_res, int32_t x_samples,
+                      int32_t y_samples, int32_t bgr, uint8_t *out, uint32_t flags, float *ms);
+/* Host writers for clipped images: write_png_file (rt/camera.cc:2158-2210,
+ * rgb bytes, last image row first, stored deflate blocks) and
+ * write_tga_file (rt/camera.cc:2102-2155, bgr bytes). */
+int32_t rt_write_png(const char *path, const uint8_t *rgb, int32_t x_res, int32_t y_res);
+int32_t rt_write_tga(const char *path, const uint8_t *bgr, int32_t x_res, int32_t y_res);
+
 /* Closest hit for n rays. Replaces bvh::find_nearest_object(ray*)
  * (ssd/bvh.cc:649-709) and its packet form (ssd/bvh.cc:406-447). */
 int32_t rt_trace_closest(rt_accel *accel, const rt_ray *rays, int64_t n, rt_hit *out, uint32_t flags);

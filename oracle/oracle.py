@@ -41,6 +41,12 @@ class oracle_render_out(C.Structure):
 
 # Phong material fields, in the order the oracle (and include/raptor_rt.h's
 # rt_material) stores them: rt/materials/phong_shader.h:38-60.
+class oracle_tone_params(C.Structure):
+    _fields_ = [("mode", C.c_int32), ("key", C.c_float), ("xw", C.c_float), ("yw", C.c_float),
+                ("desaturate", C.c_int32), ("stretch", C.c_int32), ("adapt", C.c_int32), ("glare", C.c_int32),
+                ("time_step", C.c_float)]
+
+
 def material_array(materials):
     """materials: list of dicts. Phong (kind 0, the default): ka, kd, ks (rgb
     in [0, 255]), s, tran, ri, rf. Cook-Torrance (kind 1, cook_torrance_cxy):
@@ -101,6 +107,9 @@ def _load():
         "oracle_packet_inverse_sqrt": (None, [vp, vp]),
         "oracle_packet_trace_closest": (None, [vp, vp, i64, vp, vp, i32]),
         "oracle_packet_trace_occluded": (None, [vp, vp, i64, vp, i32]),
+        "oracle_tone_map": (i32, [vp, i32, i32, C.POINTER(oracle_tone_params), vp, vp]),
+        "oracle_gamma": (None, [vp, i64, f]),
+        "oracle_clip": (None, [vp, i32, i32, i32, i32, i32, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -387,3 +396,33 @@ def refract_dir(d, n, ri, leaving):
     out = np.zeros(3, np.float32)
     ok = LIB.oracle_refract_dir(_p(d), _p(n), float(ri), 1 if leaving else 0, _p(out))
     return out if ok else None
+
+
+def tone_map(rgb, mode=0, key=0.18, xw=1.0 / 3.0, yw=1.0 / 3.0, desaturate=False, stretch=False, adapt=False,
+             time_step=0.0, adaptation_level=0.0):
+    """camera::tone_map on a copy of rgb ((H, W, 3) float32, 0-255); returns
+    (image, adaptation level, adapted log-average luminance)."""
+    img = np.array(rgb, np.float32, order="C", copy=True)
+    h, w = img.shape[0], img.shape[1]
+    tp = oracle_tone_params(int(mode), key, xw, yw, int(desaturate), int(stretch), int(adapt), 0, time_step)
+    lvl = np.array([adaptation_level], np.float32)
+    yi = np.zeros(1, np.float32)
+    if LIB.oracle_tone_map(_p(img), w, h, C.byref(tp), _p(lvl), _p(yi)) != 0:
+        raise ValueError(f"tone mapping mode {mode} is not restated")
+    return img, float(lvl[0]), float(yi[0])
+
+
+def gamma_correct(rgb, gamma):
+    img = np.array(rgb, np.float32, order="C", copy=True)
+    LIB.oracle_gamma(_p(img), img.size // 3, gamma)
+    return img
+
+
+def clip(rgb, xs=1, ys=1, bgr=False):
+    """Box down-sampling by (xs, ys), clamp and truncation to bytes:
+    (H / ys, W / xs, 3) uint8."""
+    img = np.ascontiguousarray(rgb, np.float32)
+    h, w = img.shape[0], img.shape[1]
+    out = np.zeros((h // ys, w // xs, 3), np.uint8)
+    LIB.oracle_clip(_p(img), w, h, xs, ys, int(bgr), _p(out))
+    return out

@@ -140,6 +140,27 @@ int32_t oracle_refract_dir(const float *d, const float *n, float ri, int32_t lea
 /* Primary ray of rt/camera.h:127-158 (EXACT_NORMALISE). */
 void oracle_primary_ray(const oracle_camera *cam, int32_t x, int32_t y, float *ogn, float *dir);
 
+/* Camera post-process on an image of 0-255 floats, pixel (x, y) at
+ * 3 * (x + y * x_res) (postprocess.cpp). tone_map: camera::tone_map
+ * (rt/camera.cc:1124-1461) for mode 0 (none) and the global modes 1, 4, 5,
+ * 6, 7, 9 of tone_mapping_mode_t (rt/camera.h:43-45); returns -1 for the
+ * local modes 2, 3, 8 and for glare. adaptation_level is the camera's
+ * adaptation state (read with adapt, always written); yi_out the adapted
+ * log-average luminance. gamma: camera::gamma_correct
+ * (rt/camera.cc:1466-1501). clip: clip_image_to_rgb / _bgr
+ * (rt/camera.h:289-351) of an x_res x y_res image with xs x ys samples per
+ * output pixel. */
+typedef struct {
+    int32_t mode;
+    float key, xw, yw;
+    int32_t desaturate, stretch, adapt, glare;
+    float time_step;
+} oracle_tone_params;
+int32_t oracle_tone_map(float *img, int32_t x_res, int32_t y_res, const oracle_tone_params *tp,
+                        float *adaptation_level, float *yi_out);
+void oracle_gamma(float *img, int64_t npix, float gamma);
+void oracle_clip(const float *img, int32_t x_res, int32_t y_res, int32_t xs, int32_t ys, int32_t bgr, uint8_t *out);
+
 #ifdef __cplusplus
 }
 #endif

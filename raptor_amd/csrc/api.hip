@@ -269,6 +269,41 @@ int32_t rt_scene_update(rt_scene *scene, const float *abc, uint32_t flags) {
     return RT_OK;
 }
 
+int32_t rt_image_tone_map(rt_ctx *ctx, float *rgb, int32_t x_res, int32_t y_res, rt_tone_params *params,
+                          uint32_t flags, float *ms) {
+    if (!ctx || !rgb || !params) return RT_ERR_INVALID;
+    if (int32_t rc = rt::set_device(ctx)) return rc;
+    if (x_res <= 0 || y_res <= 0) return fail(ctx, RT_ERR_INVALID, "rt_image_tone_map: resolution must be positive");
+    return rt::image_tone_map(ctx, rgb, x_res, y_res, params, flags, ms);
+}
+
+int32_t rt_image_gamma(rt_ctx *ctx, float *rgb, int64_t npix, float gamma, uint32_t flags, float *ms) {
+    if (!ctx || (npix > 0 && !rgb) || npix < 0) return RT_ERR_INVALID;
+    if (int32_t rc = rt::set_device(ctx)) return rc;
+    if (npix == 0) return RT_OK;
+    if (!(gamma > 0.0f)) return fail(ctx, RT_ERR_INVALID, "rt_image_gamma: gamma must be positive");
+    return rt::image_gamma(ctx, rgb, npix, gamma, flags, ms);
+}
+
+int32_t rt_image_clip(rt_ctx *ctx, const float *rgb, int32_t x_res, int32_t y_res, int32_t x_samples,
+                      int32_t y_samples, int32_t bgr, uint8_t *out, uint32_t flags, float *ms) {
+    if (!ctx || !rgb || !out) return RT_ERR_INVALID;
+    if (int32_t rc = rt::set_device(ctx)) return rc;
+    if (x_res <= 0 || y_res <= 0 || x_samples < 1 || y_samples < 1 || x_samples > x_res || y_samples > y_res)
+        return fail(ctx, RT_ERR_INVALID, "rt_image_clip: bad resolution or sample counts");
+    return rt::image_clip(ctx, rgb, x_res, y_res, x_samples, y_samples, bgr ? 1 : 0, out, flags, ms);
+}
+
+int32_t rt_write_png(const char *path, const uint8_t *rgb, int32_t x_res, int32_t y_res) {
+    if (!path || !rgb || x_res <= 0 || y_res <= 0) return RT_ERR_INVALID;
+    return rt::write_png(path, rgb, x_res, y_res);
+}
+
+int32_t rt_write_tga(const char *path, const uint8_t *bgr, int32_t x_res, int32_t y_res) {
+    if (!path || !bgr || x_res <= 0 || y_res <= 0 || x_res > 65535 || y_res > 65535) return RT_ERR_INVALID;
+    return rt::write_tga(path, bgr, x_res, y_res);
+}
+
 static_assert(sizeof(rt_material) == 76, "rt_material is 19 four-byte fields");
 
 int32_t rt_scene_set_materials(rt_scene *scene, const rt_material *materials, int32_t nmaterials,

@@ -1,0 +1,604 @@
+// Camera post-process on the device: camera::tone_map for its global modes
+// (rt/camera.cc:1124-1461), camera::gamma_correct (rt/camera.cc:1466-1501)
+// and the down-sampling clip to bytes of the image writers
+// (rt/camera.h:289-351). Images are camera::image layouts: 0-255 floats,
+// pixel (x, y) at 3 * (x + y * x_res).
+//
+// Tone mapping is three passes over the image (HBM-bound, 12 bytes read and
+// written per pixel and pass):
+//   k_tone_yxy      rgb -> (x, Y, y) in place (with the desaturation), and
+//                   per block: the sum of log10 Y (double), max Y, count
+//   k_tone_scalars  one block: the log-average luminance, the temporal
+//                   adaptation, the mode's scale factors
+//   k_tone_apply    the mode's luminance mapping and (x, Y, y) -> rgb; with
+//                   the contrast stretch it first leaves the mapped Y and
+//                   per-block min / max, and k_tone_stretch finishes.
+// The block partials are combined in block order, so results repeat bit for
+// bit; the reference sums log10 Y in float in pixel order, which the double
+// sum here approximates more closely than that float sum does.
+#include <algorithm>
+#include <cmath>
+#include <fstream>
+#include <string>
+#include <vector>
+
+#include "rt_internal.hpp"
+
+namespace rt {
+
+namespace {
+
+constexpr int kPostThreads = 256;
+constexpr int kPostMaxBlocks = 2048;
+constexpr float kMaxDistF = 3.402823466e+38f;
+
+// tone_mapping_mode_t (rt/camera.h:43-45).
+enum ToneMode : int { kNone = 0, kContrast = 1, kNonLinear = 4, kExposure = 5, kAvgLum = 6, kMaxLum = 7, kFerwerda = 9 };
+
+struct TonePartial {
+    double log_sum;
+    float max_y;
+    int32_t count;
+};
+
+struct ToneScalars {
+    float sf;         // luminance scale
+    float max_y;      // global_non_linear: 1 / (sf * maxY)^2; global_exposure: sf * maxY
+    float mp, ms, k;  // global_ferwerda
+    float yi;         // adapted log-average luminance
+    float level;      // adaptation level after this frame
+    float offset, scale;  // contrast stretch
+    int32_t stretch_on;
+};
+
+template <class T>
+__device__ __forceinline__ T block_reduce(T v, T (*op)(T, T), T *lds) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = op(v, __shfl_down(v, off, 64));
+    if (lane == 0) lds[wave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        T r = lds[0];
+        for (int w = 1; w < kPostThreads / 64; ++w) r = op(r, lds[w]);
+        lds[0] = r;
+    }
+    __syncthreads();
+    const T r = lds[0];
+    __syncthreads();
+    return r;
+}
+
+__device__ double add_d(double a, double b) { return a + b; }
+__device__ float max_f(float a, float b) { return fmaxf(a, b); }
+__device__ float min_f(float a, float b) { return fminf(a, b); }
+__device__ int add_i(int a, int b) { return a + b; }
+
+struct ToneArgs {
+    float *img;
+    int64_t n;
+    int mode;
+    float key, xw, yw;
+    int desaturate, stretch, adapt;
+    float time_step, level_in;
+};
+
+// Per pixel: rt/camera.cc:1137-1195.
+__global__ void __launch_bounds__(kPostThreads) k_tone_yxy(ToneArgs a, TonePartial *__restrict__ part) {
+    __shared__ double lds_d[kPostThreads / 64];
+    __shared__ float lds_f[kPostThreads / 64];
+    __shared__ int lds_i[kPostThreads / 64];
+    const float xb[3] = {0.4124f, 0.3576f, 0.1805f};
+    const float yb[3] = {0.2126f, 0.7151f, 0.0721f};
+    const float zb[3] = {0.0193f, 0.1192f, 0.9505f};
+    const float vb[3] = {0.01477f, 0.497f, 0.631f};
+    double log_sum = 0.0;
+    float max_y = 0.0f;
+    int count = 0;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kPostThreads + threadIdx.x; i < a.n;
+         i += static_cast<int64_t>(gridDim.x) * kPostThreads) {
+        float *p = a.img + 3 * i;
+        const float r = p[0] / 255.0f, g = p[1] / 255.0f, b = p[2] / 255.0f;
+        const float X = 683.0f * ((xb[0] * r) + (xb[1] * g) + (xb[2] * b));
+        float Y = 683.0f * ((yb[0] * r) + (yb[1] * g) + (yb[2] * b));
+        const float Z = 683.0f * ((zb[0] * r) + (zb[1] * g) + (zb[2] * b));
+        if (Y < 3.18e-7f) {
+            Y = 3.18e-7f;
+        } else {
+            ++count;
+            max_y = fmaxf(max_y, Y);
+            log_sum += static_cast<double>(log10f(Y));
+        }
+        const float W = Y + X + Z;
+        float x = X / W;
+        float y = Y / W;
+        if (a.desaturate) {
+            float s;
+            // The reference's unqualified log10 of a float is the double one.
+            const float log10_Y = static_cast<float>(log10(static_cast<double>(Y)));
+            if (log10_Y < -2.0f) {
+                s = 0.0f;
+            } else if (log10_Y < 0.6f) {
+                const float sl = (log10_Y + 2.0f) / 2.6f;
+                const float sl2 = sl * sl;
+                s = (3.0f * sl2) - (2.0f * sl * sl2);
+            } else {
+                s = 1.0f;
+            }
+            const float V = 1700.0f * ((vb[0] * r) + (vb[1] * g) + (vb[2] * b));
+            x = ((1.0f - s) * a.xw) + (s * (x + a.xw - (1.0f / 3.0f)));
+            y = ((1.0f - s) * a.yw) + (s * (y + a.yw - (1.0f / 3.0f)));
+            Y = (0.4468f * (1.0f - s) * V) + (s * Y);
+        }
+        p[0] = x;
+        p[1] = Y;
+        p[2] = y;
+    }
+    log_sum = block_reduce<double>(log_sum, add_d, lds_d);
+    max_y = block_reduce<float>(max_y, max_f, lds_f);
+    count = block_reduce<int>(count, add_i, lds_i);
+    if (threadIdx.x == 0) part[blockIdx.x] = TonePartial{log_sum, max_y, count};
+}
+
+// One block: rt/camera.cc:1196-1414 (everything that is not per pixel).
+// Thread t folds partials t, t + 256, ... and the block reduces the 256
+// results in a fixed tree, so the order never changes.
+__global__ void __launch_bounds__(kPostThreads) k_tone_scalars(ToneArgs a, const TonePartial *__restrict__ part,
+                                                               int nparts, ToneScalars *__restrict__ out) {
+    __shared__ double lds_d[kPostThreads / 64];
+    __shared__ float lds_f[kPostThreads / 64];
+    __shared__ int lds_i[kPostThreads / 64];
+    double log_sum = 0.0;
+    float maxY = 0.0f;
+    int numpix = 0;
+    for (int b = threadIdx.x; b < nparts; b += kPostThreads) {
+        const TonePartial q = part[b];
+        log_sum += q.log_sum;
+        maxY = fmaxf(maxY, q.max_y);
+        numpix += q.count;
+    }
+    log_sum = block_reduce<double>(log_sum, add_d, lds_d);
+    maxY = block_reduce<float>(maxY, max_f, lds_f);
+    numpix = block_reduce<int>(numpix, add_i, lds_i);
+    if (threadIdx.x != 0) return;
+    float Yi = static_cast<float>(log_sum);
+    if (numpix) Yi /= static_cast<float>(numpix);
+    Yi = powf(10.0f, Yi);
+    ToneScalars t{};
+    if (a.adapt && a.time_step > 0.0f) {
+        float k;
+        const float log_la = logf(Yi);
+        if (log_la < -2.0f) {
+            k = 1.0f;
+        } else if (log_la < 1.0f) {
+            k = 1.0f - ((log_la + 2.0f) * (1.0f / 3.0f));
+        } else {
+            k = 0.0f;
+        }
+        const float lvl = a.level_in, ts = a.time_step;
+        float level;
+        if (lvl > Yi) {
+            const float a_cone = Yi + ((lvl - Yi) * expf(-(ts / 0.2f)));
+            const float a_rod = Yi + ((lvl - Yi) * expf(-(ts / 6.0f)));
+            level = ((1.0f - k) * a_cone) + (k * a_rod);
+        } else {
+            const float r = (Yi / lvl);
+            const float c1 = lvl * (powf(r, 0.9f) - 1.0f);
+            const float c2 = Yi * (1.0f - powf(r, -0.1f));
+            const float a_cone = Yi - (c1 * expf(-(ts / 16.0e-3f))) - (c2 * expf(-(ts / 200.0e-3f)));
+            const float a_rod = Yi - (c1 * expf(-(ts / 40.0e-3f))) - (c2 * expf(-(ts / 6.0f)));
+            level = ((1.0f - k) * a_cone) + (k * a_rod);
+        }
+        t.level = level;
+        Yi = level;
+    } else {
+        t.level = Yi;
+    }
+    t.yi = Yi;
+    t.sf = 1.0f;
+    switch (a.mode) {
+        case kContrast: {
+            const float Yw = 300.0f;
+            const float num = 1.219f + powf(Yw * 0.5f, 0.4f);
+            const float den = 1.219f + powf(Yi, 0.4f);
+            t.sf = (1.0f / Yw) * powf((num / den), 2.5f);
+            break;
+        }
+        case kNonLinear: {
+            t.sf = a.key / Yi;
+            float m = maxY * t.sf;
+            t.max_y = 1.0f / (m * m);
+            break;
+        }
+        case kExposure:
+            t.sf = a.key / Yi;
+            t.max_y = maxY * t.sf;
+            break;
+        case kAvgLum:
+            // The reference averages rgb only for global_exposure
+            // (rt/camera.cc:1141-1145), so here the average is 0 and the
+            // scale stays 1 (rt/camera.cc:1331-1342).
+            t.sf = 1.0f;
+            break;
+        case kMaxLum:
+            t.sf = a.key / maxY;
+            break;
+        case kFerwerda: {
+            const float lda = 800.0f;
+            const float log_lda = logf(lda * 0.5f);
+            const float log_la = logf(Yi);
+            float tpc;
+            if (log_la <= -2.6f) {
+                tpc = -0.72f;
+            } else if (log_la <= 1.9f) {
+                tpc = powf(((0.249f * log_la) + 0.65f), 2.7f) - 0.72f;
+            } else {
+                tpc = log_la - 1.255f;
+            }
+            t.mp = (log_lda - 1.255f) / tpc;
+            float tsc;
+            if (log_la <= -3.94f) {
+                tsc = -2.86f;
+            } else if (log_la < -1.44f) {
+                tsc = powf(((0.405f * log_la) + 1.6f), 2.18f) - 2.86f;
+            } else {
+                tsc = log_la - 0.395f;
+            }
+            t.ms = (log_lda - 1.255f) / tsc;
+            if (log_la < -2.0f) {
+                t.k = 1.0f;
+            } else if (log_la < 1.0f) {
+                t.k = 1.0f - ((log_la + 2.0f) * (1.0f / 3.0f));
+            } else {
+                t.k = 0.0f;
+            }
+            break;
+        }
+        default:
+            t.sf = 1.0f / 683.0f;
+            break;
+    }
+    *out = t;
+}
+
+__device__ __forceinline__ float map_luminance(int mode, const ToneScalars &t, float g) {
+    switch (mode) {
+        case kNonLinear:
+            g *= t.sf;
+            if (t.max_y) g *= (1.0f + g * t.max_y) / (1.0f + g);
+            return g;
+        case kExposure:
+            g *= t.sf;
+            if (t.max_y) g = 1.0f - expf(-g);
+            return g;
+        case kFerwerda: {
+            const float lda = 800.0f;
+            const float ldp = t.mp * g;
+            const float lds = t.ms * g;
+            return (1.0f / lda) * (ldp + (t.k * lds));
+        }
+        default:
+            return g * t.sf;
+    }
+}
+
+// camera::convert_Yxy_to_rgb (rt/camera.cc:2080-2099).
+__device__ __forceinline__ void yxy_to_rgb(float *p, float x, float Y, float y) {
+    const float X = x * Y / y;
+    const float Z = (1.0f - x - y) * Y / y;
+    p[0] = (826.353f * X) + (-391.986f * Y) + (-127.143f * Z);
+    p[1] = (-247.0695f * X) + (478.329f * Y) + (10.5825f * Z);
+    p[2] = (14.2035f * X) + (-52.02f * Y) + (269.535f * Z);
+}
+
+// Mapped luminance, then rgb; with the stretch, the mapped Y and per-block
+// min / max only.
+__global__ void __launch_bounds__(kPostThreads) k_tone_apply(ToneArgs a, const ToneScalars *__restrict__ ts,
+                                                             float2 *__restrict__ minmax) {
+    __shared__ float lds_f[kPostThreads / 64];
+    const ToneScalars t = *ts;
+    float lo = kMaxDistF, hi = -kMaxDistF;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kPostThreads + threadIdx.x; i < a.n;
+         i += static_cast<int64_t>(gridDim.x) * kPostThreads) {
+        float *p = a.img + 3 * i;
+        const float g = map_luminance(a.mode, t, p[1]);
+        if (a.stretch) {
+            p[1] = g;
+            lo = fminf(lo, g);
+            hi = fmaxf(hi, g);
+        } else {
+            yxy_to_rgb(p, p[0], g, p[2]);
+        }
+    }
+    if (!a.stretch) return;
+    lo = block_reduce<float>(lo, min_f, lds_f);
+    hi = block_reduce<float>(hi, max_f, lds_f);
+    if (threadIdx.x == 0) minmax[blockIdx.x] = make_float2(lo, hi);
+}
+
+// rt/camera.cc:1417-1436.
+__global__ void __launch_bounds__(kPostThreads) k_stretch_scalars(const float2 *__restrict__ minmax, int nparts,
+                                                                  ToneScalars *__restrict__ ts) {
+    __shared__ float lds_f[kPostThreads / 64];
+    float minY2 = kMaxDistF, maxY2 = -kMaxDistF;
+    for (int b = threadIdx.x; b < nparts; b += kPostThreads) {
+        const float2 q = minmax[b];
+        minY2 = fminf(minY2, q.x);
+        maxY2 = fmaxf(maxY2, q.y);
+    }
+    minY2 = block_reduce<float>(minY2, min_f, lds_f);
+    maxY2 = block_reduce<float>(maxY2, max_f, lds_f);
+    if (threadIdx.x != 0) return;
+    ToneScalars t = *ts;
+    t.stretch_on = ((minY2 > 0.0f) || (maxY2 < 0.0f)) && (minY2 < maxY2);
+    t.offset = fminf(-minY2, 0.0f);
+    t.scale = 1.0f / (fminf(1.0f, maxY2) - fmaxf(0.0f, minY2));
+    *ts = t;
+}
+
+__global__ void __launch_bounds__(kPostThreads) k_tone_stretch(float *__restrict__ img, int64_t n,
+                                                               const ToneScalars *__restrict__ ts) {
+    const ToneScalars t = *ts;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kPostThreads + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * kPostThreads) {
+        float *p = img + 3 * i;
+        float g = p[1];
+        if (t.stretch_on) g = (g + t.offset) * t.scale;
+        yxy_to_rgb(p, p[0], g, p[2]);
+    }
+}
+
+// camera::gamma_correct (rt/camera.cc:1466-1501): a gamma within 0.001 of
+// 2.2 takes the sRGB curve, any other the plain power (the reference's
+// branch order).
+__global__ void __launch_bounds__(kPostThreads) k_gamma(float *__restrict__ img, int64_t n, float gamma) {
+    const bool plain = fabsf(gamma - 2.2f) > 0.001f;
+    const bool srgb = !plain && fabsf(gamma - 1.0f) > 0.001f;
+    const float inv = 1.0f / gamma;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kPostThreads + threadIdx.x; i < 3 * n;
+         i += static_cast<int64_t>(gridDim.x) * kPostThreads) {
+        float v = img[i] / 255.0f;
+        v = fminf(fmaxf(v, 0.0f), 1.0f);
+        if (plain) {
+            v = powf(v, inv);
+        } else if (srgb) {
+            v = (v <= 0.0031308f) ? (12.92f * v) : (1.055f * powf(v, (1.0f / 2.4f)) - 0.055f);
+        }
+        img[i] = v * 255.0f;
+    }
+}
+
+// clip_image_to_rgb / clip_image_to_bgr (rt/camera.h:289-351): one output
+// pixel per thread, its xs x ys samples summed column by column.
+__global__ void __launch_bounds__(kPostThreads) k_clip(const float *__restrict__ img, int x_res, int xs, int ys,
+                                                       int ox, int64_t nout, int bgr, uint8_t *__restrict__ out) {
+    const int samples = xs * ys;
+    const float inv = 1.0f / static_cast<float>(xs * ys);
+    for (int64_t o = static_cast<int64_t>(blockIdx.x) * kPostThreads + threadIdx.x; o < nout;
+         o += static_cast<int64_t>(gridDim.x) * kPostThreads) {
+        const int x = static_cast<int>(o % ox), y = static_cast<int>(o / ox);
+        const int64_t addr = static_cast<int64_t>(x) * xs + static_cast<int64_t>(y) * ys * x_res;
+        float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+        for (int ax = 0; ax < xs; ++ax)
+            for (int ay = 0; ay < ys; ++ay) {
+                const float *p = img + 3 * (addr + static_cast<int64_t>(ay) * x_res + ax);
+                a0 += p[0];
+                a1 += p[1];
+                a2 += p[2];
+            }
+        uint8_t *q = out + 3 * o;
+        if (bgr) {
+            q[0] = static_cast<uint8_t>(fminf(255.0f, fmaxf(0.0f, (a2 * inv))));
+            q[1] = static_cast<uint8_t>(fminf(255.0f, fmaxf(0.0f, (a1 * inv))));
+            q[2] = static_cast<uint8_t>(fminf(255.0f, fmaxf(0.0f, (a0 * inv))));
+        } else {
+            q[0] = static_cast<uint8_t>(fminf(255.0f, fmaxf(0.0f, (a0 / static_cast<float>(samples)))));
+            q[1] = static_cast<uint8_t>(fminf(255.0f, fmaxf(0.0f, (a1 / static_cast<float>(samples)))));
+            q[2] = static_cast<uint8_t>(fminf(255.0f, fmaxf(0.0f, (a2 / static_cast<float>(samples)))));
+        }
+    }
+}
+
+int post_blocks(int64_t n) {
+    return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(kPostMaxBlocks, (n + kPostThreads - 1) / kPostThreads)));
+}
+
+}  // namespace
+
+int32_t image_tone_map(rt_ctx *ctx, float *rgb, int32_t x_res, int32_t y_res, rt_tone_params *tp, uint32_t flags,
+                       float *ms) {
+    hipStream_t s = ctx->stream;
+    const int m = tp->mode;
+    if (m == 2 || m == 3 || m == 8)
+        return fail(ctx, RT_ERR_UNSUPPORTED, "rt_image_tone_map: the local tone mappers (histogram, bilateral) are not built");
+    if (m < 0 || m > 9) return fail(ctx, RT_ERR_INVALID, "rt_image_tone_map: unknown mode " + std::to_string(m));
+    const int64_t n = static_cast<int64_t>(x_res) * y_res;
+    BuildArena &ar = ctx->arena;
+    RT_HIP_CHECK(ctx, ar.begin());
+    float *img = rgb;
+    const bool dev = (flags & RT_DEVICE_PTRS) != 0;
+    void *v = nullptr;
+    if (!dev) {
+        RT_HIP_CHECK(ctx, ar.take(&v, static_cast<size_t>(3 * n) * sizeof(float)));
+        img = static_cast<float *>(v);
+        RT_HIP_CHECK(ctx, hipMemcpyAsync(img, rgb, 3 * n * sizeof(float), hipMemcpyHostToDevice, s));
+    }
+    const int blocks = post_blocks(n);
+    TonePartial *part = nullptr;
+    ToneScalars *scal = nullptr;
+    float2 *minmax = nullptr;
+    RT_HIP_CHECK(ctx, ar.take(&v, blocks * sizeof(TonePartial)));
+    part = static_cast<TonePartial *>(v);
+    RT_HIP_CHECK(ctx, ar.take(&v, sizeof(ToneScalars)));
+    scal = static_cast<ToneScalars *>(v);
+    RT_HIP_CHECK(ctx, ar.take(&v, blocks * sizeof(float2)));
+    minmax = static_cast<float2 *>(v);
+    const ToneArgs a{img, n, m, tp->key, tp->xw, tp->yw, tp->desaturate, tp->stretch, tp->adapt, tp->time_step,
+                     tp->adaptation_level};
+    if (ms) RT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[0], s));
+    hipLaunchKernelGGL(k_tone_yxy, dim3(blocks), dim3(kPostThreads), 0, s, a, part);
+    hipLaunchKernelGGL(k_tone_scalars, dim3(1), dim3(kPostThreads), 0, s, a, part, blocks, scal);
+    hipLaunchKernelGGL(k_tone_apply, dim3(blocks), dim3(kPostThreads), 0, s, a, scal, minmax);
+    if (tp->stretch) {
+        hipLaunchKernelGGL(k_stretch_scalars, dim3(1), dim3(kPostThreads), 0, s, minmax, blocks, scal);
+        hipLaunchKernelGGL(k_tone_stretch, dim3(blocks), dim3(kPostThreads), 0, s, img, n, scal);
+    }
+    RT_HIP_CHECK(ctx, hipGetLastError());
+    if (ms) RT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[1], s));
+    ToneScalars t{};
+    RT_HIP_CHECK(ctx, hipMemcpyAsync(&t, scal, sizeof(t), hipMemcpyDeviceToHost, s));
+    if (!dev) RT_HIP_CHECK(ctx, hipMemcpyAsync(rgb, img, 3 * n * sizeof(float), hipMemcpyDeviceToHost, s));
+    RT_HIP_CHECK(ctx, hipStreamSynchronize(s));
+    tp->adaptation_level = t.level;
+    tp->log_avg_luminance = t.yi;
+    if (ms) RT_HIP_CHECK(ctx, hipEventElapsedTime(ms, ctx->ev[0], ctx->ev[1]));
+    return RT_OK;
+}
+
+int32_t image_gamma(rt_ctx *ctx, float *rgb, int64_t npix, float gamma, uint32_t flags, float *ms) {
+    hipStream_t s = ctx->stream;
+    float *img = rgb;
+    const bool dev = (flags & RT_DEVICE_PTRS) != 0;
+    BuildArena &ar = ctx->arena;
+    RT_HIP_CHECK(ctx, ar.begin());
+    if (!dev) {
+        void *v = nullptr;
+        RT_HIP_CHECK(ctx, ar.take(&v, static_cast<size_t>(3 * npix) * sizeof(float)));
+        img = static_cast<float *>(v);
+        RT_HIP_CHECK(ctx, hipMemcpyAsync(img, rgb, 3 * npix * sizeof(float), hipMemcpyHostToDevice, s));
+    }
+    if (ms) RT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[0], s));
+    hipLaunchKernelGGL(k_gamma, dim3(post_blocks(3 * npix)), dim3(kPostThreads), 0, s, img, npix, gamma);
+    RT_HIP_CHECK(ctx, hipGetLastError());
+    if (ms) RT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[1], s));
+    if (!dev) RT_HIP_CHECK(ctx, hipMemcpyAsync(rgb, img, 3 * npix * sizeof(float), hipMemcpyDeviceToHost, s));
+    RT_HIP_CHECK(ctx, hipStreamSynchronize(s));
+    if (ms) RT_HIP_CHECK(ctx, hipEventElapsedTime(ms, ctx->ev[0], ctx->ev[1]));
+    return RT_OK;
+}
+
+int32_t image_clip(rt_ctx *ctx, const float *rgb, int32_t x_res, int32_t y_res, int32_t xs, int32_t ys, int32_t bgr,
+                   uint8_t *out, uint32_t flags, float *ms) {
+    hipStream_t s = ctx->stream;
+    const int ox = x_res / xs, oy = y_res / ys;
+    const int64_t nin = static_cast<int64_t>(x_res) * y_res, nout = static_cast<int64_t>(ox) * oy;
+    const bool dev = (flags & RT_DEVICE_PTRS) != 0;
+    const float *img = rgb;
+    uint8_t *o = out;
+    BuildArena &ar = ctx->arena;
+    RT_HIP_CHECK(ctx, ar.begin());
+    if (!dev) {
+        void *v = nullptr;
+        RT_HIP_CHECK(ctx, ar.take(&v, static_cast<size_t>(3 * nin) * sizeof(float)));
+        RT_HIP_CHECK(ctx, hipMemcpyAsync(v, rgb, 3 * nin * sizeof(float), hipMemcpyHostToDevice, s));
+        img = static_cast<const float *>(v);
+        RT_HIP_CHECK(ctx, ar.take(&v, static_cast<size_t>(3 * nout)));
+        o = static_cast<uint8_t *>(v);
+    }
+    if (ms) RT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[0], s));
+    hipLaunchKernelGGL(k_clip, dim3(post_blocks(nout)), dim3(kPostThreads), 0, s, img, x_res, xs, ys, ox, nout, bgr, o);
+    RT_HIP_CHECK(ctx, hipGetLastError());
+    if (ms) RT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[1], s));
+    if (!dev) RT_HIP_CHECK(ctx, hipMemcpyAsync(out, o, 3 * nout, hipMemcpyDeviceToHost, s));
+    RT_HIP_CHECK(ctx, hipStreamSynchronize(s));
+    if (ms) RT_HIP_CHECK(ctx, hipEventElapsedTime(ms, ctx->ev[0], ctx->ev[1]));
+    return RT_OK;
+}
+
+namespace {
+
+// CRC-32 of PNG chunks (ISO 3309 polynomial 0xEDB88320, reflected).
+uint32_t crc32_update(uint32_t crc, const uint8_t *p, size_t n) {
+    static uint32_t table[256];
+    static bool ready = false;
+    if (!ready) {
+        for (uint32_t i = 0; i < 256; ++i) {
+            uint32_t c = i;
+            for (int k = 0; k < 8; ++k) c = (c & 1) ? 0xEDB88320u ^ (c >> 1) : c >> 1;
+            table[i] = c;
+        }
+        ready = true;
+    }
+    for (size_t i = 0; i < n; ++i) crc = table[(crc ^ p[i]) & 0xFFu] ^ (crc >> 8);
+    return crc;
+}
+
+void put_be32(std::vector<uint8_t> &v, uint32_t x) {
+    for (int k = 3; k >= 0; --k) v.push_back(static_cast<uint8_t>(x >> (8 * k)));
+}
+
+void png_chunk(std::vector<uint8_t> &file, const char *type, const std::vector<uint8_t> &data) {
+    put_be32(file, static_cast<uint32_t>(data.size()));
+    const size_t start = file.size();
+    file.insert(file.end(), type, type + 4);
+    file.insert(file.end(), data.begin(), data.end());
+    put_be32(file, crc32_update(0xFFFFFFFFu, file.data() + start, file.size() - start) ^ 0xFFFFFFFFu);
+}
+
+}  // namespace
+
+// write_png_file (rt/camera.cc:2158-2210): 8-bit RGB, the image's last row
+// first (row y of camera::image is y rows up from the bottom). The zlib
+// stream holds stored (uncompressed) deflate blocks: any PNG reader decodes
+// the same pixels as from libpng's compressed stream.
+int32_t write_png(const char *path, const uint8_t *rgb, int32_t x_res, int32_t y_res) {
+    const size_t row = static_cast<size_t>(x_res) * 3 + 1;
+    std::vector<uint8_t> raw;
+    raw.reserve(row * y_res);
+    for (int y = y_res - 1; y >= 0; --y) {
+        raw.push_back(0);  // filter: none
+        const uint8_t *r = rgb + static_cast<size_t>(y) * x_res * 3;
+        raw.insert(raw.end(), r, r + static_cast<size_t>(x_res) * 3);
+    }
+    std::vector<uint8_t> z = {0x78, 0x01};
+    uint32_t s1 = 1, s2 = 0;  // Adler-32
+    for (const uint8_t b : raw) {
+        s1 = (s1 + b) % 65521u;
+        s2 = (s2 + s1) % 65521u;
+    }
+    for (size_t off = 0; off < raw.size() || off == 0;) {
+        const size_t len = std::min<size_t>(65535, raw.size() - off);
+        const bool last = off + len == raw.size();
+        z.push_back(last ? 1 : 0);
+        z.push_back(static_cast<uint8_t>(len & 0xFF));
+        z.push_back(static_cast<uint8_t>(len >> 8));
+        z.push_back(static_cast<uint8_t>(~len & 0xFF));
+        z.push_back(static_cast<uint8_t>((~len >> 8) & 0xFF));
+        z.insert(z.end(), raw.begin() + off, raw.begin() + off + len);
+        off += len;
+        if (last) break;
+    }
+    put_be32(z, (s2 << 16) | s1);
+    std::vector<uint8_t> file = {0x89, 'P', 'N', 'G', '\r', '\n', 0x1A, '\n'};
+    std::vector<uint8_t> ihdr;
+    put_be32(ihdr, static_cast<uint32_t>(x_res));
+    put_be32(ihdr, static_cast<uint32_t>(y_res));
+    ihdr.insert(ihdr.end(), {8, 2, 0, 0, 0});  // 8-bit, RGB, deflate, no filter method, no interlace
+    png_chunk(file, "IHDR", ihdr);
+    png_chunk(file, "IDAT", z);
+    png_chunk(file, "IEND", {});
+    std::ofstream f(path, std::ios::binary);
+    if (!f) return RT_ERR_INVALID;
+    f.write(reinterpret_cast<const char *>(file.data()), static_cast<std::streamsize>(file.size()));
+    return f ? RT_OK : RT_ERR_INVALID;
+}
+
+// camera::write_tga_file (rt/camera.cc:2102-2155): 18-byte header of an
+// uncompressed 24-bit image with a bottom-left origin, the bgr bytes in
+// image order, eight zero bytes.
+int32_t write_tga(const char *path, const uint8_t *bgr, int32_t x_res, int32_t y_res) {
+    uint8_t h[18] = {0, 0, 2, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 24, 0};
+    h[12] = static_cast<uint8_t>(x_res & 0xFF);
+    h[13] = static_cast<uint8_t>((x_res >> 8) & 0xFF);
+    h[14] = static_cast<uint8_t>(y_res & 0xFF);
+    h[15] = static_cast<uint8_t>((y_res >> 8) & 0xFF);
+    std::ofstream f(path, std::ios::binary);
+    if (!f) return RT_ERR_INVALID;
+    f.write(reinterpret_cast<const char *>(h), sizeof(h));
+    f.write(reinterpret_cast<const char *>(bgr), static_cast<std::streamsize>(x_res) * y_res * 3);
+    const char footer[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    f.write(footer, sizeof(footer));
+    return f ? RT_OK : RT_ERR_INVALID;
+}
+
+}  // namespace rt

@@ -169,6 +169,14 @@ void free_frame_workspace(rt_accel *a);
 int32_t refit_accel(rt_accel *a);
 int32_t render_frame(rt_accel *a, const rt_camera &cam, const rt_light *lights, int nlights,
                      const rt_render_params &p, float *rgb, rt_hit *primary, rt_hit *bounce, rt_stats *stats);
+// Camera post-process (postprocess.hip).
+int32_t image_tone_map(rt_ctx *ctx, float *rgb, int32_t x_res, int32_t y_res, rt_tone_params *tp, uint32_t flags,
+                       float *ms);
+int32_t image_gamma(rt_ctx *ctx, float *rgb, int64_t npix, float gamma, uint32_t flags, float *ms);
+int32_t image_clip(rt_ctx *ctx, const float *rgb, int32_t x_res, int32_t y_res, int32_t xs, int32_t ys, int32_t bgr,
+                   uint8_t *out, uint32_t flags, float *ms);
+int32_t write_png(const char *path, const uint8_t *rgb, int32_t x_res, int32_t y_res);
+int32_t write_tga(const char *path, const uint8_t *bgr, int32_t x_res, int32_t y_res);
 // Frame of a scene with phong materials (materials.hip).
 int32_t render_frame_materials(rt_accel *a, const rt_camera &cam, const rt_light *lights, int nlights,
                                const rt_render_params &p, float *rgb, rt_hit *primary, rt_stats *stats);

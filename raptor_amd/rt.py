@@ -33,7 +33,8 @@ EXPORTS = [
     "rt_batch_create", "rt_batch_run", "rt_batch_destroy", "rt_batch_autotune", "rt_gen_surface_rays",
     "rt_scene_set_albedo", "rt_render_tape", "rt_tape_info", "rt_render_adjoint", "rt_tape_destroy",
     "rt_invrender_create", "rt_invrender_run", "rt_invrender_destroy", "rt_scene_update", "rt_accel_refit",
-    "rt_scene_set_materials",
+    "rt_scene_set_materials", "rt_image_tone_map", "rt_image_gamma", "rt_image_clip", "rt_write_png",
+    "rt_write_tga",
 ]
 
 
@@ -77,6 +78,12 @@ class rt_invrender_params(C.Structure):
     _fields_ = [("spp", C.c_int32), ("bounces", C.c_int32), ("seed", C.c_uint64), ("lr", C.c_float),
                 ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float), ("use_graph", C.c_int32),
                 ("waves_per_cu", C.c_int32)]
+
+
+class rt_tone_params(C.Structure):
+    _fields_ = [("mode", C.c_int32), ("key", C.c_float), ("xw", C.c_float), ("yw", C.c_float),
+                ("desaturate", C.c_int32), ("stretch", C.c_int32), ("adapt", C.c_int32), ("time_step", C.c_float),
+                ("adaptation_level", C.c_float), ("log_avg_luminance", C.c_float)]
 
 
 class rt_accel_info(C.Structure):
@@ -123,6 +130,11 @@ def load_library(path=LIB_PATH):
         "rt_scene_update": (i32, [vp, fp, u32]),
         "rt_accel_refit": (i32, [vp]),
         "rt_scene_set_materials": (i32, [vp, fp, i32, fp]),
+        "rt_image_tone_map": (i32, [vp, fp, i32, i32, C.POINTER(rt_tone_params), u32, C.POINTER(C.c_float)]),
+        "rt_image_gamma": (i32, [vp, fp, i64, C.c_float, u32, C.POINTER(C.c_float)]),
+        "rt_image_clip": (i32, [vp, fp, i32, i32, i32, i32, i32, fp, u32, C.POINTER(C.c_float)]),
+        "rt_write_png": (i32, [C.c_char_p, fp, i32, i32]),
+        "rt_write_tga": (i32, [C.c_char_p, fp, i32, i32]),
         "rt_render_tape": (i32, [vp, C.POINTER(rt_camera), C.POINTER(rt_light), i32, C.POINTER(rt_render_params), fp,
                                  i64, C.POINTER(vp), C.POINTER(rt_stats)]),
         "rt_tape_info": (i32, [vp, C.POINTER(i32), C.POINTER(i64), C.POINTER(i64)]),
@@ -551,3 +563,90 @@ class InvRender:
             self.close()
         except Exception:
             pass
+
+
+# ---------------------------------------------------------------- post-process
+# Camera post-process on images of 0-255 floats, (H, W, 3) with row y at
+# [y] (camera::image order): numpy arrays are processed through host staging,
+# torch tensors on the device in place.
+
+TONE_NONE, TONE_CONTRAST, TONE_NON_LINEAR, TONE_EXPOSURE, TONE_AVG_LUMINANCE, TONE_MAX_LUMINANCE, TONE_FERWERDA = \
+    0, 1, 4, 5, 6, 7, 9
+
+
+def _image_arg(img):
+    if isinstance(img, np.ndarray):
+        a = np.ascontiguousarray(img, np.float32)
+        return a, 0
+    if not img.is_contiguous():
+        raise ValueError("device images must be contiguous")
+    return img, RT_DEVICE_PTRS
+
+
+def tone_map(ctx, img, mode=TONE_NONE, key=0.18, xw=1.0 / 3.0, yw=1.0 / 3.0, desaturate=False, stretch=False,
+             adapt=False, time_step=0.0, adaptation_level=0.0):
+    """camera::tone_map (glare off). Returns (image, adaptation level,
+    log-average luminance, device ms); a numpy image is returned as a new
+    array, a device tensor is changed in place and returned."""
+    a, flags = _image_arg(img)
+    if flags == 0:
+        a = a.copy()
+    else:
+        import torch
+
+        torch.cuda.current_stream().synchronize()
+    h, w = a.shape[0], a.shape[1]
+    tp = rt_tone_params(int(mode), key, xw, yw, int(desaturate), int(stretch), int(adapt), time_step,
+                        adaptation_level, 0.0)
+    ms = C.c_float()
+    ctx.check(_LIB.rt_image_tone_map(ctx.h, _ptr(a), w, h, C.byref(tp), flags, C.byref(ms)), "rt_image_tone_map")
+    return a, tp.adaptation_level, tp.log_avg_luminance, ms.value
+
+
+def gamma_correct(ctx, img, gamma):
+    """camera::gamma_correct; returns (image, device ms) as tone_map does."""
+    a, flags = _image_arg(img)
+    if flags == 0:
+        a = a.copy()
+    else:
+        import torch
+
+        torch.cuda.current_stream().synchronize()
+    ms = C.c_float()
+    ctx.check(_LIB.rt_image_gamma(ctx.h, _ptr(a), a.numel() // 3 if flags else a.size // 3, gamma, flags,
+                                  C.byref(ms)), "rt_image_gamma")
+    return a, ms.value
+
+
+def clip(ctx, img, x_samples=1, y_samples=1, bgr=False, out=None):
+    """Down-sampled, clamped bytes (H / y_samples, W / x_samples, 3): a numpy
+    array for a numpy image; for a device image, out (a uint8 tensor) or a
+    new one. Returns (bytes, device ms)."""
+    a, flags = _image_arg(img)
+    h, w = a.shape[0], a.shape[1]
+    oh, ow = h // y_samples, w // x_samples
+    if flags == 0:
+        o = np.zeros((oh, ow, 3), np.uint8)
+    else:
+        import torch
+
+        o = out if out is not None else torch.empty((oh, ow, 3), dtype=torch.uint8, device=a.device)
+        torch.cuda.current_stream().synchronize()
+    ms = C.c_float()
+    ctx.check(_LIB.rt_image_clip(ctx.h, _ptr(a), w, h, x_samples, y_samples, int(bgr), _ptr(o), flags, C.byref(ms)),
+              "rt_image_clip")
+    return o, ms.value
+
+
+def write_png(path, rgb_bytes):
+    """rgb_bytes: (H, W, 3) uint8 in camera::image row order (row 0 at the
+    bottom of the picture)."""
+    b = np.ascontiguousarray(rgb_bytes, np.uint8)
+    if _LIB.rt_write_png(str(path).encode(), _ptr(b), b.shape[1], b.shape[0]) != RT_OK:
+        raise RTError(f"rt_write_png: cannot write {path}")
+
+
+def write_tga(path, bgr_bytes):
+    b = np.ascontiguousarray(bgr_bytes, np.uint8)
+    if _LIB.rt_write_tga(str(path).encode(), _ptr(b), b.shape[1], b.shape[0]) != RT_OK:
+        raise RTError(f"rt_write_tga: cannot write {path}")
