@@ -175,6 +175,11 @@ int32_t rt_accel_refit(rt_accel *accel);
  * unused. */
 #define RT_MATERIAL_PHONG 0
 #define RT_MATERIAL_COOK_TORRANCE 1
+/* kind RT_MATERIAL_COLOURED: a coloured_mapper_shader without texture maps
+ * (rt/materials/coloured_mapper_shader.{h,cc}), the material an OBJ/MTL file
+ * gives: the fields and shading of RT_MATERIAL_PHONG, with the wider ranges
+ * its constructor asserts (tran and rf in [0, 1]). */
+#define RT_MATERIAL_COLOURED 2
 typedef struct {
     float ka[3], kd[3], ks[3];
     float shininess, transmittance, refractive_index, reflectance;
@@ -202,6 +207,61 @@ typedef struct {
  * materials. */
 int32_t rt_scene_set_materials(rt_scene *scene, const rt_material *materials, int32_t nmaterials,
                                const int32_t *tri_material);
+
+/* Per-vertex normals for smooth shading: vn holds 9 floats per triangle
+ * (the normals at vertices a, b, c; host array). A triangle whose first
+ * value is FLT_MAX keeps its geometric normal, as a triangle built without
+ * vertex normals does (rt/triangle.h:139-167). rt_render then shades a hit
+ * with u * n_c + v * n_b + (1 - (u + v)) * n_a, unnormalised and turned to
+ * face the ray (rt/triangle.h:464-498), for light and secondary rays; shadow
+ * and secondary ray origins keep the geometric normal. vn == NULL removes
+ * them. Like materials, only rt_render reads them. */
+int32_t rt_scene_set_vertex_normals(rt_scene *scene, const float *vn);
+
+/* ---- Scene ingestion (host): OFF, PLY, OBJ + MTL files ----
+ * Replaces off_parser, ply_parser and obj_parser
+ * (rt/parsers/off_parser.cc:14-99, rt/parsers/ply_parser.cc:399-583,
+ * rt/parsers/obj_parser.cc:762-857) with polygon triangulation
+ * (rt/parsers/polygon_to_triangles.h:141-386): the triangles come out in the
+ * order those parsers emplace them (that order is the hit id), with one
+ * material per triangle (OFF: phong kd 170; PLY: phong kd 200, or the face
+ * colour; OBJ: phong kd 200 until a usemtl or mtllib, MTL materials as
+ * RT_MATERIAL_COLOURED) and, for OBJ faces with normals, vertex normals.
+ * Texture maps are counted, not applied. PLY may be ASCII or binary, big- or
+ * little-endian. Errors (malformed input, indices out of range) return a
+ * negative code with text in rt_mesh_last_error() (per thread). These calls
+ * need no GPU. */
+#define RT_FORMAT_OFF 0
+#define RT_FORMAT_PLY 1
+#define RT_FORMAT_OBJ 2
+typedef struct rt_mesh rt_mesh;
+typedef struct {
+    int64_t triangles;
+    int32_t materials;
+    int32_t has_vertex_normals;
+    int64_t vertices, faces;
+    int64_t skipped_faces;       /* degenerate triangles and faces below three vertices */
+    int64_t skipped_statements;  /* OBJ/MTL statements with nothing to shade */
+    int32_t texture_maps;        /* MTL materials whose texture maps are not applied */
+    int32_t reserved;
+} rt_mesh_info;
+/* format: RT_FORMAT_*, or -1 to take it from the file extension. */
+int32_t rt_mesh_load(const char *path, int32_t format, rt_mesh **out);
+/* Parses a file held in memory; dir is where an OBJ's mtllib files are
+ * (NULL: the working directory). */
+int32_t rt_mesh_parse(const char *data, int64_t len, int32_t format, const char *dir, rt_mesh **out);
+int32_t rt_mesh_info_get(const rt_mesh *mesh, rt_mesh_info *info);
+/* Copies out any of: abc (triangles x 9), tri_material (triangles),
+ * materials (materials), vertex_normals (triangles x 9, only when
+ * has_vertex_normals). NULL skips an array. */
+int32_t rt_mesh_copy(const rt_mesh *mesh, float *abc, int32_t *tri_material, rt_material *materials,
+                     float *vertex_normals);
+const char *rt_mesh_last_error(void);
+void rt_mesh_destroy(rt_mesh *mesh);
+/* A scene from a parsed mesh: rt_scene_create (albedo = kd / 255, flags bit1
+ * for transparent materials), rt_scene_set_materials and, when the mesh has
+ * them, rt_scene_set_vertex_normals. The mesh may be destroyed afterwards. */
+int32_t rt_scene_create_from_mesh(rt_ctx *ctx, const rt_mesh *mesh, rt_scene **out);
 
 /* ---- Camera post-process: the step after the image buffer ----
  * Images are camera::image as rt_render writes it: x_res * y_res * 3 floats

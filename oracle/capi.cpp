@@ -183,7 +183,7 @@ int32_t oracle_render(const oracle_scene *s, const float *albedo, const oracle_c
     // reference's packet camera has no jitter); pixel_stride strides tiles.
     if (p->packet && (p->spp > 1 || p->bounce > 1)) return -1;
     if (nlights < 0 || nlights > 8) return -2;
-    if (p->materials && (p->bounce != 0 || p->packet || out->rec_levels > 0)) return -3;
+    if ((p->materials || p->vertex_normals) && (p->bounce != 0 || p->packet || out->rec_levels > 0)) return -3;
     return render_frame(s->s, albedo, *cam, lights, nlights, *p, *out);
 }
 

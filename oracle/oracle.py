@@ -28,7 +28,7 @@ class oracle_light(C.Structure):
 class oracle_render_params(C.Structure):
     _fields_ = [("bounce", C.c_int32), ("spp", C.c_int32), ("seed", C.c_uint64), ("pixel_stride", C.c_int32),
                 ("nthreads", C.c_int32), ("packet", C.c_int32), ("materials", C.c_void_p),
-                ("nmaterials", C.c_int32), ("tri_material", C.c_void_p)]
+                ("nmaterials", C.c_int32), ("tri_material", C.c_void_p), ("vertex_normals", C.c_void_p)]
 
 
 class oracle_render_out(C.Structure):
@@ -205,7 +205,7 @@ class OracleScene:
         return occ, cnt
 
     def render(self, albedo, camera, lights, bounce=False, spp=1, seed=0, pixel_stride=1, nthreads=8, packet=False,
-               records=False, materials=None, tri_material=None):
+               records=False, materials=None, tri_material=None, vertex_normals=None):
         """One frame; bounce is the number of diffuse bounces (True = 1).
         records=True adds per-sample path records over the bounce + 1 path
         vertices, level-major (vertex k, sample p * spp + s): prim (L, S)
@@ -214,7 +214,8 @@ class OracleScene:
         material_array) with tri_material (one index per triangle, -1: the
         Lambert albedo default) renders phong materials with mirror and
         refraction rays; counters then gain sec_counters (secondary rays,
-        their shadow rays)."""
+        their shadow rays). vertex_normals (n, 9) float32 (FLT_MAX first:
+        none) shade with interpolated normals and select that frame too."""
         npix = camera.x_res * camera.y_res
         nl = len(lights)
         rgb = np.full((npix, 3), np.nan, np.float32)
@@ -244,6 +245,9 @@ class OracleScene:
                 np.ascontiguousarray(materials, np.float32).reshape(-1, 19)
             tm = np.ascontiguousarray(tri_material, np.int32)
             p.materials, p.nmaterials, p.tri_material = _p(mats), mats.shape[0], _p(tm)
+        if vertex_normals is not None:
+            vn = np.ascontiguousarray(vertex_normals, np.float32).reshape(-1, 9)
+            p.vertex_normals = _p(vn)
         alb = np.ascontiguousarray(albedo, np.float32)
         rc = LIB.oracle_render(self.h, _p(alb), C.byref(camera_c(camera)), lights_c(lights), nl, C.byref(p),
                                C.byref(out))

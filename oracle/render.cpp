@@ -9,6 +9,7 @@
 // threads with a per-thread traversal stack (the reference stack at
 // ssd/bvh.h:84 is a member and is not re-entrant).
 #include <atomic>
+#include <cfloat>
 #include <cmath>
 #include <thread>
 #include <vector>
@@ -428,7 +429,15 @@ Colour trace_phong(const FrameCtx &f, Ray &r, bool primary, LocalCounters *lc, P
     if (!(h.d < kMaxDist)) return c;
     const Triangle &tri = f.s->tris[prim];
     const bool leaving = dot(tri.n, r.dir) > 0.0f;  // hit_t::in_out (rt/triangle.h:487-496)
-    const SurfaceHit sh = surface_at(tri, &r, h.d);
+    SurfaceHit sh = surface_at(tri, &r, h.d);
+    if (f.p.vertex_normals && f.p.vertex_normals[9 * static_cast<int64_t>(prim)] != FLT_MAX) {
+        // triangle::normal_at_point (rt/triangle.h:474-477): the vertex
+        // normals weighted u (c), v (b), 1 - (u + v) (a), not renormalised.
+        const float *q = f.p.vertex_normals + 9 * static_cast<int64_t>(prim);
+        const V3 na(q[0], q[1], q[2]), nb(q[3], q[4], q[5]), nc(q[6], q[7], q[8]);
+        const V3 sn = ((nc * h.u) + (nb * h.v)) + (na * (1.0f - (h.u + h.v)));
+        sh.shade_norm = leaving ? -sn : sn;
+    }
     const Phong m = material_of(f, prim);
     const V3 &n = sh.shade_norm;
     const float nv = dot(n, r.dir);  // cook_torrance_cxy: one dot product for every light
@@ -877,7 +886,7 @@ int render_frame(const Scene &s, const float *albedo, const oracle_camera &cam, 
             for (int64_t k = b; k < b + chunk; ++k) {
                 const int64_t pix = k * stride;
                 if (pix >= npix) break;
-                if (p.materials)
+                if (p.materials || p.vertex_normals)
                     render_pixel_phong(f, static_cast<int>(pix % f.cam.x_res), static_cast<int>(pix / f.cam.x_res),
                                        &counters[tid], &pcounters[tid]);
                 else

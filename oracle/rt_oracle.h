@@ -56,6 +56,11 @@ typedef struct {
     const float *materials;
     int32_t nmaterials;
     const int32_t *tri_material;
+    /* Vertex normals, 9 floats per triangle (a, b, c); FLT_MAX as the first
+     * keeps the geometric normal (rt/triangle.h:139-167). Shading then uses
+     * u n_c + v n_b + (1 - (u + v)) n_a turned to face the ray
+     * (rt/triangle.h:464-498). Non-NULL selects the material frame. */
+    const float *vertex_normals;
 } oracle_render_params;
 
 /* Outputs; any pointer may be NULL. Arrays are indexed by pixel p = y*W + x.

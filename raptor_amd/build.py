@@ -15,7 +15,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libraptor_rt.so")
-SOURCES = ["api.hip", "bvh_build.hip", "trace.hip", "render.hip", "materials.hip", "postprocess.hip", "adjoint.hip", "invrender.hip", "batch.hip", "scene_gen.cpp"]
+SOURCES = ["api.hip", "bvh_build.hip", "trace.hip", "render.hip", "materials.hip", "postprocess.hip", "adjoint.hip", "invrender.hip", "batch.hip", "scene_gen.cpp", "ingest.cpp"]
 ARCH = os.environ.get("RAPTOR_ARCH", "gfx950")
 
 HIPCC_FLAGS = [
@@ -40,19 +40,45 @@ def hipcc():
     return path
 
 
-def build(verbose=False):
-    """Compile every HIP source into raptor_amd/libraptor_rt.so."""
+def build(verbose=False, jobs=None):
+    """Compile every HIP source into raptor_amd/libraptor_rt.so: one object
+    per source, compiled in parallel under _build/, then one link."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    objdir = os.path.join(os.path.dirname(HERE), "_build")
+    os.makedirs(objdir, exist_ok=True)
+    compile_flags = [f for f in HIPCC_FLAGS if f != "-shared"]
+
+    def compile_one(src):
+        obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
+        cmd = [hipcc()] + compile_flags + ["-c", os.path.join(CSRC, src), "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        return obj, res
+
+    jobs = jobs or min(len(SOURCES), max(1, min(8, os.cpu_count() or 1)))
+    with ThreadPoolExecutor(jobs) as pool:
+        results = list(pool.map(compile_one, SOURCES))
+    failed = [res for _, res in results if res.returncode != 0]
+    for _, res in results:
+        # hipcc's host pass does not know the packed-fp32 feature and says so.
+        text = "".join(line for line in (res.stdout + res.stderr).splitlines(True)
+                       if "packed-fp32-ops' is not a recognized feature" not in line)
+        if text.strip() and (verbose or res.returncode != 0):
+            sys.stderr.write(text)
+    if failed:
+        raise RuntimeError("hipcc failed building libraptor_rt.so")
     tmp = LIB + ".tmp"
-    cmd = [hipcc()] + HIPCC_FLAGS + ["-o", tmp] + [os.path.join(CSRC, s) for s in SOURCES]
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + [obj for obj, _ in results]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError("hipcc failed building libraptor_rt.so")
+        raise RuntimeError("hipcc failed linking libraptor_rt.so")
     os.replace(tmp, LIB)
     return LIB
-
 
 if __name__ == "__main__":
     print(build(verbose=True))

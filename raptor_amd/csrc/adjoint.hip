@@ -312,8 +312,9 @@ int32_t rt_render_tape(rt_accel *accel, const rt_camera *cam, const rt_light *li
     rt_ctx *ctx = accel->scene->ctx;
     RT_HIP_CHECK(ctx, hipSetDevice(ctx->device));
     if (accel->root == rt::kEmpty) return fail(ctx, RT_ERR_UNSUPPORTED, "rt_render_tape: empty scene");
-    if (accel->scene->d_mat)
-        return fail(ctx, RT_ERR_UNSUPPORTED, "rt_render_tape: scenes with phong materials are not differentiable");
+    if (accel->scene->d_mat || accel->scene->d_vn)
+        return fail(ctx, RT_ERR_UNSUPPORTED,
+                    "rt_render_tape: scenes with materials or vertex normals are not differentiable");
     rt_render_params p{0, 1, 0, 0, 0};
     if (params) p = *params;
     p.flags |= RT_DEVICE_PTRS;

@@ -101,6 +101,7 @@ void free_scene(rt_scene *sc) {
     if (sc->d_flags) (void)hipFree(sc->d_flags);
     if (sc->d_mat) (void)hipFree(sc->d_mat);
     if (sc->d_tri_mat) (void)hipFree(sc->d_tri_mat);
+    if (sc->d_vn) (void)hipFree(sc->d_vn);
     delete sc;
 }
 
@@ -334,12 +335,16 @@ int32_t rt_scene_set_materials(rt_scene *scene, const rt_material *materials, in
         const rt_material &q = materials[m];
         bool ok = true;
         std::string range;
-        if (q.kind == RT_MATERIAL_PHONG) {
-            ok = in(q.transmittance, 0.0f, 0.95f) && in(q.reflectance, 0.0f, 0.95f) &&
+        if (q.kind == RT_MATERIAL_PHONG || q.kind == RT_MATERIAL_COLOURED) {
+            // coloured_mapper_shader asserts tran and rf up to 1
+            // (rt/materials/coloured_mapper_shader.h:26-40).
+            const float top = (q.kind == RT_MATERIAL_PHONG) ? 0.95f : 1.0f;
+            ok = in(q.transmittance, 0.0f, top) && in(q.reflectance, 0.0f, top) &&
                  (q.transmittance == 0.0f || in(q.refractive_index, 1.0f, 5.0f));
             for (int k = 0; k < 3; ++k)
                 for (const float v : {q.ka[k], q.kd[k], q.ks[k]}) ok = ok && in(v, 0.0f, 255.0f);
-            range = "colours 0..255, tran and rf 0..0.95, ri 1..5 when tran > 0";
+            range = (q.kind == RT_MATERIAL_PHONG) ? "colours 0..255, tran and rf 0..0.95, ri 1..5 when tran > 0"
+                                                  : "colours 0..255, tran and rf 0..1, ri 1..5 when tran > 0";
         } else if (q.kind == RT_MATERIAL_COOK_TORRANCE) {
             ok = in(q.chroma[0], 0.0f, 1.0f) && in(q.chroma[1], 0.0f, 1.0f) && in(q.diffuse, 0.0f, 1.0f) &&
                  in(q.reflectance, 0.0f, 1.0f) && in(q.transmittance, 0.0f, 1.0f) && in(q.roughness, 0.0f, 0.5f) &&
@@ -394,6 +399,29 @@ int32_t rt_scene_set_materials(rt_scene *scene, const rt_material *materials, in
     scene->d_mat = d_mat;
     scene->d_tri_mat = d_tri;
     scene->nmat = nmaterials;
+    return RT_OK;
+}
+
+int32_t rt_scene_set_vertex_normals(rt_scene *scene, const float *vn) {
+    if (!scene) return RT_ERR_INVALID;
+    rt_ctx *ctx = scene->ctx;
+    if (int32_t rc = rt::set_device(ctx)) return rc;
+    hipStream_t s = ctx->stream;
+    if (scene->d_vn) {
+        RT_HIP_CHECK(ctx, hipStreamSynchronize(s));
+        (void)hipFree(scene->d_vn);
+        scene->d_vn = nullptr;
+    }
+    if (!vn || scene->n == 0) return RT_OK;
+    float *d = nullptr;
+    hipError_t e = rt::upload(&d, vn, 9 * scene->n, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        if (d) (void)hipFree(d);
+        return fail(ctx, e == hipErrorOutOfMemory ? RT_ERR_OOM : RT_ERR_HIP,
+                    std::string("rt_scene_set_vertex_normals: ") + hipGetErrorString(e));
+    }
+    scene->d_vn = d;
     return RT_OK;
 }
 
@@ -472,8 +500,9 @@ int32_t rt_render(rt_accel *accel, const rt_camera *cam, const rt_light *lights,
     if (ctx->counting)
         RT_HIP_CHECK(ctx, hipMemsetAsync(ctx->d_counters, 0, 16 * sizeof(unsigned long long), ctx->stream));
     int32_t rc = RT_OK;
-    if (accel->scene->d_mat) {
-        rc = bounce ? fail(ctx, RT_ERR_UNSUPPORTED, "rt_render: the diffuse bounce is not defined with materials")
+    if (accel->scene->d_mat || accel->scene->d_vn) {
+        rc = bounce ? fail(ctx, RT_ERR_UNSUPPORTED,
+                           "rt_render: the diffuse bounce is not defined with materials or vertex normals")
                     : rt::render_frame_materials(accel, *cam, lights, nlights, p, rgb, primary, stats);
     } else {
         rc = rt::render_frame(accel, *cam, lights, nlights, p, rgb, primary, bounce, stats);

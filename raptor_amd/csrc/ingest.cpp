@@ -1,0 +1,924 @@
+// Scene ingestion: OFF, PLY and OBJ (+ MTL) files into triangles in emplace
+// order, one material per triangle and optional per-vertex normals, ready for
+// rt_scene_create_from_mesh.
+//
+// The triangles each parser emits, and their order, follow the reference's
+// parsers (rt/parsers/off_parser.cc:14-99, rt/parsers/ply_parser.cc:14-583,
+// rt/parsers/obj_parser.cc:19-857) and polygon triangulation
+// (rt/parsers/polygon_to_triangles.h:50-386), because the emplace order is the
+// hit id. Numbers are read as the reference's atof/atoi read them (a double
+// narrowed to float; a decimal integer), and every float expression of the
+// triangulation is evaluated in the reference's operand order (this file is
+// built without FMA contraction).
+//
+// Input the reference asserts on, or misreads, is an error here with a
+// message (rt_mesh_last_error) instead: a header that is not OFF/PLY, a PLY
+// property type other than uchar/uint8/float/float32, an ASCII PLY face with
+// colour or intensity, an index out of range, a face with a material-less
+// OBJ material library. Faces of fewer than three vertices give no triangle
+// (the reference loops forever or reads out of range on them). Tokens may be
+// separated by any blanks.
+#include <cerrno>
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <memory>
+#include <new>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/raptor_rt.h"
+
+namespace {
+
+thread_local std::string g_error;
+
+struct P3 {
+    float x = 0.0f, y = 0.0f, z = 0.0f;
+};
+
+inline P3 sub(const P3 &a, const P3 &b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline P3 cross(const P3 &a, const P3 &b) {
+    return {(a.y * b.z) - (a.z * b.y), (a.z * b.x) - (a.x * b.z), (a.x * b.y) - (a.y * b.x)};
+}
+inline float dot(const P3 &a, const P3 &b) { return ((a.x * b.x) + (a.y * b.y)) + (a.z * b.z); }
+inline float magnitude(const P3 &a) { return std::sqrt(((a.x * a.x) + (a.y * a.y)) + (a.z * a.z)); }
+// normalise<float> under EXACT_NORMALISE (common/point_t.h:553-569).
+inline P3 normalised(const P3 &a) {
+    const float d = std::sqrt(((a.x * a.x) + (a.y * a.y)) + (a.z * a.z));
+    return {a.x / d, a.y / d, a.z / d};
+}
+inline bool same_point(const P3 &a, const P3 &b) { return a.x == b.x && a.y == b.y && a.z == b.z; }
+inline bool all_zero(const P3 &a) { return a.x == 0.0f && a.y == 0.0f && a.z == 0.0f; }
+
+rt_material phong(float ka, float kd, float ks) {
+    rt_material m{};
+    for (int k = 0; k < 3; ++k) {
+        m.ka[k] = ka;
+        m.kd[k] = kd;
+        m.ks[k] = ks;
+    }
+    m.refractive_index = 1.0f;
+    m.kind = RT_MATERIAL_PHONG;
+    m.chroma[0] = m.chroma[1] = 0.33f;
+    return m;
+}
+
+struct Mesh {
+    std::vector<float> abc;        // 9 floats per triangle, emplace order
+    std::vector<int32_t> tri_mat;  // material of each triangle
+    std::vector<rt_material> mats;
+    std::vector<float> vn;         // 9 floats per triangle once any triangle has vertex normals
+    bool any_vn = false;
+    int64_t vertices = 0, faces = 0, skipped_faces = 0, skipped_statements = 0;
+    int32_t texture_maps = 0;
+
+    void add(const P3 &a, const P3 &b, const P3 &c, int32_t mat, const P3 *n) {
+        const int64_t i = static_cast<int64_t>(tri_mat.size());
+        for (const P3 *p : {&a, &b, &c}) {
+            abc.push_back(p->x);
+            abc.push_back(p->y);
+            abc.push_back(p->z);
+        }
+        tri_mat.push_back(mat);
+        if (n && !any_vn) {
+            // Earlier triangles carry no normals: the reference marks them
+            // with x = MAX_DIST (rt/triangle.h:153-156).
+            any_vn = true;
+            vn.assign(static_cast<size_t>(9 * i), 0.0f);
+            for (int64_t k = 0; k < i; ++k) vn[9 * k] = FLT_MAX;
+        }
+        if (any_vn) {
+            for (int k = 0; k < 3; ++k) {
+                vn.push_back(n ? n[k].x : (k == 0 ? FLT_MAX : 0.0f));
+                vn.push_back(n ? n[k].y : 0.0f);
+                vn.push_back(n ? n[k].z : 0.0f);
+            }
+        }
+    }
+};
+
+struct ParseError {
+    std::string what;
+};
+[[noreturn]] void bad(const std::string &what) { throw ParseError{what}; }
+
+// ---- Polygon triangulation (ear clipping) ----
+//
+// rt/parsers/polygon_to_triangles.h:50-386. Starting at the vertex furthest
+// from the centroid, the face normal is taken from that corner; the walk then
+// tries the ear (prev, cur, next): a straight corner is dropped without a
+// triangle, an ear facing the face normal with no polygon vertex strictly
+// inside becomes a triangle and its middle vertex is dropped, any other
+// corner is passed over. The walk ends when two vertices remain or when it
+// has passed over as many corners in a row as vertices remain.
+bool point_in_ear(const std::vector<P3> &p, const P3 &a, const P3 &b, const P3 &c, const P3 &fn) {
+    const P3 ab = sub(a, b), ca = sub(c, a), bc = sub(b, c);
+    auto on_inner_side = [&](const P3 &q, const P3 &corner, const P3 &edge) {
+        return dot(cross(sub(q, corner), edge), fn) > 0.0f;
+    };
+    for (const P3 &q : p)
+        if (on_inner_side(q, a, ab) && on_inner_side(q, b, bc) && on_inner_side(q, c, ca)) return true;
+    return false;
+}
+
+// Emits index triples (prev, cur, next) into tris.
+void triangulate(const std::vector<P3> &p, std::vector<uint32_t> *tris) {
+    const uint32_t n = static_cast<uint32_t>(p.size());
+    if (n < 3) return;
+    P3 centre;
+    for (const P3 &q : p) {
+        centre.x += q.x;
+        centre.y += q.y;
+        centre.z += q.z;
+    }
+    const float fn_count = static_cast<float>(n);
+    centre = {centre.x / fn_count, centre.y / fn_count, centre.z / fn_count};
+    uint32_t cur = 0;
+    float far = 0.0f;
+    for (uint32_t i = 0; i < n; ++i) {
+        const float d = magnitude(sub(centre, p[i]));
+        if (d > far) {
+            far = d;
+            cur = i;
+        }
+    }
+    uint32_t prev = (cur == 0) ? n - 1 : cur - 1;
+    uint32_t next = (cur == n - 1) ? 0 : cur + 1;
+    const P3 fn = normalised(cross(sub(p[prev], p[cur]), sub(p[cur], p[next])));
+    std::vector<char> gone(n, 0);
+    uint32_t left = n, passed = 0;
+    for (;;) {
+        const P3 &a = p[prev], &b = p[cur], &c = p[next];
+        bool drop = true;
+        if (!all_zero(cross(sub(b, a), sub(c, a)))) {
+            const P3 tn = normalised(cross(sub(a, b), sub(b, c)));
+            if (dot(tn, fn) > 0.0f && !point_in_ear(p, a, b, c, fn)) {
+                tris->push_back(prev);
+                tris->push_back(cur);
+                tris->push_back(next);
+            } else {
+                drop = false;
+            }
+        }
+        if (drop) {
+            --left;
+            passed = 0;
+            gone[cur] = 1;
+        } else {
+            prev = cur;
+            ++passed;
+        }
+        cur = next;
+        do {
+            next = (next + 1 == n) ? 0 : next + 1;
+        } while (gone[next]);
+        if (left == 2 || passed == left) break;
+    }
+}
+
+// A polygon of the face's vertices (and normals, when given) as triangles.
+void add_polygon(Mesh *m, const std::vector<P3> &face, const std::vector<P3> *normals, int32_t mat) {
+    if (face.size() < 3) {
+        ++m->skipped_faces;
+        return;
+    }
+    std::vector<uint32_t> tris;
+    triangulate(face, &tris);
+    for (size_t k = 0; k < tris.size(); k += 3) {
+        P3 n3[3];
+        const bool with_n = normals && !normals->empty();
+        if (with_n)
+            for (int j = 0; j < 3; ++j) n3[j] = (*normals)[tris[k + j]];
+        m->add(face[tris[k]], face[tris[k + 1]], face[tris[k + 2]], mat, with_n ? n3 : nullptr);
+    }
+}
+
+// ---- Lexing ----
+
+struct Cursor {
+    const char *p, *end;
+    bool at_end() const { return p >= end; }
+};
+
+inline bool blank(char c) { return c == ' ' || c == '\t' || c == '\r'; }
+
+// The next line, without its terminator; false at the end of the input.
+bool next_line(Cursor *c, const char **b, const char **e) {
+    if (c->at_end()) return false;
+    const char *s = c->p;
+    const char *nl = static_cast<const char *>(std::memchr(s, '\n', static_cast<size_t>(c->end - s)));
+    const char *stop = nl ? nl : c->end;
+    c->p = nl ? nl + 1 : c->end;
+    *b = s;
+    *e = stop;
+    return true;
+}
+
+// Whitespace-separated tokens of one line.
+struct Tokens {
+    const char *p, *e;
+    bool next(std::string *t) {
+        while (p < e && blank(*p)) ++p;
+        if (p >= e) return false;
+        const char *s = p;
+        while (p < e && !blank(*p)) ++p;
+        t->assign(s, static_cast<size_t>(p - s));
+        return true;
+    }
+};
+
+// atof: the decimal read as a double, then narrowed to float.
+float to_float(const std::string &t, const char *what) {
+    char *endp = nullptr;
+    const double v = std::strtod(t.c_str(), &endp);
+    if (endp == t.c_str()) bad(std::string("expected a number for ") + what + ", found '" + t + "'");
+    return static_cast<float>(v);
+}
+
+long to_int(const std::string &t, const char *what) {
+    char *endp = nullptr;
+    errno = 0;
+    const long v = std::strtol(t.c_str(), &endp, 10);
+    if (endp == t.c_str() || errno == ERANGE) bad(std::string("expected an integer for ") + what + ", found '" + t + "'");
+    return v;
+}
+
+float next_float(Tokens *tk, const char *what) {
+    std::string t;
+    if (!tk->next(&t)) bad(std::string("missing ") + what);
+    return to_float(t, what);
+}
+
+// ---- OFF (rt/parsers/off_parser.cc:14-99) ----
+// Material phong_shader(ka 0, kd 170, ks 0): every triangle shares it.
+void parse_off(const char *data, int64_t len, Mesh *m) {
+    Cursor c{data, data + len};
+    const char *b, *e;
+    std::vector<std::string> head;
+    std::string t;
+    // "OFF", then the vertex, face and edge counts (possibly on the same line).
+    while (head.size() < 4 && next_line(&c, &b, &e)) {
+        Tokens tk{b, e};
+        while (head.size() < 4 && tk.next(&t)) head.push_back(t);
+    }
+    if (head.empty() || head[0] != "OFF") bad("not an OFF file (header '" + (head.empty() ? std::string() : head[0]) + "')");
+    if (head.size() < 4) bad("OFF header lacks the vertex, face and edge counts");
+    const long nv = to_int(head[1], "the vertex count"), nf = to_int(head[2], "the face count");
+    if (nv < 0 || nf < 0) bad("negative OFF counts");
+    m->mats.push_back(phong(0.0f, 170.0f, 0.0f));
+    m->mats.back().refractive_index = 0.0f;
+    std::vector<P3> v;
+    v.reserve(static_cast<size_t>(nv));
+    for (long i = 0; i < nv; ++i) {
+        if (!next_line(&c, &b, &e)) bad("OFF file ends inside the vertex list");
+        Tokens tk{b, e};
+        P3 q;
+        q.x = next_float(&tk, "a vertex x");
+        q.y = next_float(&tk, "a vertex y");
+        q.z = next_float(&tk, "a vertex z");
+        v.push_back(q);
+    }
+    m->vertices = nv;
+    std::vector<P3> face;
+    for (long i = 0; i < nf; ++i) {
+        if (!next_line(&c, &b, &e)) bad("OFF file ends inside the face list");
+        Tokens tk{b, e};
+        if (!tk.next(&t)) bad("empty OFF face line");
+        const long k = to_int(t, "a face size");
+        face.clear();
+        for (long j = 0; j < k; ++j) {
+            if (!tk.next(&t)) bad("OFF face line shorter than its vertex count");
+            const long idx = to_int(t, "a vertex index");
+            if (idx < 0 || idx >= nv) bad("OFF vertex index " + std::to_string(idx) + " out of range");
+            face.push_back(v[static_cast<size_t>(idx)]);
+        }
+        ++m->faces;
+        // A triangle is taken as it is; larger faces are triangulated.
+        if (face.size() == 3) {
+            m->add(face[0], face[1], face[2], 0, nullptr);
+        } else {
+            add_polygon(m, face, nullptr, 0);
+        }
+    }
+}
+
+// The PLY and OBJ triangle filters: a triangle whose corners coincide or
+// whose edges are parallel gives no triangle.
+bool ply_triangle_ok(const P3 &a0, const P3 &b0, const P3 &c0) {
+    if (same_point(a0, b0) || same_point(a0, c0) || same_point(b0, c0)) return false;
+    const P3 a = normalised(sub(a0, b0)), b = normalised(sub(b0, c0));
+    const P3 diff = sub(a, b);
+    const P3 sum{a.x + b.x, a.y + b.y, a.z + b.z};
+    return !all_zero(diff) && !all_zero(sum);
+}
+
+bool obj_triangle_ok(const P3 &a0, const P3 &b0, const P3 &c0) {
+    if (same_point(a0, b0) || same_point(a0, c0) || same_point(b0, c0)) return false;
+    const P3 a = normalised(sub(a0, b0)), b = normalised(sub(b0, c0));
+    const P3 n = cross(a, b);
+    return dot(n, n) > 0.0f;
+}
+
+// ---- PLY (rt/parsers/ply_parser.cc:14-583) ----
+// ASCII or binary (big-endian as in the reference; little-endian accepted
+// too). Vertices: float x, y, z first, then any uchar/float properties.
+// Faces: an optional uchar intensity, the uchar-counted int index list, then
+// uchar properties, of which a red, green, blue run gives the face a colour
+// read from the three bytes after the indices. A colour becomes
+// phong_shader(0, rgb, 0, 0); faces without one share kd 200.
+struct ByteReader {
+    const unsigned char *p, *end;
+    bool big;
+    void need(size_t n) const {
+        if (static_cast<size_t>(end - p) < n) bad("PLY binary data ends early");
+    }
+    uint32_t u32() {
+        need(4);
+        const uint32_t v = big ? (uint32_t(p[0]) << 24) | (uint32_t(p[1]) << 16) | (uint32_t(p[2]) << 8) | p[3]
+                               : (uint32_t(p[3]) << 24) | (uint32_t(p[2]) << 16) | (uint32_t(p[1]) << 8) | p[0];
+        p += 4;
+        return v;
+    }
+    float f32() {
+        const uint32_t u = u32();
+        float f;
+        std::memcpy(&f, &u, 4);
+        return f;
+    }
+    unsigned char u8() {
+        need(1);
+        return *p++;
+    }
+    void skip(size_t n) {
+        need(n);
+        p += n;
+    }
+};
+
+bool starts(const std::string &s, const char *pre) { return s.compare(0, std::strlen(pre), pre) == 0; }
+
+int prop_size(const std::string &type) {
+    if (type == "uchar" || type == "uint8") return 1;
+    if (type == "float" || type == "float32") return 4;
+    return 0;
+}
+
+void parse_ply(const char *data, int64_t len, Mesh *m) {
+    Cursor c{data, data + len};
+    const char *b, *e;
+    std::vector<std::vector<std::string>> header;
+    bool ended = false;
+    while (next_line(&c, &b, &e)) {
+        Tokens tk{b, e};
+        std::vector<std::string> words;
+        std::string t;
+        while (tk.next(&t)) words.push_back(t);
+        if (words.empty()) continue;
+        if (words[0] == "comment" || words[0] == "obj_info") continue;
+        if (words[0] == "end_header") {
+            ended = true;
+            break;
+        }
+        header.push_back(words);
+    }
+    if (header.empty() || header[0][0] != "ply") bad("not a PLY file");
+    if (!ended) bad("PLY header has no end_header");
+    size_t h = 1;
+    if (h >= header.size() || header[h][0] != "format" || header[h].size() < 3) bad("PLY header lacks its format");
+    const std::string fmt = header[h][1];
+    const bool binary = fmt != "ascii";
+    if (fmt != "ascii" && fmt != "binary_big_endian" && fmt != "binary_little_endian")
+        bad("unknown PLY format " + fmt);
+    ++h;
+    // element vertex N: float x, y, z, then extra bytes.
+    if (h >= header.size() || header[h].size() < 3 || header[h][0] != "element" || header[h][1] != "vertex")
+        bad("PLY header must declare the vertex element first");
+    const long nv = to_int(header[h][2], "the vertex count");
+    ++h;
+    for (const char *axis : {"x", "y", "z"}) {
+        if (h >= header.size() || header[h].size() != 3 || header[h][0] != "property" ||
+            (header[h][1] != "float" && header[h][1] != "float32") || header[h][2] != axis)
+            bad(std::string("PLY vertex element must start with float ") + axis);
+        ++h;
+    }
+    size_t v_extra = 0;
+    for (; h < header.size() && header[h][0] == "property"; ++h) {
+        const int s = header[h].size() == 3 ? prop_size(header[h][1]) : 0;
+        if (!s) bad("unsupported PLY vertex property '" + header[h][1] + "'");
+        v_extra += static_cast<size_t>(s);
+    }
+    // element face N: [uchar intensity] list uchar int vertex_indices [uchar ...]
+    if (h >= header.size() || header[h].size() < 3 || header[h][0] != "element" || header[h][1] != "face")
+        bad("PLY header must declare the face element after the vertices");
+    const long nf = to_int(header[h][2], "the face count");
+    ++h;
+    size_t pre = 0;
+    if (h < header.size() && header[h].size() == 3 && header[h][0] == "property" && prop_size(header[h][1]) == 1 &&
+        header[h][2] == "intensity") {
+        pre = 1;
+        ++h;
+    }
+    if (h >= header.size() || header[h].size() != 5 || header[h][0] != "property" || header[h][1] != "list" ||
+        prop_size(header[h][2]) != 1 || (header[h][3] != "int" && header[h][3] != "int32") ||
+        header[h][4] != "vertex_indices")
+        bad("PLY faces must be a 'list uchar int vertex_indices'");
+    ++h;
+    std::vector<std::string> face_props;
+    for (; h < header.size() && header[h][0] == "property"; ++h) {
+        if (header[h].size() != 3 || prop_size(header[h][1]) != 1)
+            bad("unsupported PLY face property '" + header[h][1] + "'");
+        face_props.push_back(header[h][2]);
+    }
+    if (h != header.size()) bad("unsupported PLY element '" + header[h][0] + " " + (header[h].size() > 1 ? header[h][1] : "") + "'");
+    bool colour = false;
+    size_t post = face_props.size();
+    for (size_t k = 0; k + 2 < face_props.size(); ++k)
+        if (face_props[k] == "red" && face_props[k + 1] == "green" && face_props[k + 2] == "blue") {
+            colour = true;
+            post -= 3;
+            break;
+        }
+    if (nv < 0 || nf < 0) bad("negative PLY counts");
+
+    m->mats.push_back(phong(0.0f, 200.0f, 0.0f));
+    // Face colours keyed as the reference keys them: the three bytes read as
+    // signed chars, c0 + (c1 << 8) + (c2 << 16); the default material has key
+    // 0xdefa, so a colour with that key shades with the default.
+    std::map<int, int32_t> by_key;
+    by_key[0xdefa] = 0;
+    auto colour_material = [&](const unsigned char rgb[3]) -> int32_t {
+        const int key = static_cast<int>(static_cast<signed char>(rgb[0])) +
+                        (static_cast<int>(static_cast<signed char>(rgb[1])) * 256) +
+                        (static_cast<int>(static_cast<signed char>(rgb[2])) * 65536);
+        auto it = by_key.find(key);
+        if (it != by_key.end()) return it->second;
+        rt_material q = phong(0.0f, 0.0f, 0.0f);
+        for (int k = 0; k < 3; ++k) q.kd[k] = static_cast<float>(rgb[k]);
+        m->mats.push_back(q);
+        const int32_t id = static_cast<int32_t>(m->mats.size() - 1);
+        by_key[key] = id;
+        return id;
+    };
+
+    std::vector<P3> v;
+    v.reserve(static_cast<size_t>(nv));
+    std::vector<P3> face;
+    auto emit = [&](int32_t mat) {
+        ++m->faces;
+        if (face.size() == 3) {
+            if (ply_triangle_ok(face[0], face[1], face[2])) {
+                m->add(face[0], face[1], face[2], mat, nullptr);
+            } else {
+                ++m->skipped_faces;
+            }
+        } else {
+            add_polygon(m, face, nullptr, mat);
+        }
+    };
+    if (binary) {
+        ByteReader r{reinterpret_cast<const unsigned char *>(c.p), reinterpret_cast<const unsigned char *>(c.end),
+                     fmt == "binary_big_endian"};
+        for (long i = 0; i < nv; ++i) {
+            P3 q;
+            q.x = r.f32();
+            q.y = r.f32();
+            q.z = r.f32();
+            r.skip(v_extra);
+            v.push_back(q);
+        }
+        for (long i = 0; i < nf; ++i) {
+            r.skip(pre);
+            const unsigned k = r.u8();
+            face.clear();
+            for (unsigned j = 0; j < k; ++j) {
+                const int32_t idx = static_cast<int32_t>(r.u32());
+                if (idx < 0 || idx >= nv) bad("PLY vertex index " + std::to_string(idx) + " out of range");
+                face.push_back(v[static_cast<size_t>(idx)]);
+            }
+            int32_t mat = 0;
+            if (colour) {
+                unsigned char rgb[3];
+                for (auto &x : rgb) x = r.u8();
+                mat = colour_material(rgb);
+            }
+            r.skip(post);
+            emit(mat);
+        }
+    } else {
+        if (colour || pre) bad("ASCII PLY faces with colour or intensity are not supported");
+        std::string t;
+        for (long i = 0; i < nv; ++i) {
+            if (!next_line(&c, &b, &e)) bad("PLY file ends inside the vertex list");
+            Tokens tk{b, e};
+            P3 q;
+            q.x = next_float(&tk, "a vertex x");
+            q.y = next_float(&tk, "a vertex y");
+            q.z = next_float(&tk, "a vertex z");
+            v.push_back(q);
+        }
+        for (long i = 0; i < nf; ++i) {
+            if (!next_line(&c, &b, &e)) bad("PLY file ends inside the face list");
+            Tokens tk{b, e};
+            if (!tk.next(&t)) bad("empty PLY face line");
+            const long k = to_int(t, "a face size");
+            face.clear();
+            for (long j = 0; j < k; ++j) {
+                if (!tk.next(&t)) bad("PLY face line shorter than its vertex count");
+                const long idx = to_int(t, "a vertex index");
+                if (idx < 0 || idx >= nv) bad("PLY vertex index " + std::to_string(idx) + " out of range");
+                face.push_back(v[static_cast<size_t>(idx)]);
+            }
+            emit(0);
+        }
+    }
+    m->vertices = nv;
+}
+
+// ---- OBJ and MTL (rt/parsers/obj_parser.cc:19-857) ----
+
+struct MtlState {
+    P3 ka, kd, ks;
+    float ns = 0.0f, ri = 1.0f, rf = 0.0f, trans = 0.0f;
+    int illum = 0;  // not reset by newmtl (obj_parser.cc:380-396)
+    bool textured = false;
+};
+
+// An MTL colour: one value (grey) or three, scaled to 0..255.
+P3 mtl_colour(Tokens *tk) {
+    std::vector<float> v;
+    std::string t;
+    while (v.size() < 3 && tk->next(&t)) v.push_back(to_float(t, "a colour") * 255.0f);
+    if (v.empty()) bad("MTL colour without a value");
+    if (v.size() < 3) return {v[0], v[0], v[0]};
+    return {v[0], v[1], v[2]};
+}
+
+// The material a finished newmtl block describes (obj_parser.cc:290-378):
+// the illumination model zeroes or fixes terms, transmittance is clamped to
+// [0, 1], the ambient colour is halved; coloured_mapper_shader without
+// texture maps shades as phong_shader.
+rt_material finish_mtl(MtlState s) {
+    switch (s.illum) {
+        case 0:
+            s.ka = s.kd;
+            s.kd = P3{};
+            s.ks = P3{};
+            s.ns = 0.0f;
+            s.ri = 1.0f;
+            s.rf = 0.0f;
+            break;
+        case 1:
+            s.ks = P3{};
+            s.ns = 0.0f;
+            s.ri = 1.0f;
+            s.rf = 0.0f;
+            break;
+        case 2:
+            s.ri = 1.0f;
+            s.rf = 0.0f;
+            break;
+        case 3:
+        case 5:
+        case 7:
+            s.ri = 1.0f;
+            s.rf = 0.75f;
+            break;
+        case 4:
+            s.rf = 0.0f;
+            break;
+        default:
+            break;
+    }
+    s.trans = std::max(0.0f, std::min(1.0f, s.trans));
+    rt_material m{};
+    const float ambient = 0.5f;
+    m.ka[0] = s.ka.x * ambient;
+    m.ka[1] = s.ka.y * ambient;
+    m.ka[2] = s.ka.z * ambient;
+    m.kd[0] = s.kd.x;
+    m.kd[1] = s.kd.y;
+    m.kd[2] = s.kd.z;
+    m.ks[0] = s.ks.x;
+    m.ks[1] = s.ks.y;
+    m.ks[2] = s.ks.z;
+    m.shininess = s.ns;
+    m.transmittance = s.trans;
+    m.refractive_index = s.ri;
+    m.reflectance = s.rf;
+    m.kind = RT_MATERIAL_COLOURED;
+    m.chroma[0] = m.chroma[1] = 0.33f;
+    return m;
+}
+
+std::string read_file(const std::string &path) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) bad("cannot open " + path);
+    std::ostringstream ss;
+    ss << f.rdbuf();
+    return ss.str();
+}
+
+// Parses one MTL file into named materials; returns the id of the last one
+// defined (which the reference makes current for the faces that follow), or
+// -1 when the file defines none.
+int32_t parse_mtl(const std::string &text, Mesh *m, std::map<std::string, int32_t> *named) {
+    Cursor c{text.data(), text.data() + text.size()};
+    const char *b, *e;
+    std::string name, t;
+    MtlState s;
+    int illum = 0;
+    int32_t last = -1;
+    auto close = [&]() {
+        if (name.empty()) return;
+        s.illum = illum;
+        m->mats.push_back(finish_mtl(s));
+        last = static_cast<int32_t>(m->mats.size() - 1);
+        (*named)[name] = last;
+        if (s.textured) ++m->texture_maps;
+    };
+    while (next_line(&c, &b, &e)) {
+        Tokens tk{b, e};
+        if (!tk.next(&t) || t[0] == '#') continue;
+        if (starts(t, "newmtl")) {
+            close();
+            if (!tk.next(&name)) bad("newmtl without a name");
+            s = MtlState{};
+        } else if (starts(t, "Ka")) {
+            s.ka = mtl_colour(&tk);
+        } else if (starts(t, "Kd")) {
+            s.kd = mtl_colour(&tk);
+        } else if (starts(t, "Ks")) {
+            s.ks = mtl_colour(&tk);
+        } else if (starts(t, "Tf") || starts(t, "Ke")) {
+            // transmitted and emitted colours are not shaded
+        } else if (starts(t, "Ns")) {
+            s.ns = next_float(&tk, "Ns");
+        } else if (starts(t, "Ni")) {
+            s.ri = next_float(&tk, "Ni");
+        } else if (t[0] == 'd') {
+            // Every statement starting with 'd' is read as the dissolve.
+            std::string v;
+            s.trans = 1.0f - (tk.next(&v) ? static_cast<float>(std::atof(v.c_str())) : 0.0f);
+        } else if (starts(t, "Tr")) {
+            s.trans = next_float(&tk, "Tr");
+        } else if (starts(t, "illum")) {
+            std::string v;
+            illum = tk.next(&v) ? std::atoi(v.c_str()) : 0;
+        } else if (starts(t, "map_Ka") || starts(t, "map_Kd") || starts(t, "map_Ks") || starts(t, "map_refl") ||
+                   starts(t, "map_d")) {
+            // Texture maps are not applied; the material shades untextured.
+            s.textured = true;
+        } else if (starts(t, "map_bump") || starts(t, "bump")) {
+            // bump maps are ignored by the reference too
+        } else {
+            ++m->skipped_statements;
+        }
+    }
+    close();
+    return last;
+}
+
+// Resolves an OBJ index (1-based, or negative from the end).
+size_t obj_index(long i, size_t n, const char *what) {
+    const long r = (i < 0) ? static_cast<long>(n) + i : i - 1;
+    if (r < 0 || static_cast<size_t>(r) >= n) bad(std::string("OBJ ") + what + " index " + std::to_string(i) + " out of range");
+    return static_cast<size_t>(r);
+}
+
+void parse_obj(const char *data, int64_t len, const std::string &dir, Mesh *m) {
+    Cursor c{data, data + len};
+    const char *b, *e;
+    std::vector<P3> v, vn;
+    int64_t nvt = 0;
+    std::map<std::string, int32_t> named;
+    m->mats.push_back(phong(0.0f, 200.0f, 0.0f));
+    int32_t cur = 0;
+    std::string t;
+    std::vector<P3> face, face_n;
+    int64_t face_t = 0;
+    while (next_line(&c, &b, &e)) {
+        Tokens tk{b, e};
+        if (!tk.next(&t) || t[0] == '#') continue;
+        if (starts(t, "mtllib")) {
+            // The current material becomes the last one the last library
+            // defines (none: faces need a usemtl before they shade).
+            std::string file;
+            while (tk.next(&file)) cur = parse_mtl(read_file(dir + file), m, &named);
+        } else if (starts(t, "usemtl")) {
+            std::string name;
+            if (!tk.next(&name)) bad("usemtl without a name");
+            auto it = named.find(name);
+            if (it == named.end()) {
+                m->mats.push_back(phong(0.0f, 200.0f, 0.0f));
+                it = named.emplace(name, static_cast<int32_t>(m->mats.size() - 1)).first;
+            }
+            cur = it->second;
+        } else if (starts(t, "vn")) {
+            P3 q;
+            q.x = next_float(&tk, "vn x");
+            q.y = next_float(&tk, "vn y");
+            q.z = next_float(&tk, "vn z");
+            vn.push_back(normalised(q));
+        } else if (starts(t, "vt")) {
+            ++nvt;
+        } else if (t[0] == 'f') {
+            face.clear();
+            face_n.clear();
+            face_t = 0;
+            std::string corner;
+            while (tk.next(&corner)) {
+                // v, v/vt, v/vt/vn or v//vn
+                const size_t s1 = corner.find('/');
+                face.push_back(v[obj_index(to_int(corner.substr(0, s1), "a vertex index"), v.size(), "vertex")]);
+                if (s1 == std::string::npos) continue;
+                const size_t s2 = corner.find('/', s1 + 1);
+                const std::string vt_s = corner.substr(s1 + 1, s2 == std::string::npos ? std::string::npos : s2 - s1 - 1);
+                if (!vt_s.empty()) {
+                    obj_index(to_int(vt_s, "a texture index"), static_cast<size_t>(nvt), "texture");
+                    ++face_t;
+                }
+                if (s2 != std::string::npos)
+                    face_n.push_back(vn[obj_index(to_int(corner.substr(s2 + 1), "a normal index"), vn.size(), "normal")]);
+            }
+            if (!face_n.empty() && face_n.size() != face.size()) bad("OBJ face gives normals for some corners only");
+            if (face_t != 0 && face_t != static_cast<int64_t>(face.size()))
+                bad("OBJ face gives texture coordinates for some corners only");
+            ++m->faces;
+            if (cur < 0) bad("OBJ face without a material (its material library defines none)");
+            if (face.size() == 3) {
+                if (obj_triangle_ok(face[0], face[1], face[2])) {
+                    m->add(face[0], face[1], face[2], cur, face_n.empty() ? nullptr : face_n.data());
+                } else {
+                    ++m->skipped_faces;
+                }
+            } else {
+                add_polygon(m, face, &face_n, cur);
+            }
+        } else if (t[0] == 'v') {
+            P3 q;
+            q.x = next_float(&tk, "v x");
+            q.y = next_float(&tk, "v y");
+            q.z = next_float(&tk, "v z");
+            v.push_back(q);
+        } else if (t[0] == 'g' || t[0] == 'o' || t[0] == 's') {
+            // groups, objects and smoothing groups carry nothing to shade
+        } else {
+            ++m->skipped_statements;
+        }
+    }
+    m->vertices = static_cast<int64_t>(v.size());
+}
+
+int32_t format_of(const std::string &path) {
+    const size_t dot = path.find_last_of('.');
+    if (dot == std::string::npos) return -1;
+    std::string ext = path.substr(dot + 1);
+    for (char &ch : ext) ch = static_cast<char>(std::tolower(static_cast<unsigned char>(ch)));
+    if (ext == "off") return RT_FORMAT_OFF;
+    if (ext == "ply") return RT_FORMAT_PLY;
+    if (ext == "obj") return RT_FORMAT_OBJ;
+    return -1;
+}
+
+}  // namespace
+
+struct rt_mesh {
+    Mesh m;
+};
+
+extern "C" {
+
+const char *rt_mesh_last_error(void) { return g_error.c_str(); }
+
+int32_t rt_mesh_parse(const char *data, int64_t len, int32_t format, const char *dir, rt_mesh **out) {
+    if (!out) return RT_ERR_INVALID;
+    *out = nullptr;
+    g_error.clear();
+    if (len < 0 || (len > 0 && !data)) {
+        g_error = "rt_mesh_parse: bad buffer";
+        return RT_ERR_INVALID;
+    }
+    std::unique_ptr<rt_mesh> mesh(new (std::nothrow) rt_mesh());
+    if (!mesh) {
+        g_error = "rt_mesh_parse: host allocation";
+        return RT_ERR_OOM;
+    }
+    try {
+        switch (format) {
+            case RT_FORMAT_OFF:
+                parse_off(data, len, &mesh->m);
+                break;
+            case RT_FORMAT_PLY:
+                parse_ply(data, len, &mesh->m);
+                break;
+            case RT_FORMAT_OBJ: {
+                std::string d = dir ? dir : "";
+                if (!d.empty() && d.back() != '/') d += '/';
+                parse_obj(data, len, d, &mesh->m);
+                break;
+            }
+            default:
+                g_error = "rt_mesh_parse: unknown format " + std::to_string(format);
+                return RT_ERR_UNSUPPORTED;
+        }
+    } catch (const ParseError &pe) {
+        g_error = "rt_mesh_parse: " + pe.what;
+        return RT_ERR_INVALID;
+    } catch (const std::bad_alloc &) {
+        g_error = "rt_mesh_parse: host allocation";
+        return RT_ERR_OOM;
+    } catch (...) {
+        g_error = "rt_mesh_parse: unexpected failure";
+        return RT_ERR_INVALID;
+    }
+    *out = mesh.release();
+    return RT_OK;
+}
+
+int32_t rt_mesh_load(const char *path, int32_t format, rt_mesh **out) {
+    if (!out || !path) return RT_ERR_INVALID;
+    *out = nullptr;
+    g_error.clear();
+    const std::string p(path);
+    if (format < 0) format = format_of(p);
+    if (format < 0) {
+        g_error = "rt_mesh_load: no format given and none known for " + p;
+        return RT_ERR_UNSUPPORTED;
+    }
+    std::string text;
+    try {
+        text = read_file(p);
+    } catch (const ParseError &pe) {
+        g_error = "rt_mesh_load: " + pe.what;
+        return RT_ERR_INVALID;
+    } catch (const std::bad_alloc &) {
+        g_error = "rt_mesh_load: host allocation";
+        return RT_ERR_OOM;
+    }
+    const size_t slash = p.find_last_of('/');
+    const std::string dir = (slash == std::string::npos) ? std::string() : p.substr(0, slash + 1);
+    return rt_mesh_parse(text.data(), static_cast<int64_t>(text.size()), format, dir.c_str(), out);
+}
+
+int32_t rt_mesh_info_get(const rt_mesh *mesh, rt_mesh_info *info) {
+    if (!mesh || !info) return RT_ERR_INVALID;
+    const Mesh &m = mesh->m;
+    info->triangles = static_cast<int64_t>(m.tri_mat.size());
+    info->materials = static_cast<int32_t>(m.mats.size());
+    info->has_vertex_normals = m.any_vn ? 1 : 0;
+    info->vertices = m.vertices;
+    info->faces = m.faces;
+    info->skipped_faces = m.skipped_faces;
+    info->skipped_statements = m.skipped_statements;
+    info->texture_maps = m.texture_maps;
+    return RT_OK;
+}
+
+int32_t rt_mesh_copy(const rt_mesh *mesh, float *abc, int32_t *tri_material, rt_material *materials,
+                     float *vertex_normals) {
+    if (!mesh) return RT_ERR_INVALID;
+    const Mesh &m = mesh->m;
+    if (abc && !m.abc.empty()) std::memcpy(abc, m.abc.data(), m.abc.size() * sizeof(float));
+    if (tri_material && !m.tri_mat.empty())
+        std::memcpy(tri_material, m.tri_mat.data(), m.tri_mat.size() * sizeof(int32_t));
+    if (materials && !m.mats.empty()) std::memcpy(materials, m.mats.data(), m.mats.size() * sizeof(rt_material));
+    if (vertex_normals && m.any_vn) std::memcpy(vertex_normals, m.vn.data(), m.vn.size() * sizeof(float));
+    return RT_OK;
+}
+
+void rt_mesh_destroy(rt_mesh *mesh) { delete mesh; }
+
+int32_t rt_scene_create_from_mesh(rt_ctx *ctx, const rt_mesh *mesh, rt_scene **out) {
+    if (!ctx || !mesh || !out) return RT_ERR_INVALID;
+    *out = nullptr;
+    const Mesh &m = mesh->m;
+    const int64_t n = static_cast<int64_t>(m.tri_mat.size());
+    // Albedo kd / 255 keeps the Lambert-only frames meaningful; the flags mark
+    // triangles of transparent materials (material(tran > 0)).
+    std::vector<float> albedo(static_cast<size_t>(3 * std::max<int64_t>(n, 1)));
+    std::vector<uint8_t> flags(static_cast<size_t>(std::max<int64_t>(n, 1)), 0);
+    for (int64_t i = 0; i < n; ++i) {
+        const rt_material &q = m.mats[static_cast<size_t>(m.tri_mat[static_cast<size_t>(i)])];
+        for (int k = 0; k < 3; ++k) albedo[static_cast<size_t>(3 * i + k)] = q.kd[k] / 255.0f;
+        flags[static_cast<size_t>(i)] = q.transmittance > 0.0f ? 2 : 0;
+    }
+    rt_scene *sc = nullptr;
+    int32_t rc = rt_scene_create(ctx, m.abc.data(), albedo.data(), flags.data(), n, &sc);
+    if (rc != RT_OK) return rc;
+    rc = rt_scene_set_materials(sc, m.mats.data(), static_cast<int32_t>(m.mats.size()), m.tri_mat.data());
+    if (rc == RT_OK && m.any_vn) rc = rt_scene_set_vertex_normals(sc, m.vn.data());
+    if (rc != RT_OK) {
+        rt_scene_destroy(sc);
+        return rc;
+    }
+    *out = sc;
+    return RT_OK;
+}
+
+}  // extern "C"

@@ -455,8 +455,9 @@ int32_t rt_invrender_create(rt_ctx *ctx, const rt_invrender_item *items, int32_t
             return fail(ctx, RT_ERR_INVALID, "rt_invrender_create: items must live on the context's device");
         if (a->root == rt::kEmpty || a->root < 0)
             return fail(ctx, RT_ERR_UNSUPPORTED, "rt_invrender_create: scenes need at least two triangles");
-        if (a->scene->d_mat)
-            return fail(ctx, RT_ERR_UNSUPPORTED, "rt_invrender_create: scenes with phong materials are not differentiable");
+        if (a->scene->d_mat || a->scene->d_vn)
+            return fail(ctx, RT_ERR_UNSUPPORTED,
+                        "rt_invrender_create: scenes with materials or vertex normals are not differentiable");
         if (!it.albedo || !it.d_albedo || !it.image)
             return fail(ctx, RT_ERR_INVALID, "rt_invrender_create: albedo, gradient and image buffers are required");
         if (it.cam.x_res <= 0 || it.cam.y_res <= 0)

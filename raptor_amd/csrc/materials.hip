@@ -26,6 +26,7 @@
 // Children sit in the next generation in the order of their parents and
 // every sum has a fixed order: frames are reproducible bit for bit.
 #include <algorithm>
+#include <cfloat>
 #include <type_traits>
 #include <vector>
 
@@ -58,7 +59,7 @@ struct Mat {
 // (kd from the scene, ks = ka = 0, s = 0, no mirror or refraction).
 __device__ __forceinline__ Mat mat_of(const float4 *mats, const int32_t *tri_mat, const float4 *kd, int32_t prim) {
     Mat m;
-    const int32_t mi = tri_mat[prim];
+    const int32_t mi = tri_mat ? tri_mat[prim] : -1;
     if (mi < 0) {
         const float4 k = kd[prim];
         m.ka_s = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -171,6 +172,7 @@ struct GenArgs {
     int64_t n;
     const float4 *tri_n, *tri_c, *kd, *mats;
     const int32_t *tri_mat;
+    const float *vn;  // vertex normals, 9 per triangle, or null
 };
 
 // Padding slots past the image edge get tmax < tmin, so they never hit.
@@ -209,6 +211,15 @@ __device__ __forceinline__ HitSurface surface_of(const GenArgs &a, int64_t i) {
     const rtd::V3 d = rtd::f4v(r1);
     const bool leaving = rtd::dot(rtd::f4v(tn), d) > 0.0f;  // hit_t::in_out (rt/triangle.h:487-496)
     surface_at(rtd::f4v(r0), d, hit.x, rtd::f4v(tn), rtd::f4v(a.tri_c[prim]), &h.dst, &h.geo, &h.shn);
+    if (a.vn && a.vn[9 * static_cast<int64_t>(prim)] != FLT_MAX) {
+        // rt/triangle.h:474-477: u weights vertex c, v vertex b, 1 - (u + v)
+        // vertex a; not renormalised; turned like the geometric normal.
+        const float *q = a.vn + 9 * static_cast<int64_t>(prim);
+        const float u = hit.y, v = hit.z, w = 1.0f - (u + v);
+        rtd::V3 s = rtd::mk(((u * q[6]) + (v * q[3])) + (w * q[0]), ((u * q[7]) + (v * q[4])) + (w * q[1]),
+                            ((u * q[8]) + (v * q[5])) + (w * q[2]));
+        h.shn = leaving ? rtd::neg(s) : s;
+    }
     h.m = mat_of(a.mats, a.tri_mat, a.kd, prim);
     const float power = a.power[i];
     h.refl = h.m.rf > 0.0f && static_cast<double>(power * h.m.rf) > kMinReflectivePower;
@@ -506,7 +517,7 @@ int32_t render_frame_materials(rt_accel *a, const rt_camera &cam, const rt_light
                                  first ? 0 : 2)))
             return rc;
         if (timed) RT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[2], s));
-        const GenArgs g{rays, power, hits, n, sc->d_n, sc->d_c, sc->d_kd, sc->d_mat, sc->d_tri_mat};
+        const GenArgs g{rays, power, hits, n, sc->d_n, sc->d_c, sc->d_kd, sc->d_mat, sc->d_tri_mat, sc->d_vn};
         hipLaunchKernelGGL(k_mat_classify, grid_of(n + 1), dim3(256), 0, s, g, gen.hit_pos, kids);
         RT_HIP_CHECK(ctx, scan_u32(gen.hit_pos, n + 1, sums, s));
         RT_HIP_CHECK(ctx, scan_u32(kids, n + 1, sums, s));

@@ -102,6 +102,9 @@ struct rt_scene {
     float4 *d_mat = nullptr;
     int32_t *d_tri_mat = nullptr;
     int32_t nmat = 0;
+    // Vertex normals (rt_scene_set_vertex_normals): 9 floats per triangle,
+    // FLT_MAX as the first marks a triangle without them; null when none.
+    float *d_vn = nullptr;
     std::atomic<int> refs{1};  // the caller's handle + one per accel
 };
 

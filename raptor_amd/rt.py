@@ -34,7 +34,8 @@ EXPORTS = [
     "rt_scene_set_albedo", "rt_render_tape", "rt_tape_info", "rt_render_adjoint", "rt_tape_destroy",
     "rt_invrender_create", "rt_invrender_run", "rt_invrender_destroy", "rt_scene_update", "rt_accel_refit",
     "rt_scene_set_materials", "rt_image_tone_map", "rt_image_gamma", "rt_image_clip", "rt_write_png",
-    "rt_write_tga",
+    "rt_write_tga", "rt_scene_set_vertex_normals", "rt_mesh_load", "rt_mesh_parse", "rt_mesh_info_get",
+    "rt_mesh_copy", "rt_mesh_last_error", "rt_mesh_destroy", "rt_scene_create_from_mesh",
 ]
 
 
@@ -92,6 +93,12 @@ class rt_accel_info(C.Structure):
                 ("refit_ms", C.c_float)]
 
 
+class rt_mesh_info(C.Structure):
+    _fields_ = [("triangles", C.c_int64), ("materials", C.c_int32), ("has_vertex_normals", C.c_int32),
+                ("vertices", C.c_int64), ("faces", C.c_int64), ("skipped_faces", C.c_int64),
+                ("skipped_statements", C.c_int64), ("texture_maps", C.c_int32), ("reserved", C.c_int32)]
+
+
 def load_library(path=LIB_PATH):
     if not os.path.exists(path):
         raise RuntimeError(f"{path} is missing: build it with `python raptor_amd/build.py` (no fallback exists)")
@@ -145,6 +152,14 @@ def load_library(path=LIB_PATH):
         "rt_invrender_run": (i32, [vp, i32, i32, i32, C.POINTER(C.c_double), C.POINTER(C.c_uint64),
                                    C.POINTER(C.c_float)]),
         "rt_invrender_destroy": (None, [vp]),
+        "rt_scene_set_vertex_normals": (i32, [vp, fp]),
+        "rt_mesh_load": (i32, [C.c_char_p, i32, C.POINTER(vp)]),
+        "rt_mesh_parse": (i32, [C.c_char_p, i64, i32, C.c_char_p, C.POINTER(vp)]),
+        "rt_mesh_info_get": (i32, [vp, C.POINTER(rt_mesh_info)]),
+        "rt_mesh_copy": (i32, [vp, fp, fp, fp, fp]),
+        "rt_mesh_last_error": (C.c_char_p, []),
+        "rt_mesh_destroy": (None, [vp]),
+        "rt_scene_create_from_mesh": (i32, [vp, vp, C.POINTER(vp)]),
         "rt_gen_random_triangles": (i64, [u64, i64, C.c_float, C.c_float, C.c_float, fp, fp]),
         "rt_gen_terrain": (i64, [i32, C.c_float, C.c_float, C.c_float, i32, i32, u64, C.c_float, C.c_float, fp, fp]),
     }
@@ -203,7 +218,7 @@ def gen_terrain(grid, amplitude=0.15, zoom=0.02, persistence=0.5, octaves=8, noi
 
 # ---------------------------------------------------------------- handles
 
-MATERIAL_PHONG, MATERIAL_COOK_TORRANCE = 0, 1
+MATERIAL_PHONG, MATERIAL_COOK_TORRANCE, MATERIAL_COLOURED = 0, 1, 2
 # rt_material (include/raptor_rt.h), 76 bytes.
 MATERIAL_DTYPE = np.dtype([("ka", "<f4", 3), ("kd", "<f4", 3), ("ks", "<f4", 3), ("shininess", "<f4"),
                            ("transmittance", "<f4"), ("refractive_index", "<f4"), ("reflectance", "<f4"),
@@ -216,7 +231,8 @@ def material_table(materials):
     (rgb, 0..255; default 0), s (shininess, 0), tran (transmittance, 0), ri
     (refractive index, 1), rf (reflectance, 0). Cook-Torrance (kind 1,
     cook_torrance_cxy): x, y (chroma, 0.33), rd (0.5), rs, ts, sr (0), ri_r
-    (1), ri_i (0). A row of 13 floats (ka, kd, ks, s, tran, ri, rf) is a
+    (1), ri_i (0). Kind 2 (coloured_mapper_shader without textures) takes
+    the phong fields. A row of 13 floats (ka, kd, ks, s, tran, ri, rf) is a
     phong material."""
     out = np.zeros(len(materials), MATERIAL_DTYPE)
     for i, m in enumerate(materials):
@@ -236,7 +252,7 @@ def material_table(materials):
             r["refractive_index"] = m.get("ri_r", 1.0)
             r["extinction"] = m.get("ri_i", 0.0)
         else:
-            r["kind"] = MATERIAL_PHONG
+            r["kind"] = m.get("kind", MATERIAL_PHONG)
             r["ka"], r["kd"], r["ks"] = m.get("ka", (0, 0, 0)), m.get("kd", (0, 0, 0)), m.get("ks", (0, 0, 0))
             r["shininess"] = m.get("s", 0.0)
             r["transmittance"] = m.get("tran", 0.0)
@@ -273,6 +289,18 @@ class Context:
 
 
 class Scene:
+    @classmethod
+    def from_mesh(cls, ctx, mesh):
+        """The scene of a parsed Mesh: triangles, albedo kd / 255,
+        transparency flags, materials and vertex normals."""
+        self = cls.__new__(cls)
+        self.ctx = ctx
+        self.n = mesh.n
+        h = C.c_void_p()
+        ctx.check(_LIB.rt_scene_create_from_mesh(ctx.h, mesh.h, C.byref(h)), "rt_scene_create_from_mesh")
+        self.h = h
+        return self
+
     def __init__(self, ctx, abc, albedo=None, flags=None):
         self.ctx = ctx
         abc = np.ascontiguousarray(abc, np.float32).reshape(-1, 9)
@@ -324,6 +352,17 @@ class Scene:
             raise ValueError(f"rt_scene_set_materials: {k.shape[0]} indices, the scene has {self.n} triangles")
         self.ctx.check(_LIB.rt_scene_set_materials(self.h, _ptr(m), m.shape[0], _ptr(k)), "rt_scene_set_materials")
 
+    def set_vertex_normals(self, vn):
+        """(n, 9) float32 normals at vertices a, b, c; a first value of
+        FLT_MAX keeps that triangle's geometric normal; None removes them."""
+        if vn is None:
+            self.ctx.check(_LIB.rt_scene_set_vertex_normals(self.h, None), "rt_scene_set_vertex_normals")
+            return
+        v = np.ascontiguousarray(vn, np.float32).reshape(-1, 9)
+        if v.shape[0] != self.n:
+            raise ValueError(f"rt_scene_set_vertex_normals: {v.shape[0]} rows, the scene has {self.n} triangles")
+        self.ctx.check(_LIB.rt_scene_set_vertex_normals(self.h, _ptr(v)), "rt_scene_set_vertex_normals")
+
     def normals(self):
         out = np.empty((self.n, 3), np.float32)
         self.ctx.check(_LIB.rt_scene_normals(self.h, _ptr(out)), "rt_scene_normals")
@@ -332,6 +371,48 @@ class Scene:
     def close(self):
         if self.h:
             _LIB.rt_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+FORMAT_OFF, FORMAT_PLY, FORMAT_OBJ = 0, 1, 2
+
+
+class Mesh:
+    """A parsed OFF, PLY or OBJ file (host memory; no GPU needed). Fields:
+    abc (n, 9) float32 in emplace order, tri_material (n,) int32, materials
+    (MATERIAL_DTYPE), vertex_normals (n, 9) float32 or None, info (dict)."""
+
+    def __init__(self, path=None, data=None, fmt=-1, directory=None):
+        h = C.c_void_p()
+        if path is not None:
+            rc = _LIB.rt_mesh_load(os.fsencode(path), fmt, C.byref(h))
+        else:
+            buf = bytes(data)
+            rc = _LIB.rt_mesh_parse(buf, len(buf), fmt, None if directory is None else os.fsencode(directory),
+                                    C.byref(h))
+        if rc != RT_OK:
+            raise RTError(f"rt_mesh: {rc} {_LIB.rt_mesh_last_error().decode()}")
+        self.h = h
+        inf = rt_mesh_info()
+        _LIB.rt_mesh_info_get(h, C.byref(inf))
+        self.info = {k: getattr(inf, k) for k, _ in rt_mesh_info._fields_ if k != "reserved"}
+        self.n = inf.triangles
+        self.abc = np.empty((self.n, 9), np.float32)
+        self.tri_material = np.empty(self.n, np.int32)
+        self.materials = np.zeros(inf.materials, MATERIAL_DTYPE)
+        self.vertex_normals = np.empty((self.n, 9), np.float32) if inf.has_vertex_normals else None
+        _LIB.rt_mesh_copy(h, _ptr(self.abc), _ptr(self.tri_material), _ptr(self.materials),
+                          _ptr(self.vertex_normals))
+
+    def close(self):
+        if self.h:
+            _LIB.rt_mesh_destroy(self.h)
             self.h = None
 
     def __del__(self):
