@@ -77,6 +77,15 @@ def look_at(pos, target, up=(0.0, 1.0, 0.0)):
     return f32(x + 0.0), f32(y + 0.0), f32(z + 0.0)
 
 
+def fixture_camera(pos, x_axis, y_axis, z_axis, x_res=640, y_res=480):
+    """The camera the reference's regression fixture builds for OFF, PLY and
+    OBJ scenes (rt/regression_tests/regression_fixture.h:86-100, 150-168):
+    screen width 10, height 10 * y_res / x_res, image plane at 20."""
+    w = np.float32(10.0)
+    h = w * (np.float32(y_res) / np.float32(x_res))
+    return Camera(f32(pos), f32(x_axis), f32(y_axis), f32(z_axis), float(w), float(h), 20.0, x_res, y_res)
+
+
 @dataclass
 class Workload:
     name: str
@@ -179,6 +188,25 @@ def material_scene(n_objects=400, res=96, seed=5, mirror=None, glass=None, mirro
     w = Workload("materials", abc, albedo, cam, [Light(f32(light), (255.0, 240.0, 230.0), 0.1)], bounce=False,
                  meta=dict(triangles=int(abc.shape[0])))
     return w, materials, np.asarray(kinds, np.int32), np.asarray(flags, np.uint8)
+
+
+def mesh_view(abc, x_res=320, y_res=240, direction=(0.45, 0.35, -0.82), light_rgb=(255.0, 255.0, 255.0)):
+    """A fixture camera that frames a parsed mesh from the given direction
+    (the view direction, camera to mesh), and a point light on the camera's
+    side of the mesh, above and to the side of the camera."""
+    v = np.asarray(abc, np.float64).reshape(-1, 3)
+    lo, hi = v.min(0), v.max(0)
+    centre, radius = 0.5 * (lo + hi), 0.5 * np.linalg.norm(hi - lo)
+    d = np.asarray(direction, np.float64)
+    d /= np.linalg.norm(d)
+    pos = centre - d * (radius * 3.0)
+    x, y, z = look_at(pos, centre)
+    cam = fixture_camera(pos, x, y, z, x_res, y_res)
+    side = np.cross(d, (0.0, 1.0, 0.0))
+    side = side / np.linalg.norm(side) if np.linalg.norm(side) > 0 else np.array([1.0, 0.0, 0.0])
+    light = centre - d * (radius * 2.5) + radius * (1.5 * side + np.array([0.0, 2.0, 0.0]))
+    return Workload("mesh", np.asarray(abc, np.float32), None, cam, [Light(f32(light), light_rgb, 0.0)],
+                    bounce=False, meta=dict(triangles=int(v.shape[0] // 3)))
 
 
 def material_terrain(grid=708, x_res=1920, y_res=1080):

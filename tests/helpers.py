@@ -25,3 +25,13 @@ def rel_err(a, b, floor=1e-3):
     a = np.asarray(a, np.float64)
     b = np.asarray(b, np.float64)
     return np.abs(a - b) / np.maximum(np.abs(b), floor)
+
+
+def oracle_material_rows(materials):
+    """rt_material rows (the library's MATERIAL_DTYPE) as the oracle's
+    19-float rows: ka, kd, ks, s, tran, ri, rf, kind, cx, cy, rd, sr, ri_i."""
+    m = materials
+    return np.concatenate([m["ka"], m["kd"], m["ks"], m["shininess"][:, None], m["transmittance"][:, None],
+                           m["refractive_index"][:, None], m["reflectance"][:, None],
+                           m["kind"][:, None].astype(np.float32), m["chroma"], m["diffuse"][:, None],
+                           m["roughness"][:, None], m["extinction"][:, None]], axis=1).astype(np.float32)
