@@ -477,6 +477,83 @@ def materials_section(ctx, frames, args):
     return out
 
 
+def ingest_section(ctx, w, args, reps=3):
+    """Scene ingestion of the 1M-triangle terrain from a model file: the
+    terrain as a binary big-endian PLY (shared vertices, one triangle per
+    face, as scanned models ship), parsed on the host (rt_mesh_load), uploaded
+    with its material (rt_scene_create_from_mesh), built and rendered. The
+    parsed triangles must give the generated scene's primary hits bit for
+    bit. Host timings are medians of reps."""
+    import tempfile
+
+    import numpy as np
+    import torch
+
+    from raptor_amd import rt
+
+    v = np.ascontiguousarray(np.asarray(w.abc, np.float32).reshape(-1, 3))
+    keys, inv = np.unique(v.view(np.dtype((np.void, 12))).reshape(-1), return_inverse=True)
+    verts = keys.view(np.float32).reshape(-1, 3)
+    faces = inv.reshape(-1, 3).astype(">i4")
+    nf = faces.shape[0]
+    head = ("ply\nformat binary_big_endian 1.0\nelement vertex %d\nproperty float x\nproperty float y\n"
+            "property float z\nelement face %d\nproperty list uchar int vertex_indices\nend_header\n"
+            % (verts.shape[0], nf)).encode()
+    body = np.empty((nf, 13), np.uint8)
+    body[:, 0] = 3
+    body[:, 1:] = faces.view(np.uint8).reshape(nf, 12)
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "terrain1m.ply")
+        with open(path, "wb") as fh:
+            fh.write(head + verts.astype(">f4").tobytes() + body.tobytes())
+        size = os.path.getsize(path)
+        parse, upload = [], []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            mesh = rt.Mesh(path)
+            t1 = time.perf_counter()
+            sc = rt.Scene.from_mesh(ctx, mesh)
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            parse.append(t1 - t0)
+            upload.append(t2 - t1)
+            sc.close()
+    sc = rt.Scene.from_mesh(ctx, mesh)
+    t0 = time.perf_counter()
+    acc = rt.Accel(sc, width=args.width, leaf_size=args.leaf_size, lds_stack=args.lds_stack, sah_bins=args.sah_bins)
+    build_wall = time.perf_counter() - t0
+    cam = w.camera
+    t0 = time.perf_counter()
+    first = acc.render(cam, w.lights, want_hits=True)
+    first_wall = time.perf_counter() - t0
+    ms = sorted(acc.render(cam, w.lights, want_hits=False)["stats"]["ms"][0] for _ in range(5))[2]
+    gen_scene = rt.Scene(ctx, w.abc, w.albedo)
+    gen_acc = rt.Accel(gen_scene, width=args.width, leaf_size=args.leaf_size, lds_stack=args.lds_stack,
+                       sah_bins=args.sah_bins)
+    gen = gen_acc.render(cam, w.lights, want_hits=True)
+    same = bool(np.array_equal(first["primary"]["prim"], gen["primary"]["prim"]))
+    med = lambda xs: sorted(xs)[len(xs) // 2]  # noqa: E731
+    out = {
+        "workload": "terrain1m as a binary big-endian PLY (%d vertices, %d faces, %.1f MB): parse, upload with "
+                    "materials, BVH build, 1920x1080 frame (primary + shadow rays, phong kd 200)"
+                    % (verts.shape[0], nf, size / 1e6),
+        "triangles": int(mesh.n),
+        "parse_ms": round(med(parse) * 1e3, 2),
+        "parse_mb_s": round(size / med(parse) / 1e6, 1),
+        "parse_mtris_s": round(mesh.n / med(parse) / 1e6, 2),
+        "scene_upload_ms": round(med(upload) * 1e3, 2),
+        "build_ms": round(acc.info()["build_ms"], 2),
+        "build_wall_ms": round(build_wall * 1e3, 2),
+        "first_frame_wall_ms": round(first_wall * 1e3, 2),
+        "frame_gpu_ms": round(ms, 3),
+        "file_to_first_frame_ms": round((med(parse) + med(upload) + build_wall + first_wall) * 1e3, 1),
+        "primary_ids_identical_to_generated_scene": same,
+    }
+    for h in (gen_acc, gen_scene, acc, sc, mesh):
+        h.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -496,6 +573,7 @@ def main():
     ap.add_argument("--dynamic-frames", type=int, default=8, help="0 skips the moving-scene workload")
     ap.add_argument("--material-frames", type=int, default=5, help="0 skips the phong-material workload")
     ap.add_argument("--post-reps", type=int, default=10, help="0 skips the camera post-process workload")
+    ap.add_argument("--ingest-reps", type=int, default=3, help="0 skips the scene-ingestion workload")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each GPU would trace its own share "
@@ -603,6 +681,9 @@ def main():
     if args.material_frames > 0:
         print("bench: phong materials", file=sys.stderr, flush=True)
         line["materials"] = materials_section(ctx, args.material_frames, args)
+    if args.ingest_reps > 0:
+        print("bench: scene ingestion", file=sys.stderr, flush=True)
+        line["ingest"] = ingest_section(ctx, w, args, args.ingest_reps)
     if args.batched_scenes > 0:
         del rgb, accel, scene
         print("bench: batched workload", file=sys.stderr, flush=True)

@@ -84,13 +84,18 @@ typedef struct {
 
 /* Frame options: bounce = 1 adds one cosine-weighted diffuse bounce with
  * direct light at the bounce hit; spp samples per pixel; seed keys the
- * counter-based sampler. */
+ * counter-based sampler. background is the camera's background colour
+ * (rt/camera.h:234-240, 0-255 scale): the colour of a primary ray, or of a
+ * mirror or refraction ray, that hits nothing (rt/raytracer.cc:82-86); a
+ * diffuse bounce that hits nothing adds nothing. */
 typedef struct {
     int32_t bounce;
     int32_t spp;
     uint64_t seed;
     int32_t flags;
     int32_t reserved;
+    float background[3];
+    int32_t reserved2;
 } rt_render_params;
 
 /* Frame statistics. rays: primary, shadow, bounce, bounce-shadow.

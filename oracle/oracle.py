@@ -28,7 +28,8 @@ class oracle_light(C.Structure):
 class oracle_render_params(C.Structure):
     _fields_ = [("bounce", C.c_int32), ("spp", C.c_int32), ("seed", C.c_uint64), ("pixel_stride", C.c_int32),
                 ("nthreads", C.c_int32), ("packet", C.c_int32), ("materials", C.c_void_p),
-                ("nmaterials", C.c_int32), ("tri_material", C.c_void_p), ("vertex_normals", C.c_void_p)]
+                ("nmaterials", C.c_int32), ("tri_material", C.c_void_p), ("vertex_normals", C.c_void_p),
+                ("background", C.c_float * 3)]
 
 
 class oracle_render_out(C.Structure):
@@ -205,7 +206,7 @@ class OracleScene:
         return occ, cnt
 
     def render(self, albedo, camera, lights, bounce=False, spp=1, seed=0, pixel_stride=1, nthreads=8, packet=False,
-               records=False, materials=None, tri_material=None, vertex_normals=None):
+               records=False, materials=None, tri_material=None, vertex_normals=None, background=(0.0, 0.0, 0.0)):
         """One frame; bounce is the number of diffuse bounces (True = 1).
         records=True adds per-sample path records over the bounce + 1 path
         vertices, level-major (vertex k, sample p * spp + s): prim (L, S)
@@ -238,6 +239,7 @@ class OracleScene:
             out.rec_prim, out.rec_o, out.rec_n, out.rec_vis = (_p(rec["prim"]), _p(rec["o"]), _p(rec["n"]),
                                                                _p(rec["vis"]))
         p = oracle_render_params(nb, spp, seed, pixel_stride, nthreads, 1 if packet else 0)
+        p.background = (C.c_float * 3)(*background)
         sec = np.zeros(2, np.uint64)
         out.sec_counters = _p(sec)
         if materials is not None:

@@ -218,7 +218,7 @@ void render_pixel(const FrameCtx &f, int px, int py, LocalCounters *lc) {
             f.out.tuv[3 * p + 2] = h.v;
         }
         if (first && f.out.bounce_prim) f.out.bounce_prim[p] = -1;
-        Colour c;  // background (0, 0, 0)
+        Colour c{f.p.background[0], f.p.background[1], f.p.background[2]};  // a miss: camera::shade
         const int levels = f.p.bounce + 1;
         for (int lv = 0; lv < o.rec_levels; ++lv)
             record_surface(o.rec_prim, o.rec_o, o.rec_n, o.rec_vis, static_cast<int64_t>(lv) * f.ns + k, nl, -1,
@@ -425,8 +425,8 @@ Colour trace_phong(const FrameCtx &f, Ray &r, bool primary, LocalCounters *lc, P
     } else {
         ++pc->secondary;
     }
-    Colour c;  // camera background (0, 0, 0)
-    if (!(h.d < kMaxDist)) return c;
+    if (!(h.d < kMaxDist)) return Colour{f.p.background[0], f.p.background[1], f.p.background[2]};
+    Colour c;
     const Triangle &tri = f.s->tris[prim];
     const bool leaving = dot(tri.n, r.dir) > 0.0f;  // hit_t::in_out (rt/triangle.h:487-496)
     SurfaceHit sh = surface_at(tri, &r, h.d);

@@ -61,6 +61,9 @@ typedef struct {
      * u n_c + v n_b + (1 - (u + v)) n_a turned to face the ray
      * (rt/triangle.h:464-498). Non-NULL selects the material frame. */
     const float *vertex_normals;
+    /* The camera's background colour: a primary, mirror or refraction ray
+     * that hits nothing takes it (rt/camera.h:234-240). */
+    float background[3];
 } oracle_render_params;
 
 /* Outputs; any pointer may be NULL. Arrays are indexed by pixel p = y*W + x.

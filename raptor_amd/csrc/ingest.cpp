@@ -285,6 +285,8 @@ void parse_off(const char *data, int64_t len, Mesh *m) {
         v.push_back(q);
     }
     m->vertices = nv;
+    m->abc.reserve(static_cast<size_t>(9 * nf));
+    m->tri_mat.reserve(static_cast<size_t>(nf));
     std::vector<P3> face;
     for (long i = 0; i < nf; ++i) {
         if (!next_line(&c, &b, &e)) bad("OFF file ends inside the face list");
@@ -468,6 +470,8 @@ void parse_ply(const char *data, int64_t len, Mesh *m) {
 
     std::vector<P3> v;
     v.reserve(static_cast<size_t>(nv));
+    m->abc.reserve(static_cast<size_t>(9 * nf));
+    m->tri_mat.reserve(static_cast<size_t>(nf));
     std::vector<P3> face;
     auto emit = [&](int32_t mat) {
         ++m->faces;

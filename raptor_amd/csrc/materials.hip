@@ -371,12 +371,13 @@ struct GatherArgs {
     const int32_t *tri_mat;
     const uint32_t *child_hit_pos;  // next generation: exclusive scan of its hit flags
     const float4 *child_colour;     // next generation: full colour of its hits
+    float4 bg;                      // the camera's background colour
 };
 
-// The colour a child ray returns: its hit's colour, or the background (0).
+// The colour a child ray returns: its hit's colour, or the background.
 __device__ __forceinline__ float4 child_colour(const GatherArgs &a, int64_t j) {
     const uint32_t k = a.child_hit_pos[j];
-    if (a.child_hit_pos[j + 1] == k) return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (a.child_hit_pos[j + 1] == k) return a.bg;
     return a.child_colour[k];
 }
 
@@ -410,7 +411,8 @@ __global__ void __launch_bounds__(256) k_mat_gather(GatherArgs a) {
 // Pixel = (sum of its samples' colours) / spp, samples in order.
 __global__ void __launch_bounds__(256) k_mat_resolve(TileMap tm, int spp, int64_t npix,
                                                      const uint32_t *__restrict__ hit_pos,
-                                                     const float4 *__restrict__ colour, float *__restrict__ rgb) {
+                                                     const float4 *__restrict__ colour, float4 bg,
+                                                     float *__restrict__ rgb) {
     const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (p >= npix) return;
     const int x = static_cast<int>(p % tm.x_res), y = static_cast<int>(p / tm.x_res);
@@ -418,8 +420,7 @@ __global__ void __launch_bounds__(256) k_mat_resolve(TileMap tm, int spp, int64_
     float r = 0.0f, g = 0.0f, b = 0.0f;
     for (int s = 0; s < spp; ++s) {
         const uint32_t k = hit_pos[i0 + s];
-        if (hit_pos[i0 + s + 1] == k) continue;  // background: + 0
-        const float4 v = colour[k];
+        const float4 v = (hit_pos[i0 + s + 1] == k) ? bg : colour[k];
         r += v.x;
         g += v.y;
         b += v.z;
@@ -481,6 +482,7 @@ int32_t render_frame_materials(rt_accel *a, const rt_camera &cam, const rt_light
     const int nl = f.lights.n;
     const int64_t ns = f.tiles * 64 * f.spp;
     if (ns + 1 > int64_t(kScanTile) * kScanTile) return fail(ctx, RT_ERR_INVALID, "rt_render: frame too large");
+    const float4 background = make_float4(p.background[0], p.background[1], p.background[2], 0.0f);
     BuildArena &ar = ctx->arena;
     RT_HIP_CHECK(ctx, ar.begin());
     auto take = [&](auto **ptr, int64_t count) -> int32_t {
@@ -574,8 +576,8 @@ int32_t render_frame_materials(rt_accel *a, const rt_camera &cam, const rt_light
     for (int k = static_cast<int>(gens.size()) - 2; k >= 0; --k) {
         const Gen &g = gens[k], &c = gens[k + 1];
         if (g.nhits == 0) continue;
-        const GatherArgs ga{g.colour, g.rec_kids, g.rec_n, g.nhits, sc->d_kd, sc->d_mat, sc->d_tri_mat, c.hit_pos,
-                            c.colour};
+        const GatherArgs ga{g.colour, g.rec_kids, g.rec_n,   g.nhits,  sc->d_kd, sc->d_mat,
+                            sc->d_tri_mat, c.hit_pos, c.colour, background};
         hipLaunchKernelGGL(k_mat_gather, grid_of(g.nhits), dim3(256), 0, s, ga);
     }
     const bool dev_out = (p.flags & RT_DEVICE_PTRS) != 0;
@@ -586,7 +588,7 @@ int32_t render_frame_materials(rt_accel *a, const rt_camera &cam, const rt_light
         return rc;
     }
     hipLaunchKernelGGL(k_mat_resolve, grid_of(f.npix), dim3(256), 0, s, f.tm, f.spp, f.npix, gens[0].hit_pos,
-                       gens[0].colour, rgb_dev);
+                       gens[0].colour, background, rgb_dev);
     RT_HIP_CHECK(ctx, hipGetLastError());
     if (timed) RT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[5], s));
     if (rgb && !dev_out)

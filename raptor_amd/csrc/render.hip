@@ -230,6 +230,7 @@ struct ComposeArgs {
     const uint8_t *occ2;
     const float4 *kd;
     LightArgs lights;
+    float bg_r, bg_g, bg_b;  // the camera's background colour
     float *rgb;
 };
 
@@ -259,7 +260,7 @@ __global__ void __launch_bounds__(256) k_compose(ComposeArgs a) {
     Col acc{0.0f, 0.0f, 0.0f};
     for (int s = 0; s < a.spp; ++s) {
         const int64_t i = slot * a.spp + s;
-        Col c{0.0f, 0.0f, 0.0f};
+        Col c{a.bg_r, a.bg_g, a.bg_b};  // a miss takes the background (camera::shade)
         const int32_t q = a.qidx1[i];
         if (q >= 0) {
             const int32_t prim = __float_as_int(reinterpret_cast<const float4 *>(a.hits1)[i].w);
@@ -561,6 +562,9 @@ int32_t compose_chunk(const FramePlan &f, Workspace *w, int64_t tile0, int64_t n
     ca.nslots = ntiles * 64;
     ca.spp = f.spp;
     ca.bounce = f.p.bounce;
+    ca.bg_r = f.p.background[0];
+    ca.bg_g = f.p.background[1];
+    ca.bg_b = f.p.background[2];
     ca.hits1 = w->hits1;
     ca.qidx1 = w->qidx1;
     ca.nshade1 = w->nshade1;

@@ -55,7 +55,7 @@ class rt_build_params(C.Structure):
 
 class rt_render_params(C.Structure):
     _fields_ = [("bounce", C.c_int32), ("spp", C.c_int32), ("seed", C.c_uint64), ("flags", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("reserved", C.c_int32), ("background", C.c_float * 3), ("reserved2", C.c_int32)]
 
 
 class rt_stats(C.Structure):
@@ -516,12 +516,15 @@ class Accel:
         self.ctx.check(_LIB.rt_trace_occluded(self.h, _ptr(rays), n, _ptr(out), RT_DEVICE_PTRS), "rt_trace_occluded")
         return out
 
-    def render(self, camera, lights, bounce=False, spp=1, seed=0, want_hits=True, stats=True, regroup=False):
-        """One frame; returns dict(rgb (H,W,3), primary hits, bounce hits, stats)."""
+    def render(self, camera, lights, bounce=False, spp=1, seed=0, want_hits=True, stats=True, regroup=False,
+               background=(0.0, 0.0, 0.0)):
+        """One frame; returns dict(rgb (H,W,3), primary hits, bounce hits, stats).
+        background: the colour of rays that hit nothing (0-255 scale)."""
         cam = camera.to_c()
         nl = len(lights)
         larr = (rt_light * max(1, nl))(*[l.to_c() for l in lights])
-        p = rt_render_params(1 if bounce else 0, spp, seed, RT_REGROUP_BOUNCE if regroup else 0, 0)
+        p = rt_render_params(1 if bounce else 0, spp, seed, RT_REGROUP_BOUNCE if regroup else 0, 0,
+                             (C.c_float * 3)(*background))
         npix = camera.x_res * camera.y_res
         rgb = np.empty((camera.y_res, camera.x_res, 3), np.float32)
         prim = np.empty(npix * spp, HIT_DTYPE) if want_hits else None

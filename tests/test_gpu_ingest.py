@@ -195,3 +195,34 @@ def test_smooth_terrain(gpu_ctx, rt, O, with_materials):
     bad, ties = compare_hits(flat_gpu["primary"], flat["prim"], flat["tuv"])
     assert not bad.any()
     assert rel_err(flat_gpu["rgb"].reshape(-1, 3)[~ties], flat["rgb"][~ties]).max() <= RAD_TOL
+
+
+@pytest.mark.parametrize("with_materials", [False, True])
+def test_background_colour(gpu_ctx, rt, O, with_materials):
+    # Primary misses, and mirror or refraction rays that escape, take the
+    # camera's background colour (rt/camera.h:234-240, rt/raytracer.cc:82-86).
+    from raptor_amd import workloads
+
+    w, mats, kinds, flags = workloads.material_scene(res=96, mirror=0.6 if with_materials else None, floor=False)
+    if not with_materials:
+        mats = kinds = flags = None
+    bg = (40.0, 90.0, 200.0)
+    scene = rt.Scene(gpu_ctx, w.abc, w.albedo, flags)
+    if with_materials:
+        scene.set_materials(mats, kinds)
+    accel = rt.Accel(scene)
+    out = accel.render(w.camera, w.lights, background=bg)
+    ref = O.OracleScene(w.abc, flags).render(w.albedo, w.camera, w.lights, nthreads=8, materials=mats,
+                                             tri_material=kinds, background=bg)
+    bad, ties = compare_hits(out["primary"], ref["prim"], ref["tuv"])
+    assert not bad.any()
+    e = rel_err(out["rgb"].reshape(-1, 3)[~ties], ref["rgb"][~ties])
+    assert e.max() <= RAD_TOL, e.max()
+    miss = ref["prim"] < 0
+    assert miss.sum() > 100 and np.all(out["rgb"].reshape(-1, 3)[miss] == np.asarray(bg, np.float32))
+    black = accel.render(w.camera, w.lights)
+    hit = ~miss
+    if with_materials:  # escaping secondary rays brighten some hit pixels
+        assert (out["rgb"].reshape(-1, 3)[hit] > black["rgb"].reshape(-1, 3)[hit]).any()
+    else:
+        assert np.array_equal(out["rgb"].reshape(-1, 3)[hit], black["rgb"].reshape(-1, 3)[hit])
