@@ -277,8 +277,11 @@ int32_t rt_scene_create_from_mesh(rt_ctx *ctx, const rt_mesh *mesh, rt_scene **o
  * rt_image_tone_map replaces camera::tone_map (rt/camera.cc:1124-1461) with
  * glare off: mode is tone_mapping_mode_t (rt/camera.h:43-45): 0 none,
  * 1 global_contrast, 4 global_non_linear, 5 global_exposure,
- * 6 global_avg_luminance, 7 global_max_luminance, 9 global_ferwerda; the
- * local modes 2, 3 (histogram) and 8 (bilateral) return RT_ERR_UNSUPPORTED.
+ * 6 global_avg_luminance, 7 global_max_luminance, 8 global_bilateral_filter
+ * (camera::bilateral_filter_tone_map, rt/camera.cc:1632-1775: a bilateral
+ * grid blurred by a circular Gaussian, evaluated directly in double where
+ * the reference uses single-precision FFTs), 9 global_ferwerda; the
+ * histogram modes 2 and 3 return RT_ERR_UNSUPPORTED.
  * key, xw, yw, desaturate (ds), stretch (sc) and adapt (ta) with time_step
  * are its arguments; adaptation_level is the camera's adaptation state (read
  * when adapting, always written), log_avg_luminance receives the adapted

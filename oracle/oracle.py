@@ -110,6 +110,7 @@ def _load():
         "oracle_packet_trace_occluded": (None, [vp, vp, i64, vp, i32]),
         "oracle_tone_map": (i32, [vp, i32, i32, C.POINTER(oracle_tone_params), vp, vp]),
         "oracle_gamma": (None, [vp, i64, f]),
+        "oracle_gauss_circular3d": (None, [vp, i32, i32, i32, f, f]),
         "oracle_clip": (None, [vp, i32, i32, i32, i32, i32, vp]),
     }
     for name, (res, args) in sig.items():
@@ -416,6 +417,14 @@ def tone_map(rgb, mode=0, key=0.18, xw=1.0 / 3.0, yw=1.0 / 3.0, desaturate=False
     if LIB.oracle_tone_map(_p(img), w, h, C.byref(tp), _p(lvl), _p(yi)) != 0:
         raise ValueError(f"tone mapping mode {mode} is not restated")
     return img, float(lvl[0]), float(yi[0])
+
+
+def gauss_circular3d(grid, ss=1.0, sr=1.0):
+    """The bilateral grid's circular Gaussian convolution of a (2, D, H, W)
+    float32 grid (the two channels the reference convolves)."""
+    g = np.array(grid, np.float32, order="C", copy=True)
+    LIB.oracle_gauss_circular3d(_p(g), g.shape[3], g.shape[2], g.shape[1], ss, sr)
+    return g
 
 
 def gamma_correct(rgb, gamma):

@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <vector>
 
 #include "rt_oracle.h"
 
@@ -26,7 +27,132 @@ void yxy_to_rgb(float *p) {
     p[2] = (14.2035f * X) + (-52.02f * Y) + (269.535f * Z);
 }
 
+// The circular convolution the reference performs with FFTW
+// (rt/camera.cc:1697-1734): both channels of a W x H x D grid (channel c at
+// c * W * H * D, cell (x, y, z) at x + W * (y + H * z)) convolved with the
+// periodic kernel exp(-0.5 * ((X^2 + Y^2) / ss^2 + Z^2 / sr^2)), X the offset
+// wrapped to (-W/2, W/2] as the reference builds it. An r2c / c2r DFT pair
+// with the 1 / (W H D) scale is this sum; it is evaluated directly here in
+// double (the reference's single-precision FFT rounds differently).
+void gauss_circular3d(float *grid, int W, int H, int D, float ss, float sr) {
+    const int64_t N = static_cast<int64_t>(W) * H * D;
+    std::vector<double> kx(W), ky(H), kz(D);
+    auto wrap = [](int i, int n) { return static_cast<double>(i - ((i > (n >> 1)) ? n : 0)); };
+    for (int i = 0; i < W; ++i) kx[i] = wrap(i, W) * wrap(i, W) / (double(ss) * ss);
+    for (int i = 0; i < H; ++i) ky[i] = wrap(i, H) * wrap(i, H) / (double(ss) * ss);
+    for (int i = 0; i < D; ++i) kz[i] = wrap(i, D) * wrap(i, D) / (double(sr) * sr);
+    std::vector<double> in(N), out(N);
+    for (int c = 0; c < 2; ++c) {
+        float *g = grid + c * N;
+        for (int64_t i = 0; i < N; ++i) in[i] = g[i];
+        // Separable passes: the kernel is a product of its axis factors.
+        auto pass = [&](int axis) {
+            const int n = axis == 0 ? W : (axis == 1 ? H : D);
+            const std::vector<double> &k = axis == 0 ? kx : (axis == 1 ? ky : kz);
+            std::vector<double> f(n);
+            for (int i = 0; i < n; ++i) f[i] = std::exp(-0.5 * k[i]);
+            for (int z = 0; z < D; ++z)
+                for (int y = 0; y < H; ++y)
+                    for (int x = 0; x < W; ++x) {
+                        const int o = axis == 0 ? x : (axis == 1 ? y : z);
+                        double acc = 0.0;
+                        for (int j = 0; j < n; ++j) {
+                            const int src = (o - j + n) % n;
+                            const int xs = axis == 0 ? src : x, ys = axis == 1 ? src : y, zs = axis == 2 ? src : z;
+                            acc += in[xs + static_cast<int64_t>(W) * (ys + static_cast<int64_t>(H) * zs)] * f[j];
+                        }
+                        out[x + static_cast<int64_t>(W) * (y + static_cast<int64_t>(H) * z)] = acc;
+                    }
+            in.swap(out);
+        };
+        pass(0);
+        pass(1);
+        pass(2);
+        for (int64_t i = 0; i < N; ++i) g[i] = static_cast<float>(in[i]);
+    }
+}
+
+// camera::bilateral_filter_tone_map (rt/camera.cc:1632-1775) on the Y channel
+// of an (x, Y, y) image, with the arguments global_bilateral_filter passes
+// (rt/camera.cc:1354-1358). Pixels with Y <= 3.18e-7 have no log intensity in
+// the reference (its array entry is never written; a fresh allocation of a
+// frame's size reads zero), so they take 0; their grid depth index is
+// clamped to the grid (the reference would index outside it when the
+// smallest log intensity is positive).
+void bilateral_tone_map(float *img, int x_res, int y_res) {
+    const int n = x_res * y_res;
+    const float s_s = 0.02f * static_cast<float>(std::min(x_res, y_res));
+    const float r_s = 0.4f, r_sa = 0.4f, s_sa = s_s;
+    std::vector<float> logi(n, 0.0f);
+    float input_max = -kMaxDist, input_min = kMaxDist;
+    for (int i = 0; i < n; ++i) {
+        const float g = img[3 * static_cast<int64_t>(i) + 1];
+        if (g > 3.18e-7f) {
+            logi[i] = static_cast<float>(::log10(static_cast<double>(g)));
+            input_max = std::max(input_max, logi[i]);
+            input_min = std::min(input_min, logi[i]);
+        }
+    }
+    const float input_delta = input_max - input_min;
+    const float sigma_r = r_s / r_sa, sigma_s = s_s / s_sa;
+    const int pxy = static_cast<int>(2.0f * sigma_s) + 1, pz = static_cast<int>(2.0f * sigma_r) + 1;
+    const int W = static_cast<int>(static_cast<float>(x_res - 1) / s_sa) + 1 + (pxy << 1);
+    const int H = static_cast<int>(static_cast<float>(y_res - 1) / s_sa) + 1 + (pxy << 1);
+    const int D = static_cast<int>(input_delta / r_sa) + 1 + (pz << 1);
+    const int64_t N = static_cast<int64_t>(W) * H * D;
+    std::vector<float> grid(2 * N, 0.0f);
+    for (int x = 0; x < x_res; ++x)
+        for (int y = 0; y < y_res; ++y) {
+            const float l = logi[x + x_res * y];
+            const int sx = static_cast<int>(static_cast<float>(x) / s_sa + 0.5f) + pxy;
+            const int sy = static_cast<int>(static_cast<float>(y) / s_sa + 0.5f) + pxy;
+            const int sz = std::max(0, std::min(D - 1, static_cast<int>((l - input_min) / r_sa + 0.5f) + pz));
+            const int64_t c = sx + static_cast<int64_t>(W) * (sy + static_cast<int64_t>(H) * sz);
+            grid[c] += 1.0f;
+            grid[c + N] += l;
+        }
+    gauss_circular3d(grid.data(), W, H, D, sigma_s, sigma_r);
+    std::vector<float> result(n);
+    float max_value = -kMaxDist, min_value = kMaxDist;
+    for (int x = 0; x < x_res; ++x)
+        for (int y = 0; y < y_res; ++y) {
+            const float z = logi[x + x_res * y] - input_min;
+            const float xa = static_cast<float>(x) / s_sa + pxy;
+            const int xi = std::max(0, std::min(static_cast<int>(xa), W - 1)), xxi = std::max(0, std::min(xi + 1, W - 1));
+            const float ya = static_cast<float>(y) / s_sa + pxy;
+            const int yi = std::max(0, std::min(static_cast<int>(ya), H - 1)), yyi = std::max(0, std::min(yi + 1, H - 1));
+            const float za = z / r_sa + pz;
+            const int zi = std::max(0, std::min(static_cast<int>(za), D - 1)), zzi = std::max(0, std::min(zi + 1, D - 1));
+            const float ax = xa - xi, ay = ya - yi, az = za - zi;
+            auto at = [&](int c, int i, int j, int k) {
+                return grid[c * N + i + static_cast<int64_t>(W) * (j + static_cast<int64_t>(H) * k)];
+            };
+            auto tri = [&](int c) {
+                return (1.0f - ax) * (1.0f - ay) * (1.0f - az) * at(c, xi, yi, zi) +
+                       ax * (1.0f - ay) * (1.0f - az) * at(c, xxi, yi, zi) +
+                       (1.0f - ax) * ay * (1.0f - az) * at(c, xi, yyi, zi) +
+                       ax * ay * (1.0f - az) * at(c, xxi, yyi, zi) +
+                       (1.0f - ax) * (1.0f - ay) * az * at(c, xi, yi, zzi) +
+                       ax * (1.0f - ay) * az * at(c, xxi, yi, zzi) +
+                       (1.0f - ax) * ay * az * at(c, xi, yyi, zzi) + ax * ay * az * at(c, xxi, yyi, zzi);
+            };
+            const float r = tri(1) / tri(0);
+            result[x + x_res * y] = r;
+            max_value = std::max(max_value, r);
+            min_value = std::min(min_value, r);
+        }
+    const float gamma = std::log10(300.0f) / (max_value - min_value);
+    for (int i = 0; i < n; ++i) {
+        const float v = std::pow(10.0f, (result[i] * gamma + (logi[i] - result[i])));
+        img[3 * static_cast<int64_t>(i) + 1] = v * (1.0f / 255.0f);
+    }
+}
+
 }  // namespace
+
+extern "C" void oracle_gauss_circular3d(float *grid, int32_t W, int32_t H, int32_t D, float ss, float sr) {
+    gauss_circular3d(grid, W, H, D, ss, sr);
+}
 
 extern "C" int32_t oracle_tone_map(float *img, int32_t x_res, int32_t y_res, const oracle_tone_params *tp,
                                    float *adaptation_level, float *yi_out) {
@@ -35,7 +161,7 @@ extern "C" int32_t oracle_tone_map(float *img, int32_t x_res, int32_t y_res, con
     static const float zb[3] = {0.0193f, 0.1192f, 0.9505f};
     static const float vb[3] = {0.01477f, 0.497f, 0.631f};
     const int mode = tp->mode;
-    if (mode == 2 || mode == 3 || mode == 8 || mode < 0 || mode > 9 || tp->glare) return -1;
+    if (mode == 2 || mode == 3 || mode < 0 || mode > 9 || tp->glare) return -1;
     const int n = x_res * y_res;
     float Yi = 0.0f, maxY = 0.0f, rgbAvg = 0.0f;
     int numpix = 0;
@@ -146,6 +272,8 @@ extern "C" int32_t oracle_tone_map(float *img, int32_t x_res, int32_t y_res, con
     } else if (mode == 7) {  // global_max_luminance
         const float sf = key / maxY;
         for (int i = 0; i < n; ++i) img[3 * static_cast<int64_t>(i) + 1] *= sf;
+    } else if (mode == 8) {  // global_bilateral_filter
+        bilateral_tone_map(img, x_res, y_res);
     } else if (mode == 9) {  // global_ferwerda
         const float lda = 800.0f;
         const float log_lda = std::log(lda * 0.5f);

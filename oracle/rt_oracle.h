@@ -167,6 +167,9 @@ typedef struct {
 int32_t oracle_tone_map(float *img, int32_t x_res, int32_t y_res, const oracle_tone_params *tp,
                         float *adaptation_level, float *yi_out);
 void oracle_gamma(float *img, int64_t npix, float gamma);
+/* The bilateral grid's circular Gaussian convolution (rt/camera.cc:1697-1734)
+ * of both channels of a W x H x D grid, in place. */
+void oracle_gauss_circular3d(float *grid, int32_t W, int32_t H, int32_t D, float ss, float sr);
 void oracle_clip(const float *img, int32_t x_res, int32_t y_res, int32_t xs, int32_t ys, int32_t bgr, uint8_t *out);
 
 #ifdef __cplusplus

@@ -20,6 +20,7 @@
 #include <cmath>
 #include <fstream>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "rt_internal.hpp"
@@ -33,7 +34,9 @@ constexpr int kPostMaxBlocks = 2048;
 constexpr float kMaxDistF = 3.402823466e+38f;
 
 // tone_mapping_mode_t (rt/camera.h:43-45).
-enum ToneMode : int { kNone = 0, kContrast = 1, kNonLinear = 4, kExposure = 5, kAvgLum = 6, kMaxLum = 7, kFerwerda = 9 };
+enum ToneMode : int {
+    kNone = 0, kContrast = 1, kNonLinear = 4, kExposure = 5, kAvgLum = 6, kMaxLum = 7, kBilateral = 8, kFerwerda = 9
+};
 
 struct TonePartial {
     double log_sum;
@@ -223,6 +226,9 @@ __global__ void __launch_bounds__(kPostThreads) k_tone_scalars(ToneArgs a, const
         case kMaxLum:
             t.sf = a.key / maxY;
             break;
+        case kBilateral:  // the bilateral kernels have written the mapped Y
+            t.sf = 1.0f;
+            break;
         case kFerwerda: {
             const float lda = 800.0f;
             const float log_lda = logf(lda * 0.5f);
@@ -399,8 +405,282 @@ __global__ void __launch_bounds__(kPostThreads) k_clip(const float *__restrict__
     }
 }
 
+// ---- global_bilateral_filter: camera::bilateral_filter_tone_map
+// (rt/camera.cc:1632-1775), the bilateral grid of Paris and Durand: log10 Y
+// splatted into a grid coarser by s_sa in x and y and by 0.4 in log
+// intensity, blurred by a circular Gaussian of one cell in each axis, sliced
+// back per pixel by trilinear interpolation (the base layer), and the base
+// compressed to a contrast of 300 while the detail (log Y - base) is kept.
+// The reference convolves with single-precision FFTs; a circular convolution
+// is that same sum, evaluated here as three separable passes in double. Every
+// sum has a fixed order (the splat reduces per-lane partials in lane order),
+// so frames repeat bit for bit.
+
+struct BilGrid {
+    int W, H, D;
+    int pxy, pz;
+    float s_sa, r_sa;
+    float in_min;
+    int64_t N;  // cells per channel
+};
+
+// log10 Y of the bright pixels (the reference's unqualified log10 on a float
+// is the double one), 0 for the others (never written in the reference; a
+// fresh allocation of a frame reads zero), and per-block min / max of the
+// bright ones.
+__global__ void __launch_bounds__(kPostThreads) k_bil_log(const float *__restrict__ img, int64_t n,
+                                                          float *__restrict__ logi, float2 *__restrict__ part) {
+    __shared__ float lds_f[kPostThreads / 64];
+    float lo = kMaxDistF, hi = -kMaxDistF;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kPostThreads + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * kPostThreads) {
+        const float g = img[3 * i + 1];
+        float l = 0.0f;
+        if (g > 3.18e-7f) {
+            l = static_cast<float>(log10(static_cast<double>(g)));
+            lo = fminf(lo, l);
+            hi = fmaxf(hi, l);
+        }
+        logi[i] = l;
+    }
+    lo = block_reduce<float>(lo, min_f, lds_f);
+    hi = block_reduce<float>(hi, max_f, lds_f);
+    if (threadIdx.x == 0) part[blockIdx.x] = make_float2(lo, hi);
+}
+
+__global__ void __launch_bounds__(kPostThreads) k_minmax_fold(const float2 *__restrict__ part, int nparts,
+                                                              float2 *__restrict__ out) {
+    __shared__ float lds_f[kPostThreads / 64];
+    float lo = kMaxDistF, hi = -kMaxDistF;
+    for (int b = threadIdx.x; b < nparts; b += kPostThreads) {
+        lo = fminf(lo, part[b].x);
+        hi = fmaxf(hi, part[b].y);
+    }
+    lo = block_reduce<float>(lo, min_f, lds_f);
+    hi = block_reduce<float>(hi, max_f, lds_f);
+    if (threadIdx.x == 0) *out = make_float2(lo, hi);
+}
+
+__global__ void k_fill_zero(float *__restrict__ p, int64_t n) {
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+        p[i] = 0.0f;
+}
+
+__device__ __forceinline__ int bil_cell_x(float x, float s_sa, int pxy) {
+    return static_cast<int>(x / s_sa + 0.5f) + pxy;
+}
+
+__device__ __forceinline__ int bil_cell_z(float l, const BilGrid &g) {
+    const int z = static_cast<int>((l - g.in_min) / g.r_sa + 0.5f) + g.pz;
+    return min(max(z, 0), g.D - 1);
+}
+
+constexpr int kBilZChunk = 32;
+
+// One wave per (x, y) cell: the lanes take the pixels of a window around the
+// cell in a fixed order, keep those that round to the cell, and add count
+// and log intensity into per-lane LDS columns, kBilZChunk depth cells at a
+// time; lane z then sums row z over the lanes in lane order.
+__global__ void __launch_bounds__(64) k_bil_splat(BilGrid g, const float *__restrict__ logi, int x_res, int y_res,
+                                                  int cells_x, float *__restrict__ grid) {
+    __shared__ float cnt[kBilZChunk][64];
+    __shared__ float sum[kBilZChunk][64];
+    const int lane = threadIdx.x;
+    const int cx = g.pxy + static_cast<int>(blockIdx.x % cells_x), cy = g.pxy + static_cast<int>(blockIdx.x / cells_x);
+    // Pixels whose rounded cell is (cx, cy) lie within s_sa / 2 of its centre.
+    const float fx = static_cast<float>(cx - g.pxy) * g.s_sa, fy = static_cast<float>(cy - g.pxy) * g.s_sa;
+    const int x0 = max(0, static_cast<int>(floorf(fx - 0.5f * g.s_sa)) - 1);
+    const int x1 = min(x_res - 1, static_cast<int>(ceilf(fx + 0.5f * g.s_sa)) + 1);
+    const int y0 = max(0, static_cast<int>(floorf(fy - 0.5f * g.s_sa)) - 1);
+    const int y1 = min(y_res - 1, static_cast<int>(ceilf(fy + 0.5f * g.s_sa)) + 1);
+    const int ww = x1 - x0 + 1, npix = ww * (y1 - y0 + 1);
+    for (int z0 = 0; z0 < g.D; z0 += kBilZChunk) {
+        for (int z = 0; z < kBilZChunk; ++z) {
+            cnt[z][lane] = 0.0f;
+            sum[z][lane] = 0.0f;
+        }
+        for (int k = lane; k < npix; k += 64) {
+            const int x = x0 + k % ww, y = y0 + k / ww;
+            if (bil_cell_x(static_cast<float>(x), g.s_sa, g.pxy) != cx ||
+                bil_cell_x(static_cast<float>(y), g.s_sa, g.pxy) != cy)
+                continue;
+            const float l = logi[x + static_cast<int64_t>(x_res) * y];
+            const int z = bil_cell_z(l, g) - z0;
+            if (z < 0 || z >= kBilZChunk) continue;
+            cnt[z][lane] += 1.0f;
+            sum[z][lane] += l;
+        }
+        __syncthreads();
+        if (lane < kBilZChunk && z0 + lane < g.D) {
+            float c = 0.0f, s = 0.0f;
+            for (int j = 0; j < 64; ++j) {
+                c += cnt[lane][j];
+                s += sum[lane][j];
+            }
+            const int64_t cell = cx + static_cast<int64_t>(g.W) * (cy + static_cast<int64_t>(g.H) * (z0 + lane));
+            grid[cell] = c;
+            grid[cell + g.N] = s;
+        }
+        __syncthreads();
+    }
+}
+
+// One separable pass of the circular Gaussian along axis (0 x, 1 y, 2 z) of
+// both channels: out[o] = sum_j in[o - j mod n] f[j].
+__global__ void __launch_bounds__(kPostThreads) k_bil_conv(BilGrid g, int axis, const float *__restrict__ f,
+                                                           const float *__restrict__ in, float *__restrict__ out) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kPostThreads + threadIdx.x;
+    if (i >= 2 * g.N) return;
+    const int64_t c = i / g.N, r = i - c * g.N;
+    const int x = static_cast<int>(r % g.W), y = static_cast<int>((r / g.W) % g.H), z = static_cast<int>(r / (static_cast<int64_t>(g.W) * g.H));
+    const int n = axis == 0 ? g.W : (axis == 1 ? g.H : g.D);
+    const int o = axis == 0 ? x : (axis == 1 ? y : z);
+    const int64_t stride = axis == 0 ? 1 : (axis == 1 ? g.W : static_cast<int64_t>(g.W) * g.H);
+    const float *src = in + i - static_cast<int64_t>(o) * stride;
+    double acc = 0.0;
+    for (int j = 0; j < n; ++j) {
+        int s = o - j;
+        s += (s < 0) ? n : 0;
+        acc += static_cast<double>(src[s * stride]) * static_cast<double>(f[j]);
+    }
+    out[i] = static_cast<float>(acc);
+}
+
+// Base layer per pixel: the trilinear slice of the blurred grid, IW / W, and
+// per-block min / max (rt/camera.cc:1739-1773).
+__global__ void __launch_bounds__(kPostThreads) k_bil_slice(BilGrid g, const float *__restrict__ grid,
+                                                            const float *__restrict__ logi, int x_res, int64_t n,
+                                                            float *__restrict__ base, float2 *__restrict__ part) {
+    __shared__ float lds_f[kPostThreads / 64];
+    float lo = kMaxDistF, hi = -kMaxDistF;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kPostThreads + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * kPostThreads) {
+        const int x = static_cast<int>(i % x_res), y = static_cast<int>(i / x_res);
+        const float z = logi[i] - g.in_min;
+        const float xa = static_cast<float>(x) / g.s_sa + g.pxy;
+        const int xi = max(0, min(static_cast<int>(xa), g.W - 1)), xxi = max(0, min(xi + 1, g.W - 1));
+        const float ya = static_cast<float>(y) / g.s_sa + g.pxy;
+        const int yi = max(0, min(static_cast<int>(ya), g.H - 1)), yyi = max(0, min(yi + 1, g.H - 1));
+        const float za = z / g.r_sa + g.pz;
+        const int zi = max(0, min(static_cast<int>(za), g.D - 1)), zzi = max(0, min(zi + 1, g.D - 1));
+        const float ax = xa - xi, ay = ya - yi, az = za - zi;
+        float v[2];
+        for (int c = 0; c < 2; ++c) {
+            const float *q = grid + c * g.N;
+            auto at = [&](int a, int b, int d) { return q[a + static_cast<int64_t>(g.W) * (b + static_cast<int64_t>(g.H) * d)]; };
+            v[c] = (1.0f - ax) * (1.0f - ay) * (1.0f - az) * at(xi, yi, zi) +
+                   ax * (1.0f - ay) * (1.0f - az) * at(xxi, yi, zi) +
+                   (1.0f - ax) * ay * (1.0f - az) * at(xi, yyi, zi) +
+                   ax * ay * (1.0f - az) * at(xxi, yyi, zi) +
+                   (1.0f - ax) * (1.0f - ay) * az * at(xi, yi, zzi) +
+                   ax * (1.0f - ay) * az * at(xxi, yi, zzi) +
+                   (1.0f - ax) * ay * az * at(xi, yyi, zzi) + ax * ay * az * at(xxi, yyi, zzi);
+        }
+        const float b = v[1] / v[0];
+        base[i] = b;
+        lo = fminf(lo, b);
+        hi = fmaxf(hi, b);
+    }
+    lo = block_reduce<float>(lo, min_f, lds_f);
+    hi = block_reduce<float>(hi, max_f, lds_f);
+    if (threadIdx.x == 0) part[blockIdx.x] = make_float2(lo, hi);
+}
+
+// Y = 10^(base * gamma + (log Y - base)) / 255 with gamma = log10(300) /
+// (max base - min base) (rt/camera.cc:1777-1784).
+__global__ void __launch_bounds__(kPostThreads) k_bil_apply(float *__restrict__ img, int64_t n,
+                                                            const float *__restrict__ logi,
+                                                            const float *__restrict__ base,
+                                                            const float2 *__restrict__ range, float log10_contrast) {
+    const float2 r = *range;
+    const float gamma = log10_contrast / (r.y - r.x);
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kPostThreads + threadIdx.x; i < n;
+         i += static_cast<int64_t>(gridDim.x) * kPostThreads) {
+        const float b = base[i];
+        const float v = powf(10.0f, (b * gamma + (logi[i] - b)));
+        img[3 * i + 1] = v * (1.0f / 255.0f);
+    }
+}
+
 int post_blocks(int64_t n) {
     return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(kPostMaxBlocks, (n + kPostThreads - 1) / kPostThreads)));
+}
+
+// The bilateral tone map of the Y channel of an (x, Y, y) image on the
+// device (the arguments global_bilateral_filter passes: r_s = r_sa = 0.4,
+// s_s = s_sa = 0.02 min(x_res, y_res)). The grid's depth depends on the
+// image's range of log intensity, so that range is read back once.
+int32_t bilateral(rt_ctx *ctx, float *img, int x_res, int y_res, int blocks) {
+    hipStream_t s = ctx->stream;
+    BuildArena &ar = ctx->arena;
+    const int64_t n = static_cast<int64_t>(x_res) * y_res;
+    void *v = nullptr;
+    auto take = [&](auto **p, int64_t count) -> hipError_t {
+        const hipError_t e = ar.take(&v, static_cast<size_t>(std::max<int64_t>(1, count)) * sizeof(**p));
+        *p = static_cast<std::remove_reference_t<decltype(*p)>>(v);
+        return e;
+    };
+    float *logi = nullptr, *base = nullptr;
+    float2 *part = nullptr, *range = nullptr;
+    RT_HIP_CHECK(ctx, take(&logi, n));
+    RT_HIP_CHECK(ctx, take(&base, n));
+    RT_HIP_CHECK(ctx, take(&part, blocks));
+    RT_HIP_CHECK(ctx, take(&range, 2));
+    hipLaunchKernelGGL(k_bil_log, dim3(blocks), dim3(kPostThreads), 0, s, img, n, logi, part);
+    hipLaunchKernelGGL(k_minmax_fold, dim3(1), dim3(kPostThreads), 0, s, part, blocks, range);
+    float2 mm;
+    RT_HIP_CHECK(ctx, hipMemcpyAsync(&mm, range, sizeof(mm), hipMemcpyDeviceToHost, s));
+    RT_HIP_CHECK(ctx, hipStreamSynchronize(s));
+    BilGrid g{};
+    const float s_s = 0.02f * static_cast<float>(std::min(x_res, y_res));
+    const float r_s = 0.4f;
+    g.r_sa = 0.4f;
+    g.s_sa = s_s;
+    const float sigma_r = r_s / g.r_sa, sigma_s = s_s / g.s_sa;
+    g.pxy = static_cast<int>(2.0f * sigma_s) + 1;
+    g.pz = static_cast<int>(2.0f * sigma_r) + 1;
+    g.in_min = mm.x;
+    const float delta = mm.y - mm.x;  // a frame with no bright pixel: -inf, an empty grid depth
+    if (!(delta >= 0.0f) || delta / g.r_sa > 4096.0f)
+        return fail(ctx, RT_ERR_INVALID, "rt_image_tone_map: the bilateral filter needs a frame with a finite range "
+                                         "of bright pixels");
+    g.W = static_cast<int>(static_cast<float>(x_res - 1) / g.s_sa) + 1 + (g.pxy << 1);
+    g.H = static_cast<int>(static_cast<float>(y_res - 1) / g.s_sa) + 1 + (g.pxy << 1);
+    g.D = static_cast<int>(delta / g.r_sa) + 1 + (g.pz << 1);
+    g.N = static_cast<int64_t>(g.W) * g.H * g.D;
+    const int cells_x = static_cast<int>(static_cast<float>(x_res - 1) / g.s_sa + 0.5f) + 1;
+    const int cells_y = static_cast<int>(static_cast<float>(y_res - 1) / g.s_sa + 0.5f) + 1;
+    // Axis factors exp(-X^2 / (2 sigma^2)), X the offset wrapped as the
+    // reference builds its kernel (rt/camera.cc:1668-1690).
+    std::vector<float> f(g.W + g.H + g.D);
+    auto axis = [&](float *out, int len, float sigma) {
+        for (int i = 0; i < len; ++i) {
+            const double X = i - ((i > (len >> 1)) ? len : 0);
+            out[i] = static_cast<float>(std::exp(-0.5 * X * X / (double(sigma) * sigma)));
+        }
+    };
+    axis(f.data(), g.W, sigma_s);
+    axis(f.data() + g.W, g.H, sigma_s);
+    axis(f.data() + g.W + g.H, g.D, sigma_r);
+    float *d_f = nullptr, *grid = nullptr, *tmp = nullptr;
+    RT_HIP_CHECK(ctx, take(&d_f, static_cast<int64_t>(f.size())));
+    RT_HIP_CHECK(ctx, take(&grid, 2 * g.N));
+    RT_HIP_CHECK(ctx, take(&tmp, 2 * g.N));
+    RT_HIP_CHECK(ctx, hipMemcpyAsync(d_f, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice, s));
+    const unsigned cb = static_cast<unsigned>((2 * g.N + kPostThreads - 1) / kPostThreads);
+    hipLaunchKernelGGL(k_fill_zero, dim3(std::min(cb, 2048u)), dim3(kPostThreads), 0, s, grid, 2 * g.N);
+    hipLaunchKernelGGL(k_bil_splat, dim3(static_cast<unsigned>(cells_x * cells_y)), dim3(64), 0, s, g, logi, x_res,
+                       y_res, cells_x, grid);
+    hipLaunchKernelGGL(k_bil_conv, dim3(cb), dim3(kPostThreads), 0, s, g, 0, d_f, grid, tmp);
+    hipLaunchKernelGGL(k_bil_conv, dim3(cb), dim3(kPostThreads), 0, s, g, 1, d_f + g.W, tmp, grid);
+    hipLaunchKernelGGL(k_bil_conv, dim3(cb), dim3(kPostThreads), 0, s, g, 2, d_f + g.W + g.H, grid, tmp);
+    hipLaunchKernelGGL(k_bil_slice, dim3(blocks), dim3(kPostThreads), 0, s, g, tmp, logi, x_res, n, base, part);
+    hipLaunchKernelGGL(k_minmax_fold, dim3(1), dim3(kPostThreads), 0, s, part, blocks, range);
+    hipLaunchKernelGGL(k_bil_apply, dim3(blocks), dim3(kPostThreads), 0, s, img, n, logi, base, range,
+                       std::log10(300.0f));
+    RT_HIP_CHECK(ctx, hipGetLastError());
+    return RT_OK;
 }
 
 }  // namespace
@@ -409,8 +689,8 @@ int32_t image_tone_map(rt_ctx *ctx, float *rgb, int32_t x_res, int32_t y_res, rt
                        float *ms) {
     hipStream_t s = ctx->stream;
     const int m = tp->mode;
-    if (m == 2 || m == 3 || m == 8)
-        return fail(ctx, RT_ERR_UNSUPPORTED, "rt_image_tone_map: the local tone mappers (histogram, bilateral) are not built");
+    if (m == 2 || m == 3)
+        return fail(ctx, RT_ERR_UNSUPPORTED, "rt_image_tone_map: the histogram tone mappers are not built");
     if (m < 0 || m > 9) return fail(ctx, RT_ERR_INVALID, "rt_image_tone_map: unknown mode " + std::to_string(m));
     const int64_t n = static_cast<int64_t>(x_res) * y_res;
     BuildArena &ar = ctx->arena;
@@ -438,6 +718,9 @@ int32_t image_tone_map(rt_ctx *ctx, float *rgb, int32_t x_res, int32_t y_res, rt
     if (ms) RT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[0], s));
     hipLaunchKernelGGL(k_tone_yxy, dim3(blocks), dim3(kPostThreads), 0, s, a, part);
     hipLaunchKernelGGL(k_tone_scalars, dim3(1), dim3(kPostThreads), 0, s, a, part, blocks, scal);
+    if (m == kBilateral) {
+        if (int32_t rc = bilateral(ctx, img, x_res, y_res, blocks)) return rc;
+    }
     hipLaunchKernelGGL(k_tone_apply, dim3(blocks), dim3(kPostThreads), 0, s, a, scal, minmax);
     if (tp->stretch) {
         hipLaunchKernelGGL(k_stretch_scalars, dim3(1), dim3(kPostThreads), 0, s, minmax, blocks, scal);

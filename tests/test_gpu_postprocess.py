@@ -120,10 +120,43 @@ def test_frame_to_png_pipeline(gpu_ctx, rt, O, tmp_path):
 
 def test_post_process_errors(gpu_ctx, rt):
     img = np.zeros((4, 4, 3), np.float32)
-    for mode in (2, 3, 8):
-        with pytest.raises(rt.RTError, match="local tone mappers"):
+    for mode in (2, 3):
+        with pytest.raises(rt.RTError, match="histogram tone mappers"):
             rt.tone_map(gpu_ctx, img, mode=mode)
+    with pytest.raises(rt.RTError, match="bilateral filter needs"):
+        rt.tone_map(gpu_ctx, img, mode=8)  # no bright pixel: no log-intensity range
     with pytest.raises(rt.RTError, match="unknown mode"):
         rt.tone_map(gpu_ctx, img, mode=12)
     with pytest.raises(rt.RTError, match="sample counts"):
         rt.clip(gpu_ctx, img, 5, 1)
+
+
+@pytest.mark.parametrize("ds,sc", [(False, False), (True, False), (False, True)])
+@pytest.mark.parametrize("res", [(320, 180), (256, 256)])
+def test_bilateral_tone_map_matches_oracle(gpu_ctx, rt, O, ds, sc, res):
+    # global_bilateral_filter (rt/camera.cc:1354-1358, 1632-1775). The grid
+    # is blurred in double on both sides (the reference: single-precision
+    # FFTs); the splat and the float pow round differently, so the bound is
+    # relative to the image's range.
+    img = _frame(rt, gpu_ctx, res)
+    g, _, gyi, _ = rt.tone_map(gpu_ctx, img, mode=8, desaturate=ds, stretch=sc)
+    o, _, oyi = O.tone_map(img, mode=8, desaturate=ds, stretch=sc)
+    assert np.isfinite(g).all()
+    assert abs(gyi / oyi - 1.0) < _tol(img)
+    err = np.abs(g.astype(np.float64) - o) / max(1.0, np.abs(o).max())
+    assert err.max() <= 2e-4, (err.max(), np.abs(o).max())
+    g2, _, _, _ = rt.tone_map(gpu_ctx, img, mode=8, desaturate=ds, stretch=sc)
+    assert np.array_equal(g, g2)  # fixed-order sums: the frame repeats bit for bit
+
+
+def test_bilateral_on_hdr_image(gpu_ctx, rt, O):
+    rng = np.random.default_rng(4)
+    h, w = 150, 230
+    y, x = np.mgrid[0:h, 0:w]
+    lum = 10.0 ** (3.0 * x / w - 1.0 + 0.3 * np.sin(y / 7.0)) * (1.0 + 0.2 * rng.uniform(-1, 1, (h, w)))
+    img = (lum[..., None] * np.array([1.0, 0.8, 0.6])).astype(np.float32)
+    img[60:70, 100:140] = 0.0  # a dark patch: no log intensity
+    g, _, _, _ = rt.tone_map(gpu_ctx, img, mode=8)
+    o, _, _ = O.tone_map(img, mode=8)
+    err = np.abs(g.astype(np.float64) - o) / max(1.0, np.abs(o).max())
+    assert err.max() <= 2e-4, err.max()

@@ -189,3 +189,59 @@ def test_tga_writer(tmp_path):
     data = path.read_bytes()
     assert data[:18] == bytes([0, 0, 2, 0, 0, 0, 0, 0, 0, 0, 0, 0, 300 & 255, 300 >> 8, 5, 0, 24, 0])
     assert data[18:-8] == img.tobytes() and data[-8:] == bytes(8)
+
+
+def test_bilateral_grid_blur_is_the_fft_circular_convolution():
+    # rt/camera.cc:1697-1734 convolves with r2c / c2r FFTs and a 1 / N scale:
+    # a circular convolution with the periodic Gaussian the reference builds.
+    rng = np.random.default_rng(0)
+    for shape, ss, sr in [((2, 9, 11, 14), 1.0, 1.0), ((2, 6, 8, 5), 1.3, 0.7)]:
+        g = rng.uniform(0, 1, shape).astype(np.float32)
+        out = O.gauss_circular3d(g, ss, sr)
+        _, D, H, W = shape
+
+        def wrap(n):
+            i = np.arange(n)
+            return i - np.where(i > (n >> 1), n, 0)
+
+        X, Y, Z = np.meshgrid(wrap(W), wrap(H), wrap(D), indexing="ij")
+        k = np.exp(-0.5 * ((X * X + Y * Y) / ss ** 2 + Z * Z / sr ** 2)).transpose(2, 1, 0)
+        kf = np.fft.rfftn(k, axes=(0, 1, 2))
+        for c in range(2):
+            ref = np.fft.irfftn(np.fft.rfftn(g[c].astype(np.float64), axes=(0, 1, 2)) * kf, s=(D, H, W),
+                                axes=(0, 1, 2))
+            assert np.abs(out[c] - ref).max() <= 1e-6 * np.abs(ref).max()
+
+
+def _hdr(h=90, w=140, seed=3, detail=0.0):
+    rng = np.random.default_rng(seed)
+    y, x = np.mgrid[0:h, 0:w]
+    logl = 2.5 * x / w - 0.5 + 0.4 * np.sin(y / 9.0) + 0.05 * rng.uniform(-1, 1, (h, w))
+    logl = logl + detail * np.where((x + y) % 2 == 0, 1.0, -1.0)
+    return (10.0 ** logl)[..., None] * np.array([1.0, 0.9, 0.7])
+
+
+def test_bilateral_scales_as_a_power_of_the_exposure():
+    # Scaling the image by c shifts every log intensity, the grid's origin
+    # and the base by log10 c, so the output scales by c^gamma, one factor
+    # for every pixel.
+    img = _hdr().astype(np.float32)
+    a, _, _ = O.tone_map(img, mode=8)
+    b, _, _ = O.tone_map((img * np.float32(4.0)).astype(np.float32), mode=8)
+    ratio = (b[..., 1] / a[..., 1]).ravel()
+    assert ratio.min() > 1.0 and (ratio.max() - ratio.min()) / ratio.mean() < 2e-4
+
+
+def test_bilateral_keeps_fine_detail():
+    # A checkerboard of +-0.1 decades sits inside one depth cell (0.4) and
+    # below the grid's spatial scale, so it is detail: the output keeps its
+    # phase and its amplitude (a log ratio of about 0.2 between neighbours),
+    # while the smooth part is compressed.
+    img = _hdr(detail=0.1).astype(np.float32)
+    out, _, _ = O.tone_map(img, mode=8)
+    lum = out[..., 1].astype(np.float64)
+    lin = img[..., 1].astype(np.float64)
+    d_out = np.log10(lum[20:70, 21:121] / lum[20:70, 20:120])
+    d_in = np.log10(lin[20:70, 21:121] / lin[20:70, 20:120])
+    assert np.all(np.sign(d_out) == np.sign(d_in))
+    assert np.abs(d_out - d_in).max() < 0.02 and abs(np.abs(d_in).mean() - 0.2) < 0.01
