@@ -280,8 +280,10 @@ int32_t rt_scene_create_from_mesh(rt_ctx *ctx, const rt_mesh *mesh, rt_scene **o
  * 6 global_avg_luminance, 7 global_max_luminance, 8 global_bilateral_filter
  * (camera::bilateral_filter_tone_map, rt/camera.cc:1632-1775: a bilateral
  * grid blurred by a circular Gaussian, evaluated directly in double where
- * the reference uses single-precision FFTs), 9 global_ferwerda; the
- * histogram modes 2 and 3 return RT_ERR_UNSUPPORTED.
+ * the reference uses single-precision FFTs), 9 global_ferwerda, and the
+ * histogram modes 2 local_histogram and 3 local_human_histogram
+ * (camera::histogram_tone_map, rt/camera.cc:1843-2062), which need x_angle
+ * and y_angle of at least one degree.
  * key, xw, yw, desaturate (ds), stretch (sc) and adapt (ta) with time_step
  * are its arguments; adaptation_level is the camera's adaptation state (read
  * when adapting, always written), log_avg_luminance receives the adapted
@@ -294,6 +296,9 @@ typedef struct {
     float time_step;
     float adaptation_level;
     float log_avg_luminance;
+    /* The camera's field of view in radians, 2 w / t and 2 h / t (modes 2
+     * and 3 average the frame to one sample per degree of view). */
+    float x_angle, y_angle;
 } rt_tone_params;
 int32_t rt_image_tone_map(rt_ctx *ctx, float *rgb, int32_t x_res, int32_t y_res, rt_tone_params *params,
                           uint32_t flags, float *ms);

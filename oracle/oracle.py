@@ -45,7 +45,7 @@ class oracle_render_out(C.Structure):
 class oracle_tone_params(C.Structure):
     _fields_ = [("mode", C.c_int32), ("key", C.c_float), ("xw", C.c_float), ("yw", C.c_float),
                 ("desaturate", C.c_int32), ("stretch", C.c_int32), ("adapt", C.c_int32), ("glare", C.c_int32),
-                ("time_step", C.c_float)]
+                ("time_step", C.c_float), ("x_angle", C.c_float), ("y_angle", C.c_float)]
 
 
 def material_array(materials):
@@ -406,12 +406,13 @@ def refract_dir(d, n, ri, leaving):
 
 
 def tone_map(rgb, mode=0, key=0.18, xw=1.0 / 3.0, yw=1.0 / 3.0, desaturate=False, stretch=False, adapt=False,
-             time_step=0.0, adaptation_level=0.0):
+             time_step=0.0, adaptation_level=0.0, x_angle=0.0, y_angle=0.0):
     """camera::tone_map on a copy of rgb ((H, W, 3) float32, 0-255); returns
     (image, adaptation level, adapted log-average luminance)."""
     img = np.array(rgb, np.float32, order="C", copy=True)
     h, w = img.shape[0], img.shape[1]
-    tp = oracle_tone_params(int(mode), key, xw, yw, int(desaturate), int(stretch), int(adapt), 0, time_step)
+    tp = oracle_tone_params(int(mode), key, xw, yw, int(desaturate), int(stretch), int(adapt), 0, time_step, x_angle,
+                            y_angle)
     lvl = np.array([adaptation_level], np.float32)
     yi = np.zeros(1, np.float32)
     if LIB.oracle_tone_map(_p(img), w, h, C.byref(tp), _p(lvl), _p(yi)) != 0:

@@ -148,6 +148,143 @@ void bilateral_tone_map(float *img, int x_res, int y_res) {
     }
 }
 
+// camera::just_noticable_difference (rt/camera.cc:1778-1806); log10 of a
+// float unqualified in the reference is the double one.
+float jnd(float La) {
+    const float lLa = static_cast<float>(::log10(static_cast<double>(La)));
+    float lLt;
+    if (lLa < -3.94f) {
+        lLt = -2.86f;
+    } else if (lLa < -1.44f) {
+        lLt = std::pow(0.405f * lLa + 1.6f, 2.18f) - 2.86f;
+    } else if (lLa < -0.0184f) {
+        lLt = lLa - 0.395f;
+    } else if (lLa < 1.9f) {
+        lLt = std::pow(0.249f * lLa + 0.65f, 2.7f) - 0.72f;
+    } else {
+        lLt = lLa - 1.255f;
+    }
+    return std::pow(10.0f, lLt);
+}
+
+// camera::histogram_tone_map (rt/camera.cc:1843-2062), Ward Larson's
+// histogram adjustment, on the Y channel of an (x, Y, y) image: the frame
+// averaged to one sample per degree of view, a 100-bin histogram of log
+// luminance whose bins are cut to a ceiling (linear, or scaled by the
+// just-noticeable-difference ratio when human is set) until fewer than 2.5%
+// of the samples are trimmed, and each pixel mapped through the cumulative
+// distribution to log display luminance in [log 1, log 300].
+void histogram_tone_map(float *img, int x_res, int y_res, bool human, float x_angle, float y_angle, float lw_avg) {
+    auto G = [&](int i) -> float & { return img[3 * static_cast<int64_t>(i) + 1]; };
+    const float PI = 3.1415926535897932384626433f;
+    const int sxr = static_cast<int>(x_angle * (360.0f / (2.0f * PI)));
+    const int syr = static_cast<int>(y_angle * (360.0f / (2.0f * PI)));
+    std::vector<float> sl(static_cast<size_t>(std::max(0, sxr * syr)), 0.0f);
+    const float sxi = static_cast<float>(x_res) / static_cast<float>(sxr);
+    const float syi = static_cast<float>(y_res) / static_cast<float>(syr);
+    float y0 = 0.0f, y1 = syi;
+    for (int j = 0; j < syr; ++j) {
+        float x0 = 0.0f, x1 = sxi;
+        for (int i = 0; i < sxr; ++i) {
+            float &acc = sl[j * sxr + i];
+            // (a single sample across would reach one past the image: stop inside)
+            const int ye = std::min(static_cast<int>(y1), y_res - 1), xe = std::min(static_cast<int>(x1), x_res - 1);
+            for (int y = static_cast<int>(y0); y <= ye; ++y)
+                for (int x = static_cast<int>(x0); x <= xe; ++x) acc += G(y * x_res + x);
+            acc /= ((x1 - x0) + 1) * ((y1 - y0) + 1);
+            x0 = x1;
+            x1 = std::min((x_res - 1.0f), (x1 + sxi));
+        }
+        y0 = y1;
+        y1 = std::min((y_res - 1.0f), (y1 + syi));
+    }
+    const int bins = 100;
+    int pix_gt0 = 0;
+    float lw_min = kMaxDist, lw_max = 0.0f;
+    for (const float v : sl)
+        if (v > 0.0f) {
+            lw_min = std::min(lw_min, v);
+            lw_max = std::max(lw_max, v);
+            ++pix_gt0;
+        }
+    if (lw_min <= 0.0f) return;  // black image (never reached: lw_min starts at MAX_DIST)
+    lw_min = std::max(lw_min, 1.0e-4f);
+    const float bw_min = std::log(lw_min);
+    float bw_max = std::log(lw_max);
+    if (bw_max == bw_min) bw_max += 0.0001f * std::fabs(bw_max);
+    const float ld_max = 300.0f, ld_min = 1.0f;
+    const float bd_max = std::log(ld_max), bd_min = std::log(ld_min);
+    const int n = x_res * y_res;
+    if ((bd_max - bd_min) > (bw_max - bw_min)) {
+        const float sf = (bd_max - bd_min) / (bw_max - bw_min);
+        for (int i = 0; i < n; ++i) G(i) = G(i) * sf + bd_min;
+        return;
+    }
+    int his[bins + 1] = {0};
+    float cdf[bins + 1] = {0.0f};
+    for (const float v : sl)
+        if (v > 0.0f) {
+            int b = static_cast<int>((std::log(v) - bw_min) / (bw_max - bw_min) * bins) + 1;
+            b = std::min(std::max(b, 0), bins);
+            his[b]++;
+        }
+    float sum = 0.0f;
+    for (int i = 1; i <= bins; ++i) {
+        sum += his[i];
+        cdf[i] = sum / static_cast<float>(pix_gt0);
+    }
+    const float tolerance = 0.025f * static_cast<float>(pix_gt0);
+    const float bin_step_size = (bw_max - bw_min) / bins;
+    const float lw_step = 1.0f / static_cast<float>(bins) * (bw_max - bw_min);
+    int trimmings;
+    do {
+        trimmings = 0;
+        if (pix_gt0 < tolerance) {
+            const float num = 1.219f + std::pow(ld_max * 0.5f, 0.4f);
+            const float den = 1.219f + std::pow(lw_avg, 0.4f);
+            const float sf = (1.0f / ld_max) * std::pow((num / den), 2.5f);
+            for (int i = 0; i < n; ++i) G(i) *= sf;
+            return;
+        }
+        for (int i = 1; i <= bins; ++i) {
+            float ceiling = (pix_gt0 * bin_step_size) / (bd_max - bd_min);
+            if (human) {
+                const float bde = bd_min + (bd_max - bd_min) * cdf[i];
+                const float ld = std::exp(bde);
+                const float bw = i * lw_step + bw_min;
+                const float lw = std::exp(bw);
+                ceiling *= (jnd(ld) * lw) / (jnd(lw) * ld);
+            }
+            if (his[i] > ceiling) {
+                trimmings += (his[i] - static_cast<int>(ceiling));
+                his[i] = static_cast<int>(ceiling);
+            }
+        }
+        pix_gt0 = 0;
+        for (int i = 1; i <= bins; ++i) pix_gt0 += his[i];
+        float s2 = 0.0f;
+        for (int i = 1; i <= bins; ++i) {
+            s2 += his[i];
+            cdf[i] = s2 / static_cast<float>(pix_gt0);
+        }
+    } while (trimmings > tolerance);
+    for (int i = 0; i < n; ++i) {
+        float &g = G(i);
+        if (g > 0.0f) {
+            const float bw = std::log(g);
+            float bf = (bw - bw_min) / (bw_max - bw_min) * bins;
+            bf = std::min(std::max(bf, 0.0f), static_cast<float>(bins));
+            const int b0 = static_cast<int>(bf);
+            const int b1 = std::min((b0 + 1), bins);
+            const float s = bf - b0;
+            const float pbw = (1 - s) * cdf[b0] + s * cdf[b1];
+            const float bde = bd_min + (bd_max - bd_min) * pbw;
+            g = std::exp(bde);
+            g /= ld_max;
+        }
+    }
+}
+
 }  // namespace
 
 extern "C" void oracle_gauss_circular3d(float *grid, int32_t W, int32_t H, int32_t D, float ss, float sr) {
@@ -161,7 +298,10 @@ extern "C" int32_t oracle_tone_map(float *img, int32_t x_res, int32_t y_res, con
     static const float zb[3] = {0.0193f, 0.1192f, 0.9505f};
     static const float vb[3] = {0.01477f, 0.497f, 0.631f};
     const int mode = tp->mode;
-    if (mode == 2 || mode == 3 || mode < 0 || mode > 9 || tp->glare) return -1;
+    if (mode < 0 || mode > 9 || tp->glare) return -1;
+    if ((mode == 2 || mode == 3) && !(tp->x_angle * (360.0f / (2.0f * 3.1415926535897932384626433f)) >= 1.0f &&
+                                      tp->y_angle * (360.0f / (2.0f * 3.1415926535897932384626433f)) >= 1.0f))
+        return -2;
     const int n = x_res * y_res;
     float Yi = 0.0f, maxY = 0.0f, rgbAvg = 0.0f;
     int numpix = 0;
@@ -272,6 +412,8 @@ extern "C" int32_t oracle_tone_map(float *img, int32_t x_res, int32_t y_res, con
     } else if (mode == 7) {  // global_max_luminance
         const float sf = key / maxY;
         for (int i = 0; i < n; ++i) img[3 * static_cast<int64_t>(i) + 1] *= sf;
+    } else if (mode == 2 || mode == 3) {  // local_histogram, local_human_histogram
+        histogram_tone_map(img, x_res, y_res, mode == 3, tp->x_angle, tp->y_angle, *adaptation_level);
     } else if (mode == 8) {  // global_bilateral_filter
         bilateral_tone_map(img, x_res, y_res);
     } else if (mode == 9) {  // global_ferwerda

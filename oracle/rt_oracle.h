@@ -163,6 +163,8 @@ typedef struct {
     float key, xw, yw;
     int32_t desaturate, stretch, adapt, glare;
     float time_step;
+    /* The camera's field of view, 2 w / t and 2 h / t in radians (modes 2, 3). */
+    float x_angle, y_angle;
 } oracle_tone_params;
 int32_t oracle_tone_map(float *img, int32_t x_res, int32_t y_res, const oracle_tone_params *tp,
                         float *adaptation_level, float *yi_out);

@@ -35,7 +35,7 @@ constexpr float kMaxDistF = 3.402823466e+38f;
 
 // tone_mapping_mode_t (rt/camera.h:43-45).
 enum ToneMode : int {
-    kNone = 0, kContrast = 1, kNonLinear = 4, kExposure = 5, kAvgLum = 6, kMaxLum = 7, kBilateral = 8, kFerwerda = 9
+    kNone = 0, kContrast = 1, kHistogram = 2, kHumanHistogram = 3, kNonLinear = 4, kExposure = 5, kAvgLum = 6, kMaxLum = 7, kBilateral = 8, kFerwerda = 9
 };
 
 struct TonePartial {
@@ -84,6 +84,15 @@ struct ToneArgs {
     float key, xw, yw;
     int desaturate, stretch, adapt;
     float time_step, level_in;
+};
+
+// The histogram modes' mapping (rt/camera.cc:1843-2062): the branch the
+// reference takes and the cumulative distribution it maps through.
+constexpr int kHistBins = 100;
+struct HistState {
+    int branch;  // 0 unchanged, 1 linear g * sf + bd_min, 2 g * sf, 3 the distribution
+    float sf, bw_min, bw_max, bd_min, bd_max;
+    float cdf[kHistBins + 1];
 };
 
 // Per pixel: rt/camera.cc:1137-1195.
@@ -227,6 +236,8 @@ __global__ void __launch_bounds__(kPostThreads) k_tone_scalars(ToneArgs a, const
             t.sf = a.key / maxY;
             break;
         case kBilateral:  // the bilateral kernels have written the mapped Y
+        case kHistogram:  // k_hist_build decides the mapping
+        case kHumanHistogram:
             t.sf = 1.0f;
             break;
         case kFerwerda: {
@@ -267,6 +278,24 @@ __global__ void __launch_bounds__(kPostThreads) k_tone_scalars(ToneArgs a, const
     *out = t;
 }
 
+__device__ __forceinline__ float map_histogram(const HistState &h, float g) {
+    if (h.branch == 1) return g * h.sf + h.bd_min;
+    if (h.branch == 2) return g * h.sf;
+    if (h.branch == 3 && g > 0.0f) {
+        const float bw = logf(g);
+        float bf = (bw - h.bw_min) / (h.bw_max - h.bw_min) * kHistBins;
+        bf = fminf(fmaxf(bf, 0.0f), static_cast<float>(kHistBins));
+        const int b0 = static_cast<int>(bf);
+        const int b1 = min((b0 + 1), kHistBins);
+        const float s = bf - b0;
+        const float pbw = (1 - s) * h.cdf[b0] + s * h.cdf[b1];
+        const float bde = h.bd_min + (h.bd_max - h.bd_min) * pbw;
+        g = expf(bde);
+        g /= 300.0f;
+    }
+    return g;
+}
+
 __device__ __forceinline__ float map_luminance(int mode, const ToneScalars &t, float g) {
     switch (mode) {
         case kNonLinear:
@@ -300,14 +329,21 @@ __device__ __forceinline__ void yxy_to_rgb(float *p, float x, float Y, float y) 
 // Mapped luminance, then rgb; with the stretch, the mapped Y and per-block
 // min / max only.
 __global__ void __launch_bounds__(kPostThreads) k_tone_apply(ToneArgs a, const ToneScalars *__restrict__ ts,
+                                                             const HistState *__restrict__ hist,
                                                              float2 *__restrict__ minmax) {
     __shared__ float lds_f[kPostThreads / 64];
+    __shared__ HistState hs;
     const ToneScalars t = *ts;
+    const bool histogram = a.mode == kHistogram || a.mode == kHumanHistogram;
+    if (histogram) {
+        if (threadIdx.x == 0) hs = *hist;
+        __syncthreads();
+    }
     float lo = kMaxDistF, hi = -kMaxDistF;
     for (int64_t i = static_cast<int64_t>(blockIdx.x) * kPostThreads + threadIdx.x; i < a.n;
          i += static_cast<int64_t>(gridDim.x) * kPostThreads) {
         float *p = a.img + 3 * i;
-        const float g = map_luminance(a.mode, t, p[1]);
+        const float g = histogram ? map_histogram(hs, p[1]) : map_luminance(a.mode, t, p[1]);
         if (a.stretch) {
             p[1] = g;
             lo = fminf(lo, g);
@@ -603,6 +639,170 @@ __global__ void __launch_bounds__(kPostThreads) k_bil_apply(float *__restrict__ 
     }
 }
 
+// ---- local_histogram / local_human_histogram (rt/camera.cc:1843-2062).
+
+// One thread per sample of the one-sample-per-degree image: the block of
+// pixels the reference averages, summed in its order (rows, then columns),
+// divided by its float extent. The block bounds follow the reference's
+// running sums x1 += inc (clamped to x_res - 1), replayed per thread.
+__global__ void __launch_bounds__(kPostThreads) k_hist_downscale(const float *__restrict__ img, int x_res, int y_res,
+                                                                 int sxr, int syr, float sxi, float syi,
+                                                                 float *__restrict__ sl) {
+    const int k = static_cast<int>(blockIdx.x) * kPostThreads + threadIdx.x;
+    if (k >= sxr * syr) return;
+    const int i = k % sxr, j = k / sxr;
+    float x0 = 0.0f, x1 = sxi, y0 = 0.0f, y1 = syi;
+    for (int q = 0; q < i; ++q) {
+        x0 = x1;
+        x1 = fminf((x_res - 1.0f), (x1 + sxi));
+    }
+    for (int q = 0; q < j; ++q) {
+        y0 = y1;
+        y1 = fminf((y_res - 1.0f), (y1 + syi));
+    }
+    // With one sample across (sxr or syr 1) the reference's first bound is
+    // x_res or y_res, one past the image: the rows and columns stop inside.
+    const int ye = min(static_cast<int>(y1), y_res - 1), xe = min(static_cast<int>(x1), x_res - 1);
+    float acc = 0.0f;
+    for (int y = static_cast<int>(y0); y <= ye; ++y)
+        for (int x = static_cast<int>(x0); x <= xe; ++x)
+            acc += img[3 * (static_cast<int64_t>(y) * x_res + x) + 1];
+    acc /= ((x1 - x0) + 1) * ((y1 - y0) + 1);
+    sl[k] = acc;
+}
+
+// camera::just_noticable_difference (rt/camera.cc:1778-1806).
+__device__ float jnd(float La) {
+    const float lLa = static_cast<float>(log10(static_cast<double>(La)));
+    float lLt;
+    if (lLa < -3.94f) {
+        lLt = -2.86f;
+    } else if (lLa < -1.44f) {
+        lLt = powf(0.405f * lLa + 1.6f, 2.18f) - 2.86f;
+    } else if (lLa < -0.0184f) {
+        lLt = lLa - 0.395f;
+    } else if (lLa < 1.9f) {
+        lLt = powf(0.249f * lLa + 0.65f, 2.7f) - 0.72f;
+    } else {
+        lLt = lLa - 1.255f;
+    }
+    return powf(10.0f, lLt);
+}
+
+// One block: range of the samples, the histogram (integer LDS atomics, so
+// exact), then on one thread the reference's ceiling iteration over the 100
+// bins and the distribution the pixels map through.
+__global__ void __launch_bounds__(kPostThreads) k_hist_build(const float *__restrict__ sl, int ns, int human,
+                                                             const ToneScalars *__restrict__ ts,
+                                                             HistState *__restrict__ out) {
+    __shared__ float lds_f[kPostThreads / 64];
+    __shared__ int lds_i[kPostThreads / 64];
+    __shared__ int his[kHistBins + 1];
+    __shared__ float range[2];
+    float lo = kMaxDistF, hi = 0.0f;
+    int cnt = 0;
+    for (int k = threadIdx.x; k < kHistBins + 1; k += kPostThreads) his[k] = 0;
+    for (int k = threadIdx.x; k < ns; k += kPostThreads) {
+        const float v = sl[k];
+        if (v > 0.0f) {
+            lo = fminf(lo, v);
+            hi = fmaxf(hi, v);
+            ++cnt;
+        }
+    }
+    lo = block_reduce<float>(lo, min_f, lds_f);
+    hi = block_reduce<float>(hi, max_f, lds_f);
+    cnt = block_reduce<int>(cnt, add_i, lds_i);
+    const float ld_max = 300.0f, ld_min = 1.0f;
+    const float bd_max = logf(ld_max), bd_min = logf(ld_min);
+    float lw_min = fmaxf(lo, 1.0e-4f);
+    const float bw_min = logf(lw_min);
+    float bw_max = logf(hi);
+    if (bw_max == bw_min) bw_max += 0.0001f * fabsf(bw_max);
+    const bool linear = (bd_max - bd_min) > (bw_max - bw_min);
+    if (threadIdx.x == 0) {
+        range[0] = bw_min;
+        range[1] = bw_max;
+    }
+    __syncthreads();
+    if (!(lo <= 0.0f) && !linear) {
+        for (int k = threadIdx.x; k < ns; k += kPostThreads) {
+            const float v = sl[k];
+            if (v > 0.0f) {
+                int b = static_cast<int>((logf(v) - bw_min) / (bw_max - bw_min) * kHistBins) + 1;
+                b = min(max(b, 0), kHistBins);
+                atomicAdd(&his[b], 1);
+            }
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    HistState h{};
+    h.bw_min = bw_min;
+    h.bw_max = bw_max;
+    h.bd_min = bd_min;
+    h.bd_max = bd_max;
+    if (lo <= 0.0f) {  // black image: the reference leaves Y alone
+        h.branch = 0;
+        *out = h;
+        return;
+    }
+    if (linear) {
+        h.branch = 1;
+        h.sf = (bd_max - bd_min) / (bw_max - bw_min);
+        *out = h;
+        return;
+    }
+    int pix_gt0 = cnt;
+    float cdf[kHistBins + 1];
+    cdf[0] = 0.0f;
+    float sum = 0.0f;
+    for (int i = 1; i <= kHistBins; ++i) {
+        sum += his[i];
+        cdf[i] = sum / static_cast<float>(pix_gt0);
+    }
+    const float tolerance = 0.025f * static_cast<float>(pix_gt0);
+    const float bin_step_size = (bw_max - bw_min) / kHistBins;
+    const float lw_step = 1.0f / static_cast<float>(kHistBins) * (bw_max - bw_min);
+    const float lw_avg = ts->level;
+    int trimmings;
+    do {
+        trimmings = 0;
+        if (pix_gt0 < tolerance) {
+            const float num = 1.219f + powf(ld_max * 0.5f, 0.4f);
+            const float den = 1.219f + powf(lw_avg, 0.4f);
+            h.branch = 2;
+            h.sf = (1.0f / ld_max) * powf((num / den), 2.5f);
+            *out = h;
+            return;
+        }
+        for (int i = 1; i <= kHistBins; ++i) {
+            float ceiling = (pix_gt0 * bin_step_size) / (bd_max - bd_min);
+            if (human) {
+                const float bde = bd_min + (bd_max - bd_min) * cdf[i];
+                const float ld = expf(bde);
+                const float bw = i * lw_step + bw_min;
+                const float lw = expf(bw);
+                ceiling *= (jnd(ld) * lw) / (jnd(lw) * ld);
+            }
+            if (his[i] > ceiling) {
+                trimmings += (his[i] - static_cast<int>(ceiling));
+                his[i] = static_cast<int>(ceiling);
+            }
+        }
+        pix_gt0 = 0;
+        for (int i = 1; i <= kHistBins; ++i) pix_gt0 += his[i];
+        float s2 = 0.0f;
+        for (int i = 1; i <= kHistBins; ++i) {
+            s2 += his[i];
+            cdf[i] = s2 / static_cast<float>(pix_gt0);
+        }
+    } while (trimmings > tolerance);
+    h.branch = 3;
+    for (int i = 0; i <= kHistBins; ++i) h.cdf[i] = cdf[i];
+    *out = h;
+}
+
 int post_blocks(int64_t n) {
     return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(kPostMaxBlocks, (n + kPostThreads - 1) / kPostThreads)));
 }
@@ -689,8 +889,12 @@ int32_t image_tone_map(rt_ctx *ctx, float *rgb, int32_t x_res, int32_t y_res, rt
                        float *ms) {
     hipStream_t s = ctx->stream;
     const int m = tp->mode;
-    if (m == 2 || m == 3)
-        return fail(ctx, RT_ERR_UNSUPPORTED, "rt_image_tone_map: the histogram tone mappers are not built");
+    if ((m == 2 || m == 3) && !(tp->x_angle * (360.0f / (2.0f * 3.1415926535897932384626433f)) >= 1.0f &&
+                                tp->y_angle * (360.0f / (2.0f * 3.1415926535897932384626433f)) >= 1.0f &&
+                                tp->x_angle < 6.3f && tp->y_angle < 6.3f))
+        return fail(ctx, RT_ERR_INVALID,
+                    "rt_image_tone_map: the histogram modes need the camera's field of view (x_angle, y_angle) of "
+                    "at least one degree");
     if (m < 0 || m > 9) return fail(ctx, RT_ERR_INVALID, "rt_image_tone_map: unknown mode " + std::to_string(m));
     const int64_t n = static_cast<int64_t>(x_res) * y_res;
     BuildArena &ar = ctx->arena;
@@ -718,10 +922,27 @@ int32_t image_tone_map(rt_ctx *ctx, float *rgb, int32_t x_res, int32_t y_res, rt
     if (ms) RT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[0], s));
     hipLaunchKernelGGL(k_tone_yxy, dim3(blocks), dim3(kPostThreads), 0, s, a, part);
     hipLaunchKernelGGL(k_tone_scalars, dim3(1), dim3(kPostThreads), 0, s, a, part, blocks, scal);
+    HistState *hist = nullptr;
     if (m == kBilateral) {
         if (int32_t rc = bilateral(ctx, img, x_res, y_res, blocks)) return rc;
+    } else if (m == kHistogram || m == kHumanHistogram) {
+        // One sample per degree of view (rt/camera.cc:1846-1853).
+        const float PI = 3.1415926535897932384626433f;
+        const int sxr = static_cast<int>(tp->x_angle * (360.0f / (2.0f * PI)));
+        const int syr = static_cast<int>(tp->y_angle * (360.0f / (2.0f * PI)));
+        const float sxi = static_cast<float>(x_res) / static_cast<float>(sxr);
+        const float syi = static_cast<float>(y_res) / static_cast<float>(syr);
+        float *sl = nullptr;
+        RT_HIP_CHECK(ctx, ar.take(&v, static_cast<size_t>(sxr) * syr * sizeof(float)));
+        sl = static_cast<float *>(v);
+        RT_HIP_CHECK(ctx, ar.take(&v, sizeof(HistState)));
+        hist = static_cast<HistState *>(v);
+        hipLaunchKernelGGL(k_hist_downscale, dim3((sxr * syr + kPostThreads - 1) / kPostThreads), dim3(kPostThreads),
+                           0, s, img, x_res, y_res, sxr, syr, sxi, syi, sl);
+        hipLaunchKernelGGL(k_hist_build, dim3(1), dim3(kPostThreads), 0, s, sl, sxr * syr, m == kHumanHistogram ? 1 : 0,
+                           scal, hist);
     }
-    hipLaunchKernelGGL(k_tone_apply, dim3(blocks), dim3(kPostThreads), 0, s, a, scal, minmax);
+    hipLaunchKernelGGL(k_tone_apply, dim3(blocks), dim3(kPostThreads), 0, s, a, scal, hist, minmax);
     if (tp->stretch) {
         hipLaunchKernelGGL(k_stretch_scalars, dim3(1), dim3(kPostThreads), 0, s, minmax, blocks, scal);
         hipLaunchKernelGGL(k_tone_stretch, dim3(blocks), dim3(kPostThreads), 0, s, img, n, scal);

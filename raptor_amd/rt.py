@@ -84,7 +84,8 @@ class rt_invrender_params(C.Structure):
 class rt_tone_params(C.Structure):
     _fields_ = [("mode", C.c_int32), ("key", C.c_float), ("xw", C.c_float), ("yw", C.c_float),
                 ("desaturate", C.c_int32), ("stretch", C.c_int32), ("adapt", C.c_int32), ("time_step", C.c_float),
-                ("adaptation_level", C.c_float), ("log_avg_luminance", C.c_float)]
+                ("adaptation_level", C.c_float), ("log_avg_luminance", C.c_float), ("x_angle", C.c_float),
+                ("y_angle", C.c_float)]
 
 
 class rt_accel_info(C.Structure):
@@ -668,10 +669,12 @@ def _image_arg(img):
 
 
 def tone_map(ctx, img, mode=TONE_NONE, key=0.18, xw=1.0 / 3.0, yw=1.0 / 3.0, desaturate=False, stretch=False,
-             adapt=False, time_step=0.0, adaptation_level=0.0):
+             adapt=False, time_step=0.0, adaptation_level=0.0, x_angle=0.0, y_angle=0.0):
     """camera::tone_map (glare off). Returns (image, adaptation level,
     log-average luminance, device ms); a numpy image is returned as a new
-    array, a device tensor is changed in place and returned."""
+    array, a device tensor is changed in place and returned. x_angle,
+    y_angle: the camera's field of view in radians (2 w / t, 2 h / t), for
+    the histogram modes."""
     a, flags = _image_arg(img)
     if flags == 0:
         a = a.copy()
@@ -681,7 +684,7 @@ def tone_map(ctx, img, mode=TONE_NONE, key=0.18, xw=1.0 / 3.0, yw=1.0 / 3.0, des
         torch.cuda.current_stream().synchronize()
     h, w = a.shape[0], a.shape[1]
     tp = rt_tone_params(int(mode), key, xw, yw, int(desaturate), int(stretch), int(adapt), time_step,
-                        adaptation_level, 0.0)
+                        adaptation_level, 0.0, x_angle, y_angle)
     ms = C.c_float()
     ctx.check(_LIB.rt_image_tone_map(ctx.h, _ptr(a), w, h, C.byref(tp), flags, C.byref(ms)), "rt_image_tone_map")
     return a, tp.adaptation_level, tp.log_avg_luminance, ms.value

@@ -121,8 +121,8 @@ def test_frame_to_png_pipeline(gpu_ctx, rt, O, tmp_path):
 def test_post_process_errors(gpu_ctx, rt):
     img = np.zeros((4, 4, 3), np.float32)
     for mode in (2, 3):
-        with pytest.raises(rt.RTError, match="histogram tone mappers"):
-            rt.tone_map(gpu_ctx, img, mode=mode)
+        with pytest.raises(rt.RTError, match="field of view"):
+            rt.tone_map(gpu_ctx, img, mode=mode)  # no x_angle / y_angle
     with pytest.raises(rt.RTError, match="bilateral filter needs"):
         rt.tone_map(gpu_ctx, img, mode=8)  # no bright pixel: no log-intensity range
     with pytest.raises(rt.RTError, match="unknown mode"):
@@ -160,3 +160,34 @@ def test_bilateral_on_hdr_image(gpu_ctx, rt, O):
     o, _, _ = O.tone_map(img, mode=8)
     err = np.abs(g.astype(np.float64) - o) / max(1.0, np.abs(o).max())
     assert err.max() <= 2e-4, err.max()
+
+
+@pytest.mark.parametrize("mode", [2, 3])
+@pytest.mark.parametrize("sc", [False, True])
+def test_histogram_tone_map_matches_oracle(gpu_ctx, rt, O, mode, sc):
+    # local_histogram / local_human_histogram (rt/camera.cc:1843-2062) with
+    # the frame's field of view (camera half width 0.5 at distance 1). The
+    # one-per-degree samples are summed in the reference's order on both
+    # sides; logf / expf may round a sample across a bin edge, so the bound
+    # is relative to the image's range.
+    img = _frame(rt, gpu_ctx, (320, 180))
+    xa, ya = 1.0, 2.0 * 0.5 * 180.0 / 320.0
+    g, glvl, _, _ = rt.tone_map(gpu_ctx, img, mode=mode, stretch=sc, x_angle=xa, y_angle=ya)
+    o, olvl, _ = O.tone_map(img, mode=mode, stretch=sc, x_angle=xa, y_angle=ya)
+    assert np.isfinite(g).all()
+    err = np.abs(g.astype(np.float64) - o) / max(1.0, np.abs(o).max())
+    assert err.max() <= 2e-3 and np.median(err) <= 1e-5, (err.max(), np.median(err))
+
+
+def test_histogram_hdr_and_linear_branches(gpu_ctx, rt, O):
+    rng = np.random.default_rng(8)
+    lum = 10.0 ** rng.uniform(-2.0, 3.0, (150, 260))
+    hdr = np.repeat(lum[..., None], 3, axis=2).astype(np.float32)
+    flat = np.full((90, 160, 3), 100.0, np.float32)
+    deg = 360.0 / (2.0 * np.pi)
+    for img, xa, ya in [(hdr, 1.1, 0.63), (flat, 40.5 / deg, 30.5 / deg)]:
+        for mode in (2, 3):
+            g, _, _, _ = rt.tone_map(gpu_ctx, img, mode=mode, x_angle=xa, y_angle=ya)
+            o, _, _ = O.tone_map(img, mode=mode, x_angle=xa, y_angle=ya)
+            err = np.abs(g.astype(np.float64) - o) / max(1.0, np.abs(o).max())
+            assert err.max() <= 2e-3 and np.median(err) <= 1e-5, (mode, err.max())

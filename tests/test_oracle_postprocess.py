@@ -245,3 +245,52 @@ def test_bilateral_keeps_fine_detail():
     d_in = np.log10(lin[20:70, 21:121] / lin[20:70, 20:120])
     assert np.all(np.sign(d_out) == np.sign(d_in))
     assert np.abs(d_out - d_in).max() < 0.02 and abs(np.abs(d_in).mean() - 0.2) < 0.01
+
+
+def _lum(img):
+    f = np.float64
+    return 0.2126 * img[..., 0].astype(f) + 0.7151 * img[..., 1].astype(f) + 0.0721 * img[..., 2].astype(f)
+
+
+def test_histogram_flat_image_takes_the_linear_branch():
+    # A flat image averaged over whole blocks (40 x 30 samples of 5 x 4
+    # pixels sharing their edges: each block sums count = extent pixels) gives
+    # equal samples, so the log range is bumped by 1e-4 |ln Lw|, below the
+    # display range: Y is scaled by ln 300 / (1e-4 |ln Lw|)
+    # (rt/camera.cc:1846-1937), and rgb (= 255 M^-1 XYZ) by 683 times that.
+    deg = 360.0 / (2.0 * np.pi)
+    img = np.full((90, 160, 3), 100.0, np.float32)
+    out, _, _ = O.tone_map(img, mode=2, x_angle=40.5 / deg, y_angle=30.5 / deg)
+    f = np.float32
+    y = f(683.0) * ((f(0.2126) * f(100.0 / 255.0)) + (f(0.7151) * f(100.0 / 255.0)) + (f(0.0721) * f(100.0 / 255.0)))
+    sf = np.log(300.0) / (1e-4 * abs(np.log(float(y))))
+    ratio = _lum(out) / _lum(img)
+    assert ratio.std() / ratio.mean() < 1e-5
+    assert abs(ratio.mean() / (683.0 * sf) - 1.0) < 2e-3
+
+
+@pytest.mark.parametrize("mode", [2, 3])
+def test_histogram_mapping_is_monotone_into_the_display_range(mode):
+    # The distribution mapping is non-decreasing in luminance and lands in
+    # [1, 300] / 300 (rt/camera.cc:2038-2057).
+    rng = np.random.default_rng(5)
+    h, w = 120, 200
+    y, x = np.mgrid[0:h, 0:w]
+    lum = 10.0 ** (4.0 * x / w - 1.5 + 0.5 * np.sin(y / 11.0)) * (1.0 + 0.1 * rng.uniform(-1, 1, (h, w)))
+    img = np.repeat(lum[..., None], 3, axis=2).astype(np.float32)  # grey: Y maps straight through
+    out, _, _ = O.tone_map(img, mode=mode, x_angle=1.2, y_angle=0.72)
+    yin = (683.0 / 255.0) * lum.ravel()
+    yout = out[..., 1].ravel().astype(np.float64)  # grey in, grey out: g channel follows Y
+    order = np.argsort(yin, kind="stable")
+    assert np.all(np.diff(yout[order]) >= -1e-3 * np.abs(yout).max())
+    ymap = _lum(out).ravel() / 255.0  # rgb = 255 M^-1 XYZ
+    assert ymap.min() >= (1.0 / 300.0) * 0.999 and ymap.max() <= 1.001
+
+
+def test_human_histogram_differs_from_the_plain_one():
+    rng = np.random.default_rng(6)
+    lum = 10.0 ** rng.uniform(-2.0, 3.0, (100, 160))
+    img = np.repeat(lum[..., None], 3, axis=2).astype(np.float32)
+    a, _, _ = O.tone_map(img, mode=2, x_angle=1.0, y_angle=0.625)
+    b, _, _ = O.tone_map(img, mode=3, x_angle=1.0, y_angle=0.625)
+    assert np.abs(a - b).max() > 1.0
