@@ -641,16 +641,18 @@ __global__ void __launch_bounds__(kPostThreads) k_bil_apply(float *__restrict__ 
 
 // ---- local_histogram / local_human_histogram (rt/camera.cc:1843-2062).
 
-// One thread per sample of the one-sample-per-degree image: the block of
-// pixels the reference averages, summed in its order (rows, then columns),
-// divided by its float extent. The block bounds follow the reference's
-// running sums x1 += inc (clamped to x_res - 1), replayed per thread.
-__global__ void __launch_bounds__(kPostThreads) k_hist_downscale(const float *__restrict__ img, int x_res, int y_res,
-                                                                 int sxr, int syr, float sxi, float syi,
-                                                                 float *__restrict__ sl) {
-    const int k = static_cast<int>(blockIdx.x) * kPostThreads + threadIdx.x;
-    if (k >= sxr * syr) return;
-    const int i = k % sxr, j = k / sxr;
+// One wave per sample of the one-sample-per-degree image: the block of
+// pixels the reference averages, divided by its float extent. The block
+// bounds follow the reference's running sums x1 += inc (clamped to
+// x_res - 1), replayed per wave; lane r sums row y0 + r (and every 64th row
+// after it) left to right, and the rows are combined in a fixed tree, so
+// the sample repeats bit for bit (the reference sums row after row into one
+// float).
+__global__ void __launch_bounds__(64) k_hist_downscale(const float *__restrict__ img, int x_res, int y_res,
+                                                       int sxr, int syr, float sxi, float syi,
+                                                       float *__restrict__ sl) {
+    const int k = static_cast<int>(blockIdx.x);
+    const int i = k % sxr, j = k / sxr, lane = threadIdx.x;
     float x0 = 0.0f, x1 = sxi, y0 = 0.0f, y1 = syi;
     for (int q = 0; q < i; ++q) {
         x0 = x1;
@@ -664,11 +666,11 @@ __global__ void __launch_bounds__(kPostThreads) k_hist_downscale(const float *__
     // x_res or y_res, one past the image: the rows and columns stop inside.
     const int ye = min(static_cast<int>(y1), y_res - 1), xe = min(static_cast<int>(x1), x_res - 1);
     float acc = 0.0f;
-    for (int y = static_cast<int>(y0); y <= ye; ++y)
-        for (int x = static_cast<int>(x0); x <= xe; ++x)
-            acc += img[3 * (static_cast<int64_t>(y) * x_res + x) + 1];
-    acc /= ((x1 - x0) + 1) * ((y1 - y0) + 1);
-    sl[k] = acc;
+    for (int y = static_cast<int>(y0) + lane; y <= ye; y += 64)
+        for (int x = static_cast<int>(x0); x <= xe; ++x) acc += img[3 * (static_cast<int64_t>(y) * x_res + x) + 1];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+    if (lane == 0) sl[k] = acc / (((x1 - x0) + 1) * ((y1 - y0) + 1));
 }
 
 // camera::just_noticable_difference (rt/camera.cc:1778-1806).
@@ -690,18 +692,23 @@ __device__ float jnd(float La) {
 }
 
 // One block: range of the samples, the histogram (integer LDS atomics, so
-// exact), then on one thread the reference's ceiling iteration over the 100
-// bins and the distribution the pixels map through.
+// exact), then the reference's ceiling iteration: each bin's ceiling and
+// trim on its own thread, the trimmed total and the cumulative distribution
+// on thread 0 (sums of integer counts, exact in float in any order).
 __global__ void __launch_bounds__(kPostThreads) k_hist_build(const float *__restrict__ sl, int ns, int human,
                                                              const ToneScalars *__restrict__ ts,
                                                              HistState *__restrict__ out) {
     __shared__ float lds_f[kPostThreads / 64];
     __shared__ int lds_i[kPostThreads / 64];
     __shared__ int his[kHistBins + 1];
-    __shared__ float range[2];
+    __shared__ float cdf[kHistBins + 1];
+    __shared__ int trim, total;
     float lo = kMaxDistF, hi = 0.0f;
     int cnt = 0;
-    for (int k = threadIdx.x; k < kHistBins + 1; k += kPostThreads) his[k] = 0;
+    for (int k = threadIdx.x; k < kHistBins + 1; k += kPostThreads) {
+        his[k] = 0;
+        cdf[k] = 0.0f;
+    }
     for (int k = threadIdx.x; k < ns; k += kPostThreads) {
         const float v = sl[k];
         if (v > 0.0f) {
@@ -715,68 +722,69 @@ __global__ void __launch_bounds__(kPostThreads) k_hist_build(const float *__rest
     cnt = block_reduce<int>(cnt, add_i, lds_i);
     const float ld_max = 300.0f, ld_min = 1.0f;
     const float bd_max = logf(ld_max), bd_min = logf(ld_min);
-    float lw_min = fmaxf(lo, 1.0e-4f);
+    const float lw_min = fmaxf(lo, 1.0e-4f);
     const float bw_min = logf(lw_min);
     float bw_max = logf(hi);
     if (bw_max == bw_min) bw_max += 0.0001f * fabsf(bw_max);
-    const bool linear = (bd_max - bd_min) > (bw_max - bw_min);
-    if (threadIdx.x == 0) {
-        range[0] = bw_min;
-        range[1] = bw_max;
-    }
-    __syncthreads();
-    if (!(lo <= 0.0f) && !linear) {
-        for (int k = threadIdx.x; k < ns; k += kPostThreads) {
-            const float v = sl[k];
-            if (v > 0.0f) {
-                int b = static_cast<int>((logf(v) - bw_min) / (bw_max - bw_min) * kHistBins) + 1;
-                b = min(max(b, 0), kHistBins);
-                atomicAdd(&his[b], 1);
-            }
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x != 0) return;
     HistState h{};
     h.bw_min = bw_min;
     h.bw_max = bw_max;
     h.bd_min = bd_min;
     h.bd_max = bd_max;
     if (lo <= 0.0f) {  // black image: the reference leaves Y alone
-        h.branch = 0;
-        *out = h;
+        if (threadIdx.x == 0) *out = h;
         return;
     }
-    if (linear) {
+    if ((bd_max - bd_min) > (bw_max - bw_min)) {
         h.branch = 1;
         h.sf = (bd_max - bd_min) / (bw_max - bw_min);
-        *out = h;
+        if (threadIdx.x == 0) *out = h;
         return;
     }
-    int pix_gt0 = cnt;
-    float cdf[kHistBins + 1];
-    cdf[0] = 0.0f;
-    float sum = 0.0f;
-    for (int i = 1; i <= kHistBins; ++i) {
-        sum += his[i];
-        cdf[i] = sum / static_cast<float>(pix_gt0);
+    for (int k = threadIdx.x; k < ns; k += kPostThreads) {
+        const float v = sl[k];
+        if (v > 0.0f) {
+            int b = static_cast<int>((logf(v) - bw_min) / (bw_max - bw_min) * kHistBins) + 1;
+            b = min(max(b, 0), kHistBins);
+            atomicAdd(&his[b], 1);
+        }
     }
-    const float tolerance = 0.025f * static_cast<float>(pix_gt0);
+    __syncthreads();
+    // Thread 0: the bins' total and the cumulative distribution over div
+    // samples (the first divides by every positive sample, as the reference
+    // does, bin 0 included; later ones by the bins' total).
+    auto distribution = [&](int div) {
+        int t = 0;
+        for (int i = 1; i <= kHistBins; ++i) t += his[i];
+        if (div < 0) div = t;
+        float sum = 0.0f;
+        for (int i = 1; i <= kHistBins; ++i) {
+            sum += his[i];
+            cdf[i] = sum / static_cast<float>(div);
+        }
+        total = t;
+    };
+    if (threadIdx.x == 0) distribution(cnt);
+    __syncthreads();
+    const int pix0 = cnt;  // the first tolerance uses the count of positive samples
+    const float tolerance = 0.025f * static_cast<float>(pix0);
     const float bin_step_size = (bw_max - bw_min) / kHistBins;
     const float lw_step = 1.0f / static_cast<float>(kHistBins) * (bw_max - bw_min);
     const float lw_avg = ts->level;
-    int trimmings;
-    do {
-        trimmings = 0;
+    int pix_gt0 = pix0;
+    for (;;) {
         if (pix_gt0 < tolerance) {
             const float num = 1.219f + powf(ld_max * 0.5f, 0.4f);
             const float den = 1.219f + powf(lw_avg, 0.4f);
             h.branch = 2;
             h.sf = (1.0f / ld_max) * powf((num / den), 2.5f);
-            *out = h;
+            if (threadIdx.x == 0) *out = h;
             return;
         }
-        for (int i = 1; i <= kHistBins; ++i) {
+        if (threadIdx.x == 0) trim = 0;
+        __syncthreads();
+        const int i = threadIdx.x + 1;
+        if (i <= kHistBins) {
             float ceiling = (pix_gt0 * bin_step_size) / (bd_max - bd_min);
             if (human) {
                 const float bde = bd_min + (bd_max - bd_min) * cdf[i];
@@ -786,21 +794,22 @@ __global__ void __launch_bounds__(kPostThreads) k_hist_build(const float *__rest
                 ceiling *= (jnd(ld) * lw) / (jnd(lw) * ld);
             }
             if (his[i] > ceiling) {
-                trimmings += (his[i] - static_cast<int>(ceiling));
+                atomicAdd(&trim, his[i] - static_cast<int>(ceiling));
                 his[i] = static_cast<int>(ceiling);
             }
         }
-        pix_gt0 = 0;
-        for (int i = 1; i <= kHistBins; ++i) pix_gt0 += his[i];
-        float s2 = 0.0f;
-        for (int i = 1; i <= kHistBins; ++i) {
-            s2 += his[i];
-            cdf[i] = s2 / static_cast<float>(pix_gt0);
-        }
-    } while (trimmings > tolerance);
-    h.branch = 3;
-    for (int i = 0; i <= kHistBins; ++i) h.cdf[i] = cdf[i];
-    *out = h;
+        __syncthreads();
+        if (threadIdx.x == 0) distribution(-1);
+        __syncthreads();
+        pix_gt0 = total;
+        if (!(trim > tolerance)) break;
+        __syncthreads();  // trim is reset only after every thread has read it
+    }
+    if (threadIdx.x == 0) {
+        h.branch = 3;
+        for (int i = 0; i <= kHistBins; ++i) h.cdf[i] = cdf[i];
+        *out = h;
+    }
 }
 
 int post_blocks(int64_t n) {
@@ -937,8 +946,8 @@ int32_t image_tone_map(rt_ctx *ctx, float *rgb, int32_t x_res, int32_t y_res, rt
         sl = static_cast<float *>(v);
         RT_HIP_CHECK(ctx, ar.take(&v, sizeof(HistState)));
         hist = static_cast<HistState *>(v);
-        hipLaunchKernelGGL(k_hist_downscale, dim3((sxr * syr + kPostThreads - 1) / kPostThreads), dim3(kPostThreads),
-                           0, s, img, x_res, y_res, sxr, syr, sxi, syi, sl);
+        hipLaunchKernelGGL(k_hist_downscale, dim3(static_cast<unsigned>(sxr * syr)), dim3(64), 0, s, img, x_res,
+                           y_res, sxr, syr, sxi, syi, sl);
         hipLaunchKernelGGL(k_hist_build, dim3(1), dim3(kPostThreads), 0, s, sl, sxr * syr, m == kHumanHistogram ? 1 : 0,
                            scal, hist);
     }
