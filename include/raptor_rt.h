@@ -315,6 +315,14 @@ int32_t rt_image_clip(rt_ctx *ctx, const float *rgb, int32_t x_res, int32_t y_re
  * write_tga_file (rt/camera.cc:2102-2155, bgr bytes). */
 int32_t rt_write_png(const char *path, const uint8_t *rgb, int32_t x_res, int32_t y_res);
 int32_t rt_write_tga(const char *path, const uint8_t *bgr, int32_t x_res, int32_t y_res);
+/* camera::write_jpeg_file (rt/camera.cc:2305-2366): rgb bytes (host, or
+ * device with RT_DEVICE_PTRS), last image row first, as a baseline JFIF with
+ * libjpeg's defaults at quality 1..100 (4:2:0 chroma, the T.81 Annex K
+ * tables scaled by the IJG quality rule, Annex K Huffman codes). Colour
+ * conversion, subsampling, DCT and quantisation run on the device; ms (may
+ * be NULL) receives their time. */
+int32_t rt_write_jpeg(rt_ctx *ctx, const char *path, const uint8_t *rgb, int32_t x_res, int32_t y_res,
+                      int32_t quality, uint32_t flags, float *ms);
 
 /* Closest hit for n rays. Replaces bvh::find_nearest_object(ray*)
  * (ssd/bvh.cc:649-709) and its packet form (ssd/bvh.cc:406-447). */

@@ -179,6 +179,8 @@ int32_t image_gamma(rt_ctx *ctx, float *rgb, int64_t npix, float gamma, uint32_t
 int32_t image_clip(rt_ctx *ctx, const float *rgb, int32_t x_res, int32_t y_res, int32_t xs, int32_t ys, int32_t bgr,
                    uint8_t *out, uint32_t flags, float *ms);
 int32_t write_png(const char *path, const uint8_t *rgb, int32_t x_res, int32_t y_res);
+int32_t write_jpeg(rt_ctx *ctx, const char *path, const uint8_t *rgb, int32_t x_res, int32_t y_res, int32_t quality,
+                   uint32_t flags, float *ms);
 int32_t write_tga(const char *path, const uint8_t *bgr, int32_t x_res, int32_t y_res);
 // Frame of a scene with phong materials (materials.hip).
 int32_t render_frame_materials(rt_accel *a, const rt_camera &cam, const rt_light *lights, int nlights,

@@ -35,7 +35,7 @@ EXPORTS = [
     "rt_invrender_create", "rt_invrender_run", "rt_invrender_destroy", "rt_scene_update", "rt_accel_refit",
     "rt_scene_set_materials", "rt_image_tone_map", "rt_image_gamma", "rt_image_clip", "rt_write_png",
     "rt_write_tga", "rt_scene_set_vertex_normals", "rt_mesh_load", "rt_mesh_parse", "rt_mesh_info_get",
-    "rt_mesh_copy", "rt_mesh_last_error", "rt_mesh_destroy", "rt_scene_create_from_mesh",
+    "rt_mesh_copy", "rt_mesh_last_error", "rt_mesh_destroy", "rt_scene_create_from_mesh", "rt_write_jpeg",
 ]
 
 
@@ -143,6 +143,7 @@ def load_library(path=LIB_PATH):
         "rt_image_clip": (i32, [vp, fp, i32, i32, i32, i32, i32, fp, u32, C.POINTER(C.c_float)]),
         "rt_write_png": (i32, [C.c_char_p, fp, i32, i32]),
         "rt_write_tga": (i32, [C.c_char_p, fp, i32, i32]),
+        "rt_write_jpeg": (i32, [vp, C.c_char_p, fp, i32, i32, i32, u32, C.POINTER(C.c_float)]),
         "rt_render_tape": (i32, [vp, C.POINTER(rt_camera), C.POINTER(rt_light), i32, C.POINTER(rt_render_params), fp,
                                  i64, C.POINTER(vp), C.POINTER(rt_stats)]),
         "rt_tape_info": (i32, [vp, C.POINTER(i32), C.POINTER(i64), C.POINTER(i64)]),
@@ -737,3 +738,19 @@ def write_tga(path, bgr_bytes):
     b = np.ascontiguousarray(bgr_bytes, np.uint8)
     if _LIB.rt_write_tga(str(path).encode(), _ptr(b), b.shape[1], b.shape[0]) != RT_OK:
         raise RTError(f"rt_write_tga: cannot write {path}")
+
+
+def write_jpeg(ctx, path, rgb_bytes, quality=90):
+    """rgb_bytes: (H, W, 3) uint8 in camera::image row order, a numpy array
+    or a device tensor; written as a baseline JPEG (last row first, as
+    camera::write_jpeg_file). Returns the device ms of the transform."""
+    ms = C.c_float()
+    if isinstance(rgb_bytes, np.ndarray):
+        b = np.ascontiguousarray(rgb_bytes, np.uint8)
+        flags = 0
+    else:
+        b = rgb_bytes.contiguous()
+        flags = RT_DEVICE_PTRS
+    ctx.check(_LIB.rt_write_jpeg(ctx.h, str(path).encode(), _ptr(b), b.shape[1], b.shape[0], int(quality), flags,
+                                 C.byref(ms)), "rt_write_jpeg")
+    return ms.value

@@ -9,6 +9,8 @@ The GPU's log-average is checked against a float64 evaluation to 1e-5.
 Gamma agrees within 1e-4 (powf on either side). The clip is bit-exact on the
 same float image.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -191,3 +193,34 @@ def test_histogram_hdr_and_linear_branches(gpu_ctx, rt, O):
             o, _, _ = O.tone_map(img, mode=mode, x_angle=xa, y_angle=ya)
             err = np.abs(g.astype(np.float64) - o) / max(1.0, np.abs(o).max())
             assert err.max() <= 2e-3 and np.median(err) <= 1e-5, (mode, err.max())
+
+
+def _psnr(a, b):
+    mse = np.mean((a.astype(np.float64) - b.astype(np.float64)) ** 2)
+    return 10.0 * np.log10(255.0 ** 2 / max(mse, 1e-12))
+
+
+@pytest.mark.parametrize("quality", [50, 75, 95])
+@pytest.mark.parametrize("res", [(320, 180), (101, 67)])
+def test_jpeg_writer(gpu_ctx, rt, tmp_path, quality, res):
+    # camera::write_jpeg_file (rt/camera.cc:2305-2366): libjpeg defaults at
+    # the given quality. Checked by decoding with an independent decoder
+    # (Pillow's libjpeg) and against Pillow's own encoder with the same
+    # settings: identical quantisation tables (the IJG quality rule), and the
+    # same fidelity and size within a small margin.
+    pil = pytest.importorskip("PIL.Image")
+    img = _frame(rt, gpu_ctx, res)
+    b, _ = rt.clip(gpu_ctx, img)
+    p = tmp_path / "a.jpg"
+    rt.write_jpeg(gpu_ctx, p, b, quality)
+    ours = pil.open(p)
+    assert ours.size == (res[0], res[1]) and ours.mode == "RGB"
+    dec = np.asarray(ours.convert("RGB"))
+    src = b[::-1]  # written last row first
+    ref_path = tmp_path / "ref.jpg"
+    pil.fromarray(np.ascontiguousarray(src)).save(ref_path, quality=quality, subsampling=2)
+    ref = pil.open(ref_path)
+    assert ours.quantization == ref.quantization
+    p_ours, p_ref = _psnr(dec, src), _psnr(np.asarray(ref.convert("RGB")), src)
+    assert p_ours > 15.0 and p_ours > p_ref - 0.3, (p_ours, p_ref)  # the frame is high-frequency noise
+    assert 0.85 < os.path.getsize(p) / os.path.getsize(ref_path) < 1.15
