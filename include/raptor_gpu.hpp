@@ -48,6 +48,43 @@ class context {
     rt_ctx *ctx_ = nullptr;
 };
 
+// A model file as the reference's regression fixture loads it
+// (rt/regression_tests/regression_fixture.h:86-168: off_parser, ply_parser or
+// obj_parser by extension): the parsed triangles, materials and vertex
+// normals as a scene, and its accelerator.
+class mesh_scene {
+  public:
+    mesh_scene(const context &ctx, const std::string &path, const rt_build_params *params = nullptr) {
+        rt_mesh *mesh = nullptr;
+        if (rt_mesh_load(path.c_str(), -1, &mesh) != RT_OK) throw error(rt_mesh_last_error());
+        rt_mesh_info info{};
+        rt_mesh_info_get(mesh, &info);
+        triangles_ = info.triangles;
+        const int32_t rc = rt_scene_create_from_mesh(ctx.get(), mesh, &scene_);
+        rt_mesh_destroy(mesh);
+        ctx.check(rc, "rt_scene_create_from_mesh");
+        if (rt_accel_build(scene_, params, &accel_) != RT_OK) {
+            const std::string msg = std::string("rt_accel_build: ") + rt_last_error(ctx.get());
+            rt_scene_destroy(scene_);
+            throw error(msg);
+        }
+    }
+    ~mesh_scene() {
+        rt_accel_destroy(accel_);
+        rt_scene_destroy(scene_);
+    }
+    mesh_scene(const mesh_scene &) = delete;
+    mesh_scene &operator=(const mesh_scene &) = delete;
+    rt_accel *get() const { return accel_; }
+    rt_scene *scene() const { return scene_; }
+    int64_t triangles() const { return triangles_; }
+
+  private:
+    rt_scene *scene_ = nullptr;
+    rt_accel *accel_ = nullptr;
+    int64_t triangles_ = 0;
+};
+
 // A scene built from any store with size() and primitive(i) returning a
 // triangle with get_vertex_a/b/c() (x, y, z members), get_light() and
 // is_transparent(). AlbedoFn maps a triangle index to three floats in [0, 1].

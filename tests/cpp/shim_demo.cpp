@@ -134,8 +134,36 @@ int main() {
     for (int p = 0; p < 64 * 64; ++p)
         if (c3.image[p].r != ref3[3 * p] || c3.image[p].g != ref3[3 * p + 1] || c3.image[p].b != ref3[3 * p + 2])
             ++mat_bad;
+    // A model file: an OFF pentagon-and-quad in front of the camera, on a
+    // white background.
+    const char *off_path = "/tmp/raptor_shim_demo.off";
+    if (FILE *f = std::fopen(off_path, "w")) {
+        std::fputs("OFF\n9 2 0\n0.3 0.3 0.5\n0.7 0.3 0.5\n0.8 0.6 0.5\n0.5 0.8 0.5\n0.2 0.6 0.5\n"
+                   "0 0 0.9\n1 0 0.9\n1 1 0.9\n0 1 0.9\n5 0 1 2 3 4\n4 5 6 7 8\n", f);
+        std::fclose(f);
+    }
+    raptor_gpu::mesh_scene model(ctx, off_path);
+    rt_render_params bgp{};
+    bgp.spp = 1;
+    bgp.background[0] = bgp.background[1] = bgp.background[2] = 255.0f;
+    std::vector<float> img(3 * 64 * 64);
+    std::vector<rt_hit> hits(64 * 64);
+    if (rt_render(model.get(), &cam, lights.data(), 1, &bgp, img.data(), hits.data(), nullptr, nullptr) != RT_OK) {
+        std::printf("rt_render of the model failed\n");
+        return 1;
+    }
+    int front = 0, back = 0, white = 0;
+    for (int p = 0; p < 64 * 64; ++p) {
+        if (hits[p].prim >= 0 && hits[p].prim < 3) ++front;  // the pentagon: 3 triangles, emplaced first
+        if (hits[p].prim >= 3) ++back;                      // the quad: 2 triangles
+        if (hits[p].prim < 0 && img[3 * p] == 255.0f) ++white;
+    }
+    const bool model_ok = model.triangles() == 5 && front > 100 && back > 100 && white > 0;
+    std::printf("%s: %lld triangles, %d pentagon, %d quad, %d background pixels\n",
+                model_ok ? "model ok" : "model FAILED", static_cast<long long>(model.triangles()), front, back,
+                white);
     const bool ok = bad == 0 && id_bad == 0 && lit > 100 && moved_bad == 0 && changed > 100 && mat_bad == 0 &&
-                    st.rays[2] > 0;
+                    st.rays[2] > 0 && model_ok;
     std::printf("%s: %d pixel mismatches, %d lit pixels, %d ray mismatches, %d refit/rebuild mismatches, "
                 "%d pixels changed by the move, %d material mismatches, %llu mirror rays\n",
                 ok ? "shim ok" : "shim FAILED", bad, lit, id_bad, moved_bad, changed, mat_bad,
