@@ -239,6 +239,11 @@ int32_t rt_scene_set_vertex_normals(rt_scene *scene, const float *vn);
 #define RT_FORMAT_OFF 0
 #define RT_FORMAT_PLY 1
 #define RT_FORMAT_OBJ 2
+/* NFF (rt/parsers/nff_parser.cc:16-224): the viewpoint, background, point
+ * lights, fill materials (phong_shader(255 rgb, kd, ks, shine, T, ri, ks))
+ * and polygons / polygon patches, each triangulated; other primitives
+ * (spheres, cones) are skipped as the reference skips them. */
+#define RT_FORMAT_NFF 3
 typedef struct rt_mesh rt_mesh;
 typedef struct {
     int64_t triangles;
@@ -246,8 +251,10 @@ typedef struct {
     int32_t has_vertex_normals;
     int64_t vertices, faces;
     int64_t skipped_faces;       /* degenerate triangles and faces below three vertices */
-    int64_t skipped_statements;  /* OBJ/MTL statements with nothing to shade */
+    int64_t skipped_statements;  /* OBJ/MTL/NFF statements with nothing to shade */
     int32_t texture_maps;        /* MTL materials whose texture maps are not applied */
+    int32_t lights;              /* point lights the file defines (NFF) */
+    int32_t has_camera;          /* 1 when the file defines the camera (NFF) */
     int32_t reserved;
 } rt_mesh_info;
 /* format: RT_FORMAT_*, or -1 to take it from the file extension. */
@@ -261,6 +268,10 @@ int32_t rt_mesh_info_get(const rt_mesh *mesh, rt_mesh_info *info);
  * has_vertex_normals). NULL skips an array. */
 int32_t rt_mesh_copy(const rt_mesh *mesh, float *abc, int32_t *tri_material, rt_material *materials,
                      float *vertex_normals);
+/* The camera and background colour a file defines (has_camera), and its
+ * point lights (info.lights of them). */
+int32_t rt_mesh_camera(const rt_mesh *mesh, rt_camera *cam, float *background);
+int32_t rt_mesh_lights(const rt_mesh *mesh, rt_light *lights);
 const char *rt_mesh_last_error(void);
 void rt_mesh_destroy(rt_mesh *mesh);
 /* A scene from a parsed mesh: rt_scene_create (albedo = kd / 255, flags bit1

@@ -1,7 +1,7 @@
 """Oracle for scene ingestion — TEST INFRASTRUCTURE ONLY.
 
-A pure-Python restatement of the reference's OFF, PLY and OBJ/MTL parsers
-and of its polygon triangulation, written for the parity tests of the
+A pure-Python restatement of the reference's OFF, PLY, OBJ/MTL and NFF
+parsers and of its polygon triangulation, written for the parity tests of the
 library's parsers (only tests/ import this module). Each function cites the
 reference code it restates. Float arithmetic runs on numpy float32 scalars,
 one rounding per operation as the reference's float code has (no FMA), so
@@ -385,3 +385,68 @@ def parse_obj(text, read=None):
         elif w[0] == "v":
             v.append(_v(float(t[1]), float(t[2]), float(t[3])))
     return o.result()
+
+
+def parse_nff(text):
+    """rt/parsers/nff_parser.cc:16-224: b, v (six lines), l, f, p / pp;
+    anything else skipped. Returns the parse result plus camera (dict of
+    float32: pos, x, y, z, w, h, t, x_res, y_res), background and lights
+    ((k, 7): position, rgb, falloff 0)."""
+    o = _Out()
+    lines = text.splitlines()
+    bg = (F(0), F(0), F(0))
+    cam, cam_bg, lights, cur = None, None, [], None
+    k = 0
+
+    def vals(ln):
+        return [float(x) for x in ln.split()[1:]]
+
+    while k < len(lines):
+        t = lines[k].split()
+        k += 1
+        if not t:
+            continue
+        w = t[0]
+        if w == "b":
+            bg = tuple(F(float(x)) * F(255) for x in t[1:4])
+        elif w[0] == "v":
+            fr = _v(*vals(lines[k])[:3])
+            to = _v(*vals(lines[k + 1])[:3])
+            up = _v(*vals(lines[k + 2])[:3])
+            angle = F(vals(lines[k + 3])[0]) * (F(np.pi) / F(360))
+            ratio = F(np.sin(float(angle)) / np.cos(float(angle)))
+            h = F(vals(lines[k + 4])[0])
+            if h == 0:
+                h = F(1)
+            hw = ratio * h
+            res = vals(lines[k + 5])
+            k += 6
+            at = _sub(to, fr)
+            x = _cross(up, at)
+            up = _cross(at, x)
+            cam = dict(pos=fr, x=_norm(x), y=_norm(up), z=_norm(at), w=hw, h=hw, t=h, x_res=int(res[0]),
+                       y_res=int(res[1]))
+            cam_bg = bg
+        elif w == "l":
+            v = vals(lines[k - 1])
+            rgb = v[3:6] if len(v) >= 6 else [255.0] * 3
+            lights.append([F(x) for x in v[:3]] + [F(x) for x in rgb] + [F(0)])
+        elif w == "f":
+            r, g, b, kd, ks, s, tr, ri = [F(x) for x in vals(lines[k - 1])[:8]]
+            rgb = (r * F(255), g * F(255), b * F(255))
+            cur = _phong((0, 0, 0), tuple(c * kd for c in rgb), tuple(c * ks for c in rgb), s, tr, ri, ks)
+        elif w in ("p", "pp"):
+            n = int(F(float(t[1])))
+            face, normals = [], []
+            for i in range(n):
+                v = [float(x) for x in lines[k + i].split()]
+                face.append(_v(*v[:3]))
+                if w == "pp":
+                    with np.errstate(all="ignore"):
+                        normals.append(_norm(_v(*v[3:6])))
+            k += n
+            o.faces += 1
+            o.polygon(face, cur, normals if w == "pp" else None)
+    res = o.result()
+    res.update(camera=cam, background=cam_bg, lights=np.asarray(lights, np.float32).reshape(-1, 7))
+    return res

@@ -77,6 +77,10 @@ struct Mesh {
     bool any_vn = false;
     int64_t vertices = 0, faces = 0, skipped_faces = 0, skipped_statements = 0;
     int32_t texture_maps = 0;
+    bool has_camera = false;
+    rt_camera cam{};
+    float background[3] = {0.0f, 0.0f, 0.0f};
+    std::vector<rt_light> lights;
 
     void add(const P3 &a, const P3 &b, const P3 &c, int32_t mat, const P3 *n) {
         const int64_t i = static_cast<int64_t>(tri_mat.size());
@@ -779,6 +783,138 @@ void parse_obj(const char *data, int64_t len, const std::string &dir, Mesh *m) {
     m->vertices = static_cast<int64_t>(v.size());
 }
 
+// ---- NFF (rt/parsers/nff_parser.cc:16-224) ----
+// Line-based: b (background, 0..1), v (a six-line viewpoint block: from,
+// at, up, angle, hither, resolution), l (point light, white unless a colour
+// follows), f (fill material), p n / pp n (a polygon of n vertex lines,
+// with normals for pp). Other statements (spheres, cones, comments) are
+// skipped, as the reference skips them.
+void parse_nff(const char *data, int64_t len, Mesh *m) {
+    Cursor c{data, data + len};
+    const char *b, *e;
+    std::string t;
+    int32_t cur = -1;
+    float bg[3] = {0.0f, 0.0f, 0.0f};
+    auto line_tokens = [&](const char *what) {
+        if (!next_line(&c, &b, &e)) bad(std::string("NFF file ends inside ") + what);
+        return Tokens{b, e};
+    };
+    auto vec3 = [&](Tokens *tk, const char *what) {
+        P3 q;
+        q.x = next_float(tk, what);
+        q.y = next_float(tk, what);
+        q.z = next_float(tk, what);
+        return q;
+    };
+    std::vector<P3> face, normals;
+    while (next_line(&c, &b, &e)) {
+        Tokens tk{b, e};
+        if (!tk.next(&t)) continue;
+        if (t == "b") {
+            const P3 q = vec3(&tk, "the background");
+            bg[0] = q.x * 255.0f;
+            bg[1] = q.y * 255.0f;
+            bg[2] = q.z * 255.0f;
+        } else if (t[0] == 'v') {
+            Tokens f = line_tokens("the viewpoint");
+            f.next(&t);
+            const P3 from = vec3(&f, "from");
+            Tokens a = line_tokens("the viewpoint");
+            a.next(&t);
+            const P3 to = vec3(&a, "at");
+            P3 at_vec = sub(to, from);
+            Tokens u = line_tokens("the viewpoint");
+            u.next(&t);
+            P3 up = vec3(&u, "up");
+            Tokens an = line_tokens("the viewpoint");
+            an.next(&t);
+            const float angle = next_float(&an, "angle") * (3.1415926535897932384626433f / 360.0f);
+            // Unqualified sin / cos of a float in the reference: the double ones.
+            const float dist_ratio = static_cast<float>(std::sin(static_cast<double>(angle)) /
+                                                        std::cos(static_cast<double>(angle)));
+            Tokens hi = line_tokens("the viewpoint");
+            hi.next(&t);
+            float h = next_float(&hi, "hither");
+            if (h == 0.0f) h = 1.0f;
+            const float hw = dist_ratio * h;
+            Tokens r = line_tokens("the viewpoint");
+            r.next(&t);
+            const float xr = next_float(&r, "resolution"), yr = next_float(&r, "resolution");
+            P3 x_vec = cross(up, at_vec);
+            up = cross(at_vec, x_vec);
+            x_vec = normalised(x_vec);
+            up = normalised(up);
+            at_vec = normalised(at_vec);
+            rt_camera cam{};
+            const P3 *axes[4] = {&from, &x_vec, &up, &at_vec};
+            float *dst[4] = {cam.pos, cam.x_axis, cam.y_axis, cam.z_axis};
+            for (int k = 0; k < 4; ++k) {
+                dst[k][0] = axes[k]->x;
+                dst[k][1] = axes[k]->y;
+                dst[k][2] = axes[k]->z;
+            }
+            cam.w = hw;
+            cam.h = hw;
+            cam.t = h;
+            cam.x_res = static_cast<int32_t>(static_cast<unsigned int>(xr));
+            cam.y_res = static_cast<int32_t>(static_cast<unsigned int>(yr));
+            m->cam = cam;
+            m->has_camera = true;
+            for (int k = 0; k < 3; ++k) m->background[k] = bg[k];  // the background so far
+        } else if (t == "l") {
+            rt_light l{};
+            const P3 q = vec3(&tk, "a light");
+            l.pos[0] = q.x;
+            l.pos[1] = q.y;
+            l.pos[2] = q.z;
+            l.rgb[0] = l.rgb[1] = l.rgb[2] = 255.0f;
+            std::string v;
+            if (tk.next(&v)) {
+                l.rgb[0] = to_float(v, "a light colour");
+                l.rgb[1] = next_float(&tk, "a light colour");
+                l.rgb[2] = next_float(&tk, "a light colour");
+            }
+            l.d = 0.0f;
+            m->lights.push_back(l);
+        } else if (t == "f") {
+            const P3 rgb0 = vec3(&tk, "a fill colour");
+            const float kd = next_float(&tk, "kd"), ks = next_float(&tk, "ks"), s = next_float(&tk, "shine");
+            const float tr = next_float(&tk, "transmittance"), ri = next_float(&tk, "refractive index");
+            const float rgb[3] = {rgb0.x * 255.0f, rgb0.y * 255.0f, rgb0.z * 255.0f};
+            // phong_shader(rgb, kd, ks, s, tran, ri, rf = ks) (rt/materials/phong_shader.h:17-19)
+            rt_material q = phong(0.0f, 0.0f, 0.0f);
+            for (int k = 0; k < 3; ++k) {
+                q.kd[k] = rgb[k] * kd;
+                q.ks[k] = rgb[k] * ks;
+            }
+            q.shininess = s;
+            q.transmittance = tr;
+            q.refractive_index = ri;
+            q.reflectance = ks;
+            m->mats.push_back(q);
+            cur = static_cast<int32_t>(m->mats.size() - 1);
+        } else if (t == "p" || t == "pp") {
+            const bool patch = t == "pp";
+            std::string v;
+            if (!tk.next(&v)) bad("NFF polygon without a vertex count");
+            const long n = static_cast<long>(to_float(v, "a vertex count"));
+            if (cur < 0) bad("NFF polygon before any fill material");
+            face.clear();
+            normals.clear();
+            for (long i = 0; i < n; ++i) {
+                Tokens pv = line_tokens("a polygon");
+                face.push_back(vec3(&pv, "a polygon vertex"));
+                if (patch) normals.push_back(normalised(vec3(&pv, "a patch normal")));
+            }
+            ++m->faces;
+            m->vertices += n;
+            add_polygon(m, face, patch ? &normals : nullptr, cur);
+        } else {
+            ++m->skipped_statements;
+        }
+    }
+}
+
 int32_t format_of(const std::string &path) {
     const size_t dot = path.find_last_of('.');
     if (dot == std::string::npos) return -1;
@@ -787,6 +923,7 @@ int32_t format_of(const std::string &path) {
     if (ext == "off") return RT_FORMAT_OFF;
     if (ext == "ply") return RT_FORMAT_PLY;
     if (ext == "obj") return RT_FORMAT_OBJ;
+    if (ext == "nff") return RT_FORMAT_NFF;
     return -1;
 }
 
@@ -820,6 +957,9 @@ int32_t rt_mesh_parse(const char *data, int64_t len, int32_t format, const char 
                 break;
             case RT_FORMAT_PLY:
                 parse_ply(data, len, &mesh->m);
+                break;
+            case RT_FORMAT_NFF:
+                parse_nff(data, len, &mesh->m);
                 break;
             case RT_FORMAT_OBJ: {
                 std::string d = dir ? dir : "";
@@ -881,6 +1021,28 @@ int32_t rt_mesh_info_get(const rt_mesh *mesh, rt_mesh_info *info) {
     info->skipped_faces = m.skipped_faces;
     info->skipped_statements = m.skipped_statements;
     info->texture_maps = m.texture_maps;
+    info->lights = static_cast<int32_t>(m.lights.size());
+    info->has_camera = m.has_camera ? 1 : 0;
+    info->reserved = 0;
+    return RT_OK;
+}
+
+int32_t rt_mesh_camera(const rt_mesh *mesh, rt_camera *cam, float *background) {
+    if (!mesh) return RT_ERR_INVALID;
+    if (!mesh->m.has_camera) {
+        g_error = "rt_mesh_camera: the file defines no camera";
+        return RT_ERR_UNSUPPORTED;
+    }
+    if (cam) *cam = mesh->m.cam;
+    if (background)
+        for (int k = 0; k < 3; ++k) background[k] = mesh->m.background[k];
+    return RT_OK;
+}
+
+int32_t rt_mesh_lights(const rt_mesh *mesh, rt_light *lights) {
+    if (!mesh) return RT_ERR_INVALID;
+    if (lights && !mesh->m.lights.empty())
+        std::memcpy(lights, mesh->m.lights.data(), mesh->m.lights.size() * sizeof(rt_light));
     return RT_OK;
 }
 

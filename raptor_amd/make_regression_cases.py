@@ -1,12 +1,14 @@
 """Writes regression_cases.json: the reference's image-regression cases for the
-formats this library parses (OFF, PLY, OBJ), as data for raptor_amd.regression.
+formats this library parses (OFF, PLY, OBJ, NFF), as data for
+raptor_amd.regression.
 
 Each case is read from the reference's test sources as text
-(rt/regression_tests/{off,ply,obj}_tests.cc): the BOOST_AUTO_TEST_CASE name
+(rt/regression_tests/{off,ply,obj,nff}_tests.cc): the BOOST_AUTO_TEST_CASE name
 (which names the expected image test_data/<suite>/<name>_exp.png), the
 regression_fixture arguments (model path under $RAPTOR_DATA, format, camera
 position and axes, background colour, pan/tilt/roll, resolution) and the
-add_light calls (rgb, centre, distance falloff, radius). Run it where the
+add_light calls (rgb, centre, distance falloff, radius); an NFF case takes
+its camera, background and lights from the file. Run it where the
 reference checkout is:
 
     python -m raptor_amd.make_regression_cases /path/to/RAPTor
@@ -18,7 +20,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 OUT = os.path.join(HERE, "regression_cases.json")
-SUITES = ("off", "ply", "obj")
+SUITES = ("off", "ply", "obj", "nff")
 
 
 def split_args(s):
@@ -71,7 +73,8 @@ def parse_suite(path, suite):
         if not f:
             continue
         a = split_args(call_args(body, f.end() - 1))
-        case = dict(suite=suite, name=name, input=a[1].strip('"'), format=a[2].split("::")[-1],
+        fmt = a[2].split("::")[-1]
+        case = dict(suite=suite, name=name, input=a[1].strip('"'), format=fmt, camera_from_file=(fmt == "nff"),
                     camera=dict(pos=[0.0, 0.0, -10.0], x=[1.0, 0.0, 0.0], y=[0.0, 1.0, 0.0], z=[0.0, 0.0, 1.0]),
                     background=[0.0, 0.0, 0.0], rotation=[0.0, 0.0, 0.0], resolution=[640, 480], lights=[],
                     unsupported=[])
@@ -106,7 +109,7 @@ def main(ref_root):
     for suite in SUITES:
         cases += parse_suite(os.path.join(rt_dir, f"{suite}_tests.cc"), suite)
     with open(OUT, "w") as fh:
-        json.dump(dict(source="rt/regression_tests/{off,ply,obj}_tests.cc", cases=cases), fh, indent=1)
+        json.dump(dict(source="rt/regression_tests/{off,ply,obj,nff}_tests.cc", cases=cases), fh, indent=1)
         fh.write("\n")
     counts = {s: sum(c["suite"] == s for c in cases) for s in SUITES}
     print(f"{OUT}: {len(cases)} cases {counts}")

@@ -1,7 +1,7 @@
 """The reference's image-regression harness with this library as the
 accelerator under test.
 
-The reference renders each case of rt/regression_tests/{off,ply,obj}_tests.cc
+The reference renders each case of rt/regression_tests/{off,ply,obj,nff}_tests.cc
 with its bvh, kd-tree and BIH and compares every image with an expected PNG
 (rt/regression_tests/regression_checker.h:44-170). This module runs the same
 cases through the GPU path: parse the model under $RAPTOR_DATA
@@ -146,8 +146,14 @@ def render_case(ctx, case, model_path, width=4, mesh=None):
     mesh = mesh or rt.Mesh(model_path)
     scene = rt.Scene.from_mesh(ctx, mesh)
     accel = rt.Accel(scene, width=width)
-    cam = case_camera(case)
-    out = accel.render(cam, case_lights(case), want_hits=False, background=tuple(case["background"]))
+    if case.get("camera_from_file"):  # NFF: the viewpoint, background and lights of the file
+        from .workloads import mesh_camera_lights
+
+        cam, background, lights = mesh_camera_lights(mesh)
+        lights = lights + case_lights(case)
+    else:
+        cam, background, lights = case_camera(case), tuple(case["background"]), case_lights(case)
+    out = accel.render(cam, lights, want_hits=False, background=background)
     img, _ = rt.clip(ctx, out["rgb"])
     return out["rgb"], img, out["stats"], accel.info()
 
@@ -179,7 +185,7 @@ def run_case(ctx, case, data_dir, expected_dir=None, out_dir=None, width=4):
     if not os.path.exists(exp_path):
         return dict(res, status="missing-expected", path=exp_path)
     exp = expected_image(exp_path)
-    xr, yr = case["resolution"]
+    xr, yr = img.shape[1], img.shape[0]
     if exp.shape != (yr, xr, 3):
         return dict(res, status="fail", why=f"expected image is {exp.shape[1]}x{exp.shape[0]}")
     fails, mism = checker(img, exp, xr, yr)
@@ -191,7 +197,7 @@ def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--data", default=os.environ.get("RAPTOR_DATA"), help="the models ($RAPTOR_DATA)")
     ap.add_argument("--expected", help="the reference's regression_tests/test_data directory")
-    ap.add_argument("--suite", default="all", choices=["all", "off", "ply", "obj"])
+    ap.add_argument("--suite", default="all", choices=["all", "off", "ply", "obj", "nff"])
     ap.add_argument("--out", help="write the actual images here")
     ap.add_argument("--width", type=int, default=4)
     ap.add_argument("--limit", type=int, default=0)

@@ -36,6 +36,7 @@ EXPORTS = [
     "rt_scene_set_materials", "rt_image_tone_map", "rt_image_gamma", "rt_image_clip", "rt_write_png",
     "rt_write_tga", "rt_scene_set_vertex_normals", "rt_mesh_load", "rt_mesh_parse", "rt_mesh_info_get",
     "rt_mesh_copy", "rt_mesh_last_error", "rt_mesh_destroy", "rt_scene_create_from_mesh", "rt_write_jpeg",
+    "rt_mesh_camera", "rt_mesh_lights",
 ]
 
 
@@ -97,7 +98,8 @@ class rt_accel_info(C.Structure):
 class rt_mesh_info(C.Structure):
     _fields_ = [("triangles", C.c_int64), ("materials", C.c_int32), ("has_vertex_normals", C.c_int32),
                 ("vertices", C.c_int64), ("faces", C.c_int64), ("skipped_faces", C.c_int64),
-                ("skipped_statements", C.c_int64), ("texture_maps", C.c_int32), ("reserved", C.c_int32)]
+                ("skipped_statements", C.c_int64), ("texture_maps", C.c_int32), ("lights", C.c_int32),
+                ("has_camera", C.c_int32), ("reserved", C.c_int32)]
 
 
 def load_library(path=LIB_PATH):
@@ -160,6 +162,8 @@ def load_library(path=LIB_PATH):
         "rt_mesh_info_get": (i32, [vp, C.POINTER(rt_mesh_info)]),
         "rt_mesh_copy": (i32, [vp, fp, fp, fp, fp]),
         "rt_mesh_last_error": (C.c_char_p, []),
+        "rt_mesh_camera": (i32, [vp, C.POINTER(rt_camera), fp]),
+        "rt_mesh_lights": (i32, [vp, fp]),
         "rt_mesh_destroy": (None, [vp]),
         "rt_scene_create_from_mesh": (i32, [vp, vp, C.POINTER(vp)]),
         "rt_gen_random_triangles": (i64, [u64, i64, C.c_float, C.c_float, C.c_float, fp, fp]),
@@ -382,13 +386,16 @@ class Scene:
             pass
 
 
-FORMAT_OFF, FORMAT_PLY, FORMAT_OBJ = 0, 1, 2
+FORMAT_OFF, FORMAT_PLY, FORMAT_OBJ, FORMAT_NFF = 0, 1, 2, 3
 
 
 class Mesh:
-    """A parsed OFF, PLY or OBJ file (host memory; no GPU needed). Fields:
-    abc (n, 9) float32 in emplace order, tri_material (n,) int32, materials
-    (MATERIAL_DTYPE), vertex_normals (n, 9) float32 or None, info (dict)."""
+    """A parsed OFF, PLY, OBJ or NFF file (host memory; no GPU needed).
+    Fields: abc (n, 9) float32 in emplace order, tri_material (n,) int32,
+    materials (MATERIAL_DTYPE), vertex_normals (n, 9) float32 or None, info
+    (dict); for a file that defines them (NFF): camera (an rt_camera),
+    background (3 floats, 0-255) and lights ((k, 7) float32 rows: position,
+    rgb, falloff), else None."""
 
     def __init__(self, path=None, data=None, fmt=-1, directory=None):
         h = C.c_void_p()
@@ -411,6 +418,14 @@ class Mesh:
         self.vertex_normals = np.empty((self.n, 9), np.float32) if inf.has_vertex_normals else None
         _LIB.rt_mesh_copy(h, _ptr(self.abc), _ptr(self.tri_material), _ptr(self.materials),
                           _ptr(self.vertex_normals))
+        self.camera = self.background = None
+        if inf.has_camera:
+            self.camera = rt_camera()
+            self.background = np.zeros(3, np.float32)
+            _LIB.rt_mesh_camera(h, C.byref(self.camera), _ptr(self.background))
+        self.lights = np.zeros((inf.lights, 7), np.float32) if inf.lights else None
+        if inf.lights:
+            _LIB.rt_mesh_lights(h, _ptr(self.lights))
 
     def close(self):
         if self.h:

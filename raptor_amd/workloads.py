@@ -190,6 +190,15 @@ def material_scene(n_objects=400, res=96, seed=5, mirror=None, glass=None, mirro
     return w, materials, np.asarray(kinds, np.int32), np.asarray(flags, np.uint8)
 
 
+def mesh_camera_lights(mesh):
+    """The Camera, background and Lights a parsed file defines (NFF)."""
+    c = mesh.camera
+    cam = Camera(tuple(c.pos), tuple(c.x_axis), tuple(c.y_axis), tuple(c.z_axis), c.w, c.h, c.t, c.x_res, c.y_res)
+    lights = [Light(tuple(float(x) for x in r[0:3]), tuple(float(x) for x in r[3:6]), float(r[6]))
+              for r in (mesh.lights if mesh.lights is not None else [])]
+    return cam, tuple(float(x) for x in mesh.background), lights
+
+
 def mesh_view(abc, x_res=320, y_res=240, direction=(0.45, 0.35, -0.82), light_rgb=(255.0, 255.0, 255.0)):
     """A fixture camera that frames a parsed mesh from the given direction
     (the view direction, camera to mesh), and a point light on the camera's

@@ -226,3 +226,27 @@ def test_background_colour(gpu_ctx, rt, O, with_materials):
         assert (out["rgb"].reshape(-1, 3)[hit] > black["rgb"].reshape(-1, 3)[hit]).any()
     else:
         assert np.array_equal(out["rgb"].reshape(-1, 3)[hit], black["rgb"].reshape(-1, 3)[hit])
+
+
+def test_nff_scene_with_its_camera_and_lights(gpu_ctx, rt, O):
+    # The viewpoint, background, lights and fill materials of an NFF file
+    # (rt/parsers/nff_parser.cc:16-224), rendered as the file defines them.
+    import sys
+
+    from raptor_amd import workloads
+
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from test_ingest import NFF
+
+    mesh = rt.Mesh(data=NFF.encode(), fmt=rt.FORMAT_NFF)
+    cam, bg, lights = workloads.mesh_camera_lights(mesh)
+    scene = rt.Scene.from_mesh(gpu_ctx, mesh)
+    accel = rt.Accel(scene)
+    out = accel.render(cam, lights, background=bg)
+    albedo = (mesh.materials["kd"][mesh.tri_material] / np.float32(255.0)).astype(np.float32)
+    ref = O.OracleScene(mesh.abc, _flags(mesh.materials, mesh.tri_material)).render(
+        albedo, cam, lights, nthreads=8, materials=oracle_material_rows(mesh.materials),
+        tri_material=mesh.tri_material, vertex_normals=mesh.vertex_normals, background=bg)
+    _check(out, ref, nl=2, min_hits=1000)
+    miss = ref["prim"] < 0
+    assert miss.any() and np.all(out["rgb"].reshape(-1, 3)[miss] == np.asarray(bg, np.float32))

@@ -432,3 +432,64 @@ def test_format_from_extension_and_missing_file(rt, tmp_path):
         rt.Mesh(str(tmp_path / "absent.off"))
     with pytest.raises(rt.RTError, match="no format"):
         rt.Mesh(str(tmp_path / "scene.xyz"))
+
+
+NFF = """b 0.1 0.2 0.3
+v
+from 1.5 2.0 -6.0
+at 0.2 0.4 0.5
+up 0.0 1.0 0.0
+angle 40
+hither 1
+resolution 160 120
+l 4 6 -3
+l -3 5 -2 0.5 0.6 0.7
+f 0.8 0.6 0.4 0.7 0.3 12 0 1
+p 4
+-2 -1 -2
+2 -1 -2
+2 -1 2
+-2 -1 2
+s 0 0 0 1
+f 0.2 0.5 0.9 0.8 0.1 30 0.4 1.4
+pp 3
+-1 0 0 0 0.2 -1
+1 0 0 0.1 0 -1
+0 1.5 0.2 -0.1 0.1 -1
+p 5
+-1.5 0.5 1
+-0.5 0.5 1
+-0.3 1.5 1
+-1 2 1
+-1.7 1.4 1
+"""
+
+
+def test_nff_matches_oracle(rt):
+    m = rt.Mesh(data=NFF.encode(), fmt=rt.FORMAT_NFF)
+    ref = OI.parse_nff(NFF)
+    assert_matches_oracle(m, ref)
+    assert m.n == 2 + 1 + 3 and m.info["skipped_statements"] == 1  # the sphere
+    c = m.camera
+    got = np.concatenate([c.pos, c.x_axis, c.y_axis, c.z_axis, [c.w, c.h, c.t]]).astype(np.float32)
+    want = np.concatenate([ref["camera"][k] for k in ("pos", "x", "y", "z")] +
+                          [[ref["camera"]["w"], ref["camera"]["h"], ref["camera"]["t"]]]).astype(np.float32)
+    assert same_bits(got, want) and (c.x_res, c.y_res) == (160, 120)
+    assert same_bits(m.background, ref["background"]) and same_bits(m.lights, ref["lights"])
+
+
+def test_nff_camera_closed_form(rt):
+    # rt/parsers/nff_parser.cc:58-110: an orthonormal basis with x = up x at,
+    # half width tan(angle / 2) * hither, image plane at hither.
+    m = rt.Mesh(data=NFF.encode(), fmt=rt.FORMAT_NFF)
+    c = m.camera
+    x, y, z = (np.asarray(v, np.float64) for v in (c.x_axis, c.y_axis, c.z_axis))
+    at = np.array([0.2 - 1.5, 0.4 - 2.0, 0.5 + 6.0])
+    assert np.allclose(z, at / np.linalg.norm(at), atol=1e-6)
+    assert np.allclose([x @ y, y @ z, x @ z], 0.0, atol=1e-6) and np.allclose(np.cross(y, z), x, atol=1e-6)
+    assert y[1] > 0 and abs(c.w - np.tan(np.radians(20.0))) < 1e-6 and c.t == 1.0
+    assert np.allclose(m.background, [25.5, 51.0, 76.5]) and np.allclose(m.lights[1, 3:6], [0.5, 0.6, 0.7])
+    # fill material: kd = rgb kd, ks = rgb ks, reflectance ks (rt/materials/phong_shader.h:17-19)
+    mat = m.materials[m.tri_material[0]]
+    assert np.allclose(mat["kd"], np.array([0.8, 0.6, 0.4]) * 255 * 0.7, rtol=1e-6)
+    assert np.allclose(mat["ks"], np.array([0.8, 0.6, 0.4]) * 255 * 0.3, rtol=1e-6) and mat["reflectance"] == np.float32(0.3)

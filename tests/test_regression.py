@@ -90,12 +90,16 @@ def test_png_reader_decodes_filtered_rows(tmp_path):
 
 def test_case_table():
     cases = R.load_cases()
-    assert len(cases) == 150
-    assert {s: len(R.load_cases(s)) for s in ("off", "ply", "obj")} == {"off": 68, "ply": 23, "obj": 59}
-    assert len({c["name"] for c in cases}) == 150
+    assert len(cases) == 194
+    assert {s: len(R.load_cases(s)) for s in ("off", "ply", "obj", "nff")} == {"off": 68, "ply": 23, "obj": 59,
+                                                                               "nff": 44}
+    assert len({c["name"] for c in cases}) == 194
     for c in cases:
-        assert c["lights"] and c["resolution"][0] > 0 and c["input"].startswith("/" + c["suite"] + "_scenes/")
         assert c["format"] == c["suite"] and not c["unsupported"]
+        if c["suite"] == "nff":  # viewpoint, lights and background come from the file
+            assert c["camera_from_file"] and not c["lights"] and c["input"].endswith(".nff")
+            continue
+        assert c["lights"] and c["resolution"][0] > 0 and c["input"].startswith("/" + c["suite"] + "_scenes/")
     torus = [c for c in cases if c["name"] == "off_tst_torus_model_test"][0]
     assert torus["camera"]["pos"] == [75.0, -100.0, -11.0] and torus["resolution"] == [1920, 1080]
     assert torus["lights"] == [dict(rgb=[255.0] * 3, pos=[50.0, -200.0, -70.0], d=0.0, r=10.0)]
