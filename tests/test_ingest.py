@@ -493,3 +493,41 @@ def test_nff_camera_closed_form(rt):
     mat = m.materials[m.tri_material[0]]
     assert np.allclose(mat["kd"], np.array([0.8, 0.6, 0.4]) * 255 * 0.7, rtol=1e-6)
     assert np.allclose(mat["ks"], np.array([0.8, 0.6, 0.4]) * 255 * 0.3, rtol=1e-6) and mat["reflectance"] == np.float32(0.3)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_random_polygon_soup_matches_oracle(rt, seed):
+    # Random faces of 3 to 14 corners on a shared vertex table: star-shaped
+    # and arbitrary (self-crossing) rings, repeated corners and collinear
+    # runs, in random planes. The library and the restatement must agree bit
+    # for bit on every triangle, whatever the ear clipping makes of them.
+    rng = np.random.default_rng(100 + seed)
+    verts, faces = [], []
+    for f in range(60):
+        n = int(rng.integers(3, 15))
+        kind = f % 4
+        if kind == 0:  # star-shaped
+            ang = np.sort(rng.uniform(0, 2 * np.pi, n))
+            pts = [(r * np.cos(a), r * np.sin(a)) for a, r in zip(ang, rng.uniform(0.2, 1.0, n))]
+        elif kind == 1:  # arbitrary ring
+            pts = [tuple(p) for p in rng.uniform(-1, 1, (n, 2))]
+        elif kind == 2:  # a repeated corner
+            ang = np.sort(rng.uniform(0, 2 * np.pi, n))
+            pts = [(np.cos(a), np.sin(a)) for a in ang]
+            pts[n // 2] = pts[n // 2 - 1]
+        else:  # collinear runs along the sides of a square
+            m = max(1, n // 4)
+            side = [(t, 0.0) for t in np.linspace(0, 1, m, endpoint=False)]
+            pts = side + [(1.0, t) for t, _ in side] + [(1.0 - t, 1.0) for t, _ in side] + \
+                [(0.0, 1.0 - t) for t, _ in side]
+        base = len(verts)
+        verts += place(pts, rng)
+        faces.append(list(range(base, base + len(pts))))
+    text = off_text(verts, faces)
+    m = rt.Mesh(data=text.encode(), fmt=rt.FORMAT_OFF)
+    assert_matches_oracle(m, OI.parse_off(text))
+    ply = ply_bytes(verts, faces, "binary_little_endian")
+    assert_matches_oracle(rt.Mesh(data=ply, fmt=rt.FORMAT_PLY), OI.parse_ply(ply))
+    obj = "".join(f"v {x!r} {y!r} {z!r}\n" for x, y, z in (map(float, v) for v in verts))
+    obj += "".join("f " + " ".join(str(i + 1) for i in f) + "\n" for f in faces)
+    assert_matches_oracle(rt.Mesh(data=obj.encode(), fmt=rt.FORMAT_OBJ), OI.parse_obj(obj))
