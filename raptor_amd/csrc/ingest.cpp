@@ -19,6 +19,7 @@
 // (the reference loops forever or reads out of range on them). Tokens may be
 // separated by any blanks.
 #include <cerrno>
+#include <charconv>
 #include <cfloat>
 #include <cmath>
 #include <cstdint>
@@ -227,12 +228,18 @@ bool next_line(Cursor *c, const char **b, const char **e) {
 // Whitespace-separated tokens of one line.
 struct Tokens {
     const char *p, *e;
-    bool next(std::string *t) {
+    bool next_view(const char **b, const char **end) {
         while (p < e && blank(*p)) ++p;
         if (p >= e) return false;
-        const char *s = p;
+        *b = p;
         while (p < e && !blank(*p)) ++p;
-        t->assign(s, static_cast<size_t>(p - s));
+        *end = p;
+        return true;
+    }
+    bool next(std::string *t) {
+        const char *b, *end;
+        if (!next_view(&b, &end)) return false;
+        t->assign(b, static_cast<size_t>(end - b));
         return true;
     }
 };
@@ -253,10 +260,27 @@ long to_int(const std::string &t, const char *what) {
     return v;
 }
 
+// The common case without a copy: std::from_chars reads the same correctly
+// rounded double as strtod; signs, hex and out-of-range values go the strtod
+// way.
 float next_float(Tokens *tk, const char *what) {
-    std::string t;
-    if (!tk->next(&t)) bad(std::string("missing ") + what);
-    return to_float(t, what);
+    const char *b, *e;
+    if (!tk->next_view(&b, &e)) bad(std::string("missing ") + what);
+    double v;
+    if (*b != '+') {
+        const auto r = std::from_chars(b, e, v);
+        if (r.ec == std::errc() && (r.ptr == e || (*r.ptr != 'x' && *r.ptr != 'X'))) return static_cast<float>(v);
+    }
+    return to_float(std::string(b, e), what);
+}
+
+long next_int(Tokens *tk, const char *what) {
+    const char *b, *e;
+    if (!tk->next_view(&b, &e)) bad(std::string("missing ") + what);
+    long v;
+    const auto r = std::from_chars(b, e, v);
+    if (r.ec == std::errc() && r.ptr != b) return v;
+    return to_int(std::string(b, e), what);
 }
 
 // ---- OFF (rt/parsers/off_parser.cc:14-99) ----
@@ -299,8 +323,8 @@ void parse_off(const char *data, int64_t len, Mesh *m) {
         const long k = to_int(t, "a face size");
         face.clear();
         for (long j = 0; j < k; ++j) {
-            if (!tk.next(&t)) bad("OFF face line shorter than its vertex count");
-            const long idx = to_int(t, "a vertex index");
+            if (tk.p >= tk.e) bad("OFF face line shorter than its vertex count");
+            const long idx = next_int(&tk, "a vertex index");
             if (idx < 0 || idx >= nv) bad("OFF vertex index " + std::to_string(idx) + " out of range");
             face.push_back(v[static_cast<size_t>(idx)]);
         }
