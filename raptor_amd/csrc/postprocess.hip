@@ -858,6 +858,8 @@ int32_t bilateral(rt_ctx *ctx, float *img, int x_res, int y_res, int blocks) {
     g.H = static_cast<int>(static_cast<float>(y_res - 1) / g.s_sa) + 1 + (g.pxy << 1);
     g.D = static_cast<int>(delta / g.r_sa) + 1 + (g.pz << 1);
     g.N = static_cast<int64_t>(g.W) * g.H * g.D;
+    if (g.N > (int64_t(1) << 27))  // a frame of extreme aspect ratio: a grid of gigabytes
+        return fail(ctx, RT_ERR_INVALID, "rt_image_tone_map: the bilateral grid would exceed 2^27 cells");
     const int cells_x = static_cast<int>(static_cast<float>(x_res - 1) / g.s_sa + 0.5f) + 1;
     const int cells_y = static_cast<int>(static_cast<float>(y_res - 1) / g.s_sa + 0.5f) + 1;
     // Axis factors exp(-X^2 / (2 sigma^2)), X the offset wrapped as the
