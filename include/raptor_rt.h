@@ -99,10 +99,11 @@ typedef struct {
 } rt_render_params;
 
 /* Frame statistics. rays: primary, shadow, bounce, bounce-shadow.
- * ms: device time of the frame's kernels (HIP events on the context stream):
- * [0] whole frame, [1] primary closest hit, [2] shadow any hit,
- * [3] bounce closest hit, [4] bounce-shadow any hit, [5] ray generation and
- * shading. visits: node visits and triangle tests of each of the four trace
+ * ms: device time of the frame's kernels (HIP events): [0] whole frame,
+ * [1] primary closest hit, [2] shadow any hit, [3] bounce closest hit,
+ * [4] bounce-shadow any hit, [5] ray generation and shading. With a bounce
+ * the shadow any-hit launch runs on the context's second stream beside the
+ * bounce closest-hit launch, so [1]..[5] can sum to more than [0]. visits: node visits and triangle tests of each of the four trace
  * launches, [2*k] and [2*k+1] for k = primary, shadow, bounce, bounce-shadow
  * (all zero unless counting is on). */
 typedef struct {

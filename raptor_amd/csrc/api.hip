@@ -116,12 +116,16 @@ void release_ctx(rt_ctx *ctx) {
     if (!ctx || ctx->refs.fetch_sub(1) != 1) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->side) (void)hipStreamSynchronize(ctx->side);
     for (auto &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
+    if (ctx->fork) (void)hipEventDestroy(ctx->fork);
+    if (ctx->join) (void)hipEventDestroy(ctx->join);
     if (ctx->d_counters) (void)hipFree(ctx->d_counters);
     if (ctx->d_queue) (void)hipFree(ctx->d_queue);
     ctx->arena.release();
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    if (ctx->side) (void)hipStreamDestroy(ctx->side);
     delete ctx;
 }
 
@@ -166,6 +170,12 @@ int32_t rt_ctx_create(int32_t hip_device, rt_ctx **out) {
         delete ctx;
         return RT_ERR_HIP;
     }
+    if (hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->join, hipEventDisableTiming) != hipSuccess) {
+        rt_ctx_destroy(ctx);
+        return RT_ERR_HIP;
+    }
     for (auto &e : ctx->ev) {
         if (hipEventCreate(&e) != hipSuccess) {
             rt_ctx_destroy(ctx);
@@ -173,11 +183,14 @@ int32_t rt_ctx_create(int32_t hip_device, rt_ctx **out) {
         }
     }
     if (hipMalloc(&ctx->d_counters, 16 * sizeof(unsigned long long)) != hipSuccess ||
-        hipMalloc(&ctx->d_queue, 64 * sizeof(uint32_t)) != hipSuccess) {
+        hipMalloc(&ctx->d_queue, rt::kQueueWords * sizeof(uint32_t)) != hipSuccess) {
         rt_ctx_destroy(ctx);
         return RT_ERR_HIP;
     }
     (void)hipMemset(ctx->d_counters, 0, 16 * sizeof(unsigned long long));
+    // The persistent trace kernels expect their chunk counters at zero and
+    // leave them at zero.
+    (void)hipMemset(ctx->d_queue, 0, rt::kQueueWords * sizeof(uint32_t));
     *out = ctx;
     return RT_OK;
 }

@@ -481,10 +481,41 @@ int32_t trace_chunk(const FramePlan &f, Workspace *w, int64_t tile0, int64_t nti
     hipLaunchKernelGGL(k_shade_hit<0>, grid256(ns), dim3(256), 0, s, sa);
     RT_HIP_CHECK(ctx, hipGetLastError());
     if ((rc = rec(3))) return rc;
+    // With a bounce, the shadow any-hit launch runs on the side stream beside
+    // the bounce closest-hit launch: both read only the primary shading's
+    // queues, and the bounce launch's last waves leave most of the chip idle
+    // while the shadow rays fill it. The main stream waits for the side
+    // stream before anything after this function (the join below is enqueued
+    // on every return path, so no later call can touch the shadow buffers
+    // while they are in use).
+    struct Join {
+        rt_ctx *ctx = nullptr;
+        ~Join() {
+            if (ctx) (void)hipStreamWaitEvent(ctx->stream, ctx->join, 0);
+        }
+    } join;
+    const bool overlap = f.p.bounce && nlights > 0;
+    hipStream_t ss = s;
+    if (overlap) {
+        RT_HIP_CHECK(ctx, hipEventRecord(ctx->fork, s));
+        RT_HIP_CHECK(ctx, hipStreamWaitEvent(ctx->side, ctx->fork, 0));
+        ss = ctx->side;
+    }
+    if (timed) RT_HIP_CHECK(ctx, hipEventRecord(ev[10], ss));
     ctx->count_slot = 1;
     if (nlights > 0 &&
-        (rc = launch_occluded_q(a, w->shadow1, w->blocks1 + nblocks, nlights, ns * nlights, w->occ1, s, 1)))
+        (rc = launch_occluded_q(a, w->shadow1, w->blocks1 + nblocks, nlights, ns * nlights, w->occ1, ss, 1))) {
+        if (overlap) {
+            (void)hipEventRecord(ctx->join, ss);
+            join.ctx = ctx;
+        }
         return rc;
+    }
+    if (timed) RT_HIP_CHECK(ctx, hipEventRecord(ev[11], ss));
+    if (overlap) {
+        RT_HIP_CHECK(ctx, hipEventRecord(ctx->join, ss));
+        join.ctx = ctx;
+    }
     if ((rc = rec(4))) return rc;
     if (f.p.bounce) {
         const uint32_t *order = nullptr;
@@ -551,6 +582,9 @@ int32_t collect_chunk(const FramePlan &f, Workspace *w, int64_t ntiles, ChunkSta
     stats->hits1 = counts[0];
     stats->hits2 = counts[1];
     for (int k = 0; k < 7; ++k) RT_HIP_CHECK(ctx, hipEventElapsedTime(&stats->ms[k], ctx->ev[k], ctx->ev[k + 1]));
+    // The shadow any-hit launch has its own events (it may run on the side
+    // stream); [3] stays its time, [4] the bounce launch's alone.
+    RT_HIP_CHECK(ctx, hipEventElapsedTime(&stats->ms[3], ctx->ev[10], ctx->ev[11]));
     return RT_OK;
 }
 
@@ -641,7 +675,10 @@ int32_t render_frame(rt_accel *a, const rt_camera &cam, const rt_light *lights, 
         stats->ms[3] = cs.ms[4];
         stats->ms[4] = cs.ms[6];
         stats->ms[5] = cs.ms[0] + cs.ms[2] + cs.ms[5] + compose_ms;
-        stats->ms[0] = stats->ms[1] + stats->ms[2] + stats->ms[3] + stats->ms[4] + stats->ms[5];
+        // The whole frame on the device, from ray generation to the end of
+        // compose: with a bounce the shadow launch overlaps the bounce
+        // launch, so the stages can sum to more than this.
+        RT_HIP_CHECK(ctx, hipEventElapsedTime(&stats->ms[0], ctx->ev[0], ctx->ev[9]));
     }
     return RT_OK;
 }

@@ -69,19 +69,27 @@ struct BuildArena {
 struct rt_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    // Second stream (default priority) for a launch that can run beside the
+    // main stream's: the frame's shadow any-hit launch overlaps its bounce
+    // closest-hit launch. fork / join order the two streams.
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
     std::string last_error;
     bool counting = false;
     hipEvent_t ev[16] = {};
     // Device counters when counting is on: (node visits, triangle tests) for
     // slots 0..3, the stack-overflow flag at index 15.
     unsigned long long *d_counters = nullptr;
-    uint32_t *d_queue = nullptr;  // ray queue head of the persistent trace kernels
+    uint32_t *d_queue = nullptr;  // chunk counters of the persistent trace kernels (kQueueWords)
     rt::BuildArena arena;         // build scratch (bvh_build.hip)
     int count_slot = 0;
     std::atomic<int> refs{1};  // the caller's handle + one per scene
 };
 
 namespace rt {
+// Words of rt_ctx::d_queue: eight per-XCD chunk counters 128 bytes apart and
+// the persistent trace kernel's exit counter.
+constexpr int kQueueWords = 1024;
 // Device material row: {ka rgb, shininess}, {kd rgb, transmittance},
 // {ks rgb, refractive index}, {reflectance, kind, chroma x, chroma y},
 // {diffuse, roughness, extinction, 0}.

@@ -120,30 +120,52 @@ def cmd_kernels(args):
     print("\n".join(lines))
 
 
+def _trace_kernel(name):
+    """'rt::k_trace<4, 2, false, 8>' style key of a trace-kernel dispatch, or
+    None for other kernels and for the visit-counting variants."""
+    k = name.replace("void ", "").replace("(anonymous namespace)::", "").split("(rt::TraceArgs")[0].strip()
+    if not (k.startswith("rt::k_trace<") or k.startswith("rt::k_trace_pers<")) or "true" in k:
+        return None
+    return k
+
+
 def cmd_counters(args):
     import collections
 
     res = collections.defaultdict(lambda: collections.defaultdict(list))
+    res_regs = {}
     for d in args.dirs.split(","):
         for r in _rows(d, "*counter_collection.csv"):
-            k = r["Kernel_Name"].replace("void ", "").replace("(anonymous namespace)::", "").split("(rt::TraceArgs")[0]
+            k = _trace_kernel(r["Kernel_Name"])
+            if k is None:
+                continue
             res[k][r["Counter_Name"]].append(float(r["Counter_Value"]))
+            res_regs[k] = (r.get("VGPR_Count"), r.get("SGPR_Count"), r.get("LDS_Block_Size"), r.get("Scratch_Size"),
+                           r.get("Workgroup_Size"))
     names = {0: "primary closest", 1: "shadow any hit", 2: "bounce closest", 3: "bounce-shadow any hit"}
     lines = [f"# Counters: {args.title}", "", "Command (one rocprofv3 pass per counter group):", "",
              "    " + args.command, "", "Averages over the frame launches of each kernel (the counting frame excluded).",
              "", "| kernel | stage | VALUBusy % | VALUUtilization % | Occupancy % | waves / active CU | L2 hit % | "
-             "SQ_INSTS_VALU | SQ_INSTS_SALU |", "|---|---|---|---|---|---|---|---|---|"]
+             "SQ_INSTS_VALU | SQ_INSTS_SALU | SQ_INSTS_LDS | VALU instr / wave |",
+             "|---|---|---|---|---|---|---|---|---|---|---|"]
     for k in sorted(res):
-        if not k.startswith("rt::k_trace<") or "true" in k:
-            continue
         d = {c: sum(v) / len(v) for c, v in res[k].items()}
         st = int(k.split(",")[1])
         hit = 100 * d.get("TCC_HIT_sum", 0.0) / max(1.0, d.get("TCC_HIT_sum", 0.0) + d.get("TCC_MISS_sum", 0.0))
+        per_wave = d.get("SQ_INSTS_VALU", 0.0) / max(1.0, d.get("SQ_WAVES", 0.0))
         lines.append(f"| `{k}` | {names.get(st, st)} | {d.get('VALUBusy', 0):.1f} | {d.get('VALUUtilization', 0):.1f} | "
                      f"{d.get('OccupancyPercent', 0):.1f} | {d.get('MeanOccupancyPerActiveCU', 0):.2f} | {hit:.1f} | "
-                     f"{d.get('SQ_INSTS_VALU', 0):.3g} | {d.get('SQ_INSTS_SALU', 0):.3g} |")
+                     f"{d.get('SQ_INSTS_VALU', 0):.3g} | {d.get('SQ_INSTS_SALU', 0):.3g} | "
+                     f"{d.get('SQ_INSTS_LDS', 0):.3g} | {per_wave:.0f} |")
+    lines += ["", "Resources per kernel (from the same dispatch records):", "",
+              "| kernel | VGPRs | SGPRs | LDS bytes / workgroup | scratch bytes / lane | workgroup |",
+              "|---|---|---|---|---|---|"]
+    for k in sorted(res_regs):
+        v, s, lds, scr, wg = res_regs[k]
+        lines.append(f"| `{k}` | {v} | {s} | {lds} | {scr} | {wg} |")
     lines += ["", "VALUBusy: share of the kernel's time the VALUs issue. VALUUtilization: share of the 64 lanes",
-              "active in the VALU instructions issued (SIMD efficiency)."]
+              "active in the VALU instructions issued (SIMD efficiency). VALU instr / wave: SQ_INSTS_VALU / SQ_WAVES",
+              "(for the block-local form a wave holds many rays over its life)."]
     with open(args.out, "w") as f:
         f.write("\n".join(lines) + "\n")
     print("\n".join(lines))
