@@ -296,10 +296,11 @@ int32_t rt_scene_set_albedo(rt_scene *scene, const float *albedo, uint32_t flags
     hipLaunchKernelGGL(rt::k_set_kd, dim3(static_cast<unsigned>((scene->n + 255) / 256)), dim3(256), 0, s, src,
                        scene->n, scene->d_kd);
     hipError_t e = hipGetLastError();
-    if (tmp) {
-        if (e == hipSuccess) e = hipStreamSynchronize(s);
-        (void)hipFree(tmp);
-    }
+    // The source is borrowed for the call only: wait until the kernel has
+    // read it (the host copy's staging buffer is freed, a device source may
+    // be reused by the caller as soon as the call returns).
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (tmp) (void)hipFree(tmp);
     if (e != hipSuccess) return fail(ctx, RT_ERR_HIP, std::string("rt_scene_set_albedo: ") + hipGetErrorString(e));
     return RT_OK;
 }
@@ -328,8 +329,13 @@ int32_t rt_render_tape(rt_accel *accel, const rt_camera *cam, const rt_light *li
     }
     rt::FramePlan &f = t->plan;
     t->record_bytes = rt::chunk_bytes(f, f.tiles);
-    t->replay = max_record_bytes > 0 && t->record_bytes > max_record_bytes;
-    t->chunk_tiles = t->replay ? rt::tiles_within(f, max_record_bytes) : f.tiles;
+    // A frame beyond one hit-compaction scan cannot keep all its records in
+    // one chunk: it replays, in chunks of at most one scan.
+    const int64_t scan_tiles = rt::scan_chunk_tiles(f);
+    t->replay = (max_record_bytes > 0 && t->record_bytes > max_record_bytes) || f.tiles > scan_tiles;
+    int64_t ct = f.tiles;
+    if (t->replay && max_record_bytes > 0) ct = rt::tiles_within(f, max_record_bytes);
+    t->chunk_tiles = std::min(ct, scan_tiles);
     if (t->chunk_tiles <= 0) {
         rt::free_tape(t);
         return fail(ctx, RT_ERR_INVALID, "rt_render_tape: max_record_bytes is below one tile's records");

@@ -278,8 +278,9 @@ int32_t rt_scene_update(rt_scene *scene, const float *abc, uint32_t flags) {
     hipLaunchKernelGGL(rt::k_scene_move, dim3(static_cast<unsigned>((scene->n + 255) / 256)), dim3(256), 0, s,
                        scene->d_abc, scene->n, scene->d_n, scene->d_c);
     RT_HIP_CHECK(ctx, hipGetLastError());
-    // A host source must stay untouched until the copy has read it.
-    if (!dev) RT_HIP_CHECK(ctx, hipStreamSynchronize(s));
+    // The source (host or device) is borrowed for the call only: it must stay
+    // untouched until the copy has read it.
+    RT_HIP_CHECK(ctx, hipStreamSynchronize(s));
     return RT_OK;
 }
 

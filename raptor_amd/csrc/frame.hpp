@@ -129,7 +129,9 @@ struct Workspace {
     float4 *nshade1, *nshade2;
     uint8_t *occ1, *occ2;
     float *rgb;
-    rt_hit *bounce_out;
+    // Frame-sized image and hit staging of a chunked rt_render (grow-only).
+    void *frame_mem = nullptr;
+    size_t frame_bytes = 0;
     uint32_t *blocks1, *blocks2;  // per-block hit counts, then prefixes; total at [nblocks]
     uint32_t *skeys, *svals, *skeys_t, *svals_t, *shist, *ssums;  // bounce regrouping sort
 };
@@ -162,6 +164,9 @@ struct ChunkStats {
 
 int32_t plan_frame(rt_accel *a, const rt_camera &cam, const rt_light *lights, int nlights, const rt_render_params &p,
                    FramePlan *plan);
+// Most tiles one chunk may hold: the hit compaction's single-block scan
+// covers 1024 * kScanPer blocks of 256 samples.
+int64_t scan_chunk_tiles(const FramePlan &plan);
 // Largest number of tiles per chunk whose workspace fits in budget bytes
 // (0 if even one tile does not fit).
 int64_t tiles_within(const FramePlan &plan, int64_t budget);

@@ -854,12 +854,18 @@ int32_t bilateral(rt_ctx *ctx, float *img, int x_res, int y_res, int blocks) {
     if (!(delta >= 0.0f) || delta / g.r_sa > 4096.0f)
         return fail(ctx, RT_ERR_INVALID, "rt_image_tone_map: the bilateral filter needs a frame with a finite range "
                                          "of bright pixels");
-    g.W = static_cast<int>(static_cast<float>(x_res - 1) / g.s_sa) + 1 + (g.pxy << 1);
-    g.H = static_cast<int>(static_cast<float>(y_res - 1) / g.s_sa) + 1 + (g.pxy << 1);
-    g.D = static_cast<int>(delta / g.r_sa) + 1 + (g.pz << 1);
-    g.N = static_cast<int64_t>(g.W) * g.H * g.D;
-    if (g.N > (int64_t(1) << 27))  // a frame of extreme aspect ratio: a grid of gigabytes
+    // Grid extents from the float quotients, truncated as the reference's
+    // int conversions do, but sized in double first: a frame of extreme
+    // aspect ratio gives a quotient beyond any int (and a grid of gigabytes).
+    const double wd = std::trunc(static_cast<double>(static_cast<float>(x_res - 1) / g.s_sa)) + 1 + (g.pxy << 1);
+    const double hd = std::trunc(static_cast<double>(static_cast<float>(y_res - 1) / g.s_sa)) + 1 + (g.pxy << 1);
+    const double dd = std::trunc(static_cast<double>(delta / g.r_sa)) + 1 + (g.pz << 1);
+    if (wd * hd * dd > static_cast<double>(int64_t(1) << 27))
         return fail(ctx, RT_ERR_INVALID, "rt_image_tone_map: the bilateral grid would exceed 2^27 cells");
+    g.W = static_cast<int>(wd);
+    g.H = static_cast<int>(hd);
+    g.D = static_cast<int>(dd);
+    g.N = static_cast<int64_t>(g.W) * g.H * g.D;
     const int cells_x = static_cast<int>(static_cast<float>(x_res - 1) / g.s_sa + 0.5f) + 1;
     const int cells_y = static_cast<int>(static_cast<float>(y_res - 1) / g.s_sa + 0.5f) + 1;
     // Axis factors exp(-X^2 / (2 sigma^2)), X the offset wrapped as the

@@ -287,20 +287,22 @@ __global__ void __launch_bounds__(256) k_compose(ComposeArgs a) {
     a.rgb[3 * p + 2] = acc.b / fs;
 }
 
-// Hits back in pixel order (p * spp + s) for the caller. For bounce hits the
-// queue is in compaction order, so go through the per-sample queue index.
-__global__ void __launch_bounds__(256) k_hits_out(TileMap tm, int spp, int64_t npix, const int32_t *qidx1,
-                                                  const rt_hit *hits, rt_hit *out) {
+// Hits back in pixel order (p * spp + s) for the caller, for the samples of
+// one chunk (slots [slot0, slot0 + nslots)). For bounce hits the queue is in
+// compaction order, so go through the per-sample queue index.
+__global__ void __launch_bounds__(256) k_hits_out(TileMap tm, int spp, int64_t slot0, int64_t nslots,
+                                                  const int32_t *qidx1, const rt_hit *hits, rt_hit *out) {
     const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (j >= npix * spp) return;
-    const int64_t p = j / spp;
-    const int s = static_cast<int>(j - p * spp);
-    const int64_t i = tm.slot_of_pixel(static_cast<int>(p % tm.x_res), static_cast<int>(p / tm.x_res)) * spp + s;
-    int64_t k = i;
-    if (qidx1) k = qidx1[i];
+    if (j >= nslots * spp) return;
+    const int64_t slot = slot0 + j / spp;
+    const int s = static_cast<int>(j % spp);
+    int x, y;
+    if (!tm.pixel_of_slot(slot, &x, &y)) return;  // tile padding
+    int64_t k = j;
+    if (qidx1) k = qidx1[j];
     float4 v = make_float4(rtd::kMaxDist, 0.0f, 0.0f, __int_as_float(-1));
     if (k >= 0) v = reinterpret_cast<const float4 *>(hits)[k];
-    reinterpret_cast<float4 *>(out)[j] = v;
+    reinterpret_cast<float4 *>(out)[(static_cast<int64_t>(y) * tm.x_res + x) * spp + s] = v;
 }
 
 template <class T>
@@ -329,7 +331,6 @@ size_t workspace_layout(Workspace &w, char *base, int64_t ns, int nl) {
     w.shadow2 = carve<rt_ray>(p, ns * nl);
     w.occ2 = carve<uint8_t>(p, ns * nl);
     w.rgb = carve<float>(p, 3 * ns);
-    w.bounce_out = carve<rt_hit>(p, ns);
     const int64_t nb = (ns + 255) / 256 + 1;
     w.blocks1 = carve<uint32_t>(p, nb);
     w.blocks2 = carve<uint32_t>(p, nb);
@@ -368,7 +369,13 @@ int32_t workspace_reserve(rt_ctx *ctx, Workspace *w, int64_t ns, int nl) {
 void workspace_release(Workspace *w) {
     if (!w) return;
     if (w->mem) (void)hipFree(w->mem);
+    if (w->frame_mem) (void)hipFree(w->frame_mem);
     delete w;
+}
+
+int64_t scan_chunk_tiles(const FramePlan &plan) {
+    // The hit compaction scans at most 1024 * kScanPer blocks of 256 samples.
+    return std::max<int64_t>(1, (int64_t(1024) * kScanPer * 256) / (int64_t(64) * plan.spp));
 }
 
 int32_t plan_frame(rt_accel *a, const rt_camera &cam, const rt_light *lights, int nlights, const rt_render_params &p,
@@ -631,54 +638,100 @@ int32_t render_frame(rt_accel *a, const rt_camera &cam, const rt_light *lights, 
     int32_t rc = plan_frame(a, cam, lights, nlights, p, &f);
     if (rc) return rc;
     if (bounce_hits && !p.bounce) return fail(ctx, RT_ERR_INVALID, "bounce hits requested without bounce");
-    const int64_t ns = f.tiles * 64 * f.spp;
+    // A frame larger than one compaction scan runs as consecutive chunks of
+    // tiles through one chunk-sized workspace; sample order and arithmetic do
+    // not depend on the chunking.
+    const int64_t chunk = std::min(f.tiles, scan_chunk_tiles(f));
+    const int64_t ns = chunk * 64 * f.spp;
     Workspace *w = accel_workspace(a);
     if ((rc = workspace_reserve(ctx, w, ns, std::max(1, nlights)))) return rc;
-    if ((rc = trace_chunk(f, w, 0, f.tiles, stats != nullptr))) return rc;
-    const bool dev_out = (p.flags & RT_DEVICE_PTRS) != 0;
-    float *rgb_dst = (dev_out && rgb) ? rgb : w->rgb;
-    if (stats) RT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[8], s));
-    if ((rc = compose_chunk(f, w, 0, f.tiles, rgb_dst))) return rc;
-    if (stats) RT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[9], s));
 
-    // Outputs.
+    // Frame-sized outputs: the caller's device buffers, else staging kept on
+    // the workspace (grow-only).
+    const bool dev_out = (p.flags & RT_DEVICE_PTRS) != 0;
     const int64_t npix = f.npix;
     const int spp = f.spp;
+    const int64_t nout = npix * spp;
+    float *img = (dev_out && rgb) ? rgb : nullptr;
+    rt_hit *hp = (dev_out && primary) ? primary : nullptr;
+    rt_hit *hb = (dev_out && bounce_hits) ? bounce_hits : nullptr;
+    {
+        const size_t img_bytes = img ? 0 : ((3 * npix * sizeof(float) + 255) / 256) * 256;
+        const size_t hits_bytes = static_cast<size_t>(nout) * sizeof(rt_hit);
+        const size_t need = img_bytes + ((primary && !hp) ? hits_bytes : 0) + ((bounce_hits && !hb) ? hits_bytes : 0);
+        if (need > w->frame_bytes) {
+            if (w->frame_mem) (void)hipFree(w->frame_mem);
+            w->frame_mem = nullptr;
+            w->frame_bytes = 0;
+            hipError_t e = hipMalloc(&w->frame_mem, need);
+            if (e != hipSuccess)
+                return fail(ctx, e == hipErrorOutOfMemory ? RT_ERR_OOM : RT_ERR_HIP,
+                            std::string("frame outputs: ") + hipGetErrorString(e));
+            w->frame_bytes = need;
+        }
+        char *q = reinterpret_cast<char *>(w->frame_mem);
+        if (!img) {
+            img = reinterpret_cast<float *>(q);
+            q += img_bytes;
+        }
+        if (primary && !hp) {
+            hp = reinterpret_cast<rt_hit *>(q);
+            q += hits_bytes;
+        }
+        if (bounce_hits && !hb) hb = reinterpret_cast<rt_hit *>(q);
+    }
+
+    uint64_t hits1 = 0, hits2 = 0;
+    float ms[6] = {0, 0, 0, 0, 0, 0};
+    for (int64_t t0 = 0; t0 < f.tiles; t0 += chunk) {
+        const int64_t nt = std::min(chunk, f.tiles - t0);
+        if ((rc = trace_chunk(f, w, t0, nt, stats != nullptr))) return rc;
+        if (stats) RT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[8], s));
+        if ((rc = compose_chunk(f, w, t0, nt, img))) return rc;
+        if (stats) RT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[9], s));
+        if (hp)
+            hipLaunchKernelGGL(k_hits_out, grid256(nt * 64 * spp), dim3(256), 0, s, f.tm, spp, t0 * 64, nt * 64,
+                               nullptr, w->hits1, hp);
+        if (hb)
+            hipLaunchKernelGGL(k_hits_out, grid256(nt * 64 * spp), dim3(256), 0, s, f.tm, spp, t0 * 64, nt * 64,
+                               w->qidx1, w->hits2, hb);
+        RT_HIP_CHECK(ctx, hipGetLastError());
+        if (stats) {
+            ChunkStats cs;
+            if ((rc = collect_chunk(f, w, nt, &cs))) return rc;  // synchronises
+            float compose_ms = 0.0f, frame_ms = 0.0f;
+            RT_HIP_CHECK(ctx, hipEventElapsedTime(&compose_ms, ctx->ev[8], ctx->ev[9]));
+            RT_HIP_CHECK(ctx, hipEventElapsedTime(&frame_ms, ctx->ev[0], ctx->ev[9]));
+            hits1 += cs.hits1;
+            hits2 += cs.hits2;
+            // Trace stages: primary, shadow, bounce, bounce shadow; the rest
+            // is ray generation, shading and compose. The frame time runs
+            // from ray generation to the end of compose: with a bounce the
+            // shadow launch overlaps the bounce launch, so the stages can sum
+            // to more than it.
+            ms[1] += cs.ms[1];
+            ms[2] += cs.ms[3];
+            ms[3] += cs.ms[4];
+            ms[4] += cs.ms[6];
+            ms[5] += cs.ms[0] + cs.ms[2] + cs.ms[5] + compose_ms;
+            ms[0] += frame_ms;
+        }
+    }
+
+    // Outputs.
     if (rgb && !dev_out)
-        RT_HIP_CHECK(ctx, hipMemcpyAsync(rgb, w->rgb, 3 * npix * sizeof(float), hipMemcpyDeviceToHost, s));
-    if (primary) {
-        hipLaunchKernelGGL(k_hits_out, grid256(npix * spp), dim3(256), 0, s, f.tm, spp, npix, nullptr, w->hits1,
-                           w->bounce_out);
-        RT_HIP_CHECK(ctx, hipMemcpyAsync(primary, w->bounce_out, npix * spp * sizeof(rt_hit),
-                                         dev_out ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
-    }
-    if (bounce_hits) {
-        if (primary) RT_HIP_CHECK(ctx, hipStreamSynchronize(s));  // bounce_out is reused
-        hipLaunchKernelGGL(k_hits_out, grid256(npix * spp), dim3(256), 0, s, f.tm, spp, npix, w->qidx1, w->hits2,
-                           w->bounce_out);
-        RT_HIP_CHECK(ctx, hipMemcpyAsync(bounce_hits, w->bounce_out, npix * spp * sizeof(rt_hit),
-                                         dev_out ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
-    }
-    ChunkStats cs;
-    if (stats && (rc = collect_chunk(f, w, f.tiles, &cs))) return rc;
+        RT_HIP_CHECK(ctx, hipMemcpyAsync(rgb, img, 3 * npix * sizeof(float), hipMemcpyDeviceToHost, s));
+    if (primary && !dev_out)
+        RT_HIP_CHECK(ctx, hipMemcpyAsync(primary, hp, nout * sizeof(rt_hit), hipMemcpyDeviceToHost, s));
+    if (bounce_hits && !dev_out)
+        RT_HIP_CHECK(ctx, hipMemcpyAsync(bounce_hits, hb, nout * sizeof(rt_hit), hipMemcpyDeviceToHost, s));
     RT_HIP_CHECK(ctx, hipStreamSynchronize(s));
     if (stats) {
-        stats->rays[0] = static_cast<uint64_t>(npix * spp);
-        stats->rays[1] = cs.hits1 * nlights;
-        stats->rays[2] = p.bounce ? cs.hits1 : 0;
-        stats->rays[3] = p.bounce ? cs.hits2 * nlights : 0;
-        float compose_ms = 0.0f;
-        RT_HIP_CHECK(ctx, hipEventElapsedTime(&compose_ms, ctx->ev[8], ctx->ev[9]));
-        // Trace stages: primary, shadow, bounce, bounce shadow; the rest is shading.
-        stats->ms[1] = cs.ms[1];
-        stats->ms[2] = cs.ms[3];
-        stats->ms[3] = cs.ms[4];
-        stats->ms[4] = cs.ms[6];
-        stats->ms[5] = cs.ms[0] + cs.ms[2] + cs.ms[5] + compose_ms;
-        // The whole frame on the device, from ray generation to the end of
-        // compose: with a bounce the shadow launch overlaps the bounce
-        // launch, so the stages can sum to more than this.
-        RT_HIP_CHECK(ctx, hipEventElapsedTime(&stats->ms[0], ctx->ev[0], ctx->ev[9]));
+        stats->rays[0] = static_cast<uint64_t>(nout);
+        stats->rays[1] = hits1 * nlights;
+        stats->rays[2] = p.bounce ? hits1 : 0;
+        stats->rays[3] = p.bounce ? hits2 * nlights : 0;
+        for (int k = 0; k < 6; ++k) stats->ms[k] = ms[k];
     }
     return RT_OK;
 }

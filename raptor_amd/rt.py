@@ -184,12 +184,40 @@ def lib():
 
 
 def _ptr(a):
+    """Address of a host array (numpy) for a call without RT_DEVICE_PTRS, or
+    of a device tensor already checked by _dev."""
     if a is None:
         return None
     if isinstance(a, np.ndarray):
-        assert a.flags["C_CONTIGUOUS"]
+        if not a.flags["C_CONTIGUOUS"]:
+            raise ValueError("host arrays passed to the library must be C-contiguous")
         return a.ctypes.data
-    return int(a.data_ptr())  # torch tensor
+    return _dev(a, a.dtype, None, "tensor")
+
+
+def _dev(t, dtype, numel, name):
+    """Device address of a torch tensor the library reads or writes under
+    RT_DEVICE_PTRS. The tensor must be a contiguous GPU tensor of dtype
+    holding numel elements (at least numel when numel is negative, none
+    checked when None); anything else raises ValueError before the library
+    sees it, so a kernel never reads host memory or past the end of a
+    buffer. Torch's current stream is synchronised first: the library works
+    on its own stream, and its calls return only when their kernels are done,
+    so torch may use an output as soon as the call returns."""
+    import torch
+
+    if not isinstance(t, torch.Tensor):
+        raise ValueError(f"{name}: expected a torch tensor, got {type(t).__name__}")
+    if t.dtype != dtype:
+        raise ValueError(f"{name}: dtype {t.dtype}, expected {dtype}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name}: the tensor must be contiguous")
+    if numel is not None and ((numel >= 0 and t.numel() != numel) or (numel < 0 and t.numel() < -numel)):
+        raise ValueError(f"{name}: {t.numel()} elements, expected {'at least ' if numel < 0 else ''}{abs(numel)}")
+    if not t.is_cuda:
+        raise ValueError(f"{name}: a GPU tensor is required (got one on {t.device})")
+    torch.cuda.current_stream(t.device).synchronize()
+    return int(t.data_ptr())
 
 
 class RTError(RuntimeError):
@@ -323,8 +351,11 @@ class Scene:
             a = np.ascontiguousarray(albedo, np.float32).reshape(-1, 3)
             self.ctx.check(_LIB.rt_scene_set_albedo(self.h, _ptr(a), 0), "rt_scene_set_albedo")
         else:
-            a = albedo.detach().contiguous()
-            self.ctx.check(_LIB.rt_scene_set_albedo(self.h, _ptr(a), RT_DEVICE_PTRS), "rt_scene_set_albedo")
+            import torch
+
+            a = albedo.detach()
+            ptr = _dev(a, torch.float32, 3 * self.n, "rt_scene_set_albedo: albedo")
+            self.ctx.check(_LIB.rt_scene_set_albedo(self.h, ptr, RT_DEVICE_PTRS), "rt_scene_set_albedo")
 
     def update(self, abc):
         """New vertex positions, (n, 9) float32: a host array or a device tensor.
@@ -338,12 +369,9 @@ class Scene:
         else:
             import torch
 
-            a = abc.detach().to(torch.float32).contiguous().reshape(-1, 9)
-            if a.shape[0] != self.n:
-                raise ValueError(f"rt_scene_update: {a.shape[0]} triangles, the scene has {self.n}")
-            torch.cuda.current_stream().synchronize()  # the library's stream reads it next
-            self.ctx.check(_LIB.rt_scene_update(self.h, _ptr(a), RT_DEVICE_PTRS), "rt_scene_update")
-            torch.cuda.synchronize()  # the copy has read a before a is released
+            a = abc.detach()
+            ptr = _dev(a, torch.float32, 9 * self.n, "rt_scene_update: vertices")
+            self.ctx.check(_LIB.rt_scene_update(self.h, ptr, RT_DEVICE_PTRS), "rt_scene_update")
 
     def set_materials(self, materials, tri_material):
         """Materials for rt_render: materials is an array of MATERIAL_DTYPE
@@ -444,8 +472,23 @@ def surface_rays(scene, seed, n):
     import torch
 
     rays = torch.empty((n, 8), dtype=torch.float32, device="cuda:0")
-    scene.ctx.check(_LIB.rt_gen_surface_rays(scene.h, seed, n, _ptr(rays)), "rt_gen_surface_rays")
+    ptr = _dev(rays, torch.float32, 8 * n, "rt_gen_surface_rays: rays")
+    scene.ctx.check(_LIB.rt_gen_surface_rays(scene.h, seed, n, ptr), "rt_gen_surface_rays")
     return rays
+
+
+def _batch_items(accels, rays, hits):
+    """rt_batch_item table: per item (n, 8) float32 rays and room for n
+    hits, (n, 4) float32, both contiguous device tensors."""
+    import torch
+
+    items = []
+    for k, (a, r, h) in enumerate(zip(accels, rays, hits)):
+        n = int(r.shape[0]) if r.dim() > 0 else 0
+        rp = _dev(r, torch.float32, 8 * n, f"batch item {k}: rays")
+        hp = _dev(h, torch.float32, 4 * n, f"batch item {k}: hits")
+        items.append(rt_batch_item(a.h.value, rp, n, hp))
+    return (rt_batch_item * len(items))(*items)
 
 
 class Batch:
@@ -454,8 +497,7 @@ class Batch:
     def __init__(self, ctx, accels, rays, hits, lds_stack=8, waves_per_cu=16, grab=1, use_graph=True):
         self.ctx = ctx
         self._keep = (accels, rays, hits)
-        self.items = (rt_batch_item * len(accels))(
-            *[rt_batch_item(a.h.value, _ptr(r), int(r.shape[0]), _ptr(h)) for a, r, h in zip(accels, rays, hits)])
+        self.items = _batch_items(accels, rays, hits)
         self.params = rt_batch_params(lds_stack, waves_per_cu, grab, 1 if use_graph else 0)
         h = C.c_void_p()
         ctx.check(_LIB.rt_batch_create(ctx.h, self.items, len(accels), C.byref(self.params), C.byref(h)),
@@ -469,8 +511,7 @@ class Batch:
 
     @staticmethod
     def autotune(ctx, accels, rays, hits, reps=3):
-        items = (rt_batch_item * len(accels))(
-            *[rt_batch_item(a.h.value, _ptr(r), int(r.shape[0]), _ptr(h)) for a, r, h in zip(accels, rays, hits)])
+        items = _batch_items(accels, rays, hits)
         best = rt_batch_params()
         ms = C.c_float()
         ctx.check(_LIB.rt_batch_autotune(ctx.h, items, len(accels), reps, C.byref(best), C.byref(ms)),
@@ -515,9 +556,10 @@ class Accel:
             self.ctx.check(_LIB.rt_trace_closest(self.h, _ptr(rays), rays.shape[0], _ptr(out), 0), "rt_trace_closest")
             return out
         import torch
-        n = rays.shape[0]
+        n = int(rays.shape[0])
+        rp = _dev(rays, torch.float32, 8 * n, "rt_trace_closest: rays")
         out = torch.empty((n, 4), dtype=torch.float32, device=rays.device)
-        self.ctx.check(_LIB.rt_trace_closest(self.h, _ptr(rays), n, _ptr(out), RT_DEVICE_PTRS), "rt_trace_closest")
+        self.ctx.check(_LIB.rt_trace_closest(self.h, rp, n, _ptr(out), RT_DEVICE_PTRS), "rt_trace_closest")
         return out
 
     def trace_occluded(self, rays):
@@ -528,9 +570,10 @@ class Accel:
                            "rt_trace_occluded")
             return out
         import torch
-        n = rays.shape[0]
+        n = int(rays.shape[0])
+        rp = _dev(rays, torch.float32, 8 * n, "rt_trace_occluded: rays")
         out = torch.empty(n, dtype=torch.uint8, device=rays.device)
-        self.ctx.check(_LIB.rt_trace_occluded(self.h, _ptr(rays), n, _ptr(out), RT_DEVICE_PTRS), "rt_trace_occluded")
+        self.ctx.check(_LIB.rt_trace_occluded(self.h, rp, n, _ptr(out), RT_DEVICE_PTRS), "rt_trace_occluded")
         return out
 
     def render(self, camera, lights, bounce=False, spp=1, seed=0, want_hits=True, stats=True, regroup=False,
@@ -561,7 +604,10 @@ class Accel:
         larr = (rt_light * max(1, nl))(*[l.to_c() for l in lights])
         p = rt_render_params(1 if bounce else 0, spp, seed, RT_DEVICE_PTRS | (RT_REGROUP_BOUNCE if regroup else 0), 0)
         st = rt_stats()
-        self.ctx.check(_LIB.rt_render(self.h, C.byref(cam), larr, nl, C.byref(p), _ptr(rgb_dev), None, None,
+        import torch
+
+        ptr = _dev(rgb_dev, torch.float32, 3 * camera.x_res * camera.y_res, "rt_render: rgb")
+        self.ctx.check(_LIB.rt_render(self.h, C.byref(cam), larr, nl, C.byref(p), ptr, None, None,
                                       C.byref(st)), "rt_render")
         return {"rays": list(st.rays), "ms": list(st.ms), "visits": list(st.visits)}
 
@@ -593,7 +639,11 @@ class Tape:
         p = rt_render_params(1 if bounce else 0, spp, seed, RT_DEVICE_PTRS, 0)
         st = rt_stats()
         h = C.c_void_p()
-        self.ctx.check(_LIB.rt_render_tape(accel.h, C.byref(cam), larr, self.nlights, C.byref(p), _ptr(rgb_dev),
+        import torch
+
+        self.npix = camera.x_res * camera.y_res
+        ptr = _dev(rgb_dev, torch.float32, 3 * self.npix, "rt_render_tape: rgb")
+        self.ctx.check(_LIB.rt_render_tape(accel.h, C.byref(cam), larr, self.nlights, C.byref(p), ptr,
                                            int(max_record_bytes), C.byref(h), C.byref(st)), "rt_render_tape")
         self.h = h
         self.stats = {"rays": list(st.rays), "ms": list(st.ms)}
@@ -605,10 +655,14 @@ class Tape:
 
     def adjoint(self, d_rgb, d_albedo, d_lights):
         """Adds dL/d albedo and dL/d lights (device tensors) for d_rgb = dL/d image; returns ms."""
+        import torch
+
         ms = C.c_float()
         g = d_rgb.contiguous()
-        self.ctx.check(_LIB.rt_render_adjoint(self.h, _ptr(g), _ptr(d_albedo), _ptr(d_lights) if self.nlights else None,
-                                              C.byref(ms)), "rt_render_adjoint")
+        gp = _dev(g, torch.float32, 3 * self.npix, "rt_render_adjoint: d_rgb")
+        ap = _dev(d_albedo, torch.float32, 3 * self.accel.scene.n, "rt_render_adjoint: d_albedo")
+        lp = _dev(d_lights, torch.float32, 6 * self.nlights, "rt_render_adjoint: d_lights") if self.nlights else None
+        self.ctx.check(_LIB.rt_render_adjoint(self.h, gp, ap, lp, C.byref(ms)), "rt_render_adjoint")
         return ms.value
 
     def close(self):
@@ -633,9 +687,15 @@ class InvRender:
         self._keep = scenes
         self.n = len(scenes)
         items = []
-        for accel, camera, albedo, d_albedo, target, image in scenes:
-            items.append(rt_invrender_item(accel.h.value, camera.to_c(), _ptr(albedo), _ptr(d_albedo),
-                                           _ptr(target) if target is not None else None, _ptr(image)))
+        import torch
+
+        for k, (accel, camera, albedo, d_albedo, target, image) in enumerate(scenes):
+            npix, n = camera.x_res * camera.y_res, accel.scene.n
+            items.append(rt_invrender_item(
+                accel.h.value, camera.to_c(), _dev(albedo, torch.float32, 3 * n, f"scene {k}: albedo"),
+                _dev(d_albedo, torch.float32, 3 * n, f"scene {k}: d_albedo"),
+                _dev(target, torch.float32, 3 * npix, f"scene {k}: target") if target is not None else None,
+                _dev(image, torch.float32, 3 * npix, f"scene {k}: image")))
         self.items = (rt_invrender_item * self.n)(*items)
         larr = (rt_light * max(1, len(lights)))(*[l.to_c() for l in lights])
         self.params = rt_invrender_params(spp, bounces, seed, lr, beta1, beta2, eps, 1 if use_graph else 0,
@@ -679,8 +739,11 @@ def _image_arg(img):
     if isinstance(img, np.ndarray):
         a = np.ascontiguousarray(img, np.float32)
         return a, 0
-    if not img.is_contiguous():
-        raise ValueError("device images must be contiguous")
+    import torch
+
+    if img.dim() != 3 or img.shape[2] != 3:
+        raise ValueError(f"device images are (H, W, 3), got {tuple(img.shape)}")
+    _dev(img, torch.float32, None, "image")
     return img, RT_DEVICE_PTRS
 
 
@@ -694,10 +757,6 @@ def tone_map(ctx, img, mode=TONE_NONE, key=0.18, xw=1.0 / 3.0, yw=1.0 / 3.0, des
     a, flags = _image_arg(img)
     if flags == 0:
         a = a.copy()
-    else:
-        import torch
-
-        torch.cuda.current_stream().synchronize()
     h, w = a.shape[0], a.shape[1]
     tp = rt_tone_params(int(mode), key, xw, yw, int(desaturate), int(stretch), int(adapt), time_step,
                         adaptation_level, 0.0, x_angle, y_angle)
@@ -711,10 +770,6 @@ def gamma_correct(ctx, img, gamma):
     a, flags = _image_arg(img)
     if flags == 0:
         a = a.copy()
-    else:
-        import torch
-
-        torch.cuda.current_stream().synchronize()
     ms = C.c_float()
     ctx.check(_LIB.rt_image_gamma(ctx.h, _ptr(a), a.numel() // 3 if flags else a.size // 3, gamma, flags,
                                   C.byref(ms)), "rt_image_gamma")
@@ -734,7 +789,7 @@ def clip(ctx, img, x_samples=1, y_samples=1, bgr=False, out=None):
         import torch
 
         o = out if out is not None else torch.empty((oh, ow, 3), dtype=torch.uint8, device=a.device)
-        torch.cuda.current_stream().synchronize()
+        _dev(o, torch.uint8, 3 * oh * ow, "rt_image_clip: out")
     ms = C.c_float()
     ctx.check(_LIB.rt_image_clip(ctx.h, _ptr(a), w, h, x_samples, y_samples, int(bgr), _ptr(o), flags, C.byref(ms)),
               "rt_image_clip")
@@ -764,7 +819,12 @@ def write_jpeg(ctx, path, rgb_bytes, quality=90):
         b = np.ascontiguousarray(rgb_bytes, np.uint8)
         flags = 0
     else:
+        import torch
+
         b = rgb_bytes.contiguous()
+        if b.dim() != 3 or b.shape[2] != 3:
+            raise ValueError(f"rt_write_jpeg: bytes are (H, W, 3), got {tuple(b.shape)}")
+        _dev(b, torch.uint8, None, "rt_write_jpeg: bytes")
         flags = RT_DEVICE_PTRS
     ctx.check(_LIB.rt_write_jpeg(ctx.h, str(path).encode(), _ptr(b), b.shape[1], b.shape[0], int(quality), flags,
                                  C.byref(ms)), "rt_write_jpeg")

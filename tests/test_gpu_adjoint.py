@@ -190,3 +190,29 @@ def test_adam_recovers_albedo(setup):
         losses.append(float(loss.detach()))
     s["scene"].set_albedo(w.albedo)
     assert losses[-1] < 0.5 * losses[0], losses
+
+
+def test_albedo_written_on_torch_stream_reaches_the_forward(gpu_ctx, rt):
+    """An albedo tensor changed on torch's stream just before diff.render
+    (behind a queue of torch work) is what the library's forward shades:
+    the image equals a frame with the same albedo set from the host."""
+    import torch
+
+    from raptor_amd import diff, workloads
+
+    w = workloads.diff_terrain(grid=48, x_res=64, y_res=48, spp=1, seed=9, zoom=0.4, amplitude=0.3)
+    scene = rt.Scene(gpu_ctx, w.abc, w.albedo)
+    accel = rt.Accel(scene)
+    albedo = torch.from_numpy(w.albedo).cuda().requires_grad_(False)
+    lp = diff.light_params(w.lights)
+    torch.cuda.synchronize()
+    # Queue enough torch work that a read on another stream would overtake it.
+    m = torch.randn((2048, 2048), dtype=torch.float32, device="cuda:0")
+    for _ in range(8):
+        m = torch.mm(m, m.t().contiguous()) * 1e-3
+    albedo.mul_(0.5)
+    img = diff.render(accel, w.camera, w.lights, albedo, lp, bounce=True, spp=1, seed=w.seed)
+    got = img.detach().cpu().numpy()
+    scene.set_albedo((w.albedo * np.float32(0.5)).astype(np.float32))
+    ref = accel.render(w.camera, w.lights, bounce=True, spp=1, seed=w.seed)["rgb"]
+    assert np.array_equal(got, ref)

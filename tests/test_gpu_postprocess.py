@@ -133,6 +133,20 @@ def test_post_process_errors(gpu_ctx, rt):
         rt.clip(gpu_ctx, img, 5, 1)
 
 
+def test_bilateral_refuses_an_extremely_wide_frame(gpu_ctx, rt):
+    """One row of 50M pixels: the grid's x extent (49,999,999 / 0.02 cells)
+    is beyond any int, so it is sized in double and refused before any
+    narrowing conversion."""
+    import torch
+
+    img = torch.full((1, 50_000_000, 3), 100.0, dtype=torch.float32, device="cuda:0")
+    img[0, ::7] = 3.0  # a range of bright pixels, so the size check is reached
+    with pytest.raises(rt.RTError, match="2\\^27"):
+        rt.tone_map(gpu_ctx, img, mode=8)
+    del img
+    torch.cuda.empty_cache()
+
+
 @pytest.mark.parametrize("ds,sc", [(False, False), (True, False), (False, True)])
 @pytest.mark.parametrize("res", [(320, 180), (256, 256)])
 def test_bilateral_tone_map_matches_oracle(gpu_ctx, rt, O, ds, sc, res):
