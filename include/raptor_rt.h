@@ -354,6 +354,22 @@ int32_t rt_trace_occluded(rt_accel *accel, const rt_ray *rays, int64_t n, uint8_
 int32_t rt_render(rt_accel *accel, const rt_camera *cam, const rt_light *lights, int32_t nlights,
                   const rt_render_params *params, float *rgb, rt_hit *primary, rt_hit *bounce, rt_stats *stats);
 
+/* rt_render without any device allocation. No counterpart in the reference:
+ * for callers that cannot allocate (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc). The frame's workspace (rays, hits,
+ * queues, shadow rays, compaction counts) lives in a static device array of
+ * 64 MiB per GPU, held from the library's load on that device until the
+ * process ends; frames whose workspace exceeds it run in chunks of 8x8 tiles
+ * that fit, and a frame whose single tile does not fit (a very high spp)
+ * returns RT_ERR_INVALID. rgb (required), primary and bounce are device
+ * pointers (RT_DEVICE_PTRS is implied). Scenes with materials or vertex
+ * normals return RT_ERR_UNSUPPORTED. Every call on one GPU shares that GPU's
+ * array, from any context: calls that overlap on one GPU (other host
+ * threads, other contexts) race and must be serialised by the caller. */
+int32_t rt_render_static(rt_accel *accel, const rt_camera *cam, const rt_light *lights, int32_t nlights,
+                         const rt_render_params *params, float *rgb, rt_hit *primary, rt_hit *bounce,
+                         rt_stats *stats);
+
 /* ---- Batched tracing (many scenes in one launch) ----
  * An item is one scene's BVH with a batch of rays and room for the hits, all
  * device pointers. Items of one batch share a node width. The reference has no

@@ -191,6 +191,12 @@ int32_t rt_ctx_create(int32_t hip_device, rt_ctx **out) {
     // The persistent trace kernels expect their chunk counters at zero and
     // leave them at zero.
     (void)hipMemset(ctx->d_queue, 0, rt::kQueueWords * sizeof(uint32_t));
+    // The device address of this device's static scratch, looked up once
+    // with the device current (each GPU holds its own copy of the array).
+    if (rt::static_scratch_address(ctx, &ctx->d_static) != RT_OK) {
+        rt_ctx_destroy(ctx);
+        return RT_ERR_HIP;
+    }
     *out = ctx;
     return RT_OK;
 }
@@ -521,6 +527,37 @@ int32_t rt_render(rt_accel *accel, const rt_camera *cam, const rt_light *lights,
     } else {
         rc = rt::render_frame(accel, *cam, lights, nlights, p, rgb, primary, bounce, stats);
     }
+    ctx->count_slot = 0;
+    if (rc == RT_OK && stats) {
+        for (auto &v : stats->visits) v = 0;
+        if (ctx->counting) {
+            unsigned long long v[16];
+            RT_HIP_CHECK(ctx, hipMemcpy(v, ctx->d_counters, sizeof(v), hipMemcpyDeviceToHost));
+            for (int k = 0; k < 8; ++k) stats->visits[k] = v[k];
+            if (v[15]) return fail(ctx, RT_ERR_INVALID, "traversal stack overflow");
+        }
+    }
+    return rc;
+}
+
+int32_t rt_render_static(rt_accel *accel, const rt_camera *cam, const rt_light *lights, int32_t nlights,
+                         const rt_render_params *params, float *rgb, rt_hit *primary, rt_hit *bounce,
+                         rt_stats *stats) {
+    if (!accel || !cam || !rgb || (nlights > 0 && !lights)) return RT_ERR_INVALID;
+    rt_ctx *ctx = accel->scene->ctx;
+    if (int32_t rc = rt::set_device(ctx)) return rc;
+    if (!ctx->d_static) return fail(ctx, RT_ERR_HIP, "rt_render_static: no static scratch on this device");
+    if (accel->scene->d_mat || accel->scene->d_vn)
+        return fail(ctx, RT_ERR_UNSUPPORTED,
+                    "rt_render_static: frames with materials or vertex normals size their generations at run time; "
+                    "use rt_render");
+    rt_render_params p{0, 1, 0, 0, 0};
+    if (params) p = *params;
+    p.flags |= RT_DEVICE_PTRS;
+    if (ctx->counting)
+        RT_HIP_CHECK(ctx, hipMemsetAsync(ctx->d_counters, 0, 16 * sizeof(unsigned long long), ctx->stream));
+    int32_t rc = rt::render_frame(accel, *cam, lights, nlights, p, rgb, primary, bounce, stats, ctx->d_static,
+                                  rt::kStaticScratchBytes);
     ctx->count_slot = 0;
     if (rc == RT_OK && stats) {
         for (auto &v : stats->visits) v = 0;

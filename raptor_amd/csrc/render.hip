@@ -373,6 +373,23 @@ void workspace_release(Workspace *w) {
     delete w;
 }
 
+// Workspace of rt_render_static: one array per GPU, allocated when the
+// library's code object is loaded on that device and held until the process
+// ends, so a frame needs no hipMalloc. In host code `scratch` is only a
+// stand-in for the symbol; its device address comes from
+// static_scratch_address, once per device, and only that address is handed
+// to kernels.
+static __device__ __align__(256) char scratch[kStaticScratchBytes];
+
+int32_t static_scratch_address(rt_ctx *ctx, void **ptr) {
+    *ptr = nullptr;
+    void *p = nullptr;
+    RT_HIP_CHECK(ctx, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(ctx, RT_ERR_HIP, "static scratch: no device address");
+    *ptr = p;
+    return RT_OK;
+}
+
 int64_t scan_chunk_tiles(const FramePlan &plan) {
     // The hit compaction scans at most 1024 * kScanPer blocks of 256 samples.
     return std::max<int64_t>(1, (int64_t(1024) * kScanPer * 256) / (int64_t(64) * plan.spp));
@@ -631,20 +648,32 @@ Workspace *accel_workspace(rt_accel *a) {
 
 int32_t render_frame(rt_accel *a, const rt_camera &cam, const rt_light *lights, int nlights,
                      const rt_render_params &p, float *rgb, rt_hit *primary, rt_hit *bounce_hits,
-                     rt_stats *stats) {
+                     rt_stats *stats, void *fixed_ws, size_t fixed_bytes) {
     rt_ctx *ctx = a->scene->ctx;
     hipStream_t s = ctx->stream;
     FramePlan f;
     int32_t rc = plan_frame(a, cam, lights, nlights, p, &f);
     if (rc) return rc;
     if (bounce_hits && !p.bounce) return fail(ctx, RT_ERR_INVALID, "bounce hits requested without bounce");
-    // A frame larger than one compaction scan runs as consecutive chunks of
-    // tiles through one chunk-sized workspace; sample order and arithmetic do
-    // not depend on the chunking.
-    const int64_t chunk = std::min(f.tiles, scan_chunk_tiles(f));
-    const int64_t ns = chunk * 64 * f.spp;
-    Workspace *w = accel_workspace(a);
-    if ((rc = workspace_reserve(ctx, w, ns, std::max(1, nlights)))) return rc;
+    // A frame larger than one compaction scan (or than a fixed workspace)
+    // runs as consecutive chunks of tiles through one chunk-sized workspace;
+    // sample order and arithmetic do not depend on the chunking.
+    int64_t chunk = std::min(f.tiles, scan_chunk_tiles(f));
+    Workspace fixed;
+    Workspace *w = &fixed;
+    if (fixed_ws) {
+        const int64_t fit = tiles_within(f, static_cast<int64_t>(fixed_bytes));
+        if (fit <= 0)
+            return fail(ctx, RT_ERR_INVALID,
+                        "rt_render_static: one 8x8 tile at " + std::to_string(f.spp) + " samples per pixel and " +
+                            std::to_string(nlights) + " lights needs " + std::to_string(chunk_bytes(f, 1)) +
+                            " bytes of workspace; the static scratch holds " + std::to_string(fixed_bytes));
+        chunk = std::min(chunk, fit);
+        workspace_layout(fixed, reinterpret_cast<char *>(fixed_ws), chunk * 64 * f.spp, std::max(1, nlights));
+    } else {
+        w = accel_workspace(a);
+        if ((rc = workspace_reserve(ctx, w, chunk * 64 * f.spp, std::max(1, nlights)))) return rc;
+    }
 
     // Frame-sized outputs: the caller's device buffers, else staging kept on
     // the workspace (grow-only).
@@ -655,7 +684,9 @@ int32_t render_frame(rt_accel *a, const rt_camera &cam, const rt_light *lights, 
     float *img = (dev_out && rgb) ? rgb : nullptr;
     rt_hit *hp = (dev_out && primary) ? primary : nullptr;
     rt_hit *hb = (dev_out && bounce_hits) ? bounce_hits : nullptr;
-    {
+    if (fixed_ws && !(dev_out && img))
+        return fail(ctx, RT_ERR_INVALID, "rt_render_static: rgb and the hit outputs must be device buffers");
+    if (!fixed_ws) {
         const size_t img_bytes = img ? 0 : ((3 * npix * sizeof(float) + 255) / 256) * 256;
         const size_t hits_bytes = static_cast<size_t>(nout) * sizeof(rt_hit);
         const size_t need = img_bytes + ((primary && !hp) ? hits_bytes : 0) + ((bounce_hits && !hb) ? hits_bytes : 0);

@@ -81,6 +81,7 @@ struct rt_ctx {
     // slots 0..3, the stack-overflow flag at index 15.
     unsigned long long *d_counters = nullptr;
     uint32_t *d_queue = nullptr;  // chunk counters of the persistent trace kernels (kQueueWords)
+    void *d_static = nullptr;     // this device's static scratch (rt_render_static), not owned
     rt::BuildArena arena;         // build scratch (bvh_build.hip)
     int count_slot = 0;
     std::atomic<int> refs{1};  // the caller's handle + one per scene
@@ -178,8 +179,15 @@ int32_t launch_occluded_q(rt_accel *a, const rt_ray *d_rays, const uint32_t *d_c
 // Frame (render.hip).
 void free_frame_workspace(rt_accel *a);
 int32_t refit_accel(rt_accel *a);
+// fixed_ws: a workspace of fixed_bytes to use instead of the accel's own
+// (rt_render_static); the outputs must then be device pointers.
 int32_t render_frame(rt_accel *a, const rt_camera &cam, const rt_light *lights, int nlights,
-                     const rt_render_params &p, float *rgb, rt_hit *primary, rt_hit *bounce, rt_stats *stats);
+                     const rt_render_params &p, float *rgb, rt_hit *primary, rt_hit *bounce, rt_stats *stats,
+                     void *fixed_ws = nullptr, size_t fixed_bytes = 0);
+// The static scratch of rt_render_static (render.hip): kStaticScratchBytes
+// per GPU; its device address on the current device.
+constexpr size_t kStaticScratchBytes = size_t(64) << 20;
+int32_t static_scratch_address(rt_ctx *ctx, void **ptr);
 // Camera post-process (postprocess.hip).
 int32_t image_tone_map(rt_ctx *ctx, float *rgb, int32_t x_res, int32_t y_res, rt_tone_params *tp, uint32_t flags,
                        float *ms);

@@ -129,6 +129,8 @@ def load_library(path=LIB_PATH):
         "rt_trace_occluded": (i32, [vp, fp, i64, fp, u32]),
         "rt_render": (i32, [vp, C.POINTER(rt_camera), C.POINTER(rt_light), i32, C.POINTER(rt_render_params),
                             fp, fp, fp, C.POINTER(rt_stats)]),
+        "rt_render_static": (i32, [vp, C.POINTER(rt_camera), C.POINTER(rt_light), i32, C.POINTER(rt_render_params),
+                                   fp, fp, fp, C.POINTER(rt_stats)]),
         "rt_set_counting": (i32, [vp, i32]),
         "rt_batch_create": (i32, [vp, C.POINTER(rt_batch_item), i32, C.POINTER(rt_batch_params), C.POINTER(vp)]),
         "rt_batch_run": (i32, [vp, i32, C.POINTER(C.c_float)]),
@@ -597,18 +599,25 @@ class Accel:
             out["stats"] = {"rays": list(st.rays), "ms": list(st.ms), "visits": list(st.visits)}
         return out
 
-    def render_device(self, camera, lights, rgb_dev, bounce=False, spp=1, seed=0, regroup=False):
-        """Frame into a device buffer (no host copies); returns the stats dict."""
+    def render_device(self, camera, lights, rgb_dev, bounce=False, spp=1, seed=0, regroup=False, primary_dev=None,
+                      bounce_dev=None, static=False):
+        """Frame into device buffers (no host copies); returns the stats dict.
+        rgb_dev (H, W, 3) float32; primary_dev, bounce_dev: optional (H*W*spp,
+        4) float32 hit outputs. static=True calls rt_render_static (no device
+        allocation; the frame's workspace is the library's static scratch)."""
+        import torch
+
         cam = camera.to_c()
         nl = len(lights)
         larr = (rt_light * max(1, nl))(*[l.to_c() for l in lights])
         p = rt_render_params(1 if bounce else 0, spp, seed, RT_DEVICE_PTRS | (RT_REGROUP_BOUNCE if regroup else 0), 0)
         st = rt_stats()
-        import torch
-
+        nout = camera.x_res * camera.y_res * spp
         ptr = _dev(rgb_dev, torch.float32, 3 * camera.x_res * camera.y_res, "rt_render: rgb")
-        self.ctx.check(_LIB.rt_render(self.h, C.byref(cam), larr, nl, C.byref(p), ptr, None, None,
-                                      C.byref(st)), "rt_render")
+        pp = None if primary_dev is None else _dev(primary_dev, torch.float32, 4 * nout, "rt_render: primary")
+        bp = None if bounce_dev is None else _dev(bounce_dev, torch.float32, 4 * nout, "rt_render: bounce")
+        fn, name = (_LIB.rt_render_static, "rt_render_static") if static else (_LIB.rt_render, "rt_render")
+        self.ctx.check(fn(self.h, C.byref(cam), larr, nl, C.byref(p), ptr, pp, bp, C.byref(st)), name)
         return {"rays": list(st.rays), "ms": list(st.ms), "visits": list(st.visits)}
 
     def render_tape(self, camera, lights, rgb_dev, bounce=False, spp=1, seed=0, max_record_bytes=0):
