@@ -8,10 +8,11 @@ Each case is read from the reference's test sources as text
 regression_fixture arguments (model path under $RAPTOR_DATA, format, camera
 position and axes, background colour, pan/tilt/roll, resolution) and the
 add_light calls (rgb, centre, distance falloff, radius); an NFF case takes
-its camera, background and lights from the file. Run it where the
-reference checkout is:
+its camera, background and lights from the file. Fixture tooling, not part
+of the library: run it where the reference checkout is, from the repository
+root:
 
-    python -m raptor_amd.make_regression_cases /path/to/RAPTor
+    python tests/golden/make_regression_cases.py /path/to/RAPTor
 """
 import json
 import os
@@ -19,7 +20,7 @@ import re
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-OUT = os.path.join(HERE, "regression_cases.json")
+OUT = os.path.join(os.path.dirname(os.path.dirname(HERE)), "raptor_amd", "regression_cases.json")
 SUITES = ("off", "ply", "obj", "nff")
 
 

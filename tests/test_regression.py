@@ -107,7 +107,12 @@ def test_case_table():
 
 @pytest.mark.skipif(not os.path.isdir(REF_TESTS), reason="reference checkout absent")
 def test_case_table_matches_the_reference_sources(tmp_path, monkeypatch):
-    from raptor_amd import make_regression_cases as M
+    import importlib.util
+
+    spec = importlib.util.spec_from_file_location(
+        "make_regression_cases", os.path.join(os.path.dirname(__file__), "golden", "make_regression_cases.py"))
+    M = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(M)
 
     monkeypatch.setattr(M, "OUT", str(tmp_path / "cases.json"))
     M.main("/root/reference/RAPTor")
