@@ -29,7 +29,7 @@ HIT_DTYPE = np.dtype([("t", "<f4"), ("u", "<f4"), ("v", "<f4"), ("prim", "<i4")]
 EXPORTS = [
     "rt_ctx_create", "rt_ctx_destroy", "rt_last_error", "rt_scene_create", "rt_scene_destroy",
     "rt_scene_normals", "rt_accel_build", "rt_accel_destroy", "rt_accel_info_get", "rt_trace_closest",
-    "rt_trace_occluded", "rt_render", "rt_set_counting", "rt_gen_random_triangles", "rt_gen_terrain",
+    "rt_trace_occluded", "rt_render", "rt_render_static", "rt_set_counting", "rt_gen_random_triangles", "rt_gen_terrain",
     "rt_batch_create", "rt_batch_run", "rt_batch_destroy", "rt_batch_autotune", "rt_gen_surface_rays",
     "rt_scene_set_albedo", "rt_render_tape", "rt_tape_info", "rt_render_adjoint", "rt_tape_destroy",
     "rt_invrender_create", "rt_invrender_run", "rt_invrender_destroy", "rt_scene_update", "rt_accel_refit",
