@@ -107,8 +107,27 @@ def random65k(n=65536, res=256):
                     meta=dict(triangles=n, resolution=[res, res]))
 
 
+def terrain_arrays(grid, amplitude=0.15, zoom=0.02, noise_seed=57, albedo_seed=2, lo=0.2, hi=0.8):
+    """fBm terrain of grid x grid vertices (rt.gen_terrain) with at least the
+    1M terrain's cell size: a grid beyond 708 covers [0, (grid - 1) / 707]^2
+    in x and z instead of the unit square (heights unchanged, zoom per cell
+    unchanged, so the relief per unit of distance is the 1M terrain's). The
+    reference's Moller-Trumbore rejects |det| < 1e-6 (rt/triangle.h:190-235),
+    and det is twice the triangle's area times the cosine to the ray: on the
+    unit square a 1414^2 grid has 31% of its triangles below that area and a
+    2795^2 grid none above it, so they would be invisible to every ray."""
+    abc, alb = rt.gen_terrain(grid, amplitude, zoom, 0.5, 8, noise_seed, albedo_seed, lo, hi)
+    ext = max(1.0, (grid - 1) / 707.0)
+    if ext > 1.0:
+        v = abc.reshape(-1, 3, 3)
+        v[:, :, 0] *= np.float32(ext)
+        v[:, :, 2] *= np.float32(ext)
+        abc = v.reshape(-1, 9)
+    return abc, alb
+
+
 def terrain(grid=708, x_res=1920, y_res=1080, bounce=True, seed=7, zoom=0.02, amplitude=0.15):
-    abc, alb = rt.gen_terrain(grid, amplitude, zoom, 0.5, 8, 57, 2, 0.2, 0.8)
+    abc, alb = terrain_arrays(grid, amplitude, zoom)
     pos = (0.5, 0.55, -0.35)
     x, y, z = look_at(pos, (0.5, 0.0, 0.6))
     w = 0.5
@@ -283,7 +302,8 @@ class BatchScene:
 
 def batched(n_scenes=64, tri_lo=1e3, tri_hi=2e6, side_lo=64, side_hi=1024, seed=3):
     """Many independent scenes: triangle counts log-uniform in [tri_lo, tri_hi],
-    terrain (grid ceil(sqrt(N/2)) + 1) or random triangles 50/50, ray batches of
+    terrain (grid ceil(sqrt(N/2)) + 1, cells at least the 1M terrain's, see
+    terrain_arrays) or random triangles 50/50, ray batches of
     side x side incoherent surface rays with side log-uniform in [side_lo, side_hi]."""
     rng = np.random.default_rng(seed)
     out = []
@@ -292,7 +312,7 @@ def batched(n_scenes=64, tri_lo=1e3, tri_hi=2e6, side_lo=64, side_hi=1024, seed=
         side = int(np.exp(rng.uniform(np.log(side_lo), np.log(side_hi))))
         if rng.uniform() < 0.5:
             g = int(np.ceil(np.sqrt(n / 2.0))) + 1
-            abc, alb = rt.gen_terrain(g, 0.15, 0.02, 0.5, 8, 57 + s, 100 + s, 0.2, 0.8)
+            abc, alb = terrain_arrays(g, 0.15, 0.02, 57 + s, 100 + s, 0.2, 0.8)
             kind = "terrain"
         else:
             abc, alb = rt.gen_random_triangles(1000 + s, n, 0.01, 0.0, 1.0)
@@ -301,20 +321,24 @@ def batched(n_scenes=64, tri_lo=1e3, tri_hi=2e6, side_lo=64, side_hi=1024, seed=
     return out
 
 
-def invrender_scenes(n_scenes=16, tri_lo=5e5, tri_hi=1.6e7, res=512, seed=21):
+def invrender_scenes(n_scenes=16, tri_lo=5e5, tri_hi=1.6e7, res=512, seed=21, pick=None):
     """Inverse-rendering batch: triangle counts log-uniform in [tri_lo,
-    tri_hi], terrain (grid ceil(sqrt(N/2)) + 1, relief as the 1M terrain) or
-    random triangles 50/50, each seen res x res by the terrain camera under
+    tri_hi], terrain (grid ceil(sqrt(N/2)) + 1 at the 1M terrain's cell size,
+    see terrain_arrays) or random triangles 50/50, each seen res x res by the terrain camera under
     the four ring lights of diff_terrain. The generated albedo is the ground
-    truth that inverse rendering should recover."""
+    truth that inverse rendering should recover. pick: indices of the scenes
+    to generate (the draws of the others are made and skipped)."""
     rng = np.random.default_rng(seed)
     base = diff_terrain(grid=16, x_res=res, y_res=res, spp=2, seed=seed)
     out = []
     for s in range(n_scenes):
         n = int(np.exp(rng.uniform(np.log(tri_lo), np.log(tri_hi))))
-        if rng.uniform() < 0.5:
+        terrain_kind = rng.uniform() < 0.5
+        if pick is not None and s not in pick:
+            continue
+        if terrain_kind:
             g = int(np.ceil(np.sqrt(n / 2.0))) + 1
-            abc, alb = rt.gen_terrain(g, 0.15, 0.02 * 708.0 / g, 0.5, 8, 57 + s, 300 + s, 0.2, 0.8)
+            abc, alb = terrain_arrays(g, 0.15, 0.02, 57 + s, 300 + s, 0.2, 0.8)
             kind = "terrain"
         else:
             abc, alb = rt.gen_random_triangles(2000 + s, n, 0.01, 0.2, 0.8)
