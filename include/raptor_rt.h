@@ -336,6 +336,26 @@ int32_t rt_write_tga(const char *path, const uint8_t *bgr, int32_t x_res, int32_
 int32_t rt_write_jpeg(rt_ctx *ctx, const char *path, const uint8_t *rgb, int32_t x_res, int32_t y_res,
                       int32_t quality, uint32_t flags, float *ms);
 
+/* Inspection of the build (tests and tools; host outputs).
+ * rt_build_morton runs the first stages of rt_accel_build on the scene:
+ * per-triangle bounds, 30-bit Morton codes of the box centroids and the
+ * stable LSD radix sort by code. codes[i]: triangle i's code (emplace order);
+ * order[k]: the emplace id at position k of the sorted order. Counterpart of
+ * the reference builder's Morton stages (ssd/bvh_builder.cc:33-158), with one
+ * cell size for all axes (the largest centroid extent / 1024) where the
+ * reference quantises each axis by its own extent (ssd/bvh_builder.cc:45-48).
+ * rt_accel_download copies the tree: rt_accel_info.nodes wide nodes of width
+ * W (float lo_x[W], hi_x[W], lo_y[W], hi_y[W], lo_z[W], hi_z[W]; int32
+ * child[W]; padding to 128 bytes for W = 4, 256 for W = 8; child >= 0 an
+ * interior node, 0x80000000 | first << 4 | (count - 1) a leaf of count
+ * triangles at BVH positions [first, first + count), -1 empty) and tri_ids[n],
+ * the emplace id at each BVH position. rt_accel_root gives the root's child
+ * code (0, or a leaf code for a one-triangle scene, -1 for an empty one) and
+ * the padding the build added to every box. */
+int32_t rt_build_morton(rt_scene *scene, uint32_t *codes, uint32_t *order);
+int32_t rt_accel_download(const rt_accel *accel, void *nodes, int32_t *tri_ids);
+int32_t rt_accel_root(const rt_accel *accel, int32_t *root, float *box_pad);
+
 /* Closest hit for n rays. Replaces bvh::find_nearest_object(ray*)
  * (ssd/bvh.cc:649-709) and its packet form (ssd/bvh.cc:406-447). */
 int32_t rt_trace_closest(rt_accel *accel, const rt_ray *rays, int64_t n, rt_hit *out, uint32_t flags);

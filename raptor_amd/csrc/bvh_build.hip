@@ -20,6 +20,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstring>
 #include <vector>
 
 #include "radix_sort.hpp"
@@ -1788,3 +1789,82 @@ int32_t refit_accel(rt_accel *a) {
 }
 
 }  // namespace rt
+
+// ---------------------------------------------------------------------------
+// Inspection of the build (tests and tools): the Morton stages on their own,
+// and a download of the finished tree.
+
+extern "C" {
+
+int32_t rt_build_morton(rt_scene *scene, uint32_t *codes, uint32_t *order) {
+    if (!scene || (scene->n > 0 && (!codes || !order))) return RT_ERR_INVALID;
+    rt_ctx *ctx = scene->ctx;
+    RT_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    const int64_t n = scene->n;
+    if (n == 0) return RT_OK;
+    if (n >= (int64_t(1) << 27)) return rt::fail(ctx, RT_ERR_INVALID, "too many triangles (limit 2^27)");
+    hipStream_t s = ctx->stream;
+    rt::BuildArena &ar = ctx->arena;
+    {
+        const hipError_t e = ar.begin();
+        if (e != hipSuccess) return rt::fail(ctx, RT_ERR_OOM, std::string("build scratch: ") + hipGetErrorString(e));
+    }
+    const int64_t tiles = (n + rt::kSortTile - 1) / rt::kSortTile;
+    const int blocks = static_cast<int>((n + 255) / 256);
+    rt::DevBuf lo, hi, bounds, c, idx, c_t, idx_t, hist, sums, parts;
+    hipError_t e = hipSuccess;
+    if (e == hipSuccess) e = rt::dalloc_in<float4>(ar, lo, n);
+    if (e == hipSuccess) e = rt::dalloc_in<float4>(ar, hi, n);
+    if (e == hipSuccess) e = rt::dalloc_in<uint32_t>(ar, bounds, 12);
+    if (e == hipSuccess) e = rt::dalloc_in<uint32_t>(ar, c, n);
+    if (e == hipSuccess) e = rt::dalloc_in<uint32_t>(ar, idx, n);
+    if (e == hipSuccess) e = rt::dalloc_in<uint32_t>(ar, c_t, n);
+    if (e == hipSuccess) e = rt::dalloc_in<uint32_t>(ar, idx_t, n);
+    if (e == hipSuccess) e = rt::dalloc_in<uint32_t>(ar, hist, 256 * tiles);
+    if (e == hipSuccess) e = rt::dalloc_in<uint32_t>(ar, sums, rt::kScanTile);
+    if (e == hipSuccess) e = rt::dalloc_in<uint32_t>(ar, parts, 12 * static_cast<int64_t>(blocks));
+    if (e != hipSuccess) return rt::fail(ctx, RT_ERR_OOM, std::string("rt_build_morton: ") + hipGetErrorString(e));
+    hipLaunchKernelGGL(rt::k_tri_bounds, dim3(blocks), dim3(256), 0, s, scene->d_abc, n, (float4 *)lo.p,
+                       (float4 *)hi.p, (uint32_t *)parts.p);
+    hipLaunchKernelGGL(rt::k_bounds_final, dim3(1), dim3(256), 0, s, (const uint32_t *)parts.p, blocks,
+                       (uint32_t *)bounds.p);
+    hipLaunchKernelGGL(rt::k_morton, dim3(blocks), dim3(256), 0, s, (const float4 *)lo.p, (const float4 *)hi.p, n,
+                       (const uint32_t *)bounds.p, (uint32_t *)c.p, (uint32_t *)idx.p);
+    RT_HIP_CHECK(ctx, hipGetLastError());
+    RT_HIP_CHECK(ctx, hipMemcpyAsync(codes, c.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    RT_HIP_CHECK(ctx, rt::radix_sort_pairs((uint32_t *)c.p, (uint32_t *)idx.p, (uint32_t *)c_t.p, (uint32_t *)idx_t.p,
+                                           n, 30, (uint32_t *)hist.p, (uint32_t *)sums.p, s));
+    RT_HIP_CHECK(ctx, hipMemcpyAsync(order, idx.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    RT_HIP_CHECK(ctx, hipStreamSynchronize(s));
+    return RT_OK;
+}
+
+int32_t rt_accel_download(const rt_accel *accel, void *nodes, int32_t *tri_ids) {
+    if (!accel || (accel->nodes > 0 && !nodes) || (accel->scene->n > 0 && !tri_ids)) return RT_ERR_INVALID;
+    rt_ctx *ctx = accel->scene->ctx;
+    RT_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    const int64_t node_bytes = (accel->width == 4) ? sizeof(rt::WideNode<4>) : sizeof(rt::WideNode<8>);
+    if (accel->nodes > 0)
+        RT_HIP_CHECK(ctx, hipMemcpyAsync(nodes, accel->d_nodes, accel->nodes * node_bytes, hipMemcpyDeviceToHost, s));
+    const int64_t n = accel->scene->n;
+    std::vector<rt::LeafTri> tris(accel->d_tris ? n : 0);
+    if (!tris.empty())
+        RT_HIP_CHECK(ctx, hipMemcpyAsync(tris.data(), accel->d_tris, n * sizeof(rt::LeafTri), hipMemcpyDeviceToHost, s));
+    RT_HIP_CHECK(ctx, hipStreamSynchronize(s));
+    // The emplace id is the w word of each leaf triangle's first float4.
+    for (size_t i = 0; i < tris.size(); ++i) {
+        const float w = tris[i].a_id.w;
+        std::memcpy(&tri_ids[i], &w, sizeof(int32_t));
+    }
+    return RT_OK;
+}
+
+int32_t rt_accel_root(const rt_accel *accel, int32_t *root, float *box_pad) {
+    if (!accel || !root) return RT_ERR_INVALID;
+    *root = accel->root;
+    if (box_pad) *box_pad = accel->box_pad;
+    return RT_OK;
+}
+
+}  // extern "C"

@@ -36,7 +36,7 @@ EXPORTS = [
     "rt_scene_set_materials", "rt_image_tone_map", "rt_image_gamma", "rt_image_clip", "rt_write_png",
     "rt_write_tga", "rt_scene_set_vertex_normals", "rt_mesh_load", "rt_mesh_parse", "rt_mesh_info_get",
     "rt_mesh_copy", "rt_mesh_last_error", "rt_mesh_destroy", "rt_scene_create_from_mesh", "rt_write_jpeg",
-    "rt_mesh_camera", "rt_mesh_lights",
+    "rt_mesh_camera", "rt_mesh_lights", "rt_build_morton", "rt_accel_download", "rt_accel_root",
 ]
 
 
@@ -131,6 +131,9 @@ def load_library(path=LIB_PATH):
                             fp, fp, fp, C.POINTER(rt_stats)]),
         "rt_render_static": (i32, [vp, C.POINTER(rt_camera), C.POINTER(rt_light), i32, C.POINTER(rt_render_params),
                                    fp, fp, fp, C.POINTER(rt_stats)]),
+        "rt_build_morton": (i32, [vp, fp, fp]),
+        "rt_accel_download": (i32, [vp, fp, fp]),
+        "rt_accel_root": (i32, [vp, C.POINTER(C.c_int32), C.POINTER(C.c_float)]),
         "rt_set_counting": (i32, [vp, i32]),
         "rt_batch_create": (i32, [vp, C.POINTER(rt_batch_item), i32, C.POINTER(rt_batch_params), C.POINTER(vp)]),
         "rt_batch_run": (i32, [vp, i32, C.POINTER(C.c_float)]),
@@ -404,6 +407,14 @@ class Scene:
         self.ctx.check(_LIB.rt_scene_normals(self.h, _ptr(out)), "rt_scene_normals")
         return out
 
+    def morton(self):
+        """The build's Morton stages: (codes in emplace order, emplace ids in
+        stable code order), both uint32."""
+        codes = np.empty(self.n, np.uint32)
+        order = np.empty(self.n, np.uint32)
+        self.ctx.check(_LIB.rt_build_morton(self.h, _ptr(codes), _ptr(order)), "rt_build_morton")
+        return codes, order
+
     def close(self):
         if self.h:
             _LIB.rt_scene_destroy(self.h)
@@ -544,6 +555,22 @@ class Accel:
     def refit(self):
         """Fit the boxes to the scene's current vertices, keeping the tree."""
         self.ctx.check(_LIB.rt_accel_refit(self.h), "rt_accel_refit")
+
+    def download(self):
+        """The tree on the host: dict(width, root, box_pad, lo (nodes, 3, W),
+        hi (nodes, 3, W), child (nodes, W) int32, tri_ids (n,) int32)."""
+        info = self.info()
+        W, nn = info["width"], info["nodes"]
+        words = 32 if W == 4 else 64
+        raw = np.zeros((max(nn, 1), words), np.float32)
+        ids = np.empty(self.scene.n, np.int32)
+        self.ctx.check(_LIB.rt_accel_download(self.h, _ptr(raw), _ptr(ids)), "rt_accel_download")
+        root, pad = C.c_int32(), C.c_float()
+        self.ctx.check(_LIB.rt_accel_root(self.h, C.byref(root), C.byref(pad)), "rt_accel_root")
+        raw = raw[:nn]
+        box = raw[:, :6 * W].reshape(nn, 3, 2, W)  # axis, (lo, hi), child
+        return dict(width=W, root=root.value, box_pad=pad.value, lo=box[:, :, 0, :], hi=box[:, :, 1, :],
+                    child=raw[:, 6 * W:7 * W].view(np.int32), tri_ids=ids)
 
     def info(self):
         i = rt_accel_info()
