@@ -775,7 +775,11 @@ void render_tile_packet(const FrameCtx &f, int tx, int ty, LocalCounters *lc) {
     Colour direct[kTileRays];
     shade_batch(f, sh, kd, nh, direct, lc, 1);
     Colour col[kTileRays];
-    for (int k = 0; k < kTileRays; ++k) col[k] = Colour{};
+    int bounce_id[kTileRays];  // bounce hit of each ray, -1: none
+    for (int k = 0; k < kTileRays; ++k) {
+        col[k] = Colour{};
+        bounce_id[k] = -1;
+    }
     for (int j = 0; j < nh; ++j) col[hit_ray[j]] = direct[j];
     if (f.p.bounce) {
         // Bounce rays of the hit rays, packed in order and traced through
@@ -807,6 +811,7 @@ void render_tile_packet(const FrameCtx &f, int tx, int ty, LocalCounters *lc) {
         int nb = 0;
         for (int j = 0; j < nh; ++j) {
             if (!(bh[j].d < kMaxDist)) continue;
+            bounce_id[hit_ray[j]] = bid[j];
             bsh[nb] = surface_at(f.s->tris[bid[j]], &br[j], bh[j].d);
             bkd[nb] = kd_of(f, bid[j]);
             bsrc[nb] = j;
@@ -827,6 +832,7 @@ void render_tile_packet(const FrameCtx &f, int tx, int ty, LocalCounters *lc) {
         if (px >= cam.x_res || py >= cam.y_res) continue;
         const int64_t p = static_cast<int64_t>(py) * cam.x_res + px;
         if (f.out.prim) f.out.prim[p] = ids[k];
+        if (f.out.bounce_prim) f.out.bounce_prim[p] = f.p.bounce ? bounce_id[k] : -1;
         if (f.out.tuv) {
             f.out.tuv[3 * p + 0] = hits[k].d;
             f.out.tuv[3 * p + 1] = hits[k].u;
