@@ -286,8 +286,8 @@ int32_t rt_scene_create_from_mesh(rt_ctx *ctx, const rt_mesh *mesh, rt_scene **o
  * with RT_DEVICE_PTRS, on the device. ms (may be NULL) receives the device
  * time of the call's kernels.
  *
- * rt_image_tone_map replaces camera::tone_map (rt/camera.cc:1124-1461) with
- * glare off: mode is tone_mapping_mode_t (rt/camera.h:43-45): 0 none,
+ * rt_image_tone_map replaces camera::tone_map (rt/camera.cc:1124-1461):
+ * mode is tone_mapping_mode_t (rt/camera.h:43-45): 0 none,
  * 1 global_contrast, 4 global_non_linear, 5 global_exposure,
  * 6 global_avg_luminance, 7 global_max_luminance, 8 global_bilateral_filter
  * (camera::bilateral_filter_tone_map, rt/camera.cc:1632-1775: a bilateral
@@ -300,7 +300,17 @@ int32_t rt_scene_create_from_mesh(rt_ctx *ctx, const rt_mesh *mesh, rt_scene **o
  * are its arguments; adaptation_level is the camera's adaptation state (read
  * when adapting, always written), log_avg_luminance receives the adapted
  * log-average luminance. The log-average is summed in double (the
- * reference sums in float, pixel by pixel). */
+ * reference sums in float, pixel by pixel).
+ * glare (gf) adds camera::perform_glare_filter (rt/camera.cc:1504-1630) after
+ * the mapping: the pixels with a channel above 255 convolved circularly with
+ * the glare filter of the light level of the adapted luminance (scotopic
+ * below 10^-2, mesopic below 10^0.6, photopic above; rt/camera.cc:145-462:
+ * 2 x 200 random flare lines and a bloom, spread by Spencer's functions,
+ * scaled to 1e-4 per pixel), divided by 255 and added. It needs x_angle and
+ * a frame of at most 4096 x 4096. The filter is drawn from the reference's
+ * generator as first used in a process (MT19937 seeded 5489) and kept on the
+ * context per light level; the convolution is exact through zero-padded FFTs
+ * in single precision (the reference: fftw3f). */
 typedef struct {
     int32_t mode;
     float key, xw, yw;
@@ -309,11 +319,19 @@ typedef struct {
     float adaptation_level;
     float log_avg_luminance;
     /* The camera's field of view in radians, 2 w / t and 2 h / t (modes 2
-     * and 3 average the frame to one sample per degree of view). */
+     * and 3 average the frame to one sample per degree of view; glare uses
+     * x_angle for the spreading functions' degrees per pixel). */
     float x_angle, y_angle;
+    int32_t glare;
 } rt_tone_params;
 int32_t rt_image_tone_map(rt_ctx *ctx, float *rgb, int32_t x_res, int32_t y_res, rt_tone_params *params,
                           uint32_t flags, float *ms);
+/* The glare filter of an x_res x y_res camera with field of view x_angle
+ * (2 w / t) at light level 0 scotopic, 1 mesopic, 2 photopic: x_res * y_res
+ * rgb floats into out (host, or device with RT_DEVICE_PTRS), pixel (x, y) at
+ * 3 * (x + y * x_res). camera::generate_glare_filter (rt/camera.cc:325-462). */
+int32_t rt_glare_filter(rt_ctx *ctx, int32_t x_res, int32_t y_res, float x_angle, int32_t level, float *out,
+                        uint32_t flags);
 /* camera::gamma_correct (rt/camera.cc:1466-1501) of npix pixels, in place. */
 int32_t rt_image_gamma(rt_ctx *ctx, float *rgb, int64_t npix, float gamma, uint32_t flags, float *ms);
 /* clip_image_to_rgb (bgr = 0) or clip_image_to_bgr (bgr = 1)

@@ -112,6 +112,9 @@ def _load():
         "oracle_gamma": (None, [vp, i64, f]),
         "oracle_gauss_circular3d": (None, [vp, i32, i32, i32, f, f]),
         "oracle_clip": (None, [vp, i32, i32, i32, i32, i32, vp]),
+        "oracle_mt_stream": (None, [i32, vp]),
+        "oracle_glare_filter": (None, [i32, i32, f, i32, vp]),
+        "oracle_glare": (i32, [vp, i32, i32, f, f, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -406,18 +409,44 @@ def refract_dir(d, n, ri, leaving):
 
 
 def tone_map(rgb, mode=0, key=0.18, xw=1.0 / 3.0, yw=1.0 / 3.0, desaturate=False, stretch=False, adapt=False,
-             time_step=0.0, adaptation_level=0.0, x_angle=0.0, y_angle=0.0):
+             time_step=0.0, adaptation_level=0.0, x_angle=0.0, y_angle=0.0, glare=False):
     """camera::tone_map on a copy of rgb ((H, W, 3) float32, 0-255); returns
-    (image, adaptation level, adapted log-average luminance)."""
+    (image, adaptation level, adapted log-average luminance). glare needs
+    x_angle (2 w / t)."""
     img = np.array(rgb, np.float32, order="C", copy=True)
     h, w = img.shape[0], img.shape[1]
-    tp = oracle_tone_params(int(mode), key, xw, yw, int(desaturate), int(stretch), int(adapt), 0, time_step, x_angle,
-                            y_angle)
+    tp = oracle_tone_params(int(mode), key, xw, yw, int(desaturate), int(stretch), int(adapt), int(glare), time_step,
+                            x_angle, y_angle)
     lvl = np.array([adaptation_level], np.float32)
     yi = np.zeros(1, np.float32)
     if LIB.oracle_tone_map(_p(img), w, h, C.byref(tp), _p(lvl), _p(yi)) != 0:
         raise ValueError(f"tone mapping mode {mode} is not restated")
     return img, float(lvl[0]), float(yi[0])
+
+
+def mt_stream(count):
+    """The reference's random generator (common/common.cc:30-105): the first
+    count tempered 32-bit words after its seeding with 5489."""
+    out = np.empty(count, np.uint32)
+    LIB.oracle_mt_stream(count, _p(out))
+    return out
+
+
+def glare_filter(x_res, y_res, x_angle, level):
+    """The glare filter (y_res, x_res, 3) float32 of a camera with field of
+    view x_angle = 2 w / t at light level 0 scotopic, 1 mesopic, 2 photopic."""
+    out = np.empty((y_res, x_res, 3), np.float32)
+    LIB.oracle_glare_filter(x_res, y_res, x_angle, level, _p(out))
+    return out
+
+
+def glare(rgb, yi, x_angle):
+    """perform_glare_filter on a copy of a tone-mapped image; returns (image, light level)."""
+    img = np.array(rgb, np.float32, order="C", copy=True)
+    lvl = np.zeros(1, np.int32)
+    if LIB.oracle_glare(_p(img), img.shape[1], img.shape[0], yi, x_angle, _p(lvl)) != 0:
+        raise ValueError("glare needs a positive x_angle")
+    return img, int(lvl[0])
 
 
 def gauss_circular3d(grid, ss=1.0, sr=1.0):

@@ -298,7 +298,8 @@ extern "C" int32_t oracle_tone_map(float *img, int32_t x_res, int32_t y_res, con
     static const float zb[3] = {0.0193f, 0.1192f, 0.9505f};
     static const float vb[3] = {0.01477f, 0.497f, 0.631f};
     const int mode = tp->mode;
-    if (mode < 0 || mode > 9 || tp->glare) return -1;
+    if (mode < 0 || mode > 9) return -1;
+    if (tp->glare && !(tp->x_angle > 0.0f)) return -3;
     if ((mode == 2 || mode == 3) && !(tp->x_angle * (360.0f / (2.0f * 3.1415926535897932384626433f)) >= 1.0f &&
                                       tp->y_angle * (360.0f / (2.0f * 3.1415926535897932384626433f)) >= 1.0f))
         return -2;
@@ -473,6 +474,8 @@ extern "C" int32_t oracle_tone_map(float *img, int32_t x_res, int32_t y_res, con
         }
     }
     for (int i = 0; i < n; ++i) yxy_to_rgb(img + 3 * static_cast<int64_t>(i));
+    // rt/camera.cc:1455-1458: the glare filter at the adapted luminance.
+    if (tp->glare) oracle_glare(img, x_res, y_res, Yi, tp->x_angle, nullptr);
     return 0;
 }
 

@@ -151,8 +151,8 @@ void oracle_primary_ray(const oracle_camera *cam, int32_t x, int32_t y, float *o
 /* Camera post-process on an image of 0-255 floats, pixel (x, y) at
  * 3 * (x + y * x_res) (postprocess.cpp). tone_map: camera::tone_map
  * (rt/camera.cc:1124-1461) for mode 0 (none) and the global modes 1, 4, 5,
- * 6, 7, 9 of tone_mapping_mode_t (rt/camera.h:43-45); returns -1 for the
- * local modes 2, 3, 8 and for glare. adaptation_level is the camera's
+ * 6, 7, 9 of tone_mapping_mode_t (rt/camera.h:43-45), the local modes 2, 3
+ * and 8, and the glare filter (glare, which needs x_angle). adaptation_level is the camera's
  * adaptation state (read with adapt, always written); yi_out the adapted
  * log-average luminance. gamma: camera::gamma_correct
  * (rt/camera.cc:1466-1501). clip: clip_image_to_rgb / _bgr
@@ -173,6 +173,16 @@ void oracle_gamma(float *img, int64_t npix, float gamma);
  * of both channels of a W x H x D grid, in place. */
 void oracle_gauss_circular3d(float *grid, int32_t W, int32_t H, int32_t D, float ss, float sr);
 void oracle_clip(const float *img, int32_t x_res, int32_t y_res, int32_t xs, int32_t ys, int32_t bgr, uint8_t *out);
+
+/* The glare filter (glare.cpp): the reference's MT19937 stream (raw tempered
+ * words), the filter of an x_res x y_res camera with field of view x_angle
+ * = 2 w / t at light level 0 scotopic, 1 mesopic, 2 photopic (x_res * y_res
+ * rgb floats), and perform_glare_filter on a tone-mapped image at adapted
+ * luminance yi (level_out: the light level used). tone_map applies it when
+ * glare is set. */
+void oracle_mt_stream(int32_t count, uint32_t *out);
+void oracle_glare_filter(int32_t x_res, int32_t y_res, float x_angle, int32_t level, float *out);
+int32_t oracle_glare(float *img, int32_t x_res, int32_t y_res, float yi, float x_angle, int32_t *level_out);
 
 #ifdef __cplusplus
 }

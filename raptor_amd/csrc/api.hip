@@ -119,6 +119,7 @@ void release_ctx(rt_ctx *ctx) {
     if (ctx->side) (void)hipStreamSynchronize(ctx->side);
     for (auto &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
+    rt::release_glare(ctx);
     if (ctx->fork) (void)hipEventDestroy(ctx->fork);
     if (ctx->join) (void)hipEventDestroy(ctx->join);
     if (ctx->d_counters) (void)hipFree(ctx->d_counters);

@@ -913,6 +913,10 @@ int32_t image_tone_map(rt_ctx *ctx, float *rgb, int32_t x_res, int32_t y_res, rt
                     "rt_image_tone_map: the histogram modes need the camera's field of view (x_angle, y_angle) of "
                     "at least one degree");
     if (m < 0 || m > 9) return fail(ctx, RT_ERR_INVALID, "rt_image_tone_map: unknown mode " + std::to_string(m));
+    if (tp->glare && !(tp->x_angle > 0.0f && x_res <= 4096 && y_res <= 4096))
+        return fail(ctx, RT_ERR_INVALID,
+                    "rt_image_tone_map: glare needs the camera's field of view (x_angle > 0) and a frame of at most "
+                    "4096 x 4096 pixels");
     const int64_t n = static_cast<int64_t>(x_res) * y_res;
     BuildArena &ar = ctx->arena;
     RT_HIP_CHECK(ctx, ar.begin());
@@ -965,9 +969,15 @@ int32_t image_tone_map(rt_ctx *ctx, float *rgb, int32_t x_res, int32_t y_res, rt
         hipLaunchKernelGGL(k_tone_stretch, dim3(blocks), dim3(kPostThreads), 0, s, img, n, scal);
     }
     RT_HIP_CHECK(ctx, hipGetLastError());
-    if (ms) RT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[1], s));
     ToneScalars t{};
     RT_HIP_CHECK(ctx, hipMemcpyAsync(&t, scal, sizeof(t), hipMemcpyDeviceToHost, s));
+    if (tp->glare) {
+        // rt/camera.cc:1455-1458: the glare filter at the adapted luminance,
+        // whose light level picks the filter.
+        RT_HIP_CHECK(ctx, hipStreamSynchronize(s));
+        if (int32_t rc = apply_glare(ctx, img, x_res, y_res, t.yi, tp->x_angle)) return rc;
+    }
+    if (ms) RT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[1], s));
     if (!dev) RT_HIP_CHECK(ctx, hipMemcpyAsync(rgb, img, 3 * n * sizeof(float), hipMemcpyDeviceToHost, s));
     RT_HIP_CHECK(ctx, hipStreamSynchronize(s));
     tp->adaptation_level = t.level;

@@ -52,6 +52,8 @@ struct alignas(16) LeafTri {
 
 namespace rt {
 
+struct GlareCache;  // glare.hip: filters and their spectra, per light level
+
 // Scratch memory of BVH builds, kept on the context between builds: device
 // chunks handed out by a bump pointer. A build that outgrows the chunks takes
 // another; the next build merges them into one chunk of the high-water mark,
@@ -82,6 +84,7 @@ struct rt_ctx {
     unsigned long long *d_counters = nullptr;
     uint32_t *d_queue = nullptr;  // chunk counters of the persistent trace kernels (kQueueWords)
     void *d_static = nullptr;     // this device's static scratch (rt_render_static), not owned
+    rt::GlareCache *glare = nullptr;  // glare filters (glare.hip)
     rt::BuildArena arena;         // build scratch (bvh_build.hip)
     int count_slot = 0;
     std::atomic<int> refs{1};  // the caller's handle + one per scene
@@ -192,6 +195,11 @@ int32_t static_scratch_address(rt_ctx *ctx, void **ptr);
 int32_t image_tone_map(rt_ctx *ctx, float *rgb, int32_t x_res, int32_t y_res, rt_tone_params *tp, uint32_t flags,
                        float *ms);
 int32_t image_gamma(rt_ctx *ctx, float *rgb, int64_t npix, float gamma, uint32_t flags, float *ms);
+// Glare filter (glare.hip): perform_glare_filter on a tone-mapped device
+// image at adapted luminance yi; the cached filter of a light level; release.
+int32_t apply_glare(rt_ctx *ctx, float *img, int x_res, int y_res, float yi, float x_angle);
+int32_t glare_filter(rt_ctx *ctx, int x_res, int y_res, float x_angle, int level, const float **gf);
+void release_glare(rt_ctx *ctx);
 int32_t image_clip(rt_ctx *ctx, const float *rgb, int32_t x_res, int32_t y_res, int32_t xs, int32_t ys, int32_t bgr,
                    uint8_t *out, uint32_t flags, float *ms);
 int32_t write_png(const char *path, const uint8_t *rgb, int32_t x_res, int32_t y_res);

@@ -36,7 +36,7 @@ EXPORTS = [
     "rt_scene_set_materials", "rt_image_tone_map", "rt_image_gamma", "rt_image_clip", "rt_write_png",
     "rt_write_tga", "rt_scene_set_vertex_normals", "rt_mesh_load", "rt_mesh_parse", "rt_mesh_info_get",
     "rt_mesh_copy", "rt_mesh_last_error", "rt_mesh_destroy", "rt_scene_create_from_mesh", "rt_write_jpeg",
-    "rt_mesh_camera", "rt_mesh_lights", "rt_build_morton", "rt_accel_download", "rt_accel_root",
+    "rt_mesh_camera", "rt_mesh_lights", "rt_build_morton", "rt_accel_download", "rt_accel_root", "rt_glare_filter",
 ]
 
 
@@ -86,7 +86,7 @@ class rt_tone_params(C.Structure):
     _fields_ = [("mode", C.c_int32), ("key", C.c_float), ("xw", C.c_float), ("yw", C.c_float),
                 ("desaturate", C.c_int32), ("stretch", C.c_int32), ("adapt", C.c_int32), ("time_step", C.c_float),
                 ("adaptation_level", C.c_float), ("log_avg_luminance", C.c_float), ("x_angle", C.c_float),
-                ("y_angle", C.c_float)]
+                ("y_angle", C.c_float), ("glare", C.c_int32)]
 
 
 class rt_accel_info(C.Structure):
@@ -132,6 +132,7 @@ def load_library(path=LIB_PATH):
         "rt_render_static": (i32, [vp, C.POINTER(rt_camera), C.POINTER(rt_light), i32, C.POINTER(rt_render_params),
                                    fp, fp, fp, C.POINTER(rt_stats)]),
         "rt_build_morton": (i32, [vp, fp, fp]),
+        "rt_glare_filter": (i32, [vp, i32, i32, C.c_float, i32, fp, u32]),
         "rt_accel_download": (i32, [vp, fp, fp]),
         "rt_accel_root": (i32, [vp, C.POINTER(C.c_int32), C.POINTER(C.c_float)]),
         "rt_set_counting": (i32, [vp, i32]),
@@ -784,21 +785,29 @@ def _image_arg(img):
 
 
 def tone_map(ctx, img, mode=TONE_NONE, key=0.18, xw=1.0 / 3.0, yw=1.0 / 3.0, desaturate=False, stretch=False,
-             adapt=False, time_step=0.0, adaptation_level=0.0, x_angle=0.0, y_angle=0.0):
-    """camera::tone_map (glare off). Returns (image, adaptation level,
-    log-average luminance, device ms); a numpy image is returned as a new
-    array, a device tensor is changed in place and returned. x_angle,
-    y_angle: the camera's field of view in radians (2 w / t, 2 h / t), for
-    the histogram modes."""
+             adapt=False, time_step=0.0, adaptation_level=0.0, x_angle=0.0, y_angle=0.0, glare=False):
+    """camera::tone_map. Returns (image, adaptation level, log-average
+    luminance, device ms); a numpy image is returned as a new array, a
+    device tensor is changed in place and returned. x_angle, y_angle: the
+    camera's field of view in radians (2 w / t, 2 h / t), for the histogram
+    modes and the glare filter (glare=True)."""
     a, flags = _image_arg(img)
     if flags == 0:
         a = a.copy()
     h, w = a.shape[0], a.shape[1]
     tp = rt_tone_params(int(mode), key, xw, yw, int(desaturate), int(stretch), int(adapt), time_step,
-                        adaptation_level, 0.0, x_angle, y_angle)
+                        adaptation_level, 0.0, x_angle, y_angle, int(glare))
     ms = C.c_float()
     ctx.check(_LIB.rt_image_tone_map(ctx.h, _ptr(a), w, h, C.byref(tp), flags, C.byref(ms)), "rt_image_tone_map")
     return a, tp.adaptation_level, tp.log_avg_luminance, ms.value
+
+
+def glare_filter(ctx, x_res, y_res, x_angle, level):
+    """The glare filter (y_res, x_res, 3) float32 of a camera with field of
+    view x_angle = 2 w / t at light level 0 scotopic, 1 mesopic, 2 photopic."""
+    out = np.empty((y_res, x_res, 3), np.float32)
+    ctx.check(_LIB.rt_glare_filter(ctx.h, x_res, y_res, x_angle, level, _ptr(out), 0), "rt_glare_filter")
+    return out
 
 
 def gamma_correct(ctx, img, gamma):
