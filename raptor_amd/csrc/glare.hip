@@ -214,30 +214,60 @@ __global__ void __launch_bounds__(1024) k_glare_scan(const int *__restrict__ cnt
     }
 }
 
-// The float running sum of v[i * stride + c], i = 0 .. n-1, for each channel
-// c < NCH, in order, as one loop adds them: one wave loads 64 entries at a
-// time and adds them one lane after another. The values are non-negative, so
-// the lanes past n may hold +0.
+// The float running sum of v[i * NCH + c], i = 0 .. n-1, for each channel
+// c < NCH, in order, as one loop adds them. The adds are one dependent chain
+// per channel, so wave 0 does them all; the block's four waves stream the
+// next chunk of kSeqChunk entries into the other half of an LDS double
+// buffer meanwhile, which hides the HBM latency behind the adds.
+constexpr int kSeqChunk = 2048;
+
 template <int NCH>
-__global__ void __launch_bounds__(64) k_glare_seq_sum(const float *__restrict__ v, int64_t n, int stride,
-                                                      float *__restrict__ out) {
+__global__ void __launch_bounds__(256) k_glare_seq_sum(const float *__restrict__ v, int64_t n,
+                                                       float *__restrict__ out) {
+    __shared__ float buf[2][kSeqChunk * NCH];
+    constexpr int kPer = kSeqChunk * NCH / 256;  // floats each thread moves per chunk
+    const int64_t chunks = (n + kSeqChunk - 1) / kSeqChunk;
+    const int64_t total = n * NCH;
     float s[NCH];
     for (int c = 0; c < NCH; ++c) s[c] = 0.0f;
-    for (int64_t base = 0; base < n; base += 64) {
-        const int64_t i = base + threadIdx.x;
-        float x[NCH];
-        for (int c = 0; c < NCH; ++c) x[c] = (i < n) ? v[i * stride + c] : 0.0f;
-        const int lanes = static_cast<int>(min(static_cast<int64_t>(64), n - base));
-        if (lanes == 64) {
+    float r[kPer];
+    // Chunk 0 into buffer 0.
+    for (int k = 0; k < kPer; ++k) {
+        const int64_t e = static_cast<int64_t>(k) * 256 + threadIdx.x;
+        buf[0][k * 256 + threadIdx.x] = (e < total) ? v[e] : 0.0f;
+    }
+    __syncthreads();
+    for (int64_t ch = 0; ch < chunks; ++ch) {
+        const int cur = static_cast<int>(ch & 1);
+        const bool more = ch + 1 < chunks;
+        if (more) {
+            const int64_t base = (ch + 1) * kSeqChunk * NCH;
 #pragma unroll
-            for (int l = 0; l < 64; ++l)
-                for (int c = 0; c < NCH; ++c)
-                    s[c] += __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x[c]), l));
-        } else {
-            for (int l = 0; l < lanes; ++l)
-                for (int c = 0; c < NCH; ++c)
-                    s[c] += __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x[c]), l));
+            for (int k = 0; k < kPer; ++k) {
+                const int64_t e = base + static_cast<int64_t>(k) * 256 + threadIdx.x;
+                r[k] = (e < total) ? v[e] : 0.0f;
+            }
         }
+        if (threadIdx.x < 64) {
+            const int cnt = static_cast<int>(min(static_cast<int64_t>(kSeqChunk), n - ch * kSeqChunk));
+            const float *b = buf[cur];
+            int j = 0;
+            for (; j + 8 <= cnt; j += 8) {
+                float x[8 * NCH];
+#pragma unroll
+                for (int k = 0; k < 8 * NCH; ++k) x[k] = b[j * NCH + k];
+#pragma unroll
+                for (int k = 0; k < 8; ++k)
+#pragma unroll
+                    for (int c = 0; c < NCH; ++c) s[c] += x[k * NCH + c];
+            }
+            for (; j < cnt; ++j)
+                for (int c = 0; c < NCH; ++c) s[c] += b[j * NCH + c];
+        }
+        if (more)
+#pragma unroll
+            for (int k = 0; k < kPer; ++k) buf[cur ^ 1][k * 256 + threadIdx.x] = r[k];
+        __syncthreads();
     }
     if (threadIdx.x == 0)
         for (int c = 0; c < NCH; ++c) out[c] = s[c];
@@ -601,11 +631,11 @@ int32_t glare_filter(rt_ctx *ctx, int xr, int yr, float x_angle, int level, cons
         if (e != hipSuccess) return bail(e);
         hipLaunchKernelGGL(k_glare_line_rows<true>, slot_grid, dim3(kThreads), 0, s, d_lines, xr, yr, d_cnt, d_off,
                            d_vals);
-        hipLaunchKernelGGL(k_glare_seq_sum<1>, dim3(1), dim3(64), 0, s, d_vals, nvals, 1, d_sums);
+        hipLaunchKernelGGL(k_glare_seq_sum<1>, dim3(1), dim3(256), 0, s, d_vals, nvals, d_sums);
         const float pixdeg = (std::atan(x_angle / 2.0f) * 2.0f) / static_cast<float>(static_cast<unsigned>(xr));
         hipLaunchKernelGGL(k_glare_psf, dim3(blocks), dim3(kThreads), 0, s, d_img, d_sums, xr, yr, pixdeg, level,
                            f.gf);
-        hipLaunchKernelGGL(k_glare_seq_sum<3>, dim3(1), dim3(64), 0, s, f.gf, n, 3, d_sums + 4);
+        hipLaunchKernelGGL(k_glare_seq_sum<3>, dim3(1), dim3(256), 0, s, f.gf, n, d_sums + 4);
         const float scat = 0.0001f * static_cast<float>(static_cast<unsigned>(xr)) *
                            static_cast<float>(static_cast<unsigned>(yr));
         hipLaunchKernelGGL(k_glare_scale, dim3(static_cast<unsigned>((3 * n + kThreads - 1) / kThreads)),

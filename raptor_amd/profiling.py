@@ -90,8 +90,14 @@ def cmd_summarize(args):
               f"(copy of {copy_bytes} B read {cf:.1f} KiB)",
               f"* WRITE_SIZE {write:.1f} KiB x calibration {write_scale:.4f} "
               f"(copy of {copy_bytes} B wrote {cw:.1f} KiB)",
-              f"* traffic = {traffic / 1e6:.2f} MB per launch; algorithmic bytes "
-              f"{bench['roofline']['algorithmic_bytes_per_launch'] / 1e6:.2f} MB per launch"]
+              f"* traffic = {traffic / 1e6:.2f} MB per launch over {bench['roofline']['launch_ms'] * 1e3:.1f} us "
+              f"= {traffic / (bench['roofline']['launch_ms'] * 1e-3) / 1e9:.0f} GB/s, "
+              f"{traffic / (bench['roofline']['launch_ms'] * 1e-3) / 1e9 / bench['roofline']['peak']:.3f} of "
+              f"{bench['roofline']['peak']:.0f} GB/s",
+              f"* per-visit demand model (every node and triangle fetch priced as HBM bytes, no cache reuse): "
+              f"{bench['roofline']['visit_model']['bytes_per_launch'] / 1e6:.2f} MB per launch",
+              f"* compulsory bytes (BVH + triangles read once, rays and hits in and out): "
+              f"{bench['roofline']['compulsory']['bytes_per_launch'] / 1e6:.2f} MB per launch"]
     with open(args.out + ".md", "w") as f:
         f.write("\n".join(lines) + "\n")
     out = {"kernel": kernel, "kernel_stage": stage, "width": bench["config"]["bvh_width"],
