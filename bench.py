@@ -95,14 +95,21 @@ def cpu_model():
     return "unknown"
 
 
-def cpu_baseline(w, target_s, gpu_hits=None):
+def cpu_baseline(w, target_s, gpu_hits=None, gpu_frame_ms=None):
     """The reference's own CPU program for this frame, restated in the oracle:
     the SSE packet tracer (SIMD_PACKET_TRACING: 8x8 tiles of 16 four-ray
     packets, packet traversal and rcp-based Moller-Trumbore, shadow rays
     packed four at a time) over the reference's AAC BVH, (b) on all host
     cores with a tile work queue for about target_s seconds of whole frames
     and (a) on one core over a sample of tiles. The scalar IEEE tracer (the
-    parity oracle) is timed beside it for one frame on all cores."""
+    parity oracle) is timed beside it for one frame on all cores; its
+    traversal counters price the frame with the per-ray bytes 32 (ray) + 16
+    (closest hit) or 1 (any hit) + 32 per child-box test + 36 per triangle
+    test on the reference's tree, which against the GPU frame time is the
+    reference-tree demand model (it exceeds HBM peak: most of those reads
+    are cache hits, so it is no roofline)."""
+    import numpy as np
+
     import oracle.oracle as O
 
     cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
@@ -143,6 +150,15 @@ def cpu_baseline(w, target_s, gpu_hits=None):
         "bounce_packet_vs_scalar": int((pk["bounce_prim"] != sc["bounce_prim"]).sum()),
         "pixels": int(pk["prim"].size),
     }
+    cnt = sc["counters"].astype(np.int64)
+    ref_bytes = (32 * int(cnt[:4].sum()) + 16 * int(cnt[0] + cnt[2]) + int(cnt[1] + cnt[3]) + 32 * int(cnt[4])
+                 + 36 * int(cnt[5]))
+    ref_model = {"box_tests_per_ray": float(cnt[4]) / float(cnt[:4].sum()),
+                 "tri_tests_per_ray": float(cnt[5]) / float(cnt[:4].sum()),
+                 "bytes_per_frame": ref_bytes}
+    if gpu_frame_ms:
+        ref_model["gb_s_at_gpu_frame_time"] = round(ref_bytes / (gpu_frame_ms * 1e-3) / 1e9, 1)
+        ref_model["frac_of_hbm_peak"] = round(ref_bytes / (gpu_frame_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 3)
     if gpu_hits is not None:
         disagree["primary_packet_vs_gpu"] = int((pk["prim"] != gpu_hits[0]).sum())
         disagree["bounce_packet_vs_gpu"] = int((pk["bounce_prim"] != gpu_hits[1]).sum())
@@ -163,6 +179,7 @@ def cpu_baseline(w, target_s, gpu_hits=None):
         "scalar_value": srays / sdt / 1e6,
         "scalar_sample": f"one frame of the scalar IEEE tracer on {cores} threads ({srays} rays, {sdt:.1f} s)",
         "id_disagreements": disagree,
+        "reference_tree_model": ref_model,
     }
 
 
@@ -790,7 +807,7 @@ def main():
         line["invrender"] = invrender_section(ctx, args.invrender_steps)
     if not args.no_cpu_baseline:
         print("bench: CPU baseline", file=sys.stderr, flush=True)
-        cb = cpu_baseline(w, args.cpu_seconds, gpu_hits)
+        cb = cpu_baseline(w, args.cpu_seconds, gpu_hits, line["ms_per_step"])
         line["cpu_baseline"] = cb
         line["gpu_over_cpu"] = round(value / cb["value"], 2)
     else:
