@@ -181,18 +181,121 @@ int32_t rt_accel_refit(rt_accel *accel);
  * unused. */
 #define RT_MATERIAL_PHONG 0
 #define RT_MATERIAL_COOK_TORRANCE 1
-/* kind RT_MATERIAL_COLOURED: a coloured_mapper_shader without texture maps
+/* kind RT_MATERIAL_COLOURED: a coloured_mapper_shader
  * (rt/materials/coloured_mapper_shader.{h,cc}), the material an OBJ/MTL file
  * gives: the fields and shading of RT_MATERIAL_PHONG, with the wider ranges
- * its constructor asserts (tran and rf in [0, 1]). */
+ * its constructor asserts (tran and rf in [0, 1]), and optional texture maps
+ * tex[] = 1 + the index of a map of rt_scene_set_textures (0: none) for, in
+ * order, t_ka, t_kd, t_ks, t_ns, t_refl and t_d: a colour map replaces the
+ * colour by the texture times colour / 255 (a checker board leaves the
+ * colour where it does not paint); t_ns makes the shininess
+ * ns / 255 times the map's opacity (the reference multiplies the returned
+ * alpha, not the texel); t_refl and t_d scale the mirror and refraction
+ * rays' coefficients by red / 255 of their map while the colours those rays
+ * bring back stay weighted by rf and tran (rt/materials/coloured_mapper_shader.cc:8-101).
+ * kind RT_MATERIAL_MAPPER: a mapper_shader (rt/materials/mapper_shader.{h,cc}):
+ * kd[] the base colour rgb (0-255), mapper_kd and mapper_ks its kd (>= 0)
+ * and ks (0..1), shininess s, transmittance, refractive_index, reflectance,
+ * clrh (specular colour blend) and clrf (transmitted colour blend), and five
+ * layer lists of maps, layers[layer_first[k] .. + layer_count[k]) of
+ * rt_scene_set_textures, for k = 0 colour, 1 kd, 2 s, 3 reflectance,
+ * 4 transmittance. Layers apply in order and stop after one whose opacity
+ * makes the product 0; a monochrome layer (kd, s, reflectance,
+ * transmittance) sets the value to its texel's mean / 255. No ambient term;
+ * per light: rgb x (n.l kd) x light, plus (clrh rgb + (1 - clrh) light) x
+ * (r.v)^s ks, where r reflects the light about n scaled by 2 (n.l) kd as
+ * the reference writes it; the refracted colour adds as
+ * c += clrf c + (1 - clrf) refracted x transmittance. */
 #define RT_MATERIAL_COLOURED 2
+#define RT_MATERIAL_MAPPER 3
 typedef struct {
     float ka[3], kd[3], ks[3];
     float shininess, transmittance, refractive_index, reflectance;
     int32_t kind;
     float chroma[2];
     float diffuse, roughness, extinction;
+    int32_t tex[6];
+    float mapper_kd, mapper_ks, clrh, clrf;
+    int32_t layer_first[5], layer_count[5];
 } rt_material;
+
+/* ---- Texture maps (rt/materials/texture_mapper.h) ----
+ * A map's final state, as the reference's constructors leave it:
+ *   RT_MAP_PLANAR (planar_mapper), RT_MAP_CUBIC (cubic_mapper),
+ *   RT_MAP_CYLINDRICAL (cylindrical_mapper): an image, centre c, size s,
+ *   texture axes u and v (the first constructors derive u = (n.y + n.z, 0,
+ *   n.x), v = (0, n.x + n.z, n.y), cylindrical v = n), the cubic mapper's
+ *   normal n, the cylindrical mapper's cycles, wrapping modes and texel
+ *   offsets. A hit with texture coordinates samples at (vt.u w, vt.v h);
+ *   otherwise the mapper projects the hit point
+ *   (rt/materials/{planar,cubic,cylindrical}_mapper.cc).
+ *   The lookup is bilinear over texels (u0, v0) .. (u0 + 1, v0 + 1), u0 =
+ *   int(u_co) + u_off wrapped by the mode, the upper neighbour clamped to
+ *   the image (texture_mapper.h image_texture_mapper).
+ *   RT_MAP_CHECKER_BOARD (checker_board_mapper): colour rgb where
+ *   int((x - c.x) / s.x) is odd (the reference's test xors the z cell with
+ *   itself, so only x decides), otherwise the colour is left alone;
+ *   opacity is the map's alpha.
+ * Then, per map, a falloff (mapper_falloff: cubic, spherical, linear x, y,
+ * z around falloff_centre with gradient falloff_grad; RT_FALLOFF_NONE
+ * skips it) and inverse (255 - colour). */
+#define RT_MAP_PLANAR 0
+#define RT_MAP_CUBIC 1
+#define RT_MAP_CYLINDRICAL 2
+#define RT_MAP_CHECKER_BOARD 3
+/* texture_wrapping_mode_t (texture_mapper.h apply_wrapping_mode): blank
+ * (black outside), clamp, tile (negative texels wrap by size - 1), mirror. */
+#define RT_WRAP_BLANK 0
+#define RT_WRAP_CLAMP 1
+#define RT_WRAP_TILE 2
+#define RT_WRAP_MIRROR 3
+#define RT_FALLOFF_CUBIC 0
+#define RT_FALLOFF_SPHERICAL 1
+#define RT_FALLOFF_LINEAR_X 2
+#define RT_FALLOFF_LINEAR_Y 3
+#define RT_FALLOFF_LINEAR_Z 4
+#define RT_FALLOFF_NONE 5
+typedef struct {
+    int32_t kind;
+    int32_t image;              /* image maps: index into the images */
+    float c[3], n[3], s[3];     /* checker board: c centre, s grid size */
+    float u[3], v[3];
+    int32_t u_wrap, v_wrap;
+    int32_t u_off, v_off;
+    int32_t cycles;             /* cylindrical */
+    float rgb[3], opacity;      /* checker board */
+    int32_t inverse;
+    int32_t falloff;            /* RT_FALLOFF_* */
+    float falloff_centre[3], falloff_grad[3];
+} rt_texture_map;
+/* An image: width x height texels of cpp (1, 3 or 4) floats, 0-255, row 0
+ * first (the readers' order: the image's bottom row for TGA/PNG/JPEG),
+ * at texels + offset. */
+typedef struct {
+    int32_t width, height, cpp, reserved;
+    int64_t offset;
+} rt_texture_image;
+/* Gives the scene its maps, images, texels (host arrays) and the layer
+ * lists of RT_MATERIAL_MAPPER materials (map indices). nmaps 0 removes
+ * them. Materials (rt_scene_set_materials) may reference maps only once
+ * the scene has them; calling this again re-checks nothing already set, so
+ * set textures first. Tile and mirror wrapping need images of at least 2
+ * texels along that axis (the reference divides by size - 1). */
+int32_t rt_scene_set_textures(rt_scene *scene, const rt_texture_map *maps, int32_t nmaps,
+                              const rt_texture_image *images, int32_t nimages, const float *texels,
+                              int64_t ntexels, const int32_t *layers, int32_t nlayers);
+/* Per-triangle texture coordinates: vt holds 6 floats per triangle, (u, v)
+ * at vertices a, b, c (host array). A triangle whose first value is FLT_MAX
+ * has none (rt/triangle.h:139-167). A hit interpolates u vt_c + v vt_b +
+ * (1 - (u + v)) vt_a (rt/triangle.h:72-79). vt == NULL removes them. */
+int32_t rt_scene_set_texture_coords(rt_scene *scene, const float *vt);
+/* Reads a TGA image (uncompressed or run-length encoded true-colour or
+ * grey, 8/24/32 bits) as the reference's read_tga does through libtga
+ * (rt/materials/texture_mapper.h read_tga): rows in file order, channels
+ * swapped to RGB, one float 0-255 per channel. dims receives width,
+ * height and channels; texels (capacity floats) may be NULL to query the
+ * dimensions. Host only. */
+int32_t rt_texture_load_tga(const char *path, float *texels, int64_t capacity, int32_t *dims);
 
 /* Gives the scene's triangles materials for rt_render: tri_material[i]
  * indexes materials (host arrays), -1 keeps triangle i's Lambert shading
@@ -233,7 +336,14 @@ int32_t rt_scene_set_vertex_normals(rt_scene *scene, const float *vn);
  * material per triangle (OFF: phong kd 170; PLY: phong kd 200, or the face
  * colour; OBJ: phong kd 200 until a usemtl or mtllib, MTL materials as
  * RT_MATERIAL_COLOURED) and, for OBJ faces with normals, vertex normals.
- * Texture maps are counted, not applied. PLY may be ASCII or binary, big- or
+ * OBJ texture coordinates (vt) and MTL texture maps (map_Ka, map_Kd, map_Ks,
+ * map_refl, map_d: rt/parsers/obj_parser.cc:218-242, 507-561) come with the
+ * mesh when the image is a TGA file: each map statement becomes a planar map
+ * (centre 0, normal (1, 0, 0), size 0, mirror wrapping) sampled through the
+ * triangles' texture coordinates, map_d with the image inverted as the
+ * reference's negative() does (1 - texel); the illumination models drop maps
+ * as they drop terms. Map files in other formats are counted in
+ * texture_maps and not applied. PLY may be ASCII or binary, big- or
  * little-endian. Errors (malformed input, indices out of range) return a
  * negative code with text in rt_mesh_last_error() (per thread). These calls
  * need no GPU. */
@@ -253,10 +363,12 @@ typedef struct {
     int64_t vertices, faces;
     int64_t skipped_faces;       /* degenerate triangles and faces below three vertices */
     int64_t skipped_statements;  /* OBJ/MTL/NFF statements with nothing to shade */
-    int32_t texture_maps;        /* MTL materials whose texture maps are not applied */
+    int32_t texture_maps;        /* MTL map statements whose file cannot be decoded here (not TGA) */
     int32_t lights;              /* point lights the file defines (NFF) */
     int32_t has_camera;          /* 1 when the file defines the camera (NFF) */
-    int32_t reserved;
+    int32_t has_texture_coords;  /* 1 when some triangle has texture coordinates (OBJ vt) */
+    int32_t maps, images;        /* texture maps (OBJ/MTL map_* statements) and their distinct images */
+    int64_t texels;
 } rt_mesh_info;
 /* format: RT_FORMAT_*, or -1 to take it from the file extension. */
 int32_t rt_mesh_load(const char *path, int32_t format, rt_mesh **out);
@@ -269,6 +381,11 @@ int32_t rt_mesh_info_get(const rt_mesh *mesh, rt_mesh_info *info);
  * has_vertex_normals). NULL skips an array. */
 int32_t rt_mesh_copy(const rt_mesh *mesh, float *abc, int32_t *tri_material, rt_material *materials,
                      float *vertex_normals);
+/* Copies out any of: vt (triangles x 6, only when has_texture_coords),
+ * maps (maps), images (images), texels (texels floats); the map indices in
+ * materials' tex[] refer to these maps. NULL skips an array. */
+int32_t rt_mesh_copy_textures(const rt_mesh *mesh, float *vt, rt_texture_map *maps, rt_texture_image *images,
+                              float *texels);
 /* The camera and background colour a file defines (has_camera), and its
  * point lights (info.lights of them). */
 int32_t rt_mesh_camera(const rt_mesh *mesh, rt_camera *cam, float *background);
@@ -276,8 +393,10 @@ int32_t rt_mesh_lights(const rt_mesh *mesh, rt_light *lights);
 const char *rt_mesh_last_error(void);
 void rt_mesh_destroy(rt_mesh *mesh);
 /* A scene from a parsed mesh: rt_scene_create (albedo = kd / 255, flags bit1
- * for transparent materials), rt_scene_set_materials and, when the mesh has
- * them, rt_scene_set_vertex_normals. The mesh may be destroyed afterwards. */
+ * for transparent materials), rt_scene_set_textures (when the mesh has
+ * maps), rt_scene_set_materials and, when the mesh has them,
+ * rt_scene_set_vertex_normals and rt_scene_set_texture_coords. The mesh may
+ * be destroyed afterwards. */
 int32_t rt_scene_create_from_mesh(rt_ctx *ctx, const rt_mesh *mesh, rt_scene **out);
 
 /* ---- Camera post-process: the step after the image buffer ----

@@ -7,9 +7,13 @@ reference code it restates. Float arithmetic runs on numpy float32 scalars,
 one rounding per operation as the reference's float code has (no FMA), so
 the emitted triangles match bit for bit.
 
-Returns per file: abc (n, 9) float32 in emplace order, mats (n, 14) float32
+Returns per file: abc (n, 9) float32 in emplace order, mats (n, 20) float32
 rows of the material each triangle shades with (ka rgb, kd rgb, ks rgb, s,
-tran, ri, rf, kind), vn (n, 9) float32 or None, and counts.
+tran, ri, rf, kind, and the six texture slots t_ka, t_kd, t_ks, t_ns,
+t_refl, t_d as 1 + map index, 0 for none), vn (n, 9) float32 or None, vt
+(n, 6) float32 or None, the texture maps (image index, inverted) in
+creation order, the images ((h, w, cpp) float32, from read_tga), the count
+of map files that are not TGA, and counts.
 """
 import numpy as np
 
@@ -49,9 +53,43 @@ def _zero(a):
     return a[0] == 0 and a[1] == 0 and a[2] == 0
 
 
-def _phong(ka, kd, ks, s=0.0, tran=0.0, ri=1.0, rf=0.0, kind=0):
+def _phong(ka, kd, ks, s=0.0, tran=0.0, ri=1.0, rf=0.0, kind=0, tex=(0, 0, 0, 0, 0, 0)):
     return [F(ka[0]), F(ka[1]), F(ka[2]), F(kd[0]), F(kd[1]), F(kd[2]), F(ks[0]), F(ks[1]), F(ks[2]),
-            F(s), F(tran), F(ri), F(rf), F(kind)]
+            F(s), F(tran), F(ri), F(rf), F(kind)] + [F(t) for t in tex]
+
+
+def read_tga(data):
+    """A TGA file as libtga's TGAReadImage with TGA_RGB reads it
+    (rt/materials/texture_mapper.h read_tga): pixels in file order, BGR(A)
+    swapped to RGB(A). Types 2, 3 (raw) and 10, 11 (run-length encoded).
+    Returns (height, width, channels) float32."""
+    d = np.frombuffer(data, np.uint8)
+    id_len, cmap_type, typ = int(d[0]), int(d[1]), int(d[2])
+    cmap_len, cmap_depth = int(d[5]) | (int(d[6]) << 8), int(d[7])
+    w, h, depth = int(d[12]) | (int(d[13]) << 8), int(d[14]) | (int(d[15]) << 8), int(d[16])
+    cpp = depth // 8
+    at = 18 + id_len + (cmap_len * ((cmap_depth + 7) // 8) if cmap_type == 1 else 0)
+    n = w * h
+    if typ in (2, 3):
+        px = d[at:at + n * cpp].reshape(n, cpp)
+    else:
+        out, p = [], 0
+        while p < n:
+            hdr = int(d[at])
+            at += 1
+            cnt = (hdr & 0x7F) + 1
+            if hdr & 0x80:
+                out.append(np.repeat(d[at:at + cpp][None, :], cnt, axis=0))
+                at += cpp
+            else:
+                out.append(d[at:at + cnt * cpp].reshape(cnt, cpp))
+                at += cnt * cpp
+            p += cnt
+        px = np.concatenate(out)
+    px = px.astype(np.float32)
+    if cpp >= 3:
+        px = px[:, [2, 1, 0] + ([3] if cpp == 4 else [])]
+    return px.reshape(h, w, cpp)
 
 
 def face_to_triangles(p):
@@ -122,27 +160,34 @@ def _in_triangle(p, a, b, c, fn):
 
 class _Out:
     def __init__(self):
-        self.abc, self.mats, self.vn = [], [], []
-        self.any_vn = False
+        self.abc, self.mats, self.vn, self.vt = [], [], [], []
+        self.any_vn = self.any_vt = False
         self.faces = self.skipped = 0
+        self.maps, self.images, self.image_ids, self.unread_maps = [], [], {}, 0
 
-    def add(self, a, b, c, mat, n=None):
+    def add(self, a, b, c, mat, n=None, t=None):
         self.abc.append([*a, *b, *c])
         self.mats.append(mat)
         self.vn.append([*n[0], *n[1], *n[2]] if n is not None else [FLT_MAX] + [0.0] * 8)
         self.any_vn = self.any_vn or n is not None
+        # rt/triangle.h:139-167: a triangle without texture coordinates marks x with MAX_DIST
+        self.vt.append([*t[0], *t[1], *t[2]] if t is not None else [FLT_MAX] + [0.0] * 5)
+        self.any_vt = self.any_vt or t is not None
 
-    def polygon(self, face, mat, normals=None):
+    def polygon(self, face, mat, normals=None, texcoords=None):
         if len(face) < 3:
             self.skipped += 1
             return
         for i, j, k in face_to_triangles(face):
-            self.add(face[i], face[j], face[k], mat, None if not normals else (normals[i], normals[j], normals[k]))
+            self.add(face[i], face[j], face[k], mat, None if not normals else (normals[i], normals[j], normals[k]),
+                     None if not texcoords else (texcoords[i], texcoords[j], texcoords[k]))
 
     def result(self):
         return dict(abc=np.asarray(self.abc, np.float32).reshape(-1, 9),
-                    mats=np.asarray(self.mats, np.float32).reshape(-1, 14),
+                    mats=np.asarray(self.mats, np.float32).reshape(-1, 20),
                     vn=np.asarray(self.vn, np.float32).reshape(-1, 9) if self.any_vn else None,
+                    vt=np.asarray(self.vt, np.float32).reshape(-1, 6) if self.any_vt else None,
+                    maps=list(self.maps), images=list(self.images), unread_maps=self.unread_maps,
                     faces=self.faces, skipped=self.skipped)
 
 
@@ -273,12 +318,30 @@ def parse_ply(data):
     return o.result()
 
 
-def _mtl(text):
-    """rt/parsers/obj_parser.cc:245-666: named materials, illum rules, the
-    ambient halved; coloured_mapper_shader rows (kind 2). Returns (dict,
-    last name)."""
+def _mtl(text, o=None, read_image=None):
+    """rt/parsers/obj_parser.cc:245-666: named materials, illum rules (on
+    colours and texture maps), the ambient halved; coloured_mapper_shader
+    rows (kind 2). Map statements (map_Ka, map_Kd, map_Ks, map_refl, map_d,
+    :507-561) make maps through load_image (:218-242): read_image(name)
+    gives the TGA's bytes (None: a file that is not TGA, counted in
+    o.unread_maps); images are shared per file, map_d's inverted (1 - x,
+    picture_functions' negative). Returns (dict, last name)."""
     out, last = {}, None
     name, st, illum = None, None, 0
+
+    def load(file, invert):
+        if not file.lower().endswith(".tga"):
+            o.unread_maps += 1
+            return 0
+        key = file + ("_inv" if invert else "")
+        if key not in o.image_ids:
+            img = read_tga(read_image(file))
+            if invert:
+                img = F(1) - img
+            o.images.append(img)
+            o.image_ids[key] = len(o.images) - 1
+        o.maps.append(o.image_ids[key])
+        return len(o.maps)
 
     def close():
         nonlocal last
@@ -286,20 +349,25 @@ def _mtl(text):
             return
         ka, kd, ks = st["ka"], st["kd"], st["ks"]
         ns, ri, rf, tr = st["ns"], st["ri"], st["rf"], st["tr"]
+        maps = list(st["maps"])
         z = (F(0), F(0), F(0))
         if illum == 0:
             ka, kd, ks, ns, ri, rf = kd, z, z, F(0), F(1), F(0)
+            maps = [maps[1], None, None, None, None, maps[5]]
         elif illum == 1:
             ks, ns, ri, rf = z, F(0), F(1), F(0)
+            maps[2] = maps[3] = maps[4] = None
         elif illum == 2:
             ri, rf = F(1), F(0)
+            maps[4] = None
         elif illum in (3, 5, 7):
             ri, rf = F(1), F(0.75)
         elif illum == 4:
             rf = F(0)
         tr = max(F(0), min(F(1), tr))
         half = F(0.5)
-        out[name] = _phong(tuple(x * half for x in ka), kd, ks, ns, tr, ri, rf, kind=2)
+        tex = [0 if m is None else load(m, k == 5) for k, m in enumerate(maps)]
+        out[name] = _phong(tuple(x * half for x in ka), kd, ks, ns, tr, ri, rf, kind=2, tex=tex)
         last = name
 
     def colour(t):
@@ -315,7 +383,7 @@ def _mtl(text):
             close()
             name = t[1]
             z = (F(0), F(0), F(0))
-            st = dict(ka=z, kd=z, ks=z, ns=F(0), ri=F(1), rf=F(0), tr=F(0))
+            st = dict(ka=z, kd=z, ks=z, ns=F(0), ri=F(1), rf=F(0), tr=F(0), maps=[None] * 6)
         elif w.startswith("Ka"):
             st["ka"] = colour(t)
         elif w.startswith("Kd"):
@@ -332,16 +400,20 @@ def _mtl(text):
             st["tr"] = F(float(t[1]))
         elif w.startswith("illum"):
             illum = int(t[1])
+        elif w.startswith(("map_Ka", "map_Kd", "map_Ks", "map_refl", "map_d")):
+            slot = {"map_Ka": 0, "map_Kd": 1, "map_Ks": 2}.get(w[:6], 4 if w.startswith("map_refl") else 5)
+            st["maps"][slot] = t[1]
     close()
     return out, last
 
 
-def parse_obj(text, read=None):
+def parse_obj(text, read=None, read_image=None):
     """rt/parsers/obj_parser.cc:762-857 with parse_f_statement (:78-206):
-    v, vn (normalised), vt, f (1-based or negative indices, v/vt/vn),
-    usemtl, mtllib (read(name) gives the MTL text; the last material becomes
-    current); default material phong kd 200."""
-    v, vn = [], []
+    v, vn (normalised), vt (u, v), f (1-based or negative indices,
+    v/vt/vn), usemtl, mtllib (read(name) gives the MTL text, read_image(name)
+    a map file's bytes; the last material becomes current); default material
+    phong kd 200."""
+    v, vn, vt = [], [], []
     named = {}
     default = _phong((0, 0, 0), (200, 200, 200), (0, 0, 0))
     cur = default
@@ -353,7 +425,7 @@ def parse_obj(text, read=None):
         w = t[0]
         if w.startswith("mtllib"):
             for f in t[1:]:
-                mats, last = _mtl(read(f))
+                mats, last = _mtl(read(f), o, read_image)
                 named.update(mats)
                 cur = mats[last] if last is not None else None
         elif w.startswith("usemtl"):
@@ -364,24 +436,27 @@ def parse_obj(text, read=None):
             with np.errstate(all="ignore"):
                 vn.append(_norm(_v(float(t[1]), float(t[2]), float(t[3]))))
         elif w.startswith("vt"):
-            pass
+            vt.append((F(float(t[1])), F(float(t[2]))))
         elif w[0] == "f":
-            face, fn = [], []
+            face, fn, ft = [], [], []
             for c in t[1:]:
                 parts = c.split("/")
                 i = int(parts[0])
                 face.append(v[i - 1 if i > 0 else len(v) + i])
+                if len(parts) >= 2 and parts[1]:
+                    k = int(parts[1])
+                    ft.append(vt[k - 1 if k > 0 else len(vt) + k])
                 if len(parts) == 3:
                     k = int(parts[2])
                     fn.append(vn[k - 1 if k > 0 else len(vn) + k])
             o.faces += 1
             if len(face) == 3:
                 if _obj_tri_ok(*face):
-                    o.add(face[0], face[1], face[2], cur, tuple(fn) if fn else None)
+                    o.add(face[0], face[1], face[2], cur, tuple(fn) if fn else None, tuple(ft) if ft else None)
                 else:
                     o.skipped += 1
             else:
-                o.polygon(face, cur, fn)
+                o.polygon(face, cur, fn, ft)
         elif w[0] == "v":
             v.append(_v(float(t[1]), float(t[2]), float(t[3])))
     return o.result()

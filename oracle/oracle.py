@@ -29,7 +29,18 @@ class oracle_render_params(C.Structure):
     _fields_ = [("bounce", C.c_int32), ("spp", C.c_int32), ("seed", C.c_uint64), ("pixel_stride", C.c_int32),
                 ("nthreads", C.c_int32), ("packet", C.c_int32), ("materials", C.c_void_p),
                 ("nmaterials", C.c_int32), ("tri_material", C.c_void_p), ("vertex_normals", C.c_void_p),
-                ("background", C.c_float * 3)]
+                ("background", C.c_float * 3), ("maps", C.c_void_p), ("images", C.c_void_p),
+                ("texels", C.c_void_p), ("layers", C.c_void_p), ("texture_coords", C.c_void_p)]
+
+
+# oracle_texture_map and oracle_texture_image (rt_oracle.h).
+MAP_DTYPE = np.dtype([("kind", "<i4"), ("image", "<i4"), ("c", "<f4", 3), ("n", "<f4", 3), ("s", "<f4", 3),
+                      ("u", "<f4", 3), ("v", "<f4", 3), ("u_wrap", "<i4"), ("v_wrap", "<i4"), ("u_off", "<i4"),
+                      ("v_off", "<i4"), ("cycles", "<i4"), ("rgb", "<f4", 3), ("opacity", "<f4"),
+                      ("inverse", "<i4"), ("falloff", "<i4"), ("falloff_centre", "<f4", 3),
+                      ("falloff_grad", "<f4", 3)])
+IMAGE_DTYPE = np.dtype([("width", "<i4"), ("height", "<i4"), ("cpp", "<i4"), ("reserved", "<i4"),
+                        ("offset", "<i8")])
 
 
 class oracle_render_out(C.Structure):
@@ -51,20 +62,35 @@ class oracle_tone_params(C.Structure):
 def material_array(materials):
     """materials: list of dicts. Phong (kind 0, the default): ka, kd, ks (rgb
     in [0, 255]), s, tran, ri, rf. Cook-Torrance (kind 1, cook_torrance_cxy):
-    x, y (chroma), rd, rs, ts, sr, ri_r, ri_i. Returns an (n, 19) float32
-    array in the oracle's row layout."""
+    x, y (chroma), rd, rs, ts, sr, ri_r, ri_i. Coloured mapper (kind 2): the
+    phong fields and tex (map indices or None for t_ka, t_kd, t_ks, t_ns,
+    t_refl, t_d). Mapper shader (kind 3): rgb, kd, ks (scalars), s, tran,
+    ri, rf, clrh, clrf, layers ((first, count) x 5). Returns an (n, 39)
+    float32 array in the oracle's row layout."""
     rows = []
     for m in materials:
-        if m.get("kind", 0) == 1:
-            rows.append([0.0] * 10 + [m.get("ts", 0.0), m.get("ri_r", 1.0), m.get("rs", 0.0), 1.0,
-                                      m.get("x", 0.33), m.get("y", 0.33), m.get("rd", 0.5), m.get("sr", 0.0),
-                                      m.get("ri_i", 0.0)])
+        kind = m.get("kind", 0)
+        tail = [0.0] * 20
+        if kind == 1:
+            row = [0.0] * 10 + [m.get("ts", 0.0), m.get("ri_r", 1.0), m.get("rs", 0.0), 1.0,
+                                m.get("x", 0.33), m.get("y", 0.33), m.get("rd", 0.5), m.get("sr", 0.0),
+                                m.get("ri_i", 0.0)]
+        elif kind == 3:
+            row = [0.0] * 3 + list(m.get("rgb", (0.0, 0.0, 0.0))) + [0.0] * 3 + \
+                [m.get("s", 0.0), m.get("tran", 0.0), m.get("ri", 1.0), m.get("rf", 0.0)] + \
+                [3.0, 0.33, 0.33, 0.5, 0.0, 0.0]
+            tail[6:10] = [m.get("kd", 0.0), m.get("ks", 0.0), m.get("clrh", 0.0), m.get("clrf", 0.0)]
+            for k, (first, count) in enumerate(m.get("layers", [(0, 0)] * 5)):
+                tail[10 + k], tail[15 + k] = float(first), float(count)
         else:
-            rows.append(list(m.get("ka", (0.0, 0.0, 0.0))) + list(m.get("kd", (0.0, 0.0, 0.0))) +
-                        list(m.get("ks", (0.0, 0.0, 0.0))) +
-                        [m.get("s", 0.0), m.get("tran", 0.0), m.get("ri", 1.0), m.get("rf", 0.0)] +
-                        [0.0, 0.33, 0.33, 0.5, 0.0, 0.0])
-    return np.asarray(rows, np.float32).reshape(-1, 19)
+            row = list(m.get("ka", (0.0, 0.0, 0.0))) + list(m.get("kd", (0.0, 0.0, 0.0))) + \
+                list(m.get("ks", (0.0, 0.0, 0.0))) + \
+                [m.get("s", 0.0), m.get("tran", 0.0), m.get("ri", 1.0), m.get("rf", 0.0)] + \
+                [float(kind), 0.33, 0.33, 0.5, 0.0, 0.0]
+            for k, t in enumerate(m.get("tex", ())):
+                tail[k] = 0.0 if t is None or t < 0 else float(t + 1)
+        rows.append(row + tail)
+    return np.asarray(rows, np.float32).reshape(-1, 39)
 
 
 def build():
@@ -113,6 +139,7 @@ def _load():
         "oracle_gauss_circular3d": (None, [vp, i32, i32, i32, f, f]),
         "oracle_clip": (None, [vp, i32, i32, i32, i32, i32, vp]),
         "oracle_mt_stream": (None, [i32, vp]),
+        "oracle_texture_sample": (f, [vp, vp, vp, i32, vp, i32, f, f, i32, vp]),
         "oracle_glare_filter": (None, [i32, i32, f, i32, vp]),
         "oracle_glare": (i32, [vp, i32, i32, f, f, vp]),
     }
@@ -210,7 +237,8 @@ class OracleScene:
         return occ, cnt
 
     def render(self, albedo, camera, lights, bounce=False, spp=1, seed=0, pixel_stride=1, nthreads=8, packet=False,
-               records=False, materials=None, tri_material=None, vertex_normals=None, background=(0.0, 0.0, 0.0)):
+               records=False, materials=None, tri_material=None, vertex_normals=None, background=(0.0, 0.0, 0.0),
+               textures=None, texture_coords=None):
         """One frame; bounce is the number of diffuse bounces (True = 1).
         records=True adds per-sample path records over the bounce + 1 path
         vertices, level-major (vertex k, sample p * spp + s): prim (L, S)
@@ -220,7 +248,11 @@ class OracleScene:
         Lambert albedo default) renders phong materials with mirror and
         refraction rays; counters then gain sec_counters (secondary rays,
         their shadow rays). vertex_normals (n, 9) float32 (FLT_MAX first:
-        none) shade with interpolated normals and select that frame too."""
+        none) shade with interpolated normals and select that frame too.
+        textures = (maps, images, texels, layers): MAP_DTYPE and IMAGE_DTYPE
+        arrays (the library's rt_texture_map / rt_texture_image layout),
+        float32 texels, int32 layer lists (or None); texture_coords (n, 6)
+        float32 (FLT_MAX first: none)."""
         npix = camera.x_res * camera.y_res
         nl = len(lights)
         rgb = np.full((npix, 3), np.nan, np.float32)
@@ -247,13 +279,34 @@ class OracleScene:
         sec = np.zeros(2, np.uint64)
         out.sec_counters = _p(sec)
         if materials is not None:
-            mats = material_array(materials) if not isinstance(materials, np.ndarray) else \
-                np.ascontiguousarray(materials, np.float32).reshape(-1, 19)
+            if isinstance(materials, np.ndarray):
+                mats = np.asarray(materials, np.float32)
+                mats = mats.reshape(-1, mats.shape[-1] if mats.ndim > 1 else mats.size)
+                if mats.shape[1] == 19:  # rows without textures: no texture slots, no layers
+                    mats = np.concatenate([mats, np.zeros((mats.shape[0], 20), np.float32)], axis=1)
+                mats = np.ascontiguousarray(mats).reshape(-1, 39)
+            else:
+                mats = material_array(materials)
             tm = np.ascontiguousarray(tri_material, np.int32)
             p.materials, p.nmaterials, p.tri_material = _p(mats), mats.shape[0], _p(tm)
         if vertex_normals is not None:
             vn = np.ascontiguousarray(vertex_normals, np.float32).reshape(-1, 9)
             p.vertex_normals = _p(vn)
+        keep = []
+        if textures is not None:
+            maps, images, texels, layers = textures
+            maps = np.ascontiguousarray(maps)
+            images = np.ascontiguousarray(images)
+            if maps.dtype.itemsize != MAP_DTYPE.itemsize or (images.size and images.dtype.itemsize != 24):
+                raise ValueError("texture maps and images must have the rt_texture_map / rt_texture_image layout")
+            texels = np.ascontiguousarray(texels, np.float32)
+            layers = np.ascontiguousarray(layers if layers is not None else np.zeros(1, np.int32), np.int32)
+            keep += [maps, images, texels, layers]
+            p.maps, p.images, p.texels, p.layers = _p(maps), _p(images), _p(texels), _p(layers)
+        if texture_coords is not None:
+            vt = np.ascontiguousarray(texture_coords, np.float32).reshape(-1, 6)
+            keep.append(vt)
+            p.texture_coords = _p(vt)
         alb = np.ascontiguousarray(albedo, np.float32)
         rc = LIB.oracle_render(self.h, _p(alb), C.byref(camera_c(camera)), lights_c(lights), nl, C.byref(p),
                                C.byref(out))
@@ -422,6 +475,20 @@ def tone_map(rgb, mode=0, key=0.18, xw=1.0 / 3.0, yw=1.0 / 3.0, desaturate=False
     if LIB.oracle_tone_map(_p(img), w, h, C.byref(tp), _p(lvl), _p(yi)) != 0:
         raise ValueError(f"tone mapping mode {mode} is not restated")
     return img, float(lvl[0]), float(yi[0])
+
+
+def texture_sample(maps, images, texels, index, dst, vt=None, mono=False, rgb=(0.0, 0.0, 0.0)):
+    """texture_map (rgb in, rgb out) or texture_map_monochrome (the value) of
+    maps[index] at hit point dst, through texture coordinates vt = (u, v)
+    when given. Returns (colour or value, alpha)."""
+    maps, images = np.ascontiguousarray(maps), np.ascontiguousarray(images)
+    texels = np.ascontiguousarray(texels, np.float32)
+    d = f32a(*dst)
+    c = np.asarray(rgb, np.float32).copy()
+    a = LIB.oracle_texture_sample(_p(maps), _p(images) if images.size else None, _p(texels) if texels.size else None,
+                                  index, _p(d), 0 if vt is None else 1, 0.0 if vt is None else float(vt[0]),
+                                  0.0 if vt is None else float(vt[1]), 1 if mono else 0, _p(c))
+    return (float(c[0]) if mono else c), float(a)
 
 
 def mt_stream(count):

@@ -18,6 +18,7 @@
 #include "packet.hpp"
 #include "rt_oracle.h"
 #include "sampling.hpp"
+#include "texture.hpp"
 
 namespace oracle {
 
@@ -290,14 +291,19 @@ void render_pixel(const FrameCtx &f, int px, int py, LocalCounters *lc) {
 // double, so a power of exactly 0.1f (above 0.1) still recurses.
 constexpr double kMinReflectivePower = 0.1;
 
-constexpr int kMaterialFloats = 19;
+constexpr int kMaterialFloats = 39;
 constexpr int kCookTorrance = 1;
+constexpr int kColoured = 2;
+constexpr int kMapper = 3;
 
 struct Phong {
     Colour ka, kd, ks;
     float s = 0.0f, tran = 0.0f, ri = 1.0f, rf = 0.0f;
-    int kind = 0;  // 0 phong_shader, 1 cook_torrance_cxy
+    int kind = 0;  // 0 phong_shader, 1 cook_torrance_cxy, 2 coloured_mapper_shader, 3 mapper_shader
     float cx = 0.33f, cy = 0.33f, rd = 0.5f, sr = 0.0f, rii = 0.0f;
+    int tex[6] = {0, 0, 0, 0, 0, 0};  // 1 + map: t_ka, t_kd, t_ks, t_ns, t_refl, t_d
+    float mkd = 0.0f, mks = 0.0f, clrh = 0.0f, clrf = 0.0f;
+    int first[5] = {0, 0, 0, 0, 0}, count[5] = {0, 0, 0, 0, 0};  // ctex, dtex, stex, rtex, ttex
 };
 
 // XYZ -> RGB for the primaries and white point of rt/materials/material.h:17-45,
@@ -381,6 +387,15 @@ Phong material_of(const FrameCtx &f, int prim) {
         m.rd = q[16];
         m.sr = q[17];
         m.rii = q[18];
+        for (int k = 0; k < 6; ++k) m.tex[k] = static_cast<int>(q[19 + k]);
+        m.mkd = q[25];
+        m.mks = q[26];
+        m.clrh = q[27];
+        m.clrf = q[28];
+        for (int k = 0; k < 5; ++k) {
+            m.first[k] = static_cast<int>(q[29 + k]);
+            m.count[k] = static_cast<int>(q[34 + k]);
+        }
     } else {
         m.kd = kd_of(f, prim);  // phong_shader(255 * albedo, 1): ks = 0 * rgb, ka = 0
     }
@@ -440,6 +455,77 @@ Colour trace_phong(const FrameCtx &f, Ray &r, bool primary, LocalCounters *lc, P
     }
     const Phong m = material_of(f, prim);
     const V3 &n = sh.shade_norm;
+    // The hit's texture coordinates (rt/triangle.h:72-79): u vt_c + v vt_b +
+    // (1 - (u + v)) vt_a, when the triangle has them.
+    const oracle_tex::Tables T{f.p.maps, f.p.images, f.p.texels, f.p.layers};
+    bool has_vt = false;
+    float vtu = 0.0f, vtv = 0.0f;
+    if (f.p.texture_coords && f.p.texture_coords[6 * static_cast<int64_t>(prim)] != FLT_MAX) {
+        const float *q = f.p.texture_coords + 6 * static_cast<int64_t>(prim);
+        vtu = (h.u * q[4]) + (h.v * q[2]) + ((1.0f - (h.u + h.v)) * q[0]);
+        vtv = (h.u * q[5]) + (h.v * q[3]) + ((1.0f - (h.u + h.v)) * q[1]);
+        has_vt = true;
+    }
+    // mapper_shader's monochrome layer lists (rt/materials/mapper_shader.cc:41-66, 100-126).
+    auto layered = [&](int list, float value) {
+        float alpha = 1.0f;
+        for (int k = 0; k < m.count[list]; ++k) {
+            float z;
+            alpha *= oracle_tex::texture_map_monochrome(T, f.p.layers[m.first[list] + k], &z, sh.dst, has_vt, vtu, vtv);
+            value = z * (1.0f / 255.0f);
+            if (alpha == 0.0f) break;
+        }
+        return value;
+    };
+    // generate_rays: the reflection and refraction coefficients
+    // (coloured_mapper_shader.cc:8-38, mapper_shader.cc:41-82).
+    float refl_c = m.rf, tran_c = m.tran;
+    if (m.kind == kColoured) {
+        if (m.tex[4]) {
+            oracle_tex::Rgb3 tmp;
+            oracle_tex::texture_map(T, m.tex[4] - 1, &tmp, sh.dst, has_vt, vtu, vtv);
+            refl_c = tmp.r * (m.rf * (1.0f / 255.0f));
+        }
+        if (m.tex[5]) {
+            oracle_tex::Rgb3 tmp;
+            oracle_tex::texture_map(T, m.tex[5] - 1, &tmp, sh.dst, has_vt, vtu, vtv);
+            tran_c = tmp.r * (m.tran * (1.0f / 255.0f));
+        }
+    } else if (m.kind == kMapper) {
+        refl_c = layered(3, m.rf);
+        tran_c = layered(4, m.tran);
+    }
+    // shade: the material's colours at the hit (coloured_mapper_shader.cc:41-80,
+    // mapper_shader.cc:84-126).
+    Colour ka = m.ka, kd = m.kd, ks = m.ks;
+    float ns = m.s, mkd = m.mkd;
+    if (m.kind == kColoured) {
+        auto mapped = [&](int slot, Colour col) {
+            if (!slot) return col;
+            oracle_tex::Rgb3 t{col.r, col.g, col.b};
+            oracle_tex::texture_map(T, slot - 1, &t, sh.dst, has_vt, vtu, vtv);
+            return Colour{t.r * (col.r * (1.0f / 255.0f)), t.g * (col.g * (1.0f / 255.0f)),
+                          t.b * (col.b * (1.0f / 255.0f))};
+        };
+        if (m.tex[3]) {
+            oracle_tex::Rgb3 cur_ka;
+            ns = oracle_tex::texture_map(T, m.tex[3] - 1, &cur_ka, sh.dst, has_vt, vtu, vtv);
+            ns *= (m.s * (1.0f / 255.0f));
+        }
+        ka = mapped(m.tex[0], m.ka);
+        kd = mapped(m.tex[1], m.kd);
+        ks = mapped(m.tex[2], m.ks);
+    } else if (m.kind == kMapper) {
+        oracle_tex::Rgb3 rgb{m.kd.r, m.kd.g, m.kd.b};
+        float alpha = 1.0f;
+        for (int k = 0; k < m.count[0]; ++k) {
+            alpha *= oracle_tex::texture_map(T, f.p.layers[m.first[0] + k], &rgb, sh.dst, has_vt, vtu, vtv);
+            if (alpha == 0.0f) break;
+        }
+        kd = Colour{rgb.r, rgb.g, rgb.b};
+        mkd = layered(1, m.mkd);
+        ns = layered(2, m.s);
+    }
     const float nv = dot(n, r.dir);  // cook_torrance_cxy: one dot product for every light
     for (size_t l = 0; l < f.lights.size(); ++l) {
         Ray illum = shadow_ray(sh, f.lights[l].c);
@@ -451,12 +537,31 @@ Colour trace_phong(const FrameCtx &f, Ray &r, bool primary, LocalCounters *lc, P
         }
         illum.magn = (occluded ? 0.0f : 1.0f) / 1.0f;
         if (vis_out) vis_out[l] = occluded ? 0 : 1;
+        // rt/light.h:44-59 (no spot): rgb / ((len * d)^2 + 1).
+        const float dist = illum.length * f.lights[l].d;
+        const float d_scale = (dist * dist) + 1.0f;
+        const Colour li{f.lights[l].rgb.r / d_scale, f.lights[l].rgb.g / d_scale, f.lights[l].rgb.b / d_scale};
+        if (m.kind == kMapper) {
+            // mapper_shader.cc:128-170: the reflection is taken about n
+            // scaled by 2 (n.l) kd, and there is no ambient term.
+            const float diff_shade = dot(illum.dir, n) * mkd;
+            if (diff_shade < 0.0f) continue;
+            const float shade_x2 = 2.0f * diff_shade;
+            const V3 ref_dir(illum.dir.x - n.x * shade_x2, illum.dir.y - n.y * shade_x2, illum.dir.z - n.z * shade_x2);
+            const float ray_dot_reflection = dot(r.dir, ref_dir);
+            float spec_shade = 0.0f;
+            if (ray_dot_reflection > 0.0f) spec_shade = std::pow(ray_dot_reflection, ns) * m.mks;
+            c.r += kd.r * diff_shade * li.r * illum.magn;
+            c.g += kd.g * diff_shade * li.g * illum.magn;
+            c.b += kd.b * diff_shade * li.b * illum.magn;
+            c.r += ((m.clrh * kd.r) + ((1.0f - m.clrh) * li.r)) * spec_shade * illum.magn;
+            c.g += ((m.clrh * kd.g) + ((1.0f - m.clrh) * li.g)) * spec_shade * illum.magn;
+            c.b += ((m.clrh * kd.b) + ((1.0f - m.clrh) * li.b)) * spec_shade * illum.magn;
+            continue;
+        }
         const float shade = dot(illum.dir, n);
         if (shade < 0.0f) continue;
         if (m.kind == kCookTorrance) {
-            const float dist = illum.length * f.lights[l].d;
-            const float d_scale = (dist * dist) + 1.0f;
-            const Colour li{f.lights[l].rgb.r / d_scale, f.lights[l].rgb.g / d_scale, f.lights[l].rgb.b / d_scale};
             cook_torrance_term(m, n, r.dir, nv, illum.dir, shade, illum.magn, li, &c);
             continue;
         }
@@ -464,51 +569,56 @@ Colour trace_phong(const FrameCtx &f, Ray &r, bool primary, LocalCounters *lc, P
         const float shade_x2 = 2.0f * shade;
         const V3 ref_dir(illum.dir.x - n.x * shade_x2, illum.dir.y - n.y * shade_x2, illum.dir.z - n.z * shade_x2);
         const float ray_dot_reflection = dot(r.dir, ref_dir);
-        Colour sc{shade * m.kd.r, shade * m.kd.g, shade * m.kd.b};
+        Colour sc{shade * kd.r, shade * kd.g, shade * kd.b};
         if (ray_dot_reflection > 0.0f) {
-            const float sp = std::pow(ray_dot_reflection, m.s);
-            sc.r += sp * m.ks.r;
-            sc.g += sp * m.ks.g;
-            sc.b += sp * m.ks.b;
+            const float sp = std::pow(ray_dot_reflection, ns);
+            sc.r += sp * ks.r;
+            sc.g += sp * ks.g;
+            sc.b += sp * ks.b;
         }
-        // rt/light.h:44-59 (no spot): rgb / ((len * d)^2 + 1).
-        const float dist = illum.length * f.lights[l].d;
-        const float d_scale = (dist * dist) + 1.0f;
-        const Colour li{f.lights[l].rgb.r / d_scale, f.lights[l].rgb.g / d_scale, f.lights[l].rgb.b / d_scale};
         c.r += li.r * (sc.r * illum.magn);
         c.g += li.g * (sc.g * illum.magn);
         c.b += li.b * (sc.b * illum.magn);
     }
-    if (m.kind != kCookTorrance) {  // cook_torrance_cxy has no ambient term
-        c.r += m.ka.r;
-        c.g += m.ka.g;
-        c.b += m.ka.b;
+    if (m.kind != kCookTorrance && m.kind != kMapper) {  // no ambient term in those two
+        c.r += ka.r;
+        c.g += ka.g;
+        c.b += ka.b;
     }
-    if (m.rf > 0.0f) {
+    {
         // ray::reflect: dir - n * 2 (dir . n) from a point moved off the front.
-        const float power = r.magn * m.rf;
+        const float power = r.magn * refl_c;
         if (static_cast<double>(power) > kMinReflectivePower) {
             Ray rl;
             rl.ogn = offset_start_point(sh.dst, sh.geo_norm, 1);
             rl.dir = reflect_dir(r.dir, n);
             rl.magn = power;
             const Colour pix = trace_phong(f, rl, false, lc, pc, nullptr, nullptr, nullptr);
-            const float w = m.rf / 1.0f;
+            // phong and coloured_mapper: rf (untextured); mapper_shader: its layered value.
+            const float w = ((m.kind == kMapper) ? refl_c : m.rf) / 1.0f;
             c.r += pix.r * w;
             c.g += pix.g * w;
             c.b += pix.b * w;
         }
     }
-    if (m.tran > 0.0f) {
-        const float power = r.magn * m.tran;
+    {
+        const float power = r.magn * tran_c;
         V3 rdir;
         if (static_cast<double>(power) > kMinReflectivePower && refract_dir(r.dir, n, m.ri, leaving, &rdir)) {
-            {
-                Ray rr;
-                rr.ogn = offset_start_point(sh.dst, sh.geo_norm, -1);
-                rr.dir = rdir;
-                rr.magn = power;
-                const Colour pix = trace_phong(f, rr, false, lc, pc, nullptr, nullptr, nullptr);
+            Ray rr;
+            rr.ogn = offset_start_point(sh.dst, sh.geo_norm, -1);
+            rr.dir = rdir;
+            rr.magn = power;
+            const Colour pix = trace_phong(f, rr, false, lc, pc, nullptr, nullptr, nullptr);
+            if (m.kind == kMapper) {
+                // mapper_shader.cc:179-183: c += clrf c + (1 - clrf) refracted trans.
+                const Colour add{(m.clrf * c.r) + ((1.0f - m.clrf) * pix.r * tran_c),
+                                 (m.clrf * c.g) + ((1.0f - m.clrf) * pix.g * tran_c),
+                                 (m.clrf * c.b) + ((1.0f - m.clrf) * pix.b * tran_c)};
+                c.r += add.r;
+                c.g += add.g;
+                c.b += add.b;
+            } else {
                 const float w = m.tran / 1.0f;
                 c.r += pix.r * w;
                 c.g += pix.g * w;
@@ -926,6 +1036,23 @@ int render_frame(const Scene &s, const float *albedo, const oracle_camera &cam, 
 }  // namespace oracle
 
 extern "C" {
+
+// One texture_map (mono = 0: rgb in/out) or texture_map_monochrome (mono =
+// 1: rgb[0] out) of map `map` at hit point dst; returns the alpha.
+float oracle_texture_sample(const oracle_texture_map *maps, const oracle_texture_image *images, const float *texels,
+                            int32_t map, const float *dst, int32_t has_vt, float vtu, float vtv, int32_t mono,
+                            float *rgb) {
+    const oracle_tex::Tables T{maps, images, texels, nullptr};
+    const oracle::V3 d(dst[0], dst[1], dst[2]);
+    if (mono) return oracle_tex::texture_map_monochrome(T, map, rgb, d, has_vt != 0, vtu, vtv);
+    oracle_tex::Rgb3 c{rgb[0], rgb[1], rgb[2]};
+    const float a = oracle_tex::texture_map(T, map, &c, d, has_vt != 0, vtu, vtv);
+    rgb[0] = c.r;
+    rgb[1] = c.g;
+    rgb[2] = c.b;
+    return a;
+}
+
 
 void oracle_reflect_dir(const float *d, const float *n, float *out) {
     const oracle::V3 r = oracle::reflect_dir(oracle::V3(d[0], d[1], d[2]), oracle::V3(n[0], n[1], n[2]));

@@ -31,6 +31,31 @@ typedef struct {
     float d;
 } oracle_light;
 
+/* A texture map's state (rt/materials/texture_mapper.h and the mappers'
+ * constructors): kind 0 planar, 1 cubic, 2 cylindrical, 3 checker board;
+ * wrapping 0 blank, 1 clamp, 2 tile, 3 mirror; falloff 0 cubic, 1
+ * spherical, 2-4 linear x/y/z, 5 none. Same layout as the library's
+ * rt_texture_map. */
+typedef struct {
+    int32_t kind;
+    int32_t image;
+    float c[3], n[3], s[3];
+    float u[3], v[3];
+    int32_t u_wrap, v_wrap;
+    int32_t u_off, v_off;
+    int32_t cycles;
+    float rgb[3], opacity;
+    int32_t inverse;
+    int32_t falloff;
+    float falloff_centre[3], falloff_grad[3];
+} oracle_texture_map;
+
+/* width x height texels of cpp floats at texels + offset, row 0 first. */
+typedef struct {
+    int32_t width, height, cpp, reserved;
+    int64_t offset;
+} oracle_texture_image;
+
 /* Render options. bounce: number of cosine-weighted diffuse bounces, each
  * with direct light at its hit (radiance c = sum_k T_k direct(x_k), T_k the
  * product of kd / 255 over the earlier vertices). spp > 1 jitters the sample
@@ -42,8 +67,12 @@ typedef struct {
     int32_t pixel_stride; /* trace pixels p with p % stride == 0 (packet frame: 8x8 tiles) */
     int32_t nthreads;
     int32_t packet;       /* 1: the SSE packet frame (spp 1, bounce <= 1) */
-    /* Materials: nmaterials x 19 floats ka rgb, kd rgb, ks rgb, s, tran,
-     * ri, rf, kind, cx, cy, rd, sr, ri_i, and one index per triangle (-1:
+    /* Materials: nmaterials x 39 floats ka rgb, kd rgb, ks rgb, s, tran,
+     * ri, rf, kind, cx, cy, rd, sr, ri_i, six texture slots (1 + map, 0
+     * none: t_ka, t_kd, t_ks, t_ns, t_refl, t_d of a coloured_mapper_shader,
+     * kind 2), mapper kd, mapper ks, clrh, clrf, and five layer lists
+     * (first x 5, count x 5) of a mapper_shader (kind 3, base colour in kd
+     * rgb), and one index per triangle (-1:
      * phong_shader(255 * albedo, 1), the Lambert default). kind 0 is
      * phong_shader (rt/materials/phong_shader.h:34-53, coloured form; the
      * last five floats unused); kind 1 is cook_torrance_cxy
@@ -64,6 +93,14 @@ typedef struct {
     /* The camera's background colour: a primary, mirror or refraction ray
      * that hits nothing takes it (rt/camera.h:234-240). */
     float background[3];
+    /* Texture maps, their images and texels, the mapper_shader layer lists
+     * (map indices) and per-triangle texture coordinates (6 floats: u, v
+     * at a, b, c; FLT_MAX first: none). Any may be NULL. */
+    const oracle_texture_map *maps;
+    const oracle_texture_image *images;
+    const float *texels;
+    const int32_t *layers;
+    const float *texture_coords;
 } oracle_render_params;
 
 /* Outputs; any pointer may be NULL. Arrays are indexed by pixel p = y*W + x.
@@ -92,6 +129,12 @@ typedef struct {
     uint64_t *sec_counters;
 } oracle_render_out;
 
+/* One texture_map (mono 0: rgb in and out) or texture_map_monochrome (mono
+ * 1: the value in rgb[0]) of maps[map] at hit point dst, with texture
+ * coordinates (vtu, vtv) when has_vt; returns the alpha. */
+float oracle_texture_sample(const oracle_texture_map *maps, const oracle_texture_image *images, const float *texels,
+                            int32_t map, const float *dst, int32_t has_vt, float vtu, float vtv, int32_t mono,
+                            float *rgb);
 oracle_scene *oracle_scene_create(const float *abc, int64_t n, const uint8_t *flags);
 void oracle_scene_destroy(oracle_scene *s);
 /* AAC build (alpha, delta as ssd/bvh_builder.h:23). Returns the root. */

@@ -5,6 +5,7 @@
 
 #include "rt_device.hpp"
 #include "rt_internal.hpp"
+#include "textures.hpp"
 
 namespace rt {
 
@@ -102,6 +103,10 @@ void free_scene(rt_scene *sc) {
     if (sc->d_mat) (void)hipFree(sc->d_mat);
     if (sc->d_tri_mat) (void)hipFree(sc->d_tri_mat);
     if (sc->d_vn) (void)hipFree(sc->d_vn);
+    if (sc->d_maps) (void)hipFree(sc->d_maps);
+    if (sc->d_texels) (void)hipFree(sc->d_texels);
+    if (sc->d_layers) (void)hipFree(sc->d_layers);
+    if (sc->d_vt) (void)hipFree(sc->d_vt);
     delete sc;
 }
 
@@ -326,7 +331,7 @@ int32_t rt_write_tga(const char *path, const uint8_t *bgr, int32_t x_res, int32_
     return rt::write_tga(path, bgr, x_res, y_res);
 }
 
-static_assert(sizeof(rt_material) == 76, "rt_material is 19 four-byte fields");
+static_assert(sizeof(rt_material) == 156, "rt_material is 39 four-byte fields");
 
 int32_t rt_scene_set_materials(rt_scene *scene, const rt_material *materials, int32_t nmaterials,
                                const int32_t *tri_material) {
@@ -343,6 +348,7 @@ int32_t rt_scene_set_materials(rt_scene *scene, const rt_material *materials, in
         scene->d_mat = nullptr;
         scene->d_tri_mat = nullptr;
         scene->nmat = 0;
+        scene->maps_used = scene->layers_used = 0;
     };
     if (nmaterials == 0) {
         drop();
@@ -365,7 +371,22 @@ int32_t rt_scene_set_materials(rt_scene *scene, const rt_material *materials, in
             for (int k = 0; k < 3; ++k)
                 for (const float v : {q.ka[k], q.kd[k], q.ks[k]}) ok = ok && in(v, 0.0f, 255.0f);
             range = (q.kind == RT_MATERIAL_PHONG) ? "colours 0..255, tran and rf 0..0.95, ri 1..5 when tran > 0"
-                                                  : "colours 0..255, tran and rf 0..1, ri 1..5 when tran > 0";
+                                                  : "colours 0..255, tran and rf 0..1, ri 1..5 when tran > 0, "
+                                                    "texture slots 0 or 1 + a map of the scene";
+            for (int k = 0; k < 6; ++k) {
+                const bool none = q.tex[k] == 0;
+                ok = ok && (none || (q.kind == RT_MATERIAL_COLOURED && q.tex[k] >= 1 && q.tex[k] <= scene->nmaps));
+            }
+        } else if (q.kind == RT_MATERIAL_MAPPER) {
+            // rt/materials/mapper_shader.h:21-33.
+            ok = q.mapper_kd >= 0.0f && in(q.mapper_ks, 0.0f, 1.0f) && in(q.transmittance, 0.0f, 1.0f) &&
+                 in(q.reflectance, 0.0f, 1.0f) && (q.transmittance == 0.0f || in(q.refractive_index, 1.0f, 5.0f));
+            for (int k = 0; k < 3; ++k) ok = ok && in(q.kd[k], 0.0f, 255.0f);
+            for (int k = 0; k < 5; ++k)
+                ok = ok && q.layer_first[k] >= 0 && q.layer_count[k] >= 0 && q.layer_count[k] < 256 &&
+                     q.layer_first[k] < (1 << 23) && q.layer_first[k] + q.layer_count[k] <= scene->nlayers;
+            range = "rgb 0..255, kd >= 0, ks, tran and rf 0..1, ri 1..5 when tran > 0, layer lists inside the "
+                    "scene's layers (rt_scene_set_textures first)";
         } else if (q.kind == RT_MATERIAL_COOK_TORRANCE) {
             ok = in(q.chroma[0], 0.0f, 1.0f) && in(q.chroma[1], 0.0f, 1.0f) && in(q.diffuse, 0.0f, 1.0f) &&
                  in(q.reflectance, 0.0f, 1.0f) && in(q.transmittance, 0.0f, 1.0f) && in(q.roughness, 0.0f, 0.5f) &&
@@ -384,6 +405,20 @@ int32_t rt_scene_set_materials(rt_scene *scene, const rt_material *materials, in
         row[2] = make_float4(q.ks[0], q.ks[1], q.ks[2], q.refractive_index);
         row[3] = make_float4(q.reflectance, static_cast<float>(q.kind), q.chroma[0], q.chroma[1]);
         row[4] = make_float4(q.diffuse, q.roughness, q.extinction, 0.0f);
+        auto bits = [](int32_t v) {
+            float f;
+            std::memcpy(&f, &v, sizeof(f));
+            return f;
+        };
+        const bool mapper = q.kind == RT_MATERIAL_MAPPER;
+        int32_t lists[5];
+        for (int k = 0; k < 5; ++k) lists[k] = mapper ? ((q.layer_first[k] << 8) | q.layer_count[k]) : 0;
+        const bool col = q.kind == RT_MATERIAL_COLOURED;
+        row[5] = make_float4(bits(col ? q.tex[0] : 0), bits(col ? q.tex[1] : 0), bits(col ? q.tex[2] : 0),
+                             bits(col ? q.tex[3] : 0));
+        row[6] = make_float4(bits(col ? q.tex[4] : 0), bits(col ? q.tex[5] : 0), q.mapper_kd, q.mapper_ks);
+        row[7] = make_float4(q.clrh, q.clrf, bits(lists[0]), bits(lists[1]));
+        row[8] = make_float4(bits(lists[2]), bits(lists[3]), bits(lists[4]), 0.0f);
     }
     // A triangle is transparent to shadow rays exactly when its material is
     // (material(tran > 0), rt/materials/material.h:50-66).
@@ -420,6 +455,101 @@ int32_t rt_scene_set_materials(rt_scene *scene, const rt_material *materials, in
     scene->d_mat = d_mat;
     scene->d_tri_mat = d_tri;
     scene->nmat = nmaterials;
+    for (int32_t m = 0; m < nmaterials; ++m) {
+        const rt_material &q = materials[m];
+        if (q.kind == RT_MATERIAL_COLOURED)
+            for (int k = 0; k < 6; ++k) scene->maps_used = std::max(scene->maps_used, q.tex[k]);
+        if (q.kind == RT_MATERIAL_MAPPER)
+            for (int k = 0; k < 5; ++k)
+                if (q.layer_count[k] > 0)
+                    scene->layers_used = std::max(scene->layers_used, q.layer_first[k] + q.layer_count[k]);
+    }
+    return RT_OK;
+}
+
+int32_t rt_scene_set_textures(rt_scene *scene, const rt_texture_map *maps, int32_t nmaps,
+                              const rt_texture_image *images, int32_t nimages, const float *texels, int64_t ntexels,
+                              const int32_t *layers, int32_t nlayers) {
+    if (!scene) return RT_ERR_INVALID;
+    rt_ctx *ctx = scene->ctx;
+    if (int32_t rc = rt::set_device(ctx)) return rc;
+    hipStream_t s = ctx->stream;
+    auto bad = [&](const std::string &why) { return fail(ctx, RT_ERR_INVALID, "rt_scene_set_textures: " + why); };
+    if (nmaps < 0 || nimages < 0 || ntexels < 0 || nlayers < 0 || (nmaps > 0 && !maps) ||
+        (nimages > 0 && !images) || (ntexels > 0 && !texels) || (nlayers > 0 && !layers))
+        return bad("bad arrays");
+    std::vector<rt::DevTexMap> dev(static_cast<size_t>(nmaps));
+    for (int32_t i = 0; i < nmaps; ++i) {
+        const rt_texture_map &m = maps[i];
+        rt::DevTexMap &d = dev[i];
+        std::memset(&d, 0, sizeof(d));
+        d.m = m;
+        const std::string who = "map " + std::to_string(i) + ": ";
+        if (m.kind < RT_MAP_PLANAR || m.kind > RT_MAP_CHECKER_BOARD) return bad(who + "unknown kind");
+        if (m.falloff < RT_FALLOFF_CUBIC || m.falloff > RT_FALLOFF_NONE) return bad(who + "unknown falloff");
+        if (m.kind == RT_MAP_CHECKER_BOARD) continue;
+        if (m.image < 0 || m.image >= nimages) return bad(who + "no image " + std::to_string(m.image));
+        const rt_texture_image &im = images[m.image];
+        if (im.width <= 0 || im.height <= 0 || (im.cpp != 1 && im.cpp != 3 && im.cpp != 4) || im.offset < 0 ||
+            im.offset + static_cast<int64_t>(im.width) * im.height * im.cpp > ntexels)
+            return bad(who + "image " + std::to_string(m.image) + " has a bad shape or lies outside the texels");
+        for (const int32_t wm : {m.u_wrap, m.v_wrap})
+            if (wm < RT_WRAP_BLANK || wm > RT_WRAP_MIRROR) return bad(who + "unknown wrapping mode");
+        // Tile and mirror divide by size - 1 (texture_mapper.h apply_wrapping_mode).
+        if (((m.u_wrap == RT_WRAP_TILE || m.u_wrap == RT_WRAP_MIRROR) && im.width < 2) ||
+            ((m.v_wrap == RT_WRAP_TILE || m.v_wrap == RT_WRAP_MIRROR) && im.height < 2))
+            return bad(who + "tile and mirror wrapping need 2 texels along the axis");
+        d.w = im.width;
+        d.h = im.height;
+        d.cpp = im.cpp;
+        d.offset = im.offset;
+    }
+    for (int32_t i = 0; i < nlayers; ++i)
+        if (layers[i] < 0 || layers[i] >= nmaps) return bad("layer " + std::to_string(i) + " names no map");
+    if (nmaps < scene->maps_used || nlayers < scene->layers_used)
+        return bad("the scene's materials use " + std::to_string(scene->maps_used) + " maps and " +
+                   std::to_string(scene->layers_used) + " layers; change the materials first");
+    RT_HIP_CHECK(ctx, hipStreamSynchronize(s));
+    for (void *p : {static_cast<void *>(scene->d_maps), static_cast<void *>(scene->d_texels),
+                    static_cast<void *>(scene->d_layers)})
+        if (p) (void)hipFree(p);
+    scene->d_maps = nullptr;
+    scene->d_texels = nullptr;
+    scene->d_layers = nullptr;
+    scene->nmaps = scene->nlayers = 0;
+    if (nmaps == 0) return RT_OK;
+    hipError_t e = rt::upload(&scene->d_maps, dev.data(), nmaps, s);
+    if (e == hipSuccess) e = rt::upload(&scene->d_texels, ntexels > 0 ? texels : nullptr, ntexels, s);
+    if (e == hipSuccess && nlayers > 0) e = rt::upload(&scene->d_layers, layers, nlayers, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess)
+        return fail(ctx, e == hipErrorOutOfMemory ? RT_ERR_OOM : RT_ERR_HIP,
+                    std::string("rt_scene_set_textures: ") + hipGetErrorString(e));
+    scene->nmaps = nmaps;
+    scene->nlayers = nlayers;
+    return RT_OK;
+}
+
+int32_t rt_scene_set_texture_coords(rt_scene *scene, const float *vt) {
+    if (!scene) return RT_ERR_INVALID;
+    rt_ctx *ctx = scene->ctx;
+    if (int32_t rc = rt::set_device(ctx)) return rc;
+    hipStream_t s = ctx->stream;
+    if (scene->d_vt) {
+        RT_HIP_CHECK(ctx, hipStreamSynchronize(s));
+        (void)hipFree(scene->d_vt);
+        scene->d_vt = nullptr;
+    }
+    if (!vt || scene->n == 0) return RT_OK;
+    float *d = nullptr;
+    hipError_t e = rt::upload(&d, vt, 6 * scene->n, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        if (d) (void)hipFree(d);
+        return fail(ctx, e == hipErrorOutOfMemory ? RT_ERR_OOM : RT_ERR_HIP,
+                    std::string("rt_scene_set_texture_coords: ") + hipGetErrorString(e));
+    }
+    scene->d_vt = d;
     return RT_OK;
 }
 

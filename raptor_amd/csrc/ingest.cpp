@@ -76,6 +76,12 @@ struct Mesh {
     std::vector<rt_material> mats;
     std::vector<float> vn;         // 9 floats per triangle once any triangle has vertex normals
     bool any_vn = false;
+    std::vector<float> vt;         // 6 floats per triangle once any triangle has texture coordinates
+    bool any_vt = false;
+    std::vector<rt_texture_map> maps;
+    std::vector<rt_texture_image> images;
+    std::vector<float> texels;
+    std::map<std::string, int32_t> image_ids;  // file (+ "_inv") -> image
     int64_t vertices = 0, faces = 0, skipped_faces = 0, skipped_statements = 0;
     int32_t texture_maps = 0;
     bool has_camera = false;
@@ -83,8 +89,17 @@ struct Mesh {
     float background[3] = {0.0f, 0.0f, 0.0f};
     std::vector<rt_light> lights;
 
-    void add(const P3 &a, const P3 &b, const P3 &c, int32_t mat, const P3 *n) {
+    void add(const P3 &a, const P3 &b, const P3 &c, int32_t mat, const P3 *n, const float *t6 = nullptr) {
         const int64_t i = static_cast<int64_t>(tri_mat.size());
+        if (t6 && !any_vt) {
+            // Earlier triangles carry none: the reference marks them with
+            // x = MAX_DIST (rt/triangle.h:158-167).
+            any_vt = true;
+            vt.assign(static_cast<size_t>(6 * i), 0.0f);
+            for (int64_t k = 0; k < i; ++k) vt[6 * k] = FLT_MAX;
+        }
+        if (any_vt)
+            for (int k = 0; k < 6; ++k) vt.push_back(t6 ? t6[k] : (k == 0 ? FLT_MAX : 0.0f));
         for (const P3 *p : {&a, &b, &c}) {
             abc.push_back(p->x);
             abc.push_back(p->y);
@@ -187,8 +202,11 @@ void triangulate(const std::vector<P3> &p, std::vector<uint32_t> *tris) {
     }
 }
 
-// A polygon of the face's vertices (and normals, when given) as triangles.
-void add_polygon(Mesh *m, const std::vector<P3> &face, const std::vector<P3> *normals, int32_t mat) {
+// A polygon of the face's vertices (and normals and texture coordinates,
+// when given) as triangles; the corners of each triangle index all three
+// (rt/parsers/polygon_to_triangles.h:329-344).
+void add_polygon(Mesh *m, const std::vector<P3> &face, const std::vector<P3> *normals, int32_t mat,
+                 const std::vector<float> *texcoords = nullptr) {
     if (face.size() < 3) {
         ++m->skipped_faces;
         return;
@@ -200,7 +218,14 @@ void add_polygon(Mesh *m, const std::vector<P3> &face, const std::vector<P3> *no
         const bool with_n = normals && !normals->empty();
         if (with_n)
             for (int j = 0; j < 3; ++j) n3[j] = (*normals)[tris[k + j]];
-        m->add(face[tris[k]], face[tris[k + 1]], face[tris[k + 2]], mat, with_n ? n3 : nullptr);
+        float t6[6];
+        const bool with_t = texcoords && !texcoords->empty();
+        if (with_t)
+            for (int j = 0; j < 3; ++j) {
+                t6[2 * j] = (*texcoords)[2 * tris[k + j]];
+                t6[2 * j + 1] = (*texcoords)[2 * tris[k + j] + 1];
+            }
+        m->add(face[tris[k]], face[tris[k + 1]], face[tris[k + 2]], mat, with_n ? n3 : nullptr, with_t ? t6 : nullptr);
     }
 }
 
@@ -578,8 +603,70 @@ struct MtlState {
     P3 ka, kd, ks;
     float ns = 0.0f, ri = 1.0f, rf = 0.0f, trans = 0.0f;
     int illum = 0;  // not reset by newmtl (obj_parser.cc:380-396)
-    bool textured = false;
+    // Texture map files for t_ka, t_kd, t_ks, t_ns, t_refl, t_d
+    // (obj_parser.cc:263-268; the MTL statements never set t_ns).
+    std::string maps[6];
 };
+
+// A TGA file as read_tga reads it through libtga with TGA_IMAGE_DATA |
+// TGA_RGB (rt/materials/texture_mapper.h read_tga): true-colour or grey,
+// uncompressed or run-length encoded, 8, 24 or 32 bits; rows in file order;
+// blue and red swapped to RGB(A); one float 0-255 per channel.
+void decode_tga(const std::string &f, int *w, int *h, int *cpp, std::vector<float> *out) {
+    const unsigned char *d = reinterpret_cast<const unsigned char *>(f.data());
+    const size_t n = f.size();
+    if (n < 18) bad("TGA shorter than its header");
+    const int id_len = d[0], cmap_type = d[1], type = d[2];
+    const int cmap_len = d[5] | (d[6] << 8), cmap_depth = d[7];
+    *w = d[12] | (d[13] << 8);
+    *h = d[14] | (d[15] << 8);
+    const int depth = d[16];
+    if (type != 2 && type != 3 && type != 10 && type != 11)
+        bad("TGA image type " + std::to_string(type) + " (only true-colour and grey, raw or run-length)");
+    if (depth != 8 && depth != 24 && depth != 32) bad("TGA depth " + std::to_string(depth) + " (8, 24 or 32)");
+    if (*w <= 0 || *h <= 0) bad("TGA without pixels");
+    *cpp = depth >> 3;
+    size_t at = 18 + static_cast<size_t>(id_len);
+    if (cmap_type == 1) at += static_cast<size_t>(cmap_len) * ((cmap_depth + 7) / 8);
+    const size_t npix = static_cast<size_t>(*w) * static_cast<size_t>(*h);
+    const int bpp = *cpp;
+    std::vector<unsigned char> raw(npix * bpp);
+    if (type == 2 || type == 3) {
+        if (at + raw.size() > n) bad("TGA pixel data cut short");
+        std::memcpy(raw.data(), d + at, raw.size());
+    } else {
+        size_t p = 0;
+        while (p < npix) {
+            if (at >= n) bad("TGA run-length data cut short");
+            const int hdr = d[at++];
+            const size_t count = static_cast<size_t>(hdr & 0x7f) + 1;
+            if (p + count > npix) bad("TGA run-length packet past the image");
+            if (hdr & 0x80) {
+                if (at + bpp > n) bad("TGA run-length data cut short");
+                for (size_t k = 0; k < count; ++k) std::memcpy(&raw[(p + k) * bpp], d + at, bpp);
+                at += bpp;
+            } else {
+                if (at + count * bpp > n) bad("TGA run-length data cut short");
+                std::memcpy(&raw[p * bpp], d + at, count * bpp);
+                at += count * bpp;
+            }
+            p += count;
+        }
+    }
+    out->resize(raw.size());
+    for (size_t i = 0; i < npix; ++i) {
+        const unsigned char *px = &raw[i * bpp];
+        float *o = out->data() + i * bpp;
+        if (bpp >= 3) {
+            o[0] = px[2];
+            o[1] = px[1];
+            o[2] = px[0];
+            if (bpp == 4) o[3] = px[3];
+        } else {
+            o[0] = px[0];
+        }
+    }
+}
 
 // An MTL colour: one value (grey) or three, scaled to 0..255.
 P3 mtl_colour(Tokens *tk) {
@@ -591,13 +678,60 @@ P3 mtl_colour(Tokens *tk) {
     return {v[0], v[1], v[2]};
 }
 
+std::string read_file(const std::string &path);
+
+// load_image (obj_parser.cc:218-242): the image of a map file, read once per
+// file (and once more inverted, image = 1 - texel as picture_functions'
+// negative computes it, for map_d), under a planar_mapper centred at 0 with
+// normal (1, 0, 0), size 0 and mirror wrapping: it samples only through the
+// triangles' texture coordinates. Returns 1 + the map's index, or 0 for a
+// file this library cannot decode (only TGA), which m->texture_maps counts.
+int32_t mtl_map(Mesh *m, const std::string &file, bool invert) {
+    std::string ext = file.size() >= 4 ? file.substr(file.size() - 4) : std::string();
+    for (auto &ch : ext) ch = static_cast<char>(std::tolower(static_cast<unsigned char>(ch)));
+    if (ext != ".tga") {
+        ++m->texture_maps;
+        return 0;
+    }
+    const std::string key = file + (invert ? "_inv" : "");
+    auto it = m->image_ids.find(key);
+    if (it == m->image_ids.end()) {
+        int w, h, cpp;
+        std::vector<float> px;
+        decode_tga(read_file(file), &w, &h, &cpp, &px);
+        if (invert)
+            for (auto &v : px) v = 1.0f - v;
+        rt_texture_image im{};
+        im.width = w;
+        im.height = h;
+        im.cpp = cpp;
+        im.offset = static_cast<int64_t>(m->texels.size());
+        m->texels.insert(m->texels.end(), px.begin(), px.end());
+        m->images.push_back(im);
+        it = m->image_ids.emplace(key, static_cast<int32_t>(m->images.size() - 1)).first;
+    }
+    rt_texture_map t{};
+    t.kind = RT_MAP_PLANAR;
+    t.image = it->second;
+    t.n[0] = 1.0f;  // u = (n.y + n.z, 0, n.x), v = (0, n.x + n.z, n.y)
+    t.u[2] = 1.0f;
+    t.v[1] = 1.0f;
+    t.u_wrap = t.v_wrap = RT_WRAP_MIRROR;
+    t.opacity = 1.0f;
+    t.falloff = RT_FALLOFF_NONE;
+    m->maps.push_back(t);
+    return static_cast<int32_t>(m->maps.size());
+}
+
 // The material a finished newmtl block describes (obj_parser.cc:290-378):
-// the illumination model zeroes or fixes terms, transmittance is clamped to
-// [0, 1], the ambient colour is halved; coloured_mapper_shader without
-// texture maps shades as phong_shader.
-rt_material finish_mtl(MtlState s) {
+// the illumination model zeroes or fixes terms and texture maps,
+// transmittance is clamped to [0, 1], the ambient colour is halved; an
+// untextured coloured_mapper_shader shades as phong_shader.
+rt_material finish_mtl(MtlState s, Mesh *mesh) {
     switch (s.illum) {
         case 0:
+            s.maps[0] = s.maps[1];
+            for (int k = 1; k < 5; ++k) s.maps[k].clear();
             s.ka = s.kd;
             s.kd = P3{};
             s.ks = P3{};
@@ -606,12 +740,16 @@ rt_material finish_mtl(MtlState s) {
             s.rf = 0.0f;
             break;
         case 1:
+            s.maps[2].clear();
+            s.maps[3].clear();
+            s.maps[4].clear();
             s.ks = P3{};
             s.ns = 0.0f;
             s.ri = 1.0f;
             s.rf = 0.0f;
             break;
         case 2:
+            s.maps[4].clear();
             s.ri = 1.0f;
             s.rf = 0.0f;
             break;
@@ -645,6 +783,7 @@ rt_material finish_mtl(MtlState s) {
     m.reflectance = s.rf;
     m.kind = RT_MATERIAL_COLOURED;
     m.chroma[0] = m.chroma[1] = 0.33f;
+    for (int k = 0; k < 6; ++k) m.tex[k] = s.maps[k].empty() ? 0 : mtl_map(mesh, s.maps[k], k == 5);
     return m;
 }
 
@@ -659,7 +798,7 @@ std::string read_file(const std::string &path) {
 // Parses one MTL file into named materials; returns the id of the last one
 // defined (which the reference makes current for the faces that follow), or
 // -1 when the file defines none.
-int32_t parse_mtl(const std::string &text, Mesh *m, std::map<std::string, int32_t> *named) {
+int32_t parse_mtl(const std::string &text, const std::string &dir, Mesh *m, std::map<std::string, int32_t> *named) {
     Cursor c{text.data(), text.data() + text.size()};
     const char *b, *e;
     std::string name, t;
@@ -669,10 +808,9 @@ int32_t parse_mtl(const std::string &text, Mesh *m, std::map<std::string, int32_
     auto close = [&]() {
         if (name.empty()) return;
         s.illum = illum;
-        m->mats.push_back(finish_mtl(s));
+        m->mats.push_back(finish_mtl(s, m));
         last = static_cast<int32_t>(m->mats.size() - 1);
         (*named)[name] = last;
-        if (s.textured) ++m->texture_maps;
     };
     while (next_line(&c, &b, &e)) {
         Tokens tk{b, e};
@@ -704,8 +842,12 @@ int32_t parse_mtl(const std::string &text, Mesh *m, std::map<std::string, int32_
             illum = tk.next(&v) ? std::atoi(v.c_str()) : 0;
         } else if (starts(t, "map_Ka") || starts(t, "map_Kd") || starts(t, "map_Ks") || starts(t, "map_refl") ||
                    starts(t, "map_d")) {
-            // Texture maps are not applied; the material shades untextured.
-            s.textured = true;
+            // obj_parser.cc:507-561: the file, relative to the MTL's directory.
+            std::string file;
+            if (!tk.next(&file)) bad(t + " without a file");
+            const int slot = starts(t, "map_Ka") ? 0 : starts(t, "map_Kd") ? 1 : starts(t, "map_Ks") ? 2
+                           : starts(t, "map_refl") ? 4 : 5;
+            s.maps[slot] = dir + file;
         } else if (starts(t, "map_bump") || starts(t, "bump")) {
             // bump maps are ignored by the reference too
         } else {
@@ -727,13 +869,13 @@ void parse_obj(const char *data, int64_t len, const std::string &dir, Mesh *m) {
     Cursor c{data, data + len};
     const char *b, *e;
     std::vector<P3> v, vn;
-    int64_t nvt = 0;
+    std::vector<float> vts;  // u, v of every vt statement
     std::map<std::string, int32_t> named;
     m->mats.push_back(phong(0.0f, 200.0f, 0.0f));
     int32_t cur = 0;
     std::string t;
     std::vector<P3> face, face_n;
-    int64_t face_t = 0;
+    std::vector<float> face_t;
     while (next_line(&c, &b, &e)) {
         Tokens tk{b, e};
         if (!tk.next(&t) || t[0] == '#') continue;
@@ -741,7 +883,7 @@ void parse_obj(const char *data, int64_t len, const std::string &dir, Mesh *m) {
             // The current material becomes the last one the last library
             // defines (none: faces need a usemtl before they shade).
             std::string file;
-            while (tk.next(&file)) cur = parse_mtl(read_file(dir + file), m, &named);
+            while (tk.next(&file)) cur = parse_mtl(read_file(dir + file), dir, m, &named);
         } else if (starts(t, "usemtl")) {
             std::string name;
             if (!tk.next(&name)) bad("usemtl without a name");
@@ -758,11 +900,13 @@ void parse_obj(const char *data, int64_t len, const std::string &dir, Mesh *m) {
             q.z = next_float(&tk, "vn z");
             vn.push_back(normalised(q));
         } else if (starts(t, "vt")) {
-            ++nvt;
+            // parse_vt_statement (obj_parser.cc:732-743): u and v.
+            vts.push_back(next_float(&tk, "vt u"));
+            vts.push_back(next_float(&tk, "vt v"));
         } else if (t[0] == 'f') {
             face.clear();
             face_n.clear();
-            face_t = 0;
+            face_t.clear();
             std::string corner;
             while (tk.next(&corner)) {
                 // v, v/vt, v/vt/vn or v//vn
@@ -772,25 +916,27 @@ void parse_obj(const char *data, int64_t len, const std::string &dir, Mesh *m) {
                 const size_t s2 = corner.find('/', s1 + 1);
                 const std::string vt_s = corner.substr(s1 + 1, s2 == std::string::npos ? std::string::npos : s2 - s1 - 1);
                 if (!vt_s.empty()) {
-                    obj_index(to_int(vt_s, "a texture index"), static_cast<size_t>(nvt), "texture");
-                    ++face_t;
+                    const size_t k = obj_index(to_int(vt_s, "a texture index"), vts.size() / 2, "texture");
+                    face_t.push_back(vts[2 * k]);
+                    face_t.push_back(vts[2 * k + 1]);
                 }
                 if (s2 != std::string::npos)
                     face_n.push_back(vn[obj_index(to_int(corner.substr(s2 + 1), "a normal index"), vn.size(), "normal")]);
             }
             if (!face_n.empty() && face_n.size() != face.size()) bad("OBJ face gives normals for some corners only");
-            if (face_t != 0 && face_t != static_cast<int64_t>(face.size()))
+            if (!face_t.empty() && face_t.size() != 2 * face.size())
                 bad("OBJ face gives texture coordinates for some corners only");
             ++m->faces;
             if (cur < 0) bad("OBJ face without a material (its material library defines none)");
             if (face.size() == 3) {
                 if (obj_triangle_ok(face[0], face[1], face[2])) {
-                    m->add(face[0], face[1], face[2], cur, face_n.empty() ? nullptr : face_n.data());
+                    m->add(face[0], face[1], face[2], cur, face_n.empty() ? nullptr : face_n.data(),
+                           face_t.empty() ? nullptr : face_t.data());
                 } else {
                     ++m->skipped_faces;
                 }
             } else {
-                add_polygon(m, face, &face_n, cur);
+                add_polygon(m, face, &face_n, cur, &face_t);
             }
         } else if (t[0] == 'v') {
             P3 q;
@@ -1047,7 +1193,49 @@ int32_t rt_mesh_info_get(const rt_mesh *mesh, rt_mesh_info *info) {
     info->texture_maps = m.texture_maps;
     info->lights = static_cast<int32_t>(m.lights.size());
     info->has_camera = m.has_camera ? 1 : 0;
-    info->reserved = 0;
+    info->has_texture_coords = m.any_vt ? 1 : 0;
+    info->maps = static_cast<int32_t>(m.maps.size());
+    info->images = static_cast<int32_t>(m.images.size());
+    info->texels = static_cast<int64_t>(m.texels.size());
+    return RT_OK;
+}
+
+int32_t rt_mesh_copy_textures(const rt_mesh *mesh, float *vt, rt_texture_map *maps, rt_texture_image *images,
+                              float *texels) {
+    if (!mesh) return RT_ERR_INVALID;
+    const Mesh &m = mesh->m;
+    if (vt && m.any_vt) std::memcpy(vt, m.vt.data(), m.vt.size() * sizeof(float));
+    if (maps && !m.maps.empty()) std::memcpy(maps, m.maps.data(), m.maps.size() * sizeof(rt_texture_map));
+    if (images && !m.images.empty())
+        std::memcpy(images, m.images.data(), m.images.size() * sizeof(rt_texture_image));
+    if (texels && !m.texels.empty()) std::memcpy(texels, m.texels.data(), m.texels.size() * sizeof(float));
+    return RT_OK;
+}
+
+int32_t rt_texture_load_tga(const char *path, float *texels, int64_t capacity, int32_t *dims) {
+    if (!path || !dims) return RT_ERR_INVALID;
+    g_error.clear();
+    try {
+        int w, h, cpp;
+        std::vector<float> px;
+        decode_tga(read_file(path), &w, &h, &cpp, &px);
+        dims[0] = w;
+        dims[1] = h;
+        dims[2] = cpp;
+        if (!texels) return RT_OK;
+        if (capacity < static_cast<int64_t>(px.size())) {
+            g_error = "rt_texture_load_tga: " + std::to_string(px.size()) + " texels do not fit in " +
+                      std::to_string(capacity);
+            return RT_ERR_INVALID;
+        }
+        std::memcpy(texels, px.data(), px.size() * sizeof(float));
+    } catch (const ParseError &pe) {
+        g_error = "rt_texture_load_tga: " + pe.what;
+        return RT_ERR_INVALID;
+    } catch (const std::bad_alloc &) {
+        g_error = "rt_texture_load_tga: host allocation";
+        return RT_ERR_OOM;
+    }
     return RT_OK;
 }
 
@@ -1101,8 +1289,14 @@ int32_t rt_scene_create_from_mesh(rt_ctx *ctx, const rt_mesh *mesh, rt_scene **o
     rt_scene *sc = nullptr;
     int32_t rc = rt_scene_create(ctx, m.abc.data(), albedo.data(), flags.data(), n, &sc);
     if (rc != RT_OK) return rc;
-    rc = rt_scene_set_materials(sc, m.mats.data(), static_cast<int32_t>(m.mats.size()), m.tri_mat.data());
+    if (!m.maps.empty())
+        rc = rt_scene_set_textures(sc, m.maps.data(), static_cast<int32_t>(m.maps.size()), m.images.data(),
+                                   static_cast<int32_t>(m.images.size()), m.texels.data(),
+                                   static_cast<int64_t>(m.texels.size()), nullptr, 0);
+    if (rc == RT_OK)
+        rc = rt_scene_set_materials(sc, m.mats.data(), static_cast<int32_t>(m.mats.size()), m.tri_mat.data());
     if (rc == RT_OK && m.any_vn) rc = rt_scene_set_vertex_normals(sc, m.vn.data());
+    if (rc == RT_OK && m.any_vt) rc = rt_scene_set_texture_coords(sc, m.vt.data());
     if (rc != RT_OK) {
         rt_scene_destroy(sc);
         return rc;

@@ -33,6 +33,7 @@
 #include "frame.hpp"
 #include "radix_sort.hpp"
 #include "rt_device.hpp"
+#include "textures.hpp"
 
 namespace rt {
 
@@ -43,16 +44,24 @@ namespace {
 constexpr double kMinReflectivePower = 0.1;
 
 constexpr int kCookTorrance = 1;
+constexpr int kColoured = 2;
+constexpr int kMapper = 3;
 
 // A material row (kMaterialFloat4s float4): {ka rgb, shininess},
 // {kd rgb, transmittance}, {ks rgb, refractive index},
-// {reflectance, kind, chroma x, chroma y}, {diffuse, roughness, extinction, 0}.
+// {reflectance, kind, chroma x, chroma y}, {diffuse, roughness, extinction, 0},
+// {tex ka, kd, ks, ns}, {tex refl, d, mapper kd, mapper ks},
+// {clrh, clrf, layers 0, layers 1}, {layers 2, 3, 4, 0}; texture slots are
+// 1 + map index (0 none) and layer lists first << 8 | count, as int bits.
 // Cook-Torrance keeps rs, ts, ri_r where phong keeps rf, tran, ri.
 struct Mat {
     float4 ka_s, kd_tran, ks_ri;
     float rf;
     int kind;
     float cx, cy, rd, sr;
+    int tex[6];
+    float mkd, mks, clrh, clrf;
+    int layers[5];
 };
 
 // The triangle's material: its table entry, or phong_shader(255 * albedo, 1)
@@ -68,6 +77,9 @@ __device__ __forceinline__ Mat mat_of(const float4 *mats, const int32_t *tri_mat
         m.rf = 0.0f;
         m.kind = 0;
         m.cx = m.cy = m.rd = m.sr = 0.0f;
+        for (int k = 0; k < 6; ++k) m.tex[k] = 0;
+        m.mkd = m.mks = m.clrh = m.clrf = 0.0f;
+        for (int k = 0; k < 5; ++k) m.layers[k] = 0;
     } else {
         const float4 *q = mats + kMaterialFloat4s * static_cast<int64_t>(mi);
         m.ka_s = q[0];
@@ -80,8 +92,82 @@ __device__ __forceinline__ Mat mat_of(const float4 *mats, const int32_t *tri_mat
         m.cy = q3.w;
         m.rd = q4.x;
         m.sr = q4.y;
+        const float4 q5 = q[5], q6 = q[6], q7 = q[7], q8 = q[8];
+        m.tex[0] = __float_as_int(q5.x);
+        m.tex[1] = __float_as_int(q5.y);
+        m.tex[2] = __float_as_int(q5.z);
+        m.tex[3] = __float_as_int(q5.w);
+        m.tex[4] = __float_as_int(q6.x);
+        m.tex[5] = __float_as_int(q6.y);
+        m.mkd = q6.z;
+        m.mks = q6.w;
+        m.clrh = q7.x;
+        m.clrf = q7.y;
+        m.layers[0] = __float_as_int(q7.z);
+        m.layers[1] = __float_as_int(q7.w);
+        m.layers[2] = __float_as_int(q8.x);
+        m.layers[3] = __float_as_int(q8.y);
+        m.layers[4] = __float_as_int(q8.z);
     }
     return m;
+}
+
+// Where a hit samples its textures: the (re-projected) hit point and the
+// interpolated texture coordinates, if the triangle has them.
+struct TexPoint {
+    rtd::V3 dst;
+    bool has_vt;
+    float vtu, vtv;
+};
+
+// A monochrome layer list of mapper_shader (rt/materials/mapper_shader.cc:
+// 41-66, 89-126): each layer sets the value to its texel mean / 255; the
+// list stops once the product of the opacities is 0.
+__device__ __forceinline__ float mono_layers(const TexTables &T, int list, float value, const TexPoint &p) {
+    const int first = list >> 8, count = list & 0xff;
+    float alpha = 1.0f;
+    for (int k = 0; k < count; ++k) {
+        float z;
+        alpha *= texture_mono(T, T.layers[first + k], &z, p.dst, p.has_vt, p.vtu, p.vtv);
+        value = z * (1.0f / 255.0f);
+        if (alpha == 0.0f) break;
+    }
+    return value;
+}
+
+// The mirror and refraction coefficients a hit's rays are spawned with:
+// coloured_mapper_shader scales rf and tran by red / 255 of t_refl and t_d
+// (rt/materials/coloured_mapper_shader.cc:8-38); mapper_shader takes its
+// reflectance and transmittance layers (rt/materials/mapper_shader.cc:41-66).
+__device__ __forceinline__ void child_coefficients(const Mat &m, const TexTables &T, const TexPoint &p, float *refl,
+                                                   float *tran) {
+    *refl = m.rf;
+    *tran = m.kd_tran.w;
+    if (m.kind == kColoured) {
+        if (m.tex[4]) {
+            TexColour t{0.0f, 0.0f, 0.0f};
+            texture_colour(T, m.tex[4] - 1, &t, p.dst, p.has_vt, p.vtu, p.vtv);
+            *refl = t.r * (m.rf * (1.0f / 255.0f));
+        }
+        if (m.tex[5]) {
+            TexColour t{0.0f, 0.0f, 0.0f};
+            texture_colour(T, m.tex[5] - 1, &t, p.dst, p.has_vt, p.vtu, p.vtv);
+            *tran = t.r * (m.kd_tran.w * (1.0f / 255.0f));
+        }
+    } else if (m.kind == kMapper) {
+        *refl = mono_layers(T, m.layers[3], m.rf, p);
+        *tran = mono_layers(T, m.layers[4], m.kd_tran.w, p);
+    }
+}
+
+// A coloured_mapper_shader colour: the map's colour times colour / 255
+// (rt/materials/coloured_mapper_shader.cc:58-80).
+__device__ __forceinline__ float4 mapped_colour(const TexTables &T, int slot, float4 colour, const TexPoint &p) {
+    if (!slot) return colour;
+    TexColour t{colour.x, colour.y, colour.z};
+    texture_colour(T, slot - 1, &t, p.dst, p.has_vt, p.vtu, p.vtv);
+    return make_float4(t.r * (colour.x * (1.0f / 255.0f)), t.g * (colour.y * (1.0f / 255.0f)),
+                       t.b * (colour.z * (1.0f / 255.0f)), colour.w);
 }
 
 // XYZ -> RGB for the primaries and white point of
@@ -173,6 +259,7 @@ struct GenArgs {
     const float4 *tri_n, *tri_c, *kd, *mats;
     const int32_t *tri_mat;
     const float *vn;  // vertex normals, 9 per triangle, or null
+    TexTables tex;    // texture maps and coordinates (maps null: none)
 };
 
 // Padding slots past the image edge get tmax < tmin, so they never hit.
@@ -195,10 +282,13 @@ __global__ void __launch_bounds__(256) k_mat_raygen(CamArgs c, TileMap tm, int s
     rays[2 * i + 1] = make_float4(d.x, d.y, d.z, rtd::kMaxDist);
 }
 
-// The surface at ray i's hit and which children it spawns.
+// The surface at ray i's hit and which children it spawns, with the
+// coefficients (possibly textured) the children are spawned with.
 struct HitSurface {
     rtd::V3 dst, geo, shn, rdir;
     Mat m;
+    TexPoint tp;
+    float refl_c, tran_c;
     bool refl, refr;
 };
 
@@ -221,11 +311,73 @@ __device__ __forceinline__ HitSurface surface_of(const GenArgs &a, int64_t i) {
         h.shn = leaving ? rtd::neg(s) : s;
     }
     h.m = mat_of(a.mats, a.tri_mat, a.kd, prim);
+    h.tp.dst = h.dst;
+    h.tp.has_vt = hit_texture_coords(a.tex, prim, hit.y, hit.z, &h.tp.vtu, &h.tp.vtv);
+    if (a.tex.maps) {
+        child_coefficients(h.m, a.tex, h.tp, &h.refl_c, &h.tran_c);
+    } else {
+        h.refl_c = h.m.rf;
+        h.tran_c = h.m.kd_tran.w;
+    }
+    // ray::reflect / ray::refract spawn a ray while power x coefficient
+    // stays above MIN_REFLECTIVE_POWER (rt/ray.cc:121-128, 200-206).
     const float power = a.power[i];
-    h.refl = h.m.rf > 0.0f && static_cast<double>(power * h.m.rf) > kMinReflectivePower;
-    h.refr = h.m.kd_tran.w > 0.0f && static_cast<double>(power * h.m.kd_tran.w) > kMinReflectivePower &&
+    h.refl = h.refl_c > 0.0f && static_cast<double>(power * h.refl_c) > kMinReflectivePower;
+    h.refr = h.tran_c > 0.0f && static_cast<double>(power * h.tran_c) > kMinReflectivePower &&
              refract_dir(d, h.shn, h.m.ks_ri.w, leaving, &h.rdir);
     return h;
+}
+
+// The shading record of a hit, its material resolved at the hit:
+// r0 {ka rgb, shininess}, r1 {kd rgb (mapper: base colour), mapper kd},
+// r2 {ks rgb (mapper: ks, clrh, 0), reflected-colour weight},
+// r3 {refracted-colour weight, clrf, kind, 0}.
+struct ShadeRec {
+    float4 r0, r1, r2, r3;
+};
+
+// coloured_mapper_shader::shade's colours and shininess
+// (rt/materials/coloured_mapper_shader.cc:41-80) and mapper_shader::shade's
+// colour, kd and s layers (rt/materials/mapper_shader.cc:84-126); phong and
+// Cook-Torrance take their fields. The children's colours come back weighted
+// by rf and tran, except mapper_shader's, weighted by its layered values.
+__device__ __forceinline__ ShadeRec shade_record(const HitSurface &h, const TexTables &T) {
+    const Mat &m = h.m;
+    ShadeRec r;
+    const float kind = static_cast<float>(m.kind);
+    if (m.kind == kMapper) {
+        float4 rgb = make_float4(m.kd_tran.x, m.kd_tran.y, m.kd_tran.z, 0.0f);
+        const int first = m.layers[0] >> 8, count = m.layers[0] & 0xff;
+        float alpha = 1.0f;
+        TexColour c{rgb.x, rgb.y, rgb.z};
+        for (int k = 0; k < count; ++k) {
+            alpha *= texture_colour(T, T.layers[first + k], &c, h.tp.dst, h.tp.has_vt, h.tp.vtu, h.tp.vtv);
+            if (alpha == 0.0f) break;
+        }
+        const float kd = mono_layers(T, m.layers[1], m.mkd, h.tp);
+        const float sh = mono_layers(T, m.layers[2], m.ka_s.w, h.tp);
+        r.r0 = make_float4(0.0f, 0.0f, 0.0f, sh);
+        r.r1 = make_float4(c.r, c.g, c.b, kd);
+        r.r2 = make_float4(m.mks, m.clrh, 0.0f, h.refl_c);
+        r.r3 = make_float4(h.tran_c, m.clrf, kind, 0.0f);
+        return r;
+    }
+    float4 ka = m.ka_s, kd = m.kd_tran, ks = m.ks_ri;
+    if (m.kind == kColoured && T.maps) {
+        if (m.tex[3]) {
+            TexColour dummy{ka.x, ka.y, ka.z};
+            const float alpha = texture_colour(T, m.tex[3] - 1, &dummy, h.tp.dst, h.tp.has_vt, h.tp.vtu, h.tp.vtv);
+            ka.w = alpha * (m.ka_s.w * (1.0f / 255.0f));
+        }
+        ka = mapped_colour(T, m.tex[0], ka, h.tp);
+        kd = mapped_colour(T, m.tex[1], kd, h.tp);
+        ks = mapped_colour(T, m.tex[2], ks, h.tp);
+    }
+    r.r0 = make_float4(ka.x, ka.y, ka.z, ka.w);
+    r.r1 = make_float4(kd.x, kd.y, kd.z, 0.0f);
+    r.r2 = make_float4(ks.x, ks.y, ks.z, m.rf);
+    r.r3 = make_float4(m.kd_tran.w, 0.0f, kind, 0.0f);
+    return r;
 }
 
 // flags[i] = 1 for a hit, kids[i] = its number of children; entry n of both
@@ -251,6 +403,7 @@ struct EmitArgs {
     float4 *rec_n;      // per hit: shading normal, triangle (int bits)
     float4 *rec_d;      // per hit: incoming direction
     int2 *rec_kids;     // per hit: first child slot, children (bit 0 mirror, bit 1 refraction)
+    ShadeRec *rec_m;    // per hit: the resolved material
     float4 *shadow;     // per hit x light: 2 float4
     float4 *next_rays;  // children, 2 float4 each
     float *next_power;
@@ -268,6 +421,7 @@ __global__ void __launch_bounds__(256) k_mat_emit(EmitArgs e) {
     e.rec_d[q] = make_float4(r1.x, r1.y, r1.z, 0.0f);
     e.rec_kids[q] = make_int2(static_cast<int32_t>(c0),
                               static_cast<int32_t>(h.refl) | (static_cast<int32_t>(h.refr) << 1));
+    e.rec_m[q] = shade_record(h, a.tex);
     // Shadow rays and the mirror ray start off the front of the surface.
     const rtd::V3 org = rtd::offset_start_point(h.dst, h.geo, 1);
     for (int l = 0; l < e.lights.n; ++l) {
@@ -284,7 +438,7 @@ __global__ void __launch_bounds__(256) k_mat_emit(EmitArgs e) {
         const rtd::V3 rd = reflect_dir(rtd::f4v(r1), h.shn);
         e.next_rays[2 * c] = make_float4(org.x, org.y, org.z, rtd::kEpsilon);
         e.next_rays[2 * c + 1] = make_float4(rd.x, rd.y, rd.z, rtd::kMaxDist);
-        e.next_power[c] = power * h.m.rf;
+        e.next_power[c] = power * h.refl_c;
         ++c;
     }
     if (h.refr) {
@@ -292,12 +446,13 @@ __global__ void __launch_bounds__(256) k_mat_emit(EmitArgs e) {
         const rtd::V3 o2 = rtd::offset_start_point(h.dst, h.geo, -1);
         e.next_rays[2 * c] = make_float4(o2.x, o2.y, o2.z, rtd::kEpsilon);
         e.next_rays[2 * c + 1] = make_float4(h.rdir.x, h.rdir.y, h.rdir.z, rtd::kMaxDist);
-        e.next_power[c] = power * h.m.kd_tran.w;
+        e.next_power[c] = power * h.tran_c;
     }
 }
 
 struct LocalArgs {
     const float4 *rec_n, *rec_d;
+    const ShadeRec *rec_m;
     const float4 *shadow;
     const uint8_t *occ;
     int64_t nhits;
@@ -308,16 +463,22 @@ struct LocalArgs {
 };
 
 // The material's shade without the secondary rays, with the light intensity
-// of rt/light.h:44-59: phong_shader (rt/materials/phong_shader.cc:21-64),
-// diffuse and specular per light, then the ambient term; or
-// cook_torrance_cxy (rt/materials/cook_torrance_cxy.cc:33-76) per light.
+// of rt/light.h:44-59: phong_shader (rt/materials/phong_shader.cc:21-64) and
+// coloured_mapper_shader (rt/materials/coloured_mapper_shader.cc:82-126),
+// diffuse and specular per light, then the ambient term; cook_torrance_cxy
+// (rt/materials/cook_torrance_cxy.cc:33-76) per light; mapper_shader
+// (rt/materials/mapper_shader.cc:128-170) per light, no ambient term.
 __global__ void __launch_bounds__(256) k_mat_local(LocalArgs a) {
     const int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (q >= a.nhits) return;
     const float4 nn = a.rec_n[q];
     const rtd::V3 n = rtd::f4v(nn), d = rtd::f4v(a.rec_d[q]);
-    const Mat m = mat_of(a.mats, a.tri_mat, a.kd, __float_as_int(nn.w));
-    const bool ct = m.kind == kCookTorrance;
+    const ShadeRec rm = a.rec_m[q];
+    const int kind = static_cast<int>(rm.r3.z);
+    const bool ct = kind == kCookTorrance;
+    const bool mapper = kind == kMapper;
+    Mat m;
+    if (ct) m = mat_of(a.mats, a.tri_mat, a.kd, __float_as_int(nn.w));
     const CieMatrix cm = cie_matrix();
     const float nv = rtd::dot(n, d);  // cook_torrance_cxy: one dot product for every light
     Rgb c{0.0f, 0.0f, 0.0f};
@@ -326,13 +487,31 @@ __global__ void __launch_bounds__(256) k_mat_local(LocalArgs a) {
         const float4 sd = a.shadow[2 * k + 1];
         const rtd::V3 dn = rtd::f4v(sd);
         const float magn = (a.occ[k] ? 0.0f : 1.0f) / 1.0f;
+        const float dist = sd.w * a.lights.pos_d[l].w;
+        const float d_scale = (dist * dist) + 1.0f;
+        const float4 lrgb = a.lights.rgb[l];
+        if (mapper) {
+            const float diff_shade = rtd::dot(dn, n) * rm.r1.w;
+            if (diff_shade < 0.0f) continue;
+            const float shade_x2 = 2.0f * diff_shade;
+            const rtd::V3 ref_dir = rtd::mk(dn.x - n.x * shade_x2, dn.y - n.y * shade_x2, dn.z - n.z * shade_x2);
+            const float rdr = rtd::dot(d, ref_dir);
+            float spec = 0.0f;
+            if (rdr > 0.0f) spec = powf(rdr, rm.r0.w) * rm.r2.x;
+            const float lr = lrgb.x / d_scale, lg = lrgb.y / d_scale, lb = lrgb.z / d_scale;
+            c.r += ((rm.r1.x * diff_shade) * lr) * magn;
+            c.g += ((rm.r1.y * diff_shade) * lg) * magn;
+            c.b += ((rm.r1.z * diff_shade) * lb) * magn;
+            const float clrh = rm.r2.y, keep = 1.0f - clrh;
+            c.r += (((clrh * rm.r1.x) + (keep * lr)) * spec) * magn;
+            c.g += (((clrh * rm.r1.y) + (keep * lg)) * spec) * magn;
+            c.b += (((clrh * rm.r1.z) + (keep * lb)) * spec) * magn;
+            continue;
+        }
         const float shade = rtd::dot(dn, n);
         if (shade < 0.0f) continue;
         if (ct) {
-            const float dist = sd.w * a.lights.pos_d[l].w;
-            const float d_scale = (dist * dist) + 1.0f;
-            const float4 rgb = a.lights.rgb[l];
-            const float4 li = make_float4(rgb.x / d_scale, rgb.y / d_scale, rgb.z / d_scale, 0.0f);
+            const float4 li = make_float4(lrgb.x / d_scale, lrgb.y / d_scale, lrgb.z / d_scale, 0.0f);
             cook_torrance_term(m, cm, n, d, nv, dn, shade, magn, li, c);
             continue;
         }
@@ -340,24 +519,21 @@ __global__ void __launch_bounds__(256) k_mat_local(LocalArgs a) {
         const float shade_x2 = 2.0f * shade;
         const rtd::V3 ref_dir = rtd::mk(dn.x - n.x * shade_x2, dn.y - n.y * shade_x2, dn.z - n.z * shade_x2);
         const float rdr = rtd::dot(d, ref_dir);
-        float scr = shade * m.kd_tran.x, scg = shade * m.kd_tran.y, scb = shade * m.kd_tran.z;
+        float scr = shade * rm.r1.x, scg = shade * rm.r1.y, scb = shade * rm.r1.z;
         if (rdr > 0.0f) {
-            const float sp = powf(rdr, m.ka_s.w);
-            scr += sp * m.ks_ri.x;
-            scg += sp * m.ks_ri.y;
-            scb += sp * m.ks_ri.z;
+            const float sp = powf(rdr, rm.r0.w);
+            scr += sp * rm.r2.x;
+            scg += sp * rm.r2.y;
+            scb += sp * rm.r2.z;
         }
-        const float dist = sd.w * a.lights.pos_d[l].w;
-        const float d_scale = (dist * dist) + 1.0f;
-        const float4 rgb = a.lights.rgb[l];
-        c.r = c.r + ((rgb.x / d_scale) * (scr * magn));
-        c.g = c.g + ((rgb.y / d_scale) * (scg * magn));
-        c.b = c.b + ((rgb.z / d_scale) * (scb * magn));
+        c.r = c.r + ((lrgb.x / d_scale) * (scr * magn));
+        c.g = c.g + ((lrgb.y / d_scale) * (scg * magn));
+        c.b = c.b + ((lrgb.z / d_scale) * (scb * magn));
     }
-    if (!ct) {  // cook_torrance_cxy has no ambient term
-        c.r += m.ka_s.x;
-        c.g += m.ka_s.y;
-        c.b += m.ka_s.z;
+    if (!ct && !mapper) {  // cook_torrance_cxy and mapper_shader have no ambient term
+        c.r += rm.r0.x;
+        c.g += rm.r0.y;
+        c.b += rm.r0.z;
     }
     a.colour[q] = make_float4(c.r, c.g, c.b, 0.0f);
 }
@@ -365,10 +541,8 @@ __global__ void __launch_bounds__(256) k_mat_local(LocalArgs a) {
 struct GatherArgs {
     float4 *colour;  // this generation's hits: local colour in, full colour out
     const int2 *rec_kids;
-    const float4 *rec_n;  // triangle of each hit in w
+    const ShadeRec *rec_m;
     int64_t nhits;
-    const float4 *kd, *mats;
-    const int32_t *tri_mat;
     const uint32_t *child_hit_pos;  // next generation: exclusive scan of its hit flags
     const float4 *child_colour;     // next generation: full colour of its hits
     float4 bg;                      // the camera's background colour
@@ -382,28 +556,40 @@ __device__ __forceinline__ float4 child_colour(const GatherArgs &a, int64_t j) {
 }
 
 // rt/materials/phong_shader.cc:66-111: c += reflected * (rf / 1), then
-// c += refracted * (tran / 1).
+// c += refracted * (tran / 1); mapper_shader (rt/materials/mapper_shader.cc:
+// 172-184): c += reflected * refl, then c += clrf c + (1 - clrf) refracted
+// trans.
 __global__ void __launch_bounds__(256) k_mat_gather(GatherArgs a) {
     const int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (q >= a.nhits) return;
     const int2 kids = a.rec_kids[q];
     if (kids.y == 0) return;
-    const Mat m = mat_of(a.mats, a.tri_mat, a.kd, __float_as_int(a.rec_n[q].w));
+    const ShadeRec rm = a.rec_m[q];
     float4 c = a.colour[q];
     int64_t j = kids.x;
     if (kids.y & 1) {
         const float4 p = child_colour(a, j++);
-        const float w = m.rf / 1.0f;
+        const float w = rm.r2.w / 1.0f;
         c.x += p.x * w;
         c.y += p.y * w;
         c.z += p.z * w;
     }
     if (kids.y & 2) {
         const float4 p = child_colour(a, j);
-        const float w = m.kd_tran.w / 1.0f;
-        c.x += p.x * w;
-        c.y += p.y * w;
-        c.z += p.z * w;
+        const float w = rm.r3.x / 1.0f;
+        if (static_cast<int>(rm.r3.z) == kMapper) {
+            const float clrf = rm.r3.y, keep = 1.0f - clrf;
+            const float tx = (clrf * c.x) + ((keep * p.x) * w);
+            const float ty = (clrf * c.y) + ((keep * p.y) * w);
+            const float tz = (clrf * c.z) + ((keep * p.z) * w);
+            c.x += tx;
+            c.y += ty;
+            c.z += tz;
+        } else {
+            c.x += p.x * w;
+            c.y += p.y * w;
+            c.z += p.z * w;
+        }
     }
     a.colour[q] = c;
 }
@@ -461,6 +647,7 @@ struct Gen {
     uint32_t *hit_pos = nullptr;
     float4 *colour = nullptr, *rec_n = nullptr;
     int2 *rec_kids = nullptr;
+    ShadeRec *rec_m = nullptr;
 };
 
 // The power of a ray falls by a factor of at least 0.95 (the largest rf and
@@ -483,6 +670,7 @@ int32_t render_frame_materials(rt_accel *a, const rt_camera &cam, const rt_light
     const int64_t ns = f.tiles * 64 * f.spp;
     if (ns + 1 > int64_t(kScanTile) * kScanTile) return fail(ctx, RT_ERR_INVALID, "rt_render: frame too large");
     const float4 background = make_float4(p.background[0], p.background[1], p.background[2], 0.0f);
+    const TexTables tex{sc->d_maps, sc->d_texels, sc->d_layers, sc->d_vt};
     BuildArena &ar = ctx->arena;
     RT_HIP_CHECK(ctx, ar.begin());
     auto take = [&](auto **ptr, int64_t count) -> int32_t {
@@ -519,7 +707,8 @@ int32_t render_frame_materials(rt_accel *a, const rt_camera &cam, const rt_light
                                  first ? 0 : 2)))
             return rc;
         if (timed) RT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[2], s));
-        const GenArgs g{rays, power, hits, n, sc->d_n, sc->d_c, sc->d_kd, sc->d_mat, sc->d_tri_mat, sc->d_vn};
+        const GenArgs g{rays,      power,          hits,     n,        sc->d_n, sc->d_c,
+                        sc->d_kd,  sc->d_mat,      sc->d_tri_mat, sc->d_vn, tex};
         hipLaunchKernelGGL(k_mat_classify, grid_of(n + 1), dim3(256), 0, s, g, gen.hit_pos, kids);
         RT_HIP_CHECK(ctx, scan_u32(gen.hit_pos, n + 1, sums, s));
         RT_HIP_CHECK(ctx, scan_u32(kids, n + 1, sums, s));
@@ -542,11 +731,12 @@ int32_t render_frame_materials(rt_accel *a, const rt_camera &cam, const rt_light
             float4 *rec_d = nullptr, *shadow = nullptr;
             uint8_t *occ = nullptr;
             if ((rc = take(&gen.rec_n, nh)) || (rc = take(&rec_d, nh)) || (rc = take(&gen.rec_kids, nh)) ||
+                (rc = take(&gen.rec_m, nh)) ||
                 (rc = take(&gen.colour, nh)) || (rc = take(&shadow, 2 * nh * nl)) || (rc = take(&occ, nh * nl)) ||
                 (rc = take(&next_rays, 2 * nk)) || (rc = take(&next_power, nk)))
                 return rc;
-            const EmitArgs e{g, gen.hit_pos, kids, f.lights, gen.rec_n, rec_d, gen.rec_kids, shadow, next_rays,
-                             next_power};
+            const EmitArgs e{g,      gen.hit_pos, kids,      f.lights,  gen.rec_n, rec_d,
+                             gen.rec_kids, gen.rec_m, shadow, next_rays, next_power};
             hipLaunchKernelGGL(k_mat_emit, grid_of(n), dim3(256), 0, s, e);
             if (nl > 0) {
                 if (timed) RT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[3], s));
@@ -556,8 +746,8 @@ int32_t render_frame_materials(rt_accel *a, const rt_camera &cam, const rt_light
                     return rc;
                 if (timed) RT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[4], s));
             }
-            const LocalArgs la{gen.rec_n, rec_d, shadow, occ, nh, f.lights, sc->d_kd, sc->d_mat, sc->d_tri_mat,
-                               gen.colour};
+            const LocalArgs la{gen.rec_n, rec_d,     gen.rec_m, shadow,        occ,       nh,
+                               f.lights,  sc->d_kd, sc->d_mat, sc->d_tri_mat, gen.colour};
             hipLaunchKernelGGL(k_mat_local, grid_of(nh), dim3(256), 0, s, la);
             RT_HIP_CHECK(ctx, hipGetLastError());
             if (timed && nl > 0) {
@@ -576,8 +766,7 @@ int32_t render_frame_materials(rt_accel *a, const rt_camera &cam, const rt_light
     for (int k = static_cast<int>(gens.size()) - 2; k >= 0; --k) {
         const Gen &g = gens[k], &c = gens[k + 1];
         if (g.nhits == 0) continue;
-        const GatherArgs ga{g.colour, g.rec_kids, g.rec_n,   g.nhits,  sc->d_kd, sc->d_mat,
-                            sc->d_tri_mat, c.hit_pos, c.colour, background};
+        const GatherArgs ga{g.colour, g.rec_kids, g.rec_m, g.nhits, c.hit_pos, c.colour, background};
         hipLaunchKernelGGL(k_mat_gather, grid_of(g.nhits), dim3(256), 0, s, ga);
     }
     const bool dev_out = (p.flags & RT_DEVICE_PTRS) != 0;

@@ -96,8 +96,12 @@ namespace rt {
 constexpr int kQueueWords = 1024;
 // Device material row: {ka rgb, shininess}, {kd rgb, transmittance},
 // {ks rgb, refractive index}, {reflectance, kind, chroma x, chroma y},
-// {diffuse, roughness, extinction, 0}.
-constexpr int kMaterialFloat4s = 5;
+// {diffuse, roughness, extinction, 0}, {tex ka, kd, ks, ns},
+// {tex refl, d, mapper kd, mapper ks}, {clrh, clrf, layers 0, 1},
+// {layers 2, 3, 4, 0} (texture slots 1 + map, layer lists first << 8 |
+// count, as int bits).
+constexpr int kMaterialFloat4s = 9;
+struct DevTexMap;  // textures.hpp
 }  // namespace rt
 
 struct rt_scene {
@@ -117,6 +121,17 @@ struct rt_scene {
     // Vertex normals (rt_scene_set_vertex_normals): 9 floats per triangle,
     // FLT_MAX as the first marks a triangle without them; null when none.
     float *d_vn = nullptr;
+    // Texture maps (rt_scene_set_textures): maps with their images' shapes,
+    // texels, mapper_shader layer lists; texture coordinates
+    // (rt_scene_set_texture_coords), 6 floats per triangle. Null when none.
+    rt::DevTexMap *d_maps = nullptr;
+    float *d_texels = nullptr;
+    int32_t *d_layers = nullptr;
+    int32_t nmaps = 0, nlayers = 0;
+    // What the current materials reference: maps 1..maps_used, layers
+    // [0, layers_used).
+    int32_t maps_used = 0, layers_used = 0;
+    float *d_vt = nullptr;
     std::atomic<int> refs{1};  // the caller's handle + one per accel
 };
 

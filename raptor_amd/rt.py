@@ -37,6 +37,7 @@ EXPORTS = [
     "rt_write_tga", "rt_scene_set_vertex_normals", "rt_mesh_load", "rt_mesh_parse", "rt_mesh_info_get",
     "rt_mesh_copy", "rt_mesh_last_error", "rt_mesh_destroy", "rt_scene_create_from_mesh", "rt_write_jpeg",
     "rt_mesh_camera", "rt_mesh_lights", "rt_build_morton", "rt_accel_download", "rt_accel_root", "rt_glare_filter",
+    "rt_scene_set_textures", "rt_scene_set_texture_coords", "rt_texture_load_tga", "rt_mesh_copy_textures",
 ]
 
 
@@ -99,7 +100,8 @@ class rt_mesh_info(C.Structure):
     _fields_ = [("triangles", C.c_int64), ("materials", C.c_int32), ("has_vertex_normals", C.c_int32),
                 ("vertices", C.c_int64), ("faces", C.c_int64), ("skipped_faces", C.c_int64),
                 ("skipped_statements", C.c_int64), ("texture_maps", C.c_int32), ("lights", C.c_int32),
-                ("has_camera", C.c_int32), ("reserved", C.c_int32)]
+                ("has_camera", C.c_int32), ("has_texture_coords", C.c_int32), ("maps", C.c_int32),
+                ("images", C.c_int32), ("texels", C.c_int64)]
 
 
 def load_library(path=LIB_PATH):
@@ -163,6 +165,10 @@ def load_library(path=LIB_PATH):
                                    C.POINTER(C.c_float)]),
         "rt_invrender_destroy": (None, [vp]),
         "rt_scene_set_vertex_normals": (i32, [vp, fp]),
+        "rt_scene_set_textures": (i32, [vp, vp, i32, vp, i32, fp, i64, vp, i32]),
+        "rt_scene_set_texture_coords": (i32, [vp, fp]),
+        "rt_texture_load_tga": (i32, [C.c_char_p, fp, i64, vp]),
+        "rt_mesh_copy_textures": (i32, [vp, fp, vp, vp, fp]),
         "rt_mesh_load": (i32, [C.c_char_p, i32, C.POINTER(vp)]),
         "rt_mesh_parse": (i32, [C.c_char_p, i64, i32, C.c_char_p, C.POINTER(vp)]),
         "rt_mesh_info_get": (i32, [vp, C.POINTER(rt_mesh_info)]),
@@ -258,12 +264,135 @@ def gen_terrain(grid, amplitude=0.15, zoom=0.02, persistence=0.5, octaves=8, noi
 
 # ---------------------------------------------------------------- handles
 
-MATERIAL_PHONG, MATERIAL_COOK_TORRANCE, MATERIAL_COLOURED = 0, 1, 2
-# rt_material (include/raptor_rt.h), 76 bytes.
+MATERIAL_PHONG, MATERIAL_COOK_TORRANCE, MATERIAL_COLOURED, MATERIAL_MAPPER = 0, 1, 2, 3
+# rt_material (include/raptor_rt.h), 156 bytes.
 MATERIAL_DTYPE = np.dtype([("ka", "<f4", 3), ("kd", "<f4", 3), ("ks", "<f4", 3), ("shininess", "<f4"),
                            ("transmittance", "<f4"), ("refractive_index", "<f4"), ("reflectance", "<f4"),
                            ("kind", "<i4"), ("chroma", "<f4", 2), ("diffuse", "<f4"), ("roughness", "<f4"),
-                           ("extinction", "<f4")])
+                           ("extinction", "<f4"), ("tex", "<i4", 6), ("mapper_kd", "<f4"), ("mapper_ks", "<f4"),
+                           ("clrh", "<f4"), ("clrf", "<f4"), ("layer_first", "<i4", 5), ("layer_count", "<i4", 5)])
+assert MATERIAL_DTYPE.itemsize == 156
+
+MAP_PLANAR, MAP_CUBIC, MAP_CYLINDRICAL, MAP_CHECKER_BOARD = 0, 1, 2, 3
+WRAP_BLANK, WRAP_CLAMP, WRAP_TILE, WRAP_MIRROR = 0, 1, 2, 3
+FALLOFF_CUBIC, FALLOFF_SPHERICAL, FALLOFF_LINEAR_X, FALLOFF_LINEAR_Y, FALLOFF_LINEAR_Z, FALLOFF_NONE = range(6)
+# rt_texture_map and rt_texture_image (include/raptor_rt.h).
+TEXTURE_MAP_DTYPE = np.dtype([("kind", "<i4"), ("image", "<i4"), ("c", "<f4", 3), ("n", "<f4", 3),
+                              ("s", "<f4", 3), ("u", "<f4", 3), ("v", "<f4", 3), ("u_wrap", "<i4"),
+                              ("v_wrap", "<i4"), ("u_off", "<i4"), ("v_off", "<i4"), ("cycles", "<i4"),
+                              ("rgb", "<f4", 3), ("opacity", "<f4"), ("inverse", "<i4"), ("falloff", "<i4"),
+                              ("falloff_centre", "<f4", 3), ("falloff_grad", "<f4", 3)])
+TEXTURE_IMAGE_DTYPE = np.dtype([("width", "<i4"), ("height", "<i4"), ("cpp", "<i4"), ("reserved", "<i4"),
+                                ("offset", "<i8")])
+assert TEXTURE_MAP_DTYPE.itemsize == 136 and TEXTURE_IMAGE_DTYPE.itemsize == 24
+
+
+def _f3(v):
+    return np.asarray(v, np.float32).reshape(3)
+
+
+def _default_axes(n):
+    """u = (n.y + n.z, 0, n.x), v = (0, n.x + n.z, n.y) in float32, as the
+    image mappers' first constructors derive them
+    (rt/materials/planar_mapper.h:23-29, cubic_mapper.h:24-34)."""
+    n = _f3(n)
+    return (np.array([n[1] + n[2], 0.0, n[0]], np.float32), np.array([0.0, n[0] + n[2], n[1]], np.float32))
+
+
+def _map(kind):
+    m = np.zeros((), TEXTURE_MAP_DTYPE)
+    m["kind"] = kind
+    m["falloff"] = FALLOFF_NONE
+    m["opacity"] = 1.0
+    return m
+
+
+def image_map(kind, image, c, n, s, u_wrap=WRAP_TILE, v_wrap=WRAP_TILE, u_off=0, v_off=0, u=None, v=None,
+              cycles=1):
+    """A planar, cubic or cylindrical map of image index `image` as the
+    reference's constructors leave it. Without u, v: planar and cubic take
+    u = (n.y + n.z, 0, n.x), v = (0, n.x + n.z, n.y) and size s; the
+    cylindrical mapper takes that u, v = n, size s, tile / blank wrapping
+    (rt/materials/cylindrical_mapper.h:20-24). With u and v (the explicit
+    constructors): size = u s.x + v s.y + n s.z for planar and cubic."""
+    m = _map(kind)
+    m["image"] = image
+    m["c"], m["n"] = _f3(c), _f3(n)
+    if u is None:
+        du, dv = _default_axes(n)
+        m["u"] = du
+        m["v"] = _f3(n) if kind == MAP_CYLINDRICAL else dv
+        m["s"] = _f3(s)
+    else:
+        m["u"], m["v"] = _f3(u), _f3(v)
+        sz = _f3(s)
+        m["s"] = (_f3(u) * sz[0] + _f3(v) * sz[1]) + _f3(n) * sz[2] if kind != MAP_CYLINDRICAL else sz
+    if kind == MAP_CYLINDRICAL:
+        u_wrap, v_wrap = WRAP_TILE, WRAP_BLANK
+    m["u_wrap"], m["v_wrap"], m["u_off"], m["v_off"], m["cycles"] = u_wrap, v_wrap, u_off, v_off, cycles
+    return m
+
+
+def checker_board_map(rgb, centre, grid_size, opacity=1.0):
+    """checker_board_mapper(rgb, cnt, grid_size, op)
+    (rt/materials/checker_board_mapper.h:17-18)."""
+    m = _map(MAP_CHECKER_BOARD)
+    m["rgb"], m["c"], m["s"], m["opacity"] = _f3(rgb), _f3(centre), _f3(grid_size), opacity
+    return m
+
+
+def with_falloff(m, kind, centre, grad):
+    """A copy of map m with a mapper_falloff (rt/materials/mapper_falloff.h)."""
+    m = m.copy()
+    m["falloff"], m["falloff_centre"], m["falloff_grad"] = kind, _f3(centre), _f3(grad)
+    return m
+
+
+def texture_images(images):
+    """(rt_texture_image table, texels) from (h, w, cpp) arrays (cpp 1, 3 or 4;
+    a 2-D array is one channel), row 0 first."""
+    table = np.zeros(len(images), TEXTURE_IMAGE_DTYPE)
+    parts, off = [], 0
+    for i, im in enumerate(images):
+        a = np.asarray(im, np.float32)
+        if a.ndim == 2:
+            a = a[:, :, None]
+        table[i] = (a.shape[1], a.shape[0], a.shape[2], 0, off)
+        parts.append(a.reshape(-1))
+        off += a.size
+    texels = np.concatenate(parts) if parts else np.zeros(0, np.float32)
+    return table, np.ascontiguousarray(texels, np.float32)
+
+
+def load_tga(path):
+    """A TGA image as the reference reads it (rows in file order, RGB):
+    (height, width, channels) float32, 0-255."""
+    dims = np.zeros(3, np.int32)
+    rc = _LIB.rt_texture_load_tga(str(path).encode(), None, 0, _ptr(dims))
+    if rc != RT_OK:
+        raise RTError(f"rt_texture_load_tga: {rc} {_LIB.rt_mesh_last_error().decode()}")
+    out = np.empty((dims[1], dims[0], dims[2]), np.float32)
+    rc = _LIB.rt_texture_load_tga(str(path).encode(), _ptr(out), out.size, _ptr(dims))
+    if rc != RT_OK:
+        raise RTError(f"rt_texture_load_tga: {rc} {_LIB.rt_mesh_last_error().decode()}")
+    return out
+
+
+def layer_lists(materials):
+    """mapper_shader layer lists: dict materials of kind MATERIAL_MAPPER with
+    ctex, dtex, stex, rtex, ttex (lists of map indices: colour, kd, s,
+    reflectance, transmittance) get "layers" ((first, count) x 5) into the
+    returned int32 array."""
+    out = []
+    for m in materials:
+        if isinstance(m, dict) and m.get("kind") == MATERIAL_MAPPER:
+            lists = []
+            for key in ("ctex", "dtex", "stex", "rtex", "ttex"):
+                idx = [int(i) for i in m.get(key, ())]
+                lists.append((len(out), len(idx)))
+                out.extend(idx)
+            m["layers"] = lists
+    return np.asarray(out, np.int32)
 
 
 def material_table(materials):
@@ -271,9 +400,11 @@ def material_table(materials):
     (rgb, 0..255; default 0), s (shininess, 0), tran (transmittance, 0), ri
     (refractive index, 1), rf (reflectance, 0). Cook-Torrance (kind 1,
     cook_torrance_cxy): x, y (chroma, 0.33), rd (0.5), rs, ts, sr (0), ri_r
-    (1), ri_i (0). Kind 2 (coloured_mapper_shader without textures) takes
-    the phong fields. A row of 13 floats (ka, kd, ks, s, tran, ri, rf) is a
-    phong material."""
+    (1), ri_i (0). Kind 2 (coloured_mapper_shader) takes the phong fields and
+    tex (t_ka, t_kd, t_ks, t_ns, t_refl, t_d: map indices or None). Kind 3
+    (mapper_shader): rgb (0..255), kd, ks (scalars), s, tran, ri, rf, clrh,
+    clrf and layers ((first, count) x 5, see layer_lists). A row of 13
+    floats (ka, kd, ks, s, tran, ri, rf) is a phong material."""
     out = np.zeros(len(materials), MATERIAL_DTYPE)
     for i, m in enumerate(materials):
         r = out[i]
@@ -291,6 +422,17 @@ def material_table(materials):
             r["roughness"] = m.get("sr", 0.0)
             r["refractive_index"] = m.get("ri_r", 1.0)
             r["extinction"] = m.get("ri_i", 0.0)
+        elif m.get("kind") == MATERIAL_MAPPER:
+            r["kind"] = MATERIAL_MAPPER
+            r["kd"] = m.get("rgb", (0, 0, 0))
+            r["mapper_kd"], r["mapper_ks"] = m.get("kd", 0.0), m.get("ks", 0.0)
+            r["shininess"] = m.get("s", 0.0)
+            r["transmittance"] = m.get("tran", 0.0)
+            r["refractive_index"] = m.get("ri", 1.0)
+            r["reflectance"] = m.get("rf", 0.0)
+            r["clrh"], r["clrf"] = m.get("clrh", 0.0), m.get("clrf", 0.0)
+            for k, (first, count) in enumerate(m.get("layers", [(0, 0)] * 5)):
+                r["layer_first"][k], r["layer_count"][k] = first, count
         else:
             r["kind"] = m.get("kind", MATERIAL_PHONG)
             r["ka"], r["kd"], r["ks"] = m.get("ka", (0, 0, 0)), m.get("kd", (0, 0, 0)), m.get("ks", (0, 0, 0))
@@ -298,6 +440,8 @@ def material_table(materials):
             r["transmittance"] = m.get("tran", 0.0)
             r["refractive_index"] = m.get("ri", 1.0)
             r["reflectance"] = m.get("rf", 0.0)
+            for k, t in enumerate(m.get("tex", ())):
+                r["tex"][k] = 0 if t is None or t < 0 else int(t) + 1
     return out
 
 
@@ -392,6 +536,33 @@ class Scene:
             raise ValueError(f"rt_scene_set_materials: {k.shape[0]} indices, the scene has {self.n} triangles")
         self.ctx.check(_LIB.rt_scene_set_materials(self.h, _ptr(m), m.shape[0], _ptr(k)), "rt_scene_set_materials")
 
+    def set_textures(self, maps, images=None, texels=None, layers=None):
+        """Texture maps (TEXTURE_MAP_DTYPE array or list of maps), their images
+        (TEXTURE_IMAGE_DTYPE with texels, or a list of (h, w, cpp) arrays)
+        and mapper_shader layers (int32 map indices). Empty maps remove them."""
+        m = np.ascontiguousarray(np.asarray(list(maps) if not isinstance(maps, np.ndarray) else maps,
+                                            TEXTURE_MAP_DTYPE).reshape(-1))
+        if images is None:
+            images = []
+        if isinstance(images, np.ndarray) and images.dtype == TEXTURE_IMAGE_DTYPE:
+            table, tx = np.ascontiguousarray(images), np.ascontiguousarray(texels, np.float32).reshape(-1)
+        else:
+            table, tx = texture_images(images)
+        lay = np.ascontiguousarray(layers if layers is not None else np.zeros(0, np.int32), np.int32).reshape(-1)
+        self.ctx.check(_LIB.rt_scene_set_textures(self.h, _ptr(m), m.shape[0], _ptr(table), table.shape[0],
+                                                  _ptr(tx), tx.size, _ptr(lay), lay.size), "rt_scene_set_textures")
+
+    def set_texture_coords(self, vt):
+        """(n, 6) float32 texture coordinates (u, v) at vertices a, b, c; a
+        first value of FLT_MAX: none for that triangle; None removes them."""
+        if vt is None:
+            self.ctx.check(_LIB.rt_scene_set_texture_coords(self.h, None), "rt_scene_set_texture_coords")
+            return
+        v = np.ascontiguousarray(vt, np.float32).reshape(-1, 6)
+        if v.shape[0] != self.n:
+            raise ValueError(f"rt_scene_set_texture_coords: {v.shape[0]} rows, the scene has {self.n} triangles")
+        self.ctx.check(_LIB.rt_scene_set_texture_coords(self.h, _ptr(v)), "rt_scene_set_texture_coords")
+
     def set_vertex_normals(self, vn):
         """(n, 9) float32 normals at vertices a, b, c; a first value of
         FLT_MAX keeps that triangle's geometric normal; None removes them."""
@@ -437,7 +608,8 @@ class Mesh:
     materials (MATERIAL_DTYPE), vertex_normals (n, 9) float32 or None, info
     (dict); for a file that defines them (NFF): camera (an rt_camera),
     background (3 floats, 0-255) and lights ((k, 7) float32 rows: position,
-    rgb, falloff), else None."""
+    rgb, falloff), else None; texture_coords (n, 6) float32 or None, maps
+    (TEXTURE_MAP_DTYPE), images (TEXTURE_IMAGE_DTYPE), texels (float32)."""
 
     def __init__(self, path=None, data=None, fmt=-1, directory=None):
         h = C.c_void_p()
@@ -452,7 +624,7 @@ class Mesh:
         self.h = h
         inf = rt_mesh_info()
         _LIB.rt_mesh_info_get(h, C.byref(inf))
-        self.info = {k: getattr(inf, k) for k, _ in rt_mesh_info._fields_ if k != "reserved"}
+        self.info = {k: getattr(inf, k) for k, _ in rt_mesh_info._fields_}
         self.n = inf.triangles
         self.abc = np.empty((self.n, 9), np.float32)
         self.tri_material = np.empty(self.n, np.int32)
@@ -468,6 +640,12 @@ class Mesh:
         self.lights = np.zeros((inf.lights, 7), np.float32) if inf.lights else None
         if inf.lights:
             _LIB.rt_mesh_lights(h, _ptr(self.lights))
+        self.texture_coords = np.empty((self.n, 6), np.float32) if inf.has_texture_coords else None
+        self.maps = np.zeros(inf.maps, TEXTURE_MAP_DTYPE)
+        self.images = np.zeros(inf.images, TEXTURE_IMAGE_DTYPE)
+        self.texels = np.zeros(inf.texels, np.float32)
+        _LIB.rt_mesh_copy_textures(h, _ptr(self.texture_coords), _ptr(self.maps), _ptr(self.images),
+                                   _ptr(self.texels))
 
     def close(self):
         if self.h:

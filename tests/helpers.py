@@ -29,9 +29,12 @@ def rel_err(a, b, floor=1e-3):
 
 def oracle_material_rows(materials):
     """rt_material rows (the library's MATERIAL_DTYPE) as the oracle's
-    19-float rows: ka, kd, ks, s, tran, ri, rf, kind, cx, cy, rd, sr, ri_i."""
+    39-float rows: ka, kd, ks, s, tran, ri, rf, kind, cx, cy, rd, sr, ri_i,
+    six texture slots, mapper kd, ks, clrh, clrf, layer firsts, counts."""
     m = materials
-    return np.concatenate([m["ka"], m["kd"], m["ks"], m["shininess"][:, None], m["transmittance"][:, None],
-                           m["refractive_index"][:, None], m["reflectance"][:, None],
-                           m["kind"][:, None].astype(np.float32), m["chroma"], m["diffuse"][:, None],
-                           m["roughness"][:, None], m["extinction"][:, None]], axis=1).astype(np.float32)
+    col = lambda k: m[k][:, None].astype(np.float32)  # noqa: E731
+    return np.concatenate([m["ka"], m["kd"], m["ks"], col("shininess"), col("transmittance"),
+                           col("refractive_index"), col("reflectance"), col("kind"), m["chroma"], col("diffuse"),
+                           col("roughness"), col("extinction"), m["tex"].astype(np.float32), col("mapper_kd"),
+                           col("mapper_ks"), col("clrh"), col("clrf"), m["layer_first"].astype(np.float32),
+                           m["layer_count"].astype(np.float32)], axis=1).astype(np.float32)

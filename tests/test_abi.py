@@ -1,5 +1,6 @@
 """The C ABI without a GPU: the library loads, exports what include/raptor_rt.h
 declares, the C++ shim compiles, and the host-side generators behave."""
+import ctypes as C
 import os
 import re
 import subprocess
@@ -89,13 +90,50 @@ def test_bad_arguments_are_rejected_without_gpu():
     assert rt.lib().rt_ctx_create(0, None) == rt.RT_ERR_INVALID
 
 
-def test_material_row_matches_the_c_struct():
-    """rt_material is 19 four-byte fields (include/raptor_rt.h); the Python
-    rows must have the same layout."""
+def _c_layout(struct, fields):
+    """sizeof and offsetof of a header struct, from the C compiler."""
+    import subprocess
+    import tempfile
+
+    src = "#include <stddef.h>\n#include <stdio.h>\n#include \"raptor_rt.h\"\nint main(void) {\n"
+    src += f'printf("%zu\\n", sizeof({struct}));\n'
+    for f in fields:
+        src += f'printf("%zu\\n", offsetof({struct}, {f}));\n'
+    src += "return 0;\n}\n"
+    with tempfile.TemporaryDirectory() as d:
+        c, exe = os.path.join(d, "l.c"), os.path.join(d, "l")
+        with open(c, "w") as fh:
+            fh.write(src)
+        subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), c, "-o", exe], check=True)
+        out = [int(x) for x in subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()]
+    return out[0], dict(zip(fields, out[1:]))
+
+
+@pytest.mark.parametrize("struct,dtype_name", [("rt_material", "MATERIAL_DTYPE"),
+                                               ("rt_texture_map", "TEXTURE_MAP_DTYPE"),
+                                               ("rt_texture_image", "TEXTURE_IMAGE_DTYPE")])
+def test_python_rows_match_the_c_structs(struct, dtype_name):
+    """The numpy rows the bindings pass have the header's size and offsets."""
     from raptor_amd import rt
 
-    assert rt.MATERIAL_DTYPE.itemsize == 76
-    assert rt.MATERIAL_DTYPE.fields["kind"][1] == 52
+    dt = getattr(rt, dtype_name)
+    size, offs = _c_layout(struct, list(dt.names))
+    assert dt.itemsize == size
+    assert {k: dt.fields[k][1] for k in dt.names} == offs
+
+
+def test_mesh_info_matches_the_c_struct():
+    from raptor_amd import rt
+
+    names = [k for k, _ in rt.rt_mesh_info._fields_]
+    size, offs = _c_layout("rt_mesh_info", names)
+    assert C.sizeof(rt.rt_mesh_info) == size
+    assert {k: getattr(rt.rt_mesh_info, k).offset for k in names} == offs
+
+
+def test_material_table_fields():
+    from raptor_amd import rt
+
     t = rt.material_table([dict(kd=(1, 2, 3), rf=0.5), dict(kind=1, x=0.2, rs=0.4, ts=0.3, ri_r=1.2)])
     assert t["reflectance"][0] == np.float32(0.5) and t["kind"][1] == 1
     assert t["transmittance"][1] == np.float32(0.3) and t["refractive_index"][1] == np.float32(1.2)

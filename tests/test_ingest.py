@@ -26,7 +26,8 @@ def mat_rows(m):
     q = m.materials[m.tri_material]
     return np.concatenate([q["ka"], q["kd"], q["ks"], q["shininess"][:, None], q["transmittance"][:, None],
                            q["refractive_index"][:, None], q["reflectance"][:, None],
-                           q["kind"][:, None].astype(np.float32)], axis=1).astype(np.float32)
+                           q["kind"][:, None].astype(np.float32), q["tex"].astype(np.float32)],
+                          axis=1).astype(np.float32)
 
 
 def same_bits(a, b):
@@ -43,6 +44,24 @@ def assert_matches_oracle(m, ref):
         assert m.vertex_normals is None
     else:
         assert same_bits(m.vertex_normals, ref["vn"])
+    if ref.get("vt") is None:
+        assert m.texture_coords is None
+    else:
+        assert same_bits(m.texture_coords, ref["vt"])
+    if "maps" in ref:
+        # every MTL map: a planar map (centre 0, normal (1, 0, 0), size 0,
+        # mirror wrapping) of its image (obj_parser.cc:241)
+        assert [int(x) for x in m.maps["image"]] == ref["maps"]
+        assert np.all(m.maps["kind"] == 0) and np.all(m.maps["u_wrap"] == 3) and np.all(m.maps["v_wrap"] == 3)
+        assert np.all(m.maps["n"] == np.float32([1, 0, 0])) and np.all(m.maps["s"] == 0)
+        assert np.all(m.maps["u"] == np.float32([0, 0, 1])) and np.all(m.maps["v"] == np.float32([0, 1, 0]))
+        assert m.images.shape[0] == len(ref["images"])
+        for im, want in zip(m.images, ref["images"]):
+            h, w, cpp = want.shape
+            assert (im["height"], im["width"], im["cpp"]) == (h, w, cpp)
+            got = m.texels[im["offset"]:im["offset"] + want.size].reshape(want.shape)
+            assert same_bits(got, want)
+        assert m.info["texture_maps"] == ref["unread_maps"]
     assert m.info["faces"] == ref["faces"]
     assert m.info["skipped_faces"] == ref["skipped"]
 
@@ -348,7 +367,14 @@ o thing
 """
 
 
+TEXTURES = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "textures")
+
+
 def write_obj(tmp_path, obj=OBJ, mtl=MTL):
+    import shutil
+
+    for name in ("wood.tga", "marble.tga"):
+        shutil.copyfile(os.path.join(TEXTURES, name), tmp_path / name)
     (tmp_path / "scene.mtl").write_text(mtl)
     p = tmp_path / "scene.obj"
     p.write_text(obj)
@@ -358,9 +384,10 @@ def write_obj(tmp_path, obj=OBJ, mtl=MTL):
 def test_obj_matches_oracle(rt, tmp_path):
     p = write_obj(tmp_path)
     m = rt.Mesh(str(p))
-    ref = OI.parse_obj(OBJ, lambda name: (tmp_path / name).read_text())
+    ref = OI.parse_obj(OBJ, lambda name: (tmp_path / name).read_text(),
+                       lambda name: (tmp_path / name).read_bytes())
     assert_matches_oracle(m, ref)
-    assert m.info["texture_maps"] == 1
+    assert m.info["texture_maps"] == 0 and m.info["maps"] == 1 and m.info["images"] == 1
     assert m.info["skipped_faces"] == 1  # f 1 1 2
     # vn are normalised; triangles without normals are marked.
     vn = m.vertex_normals
