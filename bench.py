@@ -587,6 +587,31 @@ def materials_section(ctx, frames, args):
     }
     accel.close()
     scene.close()
+    # The terrain generator's textured terrain: one coloured_mapper_shader
+    # whose kd is a tiled planar map (primary and shadow rays only).
+    w, mats, kinds, flags, maps, images = workloads.textured_terrain()
+    scene = rt.Scene(ctx, w.abc, w.albedo, flags)
+    scene.set_textures(maps, images)
+    scene.set_materials(mats, kinds)
+    accel = rt.Accel(scene, width=args.width, leaf_size=args.leaf_size, lds_stack=args.lds_stack,
+                     sah_bins=args.sah_bins)
+    accel.render_device(w.camera, w.lights, rgb)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    per = [accel.render_device(w.camera, w.lights, rgb) for _ in range(frames)]
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / frames
+    rays = per[-1]["rays"]
+    out["textured"] = {
+        "workload": "terrain1m shaded as the reference's terrain generator: coloured_mapper_shader, white kd "
+                    "through a planar tile/tile map of a 256x256 image every 0.1 units, 1920x1080, 1 light",
+        "rays_per_frame": {"primary": rays[0], "shadow": rays[1]},
+        "frame_ms": round(wall * 1e3, 3),
+        "frame_gpu_ms": round(sum(s["ms"][0] for s in per) / frames, 3),
+        "mrays_s": round(sum(rays) / wall / 1e6, 1),
+    }
+    accel.close()
+    scene.close()
     return out
 
 

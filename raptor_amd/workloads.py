@@ -267,6 +267,46 @@ def material_terrain(grid=708, x_res=1920, y_res=1080):
     return out, materials, kinds, flags
 
 
+def grass_image(size=256, seed=3):
+    """A procedural RGB texture (size x size, 0-255): value noise of a few
+    octaves tinted green-brown, seeded."""
+    rng = np.random.default_rng(seed)
+    acc = np.zeros((size, size), np.float64)
+    amp = 1.0
+    for cells in (8, 16, 32, 64):
+        g = rng.random((cells + 1, cells + 1))
+        t = np.linspace(0.0, cells, size, endpoint=False)
+        i = t.astype(int)
+        f = t - i
+        f = f * f * (3.0 - 2.0 * f)
+        top = g[i][:, i] * (1 - f)[None, :] + g[i][:, i + 1] * f[None, :]
+        bot = g[i + 1][:, i] * (1 - f)[None, :] + g[i + 1][:, i + 1] * f[None, :]
+        acc += amp * (top * (1 - f)[:, None] + bot * f[:, None])
+        amp *= 0.5
+    acc = (acc - acc.min()) / (acc.max() - acc.min())
+    rgb = np.stack([60 + 90 * acc, 110 + 120 * acc, 40 + 50 * acc], -1)
+    return rgb.astype(np.float32)
+
+
+def textured_terrain(grid=708, x_res=1920, y_res=1080):
+    """The terrain of terrain1m shaded as RAPTor's terrain generator does
+    (RAPTor/terrain_generator/main.cc:141-142): a coloured_mapper_shader with
+    black ka and ks, white kd and a planar map (normal y, tile / tile) on kd,
+    here a procedural 256 x 256 image repeated every 0.1 units. Returns
+    (Workload, materials, tri_material, flags, maps, images)."""
+    from . import rt as _rt
+
+    w = terrain(grid, x_res, y_res, bounce=False)
+    n = w.abc.shape[0]
+    maps = [_rt.image_map(_rt.MAP_PLANAR, 0, (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), (0.1, 0.0, 0.1), _rt.WRAP_TILE,
+                          _rt.WRAP_TILE)]
+    materials = [dict(kind=_rt.MATERIAL_COLOURED, kd=(255.0, 255.0, 255.0), ri=1.0,
+                      tex=(None, 0, None, None, None, None))]
+    out = Workload("textured_terrain", w.abc, w.albedo, w.camera, w.lights, bounce=False,
+                   meta=dict(triangles=int(n), grid=grid, resolution=[x_res, y_res]))
+    return out, materials, np.zeros(n, np.int32), np.zeros(n, np.uint8), maps, [grass_image()]
+
+
 def terrain1m(x_res=1920, y_res=1080):
     return terrain(708, x_res, y_res)
 

@@ -202,6 +202,17 @@ def test_obj_with_texture_coordinates_and_a_tga_map(gpu_ctx, rt, O, tmp_path):
     assert (ref["prim"] >= 0).mean() > 0.2
 
 
+def test_textured_terrain(gpu_ctx, rt, O):
+    """The terrain generator's shading (a tiled planar map on kd) on a 200-cell
+    terrain at 320 x 180."""
+    from raptor_amd import workloads
+
+    w, mats, kinds, flags, maps, images = workloads.textured_terrain(grid=200, x_res=320, y_res=180)
+    out, ref = _render_both(rt, O, gpu_ctx, w, mats, kinds, flags, maps, images)
+    _check(out, ref)
+    assert (ref["prim"] >= 0).mean() > 0.5
+
+
 def test_errors(gpu_ctx, rt):
     from raptor_amd import workloads
 
