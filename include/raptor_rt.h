@@ -540,12 +540,23 @@ typedef struct {
 
 /* Persistent-kernel parameters: lds_stack (4, 8, 16), resident wavefronts per
  * CU, 64-ray chunks taken per atomic, and whether to replay a captured
- * hipGraph (1) or launch directly (0). Zeros select 8, 16, 1. */
+ * hipGraph (1) or launch directly (0). Zeros select 8, 16, 1.
+ * group_waves: wavefronts per workgroup (1, 2 or 4; 0 selects 1).
+ * refill: the compaction threshold. 0 traces each 64-ray chunk to its last
+ * ray (finished lanes idle); 1..64 lets a wavefront keep a pool of ray slots
+ * from the chunks it took and hand new rays to its finished lanes once at
+ * least refill of them are idle (lanes of one wavefront may then trace
+ * different scenes). width: 0 traces the items' BVHs; 4 or 8 traces BVHs of
+ * that node width, which the batch builds from the items' scenes (with the
+ * items' leaf size and SAH bins) when the items' width differs, and owns. */
 typedef struct {
     int32_t lds_stack;
     int32_t waves_per_cu;
     int32_t grab;
     int32_t use_graph;
+    int32_t group_waves;
+    int32_t refill;
+    int32_t width;
 } rt_batch_params;
 
 typedef struct rt_batch rt_batch;
@@ -556,8 +567,10 @@ int32_t rt_batch_create(rt_ctx *ctx, const rt_batch_item *items, int32_t nitems,
  * mode 1: one closest-hit launch per item (baseline). ms: device time. */
 int32_t rt_batch_run(rt_batch *batch, int32_t mode, float *ms);
 void rt_batch_destroy(rt_batch *batch);
-/* Times the persistent launch over lds_stack x waves_per_cu x grab (reps runs
- * each after a warm-up) and returns the fastest parameters (use_graph = 1). */
+/* Times the persistent launch (reps runs each after a warm-up) over node
+ * width (the items' and the other, built from the items' scenes) x
+ * lds_stack x waves_per_cu x group_waves x refill, then grab for the best,
+ * and returns the fastest parameters (use_graph = 1, width set). */
 int32_t rt_batch_autotune(rt_ctx *ctx, const rt_batch_item *items, int32_t nitems, int32_t reps,
                           rt_batch_params *best, float *best_ms);
 

@@ -1228,6 +1228,7 @@ int32_t build_accel(rt_scene *scene, const rt_build_params &p, rt_accel **out) {
     a->triangles = n;
     a->width = W;
     a->leaf_size = p.leaf_size;
+    a->sah_bins = p.sah_bins;
     a->lds_stack = (p.lds_stack <= 0) ? 16 : (p.lds_stack <= 4 ? 4 : (p.lds_stack <= 8 ? 8 : 16));
     auto cleanup = [&]() {
         (void)hipStreamSynchronize(s);  // the arena's buffers may be reused once this returns

@@ -139,6 +139,7 @@ struct rt_accel {
     rt_scene *scene = nullptr;
     int width = 4;
     int leaf_size = 4;
+    int sah_bins = 0;          // the build's binned-SAH refinement (0: LBVH)
     int lds_stack = 16;        // traversal stack entries kept in LDS (4, 8 or 16)
     int32_t root = 0;          // child code of the root (interior index or leaf)
     int64_t nodes = 0, leaves = 0;

@@ -135,18 +135,19 @@ __device__ __forceinline__ void sort_children<8>(float *t, int32_t *c) {
     cswap(t[5], c[5], t[6], c[6]);
 }
 
-// Traversal of one ray (one lane of a 64-thread block). On return hit holds
+// Traversal of one ray (one lane of a block of kCols threads; lane is the
+// thread's index in the block, its LDS stack column). On return hit holds
 // (t, u, v, id) for a closest-hit query (id -1 on a miss) or id >= 0 when an
 // any-hit query found an occluder. The LDS stack is declared here, so its
 // accesses stay LDS instructions after inlining (a pointer parameter loses the
 // address space and lets the compiler fold LDS and scratch reads into flat
 // loads).
-template <int W, bool CLOSEST, bool COUNT, int kLdsDepth>
+template <int W, bool CLOSEST, bool COUNT, int kLdsDepth, int kCols = 64>
 __device__ __forceinline__ void trace_ray(const WideNode<W> *__restrict__ nodes, const LeafTri *__restrict__ tris,
                                           int32_t root, float near_limit, rtd::V3 o, rtd::V3 d, float tmin, float tmax,
                                           int lane, float4 &hit, uint32_t &visits, uint32_t &tests, bool &overflow) {
     constexpr int kSpillDepth = kStackDepth - kLdsDepth;
-    __shared__ int2 lds_stack[kLdsDepth][64];
+    __shared__ int2 lds_stack[kLdsDepth][kCols];
     const rtd::V3 id = rtd::mk(safe_inv(d.x), safe_inv(d.y), safe_inv(d.z));
     const rtd::V3 oid = rtd::mk(-(o.x * id.x), -(o.y * id.y), -(o.z * id.z));
     const bool near = fmaxf(fmaxf(fabsf(o.x), fabsf(o.y)), fabsf(o.z)) <= near_limit;

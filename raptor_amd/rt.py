@@ -69,7 +69,8 @@ class rt_batch_item(C.Structure):
 
 
 class rt_batch_params(C.Structure):
-    _fields_ = [("lds_stack", C.c_int32), ("waves_per_cu", C.c_int32), ("grab", C.c_int32), ("use_graph", C.c_int32)]
+    _fields_ = [("lds_stack", C.c_int32), ("waves_per_cu", C.c_int32), ("grab", C.c_int32), ("use_graph", C.c_int32),
+                ("group_waves", C.c_int32), ("refill", C.c_int32), ("width", C.c_int32)]
 
 
 class rt_invrender_item(C.Structure):
@@ -684,13 +685,20 @@ def _batch_items(accels, rays, hits):
 
 
 class Batch:
-    """Many (BVH, rays, hits) items traced by one persistent launch (device tensors)."""
+    """Many (BVH, rays, hits) items traced by one persistent launch (device tensors).
 
-    def __init__(self, ctx, accels, rays, hits, lds_stack=8, waves_per_cu=16, grab=1, use_graph=True):
+    group_waves: wavefronts per workgroup (1, 2 or 4); refill: 0 traces a
+    chunk of 64 rays to its last ray, 1..64 hands new rays to a wavefront's
+    finished lanes once that many are idle; width: 0 traces the items' BVHs,
+    4 or 8 builds the scenes' BVHs at that node width for the batch."""
+
+    def __init__(self, ctx, accels, rays, hits, lds_stack=8, waves_per_cu=16, grab=1, use_graph=True,
+                 group_waves=1, refill=0, width=0):
         self.ctx = ctx
         self._keep = (accels, rays, hits)
         self.items = _batch_items(accels, rays, hits)
-        self.params = rt_batch_params(lds_stack, waves_per_cu, grab, 1 if use_graph else 0)
+        self.params = rt_batch_params(lds_stack, waves_per_cu, grab, 1 if use_graph else 0, group_waves, refill,
+                                      width)
         h = C.c_void_p()
         ctx.check(_LIB.rt_batch_create(ctx.h, self.items, len(accels), C.byref(self.params), C.byref(h)),
                   "rt_batch_create")
@@ -708,7 +716,8 @@ class Batch:
         ms = C.c_float()
         ctx.check(_LIB.rt_batch_autotune(ctx.h, items, len(accels), reps, C.byref(best), C.byref(ms)),
                   "rt_batch_autotune")
-        return {"lds_stack": best.lds_stack, "waves_per_cu": best.waves_per_cu, "grab": best.grab}, ms.value
+        return {"lds_stack": best.lds_stack, "waves_per_cu": best.waves_per_cu, "grab": best.grab,
+                "group_waves": best.group_waves, "refill": best.refill, "width": best.width}, ms.value
 
     def close(self):
         if self.h:

@@ -131,6 +131,15 @@ def test_mesh_info_matches_the_c_struct():
     assert {k: getattr(rt.rt_mesh_info, k).offset for k in names} == offs
 
 
+def test_batch_params_match_the_c_struct():
+    from raptor_amd import rt
+
+    names = [k for k, _ in rt.rt_batch_params._fields_]
+    size, offs = _c_layout("rt_batch_params", names)
+    assert C.sizeof(rt.rt_batch_params) == size
+    assert {k: getattr(rt.rt_batch_params, k).offset for k in names} == offs
+
+
 def test_material_table_fields():
     from raptor_amd import rt
 

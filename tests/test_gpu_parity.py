@@ -176,6 +176,24 @@ def test_batched_persistent_matches_plain_and_oracle(gpu_ctx, rt, O):
     direct.run()
     for hp, hb in zip(hits_p, hits_b):
         assert torch.equal(hp.view(torch.int32), hb.view(torch.int32))
+    # Wavefronts per workgroup, the refill form and the other node width
+    # (the batch's own BVHs, per-scene launches of them included).
+    for kw in (dict(group_waves=2), dict(group_waves=4, lds_stack=4), dict(refill=16),
+               dict(refill=1, group_waves=2, grab=2), dict(refill=64, lds_stack=16, group_waves=4),
+               dict(width=8, refill=32, group_waves=4), dict(width=8, lds_stack=4, waves_per_cu=8),
+               dict(width=4, refill=8, grab=4)):
+        for plain_run in (False, True):
+            for h in hits_b:
+                h.fill_(-7.0)
+            b = rt.Batch(gpu_ctx, accels, rays, hits_b, use_graph=not plain_run, **kw)
+            b.run(plain=plain_run)
+            torch.cuda.synchronize()
+            for hp, hb in zip(hits_p, hits_b):
+                assert torch.equal(hp.view(torch.int32), hb.view(torch.int32)), (kw, plain_run)
+            b.close()
+    for bad in (dict(group_waves=3), dict(refill=65), dict(refill=-1), dict(width=6), dict(group_waves=8)):
+        with pytest.raises(rt.RTError):
+            rt.Batch(gpu_ctx, accels, rays, hits_b, **bad)
     # Oracle on every scene's rays.
     total_hits = 0
     for sc, r, hp in zip(scenes, rays, hits_p):
