@@ -134,6 +134,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_trace_batch_refill(BatchArgs a,
     const WideNode<W> *nodes = nullptr;
     const LeafTri *tris = nullptr;
     rtd::V3 o = rtd::mk(0.0f, 0.0f, 0.0f), d = o, id = o, oid = o;
+    PlaneOffsets po{};
     float tmin = 0.0f;
     bool near = false;
     int32_t spill_code[kSpillDepth];
@@ -207,6 +208,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_trace_batch_refill(BatchArgs a,
                         tmin = r0.w;
                         id = rtd::mk(safe_inv(d.x), safe_inv(d.y), safe_inv(d.z));
                         oid = rtd::mk(-(o.x * id.x), -(o.y * id.y), -(o.z * id.z));
+                        po = plane_offsets<W>(id);
                         near = fmaxf(fmaxf(fabsf(o.x), fabsf(o.y)), fabsf(o.z)) <= sc.near_limit;
                         best_t = r1.w;
                         best_u = 0.0f;
@@ -229,7 +231,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_trace_batch_refill(BatchArgs a,
         while (true) {
             if (scene >= 0 && cur >= 0) {
                 NodeView<W> v;
-                load_node<W>(nodes, cur, v);
+                load_node<W>(nodes, cur, po, v);
                 float tn[W];
                 if (near)
                     box_test<W, true>(v, o, id, oid, best_t * kHitStretch, tn);

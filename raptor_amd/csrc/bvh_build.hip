@@ -353,8 +353,8 @@ __global__ void __launch_bounds__(256) k_collapse(CollapseArgs a) {
     int leaves = 0;
     for (int k = 0; k < W; ++k) {
         if (k >= cnt) {
-            out.lo_x[k] = out.lo_y[k] = out.lo_z[k] = 0.0f;
-            out.hi_x[k] = out.hi_y[k] = out.hi_z[k] = 0.0f;
+            out.lo_x[k] = out.lo_y[k] = out.lo_z[k] = INFINITY;  // inverted: never hit
+            out.hi_x[k] = out.hi_y[k] = out.hi_z[k] = -INFINITY;
             out.child[k] = kEmpty;
             continue;
         }
@@ -1594,6 +1594,11 @@ int32_t build_accel(rt_scene *scene, const rt_build_params &p, rt_accel **out) {
     BUILD_CHECK(hipStreamSynchronize(s));
     a->nodes = static_cast<int64_t>(counters[0] & 0xffffffffull);
     a->leaves = static_cast<int64_t>(counters[1]);
+    if (static_cast<uint64_t>(a->nodes) * node_bytes > 0xffffffffull) {
+        // Traversal addresses a node's planes with 32-bit byte offsets.
+        cleanup();
+        return fail(ctx, RT_ERR_UNSUPPORTED, "BVH node array beyond 4 GiB: use a larger leaf size or width 4");
+    }
     a->depth = depth;
     a->root = 0;
     a->build_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();

@@ -31,6 +31,7 @@ __host__ __device__ inline int32_t leaf_count(int32_t c) { return static_cast<in
 // Wide node of W children, structure of arrays so one lane fetches the whole
 // node with 16-byte loads: lo_x[W], hi_x[W], lo_y[W], hi_y[W], lo_z[W], hi_z[W],
 // child[W]. 112 bytes for W = 4 (padded to 128), 224 for W = 8 (padded to 256).
+// An empty slot has child kEmpty and the inverted box lo = +inf, hi = -inf.
 template <int W>
 struct alignas(W == 4 ? 128 : 256) WideNode {
     float lo_x[W], hi_x[W], lo_y[W], hi_y[W], lo_z[W], hi_z[W];

@@ -110,6 +110,7 @@ __global__ void __launch_bounds__(kBlockThreads) k_trace_pers(TraceArgs a) {
 
     int64_t ri = -1;  // ray of this lane, -1: none
     rtd::V3 o = rtd::mk(0.0f, 0.0f, 0.0f), d = o, id = o, oid = o;
+    PlaneOffsets po{};
     float tmin = 0.0f, tmax = 0.0f;
     bool near = false;
     int32_t spill_code[kSpillDepth];
@@ -170,6 +171,7 @@ __global__ void __launch_bounds__(kBlockThreads) k_trace_pers(TraceArgs a) {
                     tmax = r1.w;
                     id = rtd::mk(safe_inv(d.x), safe_inv(d.y), safe_inv(d.z));
                     oid = rtd::mk(-(o.x * id.x), -(o.y * id.y), -(o.z * id.z));
+                    po = plane_offsets<W>(id);
                     near = fmaxf(fmaxf(fabsf(o.x), fabsf(o.y)), fabsf(o.z)) <= a.near_limit;
                     best_t = tmax;
                     best_u = 0.0f;
@@ -187,7 +189,7 @@ __global__ void __launch_bounds__(kBlockThreads) k_trace_pers(TraceArgs a) {
         while (true) {
             if (cur >= 0) {
                 NodeView<W> v;
-                load_node<W>(nodes, cur, v);
+                load_node<W>(nodes, cur, po, v);
                 if (COUNT) ++visits;
                 float tn[W];
                 if (near)

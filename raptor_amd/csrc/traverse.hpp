@@ -38,28 +38,72 @@ __device__ __forceinline__ float safe_inv(float d) {
     return 1.0f / ((fabsf(d) > 1e-20f) ? d : copysignf(1e-20f, d));
 }
 
-template <int W>
-struct NodeView {
-    float lx[W], hx[W], ly[W], hy[W], lz[W], hz[W];
-    int32_t c[W];
+// Byte offsets, inside a node, of each axis's near and far plane arrays for
+// one ray: the near plane of an axis is lo when the ray's inverse direction on
+// that axis is positive (sign bit clear) and hi otherwise. Computed once per
+// ray, so a node visit loads the near and far planes directly and the slab
+// test needs no per-axis min/max. Because fma(p, 1/d, -o/d) and (p - o) / d
+// are monotonic in p, the near plane's distance equals the min of the two
+// planes' distances, bit for bit, and the far plane's the max: the result is
+// that of the symmetric slab test.
+struct PlaneOffsets {
+    uint32_t n[3], f[3];
 };
 
 template <int W>
-__device__ __forceinline__ void load_node(const WideNode<W> *__restrict__ nodes, int32_t idx, NodeView<W> &v) {
-    const float4 *p = reinterpret_cast<const float4 *>(nodes + idx);
-    constexpr int Q = W / 4;  // float4 per array
+__device__ __forceinline__ PlaneOffsets plane_offsets(rtd::V3 id) {
+    constexpr uint32_t A = 4u * W;  // bytes of one plane array
+    const uint32_t sx = __float_as_uint(id.x) >> 31, sy = __float_as_uint(id.y) >> 31,
+                   sz = __float_as_uint(id.z) >> 31;
+    PlaneOffsets p;
+    p.n[0] = sx * A;
+    p.f[0] = A - sx * A;
+    p.n[1] = 2u * A + sy * A;
+    p.f[1] = 3u * A - sy * A;
+    p.n[2] = 4u * A + sz * A;
+    p.f[2] = 5u * A - sz * A;
+    return p;
+}
+
+template <int W>
+struct NodeView {
+    float nx[W], fx[W], ny[W], fy[W], nz[W], fz[W];  // near and far planes per axis
+    int32_t c[W];
+};
+
+// Node addressing uses 32-bit byte offsets from the node array's base (the
+// array is limited to 4 GiB at build time), so a lane's loads take the
+// base-plus-offset form and each costs one 32-bit add.
+template <int W>
+__device__ __forceinline__ void load_planes(const char *base, uint32_t off, float *dst) {
 #pragma unroll
-    for (int q = 0; q < Q; ++q) {
-        const float4 a = p[0 * Q + q], b = p[1 * Q + q], c = p[2 * Q + q];
-        const float4 d = p[3 * Q + q], e = p[4 * Q + q], f = p[5 * Q + q];
-        const int4 g = reinterpret_cast<const int4 *>(p)[6 * Q + q];
-        v.lx[4 * q + 0] = a.x; v.lx[4 * q + 1] = a.y; v.lx[4 * q + 2] = a.z; v.lx[4 * q + 3] = a.w;
-        v.hx[4 * q + 0] = b.x; v.hx[4 * q + 1] = b.y; v.hx[4 * q + 2] = b.z; v.hx[4 * q + 3] = b.w;
-        v.ly[4 * q + 0] = c.x; v.ly[4 * q + 1] = c.y; v.ly[4 * q + 2] = c.z; v.ly[4 * q + 3] = c.w;
-        v.hy[4 * q + 0] = d.x; v.hy[4 * q + 1] = d.y; v.hy[4 * q + 2] = d.z; v.hy[4 * q + 3] = d.w;
-        v.lz[4 * q + 0] = e.x; v.lz[4 * q + 1] = e.y; v.lz[4 * q + 2] = e.z; v.lz[4 * q + 3] = e.w;
-        v.hz[4 * q + 0] = f.x; v.hz[4 * q + 1] = f.y; v.hz[4 * q + 2] = f.z; v.hz[4 * q + 3] = f.w;
-        v.c[4 * q + 0] = g.x; v.c[4 * q + 1] = g.y; v.c[4 * q + 2] = g.z; v.c[4 * q + 3] = g.w;
+    for (int q = 0; q < W / 4; ++q) {
+        const float4 x = *reinterpret_cast<const float4 *>(base + static_cast<uint32_t>(off + 16u * q));
+        dst[4 * q + 0] = x.x;
+        dst[4 * q + 1] = x.y;
+        dst[4 * q + 2] = x.z;
+        dst[4 * q + 3] = x.w;
+    }
+}
+
+template <int W>
+__device__ __forceinline__ void load_node(const WideNode<W> *__restrict__ nodes, int32_t idx, const PlaneOffsets &po,
+                                          NodeView<W> &v) {
+    const char *base = reinterpret_cast<const char *>(nodes);
+    const uint32_t o = static_cast<uint32_t>(idx) * static_cast<uint32_t>(sizeof(WideNode<W>));
+    load_planes<W>(base, o + po.n[0], v.nx);
+    load_planes<W>(base, o + po.f[0], v.fx);
+    load_planes<W>(base, o + po.n[1], v.ny);
+    load_planes<W>(base, o + po.f[1], v.fy);
+    load_planes<W>(base, o + po.n[2], v.nz);
+    load_planes<W>(base, o + po.f[2], v.fz);
+#pragma unroll
+    for (int q = 0; q < W / 4; ++q) {
+        const int4 g = *reinterpret_cast<const int4 *>(base + static_cast<uint32_t>(o + 24u * W + 16u * q));
+        v.c[4 * q + 0] = g.x;
+        v.c[4 * q + 1] = g.y;
+        v.c[4 * q + 2] = g.z;
+        v.c[4 * q + 3] = g.w;
     }
 }
 
@@ -67,32 +111,34 @@ __device__ __forceinline__ void load_node(const WideNode<W> *__restrict__ nodes,
 // fma(plane, 1/d, -o/d): one operation instead of two, with an extra error of
 // about |o| * 2^-24 in position, which the build's box padding (2^-15 of the
 // largest scene coordinate) covers while |o| stays within 256x the scene;
-// rays from farther away use the subtract-then-multiply form.
+// rays from farther away use the subtract-then-multiply form. An empty child
+// slot holds the inverted box lo = +inf, hi = -inf, whose near distance is
+// +inf and far distance -inf on every axis, so it never passes.
 template <int W, bool FUSED>
 __device__ __forceinline__ void box_test(const NodeView<W> &v, rtd::V3 o, rtd::V3 id, rtd::V3 oid, float tlimit,
                                          float *tn) {
 #pragma unroll
     for (int k = 0; k < W; ++k) {
-        float x0, x1, y0, y1, z0, z1;
+        float xn, xf, yn, yf, zn, zf;
         if (FUSED) {
-            x0 = fmaf(v.lx[k], id.x, oid.x);
-            x1 = fmaf(v.hx[k], id.x, oid.x);
-            y0 = fmaf(v.ly[k], id.y, oid.y);
-            y1 = fmaf(v.hy[k], id.y, oid.y);
-            z0 = fmaf(v.lz[k], id.z, oid.z);
-            z1 = fmaf(v.hz[k], id.z, oid.z);
+            xn = fmaf(v.nx[k], id.x, oid.x);
+            xf = fmaf(v.fx[k], id.x, oid.x);
+            yn = fmaf(v.ny[k], id.y, oid.y);
+            yf = fmaf(v.fy[k], id.y, oid.y);
+            zn = fmaf(v.nz[k], id.z, oid.z);
+            zf = fmaf(v.fz[k], id.z, oid.z);
         } else {
-            x0 = (v.lx[k] - o.x) * id.x;
-            x1 = (v.hx[k] - o.x) * id.x;
-            y0 = (v.ly[k] - o.y) * id.y;
-            y1 = (v.hy[k] - o.y) * id.y;
-            z0 = (v.lz[k] - o.z) * id.z;
-            z1 = (v.hz[k] - o.z) * id.z;
+            xn = (v.nx[k] - o.x) * id.x;
+            xf = (v.fx[k] - o.x) * id.x;
+            yn = (v.ny[k] - o.y) * id.y;
+            yf = (v.fy[k] - o.y) * id.y;
+            zn = (v.nz[k] - o.z) * id.z;
+            zf = (v.fz[k] - o.z) * id.z;
         }
-        const float tnear = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.0f));
-        const float tfar = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1));
-        const bool hit = (v.c[k] != kEmpty) && (tnear <= tfar * kFarStretch) && (tnear <= tlimit);
-        tn[k] = hit ? tnear : INFINITY;
+        const float tnear = fmaxf(fmaxf(fmaxf(xn, yn), zn), 0.0f);
+        const float tfar = fminf(fminf(xf, yf), zf);
+        // Both bounds are never NaN (finite planes or the infinite empty box).
+        tn[k] = (tnear <= fminf(tfar * kFarStretch, tlimit)) ? tnear : INFINITY;
     }
 }
 
@@ -150,6 +196,7 @@ __device__ __forceinline__ void trace_ray(const WideNode<W> *__restrict__ nodes,
     __shared__ int2 lds_stack[kLdsDepth][kCols];
     const rtd::V3 id = rtd::mk(safe_inv(d.x), safe_inv(d.y), safe_inv(d.z));
     const rtd::V3 oid = rtd::mk(-(o.x * id.x), -(o.y * id.y), -(o.z * id.z));
+    const PlaneOffsets po = plane_offsets<W>(id);
     const bool near = fmaxf(fmaxf(fabsf(o.x), fabsf(o.y)), fabsf(o.z)) <= near_limit;
 
     // Two separate arrays so that a spill read never has the type of an LDS
@@ -196,7 +243,7 @@ __device__ __forceinline__ void trace_ray(const WideNode<W> *__restrict__ nodes,
         while (true) {
             if (cur >= 0) {
                 NodeView<W> v;
-                load_node<W>(nodes, cur, v);
+                load_node<W>(nodes, cur, po, v);
                 if (COUNT) ++visits;
                 float tn[W];
                 if (near)
