@@ -734,20 +734,37 @@ def main():
     info = accel.info()
     rgb = torch.empty((w.camera.y_res, w.camera.x_res, 3), dtype=torch.float32, device="cuda:0")
 
+    # Frames are enqueued back to back (RT_ASYNC): the host does not wait
+    # for a frame before enqueueing the next, and each frame's rays are summed
+    # on the device from its own hit counts.
+    def frame(asynchronous):
+        return accel.render_device(w.camera, w.lights, rgb, bounce=w.bounce, seed=w.seed,
+                                   regroup=bool(args.regroup), asynchronous=asynchronous)
+
     for _ in range(args.warmup):
-        accel.render_device(w.camera, w.lights, rgb, bounce=w.bounce, seed=w.seed, regroup=bool(args.regroup))
+        frame(True)
     print("bench: timing", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
+    ctx.frame_rays()  # clears the sums
     t0 = time.perf_counter()
-    per_step = []
     for _ in range(args.steps):
-        per_step.append(accel.render_device(w.camera, w.lights, rgb, bounce=w.bounce, seed=w.seed, regroup=bool(args.regroup)))
+        frame(True)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    stage_rays = ctx.frame_rays()
 
-    total_rays = sum(sum(s["rays"]) for s in per_step)
+    total_rays = sum(stage_rays)
     value = total_rays / elapsed / 1e6
+    # Per-stage device times from frames that report them (each waits for
+    # itself), after the timed region.
+    per_step, sync_wall = [], []
+    for _ in range(5):
+        t1 = time.perf_counter()
+        per_step.append(frame(False))
+        sync_wall.append(time.perf_counter() - t1)
     rays = per_step[-1]["rays"]
+    if [r * args.steps for r in rays] != stage_rays:
+        raise RuntimeError(f"device ray sums {stage_rays} differ from {args.steps} x {rays}")
 
     # Dominant kernel and its roofline: one extra frame with visit counting.
     ctx.set_counting(True)
@@ -803,8 +820,10 @@ def main():
             "sah_bins": args.sah_bins,
             "lds_stack": args.lds_stack,
             "regroup_bounce": bool(args.regroup),
+            "frames": "enqueued back to back (RT_ASYNC), rays summed on the device",
         },
         "rays_per_step": {"primary": rays[0], "shadow": rays[1], "bounce": rays[2], "bounce_shadow": rays[3]},
+        "frame_waiting_for_itself_ms": round(sorted(sync_wall)[2] * 1e3, 4),
         "stage_ms": {STAGES[k]: round(stage_ms[k], 4) for k in range(4)},
         "bvh": {"nodes": info["nodes"], "leaves": info["leaves"], "depth": info["depth"],
                 "bytes": info["bytes"], "build_ms": round(info["build_ms"], 2)},

@@ -34,6 +34,13 @@ extern "C" {
 /* rt_render_params.flags: regroup bounce rays by direction octant and origin
  * before tracing them. */
 #define RT_REGROUP_BOUNCE 2u
+/* rt_render_params.flags: enqueue the frame on the context's stream and
+ * return without waiting for it (needs RT_DEVICE_PTRS and stats == NULL;
+ * frames without materials or vertex normals). Frames so enqueued back to
+ * back keep the GPU busy between frames. Their ray counts are summed on the
+ * device and read by rt_ctx_frame_rays. No counterpart in the reference,
+ * whose ray_tracer() call returns with the frame done. */
+#define RT_ASYNC 4u
 
 typedef struct rt_ctx rt_ctx;
 typedef struct rt_scene rt_scene;
@@ -128,6 +135,10 @@ typedef struct {
 int32_t rt_ctx_create(int32_t hip_device, rt_ctx **out);
 void rt_ctx_destroy(rt_ctx *ctx);
 const char *rt_last_error(const rt_ctx *ctx);
+/* Waits for the context's frames and returns the rays traced by its
+ * RT_ASYNC frames since the last call, by stage (primary, shadow, bounce,
+ * bounce shadow), then sets the sums to zero. */
+int32_t rt_ctx_frame_rays(rt_ctx *ctx, uint64_t rays[4]);
 
 /* Scene in emplace order: abc is n x 9 floats (a, b, c), albedo n x 3 in
  * [0, 1] or NULL (treated as 1), flags one byte per triangle (bit0 light,

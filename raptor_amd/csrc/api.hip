@@ -115,6 +115,19 @@ int32_t set_device(rt_ctx *ctx) {
     return RT_OK;
 }
 
+// An RT_ASYNC frame returns before the GPU has run it, so everything it
+// reports must stay on the device.
+int32_t check_async(rt_ctx *ctx, const rt_accel *a, const rt_render_params &p, const rt_stats *stats) {
+    if (!(p.flags & RT_ASYNC)) return RT_OK;
+    if (!(p.flags & RT_DEVICE_PTRS))
+        return fail(ctx, RT_ERR_INVALID, "rt_render: RT_ASYNC needs RT_DEVICE_PTRS (device outputs)");
+    if (stats) return fail(ctx, RT_ERR_INVALID, "rt_render: RT_ASYNC frames report rays by rt_ctx_frame_rays, not stats");
+    if (ctx->counting) return fail(ctx, RT_ERR_INVALID, "rt_render: RT_ASYNC with visit counting on");
+    if (a->scene->d_mat || a->scene->d_vn)
+        return fail(ctx, RT_ERR_UNSUPPORTED, "rt_render: RT_ASYNC frames without materials or vertex normals only");
+    return RT_OK;
+}
+
 }  // namespace
 
 void release_ctx(rt_ctx *ctx) {
@@ -128,6 +141,7 @@ void release_ctx(rt_ctx *ctx) {
     if (ctx->fork) (void)hipEventDestroy(ctx->fork);
     if (ctx->join) (void)hipEventDestroy(ctx->join);
     if (ctx->d_counters) (void)hipFree(ctx->d_counters);
+    if (ctx->d_frame_rays) (void)hipFree(ctx->d_frame_rays);
     if (ctx->d_queue) (void)hipFree(ctx->d_queue);
     ctx->arena.release();
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -189,11 +203,13 @@ int32_t rt_ctx_create(int32_t hip_device, rt_ctx **out) {
         }
     }
     if (hipMalloc(&ctx->d_counters, 16 * sizeof(unsigned long long)) != hipSuccess ||
+        hipMalloc(&ctx->d_frame_rays, 4 * sizeof(unsigned long long)) != hipSuccess ||
         hipMalloc(&ctx->d_queue, rt::kQueueWords * sizeof(uint32_t)) != hipSuccess) {
         rt_ctx_destroy(ctx);
         return RT_ERR_HIP;
     }
     (void)hipMemset(ctx->d_counters, 0, 16 * sizeof(unsigned long long));
+    (void)hipMemset(ctx->d_frame_rays, 0, 4 * sizeof(unsigned long long));
     // The persistent trace kernels expect their chunk counters at zero and
     // leave them at zero.
     (void)hipMemset(ctx->d_queue, 0, rt::kQueueWords * sizeof(uint32_t));
@@ -648,6 +664,7 @@ int32_t rt_render(rt_accel *accel, const rt_camera *cam, const rt_light *lights,
     if (int32_t rc = rt::set_device(ctx)) return rc;
     rt_render_params p{0, 1, 0, 0, 0};
     if (params) p = *params;
+    if (int32_t rc = rt::check_async(ctx, accel, p, stats)) return rc;
     if (ctx->counting)
         RT_HIP_CHECK(ctx, hipMemsetAsync(ctx->d_counters, 0, 16 * sizeof(unsigned long long), ctx->stream));
     int32_t rc = RT_OK;
@@ -685,6 +702,7 @@ int32_t rt_render_static(rt_accel *accel, const rt_camera *cam, const rt_light *
     rt_render_params p{0, 1, 0, 0, 0};
     if (params) p = *params;
     p.flags |= RT_DEVICE_PTRS;
+    if (int32_t rc = rt::check_async(ctx, accel, p, stats)) return rc;
     if (ctx->counting)
         RT_HIP_CHECK(ctx, hipMemsetAsync(ctx->d_counters, 0, 16 * sizeof(unsigned long long), ctx->stream));
     int32_t rc = rt::render_frame(accel, *cam, lights, nlights, p, rgb, primary, bounce, stats, ctx->d_static,
@@ -712,6 +730,17 @@ int32_t rt_gen_surface_rays(rt_scene *scene, uint64_t seed, int64_t n, rt_ray *r
                        scene->d_abc, scene->d_n, scene->n, seed, n, reinterpret_cast<float4 *>(rays));
     RT_HIP_CHECK(ctx, hipGetLastError());
     RT_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return RT_OK;
+}
+
+int32_t rt_ctx_frame_rays(rt_ctx *ctx, uint64_t rays[4]) {
+    if (!ctx || !rays) return RT_ERR_INVALID;
+    if (int32_t rc = rt::set_device(ctx)) return rc;
+    unsigned long long v[4];
+    RT_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    RT_HIP_CHECK(ctx, hipMemcpy(v, ctx->d_frame_rays, sizeof(v), hipMemcpyDeviceToHost));
+    RT_HIP_CHECK(ctx, hipMemset(ctx->d_frame_rays, 0, sizeof(v)));
+    for (int k = 0; k < 4; ++k) rays[k] = v[k];
     return RT_OK;
 }
 

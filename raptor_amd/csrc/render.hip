@@ -88,6 +88,22 @@ __global__ void __launch_bounds__(256) k_count_hits(const rt_hit *__restrict__ h
     if (threadIdx.x == 0) block_counts[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
 }
 
+// Rays of a frame chunk enqueued with RT_ASYNC, added to the context's sums
+// (primary, shadow, bounce, bounce shadow) from the compaction totals: one
+// shadow ray per hit per light, one bounce ray per primary hit. Launches on
+// one stream run in order, so one thread updates the sums.
+__global__ void k_add_frame_rays(const uint32_t *hits1, const uint32_t *hits2, uint64_t primary, int nlights,
+                                 unsigned long long *sums) {
+    if (threadIdx.x != 0) return;
+    const unsigned long long h1 = hits1[0];
+    sums[0] += primary;
+    sums[1] += h1 * static_cast<unsigned long long>(nlights);
+    if (hits2) {
+        sums[2] += h1;
+        sums[3] += static_cast<unsigned long long>(hits2[0]) * static_cast<unsigned long long>(nlights);
+    }
+}
+
 // One block of 1024 threads scans up to 1024 * kScanPer block counts.
 constexpr int kScanPer = 32;
 __global__ void __launch_bounds__(1024) k_scan_blocks(uint32_t *__restrict__ v, int n) {
@@ -348,6 +364,8 @@ int32_t workspace_reserve(rt_ctx *ctx, Workspace *w, int64_t ns, int nl) {
         workspace_layout(*w, reinterpret_cast<char *>(w->mem), w->nsamples, w->nlights);
         return RT_OK;
     }
+    // Frames enqueued without waiting (RT_ASYNC) may still use the old buffer.
+    if (w->mem) RT_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     if (w->mem) (void)hipFree(w->mem);
     w->mem = nullptr;
     w->nsamples = 0;
@@ -678,6 +696,7 @@ int32_t render_frame(rt_accel *a, const rt_camera &cam, const rt_light *lights, 
     // Frame-sized outputs: the caller's device buffers, else staging kept on
     // the workspace (grow-only).
     const bool dev_out = (p.flags & RT_DEVICE_PTRS) != 0;
+    const bool async = (p.flags & RT_ASYNC) != 0;  // checked by the entry points: device outputs, no stats
     const int64_t npix = f.npix;
     const int spp = f.spp;
     const int64_t nout = npix * spp;
@@ -691,6 +710,7 @@ int32_t render_frame(rt_accel *a, const rt_camera &cam, const rt_light *lights, 
         const size_t hits_bytes = static_cast<size_t>(nout) * sizeof(rt_hit);
         const size_t need = img_bytes + ((primary && !hp) ? hits_bytes : 0) + ((bounce_hits && !hb) ? hits_bytes : 0);
         if (need > w->frame_bytes) {
+            if (w->frame_mem) RT_HIP_CHECK(ctx, hipStreamSynchronize(s));
             if (w->frame_mem) (void)hipFree(w->frame_mem);
             w->frame_mem = nullptr;
             w->frame_bytes = 0;
@@ -717,6 +737,12 @@ int32_t render_frame(rt_accel *a, const rt_camera &cam, const rt_light *lights, 
     for (int64_t t0 = 0; t0 < f.tiles; t0 += chunk) {
         const int64_t nt = std::min(chunk, f.tiles - t0);
         if ((rc = trace_chunk(f, w, t0, nt, stats != nullptr))) return rc;
+        if (async) {
+            const int nb = static_cast<int>((nt * 64 * spp + 255) / 256);
+            hipLaunchKernelGGL(k_add_frame_rays, dim3(1), dim3(64), 0, s, w->blocks1 + nb,
+                               p.bounce ? w->blocks2 + nb : nullptr, t0 == 0 ? static_cast<uint64_t>(nout) : 0ull,
+                               nlights, ctx->d_frame_rays);
+        }
         if (stats) RT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[8], s));
         if ((rc = compose_chunk(f, w, t0, nt, img))) return rc;
         if (stats) RT_HIP_CHECK(ctx, hipEventRecord(ctx->ev[9], s));
@@ -756,7 +782,7 @@ int32_t render_frame(rt_accel *a, const rt_camera &cam, const rt_light *lights, 
         RT_HIP_CHECK(ctx, hipMemcpyAsync(primary, hp, nout * sizeof(rt_hit), hipMemcpyDeviceToHost, s));
     if (bounce_hits && !dev_out)
         RT_HIP_CHECK(ctx, hipMemcpyAsync(bounce_hits, hb, nout * sizeof(rt_hit), hipMemcpyDeviceToHost, s));
-    RT_HIP_CHECK(ctx, hipStreamSynchronize(s));
+    if (!async) RT_HIP_CHECK(ctx, hipStreamSynchronize(s));
     if (stats) {
         stats->rays[0] = static_cast<uint64_t>(nout);
         stats->rays[1] = hits1 * nlights;

@@ -83,6 +83,7 @@ struct rt_ctx {
     // Device counters when counting is on: (node visits, triangle tests) for
     // slots 0..3, the stack-overflow flag at index 15.
     unsigned long long *d_counters = nullptr;
+    unsigned long long *d_frame_rays = nullptr;  // rays of RT_ASYNC frames by stage (4)
     uint32_t *d_queue = nullptr;  // chunk counters of the persistent trace kernels (kQueueWords)
     void *d_static = nullptr;     // this device's static scratch (rt_render_static), not owned
     rt::GlareCache *glare = nullptr;  // glare filters (glare.hip)

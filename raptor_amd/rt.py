@@ -19,6 +19,7 @@ RT_ERR_OOM = -3
 RT_ERR_UNSUPPORTED = -4
 RT_DEVICE_PTRS = 1
 RT_REGROUP_BOUNCE = 2
+RT_ASYNC = 4
 
 EPSILON = np.float32(1.0e-3)  # common/common.cc:14
 MAX_DIST = np.float32(np.finfo(np.float32).max)  # common/common.cc:17
@@ -29,7 +30,7 @@ HIT_DTYPE = np.dtype([("t", "<f4"), ("u", "<f4"), ("v", "<f4"), ("prim", "<i4")]
 EXPORTS = [
     "rt_ctx_create", "rt_ctx_destroy", "rt_last_error", "rt_scene_create", "rt_scene_destroy",
     "rt_scene_normals", "rt_accel_build", "rt_accel_destroy", "rt_accel_info_get", "rt_trace_closest",
-    "rt_trace_occluded", "rt_render", "rt_render_static", "rt_set_counting", "rt_gen_random_triangles", "rt_gen_terrain",
+    "rt_trace_occluded", "rt_render", "rt_render_static", "rt_set_counting", "rt_ctx_frame_rays", "rt_gen_random_triangles", "rt_gen_terrain",
     "rt_batch_create", "rt_batch_run", "rt_batch_destroy", "rt_batch_autotune", "rt_gen_surface_rays",
     "rt_scene_set_albedo", "rt_render_tape", "rt_tape_info", "rt_render_adjoint", "rt_tape_destroy",
     "rt_invrender_create", "rt_invrender_run", "rt_invrender_destroy", "rt_scene_update", "rt_accel_refit",
@@ -139,6 +140,7 @@ def load_library(path=LIB_PATH):
         "rt_accel_download": (i32, [vp, fp, fp]),
         "rt_accel_root": (i32, [vp, C.POINTER(C.c_int32), C.POINTER(C.c_float)]),
         "rt_set_counting": (i32, [vp, i32]),
+        "rt_ctx_frame_rays": (i32, [vp, C.POINTER(C.c_uint64)]),
         "rt_batch_create": (i32, [vp, C.POINTER(rt_batch_item), i32, C.POINTER(rt_batch_params), C.POINTER(vp)]),
         "rt_batch_run": (i32, [vp, i32, C.POINTER(C.c_float)]),
         "rt_batch_destroy": (None, [vp]),
@@ -460,6 +462,13 @@ class Context:
 
     def set_counting(self, on):
         self.check(_LIB.rt_set_counting(self.h, 1 if on else 0), "rt_set_counting")
+
+    def frame_rays(self):
+        """Waits for the context's frames; rays of its asynchronous frames
+        since the last call by stage (primary, shadow, bounce, bounce shadow)."""
+        v = (C.c_uint64 * 4)()
+        self.check(_LIB.rt_ctx_frame_rays(self.h, v), "rt_ctx_frame_rays")
+        return [int(x) for x in v]
 
     def close(self):
         if self.h:
@@ -815,24 +824,30 @@ class Accel:
         return out
 
     def render_device(self, camera, lights, rgb_dev, bounce=False, spp=1, seed=0, regroup=False, primary_dev=None,
-                      bounce_dev=None, static=False):
+                      bounce_dev=None, static=False, asynchronous=False):
         """Frame into device buffers (no host copies); returns the stats dict.
         rgb_dev (H, W, 3) float32; primary_dev, bounce_dev: optional (H*W*spp,
         4) float32 hit outputs. static=True calls rt_render_static (no device
-        allocation; the frame's workspace is the library's static scratch)."""
+        allocation; the frame's workspace is the library's static scratch).
+        asynchronous=True enqueues the frame and returns None at once; its rays
+        are summed on the device (Context.frame_rays)."""
         import torch
 
         cam = camera.to_c()
         nl = len(lights)
         larr = (rt_light * max(1, nl))(*[l.to_c() for l in lights])
-        p = rt_render_params(1 if bounce else 0, spp, seed, RT_DEVICE_PTRS | (RT_REGROUP_BOUNCE if regroup else 0), 0)
+        p = rt_render_params(1 if bounce else 0, spp, seed, RT_DEVICE_PTRS | (RT_REGROUP_BOUNCE if regroup else 0)
+                             | (RT_ASYNC if asynchronous else 0), 0)
         st = rt_stats()
         nout = camera.x_res * camera.y_res * spp
         ptr = _dev(rgb_dev, torch.float32, 3 * camera.x_res * camera.y_res, "rt_render: rgb")
         pp = None if primary_dev is None else _dev(primary_dev, torch.float32, 4 * nout, "rt_render: primary")
         bp = None if bounce_dev is None else _dev(bounce_dev, torch.float32, 4 * nout, "rt_render: bounce")
         fn, name = (_LIB.rt_render_static, "rt_render_static") if static else (_LIB.rt_render, "rt_render")
-        self.ctx.check(fn(self.h, C.byref(cam), larr, nl, C.byref(p), ptr, pp, bp, C.byref(st)), name)
+        self.ctx.check(fn(self.h, C.byref(cam), larr, nl, C.byref(p), ptr, pp, bp, None if asynchronous else C.byref(st)),
+                       name)
+        if asynchronous:
+            return None
         return {"rays": list(st.rays), "ms": list(st.ms), "visits": list(st.visits)}
 
     def render_tape(self, camera, lights, rgb_dev, bounce=False, spp=1, seed=0, max_record_bytes=0):
