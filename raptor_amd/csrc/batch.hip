@@ -258,7 +258,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_trace_batch_refill(BatchArgs a,
                 cur = pop();
             }
             const bool searching = scene >= 0 && (cur >= 0) && (leaf == kEmpty);
-            if (!__any(searching)) break;
+            if (__popcll(__ballot(searching)) <= kSearchCut) break;
         }
         // ---- leaf phase
         while (leaf != kEmpty) {

@@ -217,7 +217,7 @@ __global__ void __launch_bounds__(kBlockThreads) k_trace_pers(TraceArgs a) {
                 cur = pop();
             }
             const bool searching = (cur >= 0) && (leaf == kEmpty);
-            if (!__any(searching)) break;
+            if (__popcll(__ballot(searching)) <= kSearchCut) break;
         }
         // ---- leaf phase
         bool occluded = false;
