@@ -127,6 +127,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_trace_batch_refill(BatchArgs a,
     const int slot = threadIdx.x;
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     uint32_t pool = 0, pool_end = 0;  // the wavefront's ray slots [pool, pool_end)
+    int32_t leaf = kEmpty;            // a leaf still to test (it may wait a round)
     bool drained = false;             // the global queue is empty
 
     int32_t scene = -1;  // scene of this lane's ray, -1: idle
@@ -227,7 +228,6 @@ __global__ void __launch_bounds__(64 * kWaves) k_trace_batch_refill(BatchArgs a,
         }
 
         // ---- interior phase (trace_ray's, per lane)
-        int32_t leaf = kEmpty;
         while (true) {
             if (scene >= 0 && cur >= 0) {
                 NodeView<W> v;
@@ -261,30 +261,34 @@ __global__ void __launch_bounds__(64 * kWaves) k_trace_batch_refill(BatchArgs a,
             if (__popcll(__ballot(searching)) <= kSearchCut) break;
         }
         // ---- leaf phase
-        while (leaf != kEmpty) {
-            const int32_t first = leaf_first(leaf);
-            const int32_t count = leaf_count(leaf);
-            for (int k = 0; k < count; ++k) {
-                const LeafTri lt = tris[first + k];
-                const int32_t pid = __float_as_int(lt.a_id.w);
-                float t, u, vv;
-                if (rtd::intersect_mt(o, d, rtd::f4v(lt.a_id), rtd::f4v(lt.e1), rtd::f4v(lt.e2), tmin, &t, &u, &vv)) {
-                    if (t < best_t || (t == best_t && best_id >= 0 && pid < best_id)) {
-                        best_t = t;
-                        best_u = u;
-                        best_v = vv;
-                        best_id = pid;
+        while (true) {
+            if (leaf != kEmpty) {
+                const int32_t first = leaf_first(leaf);
+                const int32_t count = leaf_count(leaf);
+                for (int k = 0; k < count; ++k) {
+                    const LeafTri lt = tris[first + k];
+                    const int32_t pid = __float_as_int(lt.a_id.w);
+                    float t, u, vv;
+                    if (rtd::intersect_mt(o, d, rtd::f4v(lt.a_id), rtd::f4v(lt.e1), rtd::f4v(lt.e2), tmin, &t, &u,
+                                          &vv)) {
+                        if (t < best_t || (t == best_t && best_id >= 0 && pid < best_id)) {
+                            best_t = t;
+                            best_u = u;
+                            best_v = vv;
+                            best_id = pid;
+                        }
                     }
                 }
+                leaf = kEmpty;
+                if (cur != kEmpty && cur < 0) {
+                    leaf = cur;
+                    cur = pop();
+                }
             }
-            leaf = kEmpty;
-            if (cur != kEmpty && cur < 0) {
-                leaf = cur;
-                cur = pop();
-            }
+            if (__popcll(__ballot(leaf != kEmpty)) <= kLeafCut) break;
         }
         // ---- finished rays write their hit and free the lane
-        if (scene >= 0 && cur == kEmpty) {
+        if (scene >= 0 && cur == kEmpty && leaf == kEmpty) {
             a.scenes[scene].hits[ri] = (best_id >= 0) ? make_float4(best_t, best_u, best_v, __int_as_float(best_id))
                                                       : make_float4(rtd::kMaxDist, 0.0f, 0.0f, __int_as_float(-1));
             scene = -1;

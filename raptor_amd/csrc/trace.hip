@@ -150,6 +150,7 @@ __global__ void __launch_bounds__(kBlockThreads) k_trace_pers(TraceArgs a) {
     };
 
     bool drained = false;  // the block's rays are all handed out
+    int32_t leaf = kEmpty;
     while (true) {
         // ---- refill finished lanes from the block's cursor
         const uint64_t idle = __ballot(ri < 0);
@@ -185,7 +186,6 @@ __global__ void __launch_bounds__(kBlockThreads) k_trace_pers(TraceArgs a) {
         if (!__any(ri >= 0)) break;
 
         // ---- interior phase (trace_ray's, per lane)
-        int32_t leaf = kEmpty;
         while (true) {
             if (cur >= 0) {
                 NodeView<W> v;
@@ -221,38 +221,42 @@ __global__ void __launch_bounds__(kBlockThreads) k_trace_pers(TraceArgs a) {
         }
         // ---- leaf phase
         bool occluded = false;
-        while (leaf != kEmpty) {
-            const int32_t first = leaf_first(leaf);
-            const int32_t count = leaf_count(leaf);
-            for (int k = 0; k < count; ++k) {
-                const LeafTri lt = a.tris[first + k];
-                const int32_t pid = __float_as_int(lt.a_id.w);
-                if (COUNT) ++tests;
-                if (!CLOSEST && (__float_as_int(lt.e1.w) & 3)) continue;  // lights, transparent
-                float t, u, vv;
-                if (rtd::intersect_mt(o, d, rtd::f4v(lt.a_id), rtd::f4v(lt.e1), rtd::f4v(lt.e2), tmin, &t, &u, &vv)) {
-                    if (CLOSEST) {
-                        if (t < best_t || (t == best_t && best_id >= 0 && pid < best_id)) {
-                            best_t = t;
-                            best_u = u;
-                            best_v = vv;
-                            best_id = pid;
+        while (true) {
+            if (leaf != kEmpty) {
+                const int32_t first = leaf_first(leaf);
+                const int32_t count = leaf_count(leaf);
+                for (int k = 0; k < count; ++k) {
+                    const LeafTri lt = a.tris[first + k];
+                    const int32_t pid = __float_as_int(lt.a_id.w);
+                    if (COUNT) ++tests;
+                    if (!CLOSEST && (__float_as_int(lt.e1.w) & 3)) continue;  // lights, transparent
+                    float t, u, vv;
+                    if (rtd::intersect_mt(o, d, rtd::f4v(lt.a_id), rtd::f4v(lt.e1), rtd::f4v(lt.e2), tmin, &t, &u,
+                                          &vv)) {
+                        if (CLOSEST) {
+                            if (t < best_t || (t == best_t && best_id >= 0 && pid < best_id)) {
+                                best_t = t;
+                                best_u = u;
+                                best_v = vv;
+                                best_id = pid;
+                            }
+                        } else if (t < tmax) {
+                            occluded = true;
+                            break;
                         }
-                    } else if (t < tmax) {
-                        occluded = true;
-                        break;
                     }
                 }
+                leaf = kEmpty;
+                if (!CLOSEST && occluded) break;
+                if (cur != kEmpty && cur < 0) {
+                    leaf = cur;
+                    cur = pop();
+                }
             }
-            leaf = kEmpty;
-            if (!CLOSEST && occluded) break;
-            if (cur != kEmpty && cur < 0) {
-                leaf = cur;
-                cur = pop();
-            }
+            if (__popcll(__ballot(leaf != kEmpty)) <= kLeafCut) break;
         }
         // ---- finished rays write their result and free the lane
-        if (ri >= 0 && (cur == kEmpty || (!CLOSEST && occluded))) {
+        if (ri >= 0 && ((cur == kEmpty && leaf == kEmpty) || (!CLOSEST && occluded))) {
             if (CLOSEST) {
                 const float ht = (best_id >= 0) ? best_t : rtd::kMaxDist;
                 const float hu = (best_id >= 0) ? best_u : 0.0f;
@@ -263,6 +267,7 @@ __global__ void __launch_bounds__(kBlockThreads) k_trace_pers(TraceArgs a) {
             }
             ri = -1;
             cur = kEmpty;
+            leaf = kEmpty;
         }
     }
     if (COUNT) {
