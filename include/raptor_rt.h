@@ -219,6 +219,16 @@ int32_t rt_accel_refit(rt_accel *accel);
  * c += clrf c + (1 - clrf) refracted x transmittance. */
 #define RT_MATERIAL_COLOURED 2
 #define RT_MATERIAL_MAPPER 3
+/* kind RT_MATERIAL_MANDELBROT: a mandelbrot_shader
+ * (rt/materials/mandelbrot_shader.{h,cc}) with iteration limit iterations
+ * (>= 1); the constructor's x and y widths do not enter its shading. One
+ * shadow ray per light and no secondary rays; colour = the escape-time
+ * colour of the hit point's fractal coordinates (taken on the y-z plane
+ * where the shading normal is exactly +-x, on x-y where it is exactly +-z,
+ * else on x-z; a point inside the set is tried once more on shifted,
+ * 1.5x scaled coordinates, then black) times the sum over lit lights of
+ * n.l where n.l >= 0; no light colour, falloff or ambient term. */
+#define RT_MATERIAL_MANDELBROT 4
 typedef struct {
     float ka[3], kd[3], ks[3];
     float shininess, transmittance, refractive_index, reflectance;
@@ -228,6 +238,7 @@ typedef struct {
     int32_t tex[6];
     float mapper_kd, mapper_ks, clrh, clrf;
     int32_t layer_first[5], layer_count[5];
+    int32_t iterations;  /* RT_MATERIAL_MANDELBROT */
 } rt_material;
 
 /* ---- Texture maps (rt/materials/texture_mapper.h) ----
@@ -247,13 +258,36 @@ typedef struct {
  *   int((x - c.x) / s.x) is odd (the reference's test xors the z cell with
  *   itself, so only x decides), otherwise the colour is left alone;
  *   opacity is the map's alpha.
+ *   RT_MAP_PERLIN_3D (perlin_noise_3d_mapper, rt/materials/
+ *   perlin_noise_3d_mapper.{h,cc}): octaves octaves of the seeded value
+ *   noise of common/perlin_noise_3d.cc at the hit point times frequency
+ *   1 / zoom, doubling, amplitudes persistence^k; total + 1 scales rgb / 2;
+ *   alpha 0 where total + 1 > 1, else opacity.
+ *   RT_MAP_GRADIENT (gradient_mapper, rt/materials/gradient_mapper.{h,cc}):
+ *   the key is the incidence angle acos(|n.dir|) in degrees; key_count
+ *   keys of 6 floats at texels + key_offset, {key, r, g, b, alpha,
+ *   interpolation}, keys non-decreasing; below the first key the first
+ *   value, from the last key on the last value, else the value interpolated
+ *   by the upper key's interpolation (RT_GRAD_LINE: colour linear in the
+ *   key, alpha the lower key's; RT_GRAD_STEP: the upper value). grad_of must
+ *   be RT_GRAD_OF_INCIDENCE_ANGLE and spline interpolation is refused: the
+ *   reference asserts on the rest. The mapper inverts its own colour when
+ *   inverse is set, before the falloff, and the map inverts it again after.
  * Then, per map, a falloff (mapper_falloff: cubic, spherical, linear x, y,
  * z around falloff_centre with gradient falloff_grad; RT_FALLOFF_NONE
- * skips it) and inverse (255 - colour). */
+ * skips it) and inverse (255 - colour). The monochrome value of a
+ * procedural map (checker board, Perlin, gradient) is its colour's mean. */
 #define RT_MAP_PLANAR 0
 #define RT_MAP_CUBIC 1
 #define RT_MAP_CYLINDRICAL 2
 #define RT_MAP_CHECKER_BOARD 3
+#define RT_MAP_PERLIN_3D 4
+#define RT_MAP_GRADIENT 5
+/* grad_interpolation_t and grad_of_t (rt/materials/gradient_mapper.h). */
+#define RT_GRAD_LINE 0
+#define RT_GRAD_SPLINE 1
+#define RT_GRAD_STEP 2
+#define RT_GRAD_OF_INCIDENCE_ANGLE 3
 /* texture_wrapping_mode_t (texture_mapper.h apply_wrapping_mode): blank
  * (black outside), clamp, tile (negative texels wrap by size - 1), mirror. */
 #define RT_WRAP_BLANK 0
@@ -278,6 +312,10 @@ typedef struct {
     int32_t inverse;
     int32_t falloff;            /* RT_FALLOFF_* */
     float falloff_centre[3], falloff_grad[3];
+    float persistence, zoom;    /* Perlin */
+    int32_t octaves, seed;      /* Perlin */
+    int32_t key_count, grad_of; /* gradient */
+    int64_t key_offset;         /* gradient: first key's float in the texels */
 } rt_texture_map;
 /* An image: width x height texels of cpp (1, 3 or 4) floats, 0-255, row 0
  * first (the readers' order: the image's bottom row for TGA/PNG/JPEG),

@@ -38,7 +38,8 @@ MAP_DTYPE = np.dtype([("kind", "<i4"), ("image", "<i4"), ("c", "<f4", 3), ("n", 
                       ("u", "<f4", 3), ("v", "<f4", 3), ("u_wrap", "<i4"), ("v_wrap", "<i4"), ("u_off", "<i4"),
                       ("v_off", "<i4"), ("cycles", "<i4"), ("rgb", "<f4", 3), ("opacity", "<f4"),
                       ("inverse", "<i4"), ("falloff", "<i4"), ("falloff_centre", "<f4", 3),
-                      ("falloff_grad", "<f4", 3)])
+                      ("falloff_grad", "<f4", 3), ("persistence", "<f4"), ("zoom", "<f4"), ("octaves", "<i4"),
+                      ("seed", "<i4"), ("key_count", "<i4"), ("grad_of", "<i4"), ("key_offset", "<i8")])
 IMAGE_DTYPE = np.dtype([("width", "<i4"), ("height", "<i4"), ("cpp", "<i4"), ("reserved", "<i4"),
                         ("offset", "<i8")])
 
@@ -65,16 +66,20 @@ def material_array(materials):
     x, y (chroma), rd, rs, ts, sr, ri_r, ri_i. Coloured mapper (kind 2): the
     phong fields and tex (map indices or None for t_ka, t_kd, t_ks, t_ns,
     t_refl, t_d). Mapper shader (kind 3): rgb, kd, ks (scalars), s, tran,
-    ri, rf, clrh, clrf, layers ((first, count) x 5). Returns an (n, 39)
-    float32 array in the oracle's row layout."""
+    ri, rf, clrh, clrf, layers ((first, count) x 5). Mandelbrot shader
+    (kind 4): iterations. Returns an (n, 40) float32 array in the oracle's
+    row layout."""
     rows = []
     for m in materials:
         kind = m.get("kind", 0)
-        tail = [0.0] * 20
+        tail = [0.0] * 21
         if kind == 1:
             row = [0.0] * 10 + [m.get("ts", 0.0), m.get("ri_r", 1.0), m.get("rs", 0.0), 1.0,
                                 m.get("x", 0.33), m.get("y", 0.33), m.get("rd", 0.5), m.get("sr", 0.0),
                                 m.get("ri_i", 0.0)]
+        elif kind == 4:
+            row = [0.0] * 13 + [4.0, 0.33, 0.33, 0.5, 0.0, 0.0]
+            tail[20] = float(m.get("iterations", 100))
         elif kind == 3:
             row = [0.0] * 3 + list(m.get("rgb", (0.0, 0.0, 0.0))) + [0.0] * 3 + \
                 [m.get("s", 0.0), m.get("tran", 0.0), m.get("ri", 1.0), m.get("rf", 0.0)] + \
@@ -90,7 +95,7 @@ def material_array(materials):
             for k, t in enumerate(m.get("tex", ())):
                 tail[k] = 0.0 if t is None or t < 0 else float(t + 1)
         rows.append(row + tail)
-    return np.asarray(rows, np.float32).reshape(-1, 39)
+    return np.asarray(rows, np.float32).reshape(-1, 40)
 
 
 def build():
@@ -139,7 +144,9 @@ def _load():
         "oracle_gauss_circular3d": (None, [vp, i32, i32, i32, f, f]),
         "oracle_clip": (None, [vp, i32, i32, i32, i32, i32, vp]),
         "oracle_mt_stream": (None, [i32, vp]),
-        "oracle_texture_sample": (f, [vp, vp, vp, i32, vp, i32, f, f, i32, vp]),
+        "oracle_texture_sample": (f, [vp, vp, vp, i32, vp, vp, vp, i32, f, f, i32, vp]),
+        "oracle_perlin_noise": (f, [f, f, f, i32, i32]),
+        "oracle_mandelbrot_colour": (None, [vp, vp, i32, vp]),
         "oracle_glare_filter": (None, [i32, i32, f, i32, vp]),
         "oracle_glare": (i32, [vp, i32, i32, f, f, vp]),
     }
@@ -284,7 +291,7 @@ class OracleScene:
                 mats = mats.reshape(-1, mats.shape[-1] if mats.ndim > 1 else mats.size)
                 if mats.shape[1] == 19:  # rows without textures: no texture slots, no layers
                     mats = np.concatenate([mats, np.zeros((mats.shape[0], 20), np.float32)], axis=1)
-                mats = np.ascontiguousarray(mats).reshape(-1, 39)
+                mats = np.ascontiguousarray(mats).reshape(-1, 40)
             else:
                 mats = material_array(materials)
             tm = np.ascontiguousarray(tri_material, np.int32)
@@ -477,18 +484,36 @@ def tone_map(rgb, mode=0, key=0.18, xw=1.0 / 3.0, yw=1.0 / 3.0, desaturate=False
     return img, float(lvl[0]), float(yi[0])
 
 
-def texture_sample(maps, images, texels, index, dst, vt=None, mono=False, rgb=(0.0, 0.0, 0.0)):
+def texture_sample(maps, images, texels, index, dst, vt=None, mono=False, rgb=(0.0, 0.0, 0.0), dir=(0.0, 0.0, 1.0),
+                   n=(0.0, 0.0, -1.0)):
     """texture_map (rgb in, rgb out) or texture_map_monochrome (the value) of
-    maps[index] at hit point dst, through texture coordinates vt = (u, v)
-    when given. Returns (colour or value, alpha)."""
+    maps[index] at hit point dst of a ray of direction dir on a surface of
+    shading normal n, through texture coordinates vt = (u, v) when given.
+    Returns (colour or value, alpha)."""
     maps, images = np.ascontiguousarray(maps), np.ascontiguousarray(images)
     texels = np.ascontiguousarray(texels, np.float32)
-    d = f32a(*dst)
+    d, rd, nn = f32a(*dst), f32a(*dir), f32a(*n)
     c = np.asarray(rgb, np.float32).copy()
     a = LIB.oracle_texture_sample(_p(maps), _p(images) if images.size else None, _p(texels) if texels.size else None,
-                                  index, _p(d), 0 if vt is None else 1, 0.0 if vt is None else float(vt[0]),
+                                  index, _p(d), _p(rd), _p(nn), 0 if vt is None else 1,
+                                  0.0 if vt is None else float(vt[0]),
                                   0.0 if vt is None else float(vt[1]), 1 if mono else 0, _p(c))
     return (float(c[0]) if mono else c), float(a)
+
+
+def perlin_noise(x, y, z, seed, lattice=False):
+    """perlin_noise_3d::noise at integer (x, y, z) (lattice) or
+    interpolated_noise at (x, y, z)."""
+    return float(LIB.oracle_perlin_noise(float(x), float(y), float(z), int(seed), 1 if lattice else 0))
+
+
+def mandelbrot_colour(dst, n, iterations):
+    """mandelbrot_shader's colour at hit point dst, shading normal n, before
+    the light term."""
+    out = np.zeros(3, np.float32)
+    d, nn = f32a(*dst), f32a(*n)
+    LIB.oracle_mandelbrot_colour(_p(d), _p(nn), int(iterations), _p(out))
+    return out
 
 
 def mt_stream(count):

@@ -291,10 +291,11 @@ void render_pixel(const FrameCtx &f, int px, int py, LocalCounters *lc) {
 // double, so a power of exactly 0.1f (above 0.1) still recurses.
 constexpr double kMinReflectivePower = 0.1;
 
-constexpr int kMaterialFloats = 39;
+constexpr int kMaterialFloats = 40;
 constexpr int kCookTorrance = 1;
 constexpr int kColoured = 2;
 constexpr int kMapper = 3;
+constexpr int kMandelbrot = 4;
 
 struct Phong {
     Colour ka, kd, ks;
@@ -304,6 +305,7 @@ struct Phong {
     int tex[6] = {0, 0, 0, 0, 0, 0};  // 1 + map: t_ka, t_kd, t_ks, t_ns, t_refl, t_d
     float mkd = 0.0f, mks = 0.0f, clrh = 0.0f, clrf = 0.0f;
     int first[5] = {0, 0, 0, 0, 0}, count[5] = {0, 0, 0, 0, 0};  // ctex, dtex, stex, rtex, ttex
+    int iters = 0;  // mandelbrot_shader's iteration limit
 };
 
 // XYZ -> RGB for the primaries and white point of rt/materials/material.h:17-45,
@@ -396,6 +398,7 @@ Phong material_of(const FrameCtx &f, int prim) {
             m.first[k] = static_cast<int>(q[29 + k]);
             m.count[k] = static_cast<int>(q[34 + k]);
         }
+        m.iters = static_cast<int>(q[39]);
     } else {
         m.kd = kd_of(f, prim);  // phong_shader(255 * albedo, 1): ks = 0 * rgb, ka = 0
     }
@@ -419,6 +422,59 @@ bool refract_dir(const V3 &dir, const V3 &n, float ri, bool leaving, V3 *out) {
     const float offset = (ri * rdn - std::sqrt(cos_t2));
     *out = (dir * ri) + (n * offset);
     return true;
+}
+
+// mandelbrot_shader::shade's colour (rt/materials/mandelbrot_shader.cc:41-123)
+// before the light term: the fractal coordinates of the hit point on the
+// plane picked by the shading normal, the escape-time loop, one retry on
+// shifted coordinates for a point that does not escape (black if it again
+// does not), and the colour ramp.
+Colour mandelbrot_colour(const V3 &dst, const V3 &n, int limit) {
+    float x0;
+    float y0;
+    if ((n.x == 1.0f) || (n.x == -1.0f)) {
+        x0 = std::fmod(std::fabs(dst.z), 2.4f) - 1.5f;
+        y0 = std::fmod(std::fabs(dst.y) - 1.5f, 2.4f) - 1.0f;
+    } else if ((n.z == 1.0f) || (n.z == -1.0f)) {
+        x0 = std::fmod(std::fabs(dst.x), 2.4f) - 1.5f;
+        y0 = std::fmod(std::fabs(dst.y) - 1.5f, 2.4f) - 1.0f;
+    } else {
+        x0 = std::fmod(std::fabs(dst.x) - 1.5f, 2.4f) - 1.0f;
+        y0 = std::fmod(std::fabs(dst.z), 2.4f) - 1.5f;
+    }
+    float x = x0;
+    float y = y0;
+    bool doomed = false;
+    for (;;) {
+        int iteration = 0;
+        while ((x * x + y * y <= (10.0f * 10.0f)) && (iteration < limit)) {
+            const float xtemp = x * x - y * y + x0;
+            y = 2.0f * x * y + y0;
+            x = xtemp;
+            ++iteration;
+        }
+        if (doomed && (iteration == limit)) return Colour{0.0f, 0.0f, 0.0f};
+        if (iteration == limit) {
+            doomed = true;
+            if ((n.x == 1.0f) || (n.x == -1.0f)) {
+                x0 = std::fmod(std::fabs(dst.z) - 1.0f, 2.4f) * 1.5f - 0.4f;
+                y0 = std::fmod(std::fabs(dst.y) - 2.0f, 2.4f) * 1.5f - 0.7f;
+            } else if ((n.z == 1.0f) || (n.z == -1.0f)) {
+                x0 = std::fmod(std::fabs(dst.x) - 1.0f, 2.4f) * 1.5f - 0.4f;
+                y0 = std::fmod(std::fabs(dst.y) - 2.0f, 2.4f) * 1.5f - 0.7f;
+            } else {
+                x0 = std::fmod(std::fabs(dst.x) - 2.0f, 2.4f) * 1.5f - 0.7f;
+                y0 = std::fmod(std::fabs(dst.z) - 1.0f, 2.4f) * 1.5f - 0.4f;
+            }
+            x = x0;
+            y = y0;
+            continue;
+        }
+        const float colour = 767.0f - ((iteration * 767.0f) / limit);
+        if (colour < 256.0f) return Colour{25.0f - colour, 255.0f, 255.0f};
+        if (colour < 512.0f) return Colour{0.0f, 256.0f - (colour - 256.0f), 255.0f};
+        return Colour{0.0f, 0.0f, 256.0f - (colour - 767.0f)};
+    }
 }
 
 struct PhongCounters {
@@ -471,7 +527,7 @@ Colour trace_phong(const FrameCtx &f, Ray &r, bool primary, LocalCounters *lc, P
         float alpha = 1.0f;
         for (int k = 0; k < m.count[list]; ++k) {
             float z;
-            alpha *= oracle_tex::texture_map_monochrome(T, f.p.layers[m.first[list] + k], &z, sh.dst, has_vt, vtu, vtv);
+            alpha *= oracle_tex::texture_map_monochrome(T, f.p.layers[m.first[list] + k], &z, sh.dst, r.dir, sh.shade_norm, has_vt, vtu, vtv);
             value = z * (1.0f / 255.0f);
             if (alpha == 0.0f) break;
         }
@@ -483,12 +539,12 @@ Colour trace_phong(const FrameCtx &f, Ray &r, bool primary, LocalCounters *lc, P
     if (m.kind == kColoured) {
         if (m.tex[4]) {
             oracle_tex::Rgb3 tmp;
-            oracle_tex::texture_map(T, m.tex[4] - 1, &tmp, sh.dst, has_vt, vtu, vtv);
+            oracle_tex::texture_map(T, m.tex[4] - 1, &tmp, sh.dst, r.dir, sh.shade_norm, has_vt, vtu, vtv);
             refl_c = tmp.r * (m.rf * (1.0f / 255.0f));
         }
         if (m.tex[5]) {
             oracle_tex::Rgb3 tmp;
-            oracle_tex::texture_map(T, m.tex[5] - 1, &tmp, sh.dst, has_vt, vtu, vtv);
+            oracle_tex::texture_map(T, m.tex[5] - 1, &tmp, sh.dst, r.dir, sh.shade_norm, has_vt, vtu, vtv);
             tran_c = tmp.r * (m.tran * (1.0f / 255.0f));
         }
     } else if (m.kind == kMapper) {
@@ -503,13 +559,13 @@ Colour trace_phong(const FrameCtx &f, Ray &r, bool primary, LocalCounters *lc, P
         auto mapped = [&](int slot, Colour col) {
             if (!slot) return col;
             oracle_tex::Rgb3 t{col.r, col.g, col.b};
-            oracle_tex::texture_map(T, slot - 1, &t, sh.dst, has_vt, vtu, vtv);
+            oracle_tex::texture_map(T, slot - 1, &t, sh.dst, r.dir, sh.shade_norm, has_vt, vtu, vtv);
             return Colour{t.r * (col.r * (1.0f / 255.0f)), t.g * (col.g * (1.0f / 255.0f)),
                           t.b * (col.b * (1.0f / 255.0f))};
         };
         if (m.tex[3]) {
             oracle_tex::Rgb3 cur_ka;
-            ns = oracle_tex::texture_map(T, m.tex[3] - 1, &cur_ka, sh.dst, has_vt, vtu, vtv);
+            ns = oracle_tex::texture_map(T, m.tex[3] - 1, &cur_ka, sh.dst, r.dir, sh.shade_norm, has_vt, vtu, vtv);
             ns *= (m.s * (1.0f / 255.0f));
         }
         ka = mapped(m.tex[0], m.ka);
@@ -519,7 +575,7 @@ Colour trace_phong(const FrameCtx &f, Ray &r, bool primary, LocalCounters *lc, P
         oracle_tex::Rgb3 rgb{m.kd.r, m.kd.g, m.kd.b};
         float alpha = 1.0f;
         for (int k = 0; k < m.count[0]; ++k) {
-            alpha *= oracle_tex::texture_map(T, f.p.layers[m.first[0] + k], &rgb, sh.dst, has_vt, vtu, vtv);
+            alpha *= oracle_tex::texture_map(T, f.p.layers[m.first[0] + k], &rgb, sh.dst, r.dir, sh.shade_norm, has_vt, vtu, vtv);
             if (alpha == 0.0f) break;
         }
         kd = Colour{rgb.r, rgb.g, rgb.b};
@@ -527,6 +583,7 @@ Colour trace_phong(const FrameCtx &f, Ray &r, bool primary, LocalCounters *lc, P
         ns = layered(2, m.s);
     }
     const float nv = dot(n, r.dir);  // cook_torrance_cxy: one dot product for every light
+    float total_shade = 0.0f;        // mandelbrot_shader: sum of n.l x visibility
     for (size_t l = 0; l < f.lights.size(); ++l) {
         Ray illum = shadow_ray(sh, f.lights[l].c);
         const bool occluded = found_nearer_object(*f.s, illum, illum.length, &lc->trav);
@@ -561,6 +618,10 @@ Colour trace_phong(const FrameCtx &f, Ray &r, bool primary, LocalCounters *lc, P
         }
         const float shade = dot(illum.dir, n);
         if (shade < 0.0f) continue;
+        if (m.kind == kMandelbrot) {  // mandelbrot_shader.cc:23-38: diffuse only, no light colour
+            total_shade += shade * illum.magn;
+            continue;
+        }
         if (m.kind == kCookTorrance) {
             cook_torrance_term(m, n, r.dir, nv, illum.dir, shade, illum.magn, li, &c);
             continue;
@@ -580,7 +641,12 @@ Colour trace_phong(const FrameCtx &f, Ray &r, bool primary, LocalCounters *lc, P
         c.g += li.g * (sc.g * illum.magn);
         c.b += li.b * (sc.b * illum.magn);
     }
-    if (m.kind != kCookTorrance && m.kind != kMapper) {  // no ambient term in those two
+    if (m.kind == kMandelbrot) {
+        c = mandelbrot_colour(sh.dst, n, m.iters);
+        c.r *= total_shade;
+        c.g *= total_shade;
+        c.b *= total_shade;
+    } else if (m.kind != kCookTorrance && m.kind != kMapper) {  // no ambient term in those two
         c.r += ka.r;
         c.g += ka.g;
         c.b += ka.b;
@@ -1040,19 +1106,35 @@ extern "C" {
 // One texture_map (mono = 0: rgb in/out) or texture_map_monochrome (mono =
 // 1: rgb[0] out) of map `map` at hit point dst; returns the alpha.
 float oracle_texture_sample(const oracle_texture_map *maps, const oracle_texture_image *images, const float *texels,
-                            int32_t map, const float *dst, int32_t has_vt, float vtu, float vtv, int32_t mono,
-                            float *rgb) {
+                            int32_t map, const float *dst, const float *dir, const float *n, int32_t has_vt,
+                            float vtu, float vtv, int32_t mono, float *rgb) {
     const oracle_tex::Tables T{maps, images, texels, nullptr};
-    const oracle::V3 d(dst[0], dst[1], dst[2]);
-    if (mono) return oracle_tex::texture_map_monochrome(T, map, rgb, d, has_vt != 0, vtu, vtv);
+    const oracle::V3 d(dst[0], dst[1], dst[2]), rd(dir[0], dir[1], dir[2]), nn(n[0], n[1], n[2]);
+    if (mono) return oracle_tex::texture_map_monochrome(T, map, rgb, d, rd, nn, has_vt != 0, vtu, vtv);
     oracle_tex::Rgb3 c{rgb[0], rgb[1], rgb[2]};
-    const float a = oracle_tex::texture_map(T, map, &c, d, has_vt != 0, vtu, vtv);
+    const float a = oracle_tex::texture_map(T, map, &c, d, rd, nn, has_vt != 0, vtu, vtv);
     rgb[0] = c.r;
     rgb[1] = c.g;
     rgb[2] = c.b;
     return a;
 }
 
+
+float oracle_perlin_noise(float x, float y, float z, int32_t seed, int32_t lattice) {
+    if (lattice)
+        return oracle_tex::perlin_noise(static_cast<uint32_t>(static_cast<int32_t>(x)),
+                                        static_cast<uint32_t>(static_cast<int32_t>(y)),
+                                        static_cast<uint32_t>(static_cast<int32_t>(z)), static_cast<uint32_t>(seed));
+    return oracle_tex::perlin_interpolated_noise(x, y, z, static_cast<uint32_t>(seed));
+}
+
+void oracle_mandelbrot_colour(const float *dst, const float *n, int32_t iterations, float *rgb) {
+    const oracle::Colour c =
+        oracle::mandelbrot_colour(oracle::V3(dst[0], dst[1], dst[2]), oracle::V3(n[0], n[1], n[2]), iterations);
+    rgb[0] = c.r;
+    rgb[1] = c.g;
+    rgb[2] = c.b;
+}
 
 void oracle_reflect_dir(const float *d, const float *n, float *out) {
     const oracle::V3 r = oracle::reflect_dir(oracle::V3(d[0], d[1], d[2]), oracle::V3(n[0], n[1], n[2]));

@@ -32,7 +32,8 @@ typedef struct {
 } oracle_light;
 
 /* A texture map's state (rt/materials/texture_mapper.h and the mappers'
- * constructors): kind 0 planar, 1 cubic, 2 cylindrical, 3 checker board;
+ * constructors): kind 0 planar, 1 cubic, 2 cylindrical, 3 checker board,
+ * 4 Perlin 3-D noise, 5 gradient over the incidence angle;
  * wrapping 0 blank, 1 clamp, 2 tile, 3 mirror; falloff 0 cubic, 1
  * spherical, 2-4 linear x/y/z, 5 none. Same layout as the library's
  * rt_texture_map. */
@@ -48,6 +49,10 @@ typedef struct {
     int32_t inverse;
     int32_t falloff;
     float falloff_centre[3], falloff_grad[3];
+    float persistence, zoom; /* kind 4: perlin_noise_3d_mapper */
+    int32_t octaves, seed;
+    int32_t key_count, grad_of; /* kind 5: gradient_mapper, keys of 6 floats */
+    int64_t key_offset;         /* {key, r, g, b, alpha, interpolation} in the texels */
 } oracle_texture_map;
 
 /* width x height texels of cpp floats at texels + offset, row 0 first. */
@@ -67,12 +72,13 @@ typedef struct {
     int32_t pixel_stride; /* trace pixels p with p % stride == 0 (packet frame: 8x8 tiles) */
     int32_t nthreads;
     int32_t packet;       /* 1: the SSE packet frame (spp 1, bounce <= 1) */
-    /* Materials: nmaterials x 39 floats ka rgb, kd rgb, ks rgb, s, tran,
+    /* Materials: nmaterials x 40 floats ka rgb, kd rgb, ks rgb, s, tran,
      * ri, rf, kind, cx, cy, rd, sr, ri_i, six texture slots (1 + map, 0
      * none: t_ka, t_kd, t_ks, t_ns, t_refl, t_d of a coloured_mapper_shader,
      * kind 2), mapper kd, mapper ks, clrh, clrf, and five layer lists
      * (first x 5, count x 5) of a mapper_shader (kind 3, base colour in kd
-     * rgb), and one index per triangle (-1:
+     * rgb), the iteration limit of a mandelbrot_shader (kind 4), and one
+     * index per triangle (-1:
      * phong_shader(255 * albedo, 1), the Lambert default). kind 0 is
      * phong_shader (rt/materials/phong_shader.h:34-53, coloured form; the
      * last five floats unused); kind 1 is cook_torrance_cxy
@@ -130,11 +136,18 @@ typedef struct {
 } oracle_render_out;
 
 /* One texture_map (mono 0: rgb in and out) or texture_map_monochrome (mono
- * 1: the value in rgb[0]) of maps[map] at hit point dst, with texture
- * coordinates (vtu, vtv) when has_vt; returns the alpha. */
+ * 1: the value in rgb[0]) of maps[map] at hit point dst of a ray with
+ * direction dir on a surface of shading normal n, with texture coordinates
+ * (vtu, vtv) when has_vt; returns the alpha. */
 float oracle_texture_sample(const oracle_texture_map *maps, const oracle_texture_image *images, const float *texels,
-                            int32_t map, const float *dst, int32_t has_vt, float vtu, float vtv, int32_t mono,
-                            float *rgb);
+                            int32_t map, const float *dst, const float *dir, const float *n, int32_t has_vt,
+                            float vtu, float vtv, int32_t mono, float *rgb);
+/* perlin_noise_3d of seed: lattice 1 gives noise(int x, int y, int z),
+ * lattice 0 interpolated_noise(x, y, z) (common/perlin_noise_3d.cc). */
+float oracle_perlin_noise(float x, float y, float z, int32_t seed, int32_t lattice);
+/* mandelbrot_shader's colour at hit point dst with shading normal n, before
+ * the light term (rt/materials/mandelbrot_shader.cc:41-123). */
+void oracle_mandelbrot_colour(const float *dst, const float *n, int32_t iterations, float *rgb);
 oracle_scene *oracle_scene_create(const float *abc, int64_t n, const uint8_t *flags);
 void oracle_scene_destroy(oracle_scene *s);
 /* AAC build (alpha, delta as ssd/bvh_builder.h:23). Returns the root. */

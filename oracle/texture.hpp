@@ -6,8 +6,11 @@
 // planar, cubic and cylindrical texture_coordinates
 // (rt/materials/planar_mapper.cc:8-17, cubic_mapper.cc:8-21,
 // cylindrical_mapper.cc:8-20), checker_board_mapper::run_procedure
-// (rt/materials/checker_board_mapper.cc:8-21), procedural_texture_mapper's
-// samples and mapper_falloff::falloff (rt/materials/mapper_falloff.cc:9-61).
+// (rt/materials/checker_board_mapper.cc:8-21), perlin_noise_3d_mapper
+// (rt/materials/perlin_noise_3d_mapper.cc:12-40 over common/perlin_noise_3d.cc),
+// gradient_mapper (rt/materials/gradient_mapper.cc:12-105),
+// procedural_texture_mapper's samples and mapper_falloff::falloff
+// (rt/materials/mapper_falloff.cc:9-61).
 // The reference's operand order is kept in every float expression.
 #pragma once
 
@@ -192,6 +195,138 @@ inline float run_procedure(const oracle_texture_map &t, Rgb3 *c, const V3 &dst) 
     return t.opacity;
 }
 
+// perlin_noise_3d::noise (common/perlin_noise_3d.cc:10-15). The reference
+// computes in int and its overflow wraps on x86; uint32_t wraps the same
+// way without the undefined behaviour.
+inline float perlin_noise(uint32_t x, uint32_t y, uint32_t z, uint32_t s) {
+    uint32_t n = x + y * 57u + z * 13u + s;
+    n = (n << 13) ^ n;
+    const int32_t masked = static_cast<int32_t>((n * (n * n * 15731u + 789221u) + 1376312589u) & 0x7fffffffu);
+    return static_cast<float>(1.0f - masked / 1073741824.0f);
+}
+
+// perlin_noise_3d::smooth_noise (perlin_noise_3d.cc:26-32).
+inline float perlin_smooth_noise(uint32_t x, uint32_t y, uint32_t z, uint32_t s) {
+    const float sides = (perlin_noise(x - 1, y, z, s) + perlin_noise(x + 1, y, z, s) + perlin_noise(x, y - 1, z, s) +
+                         perlin_noise(x, y + 1, z, s) + perlin_noise(x, y, z - 1, s) + perlin_noise(x, y, z + 1, s)) *
+                        (1.0f / 16.0f);
+    const float center = perlin_noise(x, y, z, s) * (1.0f / 4.0f);
+    return sides + center;
+}
+
+// perlin_noise_3d::interpolate (perlin_noise_3d.cc:41-46).
+inline float perlin_interpolate(float a, float b, float x) {
+    const float ft = x * kPi;
+    const float f = (1.0f - std::cos(ft)) * 0.5f;
+    return a * (1.0f - f) + b * f;
+}
+
+// perlin_noise_3d::interpolated_noise (perlin_noise_3d.cc:55-80).
+inline float perlin_interpolated_noise(float x, float y, float z, uint32_t s) {
+    const int wx = to_int(static_cast<float>(to_int(x)) + 0.5f);
+    const float px = x - wx;
+    const int wy = to_int(static_cast<float>(to_int(y)) + 0.5f);
+    const float py = y - wy;
+    const int wz = to_int(static_cast<float>(to_int(z)) + 0.5f);
+    const float pz = z - wz;
+    const uint32_t X = static_cast<uint32_t>(wx), Y = static_cast<uint32_t>(wy), Z = static_cast<uint32_t>(wz);
+    const float v1 = perlin_smooth_noise(X, Y, Z, s);
+    const float v2 = perlin_smooth_noise(X + 1, Y, Z, s);
+    const float v3 = perlin_smooth_noise(X, Y + 1, Z, s);
+    const float v4 = perlin_smooth_noise(X + 1, Y + 1, Z, s);
+    const float v5 = perlin_smooth_noise(X, Y, Z + 1, s);
+    const float v6 = perlin_smooth_noise(X + 1, Y, Z + 1, s);
+    const float v7 = perlin_smooth_noise(X, Y + 1, Z + 1, s);
+    const float v8 = perlin_smooth_noise(X + 1, Y + 1, Z + 1, s);
+    const float ix0 = perlin_interpolate(v1, v2, px);
+    const float ix1 = perlin_interpolate(v3, v4, px);
+    const float ix2 = perlin_interpolate(v5, v6, px);
+    const float ix3 = perlin_interpolate(v7, v8, px);
+    const float iy0 = perlin_interpolate(ix0, ix1, py);
+    const float iy1 = perlin_interpolate(ix2, ix3, py);
+    return perlin_interpolate(iy0, iy1, pz);
+}
+
+// perlin_noise_3d_mapper::run_procedure; its constructor keeps rgb * 0.5.
+inline float perlin_procedure(const oracle_texture_map &t, Rgb3 *c, const V3 &dst) {
+    const Rgb3 rgb{t.rgb[0] * 0.5f, t.rgb[1] * 0.5f, t.rgb[2] * 0.5f};
+    const float x_co = dst.x, y_co = dst.y, z_co = dst.z;
+    float total = 0.0f;
+    float frequency = 1.0f / t.zoom;
+    float amplitude = 1.0f;
+    for (int i = 0; i < t.octaves; ++i) {
+        total += perlin_interpolated_noise(x_co * frequency, y_co * frequency, z_co * frequency,
+                                           static_cast<uint32_t>(t.seed)) *
+                 amplitude;
+        frequency *= 2.0f;
+        amplitude *= t.persistence;
+    }
+    total++;
+    *c = Rgb3{rgb.r * total, rgb.g * total, rgb.b * total};
+    if (total > 1.0f) return 0.0f;
+    return t.opacity;
+}
+
+// gradient_mapper::run_procedure with grad_of incidence_angle; keys of six
+// floats {key, r, g, b, alpha, interpolation (0 line, 2 step)}.
+inline float gradient_procedure(const Tables &T, const oracle_texture_map &t, Rgb3 *c, const V3 &dir, const V3 &n) {
+    const float key = std::acos(std::fabs(dot(n, dir))) * (180.0f / kPi);
+    const float *K = T.texels + t.key_offset;
+    const unsigned size = static_cast<unsigned>(t.key_count);
+    // std::upper_bound over the keys.
+    unsigned upper_idx = 0;
+    {
+        unsigned first = 0, count = size;
+        while (count > 0) {
+            const unsigned step = count / 2;
+            const unsigned it = first + step;
+            if (!(key < K[6 * it])) {
+                first = it + 1;
+                count -= step + 1;
+            } else {
+                count = step;
+            }
+        }
+        upper_idx = first;
+    }
+    auto value = [&](unsigned i) { return Rgb3{K[6 * i + 1], K[6 * i + 2], K[6 * i + 3]}; };
+    float alpha = 0.0f;
+    if (upper_idx == 0) {
+        *c = value(0);
+        alpha = K[4];
+    } else if (upper_idx == size) {
+        *c = value(size - 1);
+        alpha = K[6 * (size - 1) + 4];
+    } else {
+        const int interp = static_cast<int>(K[6 * upper_idx + 5]);
+        if (interp == 0) {  // line
+            const float key_weight = (key - K[6 * (upper_idx - 1)]) / (K[6 * upper_idx] - K[6 * (upper_idx - 1)]);
+            const Rgb3 col_0 = value(upper_idx - 1), hi = value(upper_idx);
+            const Rgb3 col_dist{hi.r - col_0.r, hi.g - col_0.g, hi.b - col_0.b};
+            *c = Rgb3{col_0.r + (col_dist.r * key_weight), col_0.g + (col_dist.g * key_weight),
+                      col_0.b + (col_dist.b * key_weight)};
+            const float alp_0 = K[6 * (upper_idx - 1) + 4];
+            const float alp_dist = alp_0 - K[6 * (upper_idx - 1) + 4];
+            alpha = alp_0 + (alp_dist * key_weight);
+        } else {  // step
+            *c = value(upper_idx);
+            alpha = K[6 * upper_idx + 4];
+        }
+    }
+    if (t.inverse) *c = Rgb3{255.0f - c->r, 255.0f - c->g, 255.0f - c->b};
+    return alpha;
+}
+
+// The run_procedure of a procedural map (kinds 3, 4, 5).
+inline float procedure(const Tables &T, const oracle_texture_map &t, Rgb3 *c, const V3 &dst, const V3 &dir,
+                       const V3 &n) {
+    if (t.kind == 3) return run_procedure(t, c, dst);
+    if (t.kind == 4) return perlin_procedure(t, c, dst);
+    return gradient_procedure(T, t, c, dir, n);
+}
+
+inline bool procedural(int kind) { return kind >= 3 && kind <= 5; }
+
 // The address an image mapper samples: the interpolated texture coordinates
 // (vt.x != MAX_DIST) scaled to the image, else its projection of dst.
 inline void address(const oracle_texture_map &t, const oracle_texture_image &im, const V3 &dst, bool has_vt,
@@ -204,12 +339,14 @@ inline void address(const oracle_texture_map &t, const oracle_texture_image &im,
     }
 }
 
-// texture_mapper::texture_map.
-inline float texture_map(const Tables &T, int map, Rgb3 *c, const V3 &dst, bool has_vt, float vtu, float vtv) {
+// texture_mapper::texture_map of a hit at dst on a ray of direction dir,
+// shading normal n.
+inline float texture_map(const Tables &T, int map, Rgb3 *c, const V3 &dst, const V3 &dir, const V3 &n, bool has_vt,
+                         float vtu, float vtv) {
     const oracle_texture_map &t = T.maps[map];
     float opac = 1.0f;
-    if (t.kind == 3) {
-        opac = run_procedure(t, c, dst);
+    if (procedural(t.kind)) {
+        opac = procedure(T, t, c, dst, dir, n);
     } else {
         const oracle_texture_image &im = T.images[t.image];
         float u_co, v_co;
@@ -230,14 +367,20 @@ inline float texture_map(const Tables &T, int map, Rgb3 *c, const V3 &dst, bool 
 }
 
 // texture_mapper::texture_map_monochrome with offsets (0, 0); *z is p.z.
-inline float texture_map_monochrome(const Tables &T, int map, float *z, const V3 &dst, bool has_vt, float vtu,
-                                    float vtv) {
+inline float texture_map_monochrome(const Tables &T, int map, float *z, const V3 &dst, const V3 &dir, const V3 &n,
+                                    bool has_vt, float vtu, float vtv) {
     const oracle_texture_map &t = T.maps[map];
     float opac = 1.0f;
-    if (t.kind == 3) {
-        // procedural_texture_mapper: dst + u 0 + v 0, the colour's mean.
+    if (procedural(t.kind)) {
+        // procedural_texture_mapper::sample_texture_monochrome
+        // (texture_mapper.h:316-334) at offsets 0: dst + u 0 + v 0, the
+        // colour's mean.
+        const float x_off_dist = 0 * 0.1f, y_off_dist = 0 * 0.1f;
+        const V3 u(n.y + n.z, 0.0f, n.x);
+        const V3 v(0.0f, -n.x - n.z, -n.y);
+        const V3 q(dst + (u * x_off_dist) + (v * y_off_dist));
         Rgb3 c;
-        opac = run_procedure(t, &c, dst);
+        opac = procedure(T, t, &c, q, dir, n);
         *z = (c.r + c.g + c.b) * (1.0f / 3.0f);
     } else {
         const oracle_texture_image &im = T.images[t.image];

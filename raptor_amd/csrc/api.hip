@@ -347,7 +347,7 @@ int32_t rt_write_tga(const char *path, const uint8_t *bgr, int32_t x_res, int32_
     return rt::write_tga(path, bgr, x_res, y_res);
 }
 
-static_assert(sizeof(rt_material) == 156, "rt_material is 39 four-byte fields");
+static_assert(sizeof(rt_material) == 160, "rt_material is 40 four-byte fields");
 
 int32_t rt_scene_set_materials(rt_scene *scene, const rt_material *materials, int32_t nmaterials,
                                const int32_t *tri_material) {
@@ -408,6 +408,14 @@ int32_t rt_scene_set_materials(rt_scene *scene, const rt_material *materials, in
                  in(q.reflectance, 0.0f, 1.0f) && in(q.transmittance, 0.0f, 1.0f) && in(q.roughness, 0.0f, 0.5f) &&
                  in(q.refractive_index, 0.0f, 5.0f) && in(q.extinction, 0.0f, 1.0f);
             range = "x, y, rd, rs, ts, ri_i 0..1, sr 0..0.5, ri_r 0..5";
+        } else if (q.kind == RT_MATERIAL_MANDELBROT) {
+            // mandelbrot_shader is opaque and spawns no secondary rays
+            // (rt/materials/mandelbrot_shader.h:14-15); a hit iterates at most
+            // twice the limit, which bounds the shading kernel's time.
+            ok = q.iterations >= 1 && q.iterations <= rt::kMaxMandelbrotIterations && q.transmittance == 0.0f &&
+                 q.reflectance == 0.0f;
+            range = "iterations 1.." + std::to_string(rt::kMaxMandelbrotIterations) + ", transmittance and "
+                    "reflectance 0";
         } else {
             ok = false;
             range = "unknown kind " + std::to_string(q.kind);
@@ -434,7 +442,8 @@ int32_t rt_scene_set_materials(rt_scene *scene, const rt_material *materials, in
                              bits(col ? q.tex[3] : 0));
         row[6] = make_float4(bits(col ? q.tex[4] : 0), bits(col ? q.tex[5] : 0), q.mapper_kd, q.mapper_ks);
         row[7] = make_float4(q.clrh, q.clrf, bits(lists[0]), bits(lists[1]));
-        row[8] = make_float4(bits(lists[2]), bits(lists[3]), bits(lists[4]), 0.0f);
+        row[8] = make_float4(bits(lists[2]), bits(lists[3]), bits(lists[4]),
+                             bits(q.kind == RT_MATERIAL_MANDELBROT ? q.iterations : 0));
     }
     // A triangle is transparent to shadow rays exactly when its material is
     // (material(tran > 0), rt/materials/material.h:50-66).
@@ -501,9 +510,30 @@ int32_t rt_scene_set_textures(rt_scene *scene, const rt_texture_map *maps, int32
         std::memset(&d, 0, sizeof(d));
         d.m = m;
         const std::string who = "map " + std::to_string(i) + ": ";
-        if (m.kind < RT_MAP_PLANAR || m.kind > RT_MAP_CHECKER_BOARD) return bad(who + "unknown kind");
+        if (m.kind < RT_MAP_PLANAR || m.kind > RT_MAP_GRADIENT) return bad(who + "unknown kind");
         if (m.falloff < RT_FALLOFF_CUBIC || m.falloff > RT_FALLOFF_NONE) return bad(who + "unknown falloff");
         if (m.kind == RT_MAP_CHECKER_BOARD) continue;
+        if (m.kind == RT_MAP_PERLIN_3D) {
+            if (m.octaves < 0 || m.octaves > rt::kMaxPerlinOctaves)
+                return bad(who + "octaves must be 0.." + std::to_string(rt::kMaxPerlinOctaves));
+            continue;
+        }
+        if (m.kind == RT_MAP_GRADIENT) {
+            // gradient_mapper.cc asserts on every key source but the
+            // incidence angle and on spline interpolation; upper_bound needs
+            // sorted keys.
+            if (m.grad_of != RT_GRAD_OF_INCIDENCE_ANGLE) return bad(who + "only the incidence-angle gradient exists");
+            if (m.key_count < 1 || m.key_offset < 0 || m.key_offset + 6 * static_cast<int64_t>(m.key_count) > ntexels)
+                return bad(who + "its keys lie outside the texels");
+            const float *k = texels + m.key_offset;
+            for (int32_t j = 0; j < m.key_count; ++j) {
+                const int interp = static_cast<int>(k[6 * j + 5]);
+                if (static_cast<float>(interp) != k[6 * j + 5] || (interp != RT_GRAD_LINE && interp != RT_GRAD_STEP))
+                    return bad(who + "key " + std::to_string(j) + ": interpolation must be line or step");
+                if (j > 0 && !(k[6 * (j - 1)] <= k[6 * j])) return bad(who + "keys must be non-decreasing");
+            }
+            continue;
+        }
         if (m.image < 0 || m.image >= nimages) return bad(who + "no image " + std::to_string(m.image));
         const rt_texture_image &im = images[m.image];
         if (im.width <= 0 || im.height <= 0 || (im.cpp != 1 && im.cpp != 3 && im.cpp != 4) || im.offset < 0 ||

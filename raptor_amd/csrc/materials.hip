@@ -46,13 +46,15 @@ constexpr double kMinReflectivePower = 0.1;
 constexpr int kCookTorrance = 1;
 constexpr int kColoured = 2;
 constexpr int kMapper = 3;
+constexpr int kMandelbrot = 4;
 
 // A material row (kMaterialFloat4s float4): {ka rgb, shininess},
 // {kd rgb, transmittance}, {ks rgb, refractive index},
 // {reflectance, kind, chroma x, chroma y}, {diffuse, roughness, extinction, 0},
 // {tex ka, kd, ks, ns}, {tex refl, d, mapper kd, mapper ks},
-// {clrh, clrf, layers 0, layers 1}, {layers 2, 3, 4, 0}; texture slots are
-// 1 + map index (0 none) and layer lists first << 8 | count, as int bits.
+// {clrh, clrf, layers 0, layers 1}, {layers 2, 3, 4, iterations}; texture
+// slots are 1 + map index (0 none), layer lists first << 8 | count and the
+// Mandelbrot iteration limit as int bits.
 // Cook-Torrance keeps rs, ts, ri_r where phong keeps rf, tran, ri.
 struct Mat {
     float4 ka_s, kd_tran, ks_ri;
@@ -62,6 +64,7 @@ struct Mat {
     int tex[6];
     float mkd, mks, clrh, clrf;
     int layers[5];
+    int iters;
 };
 
 // The triangle's material: its table entry, or phong_shader(255 * albedo, 1)
@@ -80,6 +83,7 @@ __device__ __forceinline__ Mat mat_of(const float4 *mats, const int32_t *tri_mat
         for (int k = 0; k < 6; ++k) m.tex[k] = 0;
         m.mkd = m.mks = m.clrh = m.clrf = 0.0f;
         for (int k = 0; k < 5; ++k) m.layers[k] = 0;
+        m.iters = 0;
     } else {
         const float4 *q = mats + kMaterialFloat4s * static_cast<int64_t>(mi);
         m.ka_s = q[0];
@@ -108,17 +112,12 @@ __device__ __forceinline__ Mat mat_of(const float4 *mats, const int32_t *tri_mat
         m.layers[2] = __float_as_int(q8.x);
         m.layers[3] = __float_as_int(q8.y);
         m.layers[4] = __float_as_int(q8.z);
+        m.iters = __float_as_int(q8.w);
     }
     return m;
 }
 
-// Where a hit samples its textures: the (re-projected) hit point and the
-// interpolated texture coordinates, if the triangle has them.
-struct TexPoint {
-    rtd::V3 dst;
-    bool has_vt;
-    float vtu, vtv;
-};
+using TexPoint = TexHit;
 
 // A monochrome layer list of mapper_shader (rt/materials/mapper_shader.cc:
 // 41-66, 89-126): each layer sets the value to its texel mean / 255; the
@@ -128,7 +127,7 @@ __device__ __forceinline__ float mono_layers(const TexTables &T, int list, float
     float alpha = 1.0f;
     for (int k = 0; k < count; ++k) {
         float z;
-        alpha *= texture_mono(T, T.layers[first + k], &z, p.dst, p.has_vt, p.vtu, p.vtv);
+        alpha *= texture_mono(T, T.layers[first + k], &z, p);
         value = z * (1.0f / 255.0f);
         if (alpha == 0.0f) break;
     }
@@ -146,12 +145,12 @@ __device__ __forceinline__ void child_coefficients(const Mat &m, const TexTables
     if (m.kind == kColoured) {
         if (m.tex[4]) {
             TexColour t{0.0f, 0.0f, 0.0f};
-            texture_colour(T, m.tex[4] - 1, &t, p.dst, p.has_vt, p.vtu, p.vtv);
+            texture_colour(T, m.tex[4] - 1, &t, p);
             *refl = t.r * (m.rf * (1.0f / 255.0f));
         }
         if (m.tex[5]) {
             TexColour t{0.0f, 0.0f, 0.0f};
-            texture_colour(T, m.tex[5] - 1, &t, p.dst, p.has_vt, p.vtu, p.vtv);
+            texture_colour(T, m.tex[5] - 1, &t, p);
             *tran = t.r * (m.kd_tran.w * (1.0f / 255.0f));
         }
     } else if (m.kind == kMapper) {
@@ -165,7 +164,7 @@ __device__ __forceinline__ void child_coefficients(const Mat &m, const TexTables
 __device__ __forceinline__ float4 mapped_colour(const TexTables &T, int slot, float4 colour, const TexPoint &p) {
     if (!slot) return colour;
     TexColour t{colour.x, colour.y, colour.z};
-    texture_colour(T, slot - 1, &t, p.dst, p.has_vt, p.vtu, p.vtv);
+    texture_colour(T, slot - 1, &t, p);
     return make_float4(t.r * (colour.x * (1.0f / 255.0f)), t.g * (colour.y * (1.0f / 255.0f)),
                        t.b * (colour.z * (1.0f / 255.0f)), colour.w);
 }
@@ -312,6 +311,8 @@ __device__ __forceinline__ HitSurface surface_of(const GenArgs &a, int64_t i) {
     }
     h.m = mat_of(a.mats, a.tri_mat, a.kd, prim);
     h.tp.dst = h.dst;
+    h.tp.dir = d;
+    h.tp.n = h.shn;
     h.tp.has_vt = hit_texture_coords(a.tex, prim, hit.y, hit.z, &h.tp.vtu, &h.tp.vtv);
     if (a.tex.maps) {
         child_coefficients(h.m, a.tex, h.tp, &h.refl_c, &h.tran_c);
@@ -326,6 +327,61 @@ __device__ __forceinline__ HitSurface surface_of(const GenArgs &a, int64_t i) {
     h.refr = h.tran_c > 0.0f && static_cast<double>(power * h.tran_c) > kMinReflectivePower &&
              refract_dir(d, h.shn, h.m.ks_ri.w, leaving, &h.rdir);
     return h;
+}
+
+// The escape-time iterations of z -> z^2 + (x0, y0) from z = (x0, y0) while
+// |z|^2 <= 100, at most limit (rt/materials/mandelbrot_shader.cc:66-73).
+__device__ __forceinline__ int mandel_escape(float x0, float y0, int limit) {
+    float x = x0, y = y0;
+    int iteration = 0;
+    while (((x * x) + (y * y) <= (10.0f * 10.0f)) && (iteration < limit)) {
+        const float xtemp = ((x * x) - (y * y)) + x0;
+        y = ((2.0f * x) * y) + y0;
+        x = xtemp;
+        ++iteration;
+    }
+    return iteration;
+}
+
+// mandelbrot_shader::shade's colour (rt/materials/mandelbrot_shader.cc:
+// 41-120): fractal coordinates from the hit point on the plane the shading
+// normal picks (exact +-x: y-z, exact +-z: x-y, else x-z); a point that does
+// not escape is tried once more on shifted, 1.5x scaled coordinates and is
+// black if it again does not; the escape count picks a band of the ramp.
+// The light term multiplies in later.
+__device__ __forceinline__ float4 mandel_colour(rtd::V3 dst, rtd::V3 n, int limit) {
+    const bool nx = (n.x == 1.0f) || (n.x == -1.0f);
+    const bool nz = (n.z == 1.0f) || (n.z == -1.0f);
+    float x0, y0;
+    if (nx) {
+        x0 = fmodf(fabsf(dst.z), 2.4f) - 1.5f;
+        y0 = fmodf(fabsf(dst.y) - 1.5f, 2.4f) - 1.0f;
+    } else if (nz) {
+        x0 = fmodf(fabsf(dst.x), 2.4f) - 1.5f;
+        y0 = fmodf(fabsf(dst.y) - 1.5f, 2.4f) - 1.0f;
+    } else {
+        x0 = fmodf(fabsf(dst.x) - 1.5f, 2.4f) - 1.0f;
+        y0 = fmodf(fabsf(dst.z), 2.4f) - 1.5f;
+    }
+    int iteration = mandel_escape(x0, y0, limit);
+    if (iteration == limit) {
+        if (nx) {
+            x0 = fmodf(fabsf(dst.z) - 1.0f, 2.4f) * 1.5f - 0.4f;
+            y0 = fmodf(fabsf(dst.y) - 2.0f, 2.4f) * 1.5f - 0.7f;
+        } else if (nz) {
+            x0 = fmodf(fabsf(dst.x) - 1.0f, 2.4f) * 1.5f - 0.4f;
+            y0 = fmodf(fabsf(dst.y) - 2.0f, 2.4f) * 1.5f - 0.7f;
+        } else {
+            x0 = fmodf(fabsf(dst.x) - 2.0f, 2.4f) * 1.5f - 0.7f;
+            y0 = fmodf(fabsf(dst.z) - 1.0f, 2.4f) * 1.5f - 0.4f;
+        }
+        iteration = mandel_escape(x0, y0, limit);
+        if (iteration == limit) return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    const float colour = 767.0f - ((static_cast<float>(iteration) * 767.0f) / static_cast<float>(limit));
+    if (colour < 256.0f) return make_float4(25.0f - colour, 255.0f, 255.0f, 0.0f);
+    if (colour < 512.0f) return make_float4(0.0f, 256.0f - (colour - 256.0f), 255.0f, 0.0f);
+    return make_float4(0.0f, 0.0f, 256.0f - (colour - 767.0f), 0.0f);
 }
 
 // The shading record of a hit, its material resolved at the hit:
@@ -345,13 +401,20 @@ __device__ __forceinline__ ShadeRec shade_record(const HitSurface &h, const TexT
     const Mat &m = h.m;
     ShadeRec r;
     const float kind = static_cast<float>(m.kind);
+    if (m.kind == kMandelbrot) {
+        r.r0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        r.r1 = mandel_colour(h.dst, h.shn, m.iters);
+        r.r2 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        r.r3 = make_float4(0.0f, 0.0f, kind, 0.0f);
+        return r;
+    }
     if (m.kind == kMapper) {
         float4 rgb = make_float4(m.kd_tran.x, m.kd_tran.y, m.kd_tran.z, 0.0f);
         const int first = m.layers[0] >> 8, count = m.layers[0] & 0xff;
         float alpha = 1.0f;
         TexColour c{rgb.x, rgb.y, rgb.z};
         for (int k = 0; k < count; ++k) {
-            alpha *= texture_colour(T, T.layers[first + k], &c, h.tp.dst, h.tp.has_vt, h.tp.vtu, h.tp.vtv);
+            alpha *= texture_colour(T, T.layers[first + k], &c, h.tp);
             if (alpha == 0.0f) break;
         }
         const float kd = mono_layers(T, m.layers[1], m.mkd, h.tp);
@@ -366,7 +429,7 @@ __device__ __forceinline__ ShadeRec shade_record(const HitSurface &h, const TexT
     if (m.kind == kColoured && T.maps) {
         if (m.tex[3]) {
             TexColour dummy{ka.x, ka.y, ka.z};
-            const float alpha = texture_colour(T, m.tex[3] - 1, &dummy, h.tp.dst, h.tp.has_vt, h.tp.vtu, h.tp.vtv);
+            const float alpha = texture_colour(T, m.tex[3] - 1, &dummy, h.tp);
             ka.w = alpha * (m.ka_s.w * (1.0f / 255.0f));
         }
         ka = mapped_colour(T, m.tex[0], ka, h.tp);
@@ -477,6 +540,8 @@ __global__ void __launch_bounds__(256) k_mat_local(LocalArgs a) {
     const int kind = static_cast<int>(rm.r3.z);
     const bool ct = kind == kCookTorrance;
     const bool mapper = kind == kMapper;
+    const bool mandel = kind == kMandelbrot;
+    float total_shade = 0.0f;  // mandelbrot_shader: sum of n.l x visibility
     Mat m;
     if (ct) m = mat_of(a.mats, a.tri_mat, a.kd, __float_as_int(nn.w));
     const CieMatrix cm = cie_matrix();
@@ -510,6 +575,10 @@ __global__ void __launch_bounds__(256) k_mat_local(LocalArgs a) {
         }
         const float shade = rtd::dot(dn, n);
         if (shade < 0.0f) continue;
+        if (mandel) {
+            total_shade += shade * magn;
+            continue;
+        }
         if (ct) {
             const float4 li = make_float4(lrgb.x / d_scale, lrgb.y / d_scale, lrgb.z / d_scale, 0.0f);
             cook_torrance_term(m, cm, n, d, nv, dn, shade, magn, li, c);
@@ -530,7 +599,9 @@ __global__ void __launch_bounds__(256) k_mat_local(LocalArgs a) {
         c.g = c.g + ((lrgb.y / d_scale) * (scg * magn));
         c.b = c.b + ((lrgb.z / d_scale) * (scb * magn));
     }
-    if (!ct && !mapper) {  // cook_torrance_cxy and mapper_shader have no ambient term
+    if (mandel) {  // mandelbrot_shader.cc:126: the colour times the light term
+        c = Rgb{rm.r1.x * total_shade, rm.r1.y * total_shade, rm.r1.z * total_shade};
+    } else if (!ct && !mapper) {  // cook_torrance_cxy and mapper_shader have no ambient term
         c.r += rm.r0.x;
         c.g += rm.r0.y;
         c.b += rm.r0.z;

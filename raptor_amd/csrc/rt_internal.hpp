@@ -103,6 +103,10 @@ constexpr int kQueueWords = 1024;
 // {layers 2, 3, 4, 0} (texture slots 1 + map, layer lists first << 8 |
 // count, as int bits).
 constexpr int kMaterialFloat4s = 9;
+// Largest Mandelbrot iteration limit and Perlin octave count a scene takes:
+// a hit's shading loops over them.
+constexpr int kMaxMandelbrotIterations = 65536;
+constexpr int kMaxPerlinOctaves = 32;
 struct DevTexMap;  // textures.hpp
 }  // namespace rt
 

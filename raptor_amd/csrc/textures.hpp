@@ -1,9 +1,11 @@
 // Texture maps on the device: what texture_mapper::texture_map and
 // texture_map_monochrome (rt/materials/texture_mapper.h) return for one hit,
 // for the image mappers (planar, cubic, cylindrical: rt/materials/
-// planar_mapper.cc, cubic_mapper.cc, cylindrical_mapper.cc), the checker
-// board (checker_board_mapper.cc) and the falloff (mapper_falloff.cc).
-// Every float expression keeps the reference's operand order.
+// planar_mapper.cc, cubic_mapper.cc, cylindrical_mapper.cc), the procedural
+// ones (checker_board_mapper.cc, perlin_noise_3d_mapper.cc with
+// common/perlin_noise_3d.cc, gradient_mapper.cc) and the falloff
+// (mapper_falloff.cc). Every float expression keeps the reference's operand
+// order.
 #pragma once
 
 #include <cfloat>
@@ -31,6 +33,15 @@ struct TexTables {
 
 struct TexColour {
     float r, g, b;
+};
+
+// Where a hit samples its maps: the (re-projected) hit point, the incident
+// ray's direction, the shading normal and the interpolated texture
+// coordinates, if the triangle has them.
+struct TexHit {
+    rtd::V3 dst, dir, n;
+    bool has_vt;
+    float vtu, vtv;
 };
 
 namespace tex {
@@ -186,6 +197,130 @@ __device__ __forceinline__ float checker(const DevTexMap &t, TexColour *c, rtd::
     return t.m.opacity;
 }
 
+// perlin_noise_3d::noise (common/perlin_noise_3d.cc:10-15): the integer
+// hash in 32-bit wrapping arithmetic (the reference's int overflow wraps the
+// same way on x86), the masked value over 2^30 in float.
+__device__ __forceinline__ float lattice_noise(uint32_t x, uint32_t y, uint32_t z, uint32_t s) {
+    uint32_t n = ((x + y * 57u) + z * 13u) + s;
+    n = (n << 13) ^ n;
+    const int32_t m = static_cast<int32_t>((n * (n * n * 15731u + 789221u) + 1376312589u) & 0x7fffffffu);
+    return 1.0f - static_cast<float>(m) / 1073741824.0f;
+}
+
+// perlin_noise_3d::smooth_noise (perlin_noise_3d.cc:26-32): the six face
+// neighbours summed left to right times 1/16, plus the centre times 1/4.
+__device__ __forceinline__ float smooth_noise(uint32_t x, uint32_t y, uint32_t z, uint32_t s) {
+    const float sides = (((((lattice_noise(x - 1u, y, z, s) + lattice_noise(x + 1u, y, z, s)) +
+                            lattice_noise(x, y - 1u, z, s)) +
+                           lattice_noise(x, y + 1u, z, s)) +
+                          lattice_noise(x, y, z - 1u, s)) +
+                         lattice_noise(x, y, z + 1u, s)) *
+                        (1.0f / 16.0f);
+    const float center = lattice_noise(x, y, z, s) * (1.0f / 4.0f);
+    return sides + center;
+}
+
+// perlin_noise_3d::interpolate (perlin_noise_3d.cc:41-46): cosine blend.
+__device__ __forceinline__ float cos_blend(float a, float b, float x) {
+    const float ft = x * kPi;
+    const float f = (1.0f - cosf(ft)) * 0.5f;
+    return (a * (1.0f - f)) + (b * f);
+}
+
+// perlin_noise_3d::interpolated_noise (perlin_noise_3d.cc:55-80): the
+// lattice cell from int(float(int(x)) + 0.5) (which moves negative
+// coordinates one cell up), smoothed noise at its 8 corners, blended along
+// x, then y, then z.
+__device__ __forceinline__ float interpolated_noise(float x, float y, float z, uint32_t s) {
+    const int wx = to_int(static_cast<float>(to_int(x)) + 0.5f);
+    const float px = x - static_cast<float>(wx);
+    const int wy = to_int(static_cast<float>(to_int(y)) + 0.5f);
+    const float py = y - static_cast<float>(wy);
+    const int wz = to_int(static_cast<float>(to_int(z)) + 0.5f);
+    const float pz = z - static_cast<float>(wz);
+    const uint32_t ux = static_cast<uint32_t>(wx), uy = static_cast<uint32_t>(wy), uz = static_cast<uint32_t>(wz);
+    const float v1 = smooth_noise(ux, uy, uz, s);
+    const float v2 = smooth_noise(ux + 1u, uy, uz, s);
+    const float v3 = smooth_noise(ux, uy + 1u, uz, s);
+    const float v4 = smooth_noise(ux + 1u, uy + 1u, uz, s);
+    const float v5 = smooth_noise(ux, uy, uz + 1u, s);
+    const float v6 = smooth_noise(ux + 1u, uy, uz + 1u, s);
+    const float v7 = smooth_noise(ux, uy + 1u, uz + 1u, s);
+    const float v8 = smooth_noise(ux + 1u, uy + 1u, uz + 1u, s);
+    const float ix0 = cos_blend(v1, v2, px);
+    const float ix1 = cos_blend(v3, v4, px);
+    const float ix2 = cos_blend(v5, v6, px);
+    const float ix3 = cos_blend(v7, v8, px);
+    const float iy0 = cos_blend(ix0, ix1, py);
+    const float iy1 = cos_blend(ix2, ix3, py);
+    return cos_blend(iy0, iy1, pz);
+}
+
+// perlin_noise_3d_mapper::run_procedure (rt/materials/perlin_noise_3d_mapper.cc:12-40).
+__device__ __forceinline__ float perlin(const DevTexMap &t, TexColour *c, rtd::V3 dst) {
+    float total = 0.0f;
+    float frequency = 1.0f / t.m.zoom;
+    float amplitude = 1.0f;
+    const uint32_t seed = static_cast<uint32_t>(t.m.seed);
+    for (int i = 0; i < t.m.octaves; ++i) {
+        total += interpolated_noise(dst.x * frequency, dst.y * frequency, dst.z * frequency, seed) * amplitude;
+        frequency *= 2.0f;
+        amplitude *= t.m.persistence;
+    }
+    total = total + 1.0f;
+    *c = TexColour{(t.m.rgb[0] * 0.5f) * total, (t.m.rgb[1] * 0.5f) * total, (t.m.rgb[2] * 0.5f) * total};
+    return (total > 1.0f) ? 0.0f : t.m.opacity;
+}
+
+// gradient_mapper::run_procedure (rt/materials/gradient_mapper.cc:12-105)
+// for the incidence angle: the keys' upper bound, the end values outside
+// them, linear (alpha the lower key's: the reference's alpha difference is
+// zero) or step between; the mapper's own inversion.
+__device__ __forceinline__ float gradient(const DevTexMap &t, const float *texels, TexColour *c, rtd::V3 dir,
+                                          rtd::V3 n) {
+    const float key = acosf(fabsf(rtd::dot(n, dir))) * (180.0f / kPi);
+    const float *k = texels + t.m.key_offset;
+    const int count = t.m.key_count;
+    int up = 0;
+    while (up < count && !(key < k[6 * up])) ++up;
+    float alpha;
+    if (up == 0) {
+        *c = TexColour{k[1], k[2], k[3]};
+        alpha = k[4];
+    } else if (up == count) {
+        const float *b = k + 6 * (count - 1);
+        *c = TexColour{b[1], b[2], b[3]};
+        alpha = b[4];
+    } else {
+        const float *lo = k + 6 * (up - 1), *hi = k + 6 * up;
+        if (static_cast<int>(hi[5]) == RT_GRAD_LINE) {
+            const float key_weight = (key - lo[0]) / (hi[0] - lo[0]);
+            *c = TexColour{lo[1] + ((hi[1] - lo[1]) * key_weight), lo[2] + ((hi[2] - lo[2]) * key_weight),
+                           lo[3] + ((hi[3] - lo[3]) * key_weight)};
+            const float alp_0 = lo[4];
+            const float alp_dist = alp_0 - lo[4];
+            alpha = alp_0 + (alp_dist * key_weight);
+        } else {  // RT_GRAD_STEP (spline is refused at upload)
+            *c = TexColour{hi[1], hi[2], hi[3]};
+            alpha = hi[4];
+        }
+    }
+    if (t.m.inverse) *c = TexColour{255.0f - c->r, 255.0f - c->g, 255.0f - c->b};
+    return alpha;
+}
+
+// A procedural map's run_procedure: checker board, Perlin or gradient.
+__device__ __forceinline__ float procedure(const DevTexMap &t, const float *texels, TexColour *c, const TexHit &p,
+                                           rtd::V3 dst) {
+    if (t.m.kind == RT_MAP_CHECKER_BOARD) return checker(t, c, dst);
+    if (t.m.kind == RT_MAP_PERLIN_3D) return perlin(t, c, dst);
+    return gradient(t, texels, c, p.dir, p.n);
+}
+
+__device__ __forceinline__ bool is_procedural(int kind) {
+    return kind == RT_MAP_CHECKER_BOARD || kind == RT_MAP_PERLIN_3D || kind == RT_MAP_GRADIENT;
+}
+
 // The image mappers' texture address of a hit: the interpolated texture
 // coordinates scaled to the image when the triangle has them, else the
 // mapper's projection of dst.
@@ -203,15 +338,15 @@ __device__ __forceinline__ void address(const DevTexMap &t, rtd::V3 dst, bool ha
 
 // texture_mapper::texture_map: sample into *c (a checker board leaves it
 // where it does not paint), then falloff and inverse; returns the alpha.
-__device__ __forceinline__ float texture_colour(const TexTables &T, int map, TexColour *c, rtd::V3 dst, bool has_vt,
-                                                float vtu, float vtv) {
+__device__ __forceinline__ float texture_colour(const TexTables &T, int map, TexColour *c, const TexHit &p) {
     const DevTexMap &t = T.maps[map];
+    const rtd::V3 dst = p.dst;
     float alpha = 1.0f;
-    if (t.m.kind == RT_MAP_CHECKER_BOARD) {
-        alpha = tex::checker(t, c, dst);
+    if (tex::is_procedural(t.m.kind)) {
+        alpha = tex::procedure(t, T.texels, c, p, dst);
     } else {
         float u_co, v_co;
-        tex::address(t, dst, has_vt, vtu, vtv, &u_co, &v_co);
+        tex::address(t, dst, p.has_vt, p.vtu, p.vtv, &u_co, &v_co);
         int u0 = tex::to_int(u_co) + t.m.u_off;
         int v0 = tex::to_int(v_co) + t.m.v_off;
         if (!tex::wrap(&u0, t.w, t.m.u_wrap) || !tex::wrap(&v0, t.h, t.m.v_wrap)) {
@@ -228,18 +363,24 @@ __device__ __forceinline__ float texture_colour(const TexTables &T, int map, Tex
 // texture_mapper::texture_map_monochrome at offset (0, 0): the texel's mean
 // (its grey for one channel), 0 outside a blank-wrapped image; then falloff
 // and inverse; returns the alpha.
-__device__ __forceinline__ float texture_mono(const TexTables &T, int map, float *z, rtd::V3 dst, bool has_vt,
-                                              float vtu, float vtv) {
+__device__ __forceinline__ float texture_mono(const TexTables &T, int map, float *z, const TexHit &p) {
     const DevTexMap &t = T.maps[map];
+    const rtd::V3 dst = p.dst;
     float alpha = 1.0f;
     float pz;
-    if (t.m.kind == RT_MAP_CHECKER_BOARD) {
+    if (tex::is_procedural(t.m.kind)) {
+        // procedural_texture_mapper::sample_texture_monochrome at offset
+        // (0, 0): the point dst + u 0 + v 0 (which turns a -0 coordinate
+        // into +0), the colour's mean.
+        const rtd::V3 u = rtd::mk(p.n.y + p.n.z, 0.0f, p.n.x);
+        const rtd::V3 v = rtd::mk(0.0f, -p.n.x - p.n.z, -p.n.y);
+        const rtd::V3 q = rtd::add(rtd::add(dst, rtd::mul(u, 0.0f)), rtd::mul(v, 0.0f));
         TexColour c{0.0f, 0.0f, 0.0f};
-        alpha = tex::checker(t, &c, dst);
+        alpha = tex::procedure(t, T.texels, &c, p, q);
         pz = ((c.r + c.g) + c.b) * (1.0f / 3.0f);
     } else {
         float u_co, v_co;
-        tex::address(t, dst, has_vt, vtu, vtv, &u_co, &v_co);
+        tex::address(t, dst, p.has_vt, p.vtu, p.vtv, &u_co, &v_co);
         u_co += 0.0f;
         v_co += 0.0f;
         int u0 = tex::to_int(u_co) + t.m.u_off;

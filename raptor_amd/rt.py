@@ -267,16 +267,19 @@ def gen_terrain(grid, amplitude=0.15, zoom=0.02, persistence=0.5, octaves=8, noi
 
 # ---------------------------------------------------------------- handles
 
-MATERIAL_PHONG, MATERIAL_COOK_TORRANCE, MATERIAL_COLOURED, MATERIAL_MAPPER = 0, 1, 2, 3
-# rt_material (include/raptor_rt.h), 156 bytes.
+MATERIAL_PHONG, MATERIAL_COOK_TORRANCE, MATERIAL_COLOURED, MATERIAL_MAPPER, MATERIAL_MANDELBROT = 0, 1, 2, 3, 4
+# rt_material (include/raptor_rt.h), 160 bytes.
 MATERIAL_DTYPE = np.dtype([("ka", "<f4", 3), ("kd", "<f4", 3), ("ks", "<f4", 3), ("shininess", "<f4"),
                            ("transmittance", "<f4"), ("refractive_index", "<f4"), ("reflectance", "<f4"),
                            ("kind", "<i4"), ("chroma", "<f4", 2), ("diffuse", "<f4"), ("roughness", "<f4"),
                            ("extinction", "<f4"), ("tex", "<i4", 6), ("mapper_kd", "<f4"), ("mapper_ks", "<f4"),
-                           ("clrh", "<f4"), ("clrf", "<f4"), ("layer_first", "<i4", 5), ("layer_count", "<i4", 5)])
-assert MATERIAL_DTYPE.itemsize == 156
+                           ("clrh", "<f4"), ("clrf", "<f4"), ("layer_first", "<i4", 5), ("layer_count", "<i4", 5),
+                           ("iterations", "<i4")])
+assert MATERIAL_DTYPE.itemsize == 160
 
-MAP_PLANAR, MAP_CUBIC, MAP_CYLINDRICAL, MAP_CHECKER_BOARD = 0, 1, 2, 3
+MAP_PLANAR, MAP_CUBIC, MAP_CYLINDRICAL, MAP_CHECKER_BOARD, MAP_PERLIN_3D, MAP_GRADIENT = 0, 1, 2, 3, 4, 5
+GRAD_LINE, GRAD_SPLINE, GRAD_STEP = 0, 1, 2
+GRAD_OF_INCIDENCE_ANGLE = 3
 WRAP_BLANK, WRAP_CLAMP, WRAP_TILE, WRAP_MIRROR = 0, 1, 2, 3
 FALLOFF_CUBIC, FALLOFF_SPHERICAL, FALLOFF_LINEAR_X, FALLOFF_LINEAR_Y, FALLOFF_LINEAR_Z, FALLOFF_NONE = range(6)
 # rt_texture_map and rt_texture_image (include/raptor_rt.h).
@@ -284,10 +287,12 @@ TEXTURE_MAP_DTYPE = np.dtype([("kind", "<i4"), ("image", "<i4"), ("c", "<f4", 3)
                               ("s", "<f4", 3), ("u", "<f4", 3), ("v", "<f4", 3), ("u_wrap", "<i4"),
                               ("v_wrap", "<i4"), ("u_off", "<i4"), ("v_off", "<i4"), ("cycles", "<i4"),
                               ("rgb", "<f4", 3), ("opacity", "<f4"), ("inverse", "<i4"), ("falloff", "<i4"),
-                              ("falloff_centre", "<f4", 3), ("falloff_grad", "<f4", 3)])
+                              ("falloff_centre", "<f4", 3), ("falloff_grad", "<f4", 3), ("persistence", "<f4"),
+                              ("zoom", "<f4"), ("octaves", "<i4"), ("seed", "<i4"), ("key_count", "<i4"),
+                              ("grad_of", "<i4"), ("key_offset", "<i8")])
 TEXTURE_IMAGE_DTYPE = np.dtype([("width", "<i4"), ("height", "<i4"), ("cpp", "<i4"), ("reserved", "<i4"),
                                 ("offset", "<i8")])
-assert TEXTURE_MAP_DTYPE.itemsize == 136 and TEXTURE_IMAGE_DTYPE.itemsize == 24
+assert TEXTURE_MAP_DTYPE.itemsize == 168 and TEXTURE_IMAGE_DTYPE.itemsize == 24
 
 
 def _f3(v):
@@ -342,6 +347,37 @@ def checker_board_map(rgb, centre, grid_size, opacity=1.0):
     m = _map(MAP_CHECKER_BOARD)
     m["rgb"], m["c"], m["s"], m["opacity"] = _f3(rgb), _f3(centre), _f3(grid_size), opacity
     return m
+
+
+def perlin_3d_map(rgb, persistence, zoom, octaves, seed, opacity=1.0):
+    """perlin_noise_3d_mapper(rgb, p, z, o, s, op)
+    (rt/materials/perlin_noise_3d_mapper.h:20-21): rgb 0-255."""
+    m = _map(MAP_PERLIN_3D)
+    m["rgb"], m["persistence"], m["zoom"], m["octaves"], m["seed"] = _f3(rgb), persistence, zoom, octaves, seed
+    m["opacity"] = opacity
+    return m
+
+
+def gradient_map(key_offset, key_count, inverse=False):
+    """gradient_mapper over the incidence angle
+    (rt/materials/gradient_mapper.h:31-33) whose key_count keys sit at
+    texels[key_offset:] (see gradient_keys)."""
+    m = _map(MAP_GRADIENT)
+    m["grad_of"], m["key_offset"], m["key_count"], m["inverse"] = GRAD_OF_INCIDENCE_ANGLE, key_offset, key_count, inverse
+    return m
+
+
+def gradient_keys(keys, colours, alphas, interpolations):
+    """A gradient's keys as the texel rows its map reads: per key (degrees,
+    non-decreasing) {key, r, g, b, alpha, interpolation} (GRAD_LINE or
+    GRAD_STEP, the interpolation towards that key from the one below)."""
+    k = np.asarray(keys, np.float32).reshape(-1)
+    out = np.zeros((k.size, 6), np.float32)
+    out[:, 0] = k
+    out[:, 1:4] = np.asarray(colours, np.float32).reshape(-1, 3)
+    out[:, 4] = np.asarray(alphas, np.float32).reshape(-1)
+    out[:, 5] = np.asarray(interpolations, np.float32).reshape(-1)
+    return out.reshape(-1)
 
 
 def with_falloff(m, kind, centre, grad):
@@ -406,8 +442,9 @@ def material_table(materials):
     (1), ri_i (0). Kind 2 (coloured_mapper_shader) takes the phong fields and
     tex (t_ka, t_kd, t_ks, t_ns, t_refl, t_d: map indices or None). Kind 3
     (mapper_shader): rgb (0..255), kd, ks (scalars), s, tran, ri, rf, clrh,
-    clrf and layers ((first, count) x 5, see layer_lists). A row of 13
-    floats (ka, kd, ks, s, tran, ri, rf) is a phong material."""
+    clrf and layers ((first, count) x 5, see layer_lists). Kind 4
+    (mandelbrot_shader): iterations. A row of 13 floats (ka, kd, ks, s,
+    tran, ri, rf) is a phong material."""
     out = np.zeros(len(materials), MATERIAL_DTYPE)
     for i, m in enumerate(materials):
         r = out[i]
@@ -425,6 +462,9 @@ def material_table(materials):
             r["roughness"] = m.get("sr", 0.0)
             r["refractive_index"] = m.get("ri_r", 1.0)
             r["extinction"] = m.get("ri_i", 0.0)
+        elif m.get("kind") == MATERIAL_MANDELBROT:
+            r["kind"] = MATERIAL_MANDELBROT
+            r["iterations"] = m.get("iterations", 100)
         elif m.get("kind") == MATERIAL_MAPPER:
             r["kind"] = MATERIAL_MAPPER
             r["kd"] = m.get("rgb", (0, 0, 0))

@@ -29,12 +29,13 @@ def rel_err(a, b, floor=1e-3):
 
 def oracle_material_rows(materials):
     """rt_material rows (the library's MATERIAL_DTYPE) as the oracle's
-    39-float rows: ka, kd, ks, s, tran, ri, rf, kind, cx, cy, rd, sr, ri_i,
-    six texture slots, mapper kd, ks, clrh, clrf, layer firsts, counts."""
+    40-float rows: ka, kd, ks, s, tran, ri, rf, kind, cx, cy, rd, sr, ri_i,
+    six texture slots, mapper kd, ks, clrh, clrf, layer firsts, counts,
+    iterations."""
     m = materials
     col = lambda k: m[k][:, None].astype(np.float32)  # noqa: E731
     return np.concatenate([m["ka"], m["kd"], m["ks"], col("shininess"), col("transmittance"),
                            col("refractive_index"), col("reflectance"), col("kind"), m["chroma"], col("diffuse"),
                            col("roughness"), col("extinction"), m["tex"].astype(np.float32), col("mapper_kd"),
                            col("mapper_ks"), col("clrh"), col("clrf"), m["layer_first"].astype(np.float32),
-                           m["layer_count"].astype(np.float32)], axis=1).astype(np.float32)
+                           m["layer_count"].astype(np.float32), col("iterations")], axis=1).astype(np.float32)
