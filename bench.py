@@ -332,6 +332,7 @@ def invrender_section(ctx, steps, n_scenes=16, res=512, spp=2, bounces=2):
     run, each one persistent path launch (forward + in-lane adjoint) plus the
     update, replayed from a hipGraph. Also timed (median of 5): the forward
     frames alone, forward + adjoint, and the same as one launch per scene."""
+    import numpy as np
     import torch
 
     from raptor_amd import rt, workloads
@@ -390,6 +391,20 @@ def invrender_section(ctx, steps, n_scenes=16, res=512, spp=2, bounces=2):
         return (num / max(den, 1)) ** 0.5
 
     init_rmse = (sum(float(((0.5 - gt).double() ** 2).sum()) for gt in truth) / (3 * tris)) ** 0.5
+    # Identifiability: the loss constrains a triangle's albedo through the
+    # path vertices that land on it. The same samples' primary and first
+    # bounce vertices (rt_render with the same seed, spp and bounce draws
+    # as the path launch) count them per triangle.
+    well = []
+    verts_seen, n_seen = 0, 0
+    for acc, w, m in zip(accels, ws, seen):
+        out = acc.render(w.camera, ws[0].lights, bounce=True, spp=spp, seed=ws[0].seed, stats=False)
+        ids = np.concatenate([out["primary"]["prim"], out["bounce"]["prim"]])
+        cnt = np.bincount(ids[ids >= 0], minlength=int(acc.scene.n))
+        c = torch.from_numpy(cnt).to(dev)
+        verts_seen += int(c[m].sum())
+        n_seen += int(m.sum())
+        well.append(c >= 4)
     res_out = {
         "workload": f"{n_scenes} scenes, {tris} triangles (terrain and random 50/50, 0.5M-16M each), {res}x{res} "
                     f"at {spp} spp, {bounces} diffuse bounces, 4 point lights; loss sum (I - I_target)^2, targets "
@@ -415,6 +430,13 @@ def invrender_section(ctx, steps, n_scenes=16, res=512, spp=2, bounces=2):
         "albedo_rmse_seen_start": round((sum(float(((0.5 - gt)[m].double() ** 2).sum()) for gt, m in zip(truth, seen))
                                          / max(1, 3 * sum(int(m.sum()) for m in seen))) ** 0.5, 5),
         "path_record_bytes": 0,
+        "path_vertices_per_seen_triangle": round(verts_seen / max(1, n_seen), 2),
+        "well_sampled_rule": "at least 4 primary or first-bounce vertices of the step's samples on the triangle",
+        "well_sampled_triangles": int(sum(int(m.sum()) for m in well)),
+        "albedo_rmse_well_sampled_start": round((sum(float(((0.5 - gt)[m].double() ** 2).sum())
+                                                     for gt, m in zip(truth, well))
+                                                 / max(1, 3 * sum(int(m.sum()) for m in well))) ** 0.5, 5),
+        "albedo_rmse_well_sampled": round(rmse(well), 5),
     }
     inv.close()
     return res_out

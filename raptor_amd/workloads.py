@@ -361,13 +361,33 @@ def batched(n_scenes=64, tri_lo=1e3, tri_hi=2e6, side_lo=64, side_hi=1024, seed=
     return out
 
 
+def random_layer(seed, n, albedo_lo=0.2, albedo_hi=0.8, thickness=0.1, spread=0.01):
+    """n random triangles (rt.gen_random_triangles: vertices within +-spread
+    of their centroid) whose centroids are moved from the unit cube into a
+    layer y in [0, thickness) over [0, ext]^2 in x and z, ext the side the
+    terrain of the same triangle count covers (terrain_arrays). Inside the
+    unit cube a cloud of 0.5M-16M such triangles has a mean free path of
+    about 1e-3: the terrain camera sees its front face, whose shadow rays to
+    lights above run through the cloud, and the frame is black. In the layer
+    the camera sees its top, lit from above."""
+    abc, alb = rt.gen_random_triangles(seed, n, spread, albedo_lo, albedo_hi)
+    g = int(np.ceil(np.sqrt(n / 2.0))) + 1
+    ext = np.float32(max(1.0, (g - 1) / 707.0))
+    v = abc.reshape(-1, 3, 3)
+    c = v.mean(axis=1, keepdims=True, dtype=np.float32)
+    scale = np.array([ext, np.float32(thickness), ext], np.float32)
+    v += c * (scale - np.float32(1.0))
+    return v.reshape(-1, 9), alb
+
+
 def invrender_scenes(n_scenes=16, tri_lo=5e5, tri_hi=1.6e7, res=512, seed=21, pick=None):
     """Inverse-rendering batch: triangle counts log-uniform in [tri_lo,
     tri_hi], terrain (grid ceil(sqrt(N/2)) + 1 at the 1M terrain's cell size,
-    see terrain_arrays) or random triangles 50/50, each seen res x res by the terrain camera under
-    the four ring lights of diff_terrain. The generated albedo is the ground
-    truth that inverse rendering should recover. pick: indices of the scenes
-    to generate (the draws of the others are made and skipped)."""
+    see terrain_arrays) or a layer of random triangles over the same
+    footprint (random_layer) 50/50, each seen res x res by the terrain camera
+    under the four ring lights of diff_terrain. The generated albedo is the
+    ground truth that inverse rendering should recover. pick: indices of the
+    scenes to generate (the draws of the others are made and skipped)."""
     rng = np.random.default_rng(seed)
     base = diff_terrain(grid=16, x_res=res, y_res=res, spp=2, seed=seed)
     out = []
@@ -381,7 +401,7 @@ def invrender_scenes(n_scenes=16, tri_lo=5e5, tri_hi=1.6e7, res=512, seed=21, pi
             abc, alb = terrain_arrays(g, 0.15, 0.02, 57 + s, 300 + s, 0.2, 0.8)
             kind = "terrain"
         else:
-            abc, alb = rt.gen_random_triangles(2000 + s, n, 0.01, 0.2, 0.8)
+            abc, alb = random_layer(2000 + s, n)
             kind = "random"
         out.append(Workload(f"{kind}{abc.shape[0]}", abc, alb, base.camera, base.lights, bounce=True, spp=2,
                             seed=seed, meta=dict(kind=kind, triangles=int(abc.shape[0]))))
