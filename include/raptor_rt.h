@@ -539,6 +539,32 @@ int32_t rt_write_jpeg(rt_ctx *ctx, const char *path, const uint8_t *rgb, int32_t
  * code (0, or a leaf code for a one-triangle scene, -1 for an empty one) and
  * the padding the build added to every box. */
 int32_t rt_build_morton(rt_scene *scene, uint32_t *codes, uint32_t *order);
+/* The reference builder's own tree, built on the GPU: RAPTor's approximate
+ * agglomerative clustering (ssd/bvh_builder.cc:18-642, class bvh_builder),
+ * node for node: Morton codes of the box centroids on per-axis cells of the
+ * scene extent * 1.00001 / 1024 (ssd/bvh_builder.cc:33-49), the stable radix
+ * order, the top-down split at the Morton-range midpoint with the
+ * spatial-midpoint partition once codes are exhausted (build_layer,
+ * build_layer_primitive), greedy leaf packing under max_leaf_sah
+ * (build_leaf_node) and the agglomerative merge of each node's clusters by
+ * least combined surface area down to delta^0.5 n^alpha / 2 clusters, ties
+ * decided as the reference's SSE min_element decides them (combine_nodes,
+ * reduction_function; common/simd.h:857-927). alpha and delta as
+ * ssd/bvh_builder.h:23 (0.4, 20). Host outputs: codes[n] (emplace order,
+ * may be NULL), order[n] (the radix sort's order, may be NULL),
+ * indirect[n] (the triangle order after the builder's partitions and leaf
+ * packing), nodes[capacity >= 2n - 1], *nnodes, *root, *build_ms (may be
+ * NULL). A leaf covers indirect[child[0] .. child[1]); an interior node's
+ * child[0] and child[1] are the node ids the reference makes its left_b and
+ * right_e. Node ids differ from the reference's creation order; the tree is
+ * the same. */
+typedef struct {
+    float lo[3], hi[3];
+    int32_t child[2];
+    int32_t leaf;
+} rt_aac_node;
+int32_t rt_build_aac(rt_scene *scene, float alpha, float delta, uint32_t *codes, uint32_t *order, int32_t *indirect,
+                     rt_aac_node *nodes, int64_t capacity, int64_t *nnodes, int32_t *root, float *build_ms);
 int32_t rt_accel_download(const rt_accel *accel, void *nodes, int32_t *tri_ids);
 int32_t rt_accel_root(const rt_accel *accel, int32_t *root, float *box_pad);
 

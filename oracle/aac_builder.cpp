@@ -326,6 +326,32 @@ int AacBuilder::build_layer_primitive(int *cost_b, int *cost_e, const V3 &bl, co
     return cost_start_idx;
 }
 
+// ssd/bvh_builder.cc:18-49: triangle bounds, scene box, per-axis cell
+// widths and the Morton code of every box centroid.
+void aac_codes(const Scene &s, std::vector<Bound> *bounds, V3 *lo, V3 *hi, std::vector<int> *codes) {
+    const int n = static_cast<int>(s.tris.size());
+    V3 b(kMaxDist, kMaxDist, kMaxDist), t(-kMaxDist, -kMaxDist, -kMaxDist);
+    bounds->resize(n);
+    for (int i = 0; i < n; ++i) {
+        const Triangle &tri = s.tris[i];
+        // rt/triangle.h:56-57
+        (*bounds)[i].low = vmin(tri.a, vmin(tri.b, tri.c));
+        (*bounds)[i].high = vmax(tri.a, vmax(tri.b, tri.c));
+        b = vmin(b, (*bounds)[i].low);
+        t = vmax(t, (*bounds)[i].high);
+    }
+    *lo = b;
+    *hi = t;
+    const V3 scene_width = t - b;
+    const V3 width = scene_width * (1.00001f / 1024.0f);
+    const V3 width_inv = 1.0f / width;
+    codes->resize(n);
+    for (int i = 0; i < n; ++i) {
+        const V3 c = (((*bounds)[i].high + (*bounds)[i].low) * 0.5f) - b;
+        (*codes)[i] = morton_code(c.x, c.y, c.z, width_inv.x, width_inv.y, width_inv.z);
+    }
+}
+
 // ssd/bvh_builder.cc:18-178
 int AacBuilder::build() {
     const int n = static_cast<int>(s_->tris.size());
@@ -333,32 +359,17 @@ int AacBuilder::build() {
     s_->indirect.resize(n);
     for (int i = 0; i < n; ++i) s_->indirect[i] = i;
 
-    V3 b(kMaxDist, kMaxDist, kMaxDist), t(-kMaxDist, -kMaxDist, -kMaxDist);
-    bounds_.resize(n);
-    for (int i = 0; i < n; ++i) {
-        const Triangle &tri = s_->tris[i];
-        // rt/triangle.h:56-57
-        bounds_[i].low = vmin(tri.a, vmin(tri.b, tri.c));
-        bounds_[i].high = vmax(tri.a, vmax(tri.b, tri.c));
-        b = vmin(b, bounds_[i].low);
-        t = vmax(t, bounds_[i].high);
-    }
+    V3 b, t;
+    std::vector<int> codes;
+    aac_codes(*s_, &bounds_, &b, &t, &codes);
     s_->scene_lo = b;
     s_->scene_hi = t;
-
     const V3 scene_width = t - b;
-    const V3 width = scene_width * (1.00001f / 1024.0f);
-    const V3 width_inv = 1.0f / width;
     max_leaf_sah_ = ((((scene_width.x * scene_width.y) + (scene_width.x * scene_width.z)) +
                       (scene_width.y * scene_width.z)) *
                      static_cast<float>(n)) *
                     p_.max_leaf_sah_factor;
 
-    std::vector<int> codes(n);
-    for (int i = 0; i < n; ++i) {
-        const V3 c = ((bounds_[i].high + bounds_[i].low) * 0.5f) - b;
-        codes[i] = morton_code(c.x, c.y, c.z, width_inv.x, width_inv.y, width_inv.z);
-    }
     std::vector<int> order;
     morton_radix_sort(codes, &code_buffer_, &order);
     s_->indirect = order;
@@ -413,6 +424,14 @@ void morton_radix_sort(const std::vector<int> &codes, std::vector<int> *sorted_c
         (*sorted_codes)[pos] = static_cast<int>(c1[i]);
         (*order)[pos] = p1[i];
     }
+}
+
+void aac_morton(const Scene &s, std::vector<int> *codes, std::vector<int> *order) {
+    std::vector<Bound> bounds;
+    V3 lo, hi;
+    std::vector<int> sorted;
+    aac_codes(s, &bounds, &lo, &hi, codes);
+    morton_radix_sort(*codes, &sorted, order);
 }
 
 int build_aac(Scene *scene, const AacParams &p) {

@@ -80,6 +80,30 @@ int32_t oracle_scene_build(oracle_scene *s, float alpha, float delta) {
 
 int32_t oracle_scene_node_count(const oracle_scene *s) { return s->s.root + 1; }
 
+int32_t oracle_scene_tree(const oracle_scene *so, float *lohi, int32_t *child, uint8_t *leaf, int32_t *indirect) {
+    const Scene &s = so->s;
+    const int count = s.root + 1;
+    for (int i = 0; i < count; ++i) {
+        const BvhNode &nd = s.nodes[i];
+        const float v[6] = {nd.low.x, nd.low.y, nd.low.z, nd.high.x, nd.high.y, nd.high.z};
+        for (int k = 0; k < 6; ++k) lohi[6 * i + k] = v[k];
+        child[2 * i] = nd.left_b;
+        child[2 * i + 1] = nd.right_e;
+        leaf[i] = nd.leaf ? 1 : 0;
+    }
+    for (size_t i = 0; i < s.indirect.size(); ++i) indirect[i] = s.indirect[i];
+    return count;
+}
+
+void oracle_aac_morton(const oracle_scene *so, int32_t *codes, int32_t *order) {
+    std::vector<int> c, o;
+    aac_morton(so->s, &c, &o);
+    for (size_t i = 0; i < c.size(); ++i) {
+        codes[i] = c[i];
+        order[i] = o[i];
+    }
+}
+
 int32_t oracle_bvh_check(const oracle_scene *so, int32_t *stats) {
     const Scene &s = so->s;
     const int n = static_cast<int>(s.tris.size());

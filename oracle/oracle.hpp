@@ -78,6 +78,9 @@ struct AacParams {
     int max_down_phase_depth = 35;
 };
 int build_aac(Scene *scene, const AacParams &p = AacParams());
+// The builder's Morton codes of the triangles (emplace order) and the order
+// its radix sort leaves them in (ssd/bvh_builder.cc:18-158).
+void aac_morton(const Scene &s, std::vector<int> *codes, std::vector<int> *order);
 // The three-pass 10-bit LSD counting sort of ssd/bvh_builder.cc:50-158.
 void morton_radix_sort(const std::vector<int> &codes, std::vector<int> *sorted_codes, std::vector<int> *order);
 

@@ -113,6 +113,8 @@ def _load():
         "oracle_scene_destroy": (None, [vp]),
         "oracle_scene_build": (i32, [vp, f, f]),
         "oracle_scene_node_count": (i32, [vp]),
+        "oracle_scene_tree": (i32, [vp, vp, vp, vp, vp]),
+        "oracle_aac_morton": (None, [vp, vp, vp]),
         "oracle_bvh_check": (i32, [vp, vp]),
         "oracle_scene_normals": (None, [vp, vp]),
         "oracle_trace_closest": (None, [vp, vp, i64, vp, vp, i32, i32, vp]),
@@ -205,6 +207,25 @@ class OracleScene:
         out = np.empty((self.n, 3), np.float32)
         LIB.oracle_scene_normals(self.h, _p(out))
         return out
+
+    def tree(self):
+        """The AAC tree: dict(lohi (m, 6), child (m, 2), leaf (m,) bool,
+        indirect (n,), root) with nodes numbered as the builder made them."""
+        m = max(1, 2 * self.n)
+        lohi = np.empty((m, 6), np.float32)
+        child = np.empty((m, 2), np.int32)
+        leaf = np.empty(m, np.uint8)
+        ind = np.empty(max(1, self.n), np.int32)
+        cnt = LIB.oracle_scene_tree(self.h, _p(lohi), _p(child), _p(leaf), _p(ind))
+        return dict(lohi=lohi[:cnt], child=child[:cnt], leaf=leaf[:cnt].astype(bool), indirect=ind[:self.n],
+                    root=cnt - 1)
+
+    def aac_morton(self):
+        """The AAC builder's Morton codes (emplace order) and sort order."""
+        codes = np.empty(max(1, self.n), np.int32)
+        order = np.empty(max(1, self.n), np.int32)
+        LIB.oracle_aac_morton(self.h, _p(codes), _p(order))
+        return codes[:self.n], order[:self.n]
 
     def trace_closest(self, rays, use_bvh=True, nthreads=8):
         rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)

@@ -153,6 +153,14 @@ void oracle_scene_destroy(oracle_scene *s);
 /* AAC build (alpha, delta as ssd/bvh_builder.h:23). Returns the root. */
 int32_t oracle_scene_build(oracle_scene *s, float alpha, float delta);
 int32_t oracle_scene_node_count(const oracle_scene *s);
+/* The AAC tree as build_aac numbers it (root = count - 1): per node the box
+ * (lo xyz, hi xyz), child[2] (interior: left and right node; leaf: the
+ * indirect range [b, e)), leaf flag; and the indirect order. Returns the
+ * node count. */
+int32_t oracle_scene_tree(const oracle_scene *s, float *lohi, int32_t *child, uint8_t *leaf, int32_t *indirect);
+/* The AAC builder's Morton code of each triangle (emplace order) and its
+ * radix sort's order. */
+void oracle_aac_morton(const oracle_scene *s, int32_t *codes, int32_t *order);
 /* Checks BVH invariants; returns 0 when they hold. stats: [leaves, interior,
  * max depth, max leaf size]. */
 int32_t oracle_bvh_check(const oracle_scene *s, int32_t *stats);

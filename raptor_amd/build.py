@@ -15,7 +15,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libraptor_rt.so")
-SOURCES = ["api.hip", "bvh_build.hip", "trace.hip", "render.hip", "materials.hip", "postprocess.hip", "adjoint.hip", "invrender.hip", "batch.hip", "scene_gen.cpp", "ingest.cpp", "jpeg.hip", "glare.hip"]
+SOURCES = ["api.hip", "bvh_build.hip", "trace.hip", "render.hip", "materials.hip", "postprocess.hip", "adjoint.hip", "invrender.hip", "batch.hip", "scene_gen.cpp", "ingest.cpp", "jpeg.hip", "glare.hip", "aac_build.hip"]
 ARCH = os.environ.get("RAPTOR_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

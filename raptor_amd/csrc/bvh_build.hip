@@ -1866,6 +1866,40 @@ int32_t rt_accel_download(const rt_accel *accel, void *nodes, int32_t *tri_ids) 
     return RT_OK;
 }
 
+int32_t rt_build_aac(rt_scene *scene, float alpha, float delta, uint32_t *codes, uint32_t *order, int32_t *indirect,
+                     rt_aac_node *nodes, int64_t capacity, int64_t *nnodes, int32_t *root, float *build_ms) {
+    if (!scene) return RT_ERR_INVALID;
+    rt_ctx *ctx = scene->ctx;
+    RT_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    const int64_t n = scene->n;
+    if ((n > 0 && (!indirect || !nodes)) || capacity < 2 * n - 1)
+        return rt::fail(ctx, RT_ERR_INVALID, "rt_build_aac: indirect (n) and nodes (at least 2n - 1) are required");
+    rt::AacTree t;
+    if (int32_t rc = rt::build_aac_tree(scene, alpha, delta, &t)) return rc;
+    if (t.nodes > capacity) return rt::fail(ctx, RT_ERR_INVALID, "rt_build_aac: more nodes than capacity");
+    for (int64_t i = 0; i < n; ++i) {
+        if (codes) codes[i] = t.codes[i];
+        if (order) order[i] = t.order[i];
+        indirect[i] = t.indirect[i];
+    }
+    for (int64_t i = 0; i < t.nodes; ++i) {
+        rt_aac_node &o = nodes[i];
+        o.lo[0] = t.lo[i].x;
+        o.lo[1] = t.lo[i].y;
+        o.lo[2] = t.lo[i].z;
+        o.hi[0] = t.hi[i].x;
+        o.hi[1] = t.hi[i].y;
+        o.hi[2] = t.hi[i].z;
+        o.child[0] = t.kid[i].x;
+        o.child[1] = t.kid[i].y;
+        o.leaf = t.leaf[i];
+    }
+    if (nnodes) *nnodes = t.nodes;
+    if (root) *root = t.root;
+    if (build_ms) *build_ms = t.build_ms;
+    return RT_OK;
+}
+
 int32_t rt_accel_root(const rt_accel *accel, int32_t *root, float *box_pad) {
     if (!accel || !root) return RT_ERR_INVALID;
     *root = accel->root;

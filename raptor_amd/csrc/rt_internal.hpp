@@ -189,6 +189,23 @@ int32_t fail(rt_ctx *ctx, int32_t code, const std::string &msg);
 // Build entry (bvh_build.hip).
 int32_t build_accel(rt_scene *scene, const rt_build_params &p, rt_accel **out);
 
+// The reference builder's binary tree (aac_build.hip), on the host: node
+// boxes, children (interior: left and right node id; leaf: the range [b, e)
+// of indirect), leaf flags, the root id, the final triangle order, and the
+// Morton codes (emplace order) with the order the sort left them in.
+struct AacTree {
+    int64_t n = 0, nodes = 0;
+    int32_t root = -1;
+    int32_t split_nodes = 0;  // nodes of the top-down split
+    float build_ms = 0.0f;
+    std::vector<uint32_t> codes, order;
+    std::vector<int32_t> indirect;
+    std::vector<float4> lo, hi;
+    std::vector<int2> kid;
+    std::vector<uint8_t> leaf;
+};
+int32_t build_aac_tree(rt_scene *scene, float alpha, float delta, AacTree *out);
+
 // Trace launchers (trace.hip). Rays, hits and flags are device pointers.
 // stage tags the kernel symbol: 0 primary, 1 shadow, 2 bounce, 3 bounce
 // shadow, 4 / 5 direct API calls (closest / any hit).

@@ -39,6 +39,7 @@ EXPORTS = [
     "rt_mesh_copy", "rt_mesh_last_error", "rt_mesh_destroy", "rt_scene_create_from_mesh", "rt_write_jpeg",
     "rt_mesh_camera", "rt_mesh_lights", "rt_build_morton", "rt_accel_download", "rt_accel_root", "rt_glare_filter",
     "rt_scene_set_textures", "rt_scene_set_texture_coords", "rt_texture_load_tga", "rt_mesh_copy_textures",
+    "rt_build_aac",
 ]
 
 
@@ -136,6 +137,8 @@ def load_library(path=LIB_PATH):
         "rt_render_static": (i32, [vp, C.POINTER(rt_camera), C.POINTER(rt_light), i32, C.POINTER(rt_render_params),
                                    fp, fp, fp, C.POINTER(rt_stats)]),
         "rt_build_morton": (i32, [vp, fp, fp]),
+        "rt_build_aac": (i32, [vp, C.c_float, C.c_float, fp, fp, fp, fp, i64, C.POINTER(C.c_int64),
+                               C.POINTER(C.c_int32), C.POINTER(C.c_float)]),
         "rt_glare_filter": (i32, [vp, i32, i32, C.c_float, i32, fp, u32]),
         "rt_accel_download": (i32, [vp, fp, fp]),
         "rt_accel_root": (i32, [vp, C.POINTER(C.c_int32), C.POINTER(C.c_float)]),
@@ -276,6 +279,10 @@ MATERIAL_DTYPE = np.dtype([("ka", "<f4", 3), ("kd", "<f4", 3), ("ks", "<f4", 3),
                            ("clrh", "<f4"), ("clrf", "<f4"), ("layer_first", "<i4", 5), ("layer_count", "<i4", 5),
                            ("iterations", "<i4")])
 assert MATERIAL_DTYPE.itemsize == 160
+
+# rt_aac_node (include/raptor_rt.h).
+AAC_NODE_DTYPE = np.dtype([("lo", "<f4", 3), ("hi", "<f4", 3), ("child", "<i4", 2), ("leaf", "<i4")])
+assert AAC_NODE_DTYPE.itemsize == 36
 
 MAP_PLANAR, MAP_CUBIC, MAP_CYLINDRICAL, MAP_CHECKER_BOARD, MAP_PERLIN_3D, MAP_GRADIENT = 0, 1, 2, 3, 4, 5
 GRAD_LINE, GRAD_SPLINE, GRAD_STEP = 0, 1, 2
@@ -636,6 +643,25 @@ class Scene:
         order = np.empty(self.n, np.uint32)
         self.ctx.check(_LIB.rt_build_morton(self.h, _ptr(codes), _ptr(order)), "rt_build_morton")
         return codes, order
+
+    def build_aac(self, alpha=0.4, delta=20.0):
+        """The reference builder's tree built on the GPU (rt_build_aac):
+        dict(codes, order, indirect, lohi (m, 6), child (m, 2), leaf (m,)
+        bool, root, build_ms)."""
+        n = self.n
+        cap = max(1, 2 * n - 1)
+        codes = np.empty(max(1, n), np.uint32)
+        order = np.empty(max(1, n), np.uint32)
+        ind = np.empty(max(1, n), np.int32)
+        nodes = np.empty(cap, AAC_NODE_DTYPE)
+        nn, root, ms = C.c_int64(0), C.c_int32(-1), C.c_float(0.0)
+        self.ctx.check(_LIB.rt_build_aac(self.h, alpha, delta, _ptr(codes), _ptr(order), _ptr(ind), _ptr(nodes), cap,
+                                         C.byref(nn), C.byref(root), C.byref(ms)), "rt_build_aac")
+        m = nn.value
+        nd = nodes[:m]
+        return dict(codes=codes[:n], order=order[:n], indirect=ind[:n],
+                    lohi=np.concatenate([nd["lo"], nd["hi"]], axis=1), child=nd["child"].copy(),
+                    leaf=nd["leaf"] != 0, root=root.value, build_ms=ms.value)
 
     def close(self):
         if self.h:
