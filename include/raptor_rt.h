@@ -80,13 +80,21 @@ typedef struct {
  * leaf_size: most triangles per leaf (1..16); sah_bins: bins per axis of the
  * binned-SAH rebuild of the Morton LBVH (4..32; 0 = keep the LBVH: faster
  * build, slower traversal); lds_stack: traversal-stack entries per lane kept
- * in LDS before spilling to private memory (4, 8 or 16; 0 = 16). params ==
- * NULL means width 4, leaf size 2, 16 SAH bins, 8 LDS entries. */
+ * in LDS before spilling to private memory (4, 8 or 16; 0 = 16); builder:
+ * RT_BUILDER_LBVH (the Morton LBVH with the binned-SAH rebuild) or
+ * RT_BUILDER_AAC (the reference builder's own tree, rt_build_aac, collapsed
+ * to width-wide nodes; its leaves are the reference's, so leaf_size and
+ * sah_bins do not apply, and a leaf of more than 16 triangles takes more
+ * than one child slot). params == NULL means width 4, leaf size 2, 16 SAH
+ * bins, 8 LDS entries, RT_BUILDER_LBVH. */
+#define RT_BUILDER_LBVH 0
+#define RT_BUILDER_AAC 1
 typedef struct {
     int32_t width;
     int32_t leaf_size;
     int32_t sah_bins;
     int32_t lds_stack;
+    int32_t builder;
 } rt_build_params;
 
 /* Frame options: bounce = 1 adds one cosine-weighted diffuse bounce with

@@ -959,11 +959,224 @@ int32_t build_aac_tree(rt_scene *scene, float alpha, float delta, AacTree *out) 
     AAC_CHECK(hipMemcpyAsync(out->kid.data(), bn_kid, total_nodes * sizeof(int2), hipMemcpyDeviceToHost, s));
     AAC_CHECK(hipMemcpyAsync(out->leaf.data(), bn_leaf, total_nodes, hipMemcpyDeviceToHost, s));
     AAC_CHECK(hipMemcpyAsync(out->indirect.data(), order, n * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    out->tri_lo.resize(n);
+    out->tri_hi.resize(n);
+    AAC_CHECK(hipMemcpyAsync(out->tri_lo.data(), tlo, n * sizeof(float4), hipMemcpyDeviceToHost, s));
+    AAC_CHECK(hipMemcpyAsync(out->tri_hi.data(), thi, n * sizeof(float4), hipMemcpyDeviceToHost, s));
     AAC_CHECK(hipStreamSynchronize(s));
     out->split_nodes = nn;
     out->build_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
     cleanup();
 #undef AAC_CHECK
+    return RT_OK;
+}
+
+namespace {
+
+// One child slot of a wide node under construction: a binary node of the
+// tree, or a run of a leaf's triangles (a leaf of more than kMaxLeafSize
+// triangles is halved until each run fits one leaf code).
+struct Slot {
+    int32_t bin;         // binary node id, or -1 for a run
+    int32_t b, e;        // the run (and a leaf node's range)
+    float4 lo, hi;
+    bool interior;       // an interior binary node
+};
+
+float half_area_host(const float4 &lo, const float4 &hi) {
+    const float dx = hi.x - lo.x, dy = hi.y - lo.y, dz = hi.z - lo.z;
+    return (dx * dy) + (dx * dz) + (dy * dz);
+}
+
+// Opens a slot into its two halves: an interior node's children, or a long
+// run's halves (boxes from the triangles').
+void open_slot(const AacTree &t, const Slot &x, Slot out[2]) {
+    if (x.interior) {
+        for (int side = 0; side < 2; ++side) {
+            const int32_t c = side == 0 ? t.kid[x.bin].x : t.kid[x.bin].y;
+            Slot &o = out[side];
+            o.bin = c;
+            o.interior = !t.leaf[c];
+            o.lo = t.lo[c];
+            o.hi = t.hi[c];
+            o.b = o.interior ? 0 : t.kid[c].x;
+            o.e = o.interior ? 0 : t.kid[c].y;
+        }
+        return;
+    }
+    const int32_t m = x.b + (x.e - x.b + 1) / 2;
+    for (int side = 0; side < 2; ++side) {
+        Slot &o = out[side];
+        o.bin = -1;
+        o.interior = false;
+        o.b = side == 0 ? x.b : m;
+        o.e = side == 0 ? m : x.e;
+        o.lo = make_float4(INFINITY, INFINITY, INFINITY, 0.0f);
+        o.hi = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.0f);
+        for (int32_t k = o.b; k < o.e; ++k) {
+            const float4 l = t.tri_lo[t.indirect[k]], h = t.tri_hi[t.indirect[k]];
+            o.lo = make_float4(std::min(o.lo.x, l.x), std::min(o.lo.y, l.y), std::min(o.lo.z, l.z), 0.0f);
+            o.hi = make_float4(std::max(o.hi.x, h.x), std::max(o.hi.y, h.y), std::max(o.hi.z, h.z), 0.0f);
+        }
+    }
+}
+
+bool openable(const Slot &x) { return x.interior || (x.e - x.b) > kMaxLeafSize; }
+
+// Breadth-first collapse of the binary tree into W-wide nodes (so each
+// level's nodes are contiguous, as refit expects): a node opens its
+// largest-area openable child until it holds W children.
+template <int W>
+void collapse_host(const AacTree &t, const Slot &root, float pad, std::vector<WideNode<W>> *nodes,
+                   std::vector<int64_t> *level_start, int64_t *leaves) {
+    std::vector<Slot> cur{root}, next;
+    *leaves = 0;
+    while (!cur.empty()) {
+        level_start->push_back(static_cast<int64_t>(nodes->size()));
+        const int64_t base_next = static_cast<int64_t>(nodes->size() + cur.size());
+        next.clear();
+        for (const Slot &job : cur) {
+            Slot c[W];
+            int cnt = 0;
+            open_slot(t, job, c);
+            cnt = 2;
+            while (cnt < W) {
+                int best = -1;
+                float ba = -1.0f;
+                for (int k = 0; k < cnt; ++k)
+                    if (openable(c[k]) && half_area_host(c[k].lo, c[k].hi) > ba) {
+                        ba = half_area_host(c[k].lo, c[k].hi);
+                        best = k;
+                    }
+                if (best < 0) break;
+                Slot two[2];
+                open_slot(t, c[best], two);
+                c[best] = two[0];
+                c[cnt++] = two[1];
+            }
+            WideNode<W> out{};
+            for (int k = 0; k < W; ++k) {
+                if (k >= cnt) {
+                    out.lo_x[k] = out.lo_y[k] = out.lo_z[k] = INFINITY;
+                    out.hi_x[k] = out.hi_y[k] = out.hi_z[k] = -INFINITY;
+                    out.child[k] = kEmpty;
+                    continue;
+                }
+                out.lo_x[k] = c[k].lo.x - pad;
+                out.lo_y[k] = c[k].lo.y - pad;
+                out.lo_z[k] = c[k].lo.z - pad;
+                out.hi_x[k] = c[k].hi.x + pad;
+                out.hi_y[k] = c[k].hi.y + pad;
+                out.hi_z[k] = c[k].hi.z + pad;
+                if (openable(c[k])) {
+                    out.child[k] = static_cast<int32_t>(base_next + static_cast<int64_t>(next.size()));
+                    next.push_back(c[k]);
+                } else {
+                    out.child[k] = leaf_code(c[k].b, c[k].e - c[k].b);
+                    ++*leaves;
+                }
+            }
+            nodes->push_back(out);
+        }
+        std::swap(cur, next);
+    }
+}
+
+}  // namespace
+
+int32_t build_accel_aac(rt_scene *scene, const rt_build_params &p, rt_accel **out) {
+    rt_ctx *ctx = scene->ctx;
+    const int W = p.width;
+    if (W != 4 && W != 8) return fail(ctx, RT_ERR_INVALID, "width must be 4 or 8");
+    const auto t0 = std::chrono::steady_clock::now();
+    AacTree t;
+    if (int32_t rc = build_aac_tree(scene, 0.4f, 20.0f, &t)) return rc;
+    const int64_t n = scene->n;
+    hipStream_t s = ctx->stream;
+    rt_accel *a = new rt_accel();
+    a->scene = scene;
+    a->triangles = n;
+    a->width = W;
+    a->leaf_size = kMaxLeafSize;
+    a->sah_bins = 0;
+    a->lds_stack = (p.lds_stack <= 0) ? 16 : (p.lds_stack <= 4 ? 4 : (p.lds_stack <= 8 ? 8 : 16));
+    auto bad = [&](hipError_t e, const char *what) {
+        (void)hipStreamSynchronize(s);
+        if (a->d_nodes) (void)hipFree(a->d_nodes);
+        if (a->d_tris) (void)hipFree(a->d_tris);
+        delete a;
+        return fail(ctx, e == hipErrorOutOfMemory ? RT_ERR_OOM : RT_ERR_HIP,
+                    std::string("AAC accel: ") + what + ": " + hipGetErrorString(e));
+    };
+    if (n == 0) {
+        a->root = kEmpty;
+        *out = a;
+        return RT_OK;
+    }
+    const float4 rlo = t.lo[t.root], rhi = t.hi[t.root];
+    float maxabs = 0.0f;
+    for (const float v : {rlo.x, rlo.y, rlo.z, rhi.x, rhi.y, rhi.z}) maxabs = std::max(maxabs, std::fabs(v));
+    a->box_pad = std::ldexp(std::max(maxabs, 1e-30f), -15);
+    a->near_limit = 256.0f * maxabs;
+    const float ext[3] = {rhi.x - rlo.x, rhi.y - rlo.y, rhi.z - rlo.z};
+    a->scene_lo[0] = rlo.x;
+    a->scene_lo[1] = rlo.y;
+    a->scene_lo[2] = rlo.z;
+    for (int k = 0; k < 3; ++k) a->scene_inv[k] = ext[k] > 0.0f ? 1.0f / ext[k] : 0.0f;
+    // The triangles in the builder's order.
+    uint32_t *d_order = nullptr;
+    hipError_t e = hipMalloc(&d_order, n * sizeof(uint32_t));
+    if (e != hipSuccess) return bad(e, "order");
+    e = hipMemcpyAsync(d_order, t.indirect.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMalloc(&a->d_tris, n * sizeof(LeafTri));
+    if (e == hipSuccess) e = make_leaf_tris(scene->d_abc, scene->d_flags, d_order, n, a->d_tris, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    (void)hipFree(d_order);
+    if (e != hipSuccess) return bad(e, "triangles");
+    Slot root;
+    root.bin = t.root;
+    root.interior = !t.leaf[t.root];
+    root.lo = rlo;
+    root.hi = rhi;
+    root.b = root.interior ? 0 : t.kid[t.root].x;
+    root.e = root.interior ? 0 : t.kid[t.root].y;
+    if (!openable(root)) {
+        a->root = leaf_code(root.b, root.e - root.b);
+        a->nodes = 0;
+        a->leaves = 1;
+        a->depth = 0;
+    } else {
+        const float pad = a->box_pad;
+        int64_t leaves = 0;
+        size_t bytes = 0;
+        std::vector<char> host;
+        if (W == 4) {
+            std::vector<WideNode<4>> nodes;
+            collapse_host<4>(t, root, pad, &nodes, &a->level_start, &leaves);
+            a->nodes = static_cast<int64_t>(nodes.size());
+            bytes = nodes.size() * sizeof(WideNode<4>);
+            host.resize(bytes);
+            std::memcpy(host.data(), nodes.data(), bytes);
+        } else {
+            std::vector<WideNode<8>> nodes;
+            collapse_host<8>(t, root, pad, &nodes, &a->level_start, &leaves);
+            a->nodes = static_cast<int64_t>(nodes.size());
+            bytes = nodes.size() * sizeof(WideNode<8>);
+            host.resize(bytes);
+            std::memcpy(host.data(), nodes.data(), bytes);
+        }
+        if (bytes > 0xffffffffull) return bad(hipErrorInvalidValue, "node array beyond 4 GiB");
+        e = hipMalloc(&a->d_nodes, bytes);
+        if (e == hipSuccess) e = hipMemcpyAsync(a->d_nodes, host.data(), bytes, hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return bad(e, "nodes");
+        a->leaves = leaves;
+        a->depth = static_cast<int>(a->level_start.size());
+        a->level_start.push_back(a->nodes);
+        a->root = 0;
+    }
+    a->build_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    *out = a;
     return RT_OK;
 }
 

@@ -1209,8 +1209,18 @@ hipError_t dalloc_in(BuildArena &ar, DevBuf &b, size_t count) {
 
 }  // namespace
 
+hipError_t make_leaf_tris(const float *abc, const uint8_t *flags, const uint32_t *order, int64_t n, LeafTri *out,
+                          hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_leaf_tris, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, s, abc, flags, order,
+                       n, out);
+    return hipGetLastError();
+}
+
 int32_t build_accel(rt_scene *scene, const rt_build_params &p, rt_accel **out) {
     rt_ctx *ctx = scene->ctx;
+    if (p.builder == RT_BUILDER_AAC) return build_accel_aac(scene, p, out);
+    if (p.builder != RT_BUILDER_LBVH) return fail(ctx, RT_ERR_INVALID, "unknown builder");
     const int W = p.width;
     if (W != 4 && W != 8) return fail(ctx, RT_ERR_INVALID, "width must be 4 or 8");
     if (p.leaf_size < 1 || p.leaf_size > kMaxLeafSize) return fail(ctx, RT_ERR_INVALID, "leaf_size must be 1..16");

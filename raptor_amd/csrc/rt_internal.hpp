@@ -203,8 +203,15 @@ struct AacTree {
     std::vector<float4> lo, hi;
     std::vector<int2> kid;
     std::vector<uint8_t> leaf;
+    std::vector<float4> tri_lo, tri_hi;  // triangle boxes, emplace order
 };
 int32_t build_aac_tree(rt_scene *scene, float alpha, float delta, AacTree *out);
+// A wide BVH over the reference builder's tree (rt_build_params.builder
+// RT_BUILDER_AAC).
+int32_t build_accel_aac(rt_scene *scene, const rt_build_params &p, rt_accel **out);
+// LeafTri records of the triangles in the given order (bvh_build.hip).
+hipError_t make_leaf_tris(const float *abc, const uint8_t *flags, const uint32_t *order, int64_t n, LeafTri *out,
+                          hipStream_t s);
 
 // Trace launchers (trace.hip). Rays, hits and flags are device pointers.
 // stage tags the kernel symbol: 0 primary, 1 shadow, 2 bounce, 3 bounce

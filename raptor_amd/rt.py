@@ -53,8 +53,12 @@ class rt_light(C.Structure):
     _fields_ = [("pos", C.c_float * 3), ("rgb", C.c_float * 3), ("d", C.c_float)]
 
 
+BUILDER_LBVH, BUILDER_AAC = 0, 1
+
+
 class rt_build_params(C.Structure):
-    _fields_ = [("width", C.c_int32), ("leaf_size", C.c_int32), ("sah_bins", C.c_int32), ("lds_stack", C.c_int32)]
+    _fields_ = [("width", C.c_int32), ("leaf_size", C.c_int32), ("sah_bins", C.c_int32), ("lds_stack", C.c_int32),
+                ("builder", C.c_int32)]
 
 
 class rt_render_params(C.Structure):
@@ -807,10 +811,12 @@ class Batch:
 
 
 class Accel:
-    def __init__(self, scene, width=4, leaf_size=2, sah_bins=16, lds_stack=8):
+    def __init__(self, scene, width=4, leaf_size=2, sah_bins=16, lds_stack=8, builder=BUILDER_LBVH):
+        """builder BUILDER_AAC traverses the reference builder's own tree
+        (leaf_size and sah_bins do not apply)."""
         self.scene = scene
         self.ctx = scene.ctx
-        p = rt_build_params(width, leaf_size, sah_bins, lds_stack)
+        p = rt_build_params(width, leaf_size, sah_bins, lds_stack, builder)
         h = C.c_void_p()
         self.ctx.check(_LIB.rt_accel_build(scene.h, C.byref(p), C.byref(h)), "rt_accel_build")
         self.h = h

@@ -100,3 +100,62 @@ def test_terrain_1m_headline_scene(gpu_ctx, rt, O):
     abc, _ = workloads.terrain_arrays(708)
     g, t = compare(rt, O, gpu_ctx, abc)
     print(f"AAC tree of {abc.shape[0]} triangles: {len(g['leaf'])} nodes, GPU build {g['build_ms']:.1f} ms")
+
+
+# ------------------------------------------- frames on the reference's tree
+
+@pytest.mark.parametrize("width", [4, 8])
+def test_accel_on_the_aac_tree(gpu_ctx, rt, O, width):
+    """rt_accel_build with RT_BUILDER_AAC: the wide tree keeps the AAC
+    triangle order and leaves (split only where a leaf exceeds 16 triangles),
+    satisfies the tree invariants, and traces the oracle's hits."""
+    from test_gpu_build import check_tree
+    from raptor_amd import workloads
+
+    from helpers import compare_hits, tuv_exact
+
+    w = workloads.random65k()
+    scene = rt.Scene(gpu_ctx, w.abc, w.albedo)
+    acc = rt.Accel(scene, width=width, builder=rt.BUILDER_AAC)
+    check_tree(acc, w.abc, 16)
+    g = scene.build_aac()
+    t = acc.download()
+    assert np.array_equal(t["tri_ids"], g["indirect"])
+    # every wide leaf lies inside one AAC leaf
+    bounds = np.zeros(w.abc.shape[0] + 1, np.int64)
+    for x in np.nonzero(g["leaf"])[0]:
+        bounds[g["child"][x][0]] = 1
+    owner = np.cumsum(bounds[:-1]) - 1
+    codes = t["child"][(t["child"] != -1) & (t["child"] < 0)].view(np.uint32)
+    first = ((codes & np.uint32(0x7FFFFFFF)) >> np.uint32(4)).astype(np.int64)
+    count = (codes & np.uint32(15)).astype(np.int64) + 1
+    assert np.all(owner[first] == owner[first + count - 1])
+    rays = O.primary_rays(w.camera)
+    gh = acc.trace_closest(rays)
+    r, _ = O.OracleScene(w.abc).trace_closest(rays)
+    rtuv = np.stack([r["t"], r["u"], r["v"]], 1)
+    bad, ties = compare_hits(gh, r["prim"], rtuv)
+    assert not bad.any() and tuv_exact(gh, rtuv, (r["prim"] >= 0) & ~ties)
+
+
+def test_frame_and_refit_on_the_aac_tree(gpu_ctx, rt, O):
+    """A terrain frame with a bounce on the AAC-built accel, then the terrain
+    moved and refit, both against the oracle (ids bit-exact, radiance 1e-5)."""
+    from raptor_amd import workloads
+
+    from helpers import compare_hits, rel_err
+
+    w = workloads.terrain(grid=200, x_res=320, y_res=180, bounce=True)
+    scene = rt.Scene(gpu_ctx, w.abc, w.albedo)
+    acc = rt.Accel(scene, builder=rt.BUILDER_AAC)
+    for step in range(2):
+        abc = w.abc if step == 0 else workloads.terrain_wave(w.abc, 0.7)
+        if step == 1:
+            scene.update(abc)
+            acc.refit()
+        out = acc.render(w.camera, w.lights, bounce=True, seed=w.seed)
+        ref = O.OracleScene(abc).render(w.albedo, w.camera, w.lights, bounce=True, seed=w.seed, nthreads=8)
+        bad, ties = compare_hits(out["primary"], ref["prim"], ref["tuv"])
+        assert not bad.any()
+        e = rel_err(out["rgb"].reshape(-1, 3)[~ties], ref["rgb"][~ties])
+        assert e.max() <= 1e-5, e.max()
