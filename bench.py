@@ -442,6 +442,45 @@ def invrender_section(ctx, steps, n_scenes=16, res=512, spp=2, bounces=2):
     return res_out
 
 
+def aac_section(ctx, w, scene, accel, reps):
+    """The reference builder's tree (rt_build_aac) on the headline terrain:
+    GPU build time (median of reps), and synchronous frames on the accel
+    built over it (RT_BUILDER_AAC) against the default tree's."""
+    import numpy as np
+
+    from raptor_amd import rt
+
+    builds = [scene.build_aac()["build_ms"] for _ in range(reps)]
+
+    def frames(acc):
+        for _ in range(2):
+            acc.render(w.camera, w.lights, bounce=True, seed=w.seed, want_hits=False)
+        runs = [acc.render(w.camera, w.lights, bounce=True, seed=w.seed, want_hits=False) for _ in range(reps + 2)]
+        st = np.median(np.array([r["stats"]["ms"] for r in runs]), axis=0)
+        ctx.set_counting(True)
+        c = acc.render(w.camera, w.lights, bounce=True, seed=w.seed, want_hits=False)["stats"]
+        ctx.set_counting(False)
+        prim = max(1, c["rays"][0])
+        return {"frame_gpu_ms": round(float(st[0]), 4), "primary_ms": round(float(st[1]), 4),
+                "shadow_ms": round(float(st[2]), 4), "bounce_ms": round(float(st[3]), 4),
+                "node_visits_per_primary_ray": round(c["visits"][0] / prim, 2),
+                "tri_tests_per_primary_ray": round(c["visits"][1] / prim, 2)}
+
+    acc = rt.Accel(scene, builder=rt.BUILDER_AAC)
+    info = acc.info()
+    out = {
+        "workload": "the headline terrain's AAC tree (alpha 0.4, delta 20), built on the GPU node for node as "
+                    "ssd/bvh_builder.cc builds it, and frames traced on it",
+        "build_ms": round(sorted(builds)[reps // 2], 1),
+        "accel_build_ms": round(info["build_ms"], 1),
+        "wide_nodes": info["nodes"], "leaves": info["leaves"], "depth": info["depth"],
+        "on_aac_tree": frames(acc),
+        "on_default_tree": frames(accel),
+    }
+    acc.close()
+    return out
+
+
 def dynamic_section(ctx, w, accel, scene, frames, args):
     """Per-frame update + refit + frame for a moving terrain (a travelling wave
     of amplitude 0.01, one phase step per frame), against rebuilding the
@@ -737,6 +776,7 @@ def main():
     ap.add_argument("--material-frames", type=int, default=5, help="0 skips the phong-material workload")
     ap.add_argument("--post-reps", type=int, default=10, help="0 skips the camera post-process workload")
     ap.add_argument("--ingest-reps", type=int, default=3, help="0 skips the scene-ingestion workload")
+    ap.add_argument("--aac-reps", type=int, default=3, help="0 skips the reference builder's tree")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each GPU would trace its own share "
@@ -864,6 +904,9 @@ def main():
     if args.ingest_reps > 0:
         print("bench: scene ingestion", file=sys.stderr, flush=True)
         line["ingest"] = ingest_section(ctx, w, args, args.ingest_reps)
+    if args.aac_reps > 0:
+        print("bench: the reference builder's tree", file=sys.stderr, flush=True)
+        line["aac"] = aac_section(ctx, w, scene, accel, args.aac_reps)
     if args.batched_scenes > 0:
         del rgb, accel, scene
         print("bench: batched workload", file=sys.stderr, flush=True)
