@@ -25,7 +25,8 @@
 //                              matters, below) and the ids of its new nodes
 //   k_aac_merge (per level,    each node's merge, deepest level first: one
 //   deepest first)             wave per node of at most 64 clusters, one
-//                              1024-lane workgroup above
+//                              1024-lane workgroup above, which keeps each
+//                              row's least entry between merges
 //
 // No cost matrix is stored: every entry the reference reads equals the
 // combined surface area of the two clusters at its row and column in the
@@ -506,6 +507,12 @@ __device__ __forceinline__ void write_leaves(const MergeArgs &a, const MergeJob 
 // row i and the new node takes row j.
 __device__ __forceinline__ void merge_step(const MergeArgs &a, int32_t *ids, float4 *lo, float4 *hi, int32_t &k,
                                            int32_t node_id, int32_t i, int32_t j) {
+    // No least entry (every cost NaN, from non-finite vertices): merge the
+    // first two rows rather than index outside the list.
+    if (i < 1 || i >= k || j < 0 || j >= i) {
+        i = 1;
+        j = 0;
+    }
     const float4 nlo = make_float4(rmin(lo[i].x, lo[j].x), rmin(lo[i].y, lo[j].y), rmin(lo[i].z, lo[j].z), 0.0f);
     const float4 nhi = make_float4(rmax(hi[i].x, hi[j].x), rmax(hi[i].y, hi[j].y), rmax(hi[i].z, hi[j].z), 0.0f);
     a.bn_lo[node_id] = nlo;
@@ -561,23 +568,110 @@ __global__ void __launch_bounds__(64) k_aac_merge_wave(MergeArgs a) {
     for (int32_t q = lane; q < k; q += 64) a.lists[jb.b + q] = ids[q];
 }
 
-// One 1024-lane workgroup per job of more than kWaveList clusters; lists of
-// more than kBlockList clusters stay in global memory (the top of scenes of
-// several million triangles).
-__global__ void __launch_bounds__(1024) k_aac_merge_block(MergeArgs a, int32_t *g_ids, float4 *g_lo, float4 *g_hi,
-                                                          int32_t g_stride) {
-    __shared__ int32_t s_ids[kBlockList];
-    __shared__ float4 s_lo[kBlockList], s_hi[kBlockList];
-    __shared__ Key s_best[16];
-    __shared__ int32_t s_k;
+// Tie part of an entry's key, below its cost: class (head 0, aligned body 1,
+// tail 2), SSE lane in the body, then the address relative to the node's
+// matrix start. Orders entries of equal cost as key_less does.
+__device__ __forceinline__ uint64_t entry_tie(int64_t P, int64_t S, int64_t A, int64_t body_end) {
+    uint64_t cls = 2, lane = 0;
+    if (P < A) {
+        cls = 0;
+    } else if (P < body_end) {
+        cls = 1;
+        lane = static_cast<uint64_t>(P & 3);
+    }
+    return (cls << 62) | (lane << 60) | static_cast<uint64_t>(P - S);
+}
+
+__device__ __forceinline__ bool entry_less(float c1, uint64_t t1, float c2, uint64_t t2) {
+    return c1 < c2 || (c1 == c2 && t1 < t2);
+}
+
+// The region boundaries of the current packed triangle of k clusters that
+// starts at S: head entries lie before A, the aligned body before body_end.
+__device__ __forceinline__ void matrix_regions(int64_t S, int32_t k, int64_t *A, int64_t *body_end) {
+    const int64_t E = S + ((static_cast<int64_t>(k) * (k - 1)) >> 1);
+    *A = (S & ~int64_t(3)) + ((S & 3) ? 4 : 0);
+    *body_end = (E > *A) ? *A + (((E - *A) >> 2) << 2) : *A;
+}
+
+// Cluster rows of a workgroup job, in LDS or (long lists) in global memory.
+struct Rows {
+    int32_t *ids;
+    float4 *lo, *hi;
+    float *rc;        // per row r >= 1: the least cost among its entries (r, c < r)
+    uint64_t *rt;     // and that entry's tie key
+    int32_t *todo;    // rows to rescan in full
+};
+
+// Rows of more than kWaveList and at most kBlockList clusters stay in LDS.
+struct RowsLds {
+    int32_t ids[kBlockList];
+    float4 lo[kBlockList], hi[kBlockList];
+    float rc[kBlockList];
+    uint64_t rt[kBlockList];
+    int32_t todo[kBlockList];
+};
+
+// Full rescans of the listed rows, one wave per row.
+__device__ __forceinline__ void rescan_rows(const Rows &R, int32_t nrows, int64_t S, int64_t A, int64_t body_end,
+                                            int w, int lane) {
+    for (int32_t q = w; q < nrows; q += 16) {
+        const int32_t r = R.todo[q];
+        const int64_t row0 = S + ((static_cast<int64_t>(r) * (r - 1)) >> 1);
+        const float4 lr = R.lo[r], hr = R.hi[r];
+        float bc = INFINITY;
+        uint64_t bt = ~0ull;
+        for (int32_t c = lane; c < r; c += 64) {
+            const float cost = combined_area(lr, hr, R.lo[c], R.hi[c]);
+            const uint64_t tie = entry_tie(row0 + c, S, A, body_end);
+            if (entry_less(cost, tie, bc, bt)) {
+                bc = cost;
+                bt = tie;
+            }
+        }
+#pragma unroll
+        for (int m = 32; m > 0; m >>= 1) {
+            const float oc = __shfl_xor(bc, m, 64);
+            const uint64_t ot = static_cast<uint64_t>(__shfl_xor(static_cast<long long>(bt), m, 64));
+            if (entry_less(oc, ot, bc, bt)) {
+                bc = oc;
+                bt = ot;
+            }
+        }
+        if (lane == 0) {
+            R.rc[r] = bc;
+            R.rt[r] = bt;
+        }
+    }
+}
+
+// One 1024-lane workgroup per job of more than kWaveList clusters. Each row
+// keeps its least entry under the reference's min_element order; after a
+// merge of rows i > j (the new node at row j, the last row moved to row i)
+// only the entries in columns i and j change in the other rows, so a row is
+// updated from those two entries, and rescanned in full only when its least
+// entry was in one of those columns, when it is row i or j itself, or when
+// it is one of the last two rows, the only rows whose entries can move from
+// the aligned body to the tail as the triangle shrinks. Lists of more than
+// kBlockList clusters stay in global memory (the top of scenes of several
+// million triangles).
+__global__ void __launch_bounds__(1024) k_aac_merge_block(MergeArgs a, Rows g, int32_t g_stride) {
+    __shared__ RowsLds L;
+    __shared__ float s_bc[16];
+    __shared__ uint64_t s_bt[16];
+    __shared__ int32_t s_br[16];
+    __shared__ int32_t s_k, s_i, s_j, s_ntodo;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const MergeJob jb = a.jobs[blockIdx.x];
-    const bool in_lds = jb.k0 <= kBlockList;
-    int32_t *ids = in_lds ? s_ids : g_ids + static_cast<int64_t>(blockIdx.x) * g_stride;
-    float4 *lo = in_lds ? s_lo : g_lo + static_cast<int64_t>(blockIdx.x) * g_stride;
-    float4 *hi = in_lds ? s_hi : g_hi + static_cast<int64_t>(blockIdx.x) * g_stride;
+    Rows R;
+    if (jb.k0 <= kBlockList) {
+        R = Rows{L.ids, L.lo, L.hi, L.rc, L.rt, L.todo};
+    } else {
+        const int64_t o = static_cast<int64_t>(blockIdx.x) * g_stride;
+        R = Rows{g.ids + o, g.lo + o, g.hi + o, g.rc + o, g.rt + o, g.todo + o};
+    }
+    const int64_t S = jb.off;
     write_leaves(a, jb, t, 1024);
-    __syncthreads();
     for (int32_t qq = t; qq < jb.k0; qq += 1024) {
         int32_t id;
         float4 l, h;
@@ -590,35 +684,124 @@ __global__ void __launch_bounds__(1024) k_aac_merge_block(MergeArgs a, int32_t *
             l = a.bn_lo[id];
             h = a.bn_hi[id];
         }
-        ids[qq] = id;
-        lo[qq] = l;
-        hi[qq] = h;
+        R.ids[qq] = id;
+        R.lo[qq] = l;
+        R.hi[qq] = h;
+        if (qq >= 1) R.todo[qq - 1] = qq;
     }
     if (t == 0) s_k = jb.k0;
-    __threadfence_block();
+    __syncthreads();
+    {
+        int64_t A, body_end;
+        matrix_regions(S, jb.k0, &A, &body_end);
+        rescan_rows(R, jb.k0 - 1, S, A, body_end, w, lane);
+    }
     __syncthreads();
     int32_t made = 0;
     while (true) {
         const int32_t k = s_k;
         if (k <= jb.k) break;
-        Key best = scan_pairs(ids, lo, hi, k, jb.off, t, 1024);
-        best = wave_min(best);
-        if (lane == 0) s_best[w] = best;
+        // The least entry over all rows.
+        float bc = INFINITY;
+        uint64_t bt = ~0ull;
+        int32_t br = -1;
+        for (int32_t r = 1 + t; r < k; r += 1024) {
+            if (entry_less(R.rc[r], R.rt[r], bc, bt)) {
+                bc = R.rc[r];
+                bt = R.rt[r];
+                br = r;
+            }
+        }
+#pragma unroll
+        for (int m = 32; m > 0; m >>= 1) {
+            const float oc = __shfl_xor(bc, m, 64);
+            const uint64_t ot = static_cast<uint64_t>(__shfl_xor(static_cast<long long>(bt), m, 64));
+            const int32_t orr = __shfl_xor(br, m, 64);
+            if (entry_less(oc, ot, bc, bt)) {
+                bc = oc;
+                bt = ot;
+                br = orr;
+            }
+        }
+        if (lane == 0) {
+            s_bc[w] = bc;
+            s_bt[w] = bt;
+            s_br[w] = br;
+        }
         __syncthreads();
         if (t == 0) {
-            Key b = s_best[0];
-            for (int q = 1; q < 16; ++q)
-                if (key_less(s_best[q], b)) b = s_best[q];
+            float c0 = s_bc[0];
+            uint64_t t0 = s_bt[0];
+            int32_t r0 = s_br[0];
+            for (int q = 1; q < 16; ++q) {
+                if (entry_less(s_bc[q], s_bt[q], c0, t0)) {
+                    c0 = s_bc[q];
+                    t0 = s_bt[q];
+                    r0 = s_br[q];
+                }
+            }
+            int32_t i = r0;
+            int32_t j = (i >= 1) ? static_cast<int32_t>(static_cast<int64_t>(t0 & ((1ull << 60) - 1)) -
+                                                        ((static_cast<int64_t>(i) * (i - 1)) >> 1))
+                                 : -1;
+            if (i < 1 || i >= k || j < 0 || j >= i) {  // as merge_step: no least entry
+                i = 1;
+                j = 0;
+            }
             int32_t kk = k;
-            merge_step(a, ids, lo, hi, kk, jb.node_id + made, b.i, b.j);
+            merge_step(a, R.ids, R.lo, R.hi, kk, jb.node_id + made, i, j);
             s_k = kk;
-            __threadfence_block();
+            s_i = i;
+            s_j = j;
+            s_ntodo = 0;
         }
         ++made;
         __syncthreads();
+        const int32_t k1 = s_k, i = s_i, j = s_j;
+        int64_t A, body_end;
+        matrix_regions(S, k1, &A, &body_end);
+        const bool i_live = i < k1;
+        for (int32_t r = 1 + t; r < k1; r += 1024) {
+            bool full = (r == j) || (r == i) || (r >= k1 - 2);
+            if (!full) {
+                const float mc = R.rc[r];
+                const uint64_t mt = R.rt[r];
+                const int32_t col = static_cast<int32_t>(static_cast<int64_t>(mt & ((1ull << 60) - 1)) -
+                                                         ((static_cast<int64_t>(r) * (r - 1)) >> 1));
+                if (col == j || (i_live && col == i)) {
+                    full = true;
+                } else {
+                    float nc = mc;
+                    uint64_t nt = mt;
+                    const int64_t row0 = S + ((static_cast<int64_t>(r) * (r - 1)) >> 1);
+                    if (j < r) {
+                        const float c = combined_area(R.lo[r], R.hi[r], R.lo[j], R.hi[j]);
+                        const uint64_t tie = entry_tie(row0 + j, S, A, body_end);
+                        if (entry_less(c, tie, nc, nt)) {
+                            nc = c;
+                            nt = tie;
+                        }
+                    }
+                    if (i_live && i < r) {
+                        const float c = combined_area(R.lo[r], R.hi[r], R.lo[i], R.hi[i]);
+                        const uint64_t tie = entry_tie(row0 + i, S, A, body_end);
+                        if (entry_less(c, tie, nc, nt)) {
+                            nc = c;
+                            nt = tie;
+                        }
+                    }
+                    R.rc[r] = nc;
+                    R.rt[r] = nt;
+                }
+            }
+            if (full) R.todo[atomicAdd(&s_ntodo, 1)] = r;
+        }
+        __syncthreads();
+        rescan_rows(R, s_ntodo, S, A, body_end, w, lane);
+        __syncthreads();
     }
     const int32_t k = s_k;
-    for (int32_t qq = t; qq < k; qq += 1024) a.lists[jb.b + qq] = ids[qq];
+    for (int32_t qq = t; qq < k; qq += 1024) a.lists[jb.b + qq] = R.ids[qq];
 }
 
 // ssd/bvh_builder.cc:181-185: clusters a node of n triangles keeps, with the
@@ -875,8 +1058,7 @@ int32_t build_aac_tree(rt_scene *scene, float alpha, float delta, AacTree *out) 
     size_t max_jobs = 1;
     for (auto &v : by_depth) max_jobs = std::max(max_jobs, v.size());
     AAC_CHECK(dmalloc((void **)&d_jobs, max_jobs * sizeof(MergeJob)));
-    int32_t *g_ids = nullptr;
-    float4 *g_lo = nullptr, *g_hi = nullptr;
+    Rows g{};
     int32_t g_stride = 0;
     int64_t g_cap = 0;
     auto run = [&](const std::vector<MergeJob> &jobs) -> hipError_t {
@@ -902,17 +1084,20 @@ int32_t build_aac_tree(rt_scene *scene, float alpha, float delta, AacTree *out) 
                 // Lists too long for LDS: one row of kmax clusters per job.
                 const int64_t need = static_cast<int64_t>(kmax) * static_cast<int64_t>(big.size());
                 if (need > g_cap) {
-                    for (void *p : {static_cast<void *>(g_ids), static_cast<void *>(g_lo), static_cast<void *>(g_hi)}) {
+                    for (void *p : {static_cast<void *>(g.ids), static_cast<void *>(g.lo), static_cast<void *>(g.hi),
+                                    static_cast<void *>(g.rc), static_cast<void *>(g.rt), static_cast<void *>(g.todo)}) {
                         if (!p) continue;
                         bufs.erase(std::remove(bufs.begin(), bufs.end(), p), bufs.end());
                         (void)hipFree(p);
                     }
-                    g_ids = nullptr;
-                    g_lo = g_hi = nullptr;
+                    g = Rows{};
                     g_cap = 0;
-                    e = dmalloc((void **)&g_ids, need * sizeof(int32_t));
-                    if (e == hipSuccess) e = dmalloc((void **)&g_lo, need * sizeof(float4));
-                    if (e == hipSuccess) e = dmalloc((void **)&g_hi, need * sizeof(float4));
+                    e = dmalloc((void **)&g.ids, need * sizeof(int32_t));
+                    if (e == hipSuccess) e = dmalloc((void **)&g.lo, need * sizeof(float4));
+                    if (e == hipSuccess) e = dmalloc((void **)&g.hi, need * sizeof(float4));
+                    if (e == hipSuccess) e = dmalloc((void **)&g.rc, need * sizeof(float));
+                    if (e == hipSuccess) e = dmalloc((void **)&g.rt, need * sizeof(uint64_t));
+                    if (e == hipSuccess) e = dmalloc((void **)&g.todo, need * sizeof(int32_t));
                     if (e != hipSuccess) return e;
                     g_cap = need;
                 }
@@ -921,8 +1106,8 @@ int32_t build_aac_tree(rt_scene *scene, float alpha, float delta, AacTree *out) 
             e = hipMemcpyAsync(d_jobs, big.data(), big.size() * sizeof(MergeJob), hipMemcpyHostToDevice, s);
             if (e != hipSuccess) return e;
             ma.count = static_cast<int32_t>(big.size());
-            hipLaunchKernelGGL(k_aac_merge_block, dim3(static_cast<unsigned>(big.size())), dim3(1024), 0, s, ma, g_ids,
-                               g_lo, g_hi, g_stride);
+            hipLaunchKernelGGL(k_aac_merge_block, dim3(static_cast<unsigned>(big.size())), dim3(1024), 0, s, ma, g,
+                               g_stride);
             e = hipGetLastError();
             if (e != hipSuccess) return e;
             e = hipStreamSynchronize(s);

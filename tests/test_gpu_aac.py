@@ -36,6 +36,37 @@ def preorder(lohi, child, leaf, indirect, root):
     return out
 
 
+def preorder_arrays(lohi, child, leaf, indirect, root):
+    """The same comparison as preorder() for large trees, vectorised: node
+    boxes and leaf flags in preorder, leaf sizes and the concatenated leaf
+    triangles in preorder."""
+    child = np.asarray(child, np.int64)
+    leaf = np.asarray(leaf).astype(bool)
+    levels = []
+    cur = np.array([int(root)], np.int64)
+    while cur.size:
+        levels.append(cur)
+        inner = cur[~leaf[cur]]
+        cur = np.concatenate([child[inner, 0], child[inner, 1]])
+    size = np.ones(len(leaf), np.int64)
+    for lv in reversed(levels):
+        inner = lv[~leaf[lv]]
+        size[inner] = 1 + size[child[inner, 0]] + size[child[inner, 1]]
+    pre = np.full(len(leaf), -1, np.int64)
+    pre[int(root)] = 0
+    for lv in levels:
+        inner = lv[~leaf[lv]]
+        pre[child[inner, 0]] = pre[inner] + 1
+        pre[child[inner, 1]] = pre[inner] + 1 + size[child[inner, 0]]
+    reach = np.nonzero(pre >= 0)[0]
+    order = reach[np.argsort(pre[reach])]
+    boxes = np.ascontiguousarray(np.asarray(lohi, np.float32)[order]).view(np.uint32)
+    lf = order[leaf[order]]
+    b, e = child[lf, 0], child[lf, 1]
+    tris = np.concatenate([np.asarray(indirect[x:y]) for x, y in zip(b, e)]) if lf.size else np.zeros(0)
+    return boxes, leaf[order], e - b, tris
+
+
 def compare(rt, O, ctx, abc):
     scene = rt.Scene(ctx, abc, np.full((abc.shape[0], 3), 0.5, np.float32))
     g = scene.build_aac()
@@ -99,6 +130,24 @@ def test_terrain_1m_headline_scene(gpu_ctx, rt, O):
 
     abc, _ = workloads.terrain_arrays(708)
     g, t = compare(rt, O, gpu_ctx, abc)
+    print(f"AAC tree of {abc.shape[0]} triangles: {len(g['leaf'])} nodes, GPU build {g['build_ms']:.1f} ms")
+
+
+def test_terrain_beyond_lds_lists(gpu_ctx, rt, O):
+    """A 9.7M-triangle terrain: the root's two children keep more than 1024
+    clusters each, so the root's merge list (over 2048 clusters) lives in
+    global memory instead of LDS."""
+    from raptor_amd import workloads
+
+    abc, _ = workloads.terrain_arrays(2200)
+    scene = rt.Scene(gpu_ctx, abc, np.full((abc.shape[0], 3), 0.5, np.float32))
+    g = scene.build_aac()
+    t = O.OracleScene(abc).tree()
+    assert np.array_equal(g["indirect"], t["indirect"]), "triangle order differs"
+    a = preorder_arrays(g["lohi"], g["child"], g["leaf"], g["indirect"], g["root"])
+    b = preorder_arrays(t["lohi"], t["child"], t["leaf"], t["indirect"], t["root"])
+    for x, y, what in zip(a, b, ("boxes", "leaf flags", "leaf sizes", "leaf triangles")):
+        assert x.shape == y.shape and np.array_equal(x, y), what
     print(f"AAC tree of {abc.shape[0]} triangles: {len(g['leaf'])} nodes, GPU build {g['build_ms']:.1f} ms")
 
 
