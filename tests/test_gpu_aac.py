@@ -118,6 +118,47 @@ def test_small_scenes(gpu_ctx, rt, O, n):
     compare(rt, O, gpu_ctx, abc)
 
 
+def _flat_grid(n_side, y=0.25):
+    """A flat square grid in the plane y: the scene box has zero extent on y,
+    so that axis's cell width is 0 and its Morton bits come from 0 * inf."""
+    g = np.linspace(0.0, 1.0, n_side + 1, dtype=np.float32)
+    tris = []
+    for i in range(n_side):
+        for j in range(n_side):
+            a = (g[i], y, g[j])
+            b = (g[i + 1], y, g[j])
+            c = (g[i + 1], y, g[j + 1])
+            d = (g[i], y, g[j + 1])
+            tris.append(a + b + c)
+            tris.append(a + c + d)
+    return np.asarray(tris, np.float32)
+
+
+@pytest.mark.parametrize("case", ["flat_grid", "identical", "degenerate", "mixed_scales"])
+def test_edge_scenes(gpu_ctx, rt, O, case):
+    """Scenes the reference can meet: a flat model (one axis of zero extent),
+    many identical triangles (one Morton code, partitions down to the depth
+    limit), zero-area triangles among ordinary ones, and a few large
+    triangles over many small ones."""
+    rng = np.random.default_rng(17)
+    if case == "flat_grid":
+        abc = _flat_grid(30)
+    elif case == "identical":
+        one = np.array([0.1, 0.2, 0.3, 0.4, 0.2, 0.3, 0.1, 0.6, 0.35], np.float32)
+        abc = np.tile(one, (300, 1))
+    elif case == "degenerate":
+        abc, _ = rt.gen_random_triangles(21, 2000, 0.02)
+        abc = abc.copy()
+        k = rng.choice(abc.shape[0], 300, replace=False)
+        abc[k, 6:9] = abc[k, 0:3] + 0.5 * (abc[k, 3:6] - abc[k, 0:3])  # collinear
+        abc[k[:100], 3:9] = np.tile(abc[k[:100], 0:3], (1, 2))         # a point
+    else:
+        small, _ = rt.gen_random_triangles(23, 3000, 0.002)
+        big = rng.uniform(-3.0, 4.0, (12, 9)).astype(np.float32)
+        abc = np.concatenate([small, big]).astype(np.float32)
+    compare(rt, O, gpu_ctx, abc)
+
+
 def test_empty_scene(gpu_ctx, rt):
     scene = rt.Scene(gpu_ctx, np.zeros((0, 9), np.float32), np.zeros((0, 3), np.float32))
     g = scene.build_aac()
