@@ -827,6 +827,18 @@ def main():
     rays = per_step[-1]["rays"]
     if [r * args.steps for r in rays] != stage_rays:
         raise RuntimeError(f"device ray sums {stage_rays} differ from {args.steps} x {rays}")
+    # A frame that first rebuilds its BVH from the scene (a scene that changed
+    # since the last frame): wall time of build plus frame, median of 3.
+    rebuilds, rebuild_build = [], []
+    for _ in range(3):
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        acc2 = rt.Accel(scene, width=args.width, leaf_size=args.leaf_size, lds_stack=args.lds_stack,
+                        sah_bins=args.sah_bins)
+        acc2.render_device(w.camera, w.lights, rgb, bounce=w.bounce, seed=w.seed, regroup=bool(args.regroup))
+        rebuilds.append(time.perf_counter() - t1)
+        rebuild_build.append(acc2.info()["build_ms"])
+        acc2.close()
 
     # Dominant kernel and its roofline: one extra frame with visit counting.
     ctx.set_counting(True)
@@ -888,7 +900,9 @@ def main():
         "frame_waiting_for_itself_ms": round(sorted(sync_wall)[2] * 1e3, 4),
         "stage_ms": {STAGES[k]: round(stage_ms[k], 4) for k in range(4)},
         "bvh": {"nodes": info["nodes"], "leaves": info["leaves"], "depth": info["depth"],
-                "bytes": info["bytes"], "build_ms": round(info["build_ms"], 2)},
+                "bytes": info["bytes"], "build_ms": round(info["build_ms"], 2),
+                "rebuild_ms": round(sorted(rebuild_build)[1], 2)},
+        "frame_with_rebuild_ms": round(sorted(rebuilds)[1] * 1e3, 3),
         "roofline": roofline_entry(args, dom, stage_ms[dom], traffic, bytes_per_launch, model_gbs, compulsory,
                                    counted),
     }
