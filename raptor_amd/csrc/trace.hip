@@ -76,12 +76,18 @@ __global__ void __launch_bounds__(64) k_trace(TraceArgs a) {
 
 // Block-local form with dynamic ray fetch (after Aila and Laine's
 // while-while traversal with replacement of terminated rays): a 256-thread
-// block owns kBlockRays consecutive rays; each lane holds one ray and, after
-// every leaf phase, once at least kRefill lanes of a wave have finished, the
-// wave takes that many new rays from the block's LDS cursor. Incoherent rays
-// end at very different depths; without refills the finished lanes ride
-// along idle until the wave's slowest ray is done. (A single global queue
-// instead serialises its atomics across the XCDs and was 2x slower.)
+// block owns kBlockRays rays, 16 tiles of 64 consecutive rays spread over the
+// whole queue (tile j * G + block, G working blocks); each lane holds one ray
+// and, after every leaf phase, once at least kRefill lanes of a wave have
+// finished, the wave takes that many new rays from the block's LDS cursor.
+// Incoherent rays end at very different depths; without refills the finished
+// lanes ride along idle until the wave's slowest ray is done. All blocks are
+// resident at once, so the launch lasts as long as its slowest block: spread
+// tiles even the blocks' work out (1-2% on the bounce launch against 16
+// consecutive tiles). (A single global queue instead serialises its atomics
+// across the XCDs and was 2x slower; waves taking 256-ray chunks from a
+// global counter lose the tile locality of a block's waves and were 20%
+// slower.)
 // Arithmetic and tie rules are those of trace_ray, so the hits are identical.
 constexpr int kRefill = 16;
 constexpr int kBlockThreads = 256;
@@ -101,8 +107,7 @@ __global__ void __launch_bounds__(kBlockThreads) k_trace_pers(TraceArgs a) {
     const int lane = threadIdx.x & 63;
     const int slot = threadIdx.x;  // LDS stack column
     const int64_t n = a.count ? static_cast<int64_t>(a.count[0]) * a.count_mul : a.n;
-    const int64_t block_base = static_cast<int64_t>(blockIdx.x) * kBlockRays;
-    const int64_t block_end = min(block_base + kBlockRays, n);
+    const int64_t nblk = (n + kBlockRays - 1) / kBlockRays;  // working blocks
     if (threadIdx.x == 0) cursor = 0u;
     __syncthreads();
     const WideNode<W> *nodes = reinterpret_cast<const WideNode<W> *>(a.nodes);
@@ -149,7 +154,7 @@ __global__ void __launch_bounds__(kBlockThreads) k_trace_pers(TraceArgs a) {
         }
     };
 
-    bool drained = false;  // the block's rays are all handed out
+    bool drained = static_cast<int64_t>(blockIdx.x) >= nblk;  // the block's rays are all handed out
     int32_t leaf = kEmpty;
     while (true) {
         // ---- refill finished lanes from the block's cursor
@@ -159,10 +164,14 @@ __global__ void __launch_bounds__(kBlockThreads) k_trace_pers(TraceArgs a) {
             uint32_t base = 0;
             if (lane == 0) base = atomicAdd(&cursor, static_cast<uint32_t>(nidle));
             base = __builtin_amdgcn_readfirstlane(__shfl(base, 0, 64));
-            if (block_base + base + nidle >= block_end) drained = true;
+            if (static_cast<int64_t>(base) + nidle >= kBlockRays) drained = true;
             if (ri < 0) {
-                const int64_t idx = block_base + static_cast<int64_t>(base) + __popcll(idle & lt_mask);
-                if (idx < block_end) {
+                // The block's k-th ray lies in tile (k / 64) * G + block of 64
+                // rays, G the number of working blocks: a block's 16 tiles are
+                // spread over the whole queue, which evens out the blocks' work.
+                const int64_t k = static_cast<int64_t>(base) + __popcll(idle & lt_mask);
+                const int64_t idx = ((k >> 6) * nblk + blockIdx.x) * 64 + (k & 63);
+                if (k < kBlockRays && idx < n) {
                     ri = a.order ? static_cast<int64_t>(a.order[idx]) : idx;
                     const float4 r0 = reinterpret_cast<const float4 *>(a.rays)[2 * ri];
                     const float4 r1 = reinterpret_cast<const float4 *>(a.rays)[2 * ri + 1];
