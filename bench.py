@@ -43,7 +43,7 @@ def roofline_entry(args, dom, launch_ms, traffic, model_bytes, model_gbs, compul
     """The dominant trace launch against the HBM roofline. achieved / frac:
     the HBM bytes one launch moves, measured by rocprofv3 counters
     (FETCH_SIZE x 2 + WRITE_SIZE x 1 per dispatch, both scales from a copy
-    kernel of known size; profiles/traffic_terrain1m.json), over the launch
+    kernel of known size; profiles/trace_terrain1m.json), over the launch
     time measured here with HIP events on the launch's stream. Two labelled
     models sit beside it: the per-visit model prices every node and triangle
     fetch as HBM traffic (no cache reuse; the tree and triangles, 81 MB, live
@@ -860,7 +860,7 @@ def main():
     # leaf triangles once.
     compulsory = counted["rays"][dom] * (32 + (16 if closest else 1)) + info["bytes"]
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_terrain1m.json")
+    tpath = os.path.join(ROOT, "profiles", "trace_terrain1m.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
             t = json.load(f)
