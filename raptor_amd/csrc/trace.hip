@@ -95,7 +95,9 @@ constexpr int kBlockRays = 1024;
 // Launch stages (bit = STAGE) that use the block-local form. Measured on the
 // 1M-triangle terrain frame: the incoherent bounce closest-hit launch gains
 // 10% (0.593 -> 0.534 ms); coherent primary and shadow rays lose (0.40 ->
-// 0.54 ms primary) because refills break the 8x8-tile order of a wave.
+// 0.54 ms primary) because refills break the 8x8-tile order of a wave; the
+// bounce-shadow any-hit launch (494k rays from scattered bounce hits, one
+// round of waves) loses too (0.092 -> 0.108 ms).
 constexpr unsigned kPersistentStages = 1u << 2;
 
 template <int W, int STAGE, bool COUNT, int kLdsDepth>
